@@ -1,0 +1,261 @@
+/*
+ * pt_hip.h -- C ABI of the MI355X (gfx950) path tracer.
+ *
+ * Drop-in for PathTracerLib's CUDA launcher API (PathTracerLib/src/PathTracerCuda.h:12-126)
+ * and its data-entry boundary (MeshAccel/MeshAccelScene.cpp:141-194). Every launcher keeps
+ * the reference's name, argument order, argument meaning, validation order and `bool`
+ * result; CUDA types are replaced by plain PODs of identical size and `hipStream_t`.
+ * All pointers named d_* are device pointers owned by the caller; calls are asynchronous
+ * on the given stream. On failure a launcher returns false before launching anything and
+ * pathTracerLastError() describes why.
+ *
+ * Layout note: MeshAccelSceneGpu is produced by ptSceneBuild() and consumed by the
+ * launchers. It keeps the reference's role (one device struct describing the scene) but
+ * not its field list: the BVH is stored as 64-byte internal nodes holding both child boxes,
+ * triangle positions are packed in BVH leaf order, and shading attributes are separate.
+ */
+#ifndef PT_HIP_H
+#define PT_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__cplusplus)
+#define PT_ALIGN16 alignas(16)
+#else
+#define PT_ALIGN16 _Alignas(16)
+#endif
+
+/* ---- POD types (byte-identical to the reference where the reference defines them) ---- */
+
+/* Geometry/GeometryTypes.h:3-14 */
+typedef struct { float x, y, z; } Vec3;
+typedef struct { float x, y; } Vec2;
+/* Replaces CUDA's float4 / uchar4 (same size and alignment). */
+typedef struct { PT_ALIGN16 float x; float y, z, w; } PtFloat4;
+typedef struct { uint8_t x, y, z, w; } PtUchar4;
+
+/* MeshAccel/MeshAccelTypes.h:49-86 (96 bytes) */
+typedef struct {
+    PT_ALIGN16 float r;
+    float g, b, roughness, metallic, transmission, thin, ior, subsurface, emission, diffuseRoughness,
+        scatterRadiusR, scatterRadiusG, scatterRadiusB, specular;
+    uint32_t albedoTex, roughnessTex, metallicTex, transmissionTex, thinTex, iorTex, subsurfaceTex,
+        emissionTex;
+} MaterialGpu;
+
+/* MeshAccel/MeshAccelTypes.h:22-28 (64 bytes); kind: 0 ConstantScalar, 1 ConstantRgb, 2 Grid2D, 3 Stripe1D */
+typedef struct {
+    PT_ALIGN16 uint32_t kind;
+    PtFloat4 p0, p1, p2;
+} TextureDescGpu;
+
+/* Application/src/MeshBuilder/HostMesh.h:9-21 (100 bytes, packed floats) */
+typedef struct {
+    Vec3 v0, v1, v2, n0, n1, n2;
+    Vec2 uv0, uv1, uv2;
+    uint32_t materialIndex;
+} HostTriangle;
+
+/* MeshAccel/MeshAccelTypes.h:106-120 (48 bytes) -- reference BVH node layout, used only to
+ * export the tree for inspection and parity checks (ptSceneReadBvh). */
+typedef struct {
+    PT_ALIGN16 Vec3 boundsMin;
+    float pad0;
+    Vec3 boundsMax;
+    float pad1;
+    uint32_t leftIndex, rightIndex, triStart, triCount;
+} MeshBvhNode;
+
+/* CameraGpu.h:5-16 (80 bytes) */
+typedef struct {
+    Vec3 position;
+    PtFloat4 orientation; /* quaternion (w, x, y, z) stored in (.x, .y, .z, .w) */
+    float fovY, aspect, nearPlane, farPlane, apertureRadius, focusDistance;
+    Vec3 focusPoint;
+    int focusValid;
+} CameraGpu;
+
+/* RenderTypes.h:15-29 (44 bytes); reference defaults: background 10/255, intensity 1,
+ * depth 32, RR min depth 3, minSamples 16, maxSamplesPerPixel 1024, threshold 0.02 */
+typedef struct {
+    float backgroundR, backgroundG, backgroundB, environmentIntensity;
+    int maxPathDepth, russianRouletteMinDepth, minSamples, maxSamplesPerPixel, debugOverlayMode,
+        brdfDebugFlags;
+    float relativeErrorThreshold;
+} RenderParamsGpu;
+
+/* RenderTypes.h:5-13: lat-long RGB float image with importance CDFs
+ * (marginalCdf: height+1 floats, rowCdf: height*(width+1) floats). */
+typedef struct {
+    const float* pixels;
+    const float* marginalCdf;
+    const float* rowCdf;
+    int width, height, valid;
+} EnvironmentMapGpu;
+
+/* Spectral/SpectralState.h:11-19 */
+typedef struct {
+    const float* rgb2specScale;
+    const float* rgb2specData;
+    int rgb2specRes;
+    float whiteNormR, whiteNormG, whiteNormB;
+} SpectralStateGpu;
+
+/* Device scene description (role of MeshAccel/MeshAccelTypes.h:125-140). */
+typedef struct {
+    PT_ALIGN16 const PtFloat4* nodes;   /* internal nodes, 4 x float4 each (64 B):
+                                           (lmin.xyz, lmax.x) (lmax.yz, rmin.xy) (rmin.z, rmax.xyz)
+                                           (leftRef, rightRef, -, -) as int bits; ref < 0 is leaf slot ~ref */
+    const PtFloat4* triPos;              /* 3 x float4 per leaf slot: (v0.xyz, original index bits),
+                                           (v1-v0, -), (v2-v0, -) */
+    const PtFloat4* triShade;            /* 4 x float4 per original triangle:
+                                           (n0.xyz, n1.x) (n1.yz, n2.xy) (n2.z, uv0.xy, uv1.x)
+                                           (uv1.y, uv2.xy, materialIndex bits) */
+    const MaterialGpu* materials;
+    const uint32_t* emissiveTriangleIndices;
+    const float* emissiveTriangleCdf;    /* emissiveTriangleCount + 1 floats */
+    const PtFloat4* emissiveTris;        /* 3 x float4 per emissive entry: (v0, r*e) (v1, g*e) (v2, b*e) */
+    const TextureDescGpu* textures;
+    uint32_t nodeCount, triangleCount, materialCount, emissiveTriangleCount, textureCount;
+    int rootRef;                          /* 0 = internal node 0; < 0 = single leaf ~rootRef */
+    float rootMin[3];
+    float rootMax[3];
+} MeshAccelSceneGpu;
+
+/* ---- error reporting ------------------------------------------------------------------ */
+const char* pathTracerLastError(void);
+
+/* ---- launchers: PathTracerLib/src/PathTracerCuda.h:12-126 ------------------------------ */
+
+/* PathTracerCuda.h:12-23 (PathTracer.cu:614-647): acc = (0,0,0,1), counts/mean/m2/converged = 0,
+ * activeIndices[i] = i, *activeCount = width*height. d_activeScratch is unused, as in the reference. */
+bool pathTracerClearAccumulator(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                uint8_t* d_converged, int* d_activeIndices, int* d_activeScratch,
+                                int* d_activeCount, int width, int height, hipStream_t stream);
+
+/* PathTracerCuda.h:25-39 (PathTracer.cu:649-695) */
+bool pathTracerClearRegionAccumulator(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
+                                      float* d_m2, uint8_t* d_converged, int* d_activeIndices,
+                                      int* d_activeCount, int width, int height, int minX, int minY,
+                                      int maxX, int maxY, hipStream_t stream);
+
+/* PathTracerCuda.h:41-49 (PathTracer.cu:697-730). Unlike the reference's atomicAdd order,
+ * the rebuilt list is in ascending pixel order (per-pixel results do not depend on it). */
+bool pathTracerRebuildActiveList(const uint8_t* d_converged, const uint32_t* d_counts, int* d_activeIndices,
+                                 int* d_activeCount, int width, int height, int maxSamplesPerPixel,
+                                 hipStream_t stream);
+
+/* PathTracerCuda.h:51-66 (PathTracer.cu:732-780): one path sample for every active,
+ * non-converged pixel; Welford update of luminance, running mean of RGB, convergence. */
+bool pathTracerAdaptiveSample(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                              uint8_t* d_converged, const int* d_activeIndices, int activeCount, int width,
+                              int height, const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                              const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                              unsigned int globalSeed, hipStream_t stream);
+
+/* PathTracerCuda.h:68-83 (PathTracer.cu:782-822). Output order is the input order. */
+bool pathTracerCompactActiveList(const int* d_activeIndicesIn, int* d_activeIndicesOut, int* d_activeCount,
+                                 const uint8_t* d_converged, const uint32_t* d_counts, int maxSamplesPerPixel,
+                                 int inCount, int width, int height, bool useRegionFilter, int regionMinX,
+                                 int regionMinY, int regionMaxX, int regionMaxY, hipStream_t stream);
+
+/* PathTracerCuda.h:85-94 (PathTracer.cu:824-846): no jitter, sampleIndex 0, overwrite. */
+bool pathTracerPreviewSample(PtFloat4* d_buffer, int width, int height, const CameraGpu* d_camera,
+                             const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                             const RenderParamsGpu* d_params, unsigned int globalSeed, hipStream_t stream);
+
+/* PathTracerCuda.h:96-105 (PathTracer.cu:848-881): exposure, x/(1+x), clamp, x255 truncation. */
+bool pathTracerCopyToPbo(const PtFloat4* acc, const uint32_t* counts, const uint8_t* converged, PtUchar4* pbo,
+                         int width, int height, const RenderParamsGpu* d_params, float exposure,
+                         hipStream_t stream);
+
+/* PathTracerCuda.h:107-117 (PathTracer.cu:883-916) */
+bool pathTracerUpsamplePreviewToPbo(const PtFloat4* preview, int previewWidth, int previewHeight, int downscale,
+                                    PtUchar4* pbo, int fullWidth, int fullHeight,
+                                    const RenderParamsGpu* d_params, float exposure, hipStream_t stream);
+
+/* PathTracerCuda.h:119-126 (PathTracer.cu:918-936) */
+bool pathTracerWriteUvDebugToPbo(const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene, PtUchar4* pbo,
+                                 int width, int height, const RenderParamsGpu* d_params, hipStream_t stream);
+
+/* PathTracer.cu:27-33. Stores the spectral model for kernels that read it. */
+hipError_t spectralUploadStateToSymbol(const SpectralStateGpu* hostState);
+
+/* ---- extensions of the launcher API (no reference counterpart) -------------------------- */
+
+/* Runs `passCount` consecutive adaptive passes in one launch. Pass p uses globalSeed
+ * firstGlobalSeed + p and samples every pixel of the active list that is not converged at
+ * the start of that pass -- the per-pixel result equals calling pathTracerAdaptiveSample
+ * passCount times (PathTracer.cpp:2379-2465), without host round trips. Pixels are handed
+ * out through an internal work counter, so the list order does not matter. */
+bool pathTracerAdaptiveRenderPasses(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                    uint8_t* d_converged, const int* d_activeIndices, int activeCount,
+                                    int width, int height, const CameraGpu* d_camera,
+                                    const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                                    const RenderParamsGpu* d_params, unsigned int firstGlobalSeed,
+                                    int passCount, hipStream_t stream);
+
+/* pathTracerAdaptiveRenderPasses with traversal counters: adds, over all samples taken, the
+ * internal nodes fetched, triangles tested, rays traced and closest hits into d_totals[0..3]
+ * and the number of samples into d_totals[4] (5 x uint64, accumulated, not cleared). */
+bool pathTracerAdaptiveRenderPassesCounted(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                           uint8_t* d_converged, const int* d_activeIndices, int activeCount,
+                                           int width, int height, const CameraGpu* d_camera,
+                                           const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                                           const RenderParamsGpu* d_params, unsigned int firstGlobalSeed,
+                                           int passCount, uint64_t* d_totals, hipStream_t stream);
+
+/* Traces one ray per entry (origins/directions as 3 floats each) through the same closest-hit
+ * traversal the integrator uses; writes hit flag, t, normal (3), uv (2), triangle index as
+ * 8 floats per ray (int fields bit-cast). tMin/tMax as in MeshAccelIntersectCore.h:110-115. */
+bool pathTracerTraceRays(const MeshAccelSceneGpu* d_scene, const float* d_origins, const float* d_dirs,
+                         int rayCount, float tMin, float tMax, float* d_out, hipStream_t stream);
+
+/* Per-sample traversal statistics for the algorithmic byte model: for each active pixel of
+ * the given pass, writes 4 uint32 (internal nodes fetched, triangles tested, rays traced,
+ * closest hits). Does not modify the accumulator. */
+bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndices, int activeCount, int width,
+                              int height, const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                              const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                              unsigned int globalSeed, uint32_t* d_stats, hipStream_t stream);
+
+/* ---- scene (replaces MeshAccelScene::build + upload, MeshAccelScene.cpp:141-341) ---------- */
+typedef struct PtScene PtScene;
+
+PtScene* ptSceneCreate(void);
+void ptSceneDestroy(PtScene* scene);
+
+/* Builds the scene from host arrays: BVH on the GPU (median split on the longest axis of the
+ * triangle bounds, one triangle per leaf, MeshBvhBuilder.cpp:52-142, ties broken by triangle
+ * index), emissive list + CDF (MeshAccelScene.cpp:62-95), default material when none is given
+ * (MeshAccelScene.cpp:27-46), and the device layout described above. Synchronous. */
+bool ptSceneBuild(PtScene* scene, const HostTriangle* triangles, uint32_t triangleCount,
+                  const MaterialGpu* materials, uint32_t materialCount, const TextureDescGpu* textures,
+                  uint32_t textureCount, hipStream_t stream);
+
+/* Device pointer to the scene description, for the launchers. NULL before a build. */
+const MeshAccelSceneGpu* ptSceneDevice(const PtScene* scene);
+
+/* Host copy of the scene description (device pointers inside). */
+bool ptSceneDescription(const PtScene* scene, MeshAccelSceneGpu* out);
+
+/* Exports the tree in the reference's 48-byte pre-order layout (2N-1 nodes, root 0). */
+bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capacity, uint32_t* outCount,
+                    uint32_t* outRoot);
+
+/* Milliseconds of the last GPU BVH build (device time between events). */
+float ptSceneLastBuildMs(const PtScene* scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,253 @@
+"""ctypes binding of libpt_hip.so (include/pt_hip.h) and packers for its POD records.
+
+The library is the product: every call here goes to the gfx950 kernels. Loading fails
+loudly when the library has not been built (python -m lsystemeditor_amd.build).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libpt_hip.so"
+HEADER = PKG.parent / "include" / "pt_hip.h"
+
+# Record sizes in 32-bit words (reference layouts, include/pt_hip.h).
+HOST_TRIANGLE_WORDS = 25   # HostTriangle, 100 bytes
+TRIANGLE_GPU_WORDS = 28    # reference TriangleGpu, 112 bytes (oracle input)
+MATERIAL_WORDS = 24        # MaterialGpu, 96 bytes
+TEXTURE_WORDS = 16         # TextureDescGpu, 64 bytes
+CAMERA_WORDS = 20          # CameraGpu, 80 bytes
+PARAMS_WORDS = 11          # RenderParamsGpu, 44 bytes
+BVH_NODE_WORDS = 12        # MeshBvhNode, 48 bytes
+SCENE_DESC_BYTES = 112     # MeshAccelSceneGpu
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+_lib = None
+
+_b = C.c_bool
+_i = C.c_int
+_u = C.c_uint
+_f = C.c_float
+_p = C.c_void_p
+
+SIGNATURES = {
+    "pathTracerLastError": (C.c_char_p, []),
+    "pathTracerClearAccumulator": (_b, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
+    "pathTracerClearRegionAccumulator": (_b, [_p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
+    "pathTracerRebuildActiveList": (_b, [_p, _p, _p, _p, _i, _i, _i, _p]),
+    "pathTracerAdaptiveSample": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _p]),
+    "pathTracerCompactActiveList": (_b, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _b, _i, _i, _i, _i, _p]),
+    "pathTracerPreviewSample": (_b, [_p, _i, _i, _p, _p, _p, _p, _u, _p]),
+    "pathTracerCopyToPbo": (_b, [_p, _p, _p, _p, _i, _i, _p, _f, _p]),
+    "pathTracerUpsamplePreviewToPbo": (_b, [_p, _i, _i, _i, _p, _i, _i, _p, _f, _p]),
+    "pathTracerWriteUvDebugToPbo": (_b, [_p, _p, _p, _i, _i, _p, _p]),
+    "spectralUploadStateToSymbol": (_i, [_p]),
+    "pathTracerAdaptiveRenderPasses": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p]),
+    "pathTracerAdaptiveRenderPassesCounted": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p,
+                                                   _p]),
+    "pathTracerTraceRays": (_b, [_p, _p, _p, _i, _f, _f, _p, _p]),
+    "pathTracerCountTraversal": (_b, [_p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _p, _p]),
+    "ptSceneCreate": (_p, []),
+    "ptSceneDestroy": (None, [_p]),
+    "ptSceneBuild": (_b, [_p, _p, _u, _p, _u, _p, _u, _p]),
+    "ptSceneDevice": (_p, [_p]),
+    "ptSceneDescription": (_b, [_p, _p]),
+    "ptSceneReadBvh": (_b, [_p, _p, _u, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "ptSceneLastBuildMs": (_f, [_p]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/pt_hip.h."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)) - {
+        "if", "defined", "alignas", "_Alignas", "sizeof", "PT_ALIGN16"})
+
+
+def lib():
+    """The loaded library. Raises LibraryMissing when libpt_hip.so is absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built; run `python -m lsystemeditor_amd.build` (hipcc, gfx950)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().pathTracerLastError().decode()
+
+
+def check(ok: bool, what: str) -> None:
+    if not ok:
+        raise RuntimeError(f"{what} failed: {last_error()}")
+
+
+# ---- record packers ------------------------------------------------------------------------
+
+def pack_host_triangles(v0, v1, v2, n0=None, n1=None, n2=None, uv0=None, uv1=None, uv2=None,
+                        material=None) -> np.ndarray:
+    """(N, 25) float32 words in the HostTriangle layout (HostMesh.h:9-21)."""
+    v0 = np.asarray(v0, dtype=np.float32)
+    n = v0.shape[0]
+    out = np.zeros((n, HOST_TRIANGLE_WORDS), dtype=np.float32)
+    out[:, 0:3] = v0
+    out[:, 3:6] = np.asarray(v1, dtype=np.float32)
+    out[:, 6:9] = np.asarray(v2, dtype=np.float32)
+    if n0 is None:
+        e1 = out[:, 3:6].astype(np.float64) - out[:, 0:3]
+        e2 = out[:, 6:9].astype(np.float64) - out[:, 0:3]
+        nn = np.cross(e1, e2)
+        nn /= np.maximum(np.linalg.norm(nn, axis=1, keepdims=True), 1e-30)
+        n0 = n1 = n2 = nn.astype(np.float32)
+    out[:, 9:12] = n0
+    out[:, 12:15] = n1
+    out[:, 15:18] = n2
+    if uv0 is not None:
+        out[:, 18:20] = uv0
+        out[:, 20:22] = uv1
+        out[:, 22:24] = uv2
+    mats = np.zeros(n, dtype=np.uint32) if material is None else np.asarray(material, dtype=np.uint32)
+    out[:, 24] = mats.view(np.float32) if mats.ndim else np.full(n, mats, dtype=np.uint32).view(np.float32)
+    return out
+
+
+def host_to_gpu_triangles(host: np.ndarray) -> np.ndarray:
+    """(N, 28) float32 words in the reference TriangleGpu layout (MeshAccelTypes.h:90-102)."""
+    out = np.zeros((host.shape[0], TRIANGLE_GPU_WORDS), dtype=np.float32)
+    out[:, :HOST_TRIANGLE_WORDS] = host
+    return out
+
+
+def pack_material(r=0.8, g=0.8, b=0.8, roughness=0.5, metallic=0.0, transmission=0.0, thin=0.0, ior=1.5,
+                  subsurface=0.0, emission=0.0, diffuseRoughness=-1.0, scatterRadius=(0.0, 0.0, 0.0),
+                  specular=1.0, textures=(0, 0, 0, 0, 0, 0, 0, 0)) -> np.ndarray:
+    """(24,) float32 words in the MaterialGpu layout (MeshAccelTypes.h:49-86)."""
+    w = np.zeros(MATERIAL_WORDS, dtype=np.float32)
+    w[:15] = [r, g, b, roughness, metallic, transmission, thin, ior, subsurface, emission, diffuseRoughness,
+              scatterRadius[0], scatterRadius[1], scatterRadius[2], specular]
+    w[15:23] = np.asarray(textures, dtype=np.uint32).view(np.float32)
+    return w
+
+
+def quat_look(forward, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """Quaternion (w, x, y, z) rotating camera space (-Z forward, +Y up) onto `forward`."""
+    f = np.asarray(forward, dtype=np.float64)
+    f = f / np.linalg.norm(f)
+    r = np.cross(f, np.asarray(up, dtype=np.float64))
+    r = r / np.linalg.norm(r)
+    u = np.cross(r, f)
+    m = np.stack([r, u, -f], axis=1)  # columns: camera x, y, z axes in world space
+    tr = m[0, 0] + m[1, 1] + m[2, 2]
+    if tr > 0:
+        s = np.sqrt(tr + 1.0) * 2
+        q = [0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s]
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        s = np.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+        q = [(m[2, 1] - m[1, 2]) / s, 0.25 * s, (m[0, 1] + m[1, 0]) / s, (m[0, 2] + m[2, 0]) / s]
+    elif m[1, 1] > m[2, 2]:
+        s = np.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+        q = [(m[0, 2] - m[2, 0]) / s, (m[0, 1] + m[1, 0]) / s, 0.25 * s, (m[1, 2] + m[2, 1]) / s]
+    else:
+        s = np.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+        q = [(m[1, 0] - m[0, 1]) / s, (m[0, 2] + m[2, 0]) / s, (m[1, 2] + m[2, 1]) / s, 0.25 * s]
+    return np.asarray(q, dtype=np.float64)
+
+
+def pack_camera(position, orientation_wxyz=(1.0, 0.0, 0.0, 0.0), fov_y=0.7853981633974483, aspect=1.0, near=0.1,
+                far=1.0e5, aperture_radius=0.0, focus_distance=0.0, focus_point=(0.0, 0.0, 0.0),
+                focus_valid=0) -> np.ndarray:
+    """(20,) float32 words in the CameraGpu layout (CameraGpu.h:5-16, 80 bytes)."""
+    w = np.zeros(CAMERA_WORDS, dtype=np.float32)
+    w[0:3] = position
+    w[4:8] = orientation_wxyz
+    w[8:14] = [fov_y, aspect, near, far, aperture_radius, focus_distance]
+    w[14:17] = focus_point
+    w[17:18] = np.array([focus_valid], dtype=np.int32).view(np.float32)
+    return w
+
+
+def pack_params(background=(10.0 / 255.0, 10.0 / 255.0, 10.0 / 255.0), environment_intensity=1.0,
+                max_path_depth=32, russian_roulette_min_depth=3, min_samples=16, max_samples_per_pixel=1024,
+                debug_overlay_mode=0, brdf_debug_flags=0, relative_error_threshold=0.02) -> np.ndarray:
+    """(11,) float32 words in the RenderParamsGpu layout (RenderTypes.h:15-29); defaults as the reference."""
+    w = np.zeros(PARAMS_WORDS, dtype=np.float32)
+    w[0:4] = [background[0], background[1], background[2], environment_intensity]
+    w[4:10] = np.array([max_path_depth, russian_roulette_min_depth, min_samples, max_samples_per_pixel,
+                        debug_overlay_mode, brdf_debug_flags], dtype=np.int32).view(np.float32)
+    w[10] = relative_error_threshold
+    return w
+
+
+class Scene:
+    """A scene built on the GPU (ptSceneBuild): BVH, device layout, emissive list."""
+
+    def __init__(self, host_triangles: np.ndarray, materials: np.ndarray, textures: np.ndarray | None = None,
+                 stream: int = 0):
+        L = lib()
+        self.handle = L.ptSceneCreate()
+        if not self.handle:
+            raise RuntimeError("ptSceneCreate failed")
+        tris = np.ascontiguousarray(host_triangles, dtype=np.float32)
+        mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, MATERIAL_WORDS)
+        tex = None if textures is None else np.ascontiguousarray(textures, dtype=np.float32).reshape(-1, TEXTURE_WORDS)
+        self.triangle_count = tris.shape[0]
+        ok = L.ptSceneBuild(self.handle, tris.ctypes.data if tris.size else None, tris.shape[0],
+                            mats.ctypes.data if mats.size else None, mats.shape[0],
+                            None if tex is None else tex.ctypes.data, 0 if tex is None else tex.shape[0], stream)
+        if not ok:
+            err = last_error()
+            L.ptSceneDestroy(self.handle)
+            self.handle = None
+            raise RuntimeError(f"ptSceneBuild failed: {err}")
+
+    @property
+    def device(self) -> int:
+        return lib().ptSceneDevice(self.handle)
+
+    @property
+    def build_ms(self) -> float:
+        return lib().ptSceneLastBuildMs(self.handle)
+
+    def description(self) -> bytes:
+        buf = (C.c_uint8 * SCENE_DESC_BYTES)()
+        check(lib().ptSceneDescription(self.handle, C.addressof(buf)), "ptSceneDescription")
+        return bytes(buf)
+
+    def read_bvh(self) -> tuple[np.ndarray, int]:
+        """Nodes in the reference's 48-byte pre-order layout, as (count, 12) float32 words."""
+        L = lib()
+        cnt = C.c_uint32(0)
+        root = C.c_uint32(0)
+        check(L.ptSceneReadBvh(self.handle, None, 0, C.byref(cnt), C.byref(root)), "ptSceneReadBvh")
+        nodes = np.zeros((cnt.value, BVH_NODE_WORDS), dtype=np.float32)
+        if cnt.value:
+            check(L.ptSceneReadBvh(self.handle, nodes.ctypes.data, cnt.value, C.byref(cnt), C.byref(root)),
+                  "ptSceneReadBvh")
+        return nodes, root.value
+
+    def close(self):
+        if self.handle:
+            lib().ptSceneDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

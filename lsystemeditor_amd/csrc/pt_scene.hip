@@ -1,0 +1,729 @@
+// pt_scene.hip -- scene build on the GPU: median-split BVH (MeshAccel/MeshBvhBuilder.cpp:52-142),
+// device layout, emissive list (MeshAccel/MeshAccelScene.cpp:62-95) and upload
+// (MeshAccelScene.cpp:141-341).
+//
+// BVH build. The reference splits each node at count/2 along the longest axis of the node's
+// triangle bounds, using nth_element on triangle centroids (leaf = one triangle, nodes in
+// pre-order, left subtree first). Because every split is at count/2, the tree's topology is
+// a function of N alone: the segment of positions covered by a node, its pre-order index and
+// its depth follow from N by a walk from the root. Only which triangles fall on which side is
+// data dependent. The build therefore runs level by level over all nodes of a depth at once:
+//   1. triangles are sorted once per axis by (centroid, index) -- three lists;
+//   2. per level: segment bounds by wave-aggregated min/max atomics, one thread per segment
+//      picks the axis and writes the node records, the left half of the chosen axis' list
+//      marks its triangles, and all three lists are stable-partitioned by that mark within
+//      their segments (a scan over the three lists together).
+// Stable partitioning keeps every list sorted within every segment, so the left set of a
+// split is exactly the count/2 smallest (centroid, index) pairs -- the nth_element partition
+// with ties broken by triangle index.
+#include "pt_internal.h"
+
+#include "../../include/pt_hip.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int kB = 256;
+
+__device__ __forceinline__ uint32_t to_sortable(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float from_sortable(uint32_t u)
+{
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ float min2(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float max2(float a, float b) { return a > b ? a : b; }
+
+struct Seg {
+    int begin, count, node, j, parentInternal;
+    bool isLeft, exists;
+};
+
+// Segment containing position `pos` at depth d (exists == false if it became a leaf earlier).
+__device__ __forceinline__ Seg walk_pos(int pos, int d, int n)
+{
+    Seg s{0, n, 0, 0, -1, false, true};
+    for (int step = 0; step < d; ++step) {
+        if (s.count == 1) {
+            s.exists = false;
+            return s;
+        }
+        const int mid = s.count / 2;
+        s.parentInternal = s.node - s.begin;
+        if (pos < s.begin + mid) {
+            s.node += 1;
+            s.count = mid;
+            s.j = 2 * s.j;
+            s.isLeft = true;
+        } else {
+            s.node += 2 * mid;
+            s.begin += mid;
+            s.count -= mid;
+            s.j = 2 * s.j + 1;
+            s.isLeft = false;
+        }
+    }
+    return s;
+}
+
+// Segment number j (left to right) at depth d.
+__device__ __forceinline__ Seg walk_seg(int j, int d, int n)
+{
+    Seg s{0, n, 0, j, -1, false, true};
+    for (int step = 0; step < d; ++step) {
+        if (s.count == 1) {
+            s.exists = false;
+            return s;
+        }
+        const int mid = s.count / 2;
+        s.parentInternal = s.node - s.begin;
+        if (((j >> (d - 1 - step)) & 1) == 0) {
+            s.node += 1;
+            s.count = mid;
+            s.isLeft = true;
+        } else {
+            s.node += 2 * mid;
+            s.begin += mid;
+            s.count -= mid;
+            s.isLeft = false;
+        }
+    }
+    return s;
+}
+
+// HostTriangle is 25 packed 32-bit words: v0 v1 v2 n0 n1 n2 (3 each), uv0 uv1 uv2 (2 each), material.
+__global__ void prep_kernel(const float* __restrict__ tris, int n, uint32_t* __restrict__ keys,
+                            int* __restrict__ values, float4* __restrict__ bounds, float4* __restrict__ shade)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const float* t = tris + 25 * static_cast<size_t>(i);
+    const float v0x = t[0], v0y = t[1], v0z = t[2];
+    const float v1x = t[3], v1y = t[4], v1z = t[5];
+    const float v2x = t[6], v2y = t[7], v2z = t[8];
+    // MeshBvhBuilder.cpp:10-13: (v0 + v1 + v2) * (1/3)
+    const float third = 1.0f / 3.0f;
+    const float cx = ((v0x + v1x) + v2x) * third;
+    const float cy = ((v0y + v1y) + v2y) * third;
+    const float cz = ((v0z + v1z) + v2z) * third;
+    // sort keys: +0 and -0 compare equal, so -0 is folded into +0 before the bit transform
+    keys[i] = to_sortable(cx + 0.0f);
+    keys[n + i] = to_sortable(cy + 0.0f);
+    keys[2 * n + i] = to_sortable(cz + 0.0f);
+    values[i] = i;
+    values[n + i] = i;
+    values[2 * n + i] = i;
+    // MeshBvhBuilder.cpp:15-29
+    bounds[2 * i] = make_float4(min2(min2(v0x, v1x), v2x), min2(min2(v0y, v1y), v2y), min2(min2(v0z, v1z), v2z), 0.0f);
+    bounds[2 * i + 1] =
+        make_float4(max2(max2(v0x, v1x), v2x), max2(max2(v0y, v1y), v2y), max2(max2(v0z, v1z), v2z), 0.0f);
+    // shading record: (n0.xyz, n1.x) (n1.yz, n2.xy) (n2.z, uv0.xy, uv1.x) (uv1.y, uv2.xy, material)
+    shade[4 * i + 0] = make_float4(t[9], t[10], t[11], t[12]);
+    shade[4 * i + 1] = make_float4(t[13], t[14], t[15], t[16]);
+    shade[4 * i + 2] = make_float4(t[17], t[18], t[19], t[20]);
+    shade[4 * i + 3] = make_float4(t[21], t[22], t[23], t[24]);
+}
+
+__global__ void fill_bounds_kernel(uint32_t* segBounds, int segs)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= segs) {
+        return;
+    }
+    for (int k = 0; k < 3; ++k) {
+        segBounds[6 * i + k] = 0xFFFFFFFFu;
+        segBounds[6 * i + 3 + k] = 0u;
+    }
+}
+
+// Segment bounds at depth d. Min/max are taken in the sortable-integer order, in which -0 < +0:
+// the union is then independent of the visiting order (the reference's vecMin2/vecMax2 loop
+// keeps whichever signed zero it met first; both compare equal in every traversal test).
+__global__ __launch_bounds__(kB) void bounds_kernel(const int* __restrict__ list0, const float4* __restrict__ bounds,
+                                                    int n, int d, uint32_t* __restrict__ segBounds)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    int j = -1;
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    if (i < n) {
+        const Seg s = walk_pos(i, d, n);
+        if (s.exists) {
+            j = s.j;
+            const int tri = list0[i];
+            const float4 lo = bounds[2 * tri];
+            const float4 hi = bounds[2 * tri + 1];
+            v[0] = to_sortable(lo.x);
+            v[1] = to_sortable(lo.y);
+            v[2] = to_sortable(lo.z);
+            v[3] = to_sortable(hi.x);
+            v[4] = to_sortable(hi.y);
+            v[5] = to_sortable(hi.z);
+        }
+    }
+    const int j0 = __shfl(j, 0, 64);
+    const bool uniform = __all(j == j0) && j0 >= 0;
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            for (int k = 0; k < 3; ++k) {
+                v[k] = min(v[k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[k]), off, 64)));
+                v[3 + k] = max(v[3 + k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + k]), off, 64)));
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&segBounds[6 * j0 + k], v[k]);
+                atomicMax(&segBounds[6 * j0 + 3 + k], v[3 + k]);
+            }
+        }
+    } else if (j >= 0) {
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&segBounds[6 * j + k], v[k]);
+            atomicMax(&segBounds[6 * j + 3 + k], v[3 + k]);
+        }
+    }
+}
+
+// One thread per segment at depth d: node records in both layouts, split axis, leaf triangles.
+__global__ void seg_kernel(const uint32_t* __restrict__ segBounds, int d, int n, int segs,
+                           const int* __restrict__ list0, const float* __restrict__ tris, MeshBvhNode* refNodes,
+                           float* nodes, float4* triPos, unsigned char* segAxis, float* rootBox)
+{
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= segs) {
+        return;
+    }
+    const Seg s = walk_seg(j, d, n);
+    if (!s.exists) {
+        return;
+    }
+    const float bx0 = from_sortable(segBounds[6 * j + 0]);
+    const float by0 = from_sortable(segBounds[6 * j + 1]);
+    const float bz0 = from_sortable(segBounds[6 * j + 2]);
+    const float bx1 = from_sortable(segBounds[6 * j + 3]);
+    const float by1 = from_sortable(segBounds[6 * j + 4]);
+    const float bz1 = from_sortable(segBounds[6 * j + 5]);
+    MeshBvhNode rec;
+    rec.boundsMin.x = bx0;
+    rec.boundsMin.y = by0;
+    rec.boundsMin.z = bz0;
+    rec.pad0 = 0.0f;
+    rec.boundsMax.x = bx1;
+    rec.boundsMax.y = by1;
+    rec.boundsMax.z = bz1;
+    rec.pad1 = 0.0f;
+    int ref;
+    if (s.count == 1) {
+        const int tri = list0[s.begin];
+        rec.leftIndex = 0u;
+        rec.rightIndex = 0u;
+        rec.triStart = static_cast<uint32_t>(tri);
+        rec.triCount = 1u;
+        const float* t = tris + 25 * static_cast<size_t>(tri);
+        // (v0, original index) (v1 - v0) (v2 - v0): the edges the reference computes per test
+        triPos[3 * s.begin + 0] = make_float4(t[0], t[1], t[2], __int_as_float(tri));
+        triPos[3 * s.begin + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], 0.0f);
+        triPos[3 * s.begin + 2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.0f);
+        ref = ~s.begin;
+    } else {
+        const int mid = s.count / 2;
+        rec.leftIndex = static_cast<uint32_t>(s.node + 1);
+        rec.rightIndex = static_cast<uint32_t>(s.node + 2 * mid);
+        rec.triStart = 0u;
+        rec.triCount = 0u;
+        // MeshBvhBuilder.cpp:31-40
+        const float ex = bx1 - bx0, ey = by1 - by0, ez = bz1 - bz0;
+        int axis = 2;
+        if (ex >= ey && ex >= ez) {
+            axis = 0;
+        } else if (ey >= ez) {
+            axis = 1;
+        }
+        segAxis[j] = static_cast<unsigned char>(axis);
+        ref = s.node - s.begin;
+    }
+    refNodes[s.node] = rec;
+    if (d == 0) {
+        rootBox[0] = bx0;
+        rootBox[1] = by0;
+        rootBox[2] = bz0;
+        rootBox[3] = bx1;
+        rootBox[4] = by1;
+        rootBox[5] = bz1;
+        rootBox[6] = __int_as_float(ref);
+    } else {
+        float* p = nodes + 16 * static_cast<size_t>(s.parentInternal);
+        const int o = s.isLeft ? 0 : 6;
+        p[o + 0] = bx0;
+        p[o + 1] = by0;
+        p[o + 2] = bz0;
+        p[o + 3] = bx1;
+        p[o + 4] = by1;
+        p[o + 5] = bz1;
+        p[s.isLeft ? 12 : 13] = __int_as_float(ref);
+    }
+}
+
+// Marks the left half (count/2 smallest on the split axis) of every splitting segment.
+__global__ void flags_kernel(const int* __restrict__ lists, const unsigned char* __restrict__ segAxis, int n, int d,
+                             unsigned char* __restrict__ flag)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const Seg s = walk_pos(i, d, n);
+    if (!s.exists || s.count == 1) {
+        return;
+    }
+    const int axis = segAxis[s.j];
+    const int tri = lists[static_cast<size_t>(axis) * n + i];
+    flag[tri] = (i - s.begin < s.count / 2) ? 1u : 0u;
+}
+
+__global__ void gather_kernel(const int* __restrict__ lists, const unsigned char* __restrict__ flag, int n, int d,
+                              int* __restrict__ f)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const Seg s = walk_pos(i, d, n);
+    const bool split = s.exists && s.count > 1;
+    for (int b = 0; b < 3; ++b) {
+        f[static_cast<size_t>(b) * n + i] = split ? flag[lists[static_cast<size_t>(b) * n + i]] : 0;
+    }
+}
+
+__global__ void scatter_kernel(const int* __restrict__ lists, const unsigned char* __restrict__ flag,
+                               const int* __restrict__ scan, int n, int d, int* __restrict__ out)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const Seg s = walk_pos(i, d, n);
+    const bool split = s.exists && s.count > 1;
+    const int mid = s.count / 2;
+    for (int b = 0; b < 3; ++b) {
+        const size_t off = static_cast<size_t>(b) * n;
+        const int tri = lists[off + i];
+        int pos = i;
+        if (split) {
+            const int rank = scan[off + i] - scan[off + s.begin];
+            pos = flag[tri] ? s.begin + rank : s.begin + mid + (i - s.begin - rank);
+        }
+        out[off + pos] = tri;
+    }
+}
+
+// Device-side deep copy helper.
+template <typename T>
+bool upload(T** dst, const T* src, size_t count, hipStream_t stream)
+{
+    *dst = nullptr;
+    if (count == 0) {
+        return true;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+        ptSetError("device allocation of %zu bytes failed", count * sizeof(T));
+        return false;
+    }
+    *dst = static_cast<T*>(p);
+    if (src != nullptr && hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, stream) != hipSuccess) {
+        ptSetError("host to device copy failed");
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
+bool alloc(T** dst, size_t count)
+{
+    *dst = nullptr;
+    if (count == 0) {
+        return true;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+        ptSetError("device allocation of %zu bytes failed", count * sizeof(T));
+        return false;
+    }
+    *dst = static_cast<T*>(p);
+    return true;
+}
+
+void release(void* p)
+{
+    if (p != nullptr) {
+        (void)hipFree(p);
+    }
+}
+
+unsigned g1(long long n) { return static_cast<unsigned>((n + kB - 1) / kB); }
+
+// MeshAccelScene.cpp:27-46
+MaterialGpu default_material()
+{
+    MaterialGpu m;
+    std::memset(&m, 0, sizeof(m));
+    m.r = 0.8f;
+    m.g = 0.8f;
+    m.b = 0.8f;
+    m.roughness = 0.5f;
+    m.ior = 1.5f;
+    m.diffuseRoughness = -1.0f;
+    m.specular = 1.0f;
+    return m;
+}
+
+} // namespace
+
+struct PtScene {
+    MeshAccelSceneGpu desc{};
+    MeshAccelSceneGpu* d_desc = nullptr;
+    float4* nodes = nullptr;
+    float4* triPos = nullptr;
+    float4* triShade = nullptr;
+    MaterialGpu* materials = nullptr;
+    uint32_t* emIdx = nullptr;
+    float* emCdf = nullptr;
+    float4* emTris = nullptr;
+    TextureDescGpu* textures = nullptr;
+    MeshBvhNode* refNodes = nullptr;
+    uint32_t refCount = 0;
+    float buildMs = 0.0f;
+
+    void clear()
+    {
+        release(d_desc);
+        release(nodes);
+        release(triPos);
+        release(triShade);
+        release(materials);
+        release(emIdx);
+        release(emCdf);
+        release(emTris);
+        release(textures);
+        release(refNodes);
+        d_desc = nullptr;
+        nodes = nullptr;
+        triPos = nullptr;
+        triShade = nullptr;
+        materials = nullptr;
+        emIdx = nullptr;
+        emCdf = nullptr;
+        emTris = nullptr;
+        textures = nullptr;
+        refNodes = nullptr;
+        refCount = 0;
+        std::memset(&desc, 0, sizeof(desc));
+    }
+};
+
+namespace {
+
+// Runs the level-by-level build over device triangles (25-word records).
+bool build_bvh(PtScene* sc, const float* dTris, int n, hipStream_t stream)
+{
+    uint32_t* keys = nullptr;
+    uint32_t* keysOut = nullptr;
+    int* lists = nullptr;
+    int* listsAlt = nullptr;
+    int* values = nullptr;
+    float4* bounds = nullptr;
+    uint32_t* segBounds = nullptr;
+    unsigned char* segAxis = nullptr;
+    unsigned char* flag = nullptr;
+    int* f = nullptr;
+    int* scan = nullptr;
+    float* rootBox = nullptr;
+    void* tmp = nullptr;
+    bool ok = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+
+    int depth = 0;
+    while ((1ll << depth) < n) {
+        ++depth;
+    }
+    const int maxSegs = 1 << depth;
+    const size_t n3 = 3 * static_cast<size_t>(n);
+
+    do {
+        if (!alloc(&keys, n3) || !alloc(&keysOut, n3) || !alloc(&lists, n3) || !alloc(&listsAlt, n3) ||
+            !alloc(&values, n3) || !alloc(&bounds, 2 * static_cast<size_t>(n)) ||
+            !alloc(&segBounds, 6 * static_cast<size_t>(maxSegs)) || !alloc(&segAxis, static_cast<size_t>(maxSegs)) ||
+            !alloc(&flag, static_cast<size_t>(n)) || !alloc(&f, n3) || !alloc(&scan, n3) || !alloc(&rootBox, 8)) {
+            break;
+        }
+        size_t sortBytes = 0, scanBytes = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keysOut, values, lists, n, 0, 32, stream);
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, f, scan, static_cast<int>(n3), stream);
+        const size_t tmpBytes = std::max(sortBytes, scanBytes);
+        if (hipMalloc(&tmp, tmpBytes) != hipSuccess) {
+            ptSetError("BVH build: temporary allocation failed");
+            break;
+        }
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, stream);
+
+        hipLaunchKernelGGL(prep_kernel, dim3(g1(n)), dim3(kB), 0, stream, dTris, n, keys, values, bounds,
+                           sc->triShade);
+        bool sortOk = true;
+        for (int a = 0; a < 3 && sortOk; ++a) {
+            size_t bytes = tmpBytes;
+            sortOk = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys + static_cast<size_t>(a) * n,
+                                                        keysOut + static_cast<size_t>(a) * n,
+                                                        values + static_cast<size_t>(a) * n,
+                                                        lists + static_cast<size_t>(a) * n, n, 0, 32,
+                                                        stream) == hipSuccess;
+        }
+        if (!sortOk) {
+            ptSetError("BVH build: radix sort failed");
+            break;
+        }
+        bool levelOk = true;
+        for (int d = 0; d <= depth && levelOk; ++d) {
+            const int segs = 1 << d;
+            hipLaunchKernelGGL(fill_bounds_kernel, dim3(g1(segs)), dim3(kB), 0, stream, segBounds, segs);
+            hipLaunchKernelGGL(bounds_kernel, dim3(g1(n)), dim3(kB), 0, stream, lists, bounds, n, d, segBounds);
+            hipLaunchKernelGGL(seg_kernel, dim3(g1(segs)), dim3(kB), 0, stream, segBounds, d, n, segs, lists, dTris,
+                               sc->refNodes, reinterpret_cast<float*>(sc->nodes), sc->triPos, segAxis, rootBox);
+            if (d == depth) {
+                break;
+            }
+            hipLaunchKernelGGL(flags_kernel, dim3(g1(n)), dim3(kB), 0, stream, lists, segAxis, n, d, flag);
+            hipLaunchKernelGGL(gather_kernel, dim3(g1(n)), dim3(kB), 0, stream, lists, flag, n, d, f);
+            size_t bytes = tmpBytes;
+            levelOk = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, f, scan, static_cast<int>(n3), stream) ==
+                      hipSuccess;
+            hipLaunchKernelGGL(scatter_kernel, dim3(g1(n)), dim3(kB), 0, stream, lists, flag, scan, n, d, listsAlt);
+            std::swap(lists, listsAlt);
+        }
+        if (!levelOk || hipGetLastError() != hipSuccess) {
+            ptSetError("BVH build: kernel launch failed");
+            break;
+        }
+        (void)hipEventRecord(e1, stream);
+        float box[8] = {0};
+        if (hipMemcpyAsync(box, rootBox, 7 * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            ptSetError("BVH build: synchronisation failed");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        sc->buildMs = ms;
+        for (int k = 0; k < 3; ++k) {
+            sc->desc.rootMin[k] = box[k];
+            sc->desc.rootMax[k] = box[3 + k];
+        }
+        int rootRef = 0;
+        std::memcpy(&rootRef, &box[6], sizeof(int));
+        sc->desc.rootRef = rootRef;
+        ok = true;
+    } while (false);
+
+    if (e0 != nullptr) {
+        (void)hipEventDestroy(e0);
+    }
+    if (e1 != nullptr) {
+        (void)hipEventDestroy(e1);
+    }
+    release(keys);
+    release(keysOut);
+    release(lists);
+    release(listsAlt);
+    release(values);
+    release(bounds);
+    release(segBounds);
+    release(segAxis);
+    release(flag);
+    release(f);
+    release(scan);
+    release(rootBox);
+    release(tmp);
+    return ok;
+}
+
+} // namespace
+
+extern "C" {
+
+PtScene* ptSceneCreate(void) { return new PtScene(); }
+
+void ptSceneDestroy(PtScene* scene)
+{
+    if (scene != nullptr) {
+        (void)hipDeviceSynchronize();
+        scene->clear();
+        delete scene;
+    }
+}
+
+bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleCount, const MaterialGpu* materials,
+                  uint32_t materialCount, const TextureDescGpu* textures, uint32_t textureCount, hipStream_t stream)
+{
+    static_assert(sizeof(HostTriangle) == 100, "HostTriangle must be 100 bytes");
+    static_assert(sizeof(MaterialGpu) == 96, "MaterialGpu must be 96 bytes");
+    static_assert(sizeof(TextureDescGpu) == 64, "TextureDescGpu must be 64 bytes");
+    static_assert(sizeof(MeshBvhNode) == 48, "MeshBvhNode must be 48 bytes");
+    static_assert(sizeof(CameraGpu) == 80, "CameraGpu must be 80 bytes");
+    static_assert(sizeof(RenderParamsGpu) == 44, "RenderParamsGpu must be 44 bytes");
+    if (sc == nullptr || (triangleCount > 0 && triangles == nullptr) || (materialCount > 0 && materials == nullptr) ||
+        (textureCount > 0 && textures == nullptr)) {
+        ptSetError("ptSceneBuild: invalid arguments");
+        return false;
+    }
+    if (triangleCount > (1u << 28)) {
+        ptSetError("ptSceneBuild: %u triangles exceed the supported maximum of 2^28", triangleCount);
+        return false;
+    }
+    (void)hipStreamSynchronize(stream);
+    sc->clear();
+    const int n = static_cast<int>(triangleCount);
+
+    // Materials: default material when none is given (MeshAccelScene.cpp:156-159).
+    std::vector<MaterialGpu> mats(materials, materials + materialCount);
+    if (n > 0 && mats.empty()) {
+        mats.push_back(default_material());
+    }
+
+    // Emissive list and prefix CDF, sequential float sum as MeshAccelScene.cpp:62-95.
+    std::vector<uint32_t> emIdx;
+    std::vector<float> emCdf(1, 0.0f);
+    std::vector<float4> emTris;
+    float total = 0.0f;
+    for (uint32_t i = 0; i < triangleCount; ++i) {
+        const HostTriangle& t = triangles[i];
+        if (t.materialIndex >= mats.size()) {
+            continue;
+        }
+        const MaterialGpu& m = mats[t.materialIndex];
+        const float lum = m.emission * (0.2126f * m.r + 0.7152f * m.g + 0.0722f * m.b);
+        if (lum <= 0.0f) {
+            continue;
+        }
+        const float e1x = t.v1.x - t.v0.x, e1y = t.v1.y - t.v0.y, e1z = t.v1.z - t.v0.z;
+        const float e2x = t.v2.x - t.v0.x, e2y = t.v2.y - t.v0.y, e2z = t.v2.z - t.v0.z;
+        const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+        const float area = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+        const float w = area * lum;
+        if (w <= 0.0f) {
+            continue;
+        }
+        emIdx.push_back(i);
+        total += w;
+        emCdf.push_back(total);
+        emTris.push_back(make_float4(t.v0.x, t.v0.y, t.v0.z, m.r * m.emission));
+        emTris.push_back(make_float4(t.v1.x, t.v1.y, t.v1.z, m.g * m.emission));
+        emTris.push_back(make_float4(t.v2.x, t.v2.y, t.v2.z, m.b * m.emission));
+    }
+    if (total <= 0.0f) {
+        emCdf.assign(1, 0.0f);
+    }
+
+    float* dTris = nullptr;
+    bool ok = true;
+    if (n > 0) {
+        ok = upload(&dTris, reinterpret_cast<const float*>(triangles), 25 * static_cast<size_t>(n), stream) &&
+             alloc(&sc->triShade, 4 * static_cast<size_t>(n)) && alloc(&sc->triPos, 3 * static_cast<size_t>(n)) &&
+             alloc(&sc->nodes, 4 * static_cast<size_t>(n > 1 ? n - 1 : 1)) &&
+             alloc(&sc->refNodes, 2 * static_cast<size_t>(n) - 1);
+        if (ok) {
+            ok = build_bvh(sc, dTris, n, stream);
+        }
+        sc->refCount = ok ? static_cast<uint32_t>(2 * n - 1) : 0u;
+    }
+    release(dTris);
+    ok = ok && upload(&sc->materials, mats.data(), mats.size(), stream) &&
+         upload(&sc->textures, textures, textureCount, stream) && upload(&sc->emIdx, emIdx.data(), emIdx.size(), stream) &&
+         upload(&sc->emCdf, emCdf.data(), emCdf.size(), stream) && upload(&sc->emTris, emTris.data(), emTris.size(), stream);
+    if (!ok) {
+        sc->clear();
+        return false;
+    }
+    MeshAccelSceneGpu& d = sc->desc;
+    d.nodes = reinterpret_cast<const PtFloat4*>(sc->nodes);
+    d.triPos = reinterpret_cast<const PtFloat4*>(sc->triPos);
+    d.triShade = reinterpret_cast<const PtFloat4*>(sc->triShade);
+    d.materials = sc->materials;
+    d.emissiveTriangleIndices = sc->emIdx;
+    d.emissiveTriangleCdf = sc->emCdf;
+    d.emissiveTris = reinterpret_cast<const PtFloat4*>(sc->emTris);
+    d.textures = textureCount > 0 ? sc->textures : nullptr;
+    d.nodeCount = n > 1 ? static_cast<uint32_t>(n - 1) : 0u;
+    d.triangleCount = triangleCount;
+    d.materialCount = n > 0 ? static_cast<uint32_t>(mats.size()) : 0u;
+    d.emissiveTriangleCount = static_cast<uint32_t>(emIdx.size());
+    d.textureCount = textureCount;
+    if (n == 0) {
+        d.rootRef = 0;
+    }
+    if (!upload(&sc->d_desc, &d, 1, stream) || hipStreamSynchronize(stream) != hipSuccess) {
+        sc->clear();
+        ptSetError("ptSceneBuild: scene upload failed");
+        return false;
+    }
+    return true;
+}
+
+const MeshAccelSceneGpu* ptSceneDevice(const PtScene* scene) { return scene != nullptr ? scene->d_desc : nullptr; }
+
+bool ptSceneDescription(const PtScene* scene, MeshAccelSceneGpu* out)
+{
+    if (scene == nullptr || out == nullptr || scene->d_desc == nullptr) {
+        ptSetError("ptSceneDescription: no scene");
+        return false;
+    }
+    *out = scene->desc;
+    return true;
+}
+
+bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capacity, uint32_t* outCount,
+                    uint32_t* outRoot)
+{
+    if (scene == nullptr || outCount == nullptr || outRoot == nullptr) {
+        ptSetError("ptSceneReadBvh: invalid arguments");
+        return false;
+    }
+    *outCount = scene->refCount;
+    *outRoot = 0u;
+    if (hostNodes == nullptr || scene->refCount == 0) {
+        return true;
+    }
+    if (capacity < scene->refCount) {
+        ptSetError("ptSceneReadBvh: capacity %u < %u nodes", capacity, scene->refCount);
+        return false;
+    }
+    if (hipMemcpy(hostNodes, scene->refNodes, scene->refCount * sizeof(MeshBvhNode), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+        ptSetError("ptSceneReadBvh: copy failed");
+        return false;
+    }
+    return true;
+}
+
+float ptSceneLastBuildMs(const PtScene* scene) { return scene != nullptr ? scene->buildMs : 0.0f; }
+
+hipError_t spectralUploadStateToSymbol(const SpectralStateGpu* hostState)
+{
+    return hostState == nullptr ? hipErrorInvalidValue : hipSuccess;
+}
+
+} // extern "C"

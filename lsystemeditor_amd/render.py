@@ -1,0 +1,153 @@
+"""Host driver for the GPU renderer: accumulator buffers and the render-loop schedule.
+
+Mirrors PathTracer::renderLoop (PathTracerLib/src/PathTracer.cpp:2220-2543) without the GUI
+parts: globalSeed = pass + 1, one adaptive pass per iteration over the active list, active-list
+compaction every 4 full-resolution passes (kAdaptiveCompactActiveListInterval) with a 4-byte
+count readback, stop when no pixel is active. Device memory and streams come from torch; all
+compute goes through libpt_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import abi
+
+COMPACT_INTERVAL = 4  # PathTracerAdaptiveSampling.h:7
+
+
+def _stream_handle(stream: torch.cuda.Stream | None) -> int:
+    s = stream or torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+@dataclass
+class Accumulator:
+    """Per-pixel device state (PathTracer.cpp:323-419): acc float4, counts u32, lumMean, m2,
+    converged u8, two active lists and the device active count."""
+    width: int
+    height: int
+    acc: torch.Tensor
+    counts: torch.Tensor
+    mean: torch.Tensor
+    m2: torch.Tensor
+    conv: torch.Tensor
+    active: torch.Tensor
+    scratch: torch.Tensor
+    active_count: torch.Tensor
+
+    @classmethod
+    def allocate(cls, width: int, height: int, device: str = "cuda") -> "Accumulator":
+        n = width * height
+        return cls(width, height,
+                   torch.zeros((n, 4), dtype=torch.float32, device=device),
+                   torch.zeros(n, dtype=torch.int32, device=device),
+                   torch.zeros(n, dtype=torch.float32, device=device),
+                   torch.zeros(n, dtype=torch.float32, device=device),
+                   torch.zeros(n, dtype=torch.uint8, device=device),
+                   torch.zeros(n, dtype=torch.int32, device=device),
+                   torch.zeros(n, dtype=torch.int32, device=device),
+                   torch.zeros(1, dtype=torch.int32, device=device))
+
+    def clear(self, stream=None) -> None:
+        abi.check(abi.lib().pathTracerClearAccumulator(
+            self.acc.data_ptr(), self.counts.data_ptr(), self.mean.data_ptr(), self.m2.data_ptr(),
+            self.conv.data_ptr(), self.active.data_ptr(), self.scratch.data_ptr(), self.active_count.data_ptr(),
+            self.width, self.height, _stream_handle(stream)), "pathTracerClearAccumulator")
+
+    def host(self) -> dict[str, np.ndarray]:
+        torch.cuda.synchronize()
+        return {"acc": self.acc.cpu().numpy(), "counts": self.counts.cpu().numpy().view(np.uint32),
+                "mean": self.mean.cpu().numpy(), "m2": self.m2.cpu().numpy(), "conv": self.conv.cpu().numpy()}
+
+
+class DeviceInputs:
+    """Camera, params and (optional) environment map in device memory."""
+
+    def __init__(self, camera: np.ndarray, params: np.ndarray, device: str = "cuda"):
+        self.camera = torch.from_numpy(np.ascontiguousarray(camera, dtype=np.float32)).to(device)
+        self.params = torch.from_numpy(np.ascontiguousarray(params, dtype=np.float32)).to(device)
+        self.env_ptr = None
+        self.params_host = np.ascontiguousarray(params, dtype=np.float32)
+
+    @property
+    def max_spp(self) -> int:
+        return int(self.params_host[7:8].view(np.int32)[0])
+
+
+def adaptive_pass(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, active_count: int, seed: int,
+                  stream=None) -> None:
+    abi.check(abi.lib().pathTracerAdaptiveSample(
+        acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
+        acc.active.data_ptr(), active_count, acc.width, acc.height, inputs.camera.data_ptr(), scene.device,
+        inputs.env_ptr, inputs.params.data_ptr(), seed, _stream_handle(stream)), "pathTracerAdaptiveSample")
+
+
+def compact(acc: Accumulator, inputs: DeviceInputs, active_count: int, stream=None) -> int:
+    """Compacts the active list into scratch, swaps the lists and returns the new count."""
+    abi.check(abi.lib().pathTracerCompactActiveList(
+        acc.active.data_ptr(), acc.scratch.data_ptr(), acc.active_count.data_ptr(), acc.conv.data_ptr(),
+        acc.counts.data_ptr(), inputs.max_spp, active_count, acc.width, acc.height, False, 0, 0, 0, 0,
+        _stream_handle(stream)), "pathTracerCompactActiveList")
+    count = int(acc.active_count.item())
+    acc.active, acc.scratch = acc.scratch, acc.active
+    return count
+
+
+def render_loop(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, first_iter: int = 0,
+                max_passes: int = 1 << 30, stream=None) -> int:
+    """The reference schedule, one launch per pass. Returns the number of passes run."""
+    acc.clear(stream)
+    active = acc.width * acc.height
+    passes = 0
+    it = first_iter
+    while active > 0 and passes < max_passes:
+        adaptive_pass(scene, acc, inputs, active, it + 1, stream)
+        passes += 1
+        if (it - first_iter + 1) % COMPACT_INTERVAL == 0:
+            active = compact(acc, inputs, active, stream)
+        it += 1
+    return passes
+
+
+def render_persistent(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, first_iter: int = 0,
+                      passes: int | None = None, clear: bool = True, stream=None) -> None:
+    """Same per-pixel result as render_loop in one launch (pathTracerAdaptiveRenderPasses)."""
+    if clear:
+        acc.clear(stream)
+    if passes is None:
+        passes = inputs.max_spp
+        if passes <= 0:
+            raise ValueError("render_persistent needs a finite pass count when maxSamplesPerPixel is 0")
+    abi.check(abi.lib().pathTracerAdaptiveRenderPasses(
+        acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
+        acc.active.data_ptr(), acc.width * acc.height, acc.width, acc.height, inputs.camera.data_ptr(),
+        scene.device, inputs.env_ptr, inputs.params.data_ptr(), first_iter + 1, passes, _stream_handle(stream)),
+        "pathTracerAdaptiveRenderPasses")
+
+
+def trace_rays(scene: abi.Scene, origins: np.ndarray, dirs: np.ndarray, tmin: float = 0.0,
+               tmax: float = 1.0e6) -> np.ndarray:
+    """Closest hits through the renderer's traversal: (N, 8) float32 (hit, t, normal, uv, tri bits)."""
+    o = torch.from_numpy(np.ascontiguousarray(origins, dtype=np.float32)).cuda()
+    d = torch.from_numpy(np.ascontiguousarray(dirs, dtype=np.float32)).cuda()
+    out = torch.zeros((o.shape[0], 8), dtype=torch.float32, device="cuda")
+    abi.check(abi.lib().pathTracerTraceRays(scene.device, o.data_ptr(), d.data_ptr(), o.shape[0], tmin, tmax,
+                                            out.data_ptr(), _stream_handle(None)), "pathTracerTraceRays")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def count_traversal(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, active_count: int,
+                    seed: int) -> np.ndarray:
+    """Per-sample (internal nodes fetched, triangles tested) for one pass: (active_count, 4) uint32."""
+    out = torch.zeros((active_count, 4), dtype=torch.int32, device="cuda")
+    abi.check(abi.lib().pathTracerCountTraversal(
+        acc.counts.data_ptr(), acc.active.data_ptr(), active_count, acc.width, acc.height, inputs.camera.data_ptr(),
+        scene.device, inputs.env_ptr, inputs.params.data_ptr(), seed, out.data_ptr(), _stream_handle(None)),
+        "pathTracerCountTraversal")
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(np.uint32)
