@@ -171,7 +171,13 @@ PT_DEV double atan2_d(double y, double x)
 }
 } // namespace dm
 
-PT_DEV float sinf_pt(float x) { return isfinite(x) ? static_cast<float>(dm::sin_d(static_cast<double>(x))) : x - x; }
+PT_DEV float sinf_pt(float x)
+{
+    if (x == 0.0f) {
+        return x; // keeps the sign of zero
+    }
+    return isfinite(x) ? static_cast<float>(dm::sin_d(static_cast<double>(x))) : x - x;
+}
 PT_DEV float cosf_pt(float x) { return isfinite(x) ? static_cast<float>(dm::cos_d(static_cast<double>(x))) : x - x; }
 PT_DEV void sincosf_pt(float x, float& s, float& c)
 {
@@ -182,13 +188,16 @@ PT_DEV void sincosf_pt(float x, float& s, float& c)
     }
     double sd, cd;
     dm::sincos_d(static_cast<double>(x), sd, cd);
-    s = static_cast<float>(sd);
+    s = x == 0.0f ? x : static_cast<float>(sd);
     c = static_cast<float>(cd);
 }
 PT_DEV float tanf_pt(float x)
 {
     if (!isfinite(x)) {
         return x - x;
+    }
+    if (x == 0.0f) {
+        return x;
     }
     const double xd = static_cast<double>(x);
     return static_cast<float>(dm::sin_d(xd) / dm::cos_d(xd));

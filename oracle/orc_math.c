@@ -96,6 +96,9 @@ float orc_sinf(float x)
     if (!isfinite(x)) {
         return x - x;
     }
+    if (x == 0.0f) {
+        return x; /* keeps the sign of zero */
+    }
     return (float)orc_sin_d((double)x);
 }
 
@@ -111,6 +114,9 @@ float orc_tanf(float x)
 {
     if (!isfinite(x)) {
         return x - x;
+    }
+    if (x == 0.0f) {
+        return x;
     }
     const double xd = (double)x;
     return (float)(orc_sin_d(xd) / orc_cos_d(xd));
