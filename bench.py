@@ -1,9 +1,10 @@
 """Benchmark: plain adaptive forward rendering of the large L-system bush on one MI355X.
 
 A step is one complete adaptive render of the frame: clear the accumulator, then sample every
-pixel until it converges (relative error 0.01, 16..256 spp, max depth 12) -- the reference's
-renderLoop schedule executed by the persistent multi-pass kernel. Samples are counted from the
-per-pixel counts buffer. Inputs (scene, BVH, camera, params) are resident in HBM before timing.
+pixel until it converges (relative error 0.01, 16..256 spp, max depth 12), with the reference's
+per-pixel semantics (pass p uses seed p + 1, convergence tested every 4 samples), executed by the
+wavefront scheduler. Samples are counted from the per-pixel counts buffer. Inputs (scene, BVH,
+camera, params) are resident in HBM before timing.
 
 Prints one JSON line (see the contract in the repository's documentation).
 """
@@ -23,10 +24,11 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Msamples/s and grad-samples/s on 1 GPU vs plain fwd at equal RMSE; HBM GB/s %"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64       # internal node: both child boxes + child references
-TRI_BYTES = 48        # leaf triangle record: v0 + two edges (+ original index)
-SHADE_BYTES = 64 + 96  # per closest hit: shading record + material
-PIXEL_BYTES = 33 + 29  # per pixel per launch: state read (acc, count, mean, m2, flag, list entry) + written
+# Algorithmic bytes of the traversal kernels (the dominant kernels), per unit of work:
+NODE_BYTES = 64        # internal node fetched: both child boxes + child references
+TRI_BYTES = 48         # leaf triangle tested: v0, two edges, original index
+EXT_RAY_BYTES = 28 + 16  # extension ray: origin, direction, tMin read; t, u, v, triangle written
+SHADOW_RAY_BYTES = 32 + 1  # shadow ray record read; occlusion byte written
 
 
 def parse():
@@ -38,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-pass (reference cadence) timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device-to-device tensor copy (scale reference for PMC byte counters)")
     return ap.parse_args()
 
 
@@ -88,6 +92,14 @@ def cpu_baseline(spec, threads: int):
                       f"({samples} samples, {dt:.1f} s; BVH build {build_s:.1f} s excluded)"}
 
 
+def profile_read():
+    import ctypes as C
+    from lsystemeditor_amd import abi
+    out = (C.c_double * 14)()
+    abi.check(abi.lib().pathTracerWavefrontProfileRead(C.addressof(out), 14), "pathTracerWavefrontProfileRead")
+    return list(out)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -110,61 +122,60 @@ def main():
     acc = render.Accumulator.allocate(spec.width, spec.height)
     passes = spec.spp if spec.spp else inputs.max_spp
     print(f"[bench] BVH build {scene.build_ms:.2f} ms (GPU)", file=sys.stderr, flush=True)
+    L = abi.lib()
 
     def step():
-        acc.clear()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record()
-        render.render_persistent(scene, acc, inputs, passes=passes, clear=False)
-        ev1.record()
-        return ev0, ev1
+        render.render_wavefront(scene, acc, inputs, passes=passes)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    # algorithmic bytes of one launch: an instrumented run of the same work (outside the timed region)
-    acc.clear()
-    totals = torch.zeros(5, dtype=torch.int64, device="cuda")
-    abi.check(abi.lib().pathTracerAdaptiveRenderPassesCounted(
-        acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
-        acc.active.data_ptr(), spec.width * spec.height, spec.width, spec.height, inputs.camera.data_ptr(),
-        scene.device, None, inputs.params.data_ptr(), 1, passes, totals.data_ptr(), int(stream.cuda_stream)),
-        "pathTracerAdaptiveRenderPassesCounted")
+    # traversal counters of one step (a counted run of the same work, outside the timed region)
+    abi.check(L.pathTracerWavefrontProfile(0, 1), "pathTracerWavefrontProfile")
+    step()
     torch.cuda.synchronize()
-    nodes, tris, rays, hits, samples_counted = (int(v) for v in totals.cpu().tolist())
-    pixels = spec.width * spec.height
-    bytes_per_launch = nodes * NODE_BYTES + tris * TRI_BYTES + hits * SHADE_BYTES + pixels * PIXEL_BYTES
+    prof = profile_read()
+    ext_nodes, ext_tris, ext_rays, sh_nodes, sh_tris, sh_rays = (int(v) for v in prof[8:14])
+    trace_bytes = (ext_nodes + sh_nodes) * NODE_BYTES + (ext_tris + sh_tris) * TRI_BYTES \
+        + ext_rays * EXT_RAY_BYTES + sh_rays * SHADOW_RAY_BYTES
 
+    # timed region: K steps, kernel durations from hipEvents on the render stream
+    abi.check(L.pathTracerWavefrontProfile(1, 0), "pathTracerWavefrontProfile")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    events = []
-    samples = 0
     for _ in range(args.steps):
-        events.append(step())
+        step()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    wall = time.perf_counter() - t0
+    prof = profile_read()
+    abi.check(L.pathTracerWavefrontProfile(0, 0), "pathTracerWavefrontProfile")
     samples_per_step = int(acc.counts.to(torch.int64).sum().item())
     samples = samples_per_step * args.steps
-    kernel_ms = [e0.elapsed_time(e1) for e0, e1 in events]
-    wall = t1 - t0
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = bytes_per_launch / avg_kernel_s / 1e9
-    print(f"[bench] {args.steps} steps, {samples_per_step} samples/step, kernel {np.mean(kernel_ms):.1f} ms, "
-          f"wall {wall:.3f} s", file=sys.stderr, flush=True)
+    ms_ext, ms_sh, ms_shade, _ = prof[0:4]
+    n_ext, n_sh, n_shade, _ = (int(v) for v in prof[4:8])
+    trace_ms_per_step = (ms_ext + ms_sh) / args.steps
+    achieved = trace_bytes / (trace_ms_per_step / 1e3) / 1e9
+    print(f"[bench] wavefront: {args.steps} steps, {samples_per_step} samples/step, wall {wall:.3f} s; "
+          f"trace ext {ms_ext / args.steps:.1f} ms/step ({n_ext // args.steps} launches), shadow "
+          f"{ms_sh / args.steps:.1f} ms/step, shade {ms_shade / args.steps:.1f} ms/step", file=sys.stderr, flush=True)
 
-    loop_msps = None
+    extra = {}
     if not args.no_loop:
         torch.cuda.synchronize()
+        tp = time.perf_counter()
+        render.render_persistent(scene, acc, inputs, passes=passes)
+        torch.cuda.synchronize()
+        extra["persistent_megakernel_msamples_s"] = int(acc.counts.to(torch.int64).sum().item()) / \
+            (time.perf_counter() - tp) / 1e6
         tl0 = time.perf_counter()
         render.render_loop(scene, acc, inputs)
         torch.cuda.synchronize()
-        tl = time.perf_counter() - tl0
-        loop_samples = int(acc.counts.to(torch.int64).sum().item())
-        loop_msps = loop_samples / tl / 1e6
-        print(f"[bench] reference cadence (one launch per pass + compaction readbacks): {loop_msps:.1f} Msamples/s",
-              file=sys.stderr, flush=True)
+        extra["reference_cadence_msamples_s"] = int(acc.counts.to(torch.int64).sum().item()) / \
+            (time.perf_counter() - tl0) / 1e6
+        print(f"[bench] persistent megakernel {extra['persistent_megakernel_msamples_s']:.1f}, reference cadence "
+              f"(one launch per pass) {extra['reference_cadence_msamples_s']:.1f} Msamples/s", file=sys.stderr,
+              flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -173,6 +184,15 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.3f} Msamples/s on {threads} threads", file=sys.stderr,
               flush=True)
 
+    if args.copy_calibration:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] copy calibration: 536870912 bytes read, 536870912 bytes written", file=sys.stderr, flush=True)
+
+    adaptive = spec.spp is None
     out = {
         "metric": METRIC,
         "value": samples / wall / 1e6,
@@ -186,18 +206,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (deterministic L-system scene generator, seeded)",
-        "config": {"workload": f"{spec.name}: plain adaptive forward render, {spec.width}x{spec.height}, "
-                               f"{spec.triangles.shape[0]} triangles, adaptive 16-256 spp at rel. error 0.01, "
-                               f"max depth 12, 2 area lights + uniform environment",
+        "config": {"workload": f"{spec.name}: plain forward render, {spec.width}x{spec.height}, "
+                               f"{spec.triangles.shape[0]} triangles, "
+                               + ("adaptive 16-256 spp at relative error 0.01, max depth 12, 2 area lights + "
+                                  "uniform environment" if adaptive else f"{spec.spp} spp, max depth 8"),
                    "samples_per_step": samples_per_step, "bvh_build_ms": round(scene.build_ms, 3),
-                   "reference_cadence_msamples_s": loop_msps},
+                   "scheduler": "wavefront (groups of 4 samples per pixel between convergence tests)", **extra},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("render_passes_c1_traffic.txt"),
-                     "kernel": "render_passes_kernel", "kernel_ms": float(np.mean(kernel_ms)),
-                     "bytes_per_launch": bytes_per_launch,
-                     "model": {"internal_nodes": nodes, "triangle_tests": tris, "rays": rays, "closest_hits": hits,
-                               "samples": samples_counted, "node_bytes": NODE_BYTES, "tri_bytes": TRI_BYTES,
-                               "shade_bytes_per_hit": SHADE_BYTES, "pixel_bytes": PIXEL_BYTES}},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"wf_trace_{spec.name.lower()}_traffic.txt"),
+                     "kernel": "wf_trace_ext_kernel + wf_trace_shadow_kernel",
+                     "kernel_ms_per_step": trace_ms_per_step,
+                     "launches_per_step": (n_ext + n_sh) / args.steps,
+                     "bytes_per_step": trace_bytes,
+                     "model": {"ext_nodes": ext_nodes, "ext_triangles": ext_tris, "ext_rays": ext_rays,
+                               "shadow_nodes": sh_nodes, "shadow_triangles": sh_tris, "shadow_rays": sh_rays,
+                               "node_bytes": NODE_BYTES, "tri_bytes": TRI_BYTES, "ext_ray_bytes": EXT_RAY_BYTES,
+                               "shadow_ray_bytes": SHADOW_RAY_BYTES}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

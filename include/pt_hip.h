@@ -213,6 +213,32 @@ bool pathTracerAdaptiveRenderPassesCounted(PtFloat4* d_buffer, uint32_t* d_sampl
                                            const RenderParamsGpu* d_params, unsigned int firstGlobalSeed,
                                            int passCount, uint64_t* d_totals, hipStream_t stream);
 
+/* Wavefront-scheduled equivalent of pathTracerAdaptiveRenderPasses: the same per-pixel results
+ * (pass p uses globalSeed firstGlobalSeed + p; every active, unconverged pixel is sampled once
+ * per pass; convergence is tested at counts that are multiples of 4). Groups of consecutive
+ * samples of all active pixels are traced together, one bounce per launch: closest-hit and
+ * any-hit traversal kernels with an LDS stack, and a shading kernel that appends the next rays
+ * to queues. maxPathDepth is the depth the params resolve to (maxPathDepth > 0 ? that : 0 -> 256;
+ * PathIntegratorRandCore.h:219-227). maxPathsInFlight bounds the paths of one group (0 = 4M).
+ * Uses a library-owned device workspace (grown on demand); calls on one device must not overlap.
+ * The caller's active list is read, not modified. Synchronises the stream once per group. */
+bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                       uint8_t* d_converged, const int* d_activeIndices, int activeCount, int width,
+                                       int height, const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                       const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                       unsigned int firstGlobalSeed, int passCount, int maxPathDepth,
+                                       int maxPathsInFlight, hipStream_t stream);
+
+/* Measurement hooks of pathTracerAdaptiveRenderWavefront (no reference counterpart).
+ * timing != 0: every launch of the render is bracketed by hipEvents on its stream and the device
+ * time is accumulated per kernel class; counted != 0: the trace kernels also count internal
+ * nodes fetched, triangles tested and rays. Both reset the accumulated values. */
+bool pathTracerWavefrontProfile(int timing, int counted);
+
+/* out[0..3]: ms of trace-extension, trace-shadow, shade and other launches; out[4..7]: their
+ * launch counts; out[8..13]: extension nodes, triangles, rays, shadow nodes, triangles, rays. */
+bool pathTracerWavefrontProfileRead(double* out, int n);
+
 /* Traces one ray per entry (origins/directions as 3 floats each) through the same closest-hit
  * traversal the integrator uses; writes hit flag, t, normal (3), uv (2), triangle index as
  * 8 floats per ray (int fields bit-cast). tMin/tMax as in MeshAccelIntersectCore.h:110-115. */

@@ -53,6 +53,10 @@ SIGNATURES = {
     "pathTracerAdaptiveRenderPasses": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p]),
     "pathTracerAdaptiveRenderPassesCounted": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p,
                                                    _p]),
+    "pathTracerAdaptiveRenderWavefront": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _i, _i,
+                                               _p]),
+    "pathTracerWavefrontProfile": (_b, [_i, _i]),
+    "pathTracerWavefrontProfileRead": (_b, [_p, _i]),
     "pathTracerTraceRays": (_b, [_p, _p, _p, _i, _f, _f, _p, _p]),
     "pathTracerCountTraversal": (_b, [_p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _p, _p]),
     "ptSceneCreate": (_p, []),

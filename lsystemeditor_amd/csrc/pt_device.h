@@ -342,6 +342,21 @@ PT_DEV void rng_init(Rng& r, int pixel, int sample, uint32_t globalSeed)
 
 PT_DEV float u32_to_unit(uint32_t x) { return static_cast<float>((x >> 8) + 1u) * (1.0f / 16777216.0f); }
 
+// Number of uniforms drawn so far, and resumption of a stream after `draws` uniforms (the
+// wavefront kernels keep only this count between launches).
+PT_DEV uint32_t rng_draws(const Rng& r) { return r.block * 4u - static_cast<uint32_t>(r.avail); }
+
+PT_DEV void rng_resume(Rng& r, int pixel, int sample, uint32_t globalSeed, uint32_t draws)
+{
+    rng_init(r, pixel, sample, globalSeed);
+    r.block = draws >> 2;
+    if ((draws & 3u) != 0u) {
+        philox(r.block, r.pixel, r.sample, 0u, r.k0, r.k1, r.buf);
+        r.block += 1u;
+        r.avail = 4 - static_cast<int>(draws & 3u);
+    }
+}
+
 PT_DEV float rand01(Rng& r)
 {
     if (r.avail == 0) {

@@ -1,0 +1,937 @@
+// pt_wavefront.hip -- wavefront-scheduled adaptive renderer for gfx950.
+//
+// The per-pixel result is identical to the reference schedule (one adaptive pass per
+// iteration, PathTracer.cpp:2379-2465 and PathTracer.cu:186-273): pass p uses globalSeed
+// firstSeed + p, a pixel is sampled in every pass until it converges, and convergence is
+// tested only when its count reaches a multiple of 4. The samples a pixel takes between two
+// tests are independent paths, so a group of G consecutive samples of every active pixel is
+// traced at once and the pixel state is then updated sample by sample in order; samples
+// after a convergence inside the group are discarded. Each group runs the path loop
+// (PathIntegratorRandCore.h:206-363) one bounce per launch pair:
+//   trace    -- closest-hit for extension rays, any-hit for shadow rays (lean kernels: the
+//               traversal stack lives in LDS, spilling to global memory past 8 entries);
+//   shade    -- resolves the previous bounce's shadow results in the reference's order,
+//               handles hits and misses, samples the lights and the BSDF, and appends the
+//               next rays to queues (wave-aggregated atomics).
+#include "pt_device.h"
+#include "pt_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+using namespace pt;
+
+namespace {
+
+constexpr int kWfBlock = 256;
+constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
+constexpr int kSpillEntries = 32;  // global spill entries per thread (tree depth <= 28)
+
+enum : uint32_t {
+    kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
+    kFlagEnv = 2u,   // environment NEE shadow ray pending
+    kFlagEmis = 4u,  // emissive NEE shadow ray pending
+};
+
+// Path state, structure of arrays over path slots.
+struct Paths {
+    float *Tx, *Ty, *Tz, *Lx, *Ly, *Lz, *eta, *pdf;
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmin;
+    float *hitT, *hitU, *hitV;
+    float *cex, *cey, *cez, *cmx, *cmy, *cmz;
+    float *rx, *ry, *rz;
+    uint32_t *pixel, *sample, *seed, *draws, *hitTri, *flags;
+    uint8_t* occ;   // 2 per path: environment, emissive
+    int* extQ[2];   // extension ray queues (path ids)
+    int* shadeQ[2]; // shade queues (path ids)
+    float4* shadowQ; // 2 per ray: (origin, path*4 + type), (direction, -)
+    int* counters;  // per bounce: [0] ext, [1] shade, [2] shadow, [3] paths (gen)
+    int* pxBase;
+    int* pxK;
+    int2* spill;
+    int spillThreads;
+};
+
+PT_DEV int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
+
+// Appends one entry per lane with `want` set; returns the lane's slot (or -1).
+PT_DEV int wave_append(int* counter, bool want)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ull) {
+        return -1;
+    }
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    int base = 0;
+    if (lane_id() == leader) {
+        base = atomicAdd(counter, __popcll(mask));
+    }
+    base = __shfl(base, leader, 64);
+    return want ? base + __popcll(mask & ((1ull << lane_id()) - 1ull)) : -1;
+}
+
+// ----------------------------------------------------------------------------------------
+// Traversal with an LDS stack. Same visit order and culling as pt::traverse.
+// ----------------------------------------------------------------------------------------
+struct LdsStack {
+    int2* ring;  // this thread's column of the block's LDS ring
+    int2* spill; // this thread's column of the global spill area
+    int spillStride;
+    int sp, lo;
+
+    PT_DEV void push(int ref, float t)
+    {
+        if (sp - lo == kLdsEntries) {
+            spill[static_cast<size_t>(lo) * spillStride] = ring[(lo & (kLdsEntries - 1)) * kWfBlock];
+            ++lo;
+        }
+        ring[(sp & (kLdsEntries - 1)) * kWfBlock] = make_int2(ref, __float_as_int(t));
+        ++sp;
+    }
+
+    PT_DEV int2 pop()
+    {
+        --sp;
+        if (sp < lo) {
+            lo = sp;
+            return spill[static_cast<size_t>(sp) * spillStride];
+        }
+        return ring[(sp & (kLdsEntries - 1)) * kWfBlock];
+    }
+};
+
+template <bool ANY_HIT, bool STATS>
+PT_DEV bool traverse_lds(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float tMax, LdsStack& st, float& bestT,
+                         uint32_t& bestTri, float& bestU, float& bestV, TraceStats& ts)
+{
+    if (STATS) {
+        ts.rays += 1u;
+    }
+    bestT = tMax;
+    bestTri = 0xFFFFFFFFu;
+    bestU = 0.0f;
+    bestV = 0.0f;
+    if (s.triangleCount == 0) {
+        return false;
+    }
+    float te;
+    if (!slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], tMin, tMax, te)) {
+        return false;
+    }
+    st.sp = 0;
+    st.lo = 0;
+    int cur = s.rootRef;
+    while (true) {
+        if (cur < 0) {
+            const uint32_t slot = static_cast<uint32_t>(~cur);
+            const float4 a = ld4(s.triPos, 3u * slot + 0u);
+            const float4 b = ld4(s.triPos, 3u * slot + 1u);
+            const float4 c = ld4(s.triPos, 3u * slot + 2u);
+            if (STATS) {
+                ts.tris += 1u;
+            }
+            float t, u, v;
+            if (tri_test(r, a, b, c, tMin, bestT, t, u, v)) {
+                bestT = t;
+                bestU = u;
+                bestV = v;
+                bestTri = __float_as_uint(a.w);
+                if (ANY_HIT && t < tMax) {
+                    return true;
+                }
+            }
+        } else {
+            const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
+            const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
+            const float4 n2 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 2u);
+            const float4 n3 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 3u);
+            if (STATS) {
+                ts.nodes += 1u;
+            }
+            float le, re;
+            const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, bestT, le);
+            const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, bestT, re);
+            const int left = __float_as_int(n3.x);
+            const int right = __float_as_int(n3.y);
+            if (lh && rh) {
+                const bool leftFirst = le < re;
+                if (st.sp < kSpillEntries) {
+                    st.push(leftFirst ? right : left, leftFirst ? re : le);
+                }
+                cur = leftFirst ? left : right;
+                continue;
+            }
+            if (lh) {
+                cur = left;
+                continue;
+            }
+            if (rh) {
+                cur = right;
+                continue;
+            }
+        }
+        bool found = false;
+        while (st.sp > 0) {
+            const int2 e = st.pop();
+            if (!(__int_as_float(e.y) > bestT)) {
+                cur = e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) {
+            break;
+        }
+    }
+    return bestT < tMax;
+}
+
+// Closest hit of the extension rays in `queue` (identity when null). STATS adds the internal
+// nodes fetched, triangles tested and rays traced into stats[0..2].
+template <bool STATS>
+__global__ __launch_bounds__(kWfBlock) void wf_trace_ext_kernel(const MeshAccelSceneGpu* scene, Paths P,
+                                                                const int* __restrict__ queue,
+                                                                const int* __restrict__ countPtr,
+                                                                unsigned long long* stats)
+{
+    __shared__ int2 ring[kLdsEntries * kWfBlock];
+    const MeshAccelSceneGpu s = *scene;
+    const int n = *countPtr;
+    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    LdsStack st{ring + threadIdx.x, P.spill + gtid, P.spillThreads, 0, 0};
+    TraceStats ts{0u, 0u, 0u, 0u};
+    for (int i = gtid; i < n; i += static_cast<int>(gridDim.x * blockDim.x)) {
+        const int p = queue != nullptr ? queue[i] : i;
+        const Ray r = make_ray(mk(P.ox[p], P.oy[p], P.oz[p]), mk(P.dx[p], P.dy[p], P.dz[p]));
+        float t, u, v;
+        uint32_t tri;
+        const bool hit = traverse_lds<false, STATS>(s, r, P.tmin[p], kRayTMax, st, t, tri, u, v, ts);
+        P.hitT[p] = hit ? t : 0.0f;
+        P.hitU[p] = u;
+        P.hitV[p] = v;
+        P.hitTri[p] = hit ? tri : 0xFFFFFFFFu;
+    }
+    if (STATS) {
+        atomicAdd(&stats[0], static_cast<unsigned long long>(ts.nodes));
+        atomicAdd(&stats[1], static_cast<unsigned long long>(ts.tris));
+        atomicAdd(&stats[2], static_cast<unsigned long long>(ts.rays));
+    }
+}
+
+// Any hit of the shadow rays (LightSamplingCore.h:187-197, 360-369); STATS into stats[3..5].
+template <bool STATS>
+__global__ __launch_bounds__(kWfBlock) void wf_trace_shadow_kernel(const MeshAccelSceneGpu* scene, Paths P,
+                                                                   const int* __restrict__ countPtr,
+                                                                   unsigned long long* stats)
+{
+    __shared__ int2 ring[kLdsEntries * kWfBlock];
+    const MeshAccelSceneGpu s = *scene;
+    const int n = *countPtr;
+    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    LdsStack st{ring + threadIdx.x, P.spill + gtid, P.spillThreads, 0, 0};
+    TraceStats ts{0u, 0u, 0u, 0u};
+    for (int i = gtid; i < n; i += static_cast<int>(gridDim.x * blockDim.x)) {
+        const float4 a = P.shadowQ[2 * i];
+        const float4 b = P.shadowQ[2 * i + 1];
+        const Ray r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+        float t, u, v;
+        uint32_t tri;
+        const bool occluded = traverse_lds<true, STATS>(s, r, kShadowBias, kRayTMax, st, t, tri, u, v, ts);
+        const uint32_t meta = __float_as_uint(a.w);
+        P.occ[2 * (meta >> 2) + ((meta & 3u) - 1u)] = occluded ? 1u : 0u;
+    }
+    if (STATS) {
+        atomicAdd(&stats[3], static_cast<unsigned long long>(ts.nodes));
+        atomicAdd(&stats[4], static_cast<unsigned long long>(ts.tris));
+        atomicAdd(&stats[5], static_cast<unsigned long long>(ts.rays));
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Generation: camera rays for G consecutive samples of every active pixel (PathTracer.cu:69-99).
+// ----------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict__ active,
+                                                          const int* __restrict__ activeCountPtr,
+                                                          const uint32_t* __restrict__ counts,
+                                                          const uint8_t* __restrict__ converged,
+                                                          const CameraGpu* camera, const RenderParamsGpu* params,
+                                                          int width, int height, uint32_t firstSeed, int passBase,
+                                                          int passCount, int groupSize, Paths P)
+{
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int activeCount = *activeCountPtr;
+    int idx = -1;
+    int k = 0;
+    uint32_t n = 0u;
+    if (t < activeCount) {
+        idx = active[t];
+        if (idx >= 0 && converged[idx] == 0) {
+            n = counts[idx];
+            k = min(groupSize, passCount - passBase);
+            const int maxS = params->maxSamplesPerPixel;
+            if (maxS > 0) {
+                k = min(k, maxS - static_cast<int>(n));
+            }
+            k = max(k, 0);
+        }
+    }
+    // wave-level exclusive scan of k, one atomic per wave
+    int incl = k;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane_id() >= off) {
+            incl += v;
+        }
+    }
+    const int total = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane_id() == 63 && total > 0) {
+        base = atomicAdd(&P.counters[3], total);
+    }
+    base = __shfl(base, 63, 64) + incl - k;
+    if (t >= activeCount) {
+        return;
+    }
+    P.pxBase[t] = base;
+    P.pxK[t] = k;
+    if (k == 0) {
+        return;
+    }
+    const CameraGpu cam = *camera;
+    const int x = idx % width;
+    const int y = idx / width;
+    for (int j = 0; j < k; ++j) {
+        const int p = base + j;
+        const uint32_t seed = firstSeed + static_cast<uint32_t>(passBase + j);
+        Rng rng;
+        rng_init(rng, idx, static_cast<int>(n) + j, seed);
+        const float ju = rand01(rng);
+        const float jv = rand01(rng);
+        const float u = (static_cast<float>(x) + ju) / static_cast<float>(width);
+        const float v = (static_cast<float>(y) + jv) / static_cast<float>(height);
+        const float l1 = rand01(rng);
+        const float l2 = rand01(rng);
+        V3 ro, rd;
+        camera_ray(cam, u, v, l1, l2, ro, rd);
+        P.pixel[p] = static_cast<uint32_t>(idx);
+        P.sample[p] = n + static_cast<uint32_t>(j);
+        P.seed[p] = seed;
+        P.draws[p] = rng_draws(rng);
+        P.ox[p] = ro.x;
+        P.oy[p] = ro.y;
+        P.oz[p] = ro.z;
+        P.dx[p] = rd.x;
+        P.dy[p] = rd.y;
+        P.dz[p] = rd.z;
+        P.tmin[p] = 0.0f;
+        P.Tx[p] = 1.0f;
+        P.Ty[p] = 1.0f;
+        P.Tz[p] = 1.0f;
+        P.Lx[p] = 0.0f;
+        P.Ly[p] = 0.0f;
+        P.Lz[p] = 0.0f;
+        P.eta[p] = 1.0f;
+        P.pdf[p] = 0.0f;
+        P.flags[p] = kFlagExt;
+    }
+}
+
+PT_DEV void push_shadow(Paths& P, int* counter, bool want, V3 o, V3 d, int path, uint32_t type)
+{
+    const int slot = wave_append(counter, want);
+    if (want) {
+        P.shadowQ[2 * slot] = make_float4(o.x, o.y, o.z, __uint_as_float((static_cast<uint32_t>(path) << 2) | type));
+        P.shadowQ[2 * slot + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+}
+
+// One bounce of PathIntegratorRandCore.h:241-347 for every path in the shade queue.
+__global__ __launch_bounds__(kWfBlock) void wf_shade_kernel(const MeshAccelSceneGpu* scene,
+                                                            const RenderParamsGpu* params,
+                                                            const EnvironmentMapGpu* env, Paths P, int bounce,
+                                                            int maxDepth, const int* __restrict__ inQueue,
+                                                            const int* __restrict__ inCountPtr, int* extOut,
+                                                            int* shadeOut, int* counters)
+{
+    const MeshAccelSceneGpu s = *scene;
+    const RenderParamsGpu prm = *params;
+    const int n = *inCountPtr;
+    const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    // uniform trip count so the wave-level appends see every lane of the wave
+    const int trips = (n + stride - 1) / stride;
+    for (int it = 0; it < trips; ++it) {
+        const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
+        const bool live = i < n;
+        const int p = live ? (inQueue != nullptr ? inQueue[i] : i) : 0;
+        bool wantExt = false, wantEnv = false, wantEmis = false;
+        V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO;
+        if (live) {
+            uint32_t flags = P.flags[p];
+            V3 L = mk(P.Lx[p], P.Ly[p], P.Lz[p]);
+            // shadow results of the previous bounce, in the reference's order (env, then emissive)
+            if ((flags & kFlagEnv) != 0u && P.occ[2 * p] == 0u) {
+                L = add(L, mk(P.cex[p], P.cey[p], P.cez[p]));
+            }
+            if ((flags & kFlagEmis) != 0u && P.occ[2 * p + 1] == 0u) {
+                L = add(L, mk(P.cmx[p], P.cmy[p], P.cmz[p]));
+            }
+            uint32_t newFlags = 0u;
+            if ((flags & kFlagExt) != 0u) {
+                const V3 origin = mk(P.ox[p], P.oy[p], P.oz[p]);
+                const V3 dir = mk(P.dx[p], P.dy[p], P.dz[p]);
+                const uint32_t tri = P.hitTri[p];
+                if (tri == 0xFFFFFFFFu) {
+                    if (bounce == 0) {
+                        L = env_eval(env, prm, dir);  // PathIntegratorRandCore.h:215-217
+                    } else {                            // :338-345
+                        const V3 envL = env_eval(env, prm, dir);
+                        const float lpdf = env_pdf(env, dir);
+                        const float mis = mis_balance(P.pdf[p], lpdf);
+                        const V3 T = mk(P.Tx[p], P.Ty[p], P.Tz[p]);
+                        L = add(L, mul(T, scale(envL, mis)));
+                    }
+                } else if (bounce < maxDepth) {
+                    const float t = P.hitT[p];
+                    const Ray ray = make_ray(origin, dir);
+                    Hit h;
+                    h.hit = true;
+                    h.t = t;
+                    finish_hit(s, ray, tri, P.hitU[p], P.hitV[p], h);
+                    Rng rng;
+                    rng_resume(rng, static_cast<int>(P.pixel[p]), static_cast<int>(P.sample[p]), P.seed[p],
+                               P.draws[p]);
+                    V3 T = mk(P.Tx[p], P.Ty[p], P.Tz[p]);
+                    const V3 pos = add(origin, scale(dir, t));
+                    Ctx c;
+                    c.n = h.normal;
+                    c.wo = scale(dir, -1.0f);
+                    c.m = resolve_material(s, h.tri, h.uvx, h.uvy);
+                    c.w = lobe_weights(c.m);
+                    c.eta = P.eta[p];
+                    c.flags = 0;
+                    L = add(L, mul(T, mk(c.m.r * c.m.emission, c.m.g * c.m.emission, c.m.b * c.m.emission)));
+                    // environment NEE (:65-108): value computed now, added if the shadow ray is clear
+                    if (!skips_env_nee(c.m)) {
+                        const float u1 = rand01(rng);
+                        const float u2 = rand01(rng);
+                        float lightPdf = 0.0f;
+                        const V3 wi = env_sample(env, u1, u2, lightPdf);
+                        if (lightPdf > kMinPdf) {
+                            const float cosT = max2(0.0f, dot(c.n, wi));
+                            if (cosT > 0.0f) {
+                                const V3 Le = env_eval(env, prm, wi);
+                                const V3 f = brdf_eval(c, wi);
+                                const float bpdf = brdf_pdf(c, wi);
+                                const float mis = mis_balance(lightPdf, bpdf);
+                                const float sc = mis * cosT / lightPdf;
+                                const V3 cv = mul(T, mul(f, scale(Le, sc)));
+                                P.cex[p] = cv.x;
+                                P.cey[p] = cv.y;
+                                P.cez[p] = cv.z;
+                                wantEnv = true;
+                                envO = add(pos, scale(c.n, kShadowBias));
+                                envD = wi;
+                            }
+                        }
+                    }
+                    // emissive-triangle NEE (:110-167)
+                    if (s.emissiveTriangleCount != 0u) {
+                        const float uTri = rand01(rng);
+                        const float u1 = rand01(rng);
+                        const float u2 = rand01(rng);
+                        V3 lp, ln, lr;
+                        float areaPdf = 0.0f;
+                        if (sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
+                            const V3 toL = sub(lp, pos);
+                            const float d2 = dot(toL, toL);
+                            if (d2 > kMinPdf) {
+                                const V3 wi = normalize(toL);
+                                const float cosT = max2(0.0f, dot(c.n, wi));
+                                const float cosL = max2(0.0f, dot(ln, scale(wi, -1.0f)));
+                                if (cosT > 0.0f && cosL > 0.0f) {
+                                    const float lightPdf = areaPdf * d2 / cosL;
+                                    if (lightPdf > kMinPdf) {
+                                        const V3 f = brdf_eval(c, wi);
+                                        const float bpdf = brdf_pdf(c, wi);
+                                        const float mis = mis_balance(lightPdf, bpdf);
+                                        const float sc = mis * cosT / lightPdf;
+                                        const V3 cv = mul(T, mul(f, scale(lr, sc)));
+                                        P.cmx[p] = cv.x;
+                                        P.cmy[p] = cv.y;
+                                        P.cmz[p] = cv.z;
+                                        wantEmis = true;
+                                        emO = add(pos, scale(wi, kShadowBias));
+                                        emD = wi;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    // BSDF sampling, throughput, Russian roulette, continuation (:280-337)
+                    c.flags = prm.brdfDebugFlags;
+                    const float u1 = rand01(rng);
+                    const float u2 = rand01(rng);
+                    const Sample smp = brdf_sample(c, u1, u2);
+                    bool cont = smp.valid && smp.pdf > kMinPdf;
+                    if (cont) {
+                        const V3 f = brdf_eval(c, smp.dir);
+                        T = apply_throughput(T, c, smp, f);
+                        if (smp.sss && smp.steps > 0) {
+                            const float mr = max_radius(c.m);
+                            if (mr > kMinRadius) {
+                                const V3 albedo = mk(c.m.r, c.m.g, c.m.b);
+                                for (int step = 0; step < smp.steps; ++step) {
+                                    (void)rand01(rng);
+                                    (void)rand01(rng);
+                                    const float u3 = rand01(rng);
+                                    const float mfp = mr * (-logf_pt(max2(1.0e-8f, u3)));
+                                    T = mul(T, scale(albedo, expf_pt(-mfp / mr)));
+                                }
+                                const float ed = length(smp.exitOffset);
+                                if (ed > 0.0f) {
+                                    T = scale(T, expf_pt(-ed / mr));
+                                }
+                            }
+                        }
+                        if ((c.flags & 16) != 0) {
+                            T = smp.transmitted ? mul(T, mk(0.2f, 1.0f, 0.2f)) : mul(T, mk(1.0f, 0.2f, 0.2f));
+                        }
+                        const float tl = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+                        if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
+                            cont = false;
+                        }
+                        if (cont && bounce >= rrMin) {
+                            const float surv = min2(0.95f, max2(tl, 0.05f));
+                            if (rand01(rng) > surv) {
+                                cont = false;
+                            } else {
+                                T = scale(T, 1.0f / surv);
+                            }
+                        }
+                    }
+                    if (cont) {
+                        const float bias = max2(kShadowBias, kRelShadowBias * t);
+                        const float sgn = dot(c.n, smp.dir) >= 0.0f ? 1.0f : -1.0f;
+                        const V3 o2 = add(pos, scale(c.n, sgn * bias));
+                        P.ox[p] = o2.x;
+                        P.oy[p] = o2.y;
+                        P.oz[p] = o2.z;
+                        P.dx[p] = smp.dir.x;
+                        P.dy[p] = smp.dir.y;
+                        P.dz[p] = smp.dir.z;
+                        P.tmin[p] = bias;
+                        P.eta[p] = smp.nextEta;
+                        P.pdf[p] = smp.pdf;
+                        P.Tx[p] = T.x;
+                        P.Ty[p] = T.y;
+                        P.Tz[p] = T.z;
+                        wantExt = true;
+                    }
+                    P.draws[p] = rng_draws(rng);
+                }
+            }
+            newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
+            P.flags[p] = newFlags;
+            if (newFlags == 0u) {
+                const V3 r = clamp_firefly(L);  // PathTracer.cu:101-104
+                P.rx[p] = r.x;
+                P.ry[p] = r.y;
+                P.rz[p] = r.z;
+            } else {
+                P.Lx[p] = L.x;
+                P.Ly[p] = L.y;
+                P.Lz[p] = L.z;
+            }
+        }
+        const int ext = wave_append(&counters[0], wantExt);
+        if (wantExt) {
+            extOut[ext] = p;
+        }
+        const int sh = wave_append(&counters[1], wantExt || wantEnv || wantEmis);
+        if (wantExt || wantEnv || wantEmis) {
+            shadeOut[sh] = p;
+        }
+        push_shadow(P, &counters[2], wantEnv, envO, envD, p, 1u);
+        push_shadow(P, &counters[2], wantEmis, emO, emD, p, 2u);
+    }
+}
+
+// Applies each active pixel's group samples in order (PathTracer.cu:233-272).
+__global__ __launch_bounds__(kWfBlock) void wf_accumulate_kernel(const int* __restrict__ active,
+                                                                 const int* __restrict__ activeCountPtr, Paths P,
+                                                                 float4* __restrict__ acc, uint32_t* __restrict__ counts,
+                                                                 float* __restrict__ lumMean, float* __restrict__ m2,
+                                                                 uint8_t* __restrict__ converged,
+                                                                 const RenderParamsGpu* params)
+{
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= *activeCountPtr) {
+        return;
+    }
+    const int k = P.pxK[t];
+    if (k == 0) {
+        return;
+    }
+    const int idx = active[t];
+    const int base = P.pxBase[t];
+    const RenderParamsGpu prm = *params;
+    uint32_t n = counts[idx];
+    float4 a = acc[idx];
+    float mean = lumMean[idx];
+    float mm = m2[idx];
+    bool conv = false;
+    for (int j = 0; j < k && !conv; ++j) {
+        const int p = base + j;
+        const V3 r = mk(P.rx[p], P.ry[p], P.rz[p]);
+        const float sl = r.x * 0.2126f + r.y * 0.7152f + r.z * 0.0722f;
+        const float delta = sl - mean;
+        const float nm = mean + delta / static_cast<float>(n + 1u);
+        const float nM2 = mm + delta * (sl - nm);
+        mean = nm;
+        mm = nM2;
+        const float inv = 1.0f / static_cast<float>(n + 1u);
+        a = make_float4((a.x * static_cast<float>(n) + r.x) * inv, (a.y * static_cast<float>(n) + r.y) * inv,
+                        (a.z * static_cast<float>(n) + r.z) * inv, 1.0f);
+        const uint32_t nc = n + 1u;
+        n = nc;
+        if (prm.maxSamplesPerPixel > 0 && nc >= static_cast<uint32_t>(prm.maxSamplesPerPixel)) {
+            conv = true;
+        } else if ((nc % 4u) == 0u && nc >= static_cast<uint32_t>(prm.minSamples) && nm >= 1.0e-5f) {
+            const float sd = sqrtf(nM2 / static_cast<float>(nc));
+            conv = (sd / fmaxf(nm, 1.0e-4f)) < prm.relativeErrorThreshold;
+        }
+    }
+    acc[idx] = a;
+    counts[idx] = n;
+    lumMean[idx] = mean;
+    m2[idx] = mm;
+    if (conv) {
+        converged[idx] = 1u;
+    }
+}
+
+// Keeps the active entries that are still unconverged, in order (device-side count).
+__global__ void wf_compact_kernel(const int* __restrict__ in, const int* __restrict__ inCountPtr,
+                                  const uint8_t* __restrict__ converged, int* __restrict__ out, int* outCount)
+{
+    const int n = *inCountPtr;
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    const int trips = (n + stride - 1) / stride;
+    for (int it = 0; it < trips; ++it) {
+        const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
+        const int idx = t < n ? in[t] : -1;
+        const bool keep = idx >= 0 && converged[idx] == 0u;
+        const int slot = wave_append(outCount, keep);
+        if (keep) {
+            out[slot] = idx;
+        }
+    }
+}
+
+__global__ void wf_zero_kernel(int* p, int n)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) {
+        p[i] = 0;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Host side: workspace and the group loop.
+// ----------------------------------------------------------------------------------------
+struct Workspace {
+    void* mem = nullptr;
+    size_t bytes = 0;
+    int capacity = 0;  // paths
+    int pixels = 0;    // active-list entries
+    int depth = 0;
+    int spillThreads = 0;
+    Paths P{};
+    int* lists[2] = {nullptr, nullptr};
+    int* listCount = nullptr; // [2]
+};
+Workspace gWs;
+
+// Opt-in measurement: hipEvents around every launch of the render stream, accumulated per
+// kernel class, and (counted mode) traversal counters of the trace kernels.
+struct Profile {
+    bool timing = false;
+    bool counted = false;
+    double ms[4] = {0, 0, 0, 0};      // trace ext, trace shadow, shade, other
+    long long launches[4] = {0, 0, 0, 0};
+    unsigned long long* dStats = nullptr;  // 6 counters
+    unsigned long long stats[6] = {0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[2 * 2048] = {};
+    int cls[2048] = {};
+    int used = 0;
+};
+Profile gProf;
+
+void prof_mark(hipStream_t stream, int cls, bool end)
+{
+    if (!gProf.timing || gProf.used >= 2048) {
+        return;
+    }
+    const int i = gProf.used;
+    if (gProf.ev[2 * i + (end ? 1 : 0)] == nullptr) {
+        (void)hipEventCreate(&gProf.ev[2 * i + (end ? 1 : 0)]);
+    }
+    (void)hipEventRecord(gProf.ev[2 * i + (end ? 1 : 0)], stream);
+    if (end) {
+        gProf.cls[i] = cls;
+        gProf.used = i + 1;
+    }
+}
+
+void prof_collect()
+{
+    for (int i = 0; i < gProf.used; ++i) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, gProf.ev[2 * i], gProf.ev[2 * i + 1]) == hipSuccess) {
+            gProf.ms[gProf.cls[i]] += ms;
+            gProf.launches[gProf.cls[i]] += 1;
+        }
+    }
+    gProf.used = 0;
+}
+
+template <typename T>
+T* carve(char*& cur, size_t count)
+{
+    cur = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(cur) + 255u) & ~static_cast<uintptr_t>(255u));
+    T* p = reinterpret_cast<T*>(cur);
+    cur += count * sizeof(T);
+    return p;
+}
+
+size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spillThreads)
+{
+    char* cur = base;
+    Paths& P = w.P;
+    const size_t c = static_cast<size_t>(cap);
+    float** fs[] = {&P.Tx, &P.Ty, &P.Tz, &P.Lx, &P.Ly, &P.Lz, &P.eta, &P.pdf, &P.ox, &P.oy, &P.oz, &P.dx, &P.dy,
+                    &P.dz, &P.tmin, &P.hitT, &P.hitU, &P.hitV, &P.cex, &P.cey, &P.cez, &P.cmx, &P.cmy, &P.cmz,
+                    &P.rx, &P.ry, &P.rz};
+    for (float** f : fs) {
+        *f = carve<float>(cur, c);
+    }
+    uint32_t** us[] = {&P.pixel, &P.sample, &P.seed, &P.draws, &P.hitTri, &P.flags};
+    for (uint32_t** u : us) {
+        *u = carve<uint32_t>(cur, c);
+    }
+    P.occ = carve<uint8_t>(cur, 2 * c);
+    P.extQ[0] = carve<int>(cur, c);
+    P.extQ[1] = carve<int>(cur, c);
+    P.shadeQ[0] = carve<int>(cur, c);
+    P.shadeQ[1] = carve<int>(cur, c);
+    P.shadowQ = carve<float4>(cur, 4 * c);
+    P.counters = carve<int>(cur, 4 * static_cast<size_t>(depth + 2));
+    P.pxBase = carve<int>(cur, static_cast<size_t>(pixels));
+    P.pxK = carve<int>(cur, static_cast<size_t>(pixels));
+    P.spill = carve<int2>(cur, static_cast<size_t>(spillThreads) * kSpillEntries);
+    P.spillThreads = spillThreads;
+    w.lists[0] = carve<int>(cur, static_cast<size_t>(pixels));
+    w.lists[1] = carve<int>(cur, static_cast<size_t>(pixels));
+    w.listCount = carve<int>(cur, 4);
+    return static_cast<size_t>(cur - base) + 256;
+}
+
+bool ensure_workspace(int cap, int pixels, int depth, int spillThreads)
+{
+    if (gWs.mem != nullptr && gWs.capacity >= cap && gWs.pixels >= pixels && gWs.depth >= depth &&
+        gWs.spillThreads >= spillThreads) {
+        return true;
+    }
+    cap = std::max(cap, gWs.capacity);
+    pixels = std::max(pixels, gWs.pixels);
+    depth = std::max(depth, gWs.depth);
+    spillThreads = std::max(spillThreads, gWs.spillThreads);
+    Workspace probe;
+    const size_t bytes = layout(probe, nullptr, cap, pixels, depth, spillThreads);
+    if (gWs.mem != nullptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(gWs.mem);
+        gWs.mem = nullptr;
+    }
+    void* mem = nullptr;
+    if (hipMalloc(&mem, bytes) != hipSuccess) {
+        ptSetError("wavefront workspace allocation of %zu bytes failed", bytes);
+        gWs = Workspace{};
+        return false;
+    }
+    gWs.mem = mem;
+    gWs.bytes = bytes;
+    gWs.capacity = cap;
+    gWs.pixels = pixels;
+    gWs.depth = depth;
+    gWs.spillThreads = spillThreads;
+    layout(gWs, static_cast<char*>(mem), cap, pixels, depth, spillThreads);
+    return true;
+}
+
+unsigned blocks_for(long long n) { return static_cast<unsigned>(std::max<long long>(1, (n + kWfBlock - 1) / kWfBlock)); }
+
+} // namespace
+
+extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
+                                                  float* d_m2, uint8_t* d_converged, const int* d_activeIndices,
+                                                  int activeCount, int width, int height, const CameraGpu* d_camera,
+                                                  const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                                                  const RenderParamsGpu* d_params, unsigned int firstGlobalSeed,
+                                                  int passCount, int maxPathDepth, int maxPathsInFlight,
+                                                  hipStream_t stream)
+{
+    if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
+        d_converged == nullptr || d_activeIndices == nullptr || d_camera == nullptr || d_params == nullptr ||
+        d_scene == nullptr || width <= 0 || height <= 0 || activeCount <= 0 || passCount <= 0 ||
+        maxPathDepth < 0 || maxPathDepth > 256) {
+        ptSetError("pathTracerAdaptiveRenderWavefront: invalid arguments");
+        return false;
+    }
+    // maxPathDepth must equal the depth the params resolve to (PathIntegratorRandCore.h:219-227);
+    // it is passed by value because the params live in device memory.
+    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 22);
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) == hipSuccess) {
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    const int persistentBlocks = cus * 8;
+    const int spillThreads = persistentBlocks * kWfBlock;
+    const int cap = std::max(4 * activeCount, std::min(target, 64 * activeCount));
+    if (!ensure_workspace(cap, activeCount, maxPathDepth, spillThreads)) {
+        return false;
+    }
+    Paths P = gWs.P;
+    int* list = gWs.lists[0];
+    int* listNext = gWs.lists[1];
+    int* countCur = gWs.listCount;
+    int* countNext = gWs.listCount + 1;
+    if (hipMemcpyAsync(list, d_activeIndices, sizeof(int) * static_cast<size_t>(activeCount), hipMemcpyDeviceToDevice,
+                       stream) != hipSuccess ||
+        hipMemcpyAsync(countCur, &activeCount, sizeof(int), hipMemcpyHostToDevice, stream) != hipSuccess) {
+        ptSetError("pathTracerAdaptiveRenderWavefront: list copy failed");
+        return false;
+    }
+    int active = activeCount;
+    int passBase = 0;
+    const int counterInts = 4 * (maxPathDepth + 2);
+    while (active > 0 && passBase < passCount) {
+        // group size: 4 samples per pixel (one convergence interval), widened speculatively
+        // while few pixels remain so the GPU stays full
+        int mult = std::max(1, std::min(16, gWs.capacity / std::max(1, 4 * active)));
+        int group = 4 * mult;
+        group = std::min(group, passCount - passBase);
+        const long long maxPaths = static_cast<long long>(group) * active;
+        hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
+                           counterInts);
+        hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur,
+                           d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
+                           passCount, group, P);
+        const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
+        const unsigned shadowBlocks =
+            static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
+        for (int b = 0; b <= maxPathDepth; ++b) {
+            int* cIn = P.counters + 4 * b;        // produced by bounce b-1 (or gen for b == 0)
+            int* cOut = P.counters + 4 * (b + 1); // produced by this bounce
+            const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
+            const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
+            prof_mark(stream, 0, false);
+            if (gProf.counted) {
+                hipLaunchKernelGGL(wf_trace_ext_kernel<true>, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, P,
+                                   extQueue, extCount, gProf.dStats);
+            } else {
+                hipLaunchKernelGGL(wf_trace_ext_kernel<false>, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene,
+                                   P, extQueue, extCount, nullptr);
+            }
+            prof_mark(stream, 0, true);
+            if (b > 0) {
+                prof_mark(stream, 1, false);
+                if (gProf.counted) {
+                    hipLaunchKernelGGL(wf_trace_shadow_kernel<true>, dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                       d_scene, P, &cIn[2], gProf.dStats);
+                } else {
+                    hipLaunchKernelGGL(wf_trace_shadow_kernel<false>, dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                       d_scene, P, &cIn[2], nullptr);
+                }
+                prof_mark(stream, 1, true);
+            }
+            const int* shadeQueue = b == 0 ? nullptr : P.shadeQ[b & 1];
+            const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
+            prof_mark(stream, 2, false);
+            hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
+                               P, b, maxPathDepth, shadeQueue, shadeCount, P.extQ[(b + 1) & 1],
+                               P.shadeQ[(b + 1) & 1], cOut);
+            prof_mark(stream, 2, true);
+        }
+        hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
+                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
+        hipLaunchKernelGGL(wf_zero_kernel, dim3(1), dim3(kWfBlock), 0, stream, countNext, 1);
+        hipLaunchKernelGGL(wf_compact_kernel, dim3(std::min<unsigned>(blocks_for(active), 4096u)), dim3(kWfBlock), 0,
+                           stream, list, countCur, d_converged, listNext, countNext);
+        if (hipGetLastError() != hipSuccess) {
+            ptSetError("pathTracerAdaptiveRenderWavefront: launch failed");
+            return false;
+        }
+        if (hipMemcpyAsync(&active, countNext, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            ptSetError("pathTracerAdaptiveRenderWavefront: count readback failed");
+            return false;
+        }
+        prof_collect();
+        std::swap(list, listNext);
+        std::swap(countCur, countNext);
+        passBase += group;
+    }
+    return true;
+}
+
+extern "C" bool pathTracerWavefrontProfile(int timing, int counted)
+{
+    gProf.timing = timing != 0;
+    gProf.counted = counted != 0;
+    for (int k = 0; k < 4; ++k) {
+        gProf.ms[k] = 0.0;
+        gProf.launches[k] = 0;
+    }
+    gProf.used = 0;
+    if (gProf.counted) {
+        if (gProf.dStats == nullptr && hipMalloc(reinterpret_cast<void**>(&gProf.dStats), 6 * sizeof(unsigned long long)) != hipSuccess) {
+            ptSetError("pathTracerWavefrontProfile: allocation failed");
+            gProf.counted = false;
+            return false;
+        }
+        if (hipMemset(gProf.dStats, 0, 6 * sizeof(unsigned long long)) != hipSuccess) {
+            ptSetError("pathTracerWavefrontProfile: reset failed");
+            return false;
+        }
+    }
+    return true;
+}
+
+// out[0..3]: milliseconds of trace-ext, trace-shadow, shade, other launches; out[4..7]: launch
+// counts; out[8..13]: ext nodes, ext triangles, ext rays, shadow nodes, shadow triangles,
+// shadow rays (counted mode). Collected since the last pathTracerWavefrontProfile call.
+extern "C" bool pathTracerWavefrontProfileRead(double* out, int n)
+{
+    if (out == nullptr || n < 14) {
+        ptSetError("pathTracerWavefrontProfileRead: need 14 doubles");
+        return false;
+    }
+    unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+    if (gProf.dStats != nullptr &&
+        hipMemcpy(st, gProf.dStats, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) {
+        ptSetError("pathTracerWavefrontProfileRead: copy failed");
+        return false;
+    }
+    for (int k = 0; k < 4; ++k) {
+        out[k] = gProf.ms[k];
+        out[4 + k] = static_cast<double>(gProf.launches[k]);
+    }
+    for (int k = 0; k < 6; ++k) {
+        out[8 + k] = static_cast<double>(st[k]);
+    }
+    return true;
+}

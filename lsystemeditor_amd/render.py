@@ -129,6 +129,28 @@ def render_persistent(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, 
         "pathTracerAdaptiveRenderPasses")
 
 
+def resolved_depth(params_host: np.ndarray) -> int:
+    """maxPathDepth as the integrator resolves it (PathIntegratorRandCore.h:219-227)."""
+    d = int(params_host[4:5].view(np.int32)[0])
+    return d if d > 0 else (256 if d == 0 else 32)
+
+
+def render_wavefront(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, first_iter: int = 0,
+                     passes: int | None = None, clear: bool = True, max_paths: int = 0, stream=None) -> None:
+    """Same per-pixel result as render_loop, wavefront-scheduled (pathTracerAdaptiveRenderWavefront)."""
+    if clear:
+        acc.clear(stream)
+    if passes is None:
+        passes = inputs.max_spp
+        if passes <= 0:
+            raise ValueError("render_wavefront needs a finite pass count when maxSamplesPerPixel is 0")
+    abi.check(abi.lib().pathTracerAdaptiveRenderWavefront(
+        acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
+        acc.active.data_ptr(), acc.width * acc.height, acc.width, acc.height, inputs.camera.data_ptr(),
+        scene.device, inputs.env_ptr, inputs.params.data_ptr(), first_iter + 1, passes,
+        resolved_depth(inputs.params_host), max_paths, _stream_handle(stream)), "pathTracerAdaptiveRenderWavefront")
+
+
 def trace_rays(scene: abi.Scene, origins: np.ndarray, dirs: np.ndarray, tmin: float = 0.0,
                tmax: float = 1.0e6) -> np.ndarray:
     """Closest hits through the renderer's traversal: (N, 8) float32 (hit, t, normal, uv, tri bits)."""

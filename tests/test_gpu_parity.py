@@ -139,6 +139,32 @@ def test_persistent_passes_equal_loop(cuda, render_mod):
         assert np.array_equal(np.asarray(ha[k]).view(np.uint8), np.asarray(hb[k]).view(np.uint8)), k
 
 
+@pytest.mark.parametrize("kind,spp,min_spp,thr,max_paths", [
+    ("zoo", 8, None, 0.001, 0), ("plain", 8, None, 0.001, 0), ("zoo", 64, 4, 0.08, 0),
+    ("zoo", 37, 5, 0.08, 0), ("zoo", 64, 4, 0.08, 3000)])
+def test_wavefront_matches_oracle(cuda, render_mod, kind, spp, min_spp, thr, max_paths):
+    """Wavefront schedule (grouped samples, speculative groups when few pixels remain, maxSpp
+    not a multiple of 4, tiny path budgets) against the oracle's pass-by-pass schedule."""
+    w = h = 40
+    tris, mats, tex, cam, params = dense_plant(w, h, spp=spp, min_spp=min_spp, threshold=thr, materials=kind)
+    sc = abi.Scene(tris, mats, tex)
+    ref = oracle_scene(tris, mats, tex)
+    acc_o, _ = O.render_adaptive(ref, cam, params, w, h, threads=8)
+    dev = render_mod.Accumulator.allocate(w, h)
+    render_mod.render_wavefront(sc, dev, render_mod.DeviceInputs(cam, params), max_paths=max_paths)
+    compare_accumulators(dev.host(), acc_o, f"wavefront/{kind}/{spp}/{max_paths}")
+
+
+def test_wavefront_c0_matches_oracle(cuda, render_mod):
+    s = scenes.config0()
+    sc = abi.Scene(s.triangles, s.materials)
+    ref = oracle_scene(s.triangles, s.materials)
+    acc_o, _ = O.render_adaptive(ref, s.camera, s.params, s.width, s.height, threads=8)
+    dev = render_mod.Accumulator.allocate(s.width, s.height)
+    render_mod.render_wavefront(sc, dev, render_mod.DeviceInputs(s.camera, s.params))
+    compare_accumulators(dev.host(), acc_o, "wavefront/C0")
+
+
 def test_thin_lens_matches_oracle(cuda, render_mod):
     w = h = 32
     tris, mats, tex, cam, params = dense_plant(w, h, spp=4, thin_lens=True)
