@@ -28,6 +28,7 @@ namespace {
 constexpr int kWfBlock = 256;
 constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
 constexpr int kSpillEntries = 32;  // global spill entries per thread (tree depth <= 28)
+constexpr int kCountersPerBounce = 8;
 
 enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
@@ -47,7 +48,8 @@ struct Paths {
     int* extQ[2];   // extension ray queues (path ids)
     int* shadeQ[2]; // shade queues (path ids)
     float4* shadowQ; // 2 per ray: (origin, path*4 + type), (direction, -)
-    int* counters;  // per bounce: [0] ext, [1] shade, [2] shadow, [3] paths (gen)
+    int* counters;  // per bounce: [0] ext, [1] shade, [2] shadow rays produced, [3] paths (gen, bounce 0),
+                    // [4] extension and [5] shadow fetch positions of the trace kernels
     int* pxBase;
     int* pxK;
     int2* spill;
@@ -102,150 +104,170 @@ struct LdsStack {
     }
 };
 
+constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf refs are ~slot >= -2^28)
+
+// Persistent traversal: every lane takes the next ray from the queue as soon as its current ray
+// finishes (one atomic per wave for all lanes that need a ray), so the lanes of a wave stay busy
+// while per-ray node counts vary widely. One loop iteration advances a lane by one node: an
+// internal node (both child boxes tested, near child first, far child deferred on the LDS stack
+// with its entry distance), or a leaf triangle, followed by pops that skip deferred children whose
+// entry distance exceeds the current closest t. Visit order and culling equal pt::traverse.
+//   ANY_HIT = false: extension rays of the shade queue (path ids; identity when queue is null),
+//                    results to hitT/hitU/hitV/hitTri.
+//   ANY_HIT = true:  shadow rays of shadowQ, result to occ.
 template <bool ANY_HIT, bool STATS>
-PT_DEV bool traverse_lds(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float tMax, LdsStack& st, float& bestT,
-                         uint32_t& bestTri, float& bestU, float& bestV, TraceStats& ts)
+__global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
+                                                            const int* __restrict__ queue,
+                                                            const int* __restrict__ countPtr, int* fetchCounter,
+                                                            unsigned long long* stats)
 {
-    if (STATS) {
-        ts.rays += 1u;
-    }
-    bestT = tMax;
-    bestTri = 0xFFFFFFFFu;
-    bestU = 0.0f;
-    bestV = 0.0f;
-    if (s.triangleCount == 0) {
-        return false;
-    }
-    float te;
-    if (!slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], tMin, tMax, te)) {
-        return false;
-    }
-    st.sp = 0;
-    st.lo = 0;
-    int cur = s.rootRef;
+    __shared__ int2 ring[kLdsEntries * kWfBlock];
+    const MeshAccelSceneGpu s = *scene;
+    const int n = *countPtr;
+    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    LdsStack st{ring + threadIdx.x, P.spill + gtid, P.spillThreads, 0, 0};
+    TraceStats ts{0u, 0u, 0u, 0u};
+    const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
+    int item = -1;        // queue index of the lane's current ray
+    bool exhausted = false;
+    uint32_t tag = 0u;    // path id (extension) or path * 4 + type (shadow)
+    Ray r;
+    r.o = mk(0.0f, 0.0f, 0.0f);
+    r.d = r.o;
+    r.inv = r.o;
+    float tMin = 0.0f, bestT = kRayTMax, bestU = 0.0f, bestV = 0.0f;
+    uint32_t bestTri = 0xFFFFFFFFu;
+    int cur = kDone;
     while (true) {
-        if (cur < 0) {
-            const uint32_t slot = static_cast<uint32_t>(~cur);
-            const float4 a = ld4(s.triPos, 3u * slot + 0u);
-            const float4 b = ld4(s.triPos, 3u * slot + 1u);
-            const float4 c = ld4(s.triPos, 3u * slot + 2u);
-            if (STATS) {
-                ts.tris += 1u;
+        const bool need = item < 0 && !exhausted;
+        const unsigned long long m = __ballot(need);
+        if (m != 0ull) {
+            const int leader = __ffsll(static_cast<long long>(m)) - 1;
+            int base = 0;
+            if (lane_id() == leader) {
+                base = atomicAdd(fetchCounter, __popcll(m));
             }
-            float t, u, v;
-            if (tri_test(r, a, b, c, tMin, bestT, t, u, v)) {
-                bestT = t;
-                bestU = u;
-                bestV = v;
-                bestTri = __float_as_uint(a.w);
-                if (ANY_HIT && t < tMax) {
-                    return true;
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const int idx = base + __popcll(m & lowerLanes);
+                if (idx < n) {
+                    item = idx;
+                    if (ANY_HIT) {
+                        const float4 a = P.shadowQ[2 * idx];
+                        const float4 b = P.shadowQ[2 * idx + 1];
+                        tag = __float_as_uint(a.w);
+                        r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+                        tMin = kShadowBias;
+                    } else {
+                        const int p = queue != nullptr ? queue[idx] : idx;
+                        tag = static_cast<uint32_t>(p);
+                        r = make_ray(mk(P.ox[p], P.oy[p], P.oz[p]), mk(P.dx[p], P.dy[p], P.dz[p]));
+                        tMin = P.tmin[p];
+                    }
+                    bestT = kRayTMax;
+                    bestTri = 0xFFFFFFFFu;
+                    bestU = 0.0f;
+                    bestV = 0.0f;
+                    st.sp = 0;
+                    st.lo = 0;
+                    if (STATS) {
+                        ts.rays += 1u;
+                    }
+                    float te;
+                    cur = (s.triangleCount != 0u && slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
+                                                          s.rootMax[1], s.rootMax[2], tMin, kRayTMax, te))
+                        ? s.rootRef
+                        : kDone;
+                } else {
+                    exhausted = true;
                 }
             }
-        } else {
-            const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
-            const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
-            const float4 n2 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 2u);
-            const float4 n3 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 3u);
-            if (STATS) {
-                ts.nodes += 1u;
-            }
-            float le, re;
-            const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, bestT, le);
-            const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, bestT, re);
-            const int left = __float_as_int(n3.x);
-            const int right = __float_as_int(n3.y);
-            if (lh && rh) {
-                const bool leftFirst = le < re;
-                if (st.sp < kSpillEntries) {
-                    st.push(leftFirst ? right : left, leftFirst ? re : le);
+        }
+        if (__ballot(item >= 0) == 0ull) {
+            break; // every lane is idle and the queue is exhausted
+        }
+        if (item < 0) {
+            continue;
+        }
+        bool finished = cur == kDone;
+        bool pop = false;
+        if (!finished) {
+            if (cur < 0) {
+                const uint32_t slot = static_cast<uint32_t>(~cur);
+                const float4 a = ld4(s.triPos, 3u * slot + 0u);
+                const float4 b = ld4(s.triPos, 3u * slot + 1u);
+                const float4 c = ld4(s.triPos, 3u * slot + 2u);
+                if (STATS) {
+                    ts.tris += 1u;
                 }
-                cur = leftFirst ? left : right;
-                continue;
-            }
-            if (lh) {
-                cur = left;
-                continue;
-            }
-            if (rh) {
-                cur = right;
-                continue;
+                float t, u, v;
+                if (tri_test(r, a, b, c, tMin, bestT, t, u, v)) {
+                    bestT = t;
+                    bestU = u;
+                    bestV = v;
+                    bestTri = __float_as_uint(a.w);
+                    if (ANY_HIT && t < kRayTMax) {
+                        finished = true;
+                    }
+                }
+                pop = !finished;
+            } else {
+                const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
+                const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
+                const float4 n2 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 2u);
+                const float4 n3 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 3u);
+                if (STATS) {
+                    ts.nodes += 1u;
+                }
+                float le, re;
+                const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, bestT, le);
+                const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, bestT, re);
+                const int left = __float_as_int(n3.x);
+                const int right = __float_as_int(n3.y);
+                if (lh && rh) {
+                    const bool leftFirst = le < re;
+                    if (st.sp < kSpillEntries) {
+                        st.push(leftFirst ? right : left, leftFirst ? re : le);
+                    }
+                    cur = leftFirst ? left : right;
+                } else if (lh) {
+                    cur = left;
+                } else if (rh) {
+                    cur = right;
+                } else {
+                    pop = true;
+                }
             }
         }
-        bool found = false;
-        while (st.sp > 0) {
-            const int2 e = st.pop();
-            if (!(__int_as_float(e.y) > bestT)) {
-                cur = e.x;
-                found = true;
-                break;
+        if (pop) {
+            finished = true;
+            while (st.sp > 0) {
+                const int2 e = st.pop();
+                if (!(__int_as_float(e.y) > bestT)) {
+                    cur = e.x;
+                    finished = false;
+                    break;
+                }
             }
         }
-        if (!found) {
-            break;
+        if (finished) {
+            const bool hit = bestT < kRayTMax;
+            if (ANY_HIT) {
+                P.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
+            } else {
+                P.hitT[tag] = hit ? bestT : 0.0f;
+                P.hitU[tag] = bestU;
+                P.hitV[tag] = bestV;
+                P.hitTri[tag] = hit ? bestTri : 0xFFFFFFFFu;
+            }
+            item = -1;
         }
-    }
-    return bestT < tMax;
-}
-
-// Closest hit of the extension rays in `queue` (identity when null). STATS adds the internal
-// nodes fetched, triangles tested and rays traced into stats[0..2].
-template <bool STATS>
-__global__ __launch_bounds__(kWfBlock) void wf_trace_ext_kernel(const MeshAccelSceneGpu* scene, Paths P,
-                                                                const int* __restrict__ queue,
-                                                                const int* __restrict__ countPtr,
-                                                                unsigned long long* stats)
-{
-    __shared__ int2 ring[kLdsEntries * kWfBlock];
-    const MeshAccelSceneGpu s = *scene;
-    const int n = *countPtr;
-    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    LdsStack st{ring + threadIdx.x, P.spill + gtid, P.spillThreads, 0, 0};
-    TraceStats ts{0u, 0u, 0u, 0u};
-    for (int i = gtid; i < n; i += static_cast<int>(gridDim.x * blockDim.x)) {
-        const int p = queue != nullptr ? queue[i] : i;
-        const Ray r = make_ray(mk(P.ox[p], P.oy[p], P.oz[p]), mk(P.dx[p], P.dy[p], P.dz[p]));
-        float t, u, v;
-        uint32_t tri;
-        const bool hit = traverse_lds<false, STATS>(s, r, P.tmin[p], kRayTMax, st, t, tri, u, v, ts);
-        P.hitT[p] = hit ? t : 0.0f;
-        P.hitU[p] = u;
-        P.hitV[p] = v;
-        P.hitTri[p] = hit ? tri : 0xFFFFFFFFu;
     }
     if (STATS) {
-        atomicAdd(&stats[0], static_cast<unsigned long long>(ts.nodes));
-        atomicAdd(&stats[1], static_cast<unsigned long long>(ts.tris));
-        atomicAdd(&stats[2], static_cast<unsigned long long>(ts.rays));
-    }
-}
-
-// Any hit of the shadow rays (LightSamplingCore.h:187-197, 360-369); STATS into stats[3..5].
-template <bool STATS>
-__global__ __launch_bounds__(kWfBlock) void wf_trace_shadow_kernel(const MeshAccelSceneGpu* scene, Paths P,
-                                                                   const int* __restrict__ countPtr,
-                                                                   unsigned long long* stats)
-{
-    __shared__ int2 ring[kLdsEntries * kWfBlock];
-    const MeshAccelSceneGpu s = *scene;
-    const int n = *countPtr;
-    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    LdsStack st{ring + threadIdx.x, P.spill + gtid, P.spillThreads, 0, 0};
-    TraceStats ts{0u, 0u, 0u, 0u};
-    for (int i = gtid; i < n; i += static_cast<int>(gridDim.x * blockDim.x)) {
-        const float4 a = P.shadowQ[2 * i];
-        const float4 b = P.shadowQ[2 * i + 1];
-        const Ray r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
-        float t, u, v;
-        uint32_t tri;
-        const bool occluded = traverse_lds<true, STATS>(s, r, kShadowBias, kRayTMax, st, t, tri, u, v, ts);
-        const uint32_t meta = __float_as_uint(a.w);
-        P.occ[2 * (meta >> 2) + ((meta & 3u) - 1u)] = occluded ? 1u : 0u;
-    }
-    if (STATS) {
-        atomicAdd(&stats[3], static_cast<unsigned long long>(ts.nodes));
-        atomicAdd(&stats[4], static_cast<unsigned long long>(ts.tris));
-        atomicAdd(&stats[5], static_cast<unsigned long long>(ts.rays));
+        const int o = ANY_HIT ? 3 : 0;
+        atomicAdd(&stats[o + 0], static_cast<unsigned long long>(ts.nodes));
+        atomicAdd(&stats[o + 1], static_cast<unsigned long long>(ts.tris));
+        atomicAdd(&stats[o + 2], static_cast<unsigned long long>(ts.rays));
     }
 }
 
@@ -728,7 +750,7 @@ size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spil
     P.shadeQ[0] = carve<int>(cur, c);
     P.shadeQ[1] = carve<int>(cur, c);
     P.shadowQ = carve<float4>(cur, 4 * c);
-    P.counters = carve<int>(cur, 4 * static_cast<size_t>(depth + 2));
+    P.counters = carve<int>(cur, kCountersPerBounce * static_cast<size_t>(depth + 2));
     P.pxBase = carve<int>(cur, static_cast<size_t>(pixels));
     P.pxK = carve<int>(cur, static_cast<size_t>(pixels));
     P.spill = carve<int2>(cur, static_cast<size_t>(spillThreads) * kSpillEntries);
@@ -817,7 +839,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
     }
     int active = activeCount;
     int passBase = 0;
-    const int counterInts = 4 * (maxPathDepth + 2);
+    const int counterInts = kCountersPerBounce * (maxPathDepth + 2);
     while (active > 0 && passBase < passCount) {
         // group size: 4 samples per pixel (one convergence interval), widened speculatively
         // while few pixels remain so the GPU stays full
@@ -834,27 +856,27 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
         const unsigned shadowBlocks =
             static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
         for (int b = 0; b <= maxPathDepth; ++b) {
-            int* cIn = P.counters + 4 * b;        // produced by bounce b-1 (or gen for b == 0)
-            int* cOut = P.counters + 4 * (b + 1); // produced by this bounce
+            int* cIn = P.counters + kCountersPerBounce * b;        // produced by bounce b-1 (or gen)
+            int* cOut = P.counters + kCountersPerBounce * (b + 1); // produced by this bounce
             const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
             const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
             prof_mark(stream, 0, false);
             if (gProf.counted) {
-                hipLaunchKernelGGL(wf_trace_ext_kernel<true>, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, P,
-                                   extQueue, extCount, gProf.dStats);
+                hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+                                   d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats);
             } else {
-                hipLaunchKernelGGL(wf_trace_ext_kernel<false>, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene,
-                                   P, extQueue, extCount, nullptr);
+                hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+                                   d_scene, P, extQueue, extCount, &cIn[4], nullptr);
             }
             prof_mark(stream, 0, true);
             if (b > 0) {
                 prof_mark(stream, 1, false);
                 if (gProf.counted) {
-                    hipLaunchKernelGGL(wf_trace_shadow_kernel<true>, dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                       d_scene, P, &cIn[2], gProf.dStats);
+                    hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                       d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats);
                 } else {
-                    hipLaunchKernelGGL(wf_trace_shadow_kernel<false>, dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                       d_scene, P, &cIn[2], nullptr);
+                    hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                       d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr);
                 }
                 prof_mark(stream, 1, true);
             }
