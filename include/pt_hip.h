@@ -280,6 +280,22 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
 /* Milliseconds of the last GPU BVH build (device time between events). */
 float ptSceneLastBuildMs(const PtScene* scene);
 
+/* ---- environment map (replaces EnvironmentMap::buildImportanceTables + upload,
+ *      PathTracerLib/src/EnvironmentMap.cpp:76-127) --------------------------------------- */
+typedef struct PtEnvironment PtEnvironment;
+
+/* Builds the importance tables of a lat-long RGB float image (3 floats per texel, rows top to
+ * bottom) and uploads image and tables. Synchronous. NULL on failure. */
+PtEnvironment* ptEnvironmentCreate(const float* rgb, int width, int height, hipStream_t stream);
+
+/* Device pointer for the launchers' d_env argument (valid = 1). */
+const EnvironmentMapGpu* ptEnvironmentDevice(const PtEnvironment* env);
+
+/* Copies the tables: marginal (height + 1 floats) and rows (height * (width + 1) floats). */
+bool ptEnvironmentReadTables(const PtEnvironment* env, float* marginal, float* rows);
+
+void ptEnvironmentDestroy(PtEnvironment* env);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,10 @@ SIGNATURES = {
     "ptSceneDescription": (_b, [_p, _p]),
     "ptSceneReadBvh": (_b, [_p, _p, _u, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ptSceneLastBuildMs": (_f, [_p]),
+    "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
+    "ptEnvironmentDevice": (_p, [_p]),
+    "ptEnvironmentReadTables": (_b, [_p, _p, _p]),
+    "ptEnvironmentDestroy": (None, [_p]),
 }
 
 
@@ -248,6 +252,40 @@ class Scene:
     def close(self):
         if self.handle:
             lib().ptSceneDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Environment:
+    """A lat-long HDR environment with importance tables on the GPU (ptEnvironmentCreate)."""
+
+    def __init__(self, rgb: np.ndarray, stream: int = 0):
+        img = np.ascontiguousarray(rgb, dtype=np.float32)
+        if img.ndim != 3 or img.shape[2] != 3:
+            raise ValueError("environment image must be (height, width, 3) float32")
+        self.height, self.width = img.shape[0], img.shape[1]
+        self.handle = lib().ptEnvironmentCreate(img.ctypes.data, self.width, self.height, stream)
+        if not self.handle:
+            raise RuntimeError(f"ptEnvironmentCreate failed: {last_error()}")
+
+    @property
+    def device(self) -> int:
+        return lib().ptEnvironmentDevice(self.handle)
+
+    def tables(self) -> tuple[np.ndarray, np.ndarray]:
+        marg = np.zeros(self.height + 1, dtype=np.float32)
+        rows = np.zeros(self.height * (self.width + 1), dtype=np.float32)
+        check(lib().ptEnvironmentReadTables(self.handle, marg.ctypes.data, rows.ctypes.data), "ptEnvironmentReadTables")
+        return marg, rows
+
+    def close(self):
+        if self.handle:
+            lib().ptEnvironmentDestroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -13,6 +13,7 @@
 #include "../../include/pt_hip.h"
 
 #define PT_DEV __device__ __forceinline__
+#define PT_HD __host__ __device__ __forceinline__
 
 namespace pt {
 
@@ -64,7 +65,7 @@ constexpr double kLn2 = 6.93147180559945286227e-01;
 constexpr double kPi = 3.14159265358979311600e+00;
 constexpr double kPio2 = 1.57079632679489655800e+00;
 
-PT_DEV double sin_series(double r)
+PT_HD double sin_series(double r)
 {
     const double r2 = r * r;
     double p = -1.0 / 1307674368000.0;
@@ -77,7 +78,7 @@ PT_DEV double sin_series(double r)
     return r + r * (r2 * p);
 }
 
-PT_DEV double cos_series(double r)
+PT_HD double cos_series(double r)
 {
     const double r2 = r * r;
     double p = 1.0 / 20922789888000.0;
@@ -91,7 +92,7 @@ PT_DEV double cos_series(double r)
     return 1.0 + r2 * p;
 }
 
-PT_DEV double reduce(double x, int& q)
+PT_HD double reduce(double x, int& q)
 {
     const double k = rint(x * kTwoOverPi);
     const double r = (x - k * kPio2Hi) - k * kPio2Lo;
@@ -99,7 +100,7 @@ PT_DEV double reduce(double x, int& q)
     return r;
 }
 
-PT_DEV double sin_d(double x)
+PT_HD double sin_d(double x)
 {
     int q = 0;
     const double r = reduce(x, q);
@@ -107,7 +108,7 @@ PT_DEV double sin_d(double x)
     return (q & 2) ? -s : s;
 }
 
-PT_DEV double cos_d(double x)
+PT_HD double cos_d(double x)
 {
     int q = 0;
     const double r = reduce(x, q);
@@ -116,7 +117,7 @@ PT_DEV double cos_d(double x)
 }
 
 // sin and cos of one argument with a shared reduction.
-PT_DEV void sincos_d(double x, double& s, double& c)
+PT_HD void sincos_d(double x, double& s, double& c)
 {
     int q = 0;
     const double r = reduce(x, q);
@@ -130,7 +131,7 @@ PT_DEV void sincos_d(double x, double& s, double& c)
     }
 }
 
-PT_DEV double atan_unit(double z)
+PT_HD double atan_unit(double z)
 {
     double w = z / (1.0 + sqrt(1.0 + z * z));
     w = w / (1.0 + sqrt(1.0 + w * w));
@@ -152,7 +153,7 @@ PT_DEV double atan_unit(double z)
     return 4.0 * (w * p);
 }
 
-PT_DEV double atan2_d(double y, double x)
+PT_HD double atan2_d(double y, double x)
 {
     const double ay = fabs(y);
     const double ax = fabs(x);
@@ -171,17 +172,17 @@ PT_DEV double atan2_d(double y, double x)
 }
 } // namespace dm
 
-PT_DEV float sinf_pt(float x)
+PT_HD float sinf_pt(float x)
 {
     if (x == 0.0f) {
         return x; // keeps the sign of zero
     }
-    return isfinite(x) ? static_cast<float>(dm::sin_d(static_cast<double>(x))) : x - x;
+    return __builtin_isfinite(x) ? static_cast<float>(dm::sin_d(static_cast<double>(x))) : x - x;
 }
-PT_DEV float cosf_pt(float x) { return isfinite(x) ? static_cast<float>(dm::cos_d(static_cast<double>(x))) : x - x; }
-PT_DEV void sincosf_pt(float x, float& s, float& c)
+PT_HD float cosf_pt(float x) { return __builtin_isfinite(x) ? static_cast<float>(dm::cos_d(static_cast<double>(x))) : x - x; }
+PT_HD void sincosf_pt(float x, float& s, float& c)
 {
-    if (!isfinite(x)) {
+    if (!__builtin_isfinite(x)) {
         s = x - x;
         c = x - x;
         return;
@@ -191,9 +192,9 @@ PT_DEV void sincosf_pt(float x, float& s, float& c)
     s = x == 0.0f ? x : static_cast<float>(sd);
     c = static_cast<float>(cd);
 }
-PT_DEV float tanf_pt(float x)
+PT_HD float tanf_pt(float x)
 {
-    if (!isfinite(x)) {
+    if (!__builtin_isfinite(x)) {
         return x - x;
     }
     if (x == 0.0f) {
@@ -203,7 +204,7 @@ PT_DEV float tanf_pt(float x)
     return static_cast<float>(dm::sin_d(xd) / dm::cos_d(xd));
 }
 
-PT_DEV float expf_pt(float x)
+PT_HD float expf_pt(float x)
 {
     if (x != x) {
         return x;
@@ -236,7 +237,7 @@ PT_DEV float expf_pt(float x)
     return static_cast<float>(ldexp(p, static_cast<int>(k)));
 }
 
-PT_DEV float logf_pt(float x)
+PT_HD float logf_pt(float x)
 {
     if (x != x) {
         return x;
@@ -247,7 +248,7 @@ PT_DEV float logf_pt(float x)
     if (x == 0.0f) {
         return -__builtin_inff();
     }
-    if (isinf(x)) {
+    if (__builtin_isinf(x)) {
         return x;
     }
     int e = 0;
@@ -272,7 +273,7 @@ PT_DEV float logf_pt(float x)
     return static_cast<float>(static_cast<double>(e) * dm::kLn2 + 2.0 * s * p);
 }
 
-PT_DEV float atan2f_pt(float y, float x)
+PT_HD float atan2f_pt(float y, float x)
 {
     if (x != x || y != y) {
         return x + y;
@@ -280,7 +281,7 @@ PT_DEV float atan2f_pt(float y, float x)
     return static_cast<float>(dm::atan2_d(static_cast<double>(y), static_cast<double>(x)));
 }
 
-PT_DEV float acosf_pt(float x)
+PT_HD float acosf_pt(float x)
 {
     if (x != x || x > 1.0f || x < -1.0f) {
         return __builtin_nanf("");

@@ -16,6 +16,7 @@
 // Stable partitioning keeps every list sorted within every segment, so the left set of a
 // split is exactly the count/2 smallest (centroid, index) pairs -- the nth_element partition
 // with ties broken by triangle index.
+#include "pt_device.h"
 #include "pt_internal.h"
 
 #include "../../include/pt_hip.h"
@@ -43,8 +44,8 @@ __device__ __forceinline__ float from_sortable(uint32_t u)
     return __uint_as_float(b);
 }
 
-__device__ __forceinline__ float min2(float a, float b) { return a < b ? a : b; }
-__device__ __forceinline__ float max2(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float bmin2(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float bmax2(float a, float b) { return a > b ? a : b; }
 
 struct Seg {
     int begin, count, node, j, parentInternal;
@@ -128,9 +129,10 @@ __global__ void prep_kernel(const float* __restrict__ tris, int n, uint32_t* __r
     values[n + i] = i;
     values[2 * n + i] = i;
     // MeshBvhBuilder.cpp:15-29
-    bounds[2 * i] = make_float4(min2(min2(v0x, v1x), v2x), min2(min2(v0y, v1y), v2y), min2(min2(v0z, v1z), v2z), 0.0f);
+    bounds[2 * i] =
+        make_float4(bmin2(bmin2(v0x, v1x), v2x), bmin2(bmin2(v0y, v1y), v2y), bmin2(bmin2(v0z, v1z), v2z), 0.0f);
     bounds[2 * i + 1] =
-        make_float4(max2(max2(v0x, v1x), v2x), max2(max2(v0y, v1y), v2y), max2(max2(v0z, v1z), v2z), 0.0f);
+        make_float4(bmax2(bmax2(v0x, v1x), v2x), bmax2(bmax2(v0y, v1y), v2y), bmax2(bmax2(v0z, v1z), v2z), 0.0f);
     // shading record: (n0.xyz, n1.x) (n1.yz, n2.xy) (n2.z, uv0.xy, uv1.x) (uv1.y, uv2.xy, material)
     shade[4 * i + 0] = make_float4(t[9], t[10], t[11], t[12]);
     shade[4 * i + 1] = make_float4(t[13], t[14], t[15], t[16]);
@@ -724,6 +726,110 @@ float ptSceneLastBuildMs(const PtScene* scene) { return scene != nullptr ? scene
 hipError_t spectralUploadStateToSymbol(const SpectralStateGpu* hostState)
 {
     return hostState == nullptr ? hipErrorInvalidValue : hipSuccess;
+}
+
+} // extern "C"
+
+// ---- environment map (EnvironmentMap.cpp:76-127 importance tables + upload) -----------------
+
+struct PtEnvironment {
+    EnvironmentMapGpu desc{};
+    EnvironmentMapGpu* d_desc = nullptr;
+    float* pixels = nullptr;
+    float* marginal = nullptr;
+    float* rows = nullptr;
+    std::vector<float> hostMarginal;
+    std::vector<float> hostRows;
+};
+
+extern "C" {
+
+PtEnvironment* ptEnvironmentCreate(const float* rgb, int width, int height, hipStream_t stream)
+{
+    if (rgb == nullptr || width <= 0 || height <= 0 || static_cast<long long>(width) * height > (1ll << 28)) {
+        ptSetError("ptEnvironmentCreate: invalid arguments");
+        return nullptr;
+    }
+    PtEnvironment* e = new PtEnvironment();
+    const size_t w1 = static_cast<size_t>(width) + 1;
+    e->hostMarginal.assign(static_cast<size_t>(height) + 1, 0.0f);
+    e->hostRows.assign(static_cast<size_t>(height) * w1, 0.0f);
+    // rows: prefix sums of max(r,g,b) (floored at 1e-6) * sin(theta), normalised per row;
+    // marginal: prefix sums of the row sums, normalised (sequential float sums as the reference)
+    float total = 0.0f;
+    for (int y = 0; y < height; ++y) {
+        const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(height);
+        const float st = pt::sinf_pt(v * 3.14159265f);
+        float rowSum = 0.0f;
+        float* row = e->hostRows.data() + static_cast<size_t>(y) * w1;
+        row[0] = 0.0f;
+        for (int x = 0; x < width; ++x) {
+            const float* px = rgb + 3 * (static_cast<size_t>(y) * width + x);
+            const float bc = px[1] < px[2] ? px[2] : px[1]; // std::max(g, b)
+            const float lum = px[0] < bc ? bc : px[0];       // std::max(r, .)
+            const float w = (lum < 1.0e-6f ? 1.0e-6f : lum) * st;
+            rowSum += w;
+            row[x + 1] = rowSum;
+        }
+        if (rowSum > 0.0f) {
+            for (int x = 0; x < width; ++x) {
+                row[x + 1] /= rowSum;
+            }
+        }
+        total += rowSum;
+        e->hostMarginal[static_cast<size_t>(y) + 1] = total;
+    }
+    if (total > 0.0f) {
+        for (int y = 0; y < height; ++y) {
+            e->hostMarginal[static_cast<size_t>(y) + 1] /= total;
+        }
+    }
+    const size_t npx = 3 * static_cast<size_t>(width) * height;
+    bool ok = upload(&e->pixels, rgb, npx, stream) && upload(&e->marginal, e->hostMarginal.data(), e->hostMarginal.size(), stream) &&
+              upload(&e->rows, e->hostRows.data(), e->hostRows.size(), stream);
+    if (ok) {
+        e->desc.pixels = e->pixels;
+        e->desc.marginalCdf = e->marginal;
+        e->desc.rowCdf = e->rows;
+        e->desc.width = width;
+        e->desc.height = height;
+        e->desc.valid = 1;
+        ok = upload(&e->d_desc, &e->desc, 1, stream) && hipStreamSynchronize(stream) == hipSuccess;
+    }
+    if (!ok) {
+        release(e->pixels);
+        release(e->marginal);
+        release(e->rows);
+        release(e->d_desc);
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+const EnvironmentMapGpu* ptEnvironmentDevice(const PtEnvironment* env) { return env != nullptr ? env->d_desc : nullptr; }
+
+bool ptEnvironmentReadTables(const PtEnvironment* env, float* marginal, float* rows)
+{
+    if (env == nullptr || marginal == nullptr || rows == nullptr) {
+        ptSetError("ptEnvironmentReadTables: invalid arguments");
+        return false;
+    }
+    std::memcpy(marginal, env->hostMarginal.data(), env->hostMarginal.size() * sizeof(float));
+    std::memcpy(rows, env->hostRows.data(), env->hostRows.size() * sizeof(float));
+    return true;
+}
+
+void ptEnvironmentDestroy(PtEnvironment* env)
+{
+    if (env != nullptr) {
+        (void)hipDeviceSynchronize();
+        release(env->pixels);
+        release(env->marginal);
+        release(env->rows);
+        release(env->d_desc);
+        delete env;
+    }
 }
 
 } // extern "C"

@@ -29,6 +29,7 @@ constexpr int kWfBlock = 256;
 constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
 constexpr int kSpillEntries = 32;  // global spill entries per thread (tree depth <= 28)
 constexpr int kCountersPerBounce = 8;
+constexpr int kRefillLanes = 32;   // idle lanes that trigger a wave's ray refill
 
 enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
@@ -139,9 +140,11 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     uint32_t bestTri = 0xFFFFFFFFu;
     int cur = kDone;
     while (true) {
+        // Refill once half the wave is idle (or all of it): one atomic then serves >= 32 lanes.
         const bool need = item < 0 && !exhausted;
         const unsigned long long m = __ballot(need);
-        if (m != 0ull) {
+        const unsigned long long busy = __ballot(item >= 0);
+        if (m != 0ull && (__popcll(m) >= kRefillLanes || busy == 0ull)) {
             const int leader = __ffsll(static_cast<long long>(m)) - 1;
             int base = 0;
             if (lane_id() == leader) {
@@ -184,7 +187,10 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
             }
         }
         if (__ballot(item >= 0) == 0ull) {
-            break; // every lane is idle and the queue is exhausted
+            if (__ballot(!exhausted) == 0ull) {
+                break; // every lane is idle and the queue is exhausted
+            }
+            continue;
         }
         if (item < 0) {
             continue;

@@ -67,10 +67,12 @@ class Accumulator:
 class DeviceInputs:
     """Camera, params and (optional) environment map in device memory."""
 
-    def __init__(self, camera: np.ndarray, params: np.ndarray, device: str = "cuda"):
+    def __init__(self, camera: np.ndarray, params: np.ndarray, env: abi.Environment | None = None,
+                 device: str = "cuda"):
         self.camera = torch.from_numpy(np.ascontiguousarray(camera, dtype=np.float32)).to(device)
         self.params = torch.from_numpy(np.ascontiguousarray(params, dtype=np.float32)).to(device)
-        self.env_ptr = None
+        self.env = env  # keeps the device tables alive
+        self.env_ptr = env.device if env is not None else None
         self.params_host = np.ascontiguousarray(params, dtype=np.float32)
 
     @property
@@ -86,15 +88,74 @@ def adaptive_pass(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, acti
         inputs.env_ptr, inputs.params.data_ptr(), seed, _stream_handle(stream)), "pathTracerAdaptiveSample")
 
 
-def compact(acc: Accumulator, inputs: DeviceInputs, active_count: int, stream=None) -> int:
-    """Compacts the active list into scratch, swaps the lists and returns the new count."""
+def compact(acc: Accumulator, inputs: DeviceInputs, active_count: int, region: tuple | None = None,
+            stream=None) -> int:
+    """Compacts the active list into scratch (optionally keeping only pixels inside
+    region = (minX, minY, maxX, maxY)), swaps the lists and returns the new count."""
+    r = region or (0, 0, 0, 0)
     abi.check(abi.lib().pathTracerCompactActiveList(
         acc.active.data_ptr(), acc.scratch.data_ptr(), acc.active_count.data_ptr(), acc.conv.data_ptr(),
-        acc.counts.data_ptr(), inputs.max_spp, active_count, acc.width, acc.height, False, 0, 0, 0, 0,
-        _stream_handle(stream)), "pathTracerCompactActiveList")
+        acc.counts.data_ptr(), inputs.max_spp, active_count, acc.width, acc.height, region is not None,
+        r[0], r[1], r[2], r[3], _stream_handle(stream)), "pathTracerCompactActiveList")
     count = int(acc.active_count.item())
     acc.active, acc.scratch = acc.scratch, acc.active
     return count
+
+
+def clear_region(acc: Accumulator, region: tuple, stream=None) -> int:
+    """Resets region = (minX, minY, maxX, maxY) and makes it the active list; returns its size."""
+    abi.check(abi.lib().pathTracerClearRegionAccumulator(
+        acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
+        acc.active.data_ptr(), acc.active_count.data_ptr(), acc.width, acc.height, region[0], region[1], region[2],
+        region[3], _stream_handle(stream)), "pathTracerClearRegionAccumulator")
+    return int(acc.active_count.item())
+
+
+def rebuild_active(acc: Accumulator, inputs: DeviceInputs, stream=None) -> int:
+    """Lists every pixel still sampling (pathTracerRebuildActiveList); returns the count."""
+    abi.check(abi.lib().pathTracerRebuildActiveList(
+        acc.conv.data_ptr(), acc.counts.data_ptr(), acc.active.data_ptr(), acc.active_count.data_ptr(), acc.width,
+        acc.height, inputs.max_spp, _stream_handle(stream)), "pathTracerRebuildActiveList")
+    return int(acc.active_count.item())
+
+
+def preview(scene: abi.Scene, inputs: DeviceInputs, width: int, height: int, seed: int,
+            stream=None) -> torch.Tensor:
+    """One unjittered sample per pixel into a fresh (width*height, 4) buffer (pathTracerPreviewSample)."""
+    buf = torch.zeros((width * height, 4), dtype=torch.float32, device="cuda")
+    abi.check(abi.lib().pathTracerPreviewSample(buf.data_ptr(), width, height, inputs.camera.data_ptr(), scene.device,
+                                                inputs.env_ptr, inputs.params.data_ptr(), seed,
+                                                _stream_handle(stream)), "pathTracerPreviewSample")
+    return buf
+
+
+def display(acc: Accumulator, inputs: DeviceInputs, exposure: float = 1.0, stream=None) -> torch.Tensor:
+    """Tone-mapped RGBA8 of the accumulator (pathTracerCopyToPbo): (height*width, 4) uint8."""
+    pbo = torch.zeros((acc.width * acc.height, 4), dtype=torch.uint8, device="cuda")
+    abi.check(abi.lib().pathTracerCopyToPbo(acc.acc.data_ptr(), acc.counts.data_ptr(), acc.conv.data_ptr(),
+                                            pbo.data_ptr(), acc.width, acc.height, inputs.params.data_ptr(),
+                                            exposure, _stream_handle(stream)), "pathTracerCopyToPbo")
+    return pbo
+
+
+def upsample_preview(preview_buf: torch.Tensor, preview_w: int, preview_h: int, downscale: int, full_w: int,
+                     full_h: int, inputs: DeviceInputs, exposure: float = 1.0, stream=None) -> torch.Tensor:
+    """Nearest-neighbour upsampled, tone-mapped preview (pathTracerUpsamplePreviewToPbo)."""
+    pbo = torch.zeros((full_w * full_h, 4), dtype=torch.uint8, device="cuda")
+    abi.check(abi.lib().pathTracerUpsamplePreviewToPbo(preview_buf.data_ptr(), preview_w, preview_h, downscale,
+                                                       pbo.data_ptr(), full_w, full_h, inputs.params.data_ptr(),
+                                                       exposure, _stream_handle(stream)),
+              "pathTracerUpsamplePreviewToPbo")
+    return pbo
+
+
+def uv_debug(scene: abi.Scene, inputs: DeviceInputs, width: int, height: int, stream=None) -> torch.Tensor:
+    """Fractional hit UV as red/green (pathTracerWriteUvDebugToPbo): (height*width, 4) uint8."""
+    pbo = torch.zeros((width * height, 4), dtype=torch.uint8, device="cuda")
+    abi.check(abi.lib().pathTracerWriteUvDebugToPbo(inputs.camera.data_ptr(), scene.device, pbo.data_ptr(), width,
+                                                    height, inputs.params.data_ptr(), _stream_handle(stream)),
+              "pathTracerWriteUvDebugToPbo")
+    return pbo
 
 
 def render_loop(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, first_iter: int = 0,

@@ -237,6 +237,14 @@ int orc_render_adaptive(float* acc4, uint32_t* counts, float* lumMean, float* m2
 void orc_copy_to_pbo(const float* acc4, const uint32_t* counts, const uint8_t* converged, uint8_t* rgba,
                      int outW, int outH, int srcW, int srcH, int downscale, int useAcc, const OParams* params,
                      float exposure);
+/* UV debug view: pinhole primary ray, fractional hit UV as red/green. */
+/* Region reset (acc w = 1) listing the region row-major as the active set; returns its size. */
+int orc_clear_region(float* acc4, uint32_t* counts, float* mean, float* m2, uint8_t* converged, int* active,
+                     int width, int minX, int minY, int maxX, int maxY);
+/* Active pixels (not converged, below maxSamplesPerPixel when > 0) in ascending order. */
+int orc_rebuild_active(const uint8_t* converged, const uint32_t* counts, int* out, int pixelCount, int maxSpp);
+void orc_uv_debug(const OCamera* cam, const OScene* scene, uint8_t* rgba, int width, int height,
+                  const OParams* params);
 
 #ifdef __cplusplus
 }

@@ -146,6 +146,9 @@ def lib():
             "orc_render_adaptive": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp, C.POINTER(Scene),
                                               C.POINTER(Env), vp, C.c_int, C.c_int, C.c_int]),
             "orc_copy_to_pbo": (None, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, f]),
+            "orc_uv_debug": (None, [vp, C.POINTER(Scene), vp, C.c_int, C.c_int, vp]),
+            "orc_clear_region": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "orc_rebuild_active": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -251,6 +254,21 @@ class OracleScene:
 
     def trace(self, ro, rd, tmin=0.0, tmax=1.0e6) -> Hit:
         return lib().orc_trace_ray(v3(ro), v3(rd), C.byref(self.scene), tmin, tmax)
+
+
+class OracleEnv:
+    """A lat-long RGB environment with its importance tables (EnvironmentMap.cpp:76-127), as the
+    OEnv view the integrator reads. Keeps the arrays alive."""
+
+    def __init__(self, rgb: np.ndarray):
+        self.pixels = np.ascontiguousarray(rgb, dtype=np.float32)
+        self.height, self.width = self.pixels.shape[0], self.pixels.shape[1]
+        self.marginal = np.zeros(self.height + 1, np.float32)
+        self.rows = np.zeros(self.height * (self.width + 1), np.float32)
+        if not lib().orc_env_build_tables(ptr(self.pixels), self.width, self.height, ptr(self.marginal),
+                                          ptr(self.rows)):
+            raise ValueError("orc_env_build_tables rejected the image")
+        self.env = Env(ptr(self.pixels), ptr(self.marginal), ptr(self.rows), self.width, self.height, 1)
 
 
 class Accumulator:

@@ -612,3 +612,79 @@ void orc_copy_to_pbo(const float* acc, const uint32_t* counts, const uint8_t* co
         }
     }
 }
+
+/* PathTracer.cu:514-548: pinhole primary ray (CameraDevice.cuh:82-86), closest hit in
+ * [0.001, 1e30), fractional UV as red/green, background (black without params) on a miss,
+ * same clamp and x255 truncation as the display path. */
+void orc_uv_debug(const OCamera* cam, const OScene* scene, uint8_t* rgba, int width, int height,
+                  const OParams* params)
+{
+    OCamera pin = *cam;
+    pin.focusValid = 0;
+    for (int y = 0; y < height; ++y) {
+        for (int x = 0; x < width; ++x) {
+            const float u = ((float)x + 0.5f) / (float)width;
+            const float v = ((float)y + 0.5f) / (float)height;
+            OVec3 ro, rd;
+            orc_camera_ray(&pin, u, v, 0.0f, 0.0f, &ro, &rd);
+            float r = 0.0f, g = 0.0f, b = 0.0f;
+            if (params != NULL) {
+                r = params->backgroundR;
+                g = params->backgroundG;
+                b = params->backgroundB;
+            }
+            const OHit h = orc_trace_ray(ro, rd, scene, 0.001f, 1.0e30f);
+            if (h.hit) {
+                r = h.uv.x - floorf(h.uv.x);
+                g = h.uv.y - floorf(h.uv.y);
+                b = 0.0f;
+            }
+            uint8_t* o = &rgba[4 * ((size_t)y * width + x)];
+            o[0] = (uint8_t)(fminf(fmaxf(r, 0.0f), 1.0f) * 255.0f);
+            o[1] = (uint8_t)(fminf(fmaxf(g, 0.0f), 1.0f) * 255.0f);
+            o[2] = (uint8_t)(fminf(fmaxf(b, 0.0f), 1.0f) * 255.0f);
+            o[3] = 255;
+        }
+    }
+}
+
+/* PathTracer.cu:305-344: reset the pixels of [minX,maxX]x[minY,maxY] (acc w = 1) and list them
+ * row-major as the active set. Returns the region's pixel count (0 for an empty region). */
+int orc_clear_region(float* acc, uint32_t* counts, float* mean, float* m2, uint8_t* conv, int* active, int width,
+                     int minX, int minY, int maxX, int maxY)
+{
+    const int rw = maxX - minX + 1;
+    const int rh = maxY - minY + 1;
+    if (rw <= 0 || rh <= 0) {
+        return 0;
+    }
+    for (int t = 0; t < rw * rh; ++t) {
+        const int idx = (minY + t / rw) * width + (minX + t % rw);
+        acc[4 * (size_t)idx + 0] = 0.0f;
+        acc[4 * (size_t)idx + 1] = 0.0f;
+        acc[4 * (size_t)idx + 2] = 0.0f;
+        acc[4 * (size_t)idx + 3] = 1.0f;
+        counts[idx] = 0;
+        mean[idx] = 0.0f;
+        m2[idx] = 0.0f;
+        conv[idx] = 0;
+        active[t] = idx;
+    }
+    return rw * rh;
+}
+
+/* PathTracer.cu:346-370: every pixel neither converged nor at maxSamplesPerPixel, ascending. */
+int orc_rebuild_active(const uint8_t* conv, const uint32_t* counts, int* out, int pixelCount, int maxSpp)
+{
+    int n = 0;
+    for (int p = 0; p < pixelCount; ++p) {
+        if (conv[p] != 0) {
+            continue;
+        }
+        if (maxSpp > 0 && counts[p] >= (uint32_t)maxSpp) {
+            continue;
+        }
+        out[n++] = p;
+    }
+    return n;
+}
