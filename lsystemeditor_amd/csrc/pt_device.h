@@ -382,7 +382,15 @@ constexpr float kInvPi = 1.0f / kPi;
 constexpr float kMinRadius = 1.0e-5f;
 constexpr int kStackSize = 40;
 
-PT_DEV float4 ld4(const PtFloat4* p, uint32_t i) { return reinterpret_cast<const float4*>(p)[i]; }
+// Loads through the global address space: global_load instructions (vector memory counter
+// only) instead of flat ones, which also count against the LDS counter.
+typedef float NativeFloat4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const NativeFloat4 GlobalFloat4;
+PT_DEV float4 ld4(const PtFloat4* p, uint32_t i)
+{
+    const NativeFloat4 v = ((GlobalFloat4*)(p))[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 struct Ray {
     V3 o, d, inv;

@@ -78,9 +78,15 @@ PT_DEV int wave_append(int* counter, bool want)
 // ----------------------------------------------------------------------------------------
 // Traversal with an LDS stack. Same visit order and culling as pt::traverse.
 // ----------------------------------------------------------------------------------------
+// Typed address spaces keep the ring in ds_read/ds_write and the spill area in global
+// loads/stores (a pointer that may point to either would become a flat access).
+typedef int NativeInt2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) NativeInt2 LdsInt2;
+typedef __attribute__((address_space(1))) NativeInt2 GlobalInt2;
+
 struct LdsStack {
-    int2* ring;  // this thread's column of the block's LDS ring
-    int2* spill; // this thread's column of the global spill area
+    LdsInt2* ring;     // this thread's column of the block's LDS ring
+    GlobalInt2* spill; // this thread's column of the global spill area
     int spillStride;
     int sp, lo;
 
@@ -90,18 +96,27 @@ struct LdsStack {
             spill[static_cast<size_t>(lo) * spillStride] = ring[(lo & (kLdsEntries - 1)) * kWfBlock];
             ++lo;
         }
-        ring[(sp & (kLdsEntries - 1)) * kWfBlock] = make_int2(ref, __float_as_int(t));
+        NativeInt2 e;
+        e.x = ref;
+        e.y = __float_as_int(t);
+        ring[(sp & (kLdsEntries - 1)) * kWfBlock] = e;
         ++sp;
     }
 
+    // Requires sp > 0. When the ring is empty the entries below it are reloaded from the spill
+    // area first, so the pop itself is always an LDS read.
     PT_DEV int2 pop()
     {
-        --sp;
-        if (sp < lo) {
-            lo = sp;
-            return spill[static_cast<size_t>(sp) * spillStride];
+        if (sp == lo) {
+            const int n = lo < kLdsEntries ? lo : kLdsEntries;
+            for (int k = lo - n; k < lo; ++k) {
+                ring[(k & (kLdsEntries - 1)) * kWfBlock] = spill[static_cast<size_t>(k) * spillStride];
+            }
+            lo -= n;
         }
-        return ring[(sp & (kLdsEntries - 1)) * kWfBlock];
+        --sp;
+        const NativeInt2 e = ring[(sp & (kLdsEntries - 1)) * kWfBlock];
+        return make_int2(e.x, e.y);
     }
 };
 
@@ -126,7 +141,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     const MeshAccelSceneGpu s = *scene;
     const int n = *countPtr;
     const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    LdsStack st{ring + threadIdx.x, P.spill + gtid, P.spillThreads, 0, 0};
+    LdsStack st{(LdsInt2*)(ring + threadIdx.x), (GlobalInt2*)(P.spill + gtid), P.spillThreads, 0, 0};
     TraceStats ts{0u, 0u, 0u, 0u};
     const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
     int item = -1;        // queue index of the lane's current ray
@@ -203,6 +218,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 const float4 a = ld4(s.triPos, 3u * slot + 0u);
                 const float4 b = ld4(s.triPos, 3u * slot + 1u);
                 const float4 c = ld4(s.triPos, 3u * slot + 2u);
+                // all 48 bytes in one round trip: keeps the compiler from deferring v0's load
+                // past the determinant test
+                asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w));
                 if (STATS) {
                     ts.tris += 1u;
                 }
