@@ -95,8 +95,8 @@ def cpu_baseline(spec, threads: int):
 def profile_read():
     import ctypes as C
     from lsystemeditor_amd import abi
-    out = (C.c_double * 14)()
-    abi.check(abi.lib().pathTracerWavefrontProfileRead(C.addressof(out), 14), "pathTracerWavefrontProfileRead")
+    out = (C.c_double * 15)()
+    abi.check(abi.lib().pathTracerWavefrontProfileRead(C.addressof(out), 15), "pathTracerWavefrontProfileRead")
     return list(out)
 
 
@@ -152,13 +152,16 @@ def main():
     abi.check(L.pathTracerWavefrontProfile(0, 0), "pathTracerWavefrontProfile")
     samples_per_step = int(acc.counts.to(torch.int64).sum().item())
     samples = samples_per_step * args.steps
-    ms_ext, ms_sh, ms_shade, _ = prof[0:4]
+    ms_ext, ms_sh, ms_shade, ms_other = prof[0:4]
     n_ext, n_sh, n_shade, _ = (int(v) for v in prof[4:8])
+    started = int(prof[14])
     trace_ms_per_step = (ms_ext + ms_sh) / args.steps
     achieved = trace_bytes / (trace_ms_per_step / 1e3) / 1e9
     print(f"[bench] wavefront: {args.steps} steps, {samples_per_step} samples/step, wall {wall:.3f} s; "
           f"trace ext {ms_ext / args.steps:.1f} ms/step ({n_ext // args.steps} launches), shadow "
-          f"{ms_sh / args.steps:.1f} ms/step, shade {ms_shade / args.steps:.1f} ms/step", file=sys.stderr, flush=True)
+          f"{ms_sh / args.steps:.1f} ms/step, shade {ms_shade / args.steps:.1f} ms/step, gen+accumulate "
+          f"{ms_other / args.steps:.1f} ms/step; {started / args.steps:.0f} samples started/step", file=sys.stderr,
+          flush=True)
 
     extra = {}
     if not args.no_loop:
@@ -211,7 +214,9 @@ def main():
                                + ("adaptive 16-256 spp at relative error 0.01, max depth 12, 2 area lights + "
                                   "uniform environment" if adaptive else f"{spec.spp} spp, max depth 8"),
                    "samples_per_step": samples_per_step, "bvh_build_ms": round(scene.build_ms, 3),
-                   "scheduler": "wavefront (groups of 4 samples per pixel between convergence tests)", **extra},
+                   "scheduler": "wavefront (first group: the samples no pixel can skip; then 4 samples per "
+                                "active pixel between convergence tests, widened speculatively when few remain)",
+                   "samples_started_per_step": started // args.steps, **extra},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"wf_trace_{spec.name.lower()}_traffic.txt"),
                      "kernel": "wf_trace_ext_kernel + wf_trace_shadow_kernel",

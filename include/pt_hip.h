@@ -219,7 +219,11 @@ bool pathTracerAdaptiveRenderPassesCounted(PtFloat4* d_buffer, uint32_t* d_sampl
  * samples of all active pixels are traced together, one bounce per launch: closest-hit and
  * any-hit traversal kernels with an LDS stack, and a shading kernel that appends the next rays
  * to queues. maxPathDepth is the depth the params resolve to (maxPathDepth > 0 ? that : 0 -> 256;
- * PathIntegratorRandCore.h:219-227). maxPathsInFlight bounds the paths of one group (0 = 4M).
+ * PathIntegratorRandCore.h:219-227). maxPathsInFlight bounds the paths of one group (0 = 16M;
+ * at least 4 per active pixel are always allowed). The first group takes every sample that no
+ * pixel can skip (up to minSamples rounded up to 4); later groups take 4 samples per active
+ * pixel, widened speculatively while few pixels remain (samples drawn after a pixel converged
+ * are discarded, so results do not depend on the grouping).
  * Uses a library-owned device workspace (grown on demand); calls on one device must not overlap.
  * The caller's active list is read, not modified. Synchronises the stream once per group. */
 bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
@@ -235,8 +239,10 @@ bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, 
  * nodes fetched, triangles tested and rays. Both reset the accumulated values. */
 bool pathTracerWavefrontProfile(int timing, int counted);
 
-/* out[0..3]: ms of trace-extension, trace-shadow, shade and other launches; out[4..7]: their
- * launch counts; out[8..13]: extension nodes, triangles, rays, shadow nodes, triangles, rays. */
+/* out[0..3]: ms of trace-extension, trace-shadow, shade and other launches (generation,
+ * accumulation); out[4..7]: their launch counts; out[8..13]: extension nodes, triangles, rays,
+ * shadow nodes, triangles, rays; out[14] (when n >= 15): samples started, including speculative
+ * ones that were discarded. */
 bool pathTracerWavefrontProfileRead(double* out, int n);
 
 /* Traces one ray per entry (origins/directions as 3 floats each) through the same closest-hit

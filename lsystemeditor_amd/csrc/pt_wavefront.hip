@@ -336,23 +336,37 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
     if (lane_id() == 63 && total > 0) {
         base = atomicAdd(&P.counters[3], total);
     }
-    base = __shfl(base, 63, 64) + incl - k;
-    if (t >= activeCount) {
-        return;
+    const int waveBase = __shfl(base, 63, 64);
+    if (t < activeCount) {
+        P.pxBase[t] = waveBase + incl - k;
+        P.pxK[t] = k;
     }
-    P.pxBase[t] = base;
-    P.pxK[t] = k;
-    if (k == 0) {
-        return;
-    }
+    // The wave's samples occupy the contiguous slots [waveBase, waveBase + total), pixel by pixel.
+    // Lanes take consecutive slots so every store of the path state is coalesced; a slot's pixel
+    // is the lane whose inclusive prefix first exceeds the slot offset.
     const CameraGpu cam = *camera;
-    const int x = idx % width;
-    const int y = idx / width;
-    for (int j = 0; j < k; ++j) {
-        const int p = base + j;
+    for (int q0 = 0; q0 < total; q0 += 64) {
+        const int q = q0 + lane_id();
+        int owner = 0;
+        for (int step = 32; step > 0; step >>= 1) {
+            const int probe = __shfl(incl, owner + step - 1, 64);
+            if (probe <= q) {
+                owner += step;
+            }
+        }
+        const int oIdx = __shfl(idx, owner, 64);
+        const uint32_t oN = __shfl(n, owner, 64);
+        const int oExcl = __shfl(incl - k, owner, 64);
+        if (q >= total) {
+            continue;
+        }
+        const int j = q - oExcl;
+        const int p = waveBase + q;
+        const int x = oIdx % width;
+        const int y = oIdx / width;
         const uint32_t seed = firstSeed + static_cast<uint32_t>(passBase + j);
         Rng rng;
-        rng_init(rng, idx, static_cast<int>(n) + j, seed);
+        rng_init(rng, oIdx, static_cast<int>(oN) + j, seed);
         const float ju = rand01(rng);
         const float jv = rand01(rng);
         const float u = (static_cast<float>(x) + ju) / static_cast<float>(width);
@@ -361,8 +375,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         const float l2 = rand01(rng);
         V3 ro, rd;
         camera_ray(cam, u, v, l1, l2, ro, rd);
-        P.pixel[p] = static_cast<uint32_t>(idx);
-        P.sample[p] = n + static_cast<uint32_t>(j);
+        P.pixel[p] = static_cast<uint32_t>(oIdx);
+        P.sample[p] = oN + static_cast<uint32_t>(j);
         P.seed[p] = seed;
         P.draws[p] = rng_draws(rng);
         P.ox[p] = ro.x;
@@ -710,6 +724,7 @@ struct Profile {
     long long launches[4] = {0, 0, 0, 0};
     unsigned long long* dStats = nullptr;  // 6 counters
     unsigned long long stats[6] = {0, 0, 0, 0, 0, 0};
+    long long paths = 0;               // samples started (path slots generated)
     hipEvent_t ev[2 * 2048] = {};
     int cls[2048] = {};
     int used = 0;
@@ -839,7 +854,20 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
     }
     // maxPathDepth must equal the depth the params resolve to (PathIntegratorRandCore.h:219-227);
     // it is passed by value because the params live in device memory.
-    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 22);
+    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 24);
+    // Samples every active pixel is sure to take: convergence is only tested at counts that are
+    // multiples of 4 and at least minSamples, or reached at maxSamplesPerPixel. A first group of
+    // that size starts no speculative sample.
+    RenderParamsGpu hp{};
+    if (hipMemcpyAsync(&hp, d_params, sizeof(hp), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        ptSetError("pathTracerAdaptiveRenderWavefront: params read failed");
+        return false;
+    }
+    int safeEnd = std::max(4, (hp.minSamples + 3) & ~3);
+    if (hp.maxSamplesPerPixel > 0) {
+        safeEnd = std::min(safeEnd, hp.maxSamplesPerPixel);
+    }
     int device = 0, cus = 256;
     if (hipGetDevice(&device) == hipSuccess) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
@@ -867,15 +895,24 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
     while (active > 0 && passBase < passCount) {
         // group size: 4 samples per pixel (one convergence interval), widened speculatively
         // while few pixels remain so the GPU stays full
-        int mult = std::max(1, std::min(16, gWs.capacity / std::max(1, 4 * active)));
-        int group = 4 * mult;
+        int group = 0;
+        if (passBase < safeEnd) {
+            group = safeEnd - passBase;
+            const int fit = std::max(1, gWs.capacity / std::max(1, active));
+            group = std::min(group, fit >= 4 ? fit & ~3 : fit);
+        } else {
+            const int budget = std::min(gWs.capacity, 1 << 22);
+            group = 4 * std::max(1, std::min(16, budget / std::max(1, 4 * active)));
+        }
         group = std::min(group, passCount - passBase);
         const long long maxPaths = static_cast<long long>(group) * active;
         hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
                            counterInts);
+        prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur,
                            d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
                            passCount, group, P);
+        prof_mark(stream, 3, true);
         const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
         const unsigned shadowBlocks =
             static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
@@ -912,8 +949,10 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
                                P.shadeQ[(b + 1) & 1], cOut);
             prof_mark(stream, 2, true);
         }
+        prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
                            reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
+        prof_mark(stream, 3, true);
         hipLaunchKernelGGL(wf_zero_kernel, dim3(1), dim3(kWfBlock), 0, stream, countNext, 1);
         hipLaunchKernelGGL(wf_compact_kernel, dim3(std::min<unsigned>(blocks_for(active), 4096u)), dim3(kWfBlock), 0,
                            stream, list, countCur, d_converged, listNext, countNext);
@@ -925,6 +964,12 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
             hipStreamSynchronize(stream) != hipSuccess) {
             ptSetError("pathTracerAdaptiveRenderWavefront: count readback failed");
             return false;
+        }
+        if (gProf.timing || gProf.counted) {
+            int started = 0;
+            if (hipMemcpy(&started, &P.counters[3], sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) {
+                gProf.paths += started;
+            }
         }
         prof_collect();
         std::swap(list, listNext);
@@ -943,6 +988,7 @@ extern "C" bool pathTracerWavefrontProfile(int timing, int counted)
         gProf.launches[k] = 0;
     }
     gProf.used = 0;
+    gProf.paths = 0;
     if (gProf.counted) {
         if (gProf.dStats == nullptr && hipMalloc(reinterpret_cast<void**>(&gProf.dStats), 6 * sizeof(unsigned long long)) != hipSuccess) {
             ptSetError("pathTracerWavefrontProfile: allocation failed");
@@ -978,6 +1024,9 @@ extern "C" bool pathTracerWavefrontProfileRead(double* out, int n)
     }
     for (int k = 0; k < 6; ++k) {
         out[8 + k] = static_cast<double>(st[k]);
+    }
+    if (n >= 15) {
+        out[14] = static_cast<double>(gProf.paths);
     }
     return true;
 }
