@@ -222,8 +222,8 @@ bool pathTracerAdaptiveRenderPassesCounted(PtFloat4* d_buffer, uint32_t* d_sampl
  * PathIntegratorRandCore.h:219-227). maxPathsInFlight bounds the paths of one group (0 = 16M;
  * at least 4 per active pixel are always allowed). The first group takes every sample that no
  * pixel can skip (up to minSamples rounded up to 4); later groups take 4 samples per active
- * pixel, widened speculatively while few pixels remain (samples drawn after a pixel converged
- * are discarded, so results do not depend on the grouping).
+ * pixel, widened speculatively up to 64 (samples drawn after a pixel converged are discarded,
+ * so results do not depend on the grouping).
  * Uses a library-owned device workspace (grown on demand); calls on one device must not overlap.
  * The caller's active list is read, not modified. Synchronises the stream once per group. */
 bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
@@ -244,6 +244,17 @@ bool pathTracerWavefrontProfile(int timing, int counted);
  * shadow nodes, triangles, rays; out[14] (when n >= 15): samples started, including speculative
  * ones that were discarded. */
 bool pathTracerWavefrontProfileRead(double* out, int n);
+
+/* Scheduling knob of pathTracerAdaptiveRenderWavefront: once pixels may converge mid-group,
+ * groups are widened (in steps of 4 samples per pixel, at most 64) while the group's paths stay
+ * within speculativePaths (0 = the workspace capacity, the default). Results do not depend on
+ * it; larger values trade discarded speculative samples for fewer, fuller launches. */
+bool pathTracerWavefrontSchedule(int speculativePaths);
+
+/* Per-bounce breakdown of the profiled renders (pathTracerWavefrontProfile with timing on):
+ * for bounce b < n (n <= 258), out[5b..5b+4] = ms of the extension-trace, shadow-trace and
+ * shade launches, extension rays, shadow rays. */
+bool pathTracerWavefrontProfileBounces(double* out, int n);
 
 /* Traces one ray per entry (origins/directions as 3 floats each) through the same closest-hit
  * traversal the integrator uses; writes hit flag, t, normal (3), uv (2), triangle index as

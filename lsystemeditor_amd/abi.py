@@ -57,6 +57,8 @@ SIGNATURES = {
                                                _p]),
     "pathTracerWavefrontProfile": (_b, [_i, _i]),
     "pathTracerWavefrontProfileRead": (_b, [_p, _i]),
+    "pathTracerWavefrontProfileBounces": (_b, [_p, _i]),
+    "pathTracerWavefrontSchedule": (_b, [_i]),
     "pathTracerTraceRays": (_b, [_p, _p, _p, _i, _f, _f, _p, _p]),
     "pathTracerCountTraversal": (_b, [_p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _p, _p]),
     "ptSceneCreate": (_p, []),

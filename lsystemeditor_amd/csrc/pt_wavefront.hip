@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 using namespace pt;
 
@@ -140,6 +141,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     __shared__ int2 ring[kLdsEntries * kWfBlock];
     const MeshAccelSceneGpu s = *scene;
     const int n = *countPtr;
+    if (n == 0) {
+        return; // deep bounces of a group often have no rays left
+    }
     const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     LdsStack st{(LdsInt2*)(ring + threadIdx.x), (GlobalInt2*)(P.spill + gtid), P.spillThreads, 0, 0};
     TraceStats ts{0u, 0u, 0u, 0u};
@@ -725,13 +729,21 @@ struct Profile {
     unsigned long long* dStats = nullptr;  // 6 counters
     unsigned long long stats[6] = {0, 0, 0, 0, 0, 0};
     long long paths = 0;               // samples started (path slots generated)
+    double bounceMs[3][258] = {};      // per bounce: trace ext, trace shadow, shade
+    long long bounceRays[2][258] = {}; // per bounce: extension rays, shadow rays
     hipEvent_t ev[2 * 2048] = {};
     int cls[2048] = {};
+    int bounce[2048] = {};
     int used = 0;
 };
 Profile gProf;
 
-void prof_mark(hipStream_t stream, int cls, bool end)
+// Paths per group once pixels may converge mid-group (0 = the workspace capacity). Pixels of an
+// adaptive render mostly converge at minSamples (constant radiance) or run to maxSamplesPerPixel,
+// so wide groups discard few speculative samples and save many nearly empty deep-bounce launches.
+int gSpeculativeBudget = 0;
+
+void prof_mark(hipStream_t stream, int cls, bool end, int bounce = -1)
 {
     if (!gProf.timing || gProf.used >= 2048) {
         return;
@@ -743,6 +755,7 @@ void prof_mark(hipStream_t stream, int cls, bool end)
     (void)hipEventRecord(gProf.ev[2 * i + (end ? 1 : 0)], stream);
     if (end) {
         gProf.cls[i] = cls;
+        gProf.bounce[i] = bounce;
         gProf.used = i + 1;
     }
 }
@@ -754,6 +767,9 @@ void prof_collect()
         if (hipEventElapsedTime(&ms, gProf.ev[2 * i], gProf.ev[2 * i + 1]) == hipSuccess) {
             gProf.ms[gProf.cls[i]] += ms;
             gProf.launches[gProf.cls[i]] += 1;
+            if (gProf.cls[i] < 3 && gProf.bounce[i] >= 0 && gProf.bounce[i] < 258) {
+                gProf.bounceMs[gProf.cls[i]][gProf.bounce[i]] += ms;
+            }
         }
     }
     gProf.used = 0;
@@ -901,7 +917,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
             const int fit = std::max(1, gWs.capacity / std::max(1, active));
             group = std::min(group, fit >= 4 ? fit & ~3 : fit);
         } else {
-            const int budget = std::min(gWs.capacity, 1 << 22);
+            const int budget = gSpeculativeBudget > 0 ? std::min(gWs.capacity, gSpeculativeBudget) : gWs.capacity;
             group = 4 * std::max(1, std::min(16, budget / std::max(1, 4 * active)));
         }
         group = std::min(group, passCount - passBase);
@@ -921,7 +937,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
             int* cOut = P.counters + kCountersPerBounce * (b + 1); // produced by this bounce
             const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
             const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
-            prof_mark(stream, 0, false);
+            prof_mark(stream, 0, false, b);
             if (gProf.counted) {
                 hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
                                    d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats);
@@ -929,9 +945,9 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
                 hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
                                    d_scene, P, extQueue, extCount, &cIn[4], nullptr);
             }
-            prof_mark(stream, 0, true);
+            prof_mark(stream, 0, true, b);
             if (b > 0) {
-                prof_mark(stream, 1, false);
+                prof_mark(stream, 1, false, b);
                 if (gProf.counted) {
                     hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
                                        d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats);
@@ -939,15 +955,15 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
                     hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
                                        d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr);
                 }
-                prof_mark(stream, 1, true);
+                prof_mark(stream, 1, true, b);
             }
             const int* shadeQueue = b == 0 ? nullptr : P.shadeQ[b & 1];
             const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
-            prof_mark(stream, 2, false);
+            prof_mark(stream, 2, false, b);
             hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
                                P, b, maxPathDepth, shadeQueue, shadeCount, P.extQ[(b + 1) & 1],
                                P.shadeQ[(b + 1) & 1], cOut);
-            prof_mark(stream, 2, true);
+            prof_mark(stream, 2, true, b);
         }
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
@@ -966,9 +982,13 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
             return false;
         }
         if (gProf.timing || gProf.counted) {
-            int started = 0;
-            if (hipMemcpy(&started, &P.counters[3], sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) {
-                gProf.paths += started;
+            std::vector<int> cnt(static_cast<size_t>(counterInts));
+            if (hipMemcpy(cnt.data(), P.counters, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost) == hipSuccess) {
+                gProf.paths += cnt[3];
+                for (int b = 0; b <= maxPathDepth && b < 258; ++b) {
+                    gProf.bounceRays[0][b] += b == 0 ? cnt[3] : cnt[kCountersPerBounce * b + 0];
+                    gProf.bounceRays[1][b] += b == 0 ? 0 : cnt[kCountersPerBounce * b + 2];
+                }
             }
         }
         prof_collect();
@@ -989,6 +1009,8 @@ extern "C" bool pathTracerWavefrontProfile(int timing, int counted)
     }
     gProf.used = 0;
     gProf.paths = 0;
+    std::memset(gProf.bounceMs, 0, sizeof(gProf.bounceMs));
+    std::memset(gProf.bounceRays, 0, sizeof(gProf.bounceRays));
     if (gProf.counted) {
         if (gProf.dStats == nullptr && hipMalloc(reinterpret_cast<void**>(&gProf.dStats), 6 * sizeof(unsigned long long)) != hipSuccess) {
             ptSetError("pathTracerWavefrontProfile: allocation failed");
@@ -1028,5 +1050,33 @@ extern "C" bool pathTracerWavefrontProfileRead(double* out, int n)
     if (n >= 15) {
         out[14] = static_cast<double>(gProf.paths);
     }
+    return true;
+}
+
+// Per-bounce breakdown of the profiled renders: for bounce b < n, out[5 * b + 0..4] = ms of the
+// extension trace, shadow trace and shade launches, extension rays, shadow rays.
+extern "C" bool pathTracerWavefrontProfileBounces(double* out, int n)
+{
+    if (out == nullptr || n <= 0 || n > 258) {
+        ptSetError("pathTracerWavefrontProfileBounces: invalid arguments");
+        return false;
+    }
+    for (int b = 0; b < n; ++b) {
+        out[5 * b + 0] = gProf.bounceMs[0][b];
+        out[5 * b + 1] = gProf.bounceMs[1][b];
+        out[5 * b + 2] = gProf.bounceMs[2][b];
+        out[5 * b + 3] = static_cast<double>(gProf.bounceRays[0][b]);
+        out[5 * b + 4] = static_cast<double>(gProf.bounceRays[1][b]);
+    }
+    return true;
+}
+
+extern "C" bool pathTracerWavefrontSchedule(int speculativePaths)
+{
+    if (speculativePaths < 0) {
+        ptSetError("pathTracerWavefrontSchedule: invalid arguments");
+        return false;
+    }
+    gSpeculativeBudget = speculativePaths;
     return true;
 }
