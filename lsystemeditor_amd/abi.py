@@ -14,6 +14,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libpt_hip.so"
 HEADER = PKG.parent / "include" / "pt_hip.h"
+TRACER_HEADER = PKG.parent / "include" / "pt_tracer.h"
 
 # Record sizes in 32-bit words (reference layouts, include/pt_hip.h).
 HOST_TRIANGLE_WORDS = 25   # HostTriangle, 100 bytes
@@ -37,6 +38,11 @@ _i = C.c_int
 _u = C.c_uint
 _f = C.c_float
 _p = C.c_void_p
+
+
+class Vec3C(C.Structure):
+    """Vec3 passed by value (three floats)."""
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
 
 SIGNATURES = {
     "pathTracerLastError": (C.c_char_p, []),
@@ -72,15 +78,73 @@ SIGNATURES = {
     "ptEnvironmentDevice": (_p, [_p]),
     "ptEnvironmentReadTables": (_b, [_p, _p, _p]),
     "ptEnvironmentDestroy": (None, [_p]),
+    # pt_tracer.h
+    "ptTracerCreate": (_p, []),
+    "ptTracerDestroy": (None, [_p]),
+    "ptTracerSetFrameReadyCallback": (None, [_p, _p, _p]),
+    "ptTracerConfigure": (_b, [_p, _i, _i, _p, _u, _p, _u, _p, _u]),
+    "ptTracerRebuildMeshScene": (_b, [_p, _p, _u, _p, _u, _p, _u]),
+    "ptTracerStart": (None, [_p]),
+    "ptTracerStop": (None, [_p]),
+    "ptTracerResetAccumulation": (None, [_p]),
+    "ptTracerResetRegionAccumulation": (None, [_p]),
+    "ptTracerIsRunning": (_b, [_p]),
+    "ptTracerReleaseOutputSurfaces": (None, [_p]),
+    "ptTracerBeginDisplayPublish": (_b, [_p, _i]),
+    "ptTracerIsDisplayPublishReady": (_b, [_p]),
+    "ptTracerHasDisplayPublishInFlight": (_b, [_p]),
+    "ptTracerFinishDisplayPublish": (_i, [_p]),
+    "ptTracerPublishDisplayFrame": (_b, [_p, _i]),
+    "ptTracerReadDisplay": (_b, [_p, _i, _p]),
+    "ptTracerSetMaxSamplesPerPixel": (None, [_p, _i]),
+    "ptTracerMaxSamplesPerPixel": (_i, [_p]),
+    "ptTracerSetMinSamples": (None, [_p, _i]),
+    "ptTracerMinSamples": (_i, [_p]),
+    "ptTracerSetRelativeErrorThreshold": (None, [_p, _f]),
+    "ptTracerRelativeErrorThreshold": (_f, [_p]),
+    "ptTracerSetDebugOverlayMode": (None, [_p, _i]),
+    "ptTracerDebugOverlayMode": (_i, [_p]),
+    "ptTracerSetBrdfDebugFlags": (None, [_p, _i]),
+    "ptTracerBrdfDebugFlags": (_i, [_p]),
+    "ptTracerSetPreviewStepsPerLevel": (None, [_p, _i]),
+    "ptTracerPreviewStepsPerLevel": (_i, [_p]),
+    "ptTracerCurrentSampleCount": (_i, [_p]),
+    "ptTracerCurrentActivePixelCount": (_i, [_p]),
+    "ptTracerSampleBudgetTotalIterations": (_i, [_p]),
+    "ptTracerIsSampleBudgetExhausted": (_b, [_p]),
+    "ptTracerSetCamera": (None, [_p, _p]),
+    "ptTracerLastSampleCamera": (None, [_p, _p]),
+    "ptTracerSetClearColor": (None, [_p, _f, _f, _f]),
+    "ptTracerSetPhysicalCamera": (None, [_p, _f, _f, _f]),
+    "ptTracerSetFocusPoint": (None, [_p, Vec3C]),
+    "ptTracerClearFocusPoint": (None, [_p]),
+    "ptTracerPickSurface": (_b, [_p, Vec3C, Vec3C, _p]),
+    "ptTracerSetEnvironmentImage": (_b, [_p, _p, _i, _i]),
+    "ptTracerSetEnvironmentIntensity": (None, [_p, _f]),
+    "ptTracerEnvironmentIntensity": (_f, [_p]),
+    "ptTracerSetRussianRouletteMinDepth": (None, [_p, _i]),
+    "ptTracerRussianRouletteMinDepth": (_i, [_p]),
+    "ptTracerSetRegionRenderEnabled": (None, [_p, _b]),
+    "ptTracerRegionRenderEnabled": (_b, [_p]),
+    "ptTracerSetRenderRegion": (None, [_p, _i, _i, _i, _i]),
+    "ptTracerRenderRegion": (None, [_p, _p, _p, _p, _p]),
+    "ptTracerWaitUntilIdle": (_b, [_p, _i]),
+    "ptTracerReadAccumulator": (_b, [_p, _p, _p, _p, _p, _p]),
+    "ptTracerReadPreviewLevel": (_b, [_p, _i, _p, _p, _p, _p]),
+    "ptTracerExposureMultiplier": (_f, [_p]),
+    "ptTracerSetMaxPathDepth": (None, [_p, _i]),
+    "ptTracerMaxPathDepth": (_i, [_p]),
 }
 
 
 def header_symbols() -> list[str]:
-    """Function names declared in include/pt_hip.h."""
+    """Function names declared with C linkage in include/pt_hip.h and include/pt_tracer.h."""
     text = HEADER.read_text()
+    tracer = TRACER_HEADER.read_text()
+    text += tracer[: tracer.index("} /* extern \"C\" */")]
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)) - {
-        "if", "defined", "alignas", "_Alignas", "sizeof", "PT_ALIGN16"})
+        "if", "defined", "alignas", "_Alignas", "sizeof", "PT_ALIGN16", "void"})
 
 
 def lib():

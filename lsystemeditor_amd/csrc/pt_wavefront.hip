@@ -412,7 +412,10 @@ PT_DEV void push_shadow(Paths& P, int* counter, bool want, V3 o, V3 d, int path,
 }
 
 // One bounce of PathIntegratorRandCore.h:241-347 for every path in the shade queue.
-__global__ __launch_bounds__(kWfBlock) void wf_shade_kernel(const MeshAccelSceneGpu* scene,
+#ifndef PT_SHADE_ATTR
+#define PT_SHADE_ATTR
+#endif
+__global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const MeshAccelSceneGpu* scene,
                                                             const RenderParamsGpu* params,
                                                             const EnvironmentMapGpu* env, Paths P, int bounce,
                                                             int maxDepth, const int* __restrict__ inQueue,

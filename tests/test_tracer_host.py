@@ -1,0 +1,106 @@
+"""Host logic of the renderer facade (include/pt_tracer.h) that runs without a GPU: defaults and
+clamps of the setters (PathTracer.cpp:126-190, 1710-1990), the sample-budget policy
+(PathTracerSampleBudget.h, compared with the reference header compiled in oracle/_ref when
+present) and the physical-camera exposure (PhysicalCamera.cpp:397-429). No kernel is launched:
+an unconfigured tracer allocates no device memory."""
+import numpy as np
+import pytest
+
+from lsystemeditor_amd.tracer import Tracer
+from oracle import oracle as O
+
+
+@pytest.fixture
+def t():
+    tr = Tracer()
+    yield tr
+    tr.close()
+
+
+def test_defaults_match_reference(t):
+    # PathTracerImpl member initialisers (PathTracer.cpp:60-84) and RenderTypes.h:15-29
+    assert t.max_samples_per_pixel() == 8
+    assert t.min_samples() == 16
+    assert t.relative_error_threshold() == np.float32(0.02)
+    assert t.debug_overlay_mode() == 0
+    assert t.brdf_debug_flags() == 0
+    assert t.preview_steps_per_level() == 0
+    assert t.environment_intensity() == 1.0
+    assert t.russian_roulette_min_depth() == 3
+    assert t.max_path_depth() == 32
+    assert not t.is_running()
+    assert not t.region_render_enabled()
+
+
+@pytest.mark.parametrize("setter,getter,value,expected", [
+    ("set_max_samples_per_pixel", "max_samples_per_pixel", -5, 0),
+    ("set_max_samples_per_pixel", "max_samples_per_pixel", 2_000_000, 1_000_000),
+    ("set_max_samples_per_pixel", "max_samples_per_pixel", 256, 256),
+    ("set_min_samples", "min_samples", 0, 1),
+    ("set_min_samples", "min_samples", 20_000, 10_000),
+    ("set_debug_overlay_mode", "debug_overlay_mode", -1, 0),
+    ("set_debug_overlay_mode", "debug_overlay_mode", 7, 3),
+    ("set_preview_steps_per_level", "preview_steps_per_level", -2, 0),
+    ("set_preview_steps_per_level", "preview_steps_per_level", 500, 128),
+    ("set_russian_roulette_min_depth", "russian_roulette_min_depth", -3, 0),
+    ("set_russian_roulette_min_depth", "russian_roulette_min_depth", 999, 256),
+    ("set_brdf_debug_flags", "brdf_debug_flags", 16, 16),
+])
+def test_int_setter_clamps(t, setter, getter, value, expected):
+    getattr(t, setter)(value)
+    assert getattr(t, getter)() == expected
+
+
+@pytest.mark.parametrize("setter,getter,value,expected", [
+    ("set_relative_error_threshold", "relative_error_threshold", 1e-5, 0.001),
+    ("set_relative_error_threshold", "relative_error_threshold", 5.0, 1.0),
+    ("set_relative_error_threshold", "relative_error_threshold", 0.05, 0.05),
+    ("set_environment_intensity", "environment_intensity", -1.0, 0.0),
+    ("set_environment_intensity", "environment_intensity", 1000.0, 100.0),
+    ("set_environment_intensity", "environment_intensity", 2.5, 2.5),
+])
+def test_float_setter_clamps(t, setter, getter, value, expected):
+    getattr(t, setter)(value)
+    assert getattr(t, getter)() == np.float32(expected)
+
+
+@pytest.mark.parametrize("preview,max_spp", [(0, 0), (0, 8), (3, 16), (5, 0), (128, 1_000_000)])
+def test_sample_budget_total(t, preview, max_spp):
+    t.set_preview_steps_per_level(preview)
+    t.set_max_samples_per_pixel(max_spp)
+    want = -1 if max_spp <= 0 else preview + max_spp      # PathTracerSampleBudget.h:8-14
+    assert t.sample_budget_total_iterations() == want
+    ref = O.ref_policy()
+    if ref is not None:
+        assert ref.ref_budget_total(preview, max_spp) == want
+
+
+def test_budget_exhausted_without_active_pixels(t):
+    # count 0 >= preview 0: the adaptive branch, exhausted when no pixel is active
+    assert t.is_sample_budget_exhausted()
+    ref = O.ref_policy()
+    if ref is not None:
+        assert ref.ref_adaptive_exhausted(0, 0, 0) == 1
+    t.set_preview_steps_per_level(2)
+    t.set_max_samples_per_pixel(4)
+    assert not t.is_sample_budget_exhausted()  # count 0 < preview 2: 0 < 2 + 4
+
+
+def test_exposure_multiplier(t):
+    # defaults f/2.8, 1/125 s, ISO 100 (PhysicalCamera.h:36-38)
+    assert t.exposure_multiplier() == np.float32(np.float32(1.0 / 125.0) / np.float32(2.8 * 2.8))
+    t.set_physical_camera(0.5, 1e-6, 50.0)   # clamped to f/1.2, 1/8000 s, ISO 100
+    f = np.float32(1.2)
+    want = np.float32(np.float32(1.0 / 8000.0) * np.float32(1.0)) / np.float32(f * f)
+    assert abs(t.exposure_multiplier() - want) <= 1e-9
+    t.set_physical_camera(16.0, 2.0, 800.0)
+    want = np.float32(np.float32(2.0) * np.float32(8.0)) / np.float32(np.float32(16.0) * np.float32(16.0))
+    assert t.exposure_multiplier() == want
+
+
+def test_unconfigured_tracer_is_inert(t):
+    t.start()
+    assert not t.is_running()
+    assert t.wait_until_idle(10)
+    assert t.pick_surface((0, 0, 0), (0, 0, -1)) is None
+    assert not t.publish_display_frame(0)
