@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=["c1", "c0"], default="c1")
+    ap.add_argument("--bvh", choices=["median", "sah"], default="median",
+                    help="BVH build: the reference's median split, or binned SAH (same results, faster traversal)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-pass (reference cadence) timing")
@@ -117,11 +119,11 @@ def main():
     print(f"[bench] scene {spec.name}: {spec.triangles.shape[0]} triangles, {spec.width}x{spec.height}",
           file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()
-    scene = abi.Scene(spec.triangles, spec.materials, stream=int(stream.cuda_stream))
+    scene = abi.Scene(spec.triangles, spec.materials, stream=int(stream.cuda_stream), bvh=args.bvh)
     inputs = render.DeviceInputs(spec.camera, spec.params)
     acc = render.Accumulator.allocate(spec.width, spec.height)
     passes = spec.spp if spec.spp else inputs.max_spp
-    print(f"[bench] BVH build {scene.build_ms:.2f} ms (GPU)", file=sys.stderr, flush=True)
+    print(f"[bench] BVH build ({args.bvh}) {scene.build_ms:.2f} ms (GPU)", file=sys.stderr, flush=True)
     L = abi.lib()
 
     def step():
@@ -213,7 +215,7 @@ def main():
                                f"{spec.triangles.shape[0]} triangles, "
                                + ("adaptive 16-256 spp at relative error 0.01, max depth 12, 2 area lights + "
                                   "uniform environment" if adaptive else f"{spec.spp} spp, max depth 8"),
-                   "samples_per_step": samples_per_step, "bvh_build_ms": round(scene.build_ms, 3),
+                   "samples_per_step": samples_per_step, "bvh": args.bvh, "bvh_build_ms": round(scene.build_ms, 3),
                    "scheduler": "wavefront (first group: the samples no pixel can skip; then 4 samples per "
                                 "active pixel between convergence tests, widened speculatively when few remain)",
                    "samples_started_per_step": started // args.steps, **extra},

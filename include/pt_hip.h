@@ -297,6 +297,17 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
 /* Milliseconds of the last GPU BVH build (device time between events). */
 float ptSceneLastBuildMs(const PtScene* scene);
 
+/* BVH build used by the next ptSceneBuild (no reference counterpart): 0 = the reference's
+ * median split (default; node arrays equal the reference's), 1 = binned SAH over Morton-ordered
+ * triangles (same device layout, geometry-dependent topology, depth <= 48; ptSceneReadBvh then
+ * reports 0 nodes). Closest hits are the same except for exact ties between triangles. */
+bool ptSceneSetBuildMode(PtScene* scene, int mode);
+
+/* Copies the device layout: internal nodes (nodeCount x 16 floats) and leaf slots
+ * (triangleCount x 12 floats); NULL skips one. levels receives the SAH build's level count
+ * (0 for the median split). */
+bool ptSceneReadLayout(const PtScene* scene, float* nodes, float* triPos, int* levels);
+
 /* ---- environment map (replaces EnvironmentMap::buildImportanceTables + upload,
  *      PathTracerLib/src/EnvironmentMap.cpp:76-127) --------------------------------------- */
 typedef struct PtEnvironment PtEnvironment;

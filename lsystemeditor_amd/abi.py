@@ -74,6 +74,8 @@ SIGNATURES = {
     "ptSceneDescription": (_b, [_p, _p]),
     "ptSceneReadBvh": (_b, [_p, _p, _u, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ptSceneLastBuildMs": (_f, [_p]),
+    "ptSceneSetBuildMode": (_b, [_p, _i]),
+    "ptSceneReadLayout": (_b, [_p, _p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
     "ptEnvironmentDevice": (_p, [_p]),
     "ptEnvironmentReadTables": (_b, [_p, _p, _p]),
@@ -271,12 +273,16 @@ def pack_params(background=(10.0 / 255.0, 10.0 / 255.0, 10.0 / 255.0), environme
 class Scene:
     """A scene built on the GPU (ptSceneBuild): BVH, device layout, emissive list."""
 
+    BUILD_MODES = {"median": 0, "sah": 1}
+
     def __init__(self, host_triangles: np.ndarray, materials: np.ndarray, textures: np.ndarray | None = None,
-                 stream: int = 0):
+                 stream: int = 0, bvh: str = "median"):
         L = lib()
         self.handle = L.ptSceneCreate()
         if not self.handle:
             raise RuntimeError("ptSceneCreate failed")
+        self.bvh = bvh
+        check(L.ptSceneSetBuildMode(self.handle, self.BUILD_MODES[bvh]), "ptSceneSetBuildMode")
         tris = np.ascontiguousarray(host_triangles, dtype=np.float32)
         mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, MATERIAL_WORDS)
         tex = None if textures is None else np.ascontiguousarray(textures, dtype=np.float32).reshape(-1, TEXTURE_WORDS)
@@ -297,6 +303,16 @@ class Scene:
     @property
     def build_ms(self) -> float:
         return lib().ptSceneLastBuildMs(self.handle)
+
+    def read_layout(self) -> tuple[np.ndarray, np.ndarray, int]:
+        """Device layout: internal nodes (n-1, 16) float32, leaf slots (n, 12) float32, SAH levels."""
+        n = self.triangle_count
+        nodes = np.zeros((max(n - 1, 0), 16), np.float32)
+        slots = np.zeros((n, 12), np.float32)
+        levels = C.c_int(0)
+        check(lib().ptSceneReadLayout(self.handle, nodes.ctypes.data if nodes.size else None,
+                                      slots.ctypes.data if slots.size else None, C.byref(levels)), "ptSceneReadLayout")
+        return nodes, slots, levels.value
 
     def description(self) -> bytes:
         buf = (C.c_uint8 * SCENE_DESC_BYTES)()

@@ -380,7 +380,7 @@ constexpr float kMinPdf = 1.0e-8f;
 constexpr float kPi = 3.14159265f;
 constexpr float kInvPi = 1.0f / kPi;
 constexpr float kMinRadius = 1.0e-5f;
-constexpr int kStackSize = 40;
+constexpr int kStackSize = 64; // traversal stack entries (tree depth <= 48 for every build mode)
 
 // Loads through the global address space: global_load instructions (vector memory counter
 // only) instead of flat ones, which also count against the LDS counter.

@@ -5,3 +5,15 @@
 
 // Records the message returned by pathTracerLastError() (thread-local).
 void ptSetError(const char* fmt, ...);
+
+// Deepest leaf level the binned-SAH build produces; traversal stacks hold 64 entries.
+constexpr int kMaxSahDepth = 48;
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+// Binned-SAH BVH over device triangles (25-word records): internal nodes (n - 1 x 64 B), leaf
+// slots (3 x float4 each) and shading records (4 x float4 per triangle). rootOut receives the root
+// box and the root reference (bits of an int in rootOut[6]); levelsOut the number of levels.
+bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
+                      float rootOut[7], int* levelsOut);
+#endif

@@ -1,0 +1,665 @@
+// pt_sah.hip -- binned-SAH BVH build on the GPU (performance build mode).
+//
+// Produces the same device layout as the median-split build (64-byte internal nodes holding both
+// child boxes, one triangle per leaf slot), so every traversal kernel runs on it unchanged.
+//   1. Triangles are put in Morton order of their centroids (30-bit codes, radix sort).
+//   2. The tree is built top down, one level of segments at a time. Per level: each segment's
+//      triangle and centroid bounds (block-level LDS aggregation, then global atomics); 32 bins
+//      along the segment's longest centroid axis, staged in LDS per block; one thread per
+//      segment sweeps the bins for the lowest surface-area cost; a stable partition (scan)
+//      moves each segment's left side to its front. The partition is stable, so the Morton
+//      order survives inside every child: when the bins cannot separate a segment (all
+//      centroids in one bin) it is split at its middle position, which is a spatial split of a
+//      Morton-ordered range.
+//   3. Internal nodes are numbered level by level (a scan over the level's segments), so the
+//      tree is deterministic. Depth is capped: a segment that could not otherwise finish within
+//      kMaxSahDepth levels is split at its middle position.
+// Unlike the median-split build, the topology depends on the geometry; closest hits are the same
+// set of (t, triangle) minima, and only exact ties between triangles could resolve differently.
+#include "pt_device.h"
+#include "pt_internal.h"
+
+#include "../../include/pt_hip.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kBins = 32;
+constexpr int kBinWords = 7;   // count, min xyz, max xyz (sortable integers)
+constexpr int kSegWords = 12;  // triangle bounds (6) and centroid bounds (6), sortable integers
+static_assert(kMaxSahDepth <= 56, "traversal stacks hold 64 entries");
+
+__device__ __forceinline__ uint32_t sortable(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float unsortable(uint32_t u)
+{
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ float fmin2(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmax2(float a, float b) { return a > b ? a : b; }
+
+struct SegRec {
+    int begin, count, parent, side;
+};
+
+// Per-segment split decision of the current level.
+struct SplitRec {
+    int axis;      // 0..2, or -1: split at the middle position
+    float k0;      // centroid minimum on the axis
+    float scale;   // bins per unit length
+    int splitBin;  // left side: bins 0..splitBin
+    int leftCount;
+    int internal;  // 1 when the segment is an internal node
+};
+
+__global__ void sah_prep_kernel(const float* __restrict__ tris, int n, float4* __restrict__ bounds,
+                                float4* __restrict__ cent, float4* __restrict__ shade, uint32_t* __restrict__ gcb)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    if (i < n) {
+        const float* t = tris + 25 * static_cast<size_t>(i);
+        const float third = 1.0f / 3.0f;
+        const float cx = ((t[0] + t[3]) + t[6]) * third;
+        const float cy = ((t[1] + t[4]) + t[7]) * third;
+        const float cz = ((t[2] + t[5]) + t[8]) * third;
+        cent[i] = make_float4(cx, cy, cz, 0.0f);
+        bounds[2 * i] = make_float4(fmin2(fmin2(t[0], t[3]), t[6]), fmin2(fmin2(t[1], t[4]), t[7]),
+                                    fmin2(fmin2(t[2], t[5]), t[8]), 0.0f);
+        bounds[2 * i + 1] = make_float4(fmax2(fmax2(t[0], t[3]), t[6]), fmax2(fmax2(t[1], t[4]), t[7]),
+                                        fmax2(fmax2(t[2], t[5]), t[8]), 0.0f);
+        shade[4 * i + 0] = make_float4(t[9], t[10], t[11], t[12]);
+        shade[4 * i + 1] = make_float4(t[13], t[14], t[15], t[16]);
+        shade[4 * i + 2] = make_float4(t[17], t[18], t[19], t[20]);
+        shade[4 * i + 3] = make_float4(t[21], t[22], t[23], t[24]);
+        v[0] = v[3] = sortable(cx);
+        v[1] = v[4] = sortable(cy);
+        v[2] = v[5] = sortable(cz);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int k = 0; k < 3; ++k) {
+            v[k] = min(v[k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[k]), off, 64)));
+            v[3 + k] = max(v[3 + k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + k]), off, 64)));
+        }
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&gcb[k], v[k]);
+            atomicMax(&gcb[3 + k], v[3 + k]);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void morton_kernel(const float4* __restrict__ cent, int n, const uint32_t* __restrict__ gcb,
+                              uint32_t* __restrict__ keys, int* __restrict__ vals)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const float4 c = cent[i];
+    const float cc[3] = {c.x, c.y, c.z};
+    uint32_t code = 0u;
+    for (int k = 0; k < 3; ++k) {
+        const float lo = unsortable(gcb[k]);
+        const float hi = unsortable(gcb[3 + k]);
+        const float ext = hi - lo;
+        float u = ext > 0.0f ? (cc[k] - lo) / ext : 0.0f;
+        u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+        code |= expand_bits(static_cast<uint32_t>(u)) << (2 - k);
+    }
+    keys[i] = code;
+    vals[i] = i;
+}
+
+__global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, int segs)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= segs * kSegWords) {
+        return;
+    }
+    const int w = i % kSegWords;
+    segStats[i] = (w % 6) < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+// Block-uniform blocks (one segment) reduce in LDS first; others use global atomics per lane.
+__global__ __launch_bounds__(kB) void seg_bounds_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
+                                                        const float4* __restrict__ bounds,
+                                                        const float4* __restrict__ cent, int n,
+                                                        uint32_t* __restrict__ segStats)
+{
+    __shared__ uint32_t lds[kSegWords];
+    const int base = static_cast<int>(blockIdx.x * blockDim.x);
+    const int i = base + static_cast<int>(threadIdx.x);
+    const int last = min(base + kB, n) - 1;
+    const int s0 = posSeg[base];
+    const bool uniform = s0 >= 0 && posSeg[last] == s0;
+    if (uniform && threadIdx.x < kSegWords) {
+        lds[threadIdx.x] = (threadIdx.x % 6) < 3 ? 0xFFFFFFFFu : 0u;
+    }
+    __syncthreads();
+    const int s = i < n ? posSeg[i] : -1;
+    if (s >= 0) {
+        const int tri = perm[i];
+        const float4 lo = bounds[2 * tri];
+        const float4 hi = bounds[2 * tri + 1];
+        const float4 c = cent[tri];
+        const uint32_t v[kSegWords] = {sortable(lo.x), sortable(lo.y), sortable(lo.z), sortable(hi.x),
+                                       sortable(hi.y), sortable(hi.z), sortable(c.x),  sortable(c.y),
+                                       sortable(c.z),  sortable(c.x),  sortable(c.y),  sortable(c.z)};
+        uint32_t* dst = uniform ? lds : segStats + static_cast<size_t>(kSegWords) * s;
+        for (int k = 0; k < kSegWords; ++k) {
+            if ((k % 6) < 3) {
+                atomicMin(&dst[k], v[k]);
+            } else {
+                atomicMax(&dst[k], v[k]);
+            }
+        }
+    }
+    __syncthreads();
+    if (uniform && threadIdx.x < kSegWords) {
+        const int k = static_cast<int>(threadIdx.x);
+        uint32_t* g = segStats + static_cast<size_t>(kSegWords) * s0 + k;
+        if ((k % 6) < 3) {
+            atomicMin(g, lds[k]);
+        } else {
+            atomicMax(g, lds[k]);
+        }
+    }
+}
+
+__device__ __forceinline__ int ceil_log2(int c)
+{
+    int d = 0;
+    while ((1 << d) < c) {
+        ++d;
+    }
+    return d;
+}
+
+// One thread per segment: box into the parent (or the root), leaves, bin setup.
+__global__ void seg_setup_kernel(const SegRec* __restrict__ segs, int segCount, const uint32_t* __restrict__ segStats,
+                                 const int* __restrict__ perm, const float* __restrict__ tris, int level,
+                                 float* __restrict__ nodes, float4* __restrict__ triPos, float* __restrict__ rootBox,
+                                 SplitRec* __restrict__ split, uint32_t* __restrict__ bins)
+{
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= segCount) {
+        return;
+    }
+    const SegRec sg = segs[j];
+    const uint32_t* st = segStats + static_cast<size_t>(kSegWords) * j;
+    float box[6];
+    for (int k = 0; k < 6; ++k) {
+        box[k] = unsortable(st[k]);
+    }
+    float* dstBox = sg.parent < 0 ? rootBox : nodes + 16 * static_cast<size_t>(sg.parent) + 6 * sg.side;
+    for (int k = 0; k < 6; ++k) {
+        dstBox[k] = box[k];
+    }
+    SplitRec r{-1, 0.0f, 0.0f, -1, 0, 0};
+    if (sg.count == 1) {
+        const int slot = sg.begin;
+        const int tri = perm[slot];
+        const float* t = tris + 25 * static_cast<size_t>(tri);
+        triPos[3 * slot + 0] = make_float4(t[0], t[1], t[2], __int_as_float(tri));
+        triPos[3 * slot + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], 0.0f);
+        triPos[3 * slot + 2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.0f);
+        const int ref = ~slot;
+        if (sg.parent < 0) {
+            rootBox[6] = __int_as_float(ref);
+        } else {
+            nodes[16 * static_cast<size_t>(sg.parent) + 12 + sg.side] = __int_as_float(ref);
+        }
+        split[j] = r;
+        return;
+    }
+    r.internal = 1;
+    const float ext[3] = {unsortable(st[9]) - unsortable(st[6]), unsortable(st[10]) - unsortable(st[7]),
+                          unsortable(st[11]) - unsortable(st[8])};
+    int axis = 0;
+    if (ext[1] > ext[axis]) {
+        axis = 1;
+    }
+    if (ext[2] > ext[axis]) {
+        axis = 2;
+    }
+    const bool depthCap = level + ceil_log2(sg.count) >= kMaxSahDepth;
+    if (ext[axis] > 0.0f && !depthCap) {
+        r.axis = axis;
+        r.k0 = unsortable(st[6 + axis]);
+        r.scale = static_cast<float>(kBins) / ext[axis];
+        uint32_t* b = bins + static_cast<size_t>(kBins * kBinWords) * j;
+        for (int k = 0; k < kBins; ++k) {
+            b[k * kBinWords + 0] = 0u;
+            for (int q = 0; q < 3; ++q) {
+                b[k * kBinWords + 1 + q] = 0xFFFFFFFFu;
+                b[k * kBinWords + 4 + q] = 0u;
+            }
+        }
+    }
+    split[j] = r;
+}
+
+__device__ __forceinline__ int bin_of(const SplitRec& r, const float4& c)
+{
+    const float v = r.axis == 0 ? c.x : (r.axis == 1 ? c.y : c.z);
+    const int b = static_cast<int>((v - r.k0) * r.scale);
+    return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
+}
+
+__global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
+                                                 const float4* __restrict__ bounds, const float4* __restrict__ cent,
+                                                 const SplitRec* __restrict__ split, int n, uint32_t* __restrict__ bins)
+{
+    __shared__ uint32_t lds[kBins * kBinWords];
+    const int base = static_cast<int>(blockIdx.x * blockDim.x);
+    const int i = base + static_cast<int>(threadIdx.x);
+    const int last = min(base + kB, n) - 1;
+    const int s0 = posSeg[base];
+    const bool uniform = s0 >= 0 && posSeg[last] == s0 && split[s0].axis >= 0;
+    if (uniform) {
+        for (int k = static_cast<int>(threadIdx.x); k < kBins * kBinWords; k += kB) {
+            const int w = k % kBinWords;
+            lds[k] = w == 0 ? 0u : (w < 4 ? 0xFFFFFFFFu : 0u);
+        }
+    }
+    __syncthreads();
+    const int s = i < n ? posSeg[i] : -1;
+    if (s >= 0) {
+        const SplitRec r = split[s];
+        if (r.axis >= 0) {
+            const int tri = perm[i];
+            const int b = bin_of(r, cent[tri]);
+            const float4 lo = bounds[2 * tri];
+            const float4 hi = bounds[2 * tri + 1];
+            uint32_t* dst = uniform ? lds + b * kBinWords
+                                    : bins + static_cast<size_t>(kBins * kBinWords) * s + b * kBinWords;
+            atomicAdd(&dst[0], 1u);
+            atomicMin(&dst[1], sortable(lo.x));
+            atomicMin(&dst[2], sortable(lo.y));
+            atomicMin(&dst[3], sortable(lo.z));
+            atomicMax(&dst[4], sortable(hi.x));
+            atomicMax(&dst[5], sortable(hi.y));
+            atomicMax(&dst[6], sortable(hi.z));
+        }
+    }
+    __syncthreads();
+    if (uniform) {
+        uint32_t* g = bins + static_cast<size_t>(kBins * kBinWords) * s0;
+        for (int k = static_cast<int>(threadIdx.x); k < kBins * kBinWords; k += kB) {
+            const int w = k % kBinWords;
+            if (w == 0) {
+                if (lds[k] != 0u) {
+                    atomicAdd(&g[k], lds[k]);
+                }
+            } else if (w < 4) {
+                atomicMin(&g[k], lds[k]);
+            } else {
+                atomicMax(&g[k], lds[k]);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float half_area(float x0, float y0, float z0, float x1, float y1, float z1)
+{
+    const float dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// One thread per internal segment: lowest-cost bin boundary (left-to-right sweep of prefix areas,
+// right-to-left of suffix areas); the middle position when no boundary separates the segment.
+__global__ void split_kernel(const SegRec* __restrict__ segs, int segCount, const uint32_t* __restrict__ bins,
+                             SplitRec* __restrict__ split, int* __restrict__ internalFlag)
+{
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= segCount) {
+        return;
+    }
+    SplitRec r = split[j];
+    internalFlag[j] = r.internal;
+    if (r.internal == 0) {
+        return;
+    }
+    const int count = segs[j].count;
+    if (r.axis >= 0) {
+        const uint32_t* b = bins + static_cast<size_t>(kBins * kBinWords) * j;
+        float suffix[kBins];
+        int suffixN[kBins];
+        float x0 = INFINITY, y0 = INFINITY, z0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, z1 = -INFINITY;
+        int acc = 0;
+        for (int k = kBins - 1; k >= 0; --k) {
+            const uint32_t* e = b + k * kBinWords;
+            if (e[0] != 0u) {
+                x0 = fminf(x0, unsortable(e[1]));
+                y0 = fminf(y0, unsortable(e[2]));
+                z0 = fminf(z0, unsortable(e[3]));
+                x1 = fmaxf(x1, unsortable(e[4]));
+                y1 = fmaxf(y1, unsortable(e[5]));
+                z1 = fmaxf(z1, unsortable(e[6]));
+                acc += static_cast<int>(e[0]);
+            }
+            suffixN[k] = acc;
+            suffix[k] = acc > 0 ? half_area(x0, y0, z0, x1, y1, z1) : 0.0f;
+        }
+        x0 = y0 = z0 = INFINITY;
+        x1 = y1 = z1 = -INFINITY;
+        acc = 0;
+        float best = INFINITY;
+        int bestBin = -1, bestLeft = 0;
+        for (int k = 0; k < kBins - 1; ++k) {
+            const uint32_t* e = b + k * kBinWords;
+            if (e[0] != 0u) {
+                x0 = fminf(x0, unsortable(e[1]));
+                y0 = fminf(y0, unsortable(e[2]));
+                z0 = fminf(z0, unsortable(e[3]));
+                x1 = fmaxf(x1, unsortable(e[4]));
+                y1 = fmaxf(y1, unsortable(e[5]));
+                z1 = fmaxf(z1, unsortable(e[6]));
+                acc += static_cast<int>(e[0]);
+            }
+            const int right = suffixN[k + 1];
+            if (acc == 0 || right == 0) {
+                continue;
+            }
+            const float cost = half_area(x0, y0, z0, x1, y1, z1) * static_cast<float>(acc) +
+                               suffix[k + 1] * static_cast<float>(right);
+            if (cost < best) {
+                best = cost;
+                bestBin = k;
+                bestLeft = acc;
+            }
+        }
+        if (bestBin >= 0) {
+            r.splitBin = bestBin;
+            r.leftCount = bestLeft;
+        } else {
+            r.axis = -1;
+        }
+    }
+    if (r.axis < 0) {
+        r.splitBin = -1;
+        r.leftCount = count / 2;
+    }
+    split[j] = r;
+}
+
+// Internal nodes of this level get consecutive indices; each emits its two child segments.
+__global__ void emit_kernel(const SegRec* __restrict__ segs, int segCount, const SplitRec* __restrict__ split,
+                            const int* __restrict__ rank, int levelBase, float* __restrict__ nodes,
+                            float* __restrict__ rootBox, SegRec* __restrict__ next)
+{
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= segCount || split[j].internal == 0) {
+        return;
+    }
+    const SegRec sg = segs[j];
+    const int idx = levelBase + rank[j];
+    if (sg.parent < 0) {
+        rootBox[6] = __int_as_float(idx);
+    } else {
+        nodes[16 * static_cast<size_t>(sg.parent) + 12 + sg.side] = __int_as_float(idx);
+    }
+    const int lc = split[j].leftCount;
+    next[2 * rank[j]] = SegRec{sg.begin, lc, idx, 0};
+    next[2 * rank[j] + 1] = SegRec{sg.begin + lc, sg.count - lc, idx, 1};
+}
+
+__global__ void flag_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg, const SegRec* __restrict__ segs,
+                            const SplitRec* __restrict__ split, const float4* __restrict__ cent, int n,
+                            int* __restrict__ flags)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const int s = posSeg[i];
+    int f = 0;
+    if (s >= 0) {
+        const SplitRec r = split[s];
+        if (r.internal != 0) {
+            f = r.axis >= 0 ? (bin_of(r, cent[perm[i]]) <= r.splitBin ? 1 : 0)
+                            : (i - segs[s].begin < r.leftCount ? 1 : 0);
+        }
+    }
+    flags[i] = f;
+}
+
+__global__ void sah_scatter_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
+                                   const SegRec* __restrict__ segs, const SplitRec* __restrict__ split,
+                                   const int* __restrict__ rank, const int* __restrict__ flags,
+                                   const int* __restrict__ scan, int n, int* __restrict__ permOut,
+                                   int* __restrict__ posSegOut)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) {
+        return;
+    }
+    const int s = posSeg[i];
+    if (s < 0 || split[s].internal == 0) {
+        permOut[i] = perm[i];
+        posSegOut[i] = -1;
+        return;
+    }
+    const int begin = segs[s].begin;
+    const int lc = split[s].leftCount;
+    const int before = scan[i] - scan[begin];
+    const bool left = flags[i] != 0;
+    const int pos = left ? begin + before : begin + lc + (i - begin - before);
+    permOut[pos] = perm[i];
+    posSegOut[pos] = 2 * rank[s] + (left ? 0 : 1);
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) {
+        p[i] = v;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, int n, uint32_t v0, uint32_t v1, int split)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) {
+        p[i] = i < split ? v0 : v1;
+    }
+}
+
+unsigned grid(long long n) { return static_cast<unsigned>((n + kB - 1) / kB); }
+
+template <typename T>
+bool dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p != nullptr) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+}
+
+} // namespace
+
+bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
+                      float rootOut[7], int* levelsOut)
+{
+    float4* bounds = nullptr;
+    float4* cent = nullptr;
+    uint32_t* gcb = nullptr;
+    uint32_t* keys = nullptr;
+    uint32_t* keysOut = nullptr;
+    int* vals = nullptr;
+    int* perm = nullptr;
+    int* permAlt = nullptr;
+    int* posSeg = nullptr;
+    int* posSegAlt = nullptr;
+    SegRec* segs = nullptr;
+    SegRec* segsNext = nullptr;
+    SplitRec* split = nullptr;
+    uint32_t* segStats = nullptr;
+    uint32_t* bins = nullptr;
+    int* internalFlag = nullptr;
+    int* rank = nullptr;
+    int* flags = nullptr;
+    int* scan = nullptr;
+    float* rootBox = nullptr;
+    void* tmp = nullptr;
+    bool ok = false;
+    const size_t maxSegs = static_cast<size_t>(std::max(1, n)); // segments of a level <= triangles
+    do {
+        if (!dalloc(&bounds, 2 * static_cast<size_t>(n)) || !dalloc(&cent, n) || !dalloc(&gcb, 6) ||
+            !dalloc(&keys, n) || !dalloc(&keysOut, n) || !dalloc(&vals, n) || !dalloc(&perm, n) ||
+            !dalloc(&permAlt, n) || !dalloc(&posSeg, n) || !dalloc(&posSegAlt, n) || !dalloc(&segs, maxSegs) ||
+            !dalloc(&segsNext, maxSegs) || !dalloc(&split, maxSegs) ||
+            !dalloc(&segStats, static_cast<size_t>(kSegWords) * maxSegs) ||
+            !dalloc(&bins, static_cast<size_t>(kBins * kBinWords) * maxSegs) || !dalloc(&internalFlag, maxSegs) ||
+            !dalloc(&rank, maxSegs) || !dalloc(&flags, n) || !dalloc(&scan, n) || !dalloc(&rootBox, 8)) {
+            ptSetError("SAH build: device allocation failed");
+            break;
+        }
+        size_t sortBytes = 0, scanBytes = 0, segScanBytes = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keysOut, vals, perm, n, 0, 30, stream);
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, flags, scan, n, stream);
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, segScanBytes, internalFlag, rank, static_cast<int>(maxSegs),
+                                               stream);
+        const size_t tmpBytes = std::max(sortBytes, std::max(scanBytes, segScanBytes));
+        if (hipMalloc(&tmp, tmpBytes) != hipSuccess) {
+            ptSetError("SAH build: temporary allocation failed");
+            break;
+        }
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kB), 0, stream, gcb, 6, 0xFFFFFFFFu, 0u, 3);
+        hipLaunchKernelGGL(sah_prep_kernel, dim3(grid(n)), dim3(kB), 0, stream, dTris, n, bounds, cent, triShade, gcb);
+        hipLaunchKernelGGL(morton_kernel, dim3(grid(n)), dim3(kB), 0, stream, cent, n, gcb, keys, vals);
+        size_t bytes = tmpBytes;
+        if (hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keysOut, vals, perm, n, 0, 30, stream) != hipSuccess) {
+            ptSetError("SAH build: radix sort failed");
+            break;
+        }
+        hipLaunchKernelGGL(fill_int_kernel, dim3(grid(n)), dim3(kB), 0, stream, posSeg, n, 0);
+        const SegRec root{0, n, -1, 0};
+        if (hipMemcpyAsync(segs, &root, sizeof(root), hipMemcpyHostToDevice, stream) != hipSuccess) {
+            ptSetError("SAH build: upload failed");
+            break;
+        }
+        int segCount = 1;
+        int levelBase = 0;
+        int level = 0;
+        bool levelOk = true;
+        while (segCount > 0 && levelOk) {
+            hipLaunchKernelGGL(seg_reset_kernel, dim3(grid(static_cast<long long>(segCount) * kSegWords)), dim3(kB), 0,
+                               stream, segStats, segCount);
+            hipLaunchKernelGGL(seg_bounds_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, bounds, cent, n,
+                               segStats);
+            hipLaunchKernelGGL(seg_setup_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, segStats,
+                               perm, dTris, level, reinterpret_cast<float*>(nodes), triPos, rootBox, split, bins);
+            hipLaunchKernelGGL(bin_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, bounds, cent, split, n,
+                               bins);
+            hipLaunchKernelGGL(split_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, bins, split,
+                               internalFlag);
+            bytes = tmpBytes;
+            levelOk = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, internalFlag, rank, segCount, stream) == hipSuccess;
+            int tail[2] = {0, 0};
+            levelOk = levelOk &&
+                      hipMemcpyAsync(&tail[0], rank + segCount - 1, sizeof(int), hipMemcpyDeviceToHost, stream) ==
+                          hipSuccess &&
+                      hipMemcpyAsync(&tail[1], internalFlag + segCount - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                     stream) == hipSuccess &&
+                      hipStreamSynchronize(stream) == hipSuccess;
+            if (!levelOk) {
+                break;
+            }
+            const int internals = tail[0] + tail[1];
+            if (internals == 0) {
+                break;
+            }
+            if (2 * static_cast<size_t>(internals) > maxSegs) {
+                ptSetError("SAH build: segment table overflow");
+                levelOk = false;
+                break;
+            }
+            hipLaunchKernelGGL(emit_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, split, rank,
+                               levelBase, reinterpret_cast<float*>(nodes), rootBox, segsNext);
+            hipLaunchKernelGGL(flag_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, split, cent, n,
+                               flags);
+            bytes = tmpBytes;
+            levelOk = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flags, scan, n, stream) == hipSuccess;
+            hipLaunchKernelGGL(sah_scatter_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, split, rank,
+                               flags, scan, n, permAlt, posSegAlt);
+            std::swap(perm, permAlt);
+            std::swap(posSeg, posSegAlt);
+            std::swap(segs, segsNext);
+            levelBase += internals;
+            segCount = 2 * internals;
+            ++level;
+        }
+        if (!levelOk || hipGetLastError() != hipSuccess) {
+            if (levelOk) {
+                ptSetError("SAH build: kernel launch failed");
+            }
+            break;
+        }
+        if (levelBase != n - 1) {
+            ptSetError("SAH build: %d internal nodes for %d triangles", levelBase, n);
+            break;
+        }
+        if (hipMemcpyAsync(rootOut, rootBox, 7 * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            ptSetError("SAH build: synchronisation failed");
+            break;
+        }
+        *levelsOut = level + 1;
+        ok = true;
+    } while (false);
+    dfree(bounds);
+    dfree(cent);
+    dfree(gcb);
+    dfree(keys);
+    dfree(keysOut);
+    dfree(vals);
+    dfree(perm);
+    dfree(permAlt);
+    dfree(posSeg);
+    dfree(posSegAlt);
+    dfree(segs);
+    dfree(segsNext);
+    dfree(split);
+    dfree(segStats);
+    dfree(bins);
+    dfree(internalFlag);
+    dfree(rank);
+    dfree(flags);
+    dfree(scan);
+    dfree(rootBox);
+    if (tmp != nullptr) {
+        (void)hipFree(tmp);
+    }
+    return ok;
+}

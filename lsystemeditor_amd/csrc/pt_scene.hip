@@ -409,6 +409,8 @@ struct PtScene {
     MeshBvhNode* refNodes = nullptr;
     uint32_t refCount = 0;
     float buildMs = 0.0f;
+    int buildMode = 0; // 0: median split (reference topology), 1: binned SAH
+    int levels = 0;
 
     void clear()
     {
@@ -647,10 +649,28 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
              alloc(&sc->triShade, 4 * static_cast<size_t>(n)) && alloc(&sc->triPos, 3 * static_cast<size_t>(n)) &&
              alloc(&sc->nodes, 4 * static_cast<size_t>(n > 1 ? n - 1 : 1)) &&
              alloc(&sc->refNodes, 2 * static_cast<size_t>(n) - 1);
-        if (ok) {
+        if (ok && sc->buildMode == 1) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            (void)hipEventRecord(e0, stream);
+            float root[7] = {0};
+            ok = pt_build_bvh_sah(dTris, n, sc->nodes, sc->triPos, sc->triShade, stream, root, &sc->levels);
+            (void)hipEventRecord(e1, stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&sc->buildMs, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            for (int k = 0; k < 3; ++k) {
+                sc->desc.rootMin[k] = root[k];
+                sc->desc.rootMax[k] = root[3 + k];
+            }
+            std::memcpy(&sc->desc.rootRef, &root[6], sizeof(int));
+            sc->refCount = 0u; // the reference's 48-byte pre-order layout describes median-split trees only
+        } else if (ok) {
             ok = build_bvh(sc, dTris, n, stream);
+            sc->refCount = ok ? static_cast<uint32_t>(2 * n - 1) : 0u;
         }
-        sc->refCount = ok ? static_cast<uint32_t>(2 * n - 1) : 0u;
     }
     release(dTris);
     ok = ok && upload(&sc->materials, mats.data(), mats.size(), stream) &&
@@ -686,6 +706,36 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
 }
 
 const MeshAccelSceneGpu* ptSceneDevice(const PtScene* scene) { return scene != nullptr ? scene->d_desc : nullptr; }
+
+bool ptSceneSetBuildMode(PtScene* scene, int mode)
+{
+    if (scene == nullptr || mode < 0 || mode > 1) {
+        ptSetError("ptSceneSetBuildMode: invalid arguments");
+        return false;
+    }
+    scene->buildMode = mode;
+    return true;
+}
+
+bool ptSceneReadLayout(const PtScene* scene, float* nodes, float* triPos, int* levels)
+{
+    if (scene == nullptr || scene->d_desc == nullptr) {
+        ptSetError("ptSceneReadLayout: no scene");
+        return false;
+    }
+    const size_t nn = scene->desc.nodeCount, nt = scene->desc.triangleCount;
+    if (levels != nullptr) {
+        *levels = scene->levels;
+    }
+    if ((nodes != nullptr && nn > 0 &&
+         hipMemcpy(nodes, scene->nodes, nn * 64, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (triPos != nullptr && nt > 0 &&
+         hipMemcpy(triPos, scene->triPos, nt * 48, hipMemcpyDeviceToHost) != hipSuccess)) {
+        ptSetError("ptSceneReadLayout: copy failed");
+        return false;
+    }
+    return true;
+}
 
 bool ptSceneDescription(const PtScene* scene, MeshAccelSceneGpu* out)
 {

@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kWfBlock = 256;
 constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
-constexpr int kSpillEntries = 32;  // global spill entries per thread (tree depth <= 28)
+constexpr int kSpillEntries = 64;  // stack entries per thread in LDS ring + global spill (tree depth <= 48)
 constexpr int kCountersPerBounce = 8;
 constexpr int kRefillLanes = 32;   // idle lanes that trigger a wave's ray refill
 
