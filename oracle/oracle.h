@@ -237,6 +237,22 @@ int orc_render_adaptive(float* acc4, uint32_t* counts, float* lumMean, float* m2
 void orc_copy_to_pbo(const float* acc4, const uint32_t* counts, const uint8_t* converged, uint8_t* rgba,
                      int outW, int outH, int srcW, int srcH, int downscale, int useAcc, const OParams* params,
                      float exposure);
+/* ---- differentiable rendering (detached sampling, adjoint path replay) ----- */
+/* Radiance of one path whose sampling uses the scene's materials and whose values use evalMats
+ * (NULL: the scene's, giving orc_trace_path's result). With gw != NULL the path is replayed as
+ * the adjoint: C is its primal radiance, gw = dLoss/dRadiance (3 doubles), and
+ * grad[7 * material + (r, g, b, roughness, metallic, specular, emission)] accumulates. */
+OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* scene, const OMaterial* evalMats, const OParams* params,
+                        const OEnv* env, ORng* rng, const double* gw, OVec3 C, double* grad);
+/* Contracted derivative of the opaque Principled BSDF value: out6[k] += sum_c a3[c] df_c/dtheta_k,
+ * theta = (r, g, b, roughness, metallic, specular); raw = the unresolved material (live mask). */
+void orc_brdf_grad(const OBrdfCtx* ctx, OVec3 wi, const double* a3, const OMaterial* raw, double* out6);
+void orc_clamp_vjp(OVec3 C, const double* g3, double* out3);
+void orc_render_detached(float* img3, int width, int height, const OCamera* cam, const OScene* scene,
+                         const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp, int threads);
+void orc_render_adjoint(const double* gradImg3, int width, int height, const OCamera* cam, const OScene* scene,
+                        const OEnv* env, const OParams* params, int spp, double* grad, int threads);
+
 /* UV debug view: pinhole primary ray, fractional hit UV as red/green. */
 /* Region reset (acc w = 1) listing the region row-major as the active set; returns its size. */
 int orc_clear_region(float* acc4, uint32_t* counts, float* mean, float* m2, uint8_t* converged, int* active,

@@ -147,6 +147,14 @@ def lib():
                                               C.POINTER(Env), vp, C.c_int, C.c_int, C.c_int]),
             "orc_copy_to_pbo": (None, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, f]),
             "orc_uv_debug": (None, [vp, C.POINTER(Scene), vp, C.c_int, C.c_int, vp]),
+            "orc_trace_path_ad": (Vec3, [Vec3, Vec3, C.POINTER(Scene), vp, vp, C.POINTER(Env), vp, vp, Vec3, vp]),
+            "orc_trace_path": (Vec3, [Vec3, Vec3, C.POINTER(Scene), vp, C.POINTER(Env), vp]),
+            "orc_brdf_grad": (None, [vp, Vec3, vp, vp, vp]),
+            "orc_clamp_vjp": (None, [Vec3, vp, vp]),
+            "orc_render_detached": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, C.POINTER(Env), vp, C.c_int,
+                                           C.c_int]),
+            "orc_render_adjoint": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), C.POINTER(Env), vp, C.c_int, vp,
+                                          C.c_int]),
             "orc_clear_region": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "orc_rebuild_active": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
         }

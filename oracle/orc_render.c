@@ -688,3 +688,536 @@ int orc_rebuild_active(const uint8_t* conv, const uint32_t* counts, int* out, in
     }
     return n;
 }
+
+/* ---- differentiable rendering: detached estimator and adjoint path replay ------------------
+ *
+ * Parameters per material k: grad[7k + 0..6] = base r, g, b, roughness, metallic, specular,
+ * emission. Derivatives use "detached" sampling: directions, pdfs, MIS weights and Russian
+ * roulette are those of the primal parameters, only the evaluated values (emission, BSDF values
+ * in throughputs and next-event terms) depend on the parameters. This is an unbiased estimator
+ * of the derivative of the expected image (sampling with any fixed density is unbiased), and it
+ * is exactly the derivative of orc_trace_path_ad's radiance with respect to evalMats at
+ * evalMats = the scene's materials -- which is what the finite-difference tests use.
+ * BSDF derivatives cover the opaque Principled lobes (diffuse + specular); a material with
+ * transmission or subsurface weight contributes emission derivatives only. Parameters are live
+ * where the forward is differentiable: untextured, inside the open clamp interval (0, 1) for
+ * roughness/metallic/specular, emission > 0. */
+
+typedef struct {
+    int rgb, rough, metal, spec, emis, opaque;
+    int onRough; /* Oren-Nayar sigma follows roughness (unresolved diffuseRoughness < 0) */
+} LiveMask;
+
+static int untextured(uint32_t ti, const OScene* s)
+{
+    return ti == 0u || s == NULL || s->textures == NULL || ti >= s->textureCount;
+}
+
+static LiveMask live_mask(const OMaterial* raw, const OMaterial* resolved, const OScene* s)
+{
+    LiveMask l;
+    l.rgb = untextured(raw->albedoTex, s);
+    l.rough = untextured(raw->roughnessTex, s) && raw->roughness > 0.0f && raw->roughness < 1.0f;
+    l.metal = untextured(raw->metallicTex, s) && raw->metallic > 0.0f && raw->metallic < 1.0f;
+    l.spec = raw->specular > 0.0f && raw->specular < 1.0f;
+    l.emis = untextured(raw->emissionTex, s) && raw->emission > 0.0f;
+    l.opaque = clamp01f(resolved->transmission) == 0.0f && clamp01f(resolved->subsurface) == 0.0f;
+    l.onRough = raw->diffuseRoughness < 0.0f;
+    return l;
+}
+
+/* out[k] += sum_c a_c * d f_c / d theta_k, theta = (r, g, b, roughness, metallic, specular), for
+ * the opaque Principled BSDF f = base wd ON / pi + F ws ec D G / max(4 cosO cosI, 1e-8) with
+ * wd = (1 - m) / (2 - m), ws = 1 / (2 - m) (normalised lobe weights at transmission =
+ * subsurface = 0). The geometry is computed as the forward does, the algebra in double. */
+static void brdf_grad(const OBrdfCtx* c, OVec3 wi, const double a[3], LiveMask lm, double out[6])
+{
+    if (!lm.opaque) {
+        return;
+    }
+    const OMaterial* m = &c->material;
+    const float cosWi = fmax2(0.0f, vdot(c->normal, wi));
+    const float cosWo = fmax2(0.0f, vdot(c->normal, c->wo));
+    if (cosWi <= 0.0f || cosWo <= 0.0f) {
+        return;
+    }
+    const double pi = (double)3.14159265f;
+    const double base[3] = {m->r, m->g, m->b};
+    const double mm = clamp01f(m->metallic);
+    const double wd = (1.0 - mm) / (2.0 - mm);
+    const double ws = 1.0 / (2.0 - mm);
+    /* diffuse lobe */
+    const float dr = fmax2(0.0f, fmin2(1.0f, m->diffuseRoughness < 0.0f ? m->roughness : m->diffuseRoughness));
+    const float sigma = dr * 0.5f * 3.14159265f;
+    const double on = orc_oren_nayar_factor(sigma, cosWi, cosWo, vdot(wi, c->wo));
+    for (int k = 0; k < 3; ++k) {
+        if (lm.rgb) {
+            out[k] += a[k] * wd * on / pi;
+        }
+    }
+    const double aBase = a[0] * base[0] + a[1] * base[1] + a[2] * base[2];
+    if (lm.metal) {
+        out[4] += aBase * on / pi * (-1.0 / ((2.0 - mm) * (2.0 - mm)));
+    }
+    if (lm.rough && lm.onRough && sigma > 1.0e-4f) {
+        /* ON = A(s2) + B(s2) * geo with geo = max(0, cosPhi) sinAlpha tanBeta (independent of sigma) */
+        const double s2 = (double)sigma * sigma;
+        const float sinI = sqrtf(fmax2(0.0f, 1.0f - cosWi * cosWi));
+        const float sinO = sqrtf(fmax2(0.0f, 1.0f - cosWo * cosWo));
+        float sa = cosWi > cosWo ? sinI : sinO;
+        float sb = cosWi > cosWo ? sinO : sinI;
+        float cosPhi = 1.0f;
+        const float d = vdot(wi, c->wo) - cosWi * cosWo;
+        if (d <= 0.0f) {
+            cosPhi = fmax2(-1.0f, fmin2(1.0f, d / fmax2(sa * sb, 1.0e-8f)));
+        } else {
+            sa = 0.0f;
+            sb = 0.0f;
+        }
+        const double geo = (double)fmax2(0.0f, cosPhi) * sa * (sb / fmax2(cosWi, cosWo));
+        const double dA = -0.5 * 0.33 / ((s2 + 0.33) * (s2 + 0.33));
+        const double dB = 0.45 * 0.09 / ((s2 + 0.09) * (s2 + 0.09));
+        const double dOn = (dA + dB * geo) * 2.0 * sigma * 0.5 * pi;
+        out[3] += aBase * wd * dOn / pi;
+    }
+    /* specular lobe */
+    const OVec3 h = vnorm(vadd(wi, c->wo));
+    const float cosH = fmax2(0.0f, vdot(c->normal, h));
+    const float cosHo = fmax2(0.0f, vdot(h, c->wo));
+    if (cosHo <= 0.0f) {
+        return;
+    }
+    const float rough = m->roughness;
+    const double rc = fmax2(rough, 0.04f);
+    const double alpha = rc * rc;
+    const double a2 = alpha * alpha;
+    const double c2 = (double)cosH * cosH;
+    const double dd = c2 * (a2 - 1.0) + 1.0;
+    const double D = a2 / (pi * dd * dd);
+    double g1[2], dg1[2];
+    const float cs[2] = {cosWo, cosWi};
+    for (int q = 0; q < 2; ++q) {
+        const double cc = fmax((double)cs[q] * cs[q], 1.0e-8);
+        const double tan2 = (1.0 - cc) / cc;
+        const double sq = sqrt(1.0 + a2 * tan2);
+        g1[q] = 2.0 / (1.0 + sq);
+        dg1[q] = -tan2 / ((1.0 + sq) * (1.0 + sq) * sq); /* d G1 / d a2 */
+    }
+    const double G = g1[0] * g1[1];
+    const double ior = fmax2(m->ior, 1.0e-3f);
+    const double f0s = (ior - 1.0) / (ior + 1.0);
+    const double specv = clamp01f(clamp01f(m->specular));
+    const double f0v = f0s * f0s * specv;
+    const double t = fmax(0.0, fmin(1.0, 1.0 - (double)cosHo));
+    const double t5 = t * t * t * t * t;
+    double F[3];
+    for (int k = 0; k < 3; ++k) {
+        F[k] = (f0v + (base[k] - f0v) * mm) * (1.0 - t5) + t5;
+    }
+    const double cw = fmax(cosWo, 1.0e-4);
+    const double ess = 1.0 - alpha * (0.3316 + 0.3237 * alpha);
+    const double ems = (1.0 - ess) * (1.0 - cw);
+    const double essC = fmax(ess, 1.0e-4);
+    const double ec = 1.0 + ems / essC;
+    const double denom = fmax(4.0 * cosWo * cosWi, 1.0e-8);
+    const double S = ws * ec * D * G / denom;
+    const double aF = a[0] * F[0] + a[1] * F[1] + a[2] * F[2];
+    for (int k = 0; k < 3; ++k) {
+        if (lm.rgb) {
+            out[k] += a[k] * mm * (1.0 - t5) * S;
+        }
+    }
+    if (lm.metal) {
+        double dF = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            dF += a[k] * (base[k] - f0v) * (1.0 - t5);
+        }
+        out[4] += dF * S + aF * S / (2.0 - mm);
+    }
+    if (lm.spec) {
+        out[5] += (a[0] + a[1] + a[2]) * (1.0 - mm) * (1.0 - t5) * f0s * f0s * S;
+    }
+    if (lm.rough && rough > 0.04f) {
+        const double dD = (dd - 2.0 * a2 * c2) / (pi * dd * dd * dd) * 2.0 * alpha;   /* dD/dalpha */
+        const double dG = (dg1[0] * g1[1] + g1[0] * dg1[1]) * 2.0 * alpha;          /* dG/dalpha */
+        const double dEss = -(0.3316 + 2.0 * 0.3237 * alpha);
+        const double dEms = -dEss * (1.0 - cw);
+        const double dEc = ess > 1.0e-4 ? (dEms * ess - ems * dEss) / (ess * ess) : dEms / 1.0e-4;
+        const double dS = ws / denom * (dEc * D * G + ec * dD * G + ec * D * dG);
+        out[3] += aF * dS * 2.0 * rough;
+    }
+}
+
+void orc_brdf_grad(const OBrdfCtx* c, OVec3 wi, const double* a3, const OMaterial* raw, double* out6)
+{
+    OMaterial none = *raw;
+    none.albedoTex = none.roughnessTex = none.metallicTex = none.emissionTex = 0u;
+    const LiveMask lm = live_mask(&none, &c->material, NULL);
+    brdf_grad(c, wi, a3, lm, out6);
+}
+
+/* Next-event terms of the sampling material (pdfs, MIS) as f-weights: contribution = f(wi) * w. */
+static int env_nee_terms(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta, const OScene* s, const OEnv* env,
+                         const OParams* params, ORng* rng, OVec3* wiOut, OVec3* wOut)
+{
+    if (orc_skips_env_nee(m)) {
+        return 0;
+    }
+    const float u1 = orc_rand01(rng);
+    const float u2 = orc_rand01(rng);
+    float lightPdf = 0.0f;
+    const OVec3 wi = orc_env_sample(env, params, u1, u2, &lightPdf);
+    if (lightPdf <= K_MIN_PDF || occluded(p, n, wi, s)) {
+        return 0;
+    }
+    const float cosT = fmax2(0.0f, vdot(n, wi));
+    if (cosT <= 0.0f) {
+        return 0;
+    }
+    const OBrdfCtx ctx = make_ctx(n, wo, m, eta, 0);
+    const OVec3 L = orc_env_eval(env, params, wi);
+    const float bpdf = orc_brdf_pdf(&ctx, wi);
+    const float mis = orc_mis_balance(lightPdf, bpdf);
+    *wiOut = wi;
+    *wOut = vscale(L, mis * cosT / lightPdf);
+    return 1;
+}
+
+static int emissive_nee_terms(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta, const OScene* s, ORng* rng,
+                              OVec3* wiOut, OVec3* wOut)
+{
+    if (s == NULL || s->emissiveCount == 0) {
+        return 0;
+    }
+    const float uTri = orc_rand01(rng);
+    const float u1 = orc_rand01(rng);
+    const float u2 = orc_rand01(rng);
+    OVec3 lp, ln, lr;
+    float areaPdf = 0.0f;
+    if (!orc_light_sample_emissive(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf)) {
+        return 0;
+    }
+    const OVec3 toL = vsub(lp, p);
+    const float d2 = vdot(toL, toL);
+    if (d2 <= K_MIN_PDF) {
+        return 0;
+    }
+    const OVec3 wi = vnorm(toL);
+    const float cosT = fmax2(0.0f, vdot(n, wi));
+    const float cosL = fmax2(0.0f, vdot(ln, vscale(wi, -1.0f)));
+    if (cosT <= 0.0f || cosL <= 0.0f || occluded_from(p, wi, s)) {
+        return 0;
+    }
+    const float lightPdf = areaPdf * d2 / cosL;
+    if (lightPdf <= K_MIN_PDF) {
+        return 0;
+    }
+    const OBrdfCtx ctx = make_ctx(n, wo, m, eta, 0);
+    const float bpdf = orc_brdf_pdf(&ctx, wi);
+    const float mis = orc_mis_balance(lightPdf, bpdf);
+    *wiOut = wi;
+    *wOut = vscale(lr, mis * cosT / lightPdf);
+    return 1;
+}
+
+/* Fetches material `tri`'s record from an alternative table (same index rules as
+ * orc_fetch_material). */
+static OMaterial fetch_from(const OScene* s, const OMaterial* table, uint32_t tri, int* index)
+{
+    *index = -1;
+    if (table != NULL && s != NULL && s->triangles != NULL && tri < s->triangleCount &&
+        s->triangles[tri].materialIndex < s->materialCount) {
+        *index = (int)s->triangles[tri].materialIndex;
+        return table[*index];
+    }
+    return orc_fetch_material(s, tri);
+}
+
+OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* evalMats, const OParams* params,
+                        const OEnv* env, ORng* rng, const double* gw, OVec3 C, double* grad)
+{
+    const OHit first = trace(ro, rd, s, 0.0f, K_RAY_TMAX, 1);
+    if (!first.hit) {
+        return orc_env_eval(env, params, rd);
+    }
+    int maxDepth = 32, rrMin = 3, flags = 0;
+    if (params != NULL) {
+        maxDepth = params->maxPathDepth > 0 ? params->maxPathDepth : (params->maxPathDepth == 0 ? 256 : 32);
+        rrMin = params->russianRouletteMinDepth >= 0 ? params->russianRouletteMinDepth : 3;
+        flags = params->brdfDebugFlags;
+    }
+    const OMaterial* evalTable = evalMats != NULL ? evalMats : (s != NULL ? s->materials : NULL);
+    OVec3 L = v3(0.0f, 0.0f, 0.0f);
+    OVec3 T = v3(1.0f, 1.0f, 1.0f);  /* evaluated throughput */
+    OVec3 T0 = T;                    /* primal throughput: Russian roulette and termination */
+    OVec3 origin = ro, dir = rd;
+    OHit hit = first;
+    float eta = 1.0f;
+    for (int depth = 0; depth < maxDepth; ++depth) {
+        const OVec3 p = vray(origin, dir, hit.t);
+        const OVec3 n = hit.normal;
+        const OVec3 wo = vscale(dir, -1.0f);
+        const OMaterial src = orc_fetch_material(s, hit.triangleIndex);
+        const OMaterial m = orc_resolve_material(&src, hit.uv, s);
+        int mi = -1;
+        const OMaterial srcE = fetch_from(s, evalTable, hit.triangleIndex, &mi);
+        const OMaterial mE = orc_resolve_material(&srcE, hit.uv, s);
+        const LiveMask lm = live_mask(&srcE, &mE, s);
+        const OBrdfCtx ctxE = make_ctx(n, wo, &mE, eta, 0);
+        double* g = (grad != NULL && gw != NULL && mi >= 0) ? grad + 7 * mi : NULL;
+
+        L = vadd(L, vmul(T, v3(mE.r * mE.emission, mE.g * mE.emission, mE.b * mE.emission)));
+        if (g != NULL) {
+            const double tb[3] = {(double)T.x * mE.r, (double)T.y * mE.g, (double)T.z * mE.b};
+            if (lm.emis) {
+                g[6] += gw[0] * tb[0] + gw[1] * tb[1] + gw[2] * tb[2];
+            }
+            if (lm.rgb) {
+                g[0] += gw[0] * T.x * mE.emission;
+                g[1] += gw[1] * T.y * mE.emission;
+                g[2] += gw[2] * T.z * mE.emission;
+            }
+        }
+        OVec3 wi, w;
+        if (env_nee_terms(p, n, wo, &m, eta, s, env, params, rng, &wi, &w)) {
+            L = vadd(L, vmul(T, vmul(orc_brdf_eval(&ctxE, wi), w)));
+            if (g != NULL) {
+                const double a[3] = {gw[0] * T.x * w.x, gw[1] * T.y * w.y, gw[2] * T.z * w.z};
+                brdf_grad(&ctxE, wi, a, lm, g);
+            }
+        }
+        if (emissive_nee_terms(p, n, wo, &m, eta, s, rng, &wi, &w)) {
+            L = vadd(L, vmul(T, vmul(orc_brdf_eval(&ctxE, wi), w)));
+            if (g != NULL) {
+                const double a[3] = {gw[0] * T.x * w.x, gw[1] * T.y * w.y, gw[2] * T.z * w.z};
+                brdf_grad(&ctxE, wi, a, lm, g);
+            }
+        }
+        const OBrdfCtx ctx = make_ctx(n, wo, &m, eta, flags);
+        const float u1 = orc_rand01(rng);
+        const float u2 = orc_rand01(rng);
+        const OSample smp = orc_brdf_sample(&ctx, u1, u2);
+        if (!smp.valid || smp.pdf <= K_MIN_PDF) {
+            break;
+        }
+        const OVec3 f0 = orc_brdf_eval(&ctx, smp.direction);
+        const OVec3 fE = orc_brdf_eval(&ctxE, smp.direction);
+        if (g != NULL) {
+            /* radiance still to come after this vertex: linear in fE */
+            const double R[3] = {(double)C.x - L.x, (double)C.y - L.y, (double)C.z - L.z};
+            const double fe[3] = {fE.x, fE.y, fE.z};
+            double a[3];
+            for (int k = 0; k < 3; ++k) {
+                a[k] = fe[k] != 0.0 ? gw[k] * R[k] / fe[k] : 0.0;
+            }
+            brdf_grad(&ctxE, smp.direction, a, lm, g);
+        }
+        const OVec3 T0b = T0;
+        T0 = orc_apply_throughput(T0, &ctx, &smp, f0);
+        if (evalMats == NULL) {
+            T = T0;
+            T0 = subsurface_tier_b(T0, &m, n, &smp, rng);
+            T = T0;
+        } else {
+            T = orc_apply_throughput(T, &ctx, &smp, fE);
+            const OVec3 before = T0;
+            T0 = subsurface_tier_b(T0, &m, n, &smp, rng);
+            const OVec3 k3 = v3(before.x != 0.0f ? T0.x / before.x : 0.0f, before.y != 0.0f ? T0.y / before.y : 0.0f,
+                                before.z != 0.0f ? T0.z / before.z : 0.0f);
+            T = vmul(T, k3);
+        }
+        (void)T0b;
+        if ((flags & (1 << 4)) != 0) {
+            const OVec3 tint = smp.transmitted ? v3(0.2f, 1.0f, 0.2f) : v3(1.0f, 0.2f, 0.2f);
+            T0 = vmul(T0, tint);
+            T = vmul(T, tint);
+        }
+        const float tl = lum3(T0);
+        if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
+            break;
+        }
+        if (depth >= rrMin) {
+            const float surv = fmin2(0.95f, fmax2(tl, 0.05f));
+            if (orc_rand01(rng) > surv) {
+                break;
+            }
+            T0 = vscale(T0, 1.0f / surv);
+            T = vscale(T, 1.0f / surv);
+        }
+        eta = smp.nextMediumEta;
+        const float bias = fmax2(K_SHADOW_BIAS, K_REL_SHADOW_BIAS * hit.t);
+        const float sign = vdot(n, smp.direction) >= 0.0f ? 1.0f : -1.0f;
+        origin = vadd(p, vscale(n, sign * bias));
+        dir = smp.direction;
+        hit = trace(origin, dir, s, bias, K_RAY_TMAX, 1);
+        if (!hit.hit) {
+            const OVec3 envL = orc_env_eval(env, params, dir);
+            const float lpdf = orc_env_pdf(env, params, dir);
+            const float mis = orc_mis_balance(smp.pdf, lpdf);
+            L = vadd(L, vmul(T, vscale(envL, mis)));
+            break;
+        }
+    }
+    return L;
+}
+
+/* d(clamped radiance)/d(radiance)^T g for the firefly clamp (PathTracer.cu:49-67). */
+void orc_clamp_vjp(OVec3 C, const double* g, double* out)
+{
+    out[0] = out[1] = out[2] = 0.0;
+    if (!isfinite(C.x) || !isfinite(C.y) || !isfinite(C.z)) {
+        return;
+    }
+    const double r[3] = {fmax2(0.0f, C.x), fmax2(0.0f, C.y), fmax2(0.0f, C.z)};
+    const int pos[3] = {C.x > 0.0f, C.y > 0.0f, C.z > 0.0f};
+    const double wl[3] = {0.2126, 0.7152, 0.0722};
+    const float lum = fmax2(0.0f, C.x) * 0.2126f + fmax2(0.0f, C.y) * 0.7152f + fmax2(0.0f, C.z) * 0.0722f;
+    double gr[3] = {g[0], g[1], g[2]};
+    if (lum > 100.0f) {
+        const double l = lum;
+        const double gdotr = g[0] * r[0] + g[1] * r[1] + g[2] * r[2];
+        for (int k = 0; k < 3; ++k) {
+            gr[k] = 100.0 / l * (g[k] - wl[k] * gdotr / l);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        out[k] = pos[k] ? gr[k] : 0.0;
+    }
+}
+
+typedef struct {
+    float* img;
+    const double* gradImg;
+    double* grad;
+    int w, h, spp, y0, y1, gradCount;
+    const OCamera* cam;
+    const OScene* s;
+    const OMaterial* evalMats;
+    const OEnv* env;
+    const OParams* params;
+} AdJob;
+
+static void ad_rows(const AdJob* j)
+{
+    for (int y = j->y0; y < j->y1; ++y) {
+        for (int x = 0; x < j->w; ++x) {
+            const int idx = y * j->w + x;
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            for (int k = 0; k < j->spp; ++k) {
+                ORng rng;
+                orc_rng_init(&rng, idx, k, (uint32_t)k + 1u);
+                const float ju = orc_rand01(&rng);
+                const float jv = orc_rand01(&rng);
+                const float u = ((float)x + ju) / (float)j->w;
+                const float v = ((float)y + jv) / (float)j->h;
+                const float l1 = orc_rand01(&rng);
+                const float l2 = orc_rand01(&rng);
+                OVec3 ro, rd;
+                orc_camera_ray(j->cam, u, v, l1, l2, &ro, &rd);
+                if (j->gradImg == NULL) {
+                    const OVec3 c = orc_clamp_firefly(
+                        orc_trace_path_ad(ro, rd, j->s, j->evalMats, j->params, j->env, &rng, NULL, v3(0, 0, 0), NULL));
+                    const float inv = 1.0f / (float)(k + 1);
+                    acc[0] = (acc[0] * (float)k + c.x) * inv;
+                    acc[1] = (acc[1] * (float)k + c.y) * inv;
+                    acc[2] = (acc[2] * (float)k + c.z) * inv;
+                } else {
+                    ORng replay = rng;
+                    const OVec3 C = orc_trace_path_ad(ro, rd, j->s, NULL, j->params, j->env, &rng, NULL, v3(0, 0, 0),
+                                                      NULL);
+                    double gs[3] = {j->gradImg[3 * idx] / j->spp, j->gradImg[3 * idx + 1] / j->spp,
+                                    j->gradImg[3 * idx + 2] / j->spp};
+                    double gw[3];
+                    orc_clamp_vjp(C, gs, gw);
+                    if (gw[0] != 0.0 || gw[1] != 0.0 || gw[2] != 0.0) {
+                        (void)orc_trace_path_ad(ro, rd, j->s, NULL, j->params, j->env, &replay, gw, C, j->grad);
+                    }
+                }
+            }
+            if (j->img != NULL) {
+                j->img[3 * idx] = acc[0];
+                j->img[3 * idx + 1] = acc[1];
+                j->img[3 * idx + 2] = acc[2];
+            }
+        }
+    }
+}
+
+static void* ad_worker(void* arg)
+{
+    ad_rows((const AdJob*)arg);
+    return NULL;
+}
+
+static void ad_run(AdJob base, int threads, double* gradOut)
+{
+    if (threads < 1) {
+        threads = 1;
+    }
+    if (threads > 64) {
+        threads = 64;
+    }
+    pthread_t tid[64];
+    AdJob jobs[64];
+    double* partial = NULL;
+    if (gradOut != NULL) {
+        partial = (double*)calloc((size_t)threads * (size_t)base.gradCount, sizeof(double));
+    }
+    for (int t = 0; t < threads; ++t) {
+        jobs[t] = base;
+        jobs[t].y0 = base.h * t / threads;
+        jobs[t].y1 = base.h * (t + 1) / threads;
+        jobs[t].grad = partial != NULL ? partial + (size_t)t * base.gradCount : NULL;
+        pthread_create(&tid[t], NULL, ad_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(tid[t], NULL);
+    }
+    if (partial != NULL) {
+        for (int t = 0; t < threads; ++t) {
+            for (int k = 0; k < base.gradCount; ++k) {
+                gradOut[k] += partial[(size_t)t * base.gradCount + k];
+            }
+        }
+        free(partial);
+    }
+}
+
+/* Fixed-spp image of the detached estimator: sample k of every pixel uses sample index k and
+ * seed k + 1 (the adaptive schedule with minSamples = maxSamplesPerPixel = spp); img holds the
+ * running mean of the clamped radiance (3 floats per pixel). */
+void orc_render_detached(float* img3, int width, int height, const OCamera* cam, const OScene* s,
+                         const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp, int threads)
+{
+    AdJob j;
+    memset(&j, 0, sizeof(j));
+    j.img = img3;
+    j.w = width;
+    j.h = height;
+    j.spp = spp;
+    j.cam = cam;
+    j.s = s;
+    j.evalMats = evalMats;
+    j.env = env;
+    j.params = params;
+    ad_run(j, threads, NULL);
+}
+
+/* Adjoint of the same render: grad[7 * materialCount] += sum over pixels and samples of
+ * gradImg . d(pixel)/d(theta), with gradImg = dLoss/dImage (3 doubles per pixel). */
+void orc_render_adjoint(const double* gradImg, int width, int height, const OCamera* cam, const OScene* s,
+                        const OEnv* env, const OParams* params, int spp, double* grad, int threads)
+{
+    AdJob j;
+    memset(&j, 0, sizeof(j));
+    j.gradImg = gradImg;
+    j.w = width;
+    j.h = height;
+    j.spp = spp;
+    j.cam = cam;
+    j.s = s;
+    j.env = env;
+    j.params = params;
+    j.gradCount = 7 * (int)s->materialCount;
+    ad_run(j, threads, grad);
+}

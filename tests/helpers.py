@@ -39,6 +39,15 @@ def dense_plant(width=64, height=64, spp=8, iterations=3, radius=0.35, max_depth
         seg = np.arange(body.shape[0]) // per
         body[:, 24] = (seg % nmat).astype(np.uint32).view(np.float32)
         parts.append(body)
+    elif materials == "diff":
+        # opaque materials with every differentiable parameter strictly inside its clamp range
+        mats = [abi.pack_material(0.4, 0.3, 0.2, roughness=0.6, metallic=0.2, specular=0.5),
+                abi.pack_material(0.2, 0.6, 0.15, roughness=0.35, metallic=0.1, specular=0.7)]
+        nmat = 2
+        body = scenes.cylinders(t, 8, 0)
+        seg = np.arange(body.shape[0]) // 16
+        body[:, 24] = (seg % nmat).astype(np.uint32).view(np.float32)
+        parts.append(body)
     else:
         mats = [abi.pack_material(0.4, 0.3, 0.2, roughness=0.6)]
         nmat = 1
@@ -49,8 +58,12 @@ def dense_plant(width=64, height=64, spp=8, iterations=3, radius=0.35, max_depth
     ext = 0.5 * (hi - lo) + 2.0
     box_m = nmat
     light_m = nmat + 1
-    mats.append(abi.pack_material(0.7, 0.7, 0.7, roughness=0.9))
-    mats.append(abi.pack_material(1.0, 0.9, 0.8, emission=8.0))
+    if materials == "diff":
+        mats.append(abi.pack_material(0.7, 0.7, 0.7, roughness=0.9, metallic=0.05, specular=0.5))
+        mats.append(abi.pack_material(1.0, 0.9, 0.8, emission=8.0, roughness=0.5, metallic=0.1, specular=0.5))
+    else:
+        mats.append(abi.pack_material(0.7, 0.7, 0.7, roughness=0.9))
+        mats.append(abi.pack_material(1.0, 0.9, 0.8, emission=8.0))
     # floor, back wall, left wall (open front and right side so the environment is visible)
     parts.append(scenes.quad((c[0], lo[1] - 0.01, c[2]), (0.0, 0.0, ext[2]), (ext[0], 0.0, 0.0), box_m))
     parts.append(scenes.quad((c[0], c[1], c[2] - ext[2]), (ext[0], 0.0, 0.0), (0.0, ext[1], 0.0), box_m))

@@ -1,0 +1,139 @@
+"""CPU pins of the oracle's differentiable renderer: the detached estimator reduces to the
+forward integrator at the primal parameters (bit for bit), the analytic BSDF derivative matches
+central differences of orc_brdf_eval, the firefly-clamp VJP matches differences of
+orc_clamp_firefly, and the FP64 adjoint gradient of an image loss matches central finite
+differences of the detached estimator (h = 1e-3) on a small scene."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from lsystemeditor_amd import abi
+from oracle import oracle as O
+from tests.helpers import dense_plant
+
+PARAMS = ["r", "g", "b", "roughness", "metallic", "specular", "emission"]
+WORD = {"r": 0, "g": 1, "b": 2, "roughness": 3, "metallic": 4, "specular": 14, "emission": 9}
+
+
+def small_scene(w=10, h=10, spp=4, depth=4):
+    tris, mats, tex, cam, params = dense_plant(w, h, spp=spp, max_depth=depth, materials="diff")
+    return tris, mats, cam, params
+
+
+def test_detached_estimator_is_the_forward_at_primal_parameters():
+    w = h = 16
+    for kind in ("diff", "zoo"):
+        tris, mats, tex, cam, params = dense_plant(w, h, spp=4, materials=kind)
+        sc = O.OracleScene(abi.host_to_gpu_triangles(tris), mats, tex)
+        acc, _ = O.render_adaptive(sc, cam, params, w, h, threads=8)
+        img = np.zeros((w * h, 3), np.float32)
+        O.lib().orc_render_detached(O.ptr(img), w, h, O.ptr(cam), C.byref(sc.scene), None, C.byref(O.Env()),
+                                    O.ptr(params), 4, 8)
+        assert np.array_equal(img.view(np.uint32), acc.acc[:, :3].copy().view(np.uint32)), kind
+        # evaluation table equal to the scene's: also identical
+        mats_copy = np.ascontiguousarray(sc.mats.copy())
+        img2 = np.zeros_like(img)
+        O.lib().orc_render_detached(O.ptr(img2), w, h, O.ptr(cam), C.byref(sc.scene), O.ptr(mats_copy),
+                                    C.byref(O.Env()), O.ptr(params), 4, 8)
+        if kind == "diff":   # (subsurface paths take the factor form when a table is given)
+            assert np.array_equal(img2.view(np.uint32), img.view(np.uint32))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_brdf_gradient_matches_central_differences(seed):
+    rng = np.random.default_rng(seed)
+    L = O.lib()
+    base = dict(r=rng.uniform(0.2, 0.8), g=rng.uniform(0.2, 0.8), b=rng.uniform(0.2, 0.8),
+                roughness=rng.uniform(0.2, 0.8), metallic=rng.uniform(0.2, 0.8), specular=rng.uniform(0.2, 0.8))
+    words = abi.pack_material(base["r"], base["g"], base["b"], roughness=base["roughness"],
+                              metallic=base["metallic"], specular=base["specular"])
+    for trial in range(20):
+        wo = rng.normal(size=3)
+        wo[2] = abs(wo[2]) + 0.2
+        wi = rng.normal(size=3)
+        wi[2] = abs(wi[2]) + 0.2
+        wo /= np.linalg.norm(wo)
+        wi /= np.linalg.norm(wi)
+        a = rng.uniform(0.5, 1.5, 3)
+        grad = np.zeros(6)
+        m = O.Material.from_buffer_copy(words.tobytes())
+        c = O.ctx(m, wo=wo)
+        L.orc_brdf_grad(C.byref(c), O.v3(wi), O.ptr(np.ascontiguousarray(a)), O.ptr(words), O.ptr(grad))
+        for k, name in enumerate(PARAMS[:6]):
+            def fd_at(hstep):
+                vals = []
+                for sgn in (+1, -1):
+                    ww = words.copy()
+                    ww[WORD[name]] += sgn * hstep
+                    f = L.orc_brdf_eval(C.byref(O.ctx(O.Material.from_buffer_copy(ww.tobytes()), wo=wo)), O.v3(wi))
+                    vals.append(a[0] * f.x + a[1] * f.y + a[2] * f.z)
+                return (vals[0] - vals[1]) / (2 * hstep)
+            # float rounding of sharp specular peaks makes the difference quotient itself uncertain:
+            # its spread between two step sizes is part of the tolerance
+            fd, fd2 = fd_at(1e-3), fd_at(2e-3)
+            tol = 2e-3 * max(abs(fd), 1e-2) + 2.0 * abs(fd - fd2)
+            assert abs(grad[k] - fd) <= tol, (name, grad[k], fd, fd2)
+
+
+def test_clamp_vjp_matches_differences():
+    L = O.lib()
+    rng = np.random.default_rng(3)
+    for C3 in ([0.5, 0.2, 0.1], [300.0, 80.0, 40.0], [-0.2, 2.0, 1.0], [150.0, -3.0, 500.0]):
+        g = rng.normal(size=3)
+        out = np.zeros(3)
+        L.orc_clamp_vjp(O.v3(C3), O.ptr(g), O.ptr(out))
+        for k in range(3):
+            hstep = 1e-2 * max(1.0, abs(C3[k]))
+            up, dn = list(C3), list(C3)
+            up[k] += hstep
+            dn[k] -= hstep
+            cu = L.orc_clamp_firefly(O.v3(up))
+            cd = L.orc_clamp_firefly(O.v3(dn))
+            fd = sum(g[j] * ((cu.x, cu.y, cu.z)[j] - (cd.x, cd.y, cd.z)[j]) for j in range(3)) / (2 * hstep)
+            if C3[k] < 0:
+                assert out[k] == 0.0
+            else:
+                assert abs(out[k] - fd) <= 1e-3 * max(1.0, abs(fd)), (C3, k, out[k], fd)
+
+
+def render(sc, cam, params, w, h, spp, eval_mats=None):
+    img = np.zeros((w * h, 3), np.float32)
+    O.lib().orc_render_detached(O.ptr(img), w, h, O.ptr(cam), C.byref(sc.scene),
+                                None if eval_mats is None else O.ptr(eval_mats), C.byref(O.Env()), O.ptr(params),
+                                spp, 8)
+    return img
+
+
+def test_adjoint_matches_finite_differences_of_the_detached_estimator():
+    w = h = 12
+    spp = 4
+    tris, mats, cam, params = small_scene(w, h, spp)
+    sc = O.OracleScene(abi.host_to_gpu_triangles(tris), mats)
+    img = render(sc, cam, params, w, h, spp)
+    target = (img * np.float32(0.8) + np.float32(0.02)).astype(np.float32)
+    grad_img = np.ascontiguousarray(2.0 * (img.astype(np.float64) - target))
+    grad = np.zeros(7 * sc.mats.shape[0])
+    O.lib().orc_render_adjoint(O.ptr(grad_img), w, h, O.ptr(cam), C.byref(sc.scene), C.byref(O.Env()),
+                               O.ptr(params), spp, O.ptr(grad), 8)
+    grad = grad.reshape(-1, 7)
+    scale = np.abs(grad).max()
+    assert scale > 0
+    hstep = 1e-3
+    checked = 0
+    for mi in range(sc.mats.shape[0]):
+        for k, name in enumerate(PARAMS):
+            v = float(sc.mats[mi, WORD[name]])
+            if name == "emission" and v <= 0.0:
+                assert grad[mi, k] == 0.0
+                continue
+            losses = []
+            for sgn in (+1, -1):
+                em = np.ascontiguousarray(sc.mats.copy())
+                em[mi, WORD[name]] = np.float32(v + sgn * hstep)
+                im = render(sc, cam, params, w, h, spp, em)
+                losses.append(float(((im.astype(np.float64) - target) ** 2).sum()))
+            fd = (losses[0] - losses[1]) / (2 * hstep)
+            assert abs(grad[mi, k] - fd) <= 2e-3 * abs(fd) + 2e-4 * scale, (mi, name, grad[mi, k], fd)
+            checked += 1
+    assert checked >= 20
