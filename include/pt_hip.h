@@ -270,6 +270,20 @@ bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndi
                               const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
                               unsigned int globalSeed, uint32_t* d_stats, hipStream_t stream);
 
+/* Adjoint (path replay) of the fixed-spp render -- sample k of every pixel uses sample index k
+ * and seed k + 1, i.e. pathTracerAdaptiveRenderWavefront or the per-pass loop with
+ * minSamples = maxSamplesPerPixel = samplesPerPixel from a cleared accumulator. d_gradImage
+ * holds dLoss/dPixel (3 floats per pixel, row-major); d_grad (7 doubles per material:
+ * base r, g, b, roughness, metallic, specular, emission) accumulates dLoss/dParameter.
+ * Sampling decisions are detached (those of the current parameters); BSDF derivatives cover the
+ * opaque Principled lobes; parameters are differentiated where the forward is differentiable
+ * (untextured; roughness/metallic/specular strictly inside (0, 1); emission > 0).
+ * materialCount <= 64. No reference counterpart. */
+bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
+                             const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                             const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, int materialCount,
+                             double* d_grad, hipStream_t stream);
+
 /* ---- scene (replaces MeshAccelScene::build + upload, MeshAccelScene.cpp:141-341) ---------- */
 typedef struct PtScene PtScene;
 
@@ -296,6 +310,13 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
 
 /* Milliseconds of the last GPU BVH build (device time between events). */
 float ptSceneLastBuildMs(const PtScene* scene);
+
+/* Replaces the material table of a built scene (no reference counterpart; for differentiable
+ * rendering and optimisation loops): same count; a material may not start or stop emitting.
+ * The emissive list keeps its triangles; their radiance and the selection CDF are recomputed
+ * as the build does. Synchronous. */
+bool ptSceneUpdateMaterials(PtScene* scene, const MaterialGpu* materials, uint32_t materialCount,
+                            hipStream_t stream);
 
 /* BVH build used by the next ptSceneBuild (no reference counterpart): 0 = the reference's
  * median split (default; node arrays equal the reference's), 1 = binned SAH over Morton-ordered

@@ -75,6 +75,8 @@ SIGNATURES = {
     "ptSceneReadBvh": (_b, [_p, _p, _u, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ptSceneLastBuildMs": (_f, [_p]),
     "ptSceneSetBuildMode": (_b, [_p, _i]),
+    "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
+    "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p]),
     "ptSceneReadLayout": (_b, [_p, _p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
     "ptEnvironmentDevice": (_p, [_p]),

@@ -411,6 +411,12 @@ struct PtScene {
     float buildMs = 0.0f;
     int buildMode = 0; // 0: median split (reference topology), 1: binned SAH
     int levels = 0;
+    // host copies for ptSceneUpdateMaterials: materials, and per emissive entry its area and
+    // material index
+    std::vector<MaterialGpu> matsHost;
+    std::vector<float> emAreaHost;
+    std::vector<uint32_t> emMatHost;
+    std::vector<float4> emTrisHost;
 
     void clear()
     {
@@ -600,6 +606,8 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
     }
     (void)hipStreamSynchronize(stream);
     sc->clear();
+    sc->emAreaHost.clear();
+    sc->emMatHost.clear();
     const int n = static_cast<int>(triangleCount);
 
     // Materials: default material when none is given (MeshAccelScene.cpp:156-159).
@@ -632,6 +640,8 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
             continue;
         }
         emIdx.push_back(i);
+        sc->emAreaHost.push_back(area);
+        sc->emMatHost.push_back(t.materialIndex);
         total += w;
         emCdf.push_back(total);
         emTris.push_back(make_float4(t.v0.x, t.v0.y, t.v0.z, m.r * m.emission));
@@ -641,6 +651,8 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
     if (total <= 0.0f) {
         emCdf.assign(1, 0.0f);
     }
+    sc->matsHost = mats;
+    sc->emTrisHost = emTris;
 
     float* dTris = nullptr;
     bool ok = true;
@@ -706,6 +718,59 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
 }
 
 const MeshAccelSceneGpu* ptSceneDevice(const PtScene* scene) { return scene != nullptr ? scene->d_desc : nullptr; }
+
+bool ptSceneUpdateMaterials(PtScene* sc, const MaterialGpu* materials, uint32_t materialCount, hipStream_t stream)
+{
+    if (sc == nullptr || sc->d_desc == nullptr || materials == nullptr ||
+        materialCount != static_cast<uint32_t>(sc->matsHost.size())) {
+        ptSetError("ptSceneUpdateMaterials: invalid arguments (the material count must not change)");
+        return false;
+    }
+    auto lum = [](const MaterialGpu& m) { return m.emission * (0.2126f * m.r + 0.7152f * m.g + 0.0722f * m.b); };
+    for (uint32_t k = 0; k < materialCount; ++k) {
+        if ((lum(sc->matsHost[k]) > 0.0f) != (lum(materials[k]) > 0.0f)) {
+            ptSetError("ptSceneUpdateMaterials: material %u changes whether it emits; rebuild the scene", k);
+            return false;
+        }
+    }
+    // same sequential float CDF as the build (MeshAccelScene.cpp:62-95)
+    const size_t ne = sc->emAreaHost.size();
+    std::vector<float> cdf(1, 0.0f);
+    std::vector<float> radiance(3 * ne);
+    float total = 0.0f;
+    for (size_t e = 0; e < ne; ++e) {
+        const MaterialGpu& m = materials[sc->emMatHost[e]];
+        total += sc->emAreaHost[e] * lum(m);
+        cdf.push_back(total);
+        radiance[3 * e + 0] = m.r * m.emission;
+        radiance[3 * e + 1] = m.g * m.emission;
+        radiance[3 * e + 2] = m.b * m.emission;
+    }
+    if (total <= 0.0f) {
+        cdf.assign(1, 0.0f);
+    }
+    bool ok = hipMemcpyAsync(sc->materials, materials, materialCount * sizeof(MaterialGpu), hipMemcpyHostToDevice,
+                             stream) == hipSuccess;
+    if (ok && ne > 0) {
+        ok = hipMemcpyAsync(sc->emCdf, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice, stream) ==
+             hipSuccess;
+        // radiance lives in the .w lane of the three float4 records of each emissive entry
+        for (size_t e = 0; e < ne; ++e) {
+            for (int q = 0; q < 3; ++q) {
+                sc->emTrisHost[3 * e + q].w = radiance[3 * e + q];
+            }
+        }
+        ok = ok && hipMemcpyAsync(sc->emTris, sc->emTrisHost.data(), sc->emTrisHost.size() * sizeof(float4),
+                                  hipMemcpyHostToDevice, stream) == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(stream) == hipSuccess;
+    if (!ok) {
+        ptSetError("ptSceneUpdateMaterials: upload failed");
+        return false;
+    }
+    sc->matsHost.assign(materials, materials + materialCount);
+    return true;
+}
 
 bool ptSceneSetBuildMode(PtScene* scene, int mode)
 {
