@@ -4,7 +4,10 @@ A step is one complete adaptive render of the frame: clear the accumulator, then
 pixel until it converges (relative error 0.01, 16..256 spp, max depth 12), with the reference's
 per-pixel semantics (pass p uses seed p + 1, convergence tested every 4 samples), executed by the
 wavefront scheduler. Samples are counted from the per-pixel counts buffer. Inputs (scene, BVH,
-camera, params) are resident in HBM before timing.
+camera, params) are resident in HBM before timing. The BVH is the binned-SAH build by default
+(closest hits equal the reference's median-split tree's; --bvh median uses that tree, and the
+median-split throughput is reported beside the headline). After the headline measurement the
+differentiable C2 workload (adjoint cost, gradient checks) and the CPU baseline are measured.
 
 Prints one JSON line (see the contract in the repository's documentation).
 """
@@ -37,11 +40,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=["c1", "c0"], default="c1")
-    ap.add_argument("--bvh", choices=["median", "sah"], default="median",
+    ap.add_argument("--bvh", choices=["median", "sah"], default="sah",
                     help="BVH build: the reference's median split, or binned SAH (same results, faster traversal)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-pass (reference cadence) timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-render (C2) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device tensor copy (scale reference for PMC byte counters)")
     return ap.parse_args()
@@ -92,6 +96,70 @@ def cpu_baseline(spec, threads: int):
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"oracle (C, pthreads) on a centred {win}x{win} window of the {w}x{h} frame, passes 1-{passes} "
                       f"({samples} samples, {dt:.1f} s; BVH build {build_s:.1f} s excluded)"}
+
+
+def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
+    """C2: differentiable render of the C1 bush at 512x512, 32 spp, depth 8, five material groups
+    (bark, leaf, ground, two lights). Times the fixed-spp forward and the adjoint on the same
+    samples, checks the adjoint against central differences (h = 1e-3) of the detached estimator
+    on the GPU for every live parameter, and against the oracle's FP64 adjoint on a 32x32 window."""
+    import ctypes as C
+    import torch
+    from lsystemeditor_amd import abi, diff, render, scenes
+    from oracle import oracle as O
+
+    spec = scenes.config2(c1=c1)
+    w, h, spp = spec.width, spec.height, spec.spp
+    scene = abi.Scene(spec.triangles, spec.materials, bvh=bvh)
+    r = diff.DiffRenderer(scene, spec.materials, spec.camera, spec.params, w, h, spp)
+    theta0 = r.theta0()
+    target = r.forward(torch.from_numpy(scenes.perturbed_target(theta0.numpy())).cuda())
+    img = r.forward(theta0.cuda())
+    grad_img = (2.0 * (img - target)).contiguous()
+    r.adjoint(grad_img)
+    torch.cuda.synchronize()
+    reps = 3
+    th = theta0.cuda()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.forward(th)   # includes the per-sample radiance capture the adjoint reuses
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = r.adjoint(grad_img)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / reps
+    samples = w * h * spp
+    print(f"[bench] C2 forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms ({samples / t_adj / 1e6:.1f} "
+          f"grad-Msamples/s)", file=sys.stderr, flush=True)
+    res = diff.gradient_check(r, theta0.cuda(), target, h=1e-3)
+    print(f"[bench] C2 gradient vs central differences: max rel err {res['max_rel_err']:.2e}, median "
+          f"{res['median_rel_err']:.2e} over {int(res['live'].sum())} parameters", file=sys.stderr, flush=True)
+    # FP64 CPU reference on a 32x32 window (the loss restricted to it)
+    win = torch.zeros_like(grad_img).view(h, w, 3)
+    y0, x0 = h // 2 - 16, w // 2 - 16
+    win[y0:y0 + 32, x0:x0 + 32] = grad_img.view(h, w, 3)[y0:y0 + 32, x0:x0 + 32]
+    win = win.reshape(-1, 3).contiguous()
+    g_gpu = r.adjoint(win).cpu().numpy()
+    ref = O.OracleScene(abi.host_to_gpu_triangles(spec.triangles), spec.materials)
+    gi = np.ascontiguousarray(win.cpu().numpy().astype(np.float64))
+    g_cpu = np.zeros(7 * spec.materials.shape[0])
+    O.lib().orc_render_adjoint(O.ptr(gi), w, h, O.ptr(spec.camera), C.byref(ref.scene), C.byref(O.Env()),
+                               O.ptr(r.params_host), spp, O.ptr(g_cpu), cpu_threads)
+    g_cpu = g_cpu.reshape(-1, 7)
+    live = res["live"] & (g_cpu != 0.0)
+    rel_cpu = np.abs(g_gpu - g_cpu)[live] / np.maximum(np.abs(g_cpu[live]), 1e-3 * np.abs(g_cpu).max())
+    return {"workload": f"C2: C1 bush, {w}x{h}, {spp} spp fixed, max depth 8, {spec.materials.shape[0]} material "
+                        "groups x (base rgb, roughness, metallic, specular, emission); L2 loss against the render "
+                        "at parameters perturbed uniformly by +-20% (seeded)",
+            "grad_samples_per_s": samples / t_adj, "forward_samples_per_s": samples / t_fwd,
+            "cost_ratio_adjoint_vs_forward": t_adj / t_fwd,
+            "cost_ratio_forward_plus_adjoint_vs_forward": (t_fwd + t_adj) / t_fwd,
+            "equal_rmse": "same samples in both",
+            "params_checked": int(res["live"].sum()), "fd_step": 1e-3,
+            "fd_max_rel_err": res["max_rel_err"], "fd_median_rel_err": res["median_rel_err"],
+            "fp64_cpu_window": "32x32", "fp64_cpu_max_rel_err": float(rel_cpu.max()) if rel_cpu.size else None}
 
 
 def profile_read():
@@ -178,9 +246,24 @@ def main():
         torch.cuda.synchronize()
         extra["reference_cadence_msamples_s"] = int(acc.counts.to(torch.int64).sum().item()) / \
             (time.perf_counter() - tl0) / 1e6
+        if args.bvh == "sah":
+            # the same wavefront render over the reference's median-split tree
+            med = abi.Scene(spec.triangles, spec.materials, stream=int(stream.cuda_stream), bvh="median")
+            render.render_wavefront(med, acc, inputs, passes=passes)
+            torch.cuda.synchronize()
+            tm = time.perf_counter()
+            render.render_wavefront(med, acc, inputs, passes=passes)
+            torch.cuda.synchronize()
+            extra["median_split_bvh_msamples_s"] = int(acc.counts.to(torch.int64).sum().item()) / \
+                (time.perf_counter() - tm) / 1e6
+            extra["median_split_bvh_build_ms"] = round(med.build_ms, 3)
+            del med
         print(f"[bench] persistent megakernel {extra['persistent_megakernel_msamples_s']:.1f}, reference cadence "
               f"(one launch per pass) {extra['reference_cadence_msamples_s']:.1f} Msamples/s", file=sys.stderr,
               flush=True)
+
+    if not args.no_adjoint and args.config == "c1":
+        extra["adjoint"] = adjoint_bench(spec, args.bvh, args.cpu_threads or min(16, os.cpu_count() or 1))
 
     cpu = None
     if not args.no_cpu:

@@ -278,11 +278,29 @@ bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndi
  * Sampling decisions are detached (those of the current parameters); BSDF derivatives cover the
  * opaque Principled lobes; parameters are differentiated where the forward is differentiable
  * (untextured; roughness/metallic/specular strictly inside (0, 1); emission > 0).
- * materialCount <= 64. No reference counterpart. */
+ * materialCount <= 64. d_sampleRadiance (optional) holds the primal, unclamped radiance of every
+ * (pixel, sample), 3 floats at (pixel * samplesPerPixel + sample), as
+ * pathTracerWavefrontCaptureRadiance records it during the forward; without it each primal path
+ * is traced again before its replay. No reference counterpart. */
 bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
                              const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
                              const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, int materialCount,
-                             double* d_grad, hipStream_t stream);
+                             const float* d_sampleRadiance, double* d_grad, hipStream_t stream);
+
+/* While d_out is set, pathTracerAdaptiveRenderWavefront writes the unclamped radiance of every
+ * finished sample with index < samplesPerPixel to d_out[3 * (pixel * samplesPerPixel + sample)]
+ * (width * height * samplesPerPixel * 3 floats). NULL turns it off. No reference counterpart. */
+bool pathTracerWavefrontCaptureRadiance(float* d_out, int samplesPerPixel);
+
+/* The detached estimator of the same render as an image (3 floats per pixel, running mean of the
+ * clamped radiance): paths sampled with the scene's materials (directions, pdfs, MIS weights,
+ * Russian roulette), values evaluated with d_evalMaterials (same count and layout). With
+ * d_evalMaterials equal to the scene's table it is the fixed-spp render itself; central
+ * differences in d_evalMaterials are the finite-difference reference of
+ * pathTracerAdjointRender. No reference counterpart. */
+bool pathTracerDetachedRender(float* d_image, int width, int height, int samplesPerPixel, const CameraGpu* d_camera,
+                              const MeshAccelSceneGpu* d_scene, const MaterialGpu* d_evalMaterials,
+                              const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, hipStream_t stream);
 
 /* ---- scene (replaces MeshAccelScene::build + upload, MeshAccelScene.cpp:141-341) ---------- */
 typedef struct PtScene PtScene;

@@ -76,7 +76,9 @@ SIGNATURES = {
     "ptSceneLastBuildMs": (_f, [_p]),
     "ptSceneSetBuildMode": (_b, [_p, _i]),
     "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
-    "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p]),
+    "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
+    "pathTracerWavefrontCaptureRadiance": (_b, [_p, _i]),
+    "pathTracerDetachedRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "ptSceneReadLayout": (_b, [_p, _p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
     "ptEnvironmentDevice": (_p, [_p]),

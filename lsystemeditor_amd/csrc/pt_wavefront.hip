@@ -420,7 +420,8 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                                                             const EnvironmentMapGpu* env, Paths P, int bounce,
                                                             int maxDepth, const int* __restrict__ inQueue,
                                                             const int* __restrict__ inCountPtr, int* extOut,
-                                                            int* shadeOut, int* counters)
+                                                            int* shadeOut, int* counters, float* capture,
+                                                            int captureSpp)
 {
     const MeshAccelSceneGpu s = *scene;
     const RenderParamsGpu prm = *params;
@@ -603,6 +604,13 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
             newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
             P.flags[p] = newFlags;
             if (newFlags == 0u) {
+                if (capture != nullptr && P.sample[p] < static_cast<uint32_t>(captureSpp)) {
+                    // unclamped radiance of (pixel, sample) for the adjoint
+                    const size_t slot = 3 * (static_cast<size_t>(P.pixel[p]) * captureSpp + P.sample[p]);
+                    capture[slot + 0] = L.x;
+                    capture[slot + 1] = L.y;
+                    capture[slot + 2] = L.z;
+                }
                 const V3 r = clamp_firefly(L);  // PathTracer.cu:101-104
                 P.rx[p] = r.x;
                 P.ry[p] = r.y;
@@ -745,6 +753,10 @@ Profile gProf;
 // adaptive render mostly converge at minSamples (constant radiance) or run to maxSamplesPerPixel,
 // so wide groups discard few speculative samples and save many nearly empty deep-bounce launches.
 int gSpeculativeBudget = 0;
+
+// Optional per-sample radiance capture (pathTracerWavefrontCaptureRadiance).
+float* gCapture = nullptr;
+int gCaptureSpp = 0;
 
 void prof_mark(hipStream_t stream, int cls, bool end, int bounce = -1)
 {
@@ -965,7 +977,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
             prof_mark(stream, 2, false, b);
             hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
                                P, b, maxPathDepth, shadeQueue, shadeCount, P.extQ[(b + 1) & 1],
-                               P.shadeQ[(b + 1) & 1], cOut);
+                               P.shadeQ[(b + 1) & 1], cOut, gCapture, gCaptureSpp);
             prof_mark(stream, 2, true, b);
         }
         prof_mark(stream, 3, false);
@@ -1081,5 +1093,16 @@ extern "C" bool pathTracerWavefrontSchedule(int speculativePaths)
         return false;
     }
     gSpeculativeBudget = speculativePaths;
+    return true;
+}
+
+extern "C" bool pathTracerWavefrontCaptureRadiance(float* d_out, int samplesPerPixel)
+{
+    if (d_out != nullptr && samplesPerPixel <= 0) {
+        ptSetError("pathTracerWavefrontCaptureRadiance: invalid arguments");
+        return false;
+    }
+    gCapture = d_out;
+    gCaptureSpp = d_out != nullptr ? samplesPerPixel : 0;
     return true;
 }

@@ -32,6 +32,9 @@ class DiffRenderer:
         self.width, self.height, self.spp = width, height, spp
         self.acc = render.Accumulator.allocate(width, height)
         self.stream = stream
+        # per-sample primal radiance of the last forward, so the adjoint replays without re-tracing
+        self.radiance = torch.empty((width * height * spp, 3), dtype=torch.float32, device="cuda")
+        self.radiance_valid = False
 
     @property
     def material_count(self) -> int:
@@ -42,6 +45,7 @@ class DiffRenderer:
         return torch.from_numpy(self.base[:, list(PARAM_WORDS)].copy())
 
     def set_theta(self, theta: torch.Tensor) -> None:
+        self.radiance_valid = False
         t = theta.detach().to("cpu", torch.float32).numpy()
         mats = self.base.copy()
         mats[:, list(PARAM_WORDS)] = t
@@ -51,18 +55,77 @@ class DiffRenderer:
     def forward(self, theta: torch.Tensor) -> torch.Tensor:
         """(height * width, 3) float32 image of the fixed-spp render at parameters theta."""
         self.set_theta(theta)
-        render.render_wavefront(self.scene, self.acc, self.inputs, passes=self.spp, stream=self.stream)
+        abi.check(abi.lib().pathTracerWavefrontCaptureRadiance(self.radiance.data_ptr(), self.spp),
+                  "pathTracerWavefrontCaptureRadiance")
+        try:
+            render.render_wavefront(self.scene, self.acc, self.inputs, passes=self.spp, stream=self.stream)
+        finally:
+            abi.lib().pathTracerWavefrontCaptureRadiance(None, 0)
+        self.radiance_valid = True
+        self.last_theta = theta.detach().to("cpu", torch.float32).clone()
         return self.acc.acc[:, :3].clone()
 
-    def adjoint(self, grad_image: torch.Tensor) -> torch.Tensor:
-        """dLoss/dtheta (materials, 7) float64 for dLoss/dImage = grad_image (height * width, 3)."""
+    def adjoint(self, grad_image: torch.Tensor, reuse_forward: bool = True) -> torch.Tensor:
+        """dLoss/dtheta (materials, 7) float64 for dLoss/dImage = grad_image (height * width, 3),
+        at the parameters of the last forward (whose per-sample radiance is reused unless
+        reuse_forward is False)."""
         g = grad_image.detach().to("cuda", torch.float32).contiguous()
         out = torch.zeros((self.material_count, 7), dtype=torch.float64, device="cuda")
+        rad = self.radiance.data_ptr() if (reuse_forward and self.radiance_valid) else None
         abi.check(abi.lib().pathTracerAdjointRender(
             g.data_ptr(), self.width, self.height, self.spp, self.inputs.camera.data_ptr(), self.scene.device,
-            self.inputs.env_ptr, self.inputs.params.data_ptr(), self.material_count, out.data_ptr(),
+            self.inputs.env_ptr, self.inputs.params.data_ptr(), self.material_count, rad, out.data_ptr(),
             render._stream_handle(self.stream)), "pathTracerAdjointRender")
         return out
+
+    def detached(self, eval_theta: torch.Tensor) -> torch.Tensor:
+        """Image of the detached estimator: paths sampled with the scene's current materials,
+        values evaluated with eval_theta (pathTracerDetachedRender)."""
+        t = eval_theta.detach().to("cpu", torch.float32).numpy()
+        mats = self.base.copy()
+        mats[:, list(PARAM_WORDS)] = t
+        d_mats = torch.from_numpy(mats).cuda()
+        img = torch.empty((self.width * self.height, 3), dtype=torch.float32, device="cuda")
+        abi.check(abi.lib().pathTracerDetachedRender(
+            img.data_ptr(), self.width, self.height, self.spp, self.inputs.camera.data_ptr(), self.scene.device,
+            d_mats.data_ptr(), self.inputs.env_ptr, self.inputs.params.data_ptr(), render._stream_handle(self.stream)),
+            "pathTracerDetachedRender")
+        torch.cuda.synchronize()
+        return img
+
+    def live_mask(self) -> np.ndarray:
+        """(materials, 7) bool: parameters the forward is differentiable in."""
+        t = self.base[:, list(PARAM_WORDS)]
+        live = np.ones(t.shape, bool)
+        live[:, 3:6] = (t[:, 3:6] > 0.0) & (t[:, 3:6] < 1.0)
+        live[:, 6] = t[:, 6] > 0.0
+        return live
+
+
+def gradient_check(r: DiffRenderer, theta0: torch.Tensor, target: torch.Tensor, h: float = 1e-3) -> dict:
+    """Adjoint gradient of L = sum (image - target)^2 at theta0 against central differences
+    (step h) of the detached estimator, for every live parameter."""
+    img = r.forward(theta0)
+    grad_img = (2.0 * (img - target)).contiguous()
+    adj = r.adjoint(grad_img).cpu().numpy()
+    live = r.live_mask()
+    fd = np.zeros_like(adj)
+    for i in range(adj.shape[0]):
+        for k in range(adj.shape[1]):
+            if not live[i, k]:
+                continue
+            losses = []
+            for sgn in (1.0, -1.0):
+                th = theta0.detach().cpu().clone()
+                th[i, k] += sgn * h
+                im = r.detached(th)
+                losses.append(float(((im.double() - target.double()) ** 2).sum().item()))
+            fd[i, k] = (losses[0] - losses[1]) / (2.0 * h)
+    floor = 1e-3 * float(np.abs(fd[live]).max())
+    rel = np.abs(adj - fd) / np.maximum(np.abs(fd), floor)
+    rel[~live] = 0.0
+    return {"adjoint": adj, "fd": fd, "live": live, "rel_err": rel, "max_rel_err": float(rel[live].max()),
+            "median_rel_err": float(np.median(rel[live]))}
 
 
 class RenderFunction(torch.autograd.Function):
@@ -76,9 +139,12 @@ class RenderFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_image: torch.Tensor):
-        # the adjoint replays the paths of the primal parameters
-        ctx.renderer.set_theta(ctx.theta)
-        grad = ctx.renderer.adjoint(grad_image)
+        # the adjoint replays the paths of the primal parameters; the stored per-sample radiance
+        # is reused when no other forward ran in between
+        r = ctx.renderer
+        if not r.radiance_valid or not torch.equal(r.last_theta, ctx.theta.cpu()):
+            r.set_theta(ctx.theta)
+        grad = r.adjoint(grad_image)
         return grad.to(ctx.theta.device, ctx.theta.dtype), None
 
 

@@ -317,3 +317,37 @@ def config1(width: int = 1024, height: int = 1024, iterations: int = 7, seed: in
     return SceneSpec("C1", tris, mats, cam, params, width, height, None,
                      {"segments": int(t.p0.shape[0]), "leaves": int(t.leaf_pos.shape[0]),
                       "iterations": iterations, "seed": seed})
+
+
+def config2(width: int = 512, height: int = 512, spp: int = 32, iterations: int = 7, seed: int = 1,
+            c1: SceneSpec | None = None) -> SceneSpec:
+    """Differentiable render of the C1 bush at 512x512, 32 spp, max depth 8. Material groups bark,
+    leaf, ground and the two lights; every group's base colour, roughness, metallic, specular and
+    (lights) emission are parameters, so their values sit strictly inside the clamp ranges."""
+    if c1 is None:
+        c1 = config1(width, height, iterations=iterations, seed=seed)
+    camera = c1.camera.copy()
+    camera[9] = np.float32(width / height)  # aspect
+    mats = np.stack([pack_material(0.4, 0.3, 0.2, roughness=0.6, metallic=0.05, specular=0.5),
+                     pack_material(0.2, 0.5, 0.1, roughness=0.5, metallic=0.05, specular=0.5),
+                     pack_material(0.35, 0.35, 0.3, roughness=0.8, metallic=0.05, specular=0.5),
+                     pack_material(1.0, 0.95, 0.9, roughness=0.5, metallic=0.05, specular=0.5, emission=10.0),
+                     pack_material(0.9, 0.95, 1.0, roughness=0.5, metallic=0.05, specular=0.5, emission=5.0)])
+    params = pack_params(background=(0.6, 0.7, 0.9), environment_intensity=1.0, max_path_depth=8,
+                         russian_roulette_min_depth=3, min_samples=spp, max_samples_per_pixel=spp,
+                         relative_error_threshold=0.01)
+    return SceneSpec("C2", c1.triangles, mats, camera, params, width, height, spp, dict(c1.info))
+
+
+def perturbed_target(theta: np.ndarray, seed: int = 7, spread: float = 0.2) -> np.ndarray:
+    """Target parameters: each one scaled by an independent uniform factor in [1 - spread,
+    1 + spread] (SplitMix64-seeded), clamped back inside the differentiable ranges."""
+    rng = SplitMix64(0xC2000000 + seed)
+    out = np.array(theta, dtype=np.float64, copy=True)
+    for i in range(out.shape[0]):
+        for k in range(out.shape[1]):
+            u = rng.uniform()
+            out[i, k] *= 1.0 - spread + 2.0 * spread * u
+    out[:, 0:3] = np.clip(out[:, 0:3], 0.0, 1.0)
+    out[:, 3:6] = np.clip(out[:, 3:6], 0.01, 0.99)
+    return out.astype(np.float32)

@@ -1121,6 +1121,9 @@ static void ad_rows(const AdJob* j)
                     acc[0] = (acc[0] * (float)k + c.x) * inv;
                     acc[1] = (acc[1] * (float)k + c.y) * inv;
                     acc[2] = (acc[2] * (float)k + c.z) * inv;
+                } else if (j->gradImg[3 * idx] == 0.0 && j->gradImg[3 * idx + 1] == 0.0 &&
+                           j->gradImg[3 * idx + 2] == 0.0) {
+                    break; /* pixel outside the loss */
                 } else {
                     ORng replay = rng;
                     const OVec3 C = orc_trace_path_ad(ro, rd, j->s, NULL, j->params, j->env, &rng, NULL, v3(0, 0, 0),

@@ -94,3 +94,45 @@ def test_autograd_backward_equals_adjoint(D):
     loss.backward()
     direct = r.adjoint((2.0 * (img.detach() - target)).contiguous()).to(torch.float32)
     assert torch.allclose(theta.grad, direct, rtol=1e-6, atol=0.0)
+
+
+def test_detached_render_matches_oracle(D):
+    import torch
+    w = h = 20
+    r, ref, cam, params = setup(D, w, h, 4)
+    theta = r.theta0()
+    base = r.forward(theta.cuda()).cpu().numpy()
+    same = r.detached(theta).cpu().numpy()
+    assert np.array_equal(same.view(np.uint32), base.view(np.uint32))
+    pert = theta.clone()
+    pert[0, 3] += 0.05
+    pert[1, 0] -= 0.05
+    pert[-1, 6] *= 1.2
+    got = r.detached(pert).cpu().numpy()
+    em = r.base.copy()
+    em[:, list(D.PARAM_WORDS)] = pert.numpy()
+    want = oracle_image(ref, cam, params, w, h, 4, np.ascontiguousarray(em))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_gradient_check_against_finite_differences(D):
+    import torch
+    from lsystemeditor_amd import scenes
+    w = h = 32
+    r, ref, cam, params = setup(D, w, h, 8)
+    theta0 = r.theta0()
+    target_theta = torch.from_numpy(scenes.perturbed_target(theta0.numpy()))
+    target = r.forward(target_theta.cuda())
+    res = D.gradient_check(r, theta0.cuda(), target, h=1e-3)
+    assert res["max_rel_err"] <= 1e-2, res["rel_err"]
+    assert res["median_rel_err"] <= 1e-3, res["rel_err"]
+
+
+def test_adjoint_reusing_forward_radiance_equals_recomputed(D):
+    w = h = 24
+    r, ref, cam, params = setup(D, w, h, 8)
+    img = r.forward(r.theta0().cuda())
+    g_img = (img * 0.3 - 0.01).contiguous()
+    a = r.adjoint(g_img).cpu().numpy()
+    b = r.adjoint(g_img, reuse_forward=False).cpu().numpy()
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-14 * np.abs(b).max())
