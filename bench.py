@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-pass (reference cadence) timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-render (C2) measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-scene (C3) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device tensor copy (scale reference for PMC byte counters)")
     return ap.parse_args()
@@ -134,8 +135,12 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
     print(f"[bench] C2 forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms ({samples / t_adj / 1e6:.1f} "
           f"grad-Msamples/s)", file=sys.stderr, flush=True)
     res = diff.gradient_check(r, theta0.cuda(), target, h=1e-3)
-    print(f"[bench] C2 gradient vs central differences: max rel err {res['max_rel_err']:.2e}, median "
-          f"{res['median_rel_err']:.2e} over {int(res['live'].sum())} parameters", file=sys.stderr, flush=True)
+    # the float renders' rounding noise dominates small-gradient differences at h = 1e-3; a wider
+    # step shows the agreement once that noise is below the truncation error
+    res2 = diff.gradient_check(r, theta0.cuda(), target, h=1e-2)
+    print(f"[bench] C2 gradient vs central differences: h=1e-3 max rel err {res['max_rel_err']:.2e} (median "
+          f"{res['median_rel_err']:.2e}), h=1e-2 max {res2['max_rel_err']:.2e} (median {res2['median_rel_err']:.2e}) "
+          f"over {int(res['live'].sum())} parameters", file=sys.stderr, flush=True)
     # FP64 CPU reference on a 32x32 window (the loss restricted to it)
     win = torch.zeros_like(grad_img).view(h, w, 3)
     y0, x0 = h // 2 - 16, w // 2 - 16
@@ -159,7 +164,47 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
             "equal_rmse": "same samples in both",
             "params_checked": int(res["live"].sum()), "fd_step": 1e-3,
             "fd_max_rel_err": res["max_rel_err"], "fd_median_rel_err": res["median_rel_err"],
+            "fd_step_1e-2_max_rel_err": res2["max_rel_err"], "fd_step_1e-2_median_rel_err": res2["median_rel_err"],
             "fp64_cpu_window": "32x32", "fp64_cpu_max_rel_err": float(rel_cpu.max()) if rel_cpu.size else None}
+
+
+def batch_bench(bvh: str) -> dict:
+    """C3: 64 bushes (seeds 0..63, iterations 3..7, 128^2..1024^2, 64 spp, depth 8) rendered by one
+    graph replay, against the same 64 renders issued one by one through the adaptive entry point.
+    The paths-in-flight bound is autotuned on the batch's largest scene."""
+    import torch
+    from lsystemeditor_amd import batch
+
+    t0 = time.perf_counter()
+    specs = batch.batch_specs()
+    gen_s = time.perf_counter() - t0
+    largest = max(specs, key=lambda s: s.width * s.height * s.triangles.shape[0])
+    best, tune = batch.autotune_paths(largest, bvh=bvh)
+    br = batch.BatchRenderer(specs, bvh=bvh, max_paths=best)
+    samples = br.samples
+    br.render_sequential_adaptive_api()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br.render_sequential_adaptive_api()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    br.capture()
+    br.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br.replay()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    tri = sum(s.triangles.shape[0] for s in specs)
+    print(f"[bench] C3 {len(specs)} scenes, {tri} triangles, {samples / 1e6:.0f} Msamples: graph "
+          f"{samples / t_graph / 1e6:.1f} Msamples/s, sequential {samples / t_seq / 1e6:.1f} Msamples/s "
+          f"(paths in flight {best})", file=sys.stderr, flush=True)
+    return {"workload": "C3: 64 bushes, seeds 0-63, iterations 3-7, resolutions 128^2-1024^2, 64 spp, depth 8",
+            "scenes": len(specs), "triangles": int(tri), "samples": samples,
+            "graph_msamples_s": samples / t_graph / 1e6, "sequential_msamples_s": samples / t_seq / 1e6,
+            "speedup_vs_sequential": t_seq / t_graph, "autotuned_paths_in_flight": best,
+            "autotune_ms": {str(k): round(v, 2) for k, v in tune.items()},
+            "bvh_build_ms_total": round(br.build_ms, 1), "scene_generation_s": round(gen_s, 1)}
 
 
 def profile_read():
@@ -261,6 +306,9 @@ def main():
         print(f"[bench] persistent megakernel {extra['persistent_megakernel_msamples_s']:.1f}, reference cadence "
               f"(one launch per pass) {extra['reference_cadence_msamples_s']:.1f} Msamples/s", file=sys.stderr,
               flush=True)
+
+    if not args.no_batch and args.config == "c1":
+        extra["batch"] = batch_bench(args.bvh)
 
     if not args.no_adjoint and args.config == "c1":
         extra["adjoint"] = adjoint_bench(spec, args.bvh, args.cpu_threads or min(16, os.cpu_count() or 1))

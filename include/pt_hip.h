@@ -233,6 +233,23 @@ bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, 
                                        unsigned int firstGlobalSeed, int passCount, int maxPathDepth,
                                        int maxPathsInFlight, hipStream_t stream);
 
+/* Fixed-spp wavefront render with no host synchronisation, so it can be captured into a HIP
+ * graph: every pixel takes samplesPerPixel samples (sample index k, seed k + 1, as the adaptive
+ * schedule with minSamples = maxSamplesPerPixel = samplesPerPixel), in groups of up to
+ * maxPathsInFlight paths (0 = 16M), accumulated into the cleared accumulator as the reference
+ * does. While a stream is capturing, the workspace is not allocated: reserve it first with
+ * pathTracerWavefrontReserve. Calls on one device must not overlap (shared workspace).
+ * No reference counterpart. */
+bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                    uint8_t* d_converged, int width, int height, const CameraGpu* d_camera,
+                                    const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                                    const RenderParamsGpu* d_params, int samplesPerPixel, int maxPathDepth,
+                                    int maxPathsInFlight, hipStream_t stream);
+
+/* Grows the wavefront workspace for groups of maxPathsInFlight paths over `pixels` pixels at depth
+ * maxPathDepth (grow-only; a single allocation). */
+bool pathTracerWavefrontReserve(int maxPathsInFlight, int pixels, int maxPathDepth);
+
 /* Measurement hooks of pathTracerAdaptiveRenderWavefront (no reference counterpart).
  * timing != 0: every launch of the render is bracketed by hipEvents on its stream and the device
  * time is accumulated per kernel class; counted != 0: the trace kernels also count internal

@@ -78,6 +78,8 @@ SIGNATURES = {
     "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
     "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
     "pathTracerWavefrontCaptureRadiance": (_b, [_p, _i]),
+    "pathTracerRenderFixedWavefront": (_b, [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _i, _i, _i, _p]),
+    "pathTracerWavefrontReserve": (_b, [_i, _i, _i]),
     "pathTracerDetachedRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "ptSceneReadLayout": (_b, [_p, _p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),

@@ -1,0 +1,114 @@
+"""Batched rendering of many L-system scenes of uneven size: one HIP graph replays every scene's
+fixed-spp wavefront render back to back (no host synchronisation between or inside scenes), with
+the wavefront group size (paths in flight) picked by a timing autotuner.
+
+Workload C3: 64 stochastic bushes, seeds 0..63, iterations uniform in 3..7 (3 k .. 2 M triangles),
+resolutions drawn from 128^2 .. 1024^2, 64 spp, max depth 8.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from . import abi, render, scenes
+
+RESOLUTIONS = (128, 256, 512, 1024)
+
+
+def batch_specs(count: int = 64, spp: int = 64, depth: int = 8, first_seed: int = 0,
+                iterations=(3, 7)) -> list[scenes.SceneSpec]:
+    """Scene s: iterations and resolution drawn with SplitMix64 from the seed; the bush itself is
+    the C1 generator with that seed."""
+    out = []
+    lo, hi = iterations
+    for s in range(first_seed, first_seed + count):
+        g = scenes.SplitMix64(0xC3000000 + s)
+        it = lo + g.next() % (hi - lo + 1)
+        res = RESOLUTIONS[g.next() % len(RESOLUTIONS)]
+        spec = scenes.config1(width=res, height=res, iterations=int(it), seed=s, min_spp=spp, max_spp=spp,
+                              threshold=0.01)
+        spec.params[4:5] = np.array([depth], np.int32).view(np.float32)
+        spec.spp = spp
+        spec.info = dict(spec.info, resolution=res)
+        out.append(spec)
+    return out
+
+
+class BatchRenderer:
+    """Scenes, accumulators and device inputs of a batch, plus its captured graph."""
+
+    def __init__(self, specs: list[scenes.SceneSpec], bvh: str = "sah", max_paths: int = 0):
+        self.specs = specs
+        self.stream = torch.cuda.Stream()
+        h = int(self.stream.cuda_stream)
+        self.scenes = [abi.Scene(s.triangles, s.materials, stream=h, bvh=bvh) for s in specs]
+        self.build_ms = float(sum(sc.build_ms for sc in self.scenes))
+        self.inputs = [render.DeviceInputs(s.camera, s.params) for s in specs]
+        self.accs = [render.Accumulator.allocate(s.width, s.height) for s in specs]
+        self.max_paths = max_paths
+        self.graph = None
+        torch.cuda.synchronize()
+
+    @property
+    def samples(self) -> int:
+        return int(sum(s.width * s.height * s.spp for s in self.specs))
+
+    def _reserve(self):
+        depth = max(render.resolved_depth(i.params_host) for i in self.inputs)
+        pixels = max(s.width * s.height for s in self.specs)
+        paths = self.max_paths if self.max_paths > 0 else (1 << 24)
+        paths = max(paths, pixels)
+        abi.check(abi.lib().pathTracerWavefrontReserve(paths, pixels, depth), "pathTracerWavefrontReserve")
+
+    def render_all(self):
+        """Every scene's fixed-spp render, back to back on the batch stream (no graph)."""
+        self._reserve()
+        with torch.cuda.stream(self.stream):
+            for sc, acc, inp, spec in zip(self.scenes, self.accs, self.inputs, self.specs):
+                render.render_fixed(sc, acc, inp, spec.spp, max_paths=self.max_paths, stream=self.stream)
+
+    def capture(self):
+        """Records render_all into one graph."""
+        self._reserve()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            for sc, acc, inp, spec in zip(self.scenes, self.accs, self.inputs, self.specs):
+                render.render_fixed(sc, acc, inp, spec.spp, max_paths=self.max_paths,
+                                    stream=torch.cuda.current_stream())
+        self.graph = g
+
+    def replay(self):
+        self.graph.replay()
+
+    def render_sequential_adaptive_api(self):
+        """The reference-style sequence: one plain render per scene through the adaptive
+        wavefront entry point, with its host synchronisation per group."""
+        with torch.cuda.stream(self.stream):
+            for sc, acc, inp, spec in zip(self.scenes, self.accs, self.inputs, self.specs):
+                render.render_wavefront(sc, acc, inp, passes=spec.spp, stream=self.stream)
+
+
+def autotune_paths(spec: scenes.SceneSpec, candidates=(1 << 21, 1 << 22, 1 << 23, 1 << 24), bvh: str = "sah",
+                   reps: int = 2) -> tuple[int, dict]:
+    """Times the fixed-spp render of one scene for each paths-in-flight candidate; returns the
+    fastest and all timings (ms)."""
+    sc = abi.Scene(spec.triangles, spec.materials, bvh=bvh)
+    inp = render.DeviceInputs(spec.camera, spec.params)
+    acc = render.Accumulator.allocate(spec.width, spec.height)
+    times = {}
+    for c in candidates:
+        abi.check(abi.lib().pathTracerWavefrontReserve(max(c, spec.width * spec.height), spec.width * spec.height,
+                                                        render.resolved_depth(inp.params_host)),
+                  "pathTracerWavefrontReserve")
+        render.render_fixed(sc, acc, inp, spec.spp, max_paths=c)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            render.render_fixed(sc, acc, inp, spec.spp, max_paths=c)
+        torch.cuda.synchronize()
+        times[c] = (time.perf_counter() - t0) / reps * 1e3
+    best = min(times, key=times.get)
+    return best, times

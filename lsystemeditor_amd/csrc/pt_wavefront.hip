@@ -706,6 +706,18 @@ __global__ void wf_compact_kernel(const int* __restrict__ in, const int* __restr
     }
 }
 
+// Active list 0..n-1 and its device count (fixed-spp renders sample every pixel every pass).
+__global__ void wf_iota_kernel(int* list, int* count, int n)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) {
+        list[i] = i;
+    }
+    if (i == 0) {
+        *count = n;
+    }
+}
+
 __global__ void wf_zero_kernel(int* p, int n)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -758,9 +770,11 @@ int gSpeculativeBudget = 0;
 float* gCapture = nullptr;
 int gCaptureSpp = 0;
 
+bool gCapturing = false; // set while a fixed-spp render is being captured into a graph
+
 void prof_mark(hipStream_t stream, int cls, bool end, int bounce = -1)
 {
-    if (!gProf.timing || gProf.used >= 2048) {
+    if (!gProf.timing || gCapturing || gProf.used >= 2048) {
         return;
     }
     const int i = gProf.used;
@@ -868,6 +882,50 @@ unsigned blocks_for(long long n) { return static_cast<unsigned>(std::max<long lo
 
 } // namespace
 
+// One group's bounce loop: per bounce, the closest-hit trace of the extension queue, the any-hit
+// trace of the shadow queue (from bounce 1 on) and the shade launch that fills the next queues.
+// Launch counts depend only on the host-known depth and path bound (no readbacks).
+void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, const RenderParamsGpu* d_params,
+                    const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks)
+{
+    const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
+    const unsigned shadowBlocks =
+        static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
+    for (int b = 0; b <= maxPathDepth; ++b) {
+        int* cIn = P.counters + kCountersPerBounce * b;        // produced by bounce b-1 (or gen)
+        int* cOut = P.counters + kCountersPerBounce * (b + 1); // produced by this bounce
+        const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
+        const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
+        prof_mark(stream, 0, false, b);
+        if (gProf.counted) {
+            hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+                               d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats);
+        } else {
+            hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+                               d_scene, P, extQueue, extCount, &cIn[4], nullptr);
+        }
+        prof_mark(stream, 0, true, b);
+        if (b > 0) {
+            prof_mark(stream, 1, false, b);
+            if (gProf.counted) {
+                hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                   d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats);
+            } else {
+                hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                   d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr);
+            }
+            prof_mark(stream, 1, true, b);
+        }
+        const int* shadeQueue = b == 0 ? nullptr : P.shadeQ[b & 1];
+        const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
+        prof_mark(stream, 2, false, b);
+        hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
+                           P, b, maxPathDepth, shadeQueue, shadeCount, P.extQ[(b + 1) & 1],
+                           P.shadeQ[(b + 1) & 1], cOut, gCapture, gCaptureSpp);
+        prof_mark(stream, 2, true, b);
+    }
+}
+
 extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
                                                   float* d_m2, uint8_t* d_converged, const int* d_activeIndices,
                                                   int activeCount, int width, int height, const CameraGpu* d_camera,
@@ -944,42 +1002,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
                            d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
                            passCount, group, P);
         prof_mark(stream, 3, true);
-        const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
-        const unsigned shadowBlocks =
-            static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
-        for (int b = 0; b <= maxPathDepth; ++b) {
-            int* cIn = P.counters + kCountersPerBounce * b;        // produced by bounce b-1 (or gen)
-            int* cOut = P.counters + kCountersPerBounce * (b + 1); // produced by this bounce
-            const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
-            const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
-            prof_mark(stream, 0, false, b);
-            if (gProf.counted) {
-                hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats);
-            } else {
-                hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, extQueue, extCount, &cIn[4], nullptr);
-            }
-            prof_mark(stream, 0, true, b);
-            if (b > 0) {
-                prof_mark(stream, 1, false, b);
-                if (gProf.counted) {
-                    hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                       d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats);
-                } else {
-                    hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                       d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr);
-                }
-                prof_mark(stream, 1, true, b);
-            }
-            const int* shadeQueue = b == 0 ? nullptr : P.shadeQ[b & 1];
-            const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
-            prof_mark(stream, 2, false, b);
-            hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
-                               P, b, maxPathDepth, shadeQueue, shadeCount, P.extQ[(b + 1) & 1],
-                               P.shadeQ[(b + 1) & 1], cOut, gCapture, gCaptureSpp);
-            prof_mark(stream, 2, true, b);
-        }
+        launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks);
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
                            reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
@@ -1104,5 +1127,81 @@ extern "C" bool pathTracerWavefrontCaptureRadiance(float* d_out, int samplesPerP
     }
     gCapture = d_out;
     gCaptureSpp = d_out != nullptr ? samplesPerPixel : 0;
+    return true;
+}
+
+extern "C" bool pathTracerWavefrontReserve(int maxPathsInFlight, int pixels, int maxPathDepth)
+{
+    if (maxPathsInFlight <= 0 || pixels <= 0 || maxPathDepth < 0 || maxPathDepth > 256) {
+        ptSetError("pathTracerWavefrontReserve: invalid arguments");
+        return false;
+    }
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) == hipSuccess) {
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    return ensure_workspace(std::max(maxPathsInFlight, 4), pixels, maxPathDepth, cus * 8 * kWfBlock);
+}
+
+extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                               uint8_t* d_converged, int width, int height,
+                                               const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                               const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                               int samplesPerPixel, int maxPathDepth, int maxPathsInFlight,
+                                               hipStream_t stream)
+{
+    if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
+        d_converged == nullptr || d_camera == nullptr || d_scene == nullptr || d_params == nullptr || width <= 0 ||
+        height <= 0 || samplesPerPixel <= 0 || maxPathDepth < 0 || maxPathDepth > 256) {
+        ptSetError("pathTracerRenderFixedWavefront: invalid arguments");
+        return false;
+    }
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(stream, &capture);
+    const bool capturing = capture == hipStreamCaptureStatusActive;
+    const int pixels = width * height;
+    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 24);
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) == hipSuccess) {
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    const int persistentBlocks = cus * 8;
+    const int spillThreads = persistentBlocks * kWfBlock;
+    const int cap = std::max(pixels, std::min(target, samplesPerPixel * pixels));
+    const bool fits = gWs.mem != nullptr && gWs.capacity >= cap && gWs.pixels >= pixels &&
+                      gWs.depth >= maxPathDepth && gWs.spillThreads >= spillThreads;
+    if (!fits) {
+        if (capturing) {
+            ptSetError("pathTracerRenderFixedWavefront: workspace too small for graph capture "
+                       "(call pathTracerWavefrontReserve first)");
+            return false;
+        }
+        if (!ensure_workspace(cap, pixels, maxPathDepth, spillThreads)) {
+            return false;
+        }
+    }
+    gCapturing = capturing;
+    Paths P = gWs.P;
+    int* list = gWs.lists[0];
+    int* count = gWs.listCount;
+    const int counterInts = kCountersPerBounce * (maxPathDepth + 2);
+    const int group = std::max(1, std::min(samplesPerPixel, std::min(gWs.capacity, target) / pixels));
+    hipLaunchKernelGGL(wf_iota_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, pixels);
+    for (int passBase = 0; passBase < samplesPerPixel; passBase += group) {
+        const int g = std::min(group, samplesPerPixel - passBase);
+        hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
+                           counterInts);
+        hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, d_samples,
+                           d_converged, d_camera, d_params, width, height, 1u, passBase, samplesPerPixel, g, P);
+        launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, static_cast<long long>(g) * pixels,
+                       persistentBlocks);
+        hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, P,
+                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
+    }
+    gCapturing = false;
+    if (hipGetLastError() != hipSuccess) {
+        ptSetError("pathTracerRenderFixedWavefront: launch failed");
+        return false;
+    }
     return true;
 }

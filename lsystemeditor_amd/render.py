@@ -234,3 +234,15 @@ def count_traversal(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, ac
         "pathTracerCountTraversal")
     torch.cuda.synchronize()
     return out.cpu().numpy().view(np.uint32)
+
+
+def render_fixed(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, spp: int, max_paths: int = 0,
+                 clear: bool = True, stream=None) -> None:
+    """Fixed-spp wavefront render without host synchronisation (pathTracerRenderFixedWavefront);
+    capturable into a CUDA/HIP graph once the workspace is reserved."""
+    if clear:
+        acc.clear(stream)
+    abi.check(abi.lib().pathTracerRenderFixedWavefront(
+        acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
+        acc.width, acc.height, inputs.camera.data_ptr(), scene.device, inputs.env_ptr, inputs.params.data_ptr(),
+        spp, resolved_depth(inputs.params_host), max_paths, _stream_handle(stream)), "pathTracerRenderFixedWavefront")
