@@ -115,7 +115,7 @@ typedef struct {
                                            (lmin.xyz, lmax.x) (lmax.yz, rmin.xy) (rmin.z, rmax.xyz)
                                            (leftRef, rightRef, -, -) as int bits; ref < 0 is leaf slot ~ref */
     const PtFloat4* triPos;              /* 3 x float4 per leaf slot: (v0.xyz, original index bits),
-                                           (v1-v0, -), (v2-v0, -) */
+                                           (v1-v0, material index bits), (v2-v0, -) */
     const PtFloat4* triShade;            /* 4 x float4 per original triangle:
                                            (n0.xyz, n1.x) (n1.yz, n2.xy) (n2.z, uv0.xy, uv1.x)
                                            (uv1.y, uv2.xy, materialIndex bits) */

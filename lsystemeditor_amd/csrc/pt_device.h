@@ -687,15 +687,10 @@ PT_DEV float chan(const MeshAccelSceneGpu& s, float inl, uint32_t ti, float uvx,
 }
 
 // lightFetchMaterial + resolveMaterial + materialFromResolved
-PT_DEV Mat resolve_material(const MeshAccelSceneGpu& s, uint32_t tri, float uvx, float uvy)
+// Material `mi` resolved at uv (the fallback grey when !valid, LightSamplingCore.h:199-213).
+PT_DEV Mat resolve_material_index(const MeshAccelSceneGpu& s, uint32_t mi, bool valid, float uvx, float uvy)
 {
     MaterialGpu m;
-    bool valid = s.materials != nullptr && tri < s.triangleCount;
-    uint32_t mi = 0;
-    if (valid) {
-        mi = tri_material(s, tri);
-        valid = mi < s.materialCount;
-    }
     if (valid) {
         m = s.materials[mi];
     } else {
@@ -738,6 +733,17 @@ PT_DEV Mat resolve_material(const MeshAccelSceneGpu& s, uint32_t tri, float uvx,
     r.srB = fmaxf(0.0f, m.scatterRadiusB);
     r.specular = tex_clamp01(m.specular);
     return r;
+}
+
+PT_DEV Mat resolve_material(const MeshAccelSceneGpu& s, uint32_t tri, float uvx, float uvy)
+{
+    bool valid = s.materials != nullptr && tri < s.triangleCount;
+    uint32_t mi = 0;
+    if (valid) {
+        mi = tri_material(s, tri);
+        valid = mi < s.materialCount;
+    }
+    return resolve_material_index(s, mi, valid, uvx, uvy);
 }
 
 // ----------------------------------------------------------------------------------------

@@ -224,7 +224,7 @@ __global__ void seg_setup_kernel(const SegRec* __restrict__ segs, int segCount, 
         const int tri = perm[slot];
         const float* t = tris + 25 * static_cast<size_t>(tri);
         triPos[3 * slot + 0] = make_float4(t[0], t[1], t[2], __int_as_float(tri));
-        triPos[3 * slot + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], 0.0f);
+        triPos[3 * slot + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], t[24]); // .w: material
         triPos[3 * slot + 2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.0f);
         const int ref = ~slot;
         if (sg.parent < 0) {

@@ -235,9 +235,10 @@ __global__ void seg_kernel(const uint32_t* __restrict__ segBounds, int d, int n,
         rec.triStart = static_cast<uint32_t>(tri);
         rec.triCount = 1u;
         const float* t = tris + 25 * static_cast<size_t>(tri);
-        // (v0, original index) (v1 - v0) (v2 - v0): the edges the reference computes per test
+        // (v0, original index) (v1 - v0, material index) (v2 - v0): the edges the reference
+        // computes per test; the material index rides along so a hit record can carry it
         triPos[3 * s.begin + 0] = make_float4(t[0], t[1], t[2], __int_as_float(tri));
-        triPos[3 * s.begin + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], 0.0f);
+        triPos[3 * s.begin + 1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], t[24]); // .w: material
         triPos[3 * s.begin + 2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.0f);
         ref = ~s.begin;
     } else {

@@ -42,13 +42,17 @@ enum : uint32_t {
 struct Paths {
     float *Tx, *Ty, *Tz, *Lx, *Ly, *Lz, *eta, *pdf;
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmin;
-    float *hitT, *hitU, *hitV;
     float *cex, *cey, *cez, *cmx, *cmy, *cmz;
     float *rx, *ry, *rz;
-    uint32_t *pixel, *sample, *seed, *draws, *hitTri, *flags;
+    uint32_t *pixel, *sample, *seed, *draws, *flags;
+    // Hit records in shade-queue order, double-buffered by bounce parity: (path, triangle or -1,
+    // material or -1, t bits) and (u, v). The shade kernel reads its whole input with one coalesced
+    // load; the extension trace writes each ray's result at the path's shade slot.
+    int4* hitA[2];
+    float2* hitB[2];
+    int* shadeSlot; // per path: its index in the next shade queue
     uint8_t* occ;   // 2 per path: environment, emissive
     int* extQ[2];   // extension ray queues (path ids)
-    int* shadeQ[2]; // shade queues (path ids)
     float4* shadowQ; // 2 per ray: (origin, path*4 + type), (direction, -)
     int* counters;  // per bounce: [0] ext, [1] shade, [2] shadow rays produced, [3] paths (gen, bounce 0),
                     // [4] extension and [5] shadow fetch positions of the trace kernels
@@ -129,14 +133,15 @@ constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf
 // internal node (both child boxes tested, near child first, far child deferred on the LDS stack
 // with its entry distance), or a leaf triangle, followed by pops that skip deferred children whose
 // entry distance exceeds the current closest t. Visit order and culling equal pt::traverse.
-//   ANY_HIT = false: extension rays of the shade queue (path ids; identity when queue is null),
-//                    results to hitT/hitU/hitV/hitTri.
+//   ANY_HIT = false: extension rays of the extension queue (path ids; identity when queue is
+//                    null), results to the hit record at the path's shade slot.
 //   ANY_HIT = true:  shadow rays of shadowQ, result to occ.
 template <bool ANY_HIT, bool STATS>
 __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
                                                             const int* __restrict__ queue,
                                                             const int* __restrict__ countPtr, int* fetchCounter,
-                                                            unsigned long long* stats)
+                                                            unsigned long long* stats, int4* __restrict__ hitA,
+                                                            float2* __restrict__ hitB)
 {
     __shared__ int2 ring[kLdsEntries * kWfBlock];
     const MeshAccelSceneGpu s = *scene;
@@ -151,6 +156,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     int item = -1;        // queue index of the lane's current ray
     bool exhausted = false;
     uint32_t tag = 0u;    // path id (extension) or path * 4 + type (shadow)
+    int slot = 0;         // extension: the path's shade slot
+    int bestMat = -1;
     Ray r;
     r.o = mk(0.0f, 0.0f, 0.0f);
     r.d = r.o;
@@ -183,11 +190,13 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     } else {
                         const int p = queue != nullptr ? queue[idx] : idx;
                         tag = static_cast<uint32_t>(p);
+                        slot = P.shadeSlot[p];
                         r = make_ray(mk(P.ox[p], P.oy[p], P.oz[p]), mk(P.dx[p], P.dy[p], P.dz[p]));
                         tMin = P.tmin[p];
                     }
                     bestT = kRayTMax;
                     bestTri = 0xFFFFFFFFu;
+                    bestMat = -1;
                     bestU = 0.0f;
                     bestV = 0.0f;
                     st.sp = 0;
@@ -234,6 +243,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     bestU = u;
                     bestV = v;
                     bestTri = __float_as_uint(a.w);
+                    bestMat = __float_as_int(b.w);
                     if (ANY_HIT && t < kRayTMax) {
                         finished = true;
                     }
@@ -283,10 +293,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
             if (ANY_HIT) {
                 P.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
             } else {
-                P.hitT[tag] = hit ? bestT : 0.0f;
-                P.hitU[tag] = bestU;
-                P.hitV[tag] = bestV;
-                P.hitTri[tag] = hit ? bestTri : 0xFFFFFFFFu;
+                hitA[slot] = make_int4(static_cast<int>(tag), hit ? static_cast<int>(bestTri) : -1, hit ? bestMat : -1,
+                                       __float_as_int(hit ? bestT : 0.0f));
+                hitB[slot] = make_float2(bestU, bestV);
             }
             item = -1;
         }
@@ -399,6 +408,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         P.eta[p] = 1.0f;
         P.pdf[p] = 0.0f;
         P.flags[p] = kFlagExt;
+        P.shadeSlot[p] = p; // bounce 0 shades in path order
     }
 }
 
@@ -418,9 +428,10 @@ PT_DEV void push_shadow(Paths& P, int* counter, bool want, V3 o, V3 d, int path,
 __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const MeshAccelSceneGpu* scene,
                                                             const RenderParamsGpu* params,
                                                             const EnvironmentMapGpu* env, Paths P, int bounce,
-                                                            int maxDepth, const int* __restrict__ inQueue,
+                                                            int maxDepth, const int4* __restrict__ inA,
+                                                            const float2* __restrict__ inB,
                                                             const int* __restrict__ inCountPtr, int* extOut,
-                                                            int* shadeOut, int* counters, float* capture,
+                                                            int4* __restrict__ outA, int* counters, float* capture,
                                                             int captureSpp)
 {
     const MeshAccelSceneGpu s = *scene;
@@ -433,7 +444,13 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
     for (int it = 0; it < trips; ++it) {
         const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
         const bool live = i < n;
-        const int p = live ? (inQueue != nullptr ? inQueue[i] : i) : 0;
+        int4 ha = make_int4(0, -1, -1, 0);
+        float2 hb = make_float2(0.0f, 0.0f);
+        if (live) {
+            ha = inA[i];
+            hb = inB[i];
+        }
+        const int p = ha.x;
         bool wantExt = false, wantEnv = false, wantEmis = false;
         V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO;
         if (live) {
@@ -450,7 +467,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
             if ((flags & kFlagExt) != 0u) {
                 const V3 origin = mk(P.ox[p], P.oy[p], P.oz[p]);
                 const V3 dir = mk(P.dx[p], P.dy[p], P.dz[p]);
-                const uint32_t tri = P.hitTri[p];
+                const uint32_t tri = static_cast<uint32_t>(ha.y);
                 if (tri == 0xFFFFFFFFu) {
                     if (bounce == 0) {
                         L = env_eval(env, prm, dir);  // PathIntegratorRandCore.h:215-217
@@ -462,12 +479,12 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                         L = add(L, mul(T, scale(envL, mis)));
                     }
                 } else if (bounce < maxDepth) {
-                    const float t = P.hitT[p];
+                    const float t = __int_as_float(ha.w);
                     const Ray ray = make_ray(origin, dir);
                     Hit h;
                     h.hit = true;
                     h.t = t;
-                    finish_hit(s, ray, tri, P.hitU[p], P.hitV[p], h);
+                    finish_hit(s, ray, tri, hb.x, hb.y, h);
                     Rng rng;
                     rng_resume(rng, static_cast<int>(P.pixel[p]), static_cast<int>(P.sample[p]), P.seed[p],
                                P.draws[p]);
@@ -476,7 +493,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     Ctx c;
                     c.n = h.normal;
                     c.wo = scale(dir, -1.0f);
-                    c.m = resolve_material(s, h.tri, h.uvx, h.uvy);
+                    const uint32_t mi = static_cast<uint32_t>(ha.z);
+                    c.m = resolve_material_index(s, mi, s.materials != nullptr && tri < s.triangleCount &&
+                                                            mi < s.materialCount, h.uvx, h.uvy);
                     c.w = lobe_weights(c.m);
                     c.eta = P.eta[p];
                     c.flags = 0;
@@ -627,7 +646,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
         }
         const int sh = wave_append(&counters[1], wantExt || wantEnv || wantEmis);
         if (wantExt || wantEnv || wantEmis) {
-            shadeOut[sh] = p;
+            // next bounce's record; the extension trace fills in its hit
+            outA[sh] = make_int4(p, -1, -1, 0);
+            P.shadeSlot[p] = sh;
         }
         push_shadow(P, &counters[2], wantEnv, envO, envD, p, 1u);
         push_shadow(P, &counters[2], wantEmis, emO, emD, p, 2u);
@@ -819,20 +840,22 @@ size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spil
     Paths& P = w.P;
     const size_t c = static_cast<size_t>(cap);
     float** fs[] = {&P.Tx, &P.Ty, &P.Tz, &P.Lx, &P.Ly, &P.Lz, &P.eta, &P.pdf, &P.ox, &P.oy, &P.oz, &P.dx, &P.dy,
-                    &P.dz, &P.tmin, &P.hitT, &P.hitU, &P.hitV, &P.cex, &P.cey, &P.cez, &P.cmx, &P.cmy, &P.cmz,
-                    &P.rx, &P.ry, &P.rz};
+                    &P.dz, &P.tmin, &P.cex, &P.cey, &P.cez, &P.cmx, &P.cmy, &P.cmz, &P.rx, &P.ry, &P.rz};
     for (float** f : fs) {
         *f = carve<float>(cur, c);
     }
-    uint32_t** us[] = {&P.pixel, &P.sample, &P.seed, &P.draws, &P.hitTri, &P.flags};
+    uint32_t** us[] = {&P.pixel, &P.sample, &P.seed, &P.draws, &P.flags};
     for (uint32_t** u : us) {
         *u = carve<uint32_t>(cur, c);
     }
     P.occ = carve<uint8_t>(cur, 2 * c);
     P.extQ[0] = carve<int>(cur, c);
     P.extQ[1] = carve<int>(cur, c);
-    P.shadeQ[0] = carve<int>(cur, c);
-    P.shadeQ[1] = carve<int>(cur, c);
+    P.hitA[0] = carve<int4>(cur, c);
+    P.hitA[1] = carve<int4>(cur, c);
+    P.hitB[0] = carve<float2>(cur, c);
+    P.hitB[1] = carve<float2>(cur, c);
+    P.shadeSlot = carve<int>(cur, c);
     P.shadowQ = carve<float4>(cur, 4 * c);
     P.counters = carve<int>(cur, kCountersPerBounce * static_cast<size_t>(depth + 2));
     P.pxBase = carve<int>(cur, static_cast<size_t>(pixels));
@@ -899,29 +922,28 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
         prof_mark(stream, 0, false, b);
         if (gProf.counted) {
             hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats);
+                               d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats, P.hitA[b & 1], P.hitB[b & 1]);
         } else {
             hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, extQueue, extCount, &cIn[4], nullptr);
+                               d_scene, P, extQueue, extCount, &cIn[4], nullptr, P.hitA[b & 1], P.hitB[b & 1]);
         }
         prof_mark(stream, 0, true, b);
         if (b > 0) {
             prof_mark(stream, 1, false, b);
             if (gProf.counted) {
                 hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats);
+                                   d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats, nullptr, nullptr);
             } else {
                 hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr);
+                                   d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr, nullptr, nullptr);
             }
             prof_mark(stream, 1, true, b);
         }
-        const int* shadeQueue = b == 0 ? nullptr : P.shadeQ[b & 1];
         const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
         prof_mark(stream, 2, false, b);
         hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
-                           P, b, maxPathDepth, shadeQueue, shadeCount, P.extQ[(b + 1) & 1],
-                           P.shadeQ[(b + 1) & 1], cOut, gCapture, gCaptureSpp);
+                           P, b, maxPathDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
+                           P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp);
         prof_mark(stream, 2, true, b);
     }
 }
