@@ -186,8 +186,37 @@ bool pathTracerUpsamplePreviewToPbo(const PtFloat4* preview, int previewWidth, i
 bool pathTracerWriteUvDebugToPbo(const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene, PtUchar4* pbo,
                                  int width, int height, const RenderParamsGpu* d_params, hipStream_t stream);
 
-/* PathTracer.cu:27-33. Stores the spectral model for kernels that read it. */
+/* PathTracer.cu:27-33. Stores the spectral model (device tables + white normalisation) that
+ * the launchers pass to their kernels when hero-wavelength sampling is on. The tables must stay
+ * allocated while it is stored. */
 hipError_t spectralUploadStateToSymbol(const SpectralStateGpu* hostState);
+
+/* ---- spectral state (role of SpectralStateHost, Spectral/SpectralState.cpp:15-141) --------- */
+
+typedef struct PtSpectral PtSpectral;
+
+/* White normalisation of an rgb2spec table: the linear-sRGB response of the table's (1,1,1)
+ * spectrum summed over 360..830 nm in 1-nm steps (SpectralState.cpp:15-46). Host only. */
+bool ptSpectralWhiteNorm(const float* scale, const float* data, int res, float* out3);
+/* Host tables (scale[res], data[3*3*res^3]) -> device copies + white normalisation. */
+PtSpectral* ptSpectralCreate(const float* scale, const float* data, int res);
+/* SpectralState.cpp:48-103: reads a "SPEC" coefficient file, then as ptSpectralCreate. */
+PtSpectral* ptSpectralLoadCoeffFile(const char* path);
+/* The SpectralStateGpu to pass to spectralUploadStateToSymbol. */
+bool ptSpectralGetState(const PtSpectral* spectral, SpectralStateGpu* out);
+/* Frees the device tables; if they are the stored state, the library returns to RGB rendering. */
+void ptSpectralDestroy(PtSpectral* spectral);
+
+/* Hero-wavelength rendering (extension; the reference's integrator is RGB): with
+ * wavelengthsPerSample = 4 every launcher and wavefront render samples four wavelengths per
+ * path sample, lambda_k = lambdaMin + frac(u + k/4) (lambdaMax - lambdaMin), from one extra
+ * uniform draw after the lens draws; colours are upsampled with the stored rgb2spec table and
+ * each sample is converted to linear sRGB (CIE 1931 x D65, divided by the white response and
+ * the wavelength pdf) before the firefly clamp, which keeps negative channels in this mode.
+ * 0 returns to RGB. Needs 360 <= lambdaMin < lambdaMax <= 830 and a stored table. Process-wide
+ * setting; change it only between renders. The adjoint entry points reject spectral mode. */
+bool pathTracerSetSpectralSampling(int wavelengthsPerSample, float lambdaMinNm, float lambdaMaxNm);
+int pathTracerSpectralSampling(void);
 
 /* ---- extensions of the launcher API (no reference counterpart) -------------------------- */
 

@@ -56,6 +56,13 @@ SIGNATURES = {
     "pathTracerUpsamplePreviewToPbo": (_b, [_p, _i, _i, _i, _p, _i, _i, _p, _f, _p]),
     "pathTracerWriteUvDebugToPbo": (_b, [_p, _p, _p, _i, _i, _p, _p]),
     "spectralUploadStateToSymbol": (_i, [_p]),
+    "ptSpectralWhiteNorm": (_b, [_p, _p, _i, _p]),
+    "ptSpectralCreate": (_p, [_p, _p, _i]),
+    "ptSpectralLoadCoeffFile": (_p, [C.c_char_p]),
+    "ptSpectralGetState": (_b, [_p, _p]),
+    "ptSpectralDestroy": (None, [_p]),
+    "pathTracerSetSpectralSampling": (_b, [_i, _f, _f]),
+    "pathTracerSpectralSampling": (_i, []),
     "pathTracerAdaptiveRenderPasses": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p]),
     "pathTracerAdaptiveRenderPassesCounted": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p,
                                                    _p]),
@@ -381,3 +388,55 @@ class Environment:
             self.close()
         except Exception:
             pass
+
+
+class SpectralStateGpu(C.Structure):
+    """Spectral/SpectralState.h:11-19 (device table pointers and white normalisation)."""
+    _fields_ = [("rgb2specScale", C.c_void_p), ("rgb2specData", C.c_void_p), ("rgb2specRes", C.c_int),
+                ("whiteNormR", C.c_float), ("whiteNormG", C.c_float), ("whiteNormB", C.c_float)]
+
+
+class Spectral:
+    """An rgb2spec coefficient table on the GPU (ptSpectralCreate), the role of SpectralStateHost."""
+
+    def __init__(self, scale: np.ndarray, data: np.ndarray):
+        self.scale = np.ascontiguousarray(scale, dtype=np.float32)
+        self.data = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+        self.res = int(self.scale.shape[0])
+        if self.data.size != 9 * self.res ** 3:
+            raise ValueError("data must hold 3 * 3 * res^3 coefficients")
+        self.handle = lib().ptSpectralCreate(self.scale.ctypes.data, self.data.ctypes.data, self.res)
+        if not self.handle:
+            raise RuntimeError(f"ptSpectralCreate failed: {last_error()}")
+
+    def state(self) -> SpectralStateGpu:
+        st = SpectralStateGpu()
+        check(lib().ptSpectralGetState(self.handle, C.byref(st)), "ptSpectralGetState")
+        return st
+
+    def white_norm(self) -> tuple[float, float, float]:
+        st = self.state()
+        return st.whiteNormR, st.whiteNormG, st.whiteNormB
+
+    def enable(self, lambda_min: float = 360.0, lambda_max: float = 830.0) -> None:
+        """Stores this table (spectralUploadStateToSymbol) and turns hero-wavelength sampling on."""
+        st = self.state()
+        if lib().spectralUploadStateToSymbol(C.byref(st)) != 0:
+            raise RuntimeError("spectralUploadStateToSymbol failed")
+        check(lib().pathTracerSetSpectralSampling(4, lambda_min, lambda_max), "pathTracerSetSpectralSampling")
+
+    @staticmethod
+    def disable() -> None:
+        check(lib().pathTracerSetSpectralSampling(0, 0.0, 0.0), "pathTracerSetSpectralSampling")
+
+    def close(self):
+        if self.handle:
+            lib().ptSpectralDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

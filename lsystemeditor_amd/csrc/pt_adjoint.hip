@@ -578,6 +578,10 @@ extern "C" bool pathTracerDetachedRender(float* d_image, int width, int height, 
         ptSetError("pathTracerDetachedRender: invalid arguments");
         return false;
     }
+    if (pt_spectral_enabled()) {
+        ptSetError("pathTracerDetachedRender: RGB rendering only (hero-wavelength sampling is on)");
+        return false;
+    }
     const long long n = static_cast<long long>(width) * height;
     hipLaunchKernelGGL(detached_kernel, dim3(static_cast<unsigned>((n + kAdBlock - 1) / kAdBlock)), dim3(kAdBlock), 0,
                        stream, d_image, width, height, samplesPerPixel, d_camera, d_scene, d_evalMaterials, d_env,
@@ -599,6 +603,10 @@ extern "C" bool pathTracerAdjointRender(const float* d_gradImage, int width, int
         d_grad == nullptr || width <= 0 || height <= 0 || samplesPerPixel <= 0 || materialCount <= 0 ||
         materialCount > kMaxGradMaterials) {
         ptSetError("pathTracerAdjointRender: invalid arguments (materialCount must be 1..%d)", kMaxGradMaterials);
+        return false;
+    }
+    if (pt_spectral_enabled()) {
+        ptSetError("pathTracerAdjointRender: RGB rendering only (hero-wavelength sampling is on)");
         return false;
     }
     int device = 0, cus = 256;

@@ -42,6 +42,51 @@ PT_DEV V3 normalize(V3 v)
     const float inv = 1.0f / len;
     return scale(v, inv);
 }
+// Per-wavelength values of the four hero wavelengths of a spectral sample. Colour code that is
+// generic over RGB (V3) and spectral (S4) values works per component (cget/cset).
+struct S4 {
+    float x, y, z, w;
+};
+
+PT_DEV S4 mk4(float x, float y, float z, float w) { return S4{x, y, z, w}; }
+PT_DEV S4 add(S4 a, S4 b) { return mk4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+PT_DEV S4 scale(S4 v, float s) { return mk4(v.x * s, v.y * s, v.z * s, v.w * s); }
+PT_DEV S4 mul(S4 a, S4 b) { return mk4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+
+template <class C> struct ColN;
+template <> struct ColN<V3> {
+    static constexpr int n = 3;
+    PT_DEV static V3 splat(float v) { return mk(v, v, v); }
+};
+template <> struct ColN<S4> {
+    static constexpr int n = 4;
+    PT_DEV static S4 splat(float v) { return mk4(v, v, v, v); }
+};
+PT_DEV float cget(const V3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+PT_DEV float cget(const S4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+PT_DEV void cset(V3& v, int k, float f)
+{
+    if (k == 0) {
+        v.x = f;
+    } else if (k == 1) {
+        v.y = f;
+    } else {
+        v.z = f;
+    }
+}
+PT_DEV void cset(S4& v, int k, float f)
+{
+    if (k == 0) {
+        v.x = f;
+    } else if (k == 1) {
+        v.y = f;
+    } else if (k == 2) {
+        v.z = f;
+    } else {
+        v.w = f;
+    }
+}
+
 // MathCore.h:68-92 (comparison forms, not fmaxf/fminf)
 PT_DEV float max2(float a, float b) { return a > b ? a : b; }
 PT_DEV float min2(float a, float b) { return a < b ? a : b; }
@@ -986,29 +1031,35 @@ PT_DEV bool transmit_valid(const Ctx& c, float cosWoS)
     return cosWoS > 0.0f || c.eta > 1.0f + 1.0e-4f;
 }
 
-PT_DEV V3 eval_diffuse(const Ctx& c, V3 wi)
+// The lobe evaluations are generic over the colour type: `base` is the base colour (RGB, or
+// its spectrum at the hero wavelengths) and `rad` the subsurface radii; every colour operation
+// is per component and in the reference's order, so the RGB instantiation is the reference's
+// arithmetic.
+template <class C>
+PT_DEV C eval_diffuse_t(const Ctx& c, V3 wi, const C& base)
 {
     const float cosWi = max2(0.0f, dot(c.n, wi));
     const float cosWo = max2(0.0f, dot(c.n, c.wo));
     if (cosWi <= 0.0f || cosWo <= 0.0f) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
     const float on = on_factor(c, wi);
-    return scale(mk(c.m.r, c.m.g, c.m.b), c.w.diffuse * on * kInvPi);
+    return scale(base, c.w.diffuse * on * kInvPi);
 }
 
-PT_DEV V3 eval_specular(const Ctx& c, V3 wi)
+template <class C>
+PT_DEV C eval_specular_t(const Ctx& c, V3 wi, const C& base)
 {
     const float cosWo = max2(0.0f, dot(c.n, c.wo));
     const float cosWi = max2(0.0f, dot(c.n, wi));
     if (cosWo <= 0.0f || cosWi <= 0.0f) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
     const V3 h = normalize(add(wi, c.wo));
     const float cosH = max2(0.0f, dot(c.n, h));
     const float cosHo = max2(0.0f, dot(h, c.wo));
     if (cosHo <= 0.0f) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
     const float alpha = alpha_of(c.m.roughness);
     const float D = ggx_d(cosH, alpha);
@@ -1018,25 +1069,29 @@ PT_DEV V3 eval_specular(const Ctx& c, V3 wi)
     const float ior = material_ior(c);
     const float f0s = (ior - 1.0f) / (ior + 1.0f);
     const float f0v = f0s * f0s * clamp01(clamp01(c.m.specular));
-    const V3 f0 = mk(f0v + (c.m.r - f0v) * metallic, f0v + (c.m.g - f0v) * metallic, f0v + (c.m.b - f0v) * metallic);
+    C f0;
+#pragma unroll
+    for (int k = 0; k < ColN<C>::n; ++k) {
+        cset(f0, k, f0v + (cget(base, k) - f0v) * metallic);
+    }
     const float t = max2(0.0f, min2(1.0f, 1.0f - cosHo));
     const float t2 = t * t;
     const float t5 = t2 * t2 * t;
-    const V3 F = add(scale(f0, 1.0f - t5), mk(t5, t5, t5));
+    const C F = add(scale(f0, 1.0f - t5), ColN<C>::splat(t5));
     const float ec = energy_comp(c.m.roughness, cosWo);
     const float denom = max2(4.0f * cosWo * cosWi, 1.0e-8f);
     const float sc = c.w.specular * ec * (D * G) / denom;
-    return mk(F.x * sc, F.y * sc, F.z * sc);
+    return scale(F, sc);
 }
 
-PT_DEV V3 eval_transmit(const Ctx& c, V3 wi)
+template <class C>
+PT_DEV C eval_transmit_t(const Ctx& c, V3 wi, const C& base)
 {
     const float cosWoS = dot(c.n, c.wo);
     if (!transmit_valid(c, cosWoS)) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
     const float thin = clamp01(c.m.thin);
-    const V3 base = mk(c.m.r, c.m.g, c.m.b);
     const float ior = material_ior(c);
     const float etaT = cosWoS > 0.0f ? ior : 1.0f;
     const float refl = dielectric_fresnel(cosWoS, c.eta, etaT);
@@ -1051,14 +1106,17 @@ PT_DEV V3 eval_transmit(const Ctx& c, V3 wi)
     if (thin > 0.5f) {
         tr = normalize(scale(c.wo, -1.0f));
     } else if (!refract_dir(c, c.n, tr, next)) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
     if (dot(wi, tr) > 0.999f) {
         const float cosWi = absf(dot(c.n, tr));
-        const V3 tint = thin > 0.5f ? base : mk(1.0f, 1.0f, 1.0f);
-        return mk(tint.x * c.w.transmit * trans / max2(cosWi, 1.0e-8f),
-                  tint.y * c.w.transmit * trans / max2(cosWi, 1.0e-8f),
-                  tint.z * c.w.transmit * trans / max2(cosWi, 1.0e-8f));
+        const C tint = thin > 0.5f ? base : ColN<C>::splat(1.0f);
+        C out;
+#pragma unroll
+        for (int k = 0; k < ColN<C>::n; ++k) {
+            cset(out, k, cget(tint, k) * c.w.transmit * trans / max2(cosWi, 1.0e-8f));
+        }
+        return out;
     }
     const float alpha = alpha_of(c.m.roughness);
     if (alpha > 0.04f) {
@@ -1067,50 +1125,89 @@ PT_DEV V3 eval_transmit(const Ctx& c, V3 wi)
         const float D = ggx_d(cosH, alpha);
         const float roughPdf = D * cosH;
         if (roughPdf > kMinPdf) {
-            const V3 tint = thin > 0.5f ? base : mk(1.0f, 1.0f, 1.0f);
+            const C tint = thin > 0.5f ? base : ColN<C>::splat(1.0f);
             const float sc = c.w.transmit * trans * roughPdf;
-            return mk(tint.x * sc, tint.y * sc, tint.z * sc);
+            return scale(tint, sc);
         }
     }
-    return mk(0.0f, 0.0f, 0.0f);
+    return ColN<C>::splat(0.0f);
 }
 
-PT_DEV V3 eval_subsurface(const Ctx& c, V3 wi)
+// Subsurface lobe. RGB: the reference's per-channel Burley profiles. Hero wavelengths: each
+// wavelength is evaluated as a grey material whose scatter radius is that wavelength's radius.
+template <class C>
+PT_DEV C eval_subsurface_t(const Ctx& c, V3 wi, const C& base, const C& rad)
 {
     const float cosWi = max2(0.0f, dot(c.n, wi));
     const float cosWo = max2(0.0f, dot(c.n, c.wo));
     if (cosWi <= 0.0f || cosWo <= 0.0f) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
-    const V3 base = mk(c.m.r, c.m.g, c.m.b);
-    const float mr = max_radius(c.m);
-    if (mr <= kMinRadius) {
-        const float on = on_factor(c, wi);
-        return scale(base, c.w.subsurface * on * kInvPi);
+    if constexpr (ColN<C>::n == 3) {
+        const float mr = max_radius(c.m);
+        if (mr <= kMinRadius) {
+            const float on = on_factor(c, wi);
+            return scale(base, c.w.subsurface * on * kInvPi);
+        }
+        V3 t, b;
+        basis(c.n, t, b);
+        const V3 wt = add(scale(t, dot(wi, t)), scale(b, dot(wi, b)));
+        const float dist = sqrtf(dot(wt, wt));
+        const float sum = max2(rad.x + rad.y + rad.z, kMinRadius);
+        const V3 cw = mk(rad.x / sum, rad.y / sum, rad.z / sum);
+        const float pr = burley_profile(dist, max2(rad.x, kMinRadius));
+        const float pg = burley_profile(dist, max2(rad.y, kMinRadius));
+        const float pb = burley_profile(dist, max2(rad.z, kMinRadius));
+        const float dif = max2(cosWi, 0.0f) * max2(cosWo, 0.0f);
+        const V3 bur = mk(cw.x * pr * dif, cw.y * pg * dif, cw.z * pb * dif);
+        return mk(base.x * c.w.subsurface * bur.x, base.y * c.w.subsurface * bur.y,
+                  base.z * c.w.subsurface * bur.z);
+    } else {
+        V3 t, b;
+        basis(c.n, t, b);
+        const V3 wt = add(scale(t, dot(wi, t)), scale(b, dot(wi, b)));
+        const float dist = sqrtf(dot(wt, wt));
+        const float dif = max2(cosWi, 0.0f) * max2(cosWo, 0.0f);
+        float flat = 0.0f;
+        bool haveFlat = false;
+        C out;
+#pragma unroll
+        for (int k = 0; k < ColN<C>::n; ++k) {
+            const float rk = cget(rad, k);
+            if (rk <= kMinRadius) {
+                if (!haveFlat) {
+                    flat = c.w.subsurface * on_factor(c, wi) * kInvPi;
+                    haveFlat = true;
+                }
+                cset(out, k, cget(base, k) * flat);
+            } else {
+                const float sum = max2(rk + rk + rk, kMinRadius);
+                const float bur = rk / sum * burley_profile(dist, max2(rk, kMinRadius)) * dif;
+                cset(out, k, cget(base, k) * c.w.subsurface * bur);
+            }
+        }
+        return out;
     }
-    V3 t, b;
-    basis(c.n, t, b);
-    const V3 wt = add(scale(t, dot(wi, t)), scale(b, dot(wi, b)));
-    const float dist = sqrtf(dot(wt, wt));
-    const V3 rad = mk(max2(0.0f, c.m.srR), max2(0.0f, c.m.srG), max2(0.0f, c.m.srB));
-    const float sum = max2(rad.x + rad.y + rad.z, kMinRadius);
-    const V3 cw = mk(rad.x / sum, rad.y / sum, rad.z / sum);
-    const float pr = burley_profile(dist, max2(rad.x, kMinRadius));
-    const float pg = burley_profile(dist, max2(rad.y, kMinRadius));
-    const float pb = burley_profile(dist, max2(rad.z, kMinRadius));
-    const float dif = max2(cosWi, 0.0f) * max2(cosWo, 0.0f);
-    const V3 bur = mk(cw.x * pr * dif, cw.y * pg * dif, cw.z * pb * dif);
-    return mk(base.x * c.w.subsurface * bur.x, base.y * c.w.subsurface * bur.y, base.z * c.w.subsurface * bur.z);
 }
 
-PT_DEV V3 brdf_eval(const Ctx& c, V3 wi)
+template <class C>
+PT_DEV C brdf_eval_t(const Ctx& c, V3 wi, const C& base, const C& rad)
 {
-    const V3 d = eval_diffuse(c, wi);
-    const V3 s = eval_specular(c, wi);
-    const V3 t = c.w.transmit != 0.0f ? eval_transmit(c, wi) : mk(0.0f, 0.0f, 0.0f);
-    const V3 ss = c.w.subsurface != 0.0f ? eval_subsurface(c, wi) : mk(0.0f, 0.0f, 0.0f);
+    const C d = eval_diffuse_t(c, wi, base);
+    const C s = eval_specular_t(c, wi, base);
+    const C t = c.w.transmit != 0.0f ? eval_transmit_t(c, wi, base) : ColN<C>::splat(0.0f);
+    const C ss = c.w.subsurface != 0.0f ? eval_subsurface_t(c, wi, base, rad) : ColN<C>::splat(0.0f);
     return add(add(d, s), add(t, ss));
 }
+
+PT_DEV V3 mat_base(const Mat& m) { return mk(m.r, m.g, m.b); }
+PT_DEV V3 mat_radii(const Mat& m) { return mk(max2(0.0f, m.srR), max2(0.0f, m.srG), max2(0.0f, m.srB)); }
+
+PT_DEV V3 eval_diffuse(const Ctx& c, V3 wi) { return eval_diffuse_t(c, wi, mat_base(c.m)); }
+PT_DEV V3 eval_specular(const Ctx& c, V3 wi) { return eval_specular_t(c, wi, mat_base(c.m)); }
+PT_DEV V3 eval_transmit(const Ctx& c, V3 wi) { return eval_transmit_t(c, wi, mat_base(c.m)); }
+PT_DEV V3 eval_subsurface(const Ctx& c, V3 wi) { return eval_subsurface_t(c, wi, mat_base(c.m), mat_radii(c.m)); }
+PT_DEV V3 brdf_eval(const Ctx& c, V3 wi) { return brdf_eval_t(c, wi, mat_base(c.m), mat_radii(c.m)); }
 
 PT_DEV float pdf_specular(const Ctx& c, V3 wi)
 {
@@ -1362,21 +1459,22 @@ PT_DEV Sample brdf_sample(const Ctx& c, float u1, float u2)
 }
 
 // BrdfDispatch.h:42-76
-PT_DEV V3 apply_throughput(V3 T, const Ctx& c, const Sample& s, V3 f)
+template <class C>
+PT_DEV C apply_throughput_t(C T, const Ctx& c, const Sample& s, C f, const C& base)
 {
     if (!s.valid || s.pdf <= kMinPdf) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return ColN<C>::splat(0.0f);
     }
     const float tr = max2(0.0f, min2(1.0f, c.m.transmission));
     const float thin = max2(0.0f, min2(1.0f, c.m.thin));
     const bool pure = tr > 0.99f && c.m.metallic < 0.01f;
-    V3 sc;
+    C sc;
     if (pure && thin < 0.5f) {
         sc = scale(f, 1.0f / s.pdf);
     } else if (s.transmitted) {
         sc = scale(f, absf(dot(c.n, s.dir)) / s.pdf);
         if (thin > 0.5f) {
-            sc = mk(sc.x * c.m.r, sc.y * c.m.g, sc.z * c.m.b);
+            sc = mul(sc, base);
         }
     } else if (s.sss) {
         sc = scale(f, max2(0.0f, dot(c.n, s.dir)) / s.pdf);
@@ -1384,7 +1482,12 @@ PT_DEV V3 apply_throughput(V3 T, const Ctx& c, const Sample& s, V3 f)
         const float cs = c.eta > 1.0f + 1.0e-4f ? absf(dot(c.n, s.dir)) : max2(0.0f, dot(c.n, s.dir));
         sc = scale(f, cs / s.pdf);
     }
-    return mk(T.x * sc.x, T.y * sc.y, T.z * sc.z);
+    return mul(T, sc);
+}
+
+PT_DEV V3 apply_throughput(V3 T, const Ctx& c, const Sample& s, V3 f)
+{
+    return apply_throughput_t(T, c, s, f, mat_base(c.m));
 }
 
 PT_DEV bool skips_env_nee(const Mat& m)

@@ -1,36 +1,42 @@
 // pt_integrator.h -- the per-sample path integrator shared by the megakernel launchers and the
-// adjoint (PathIntegratorRandCore.h:65-363).
+// adjoint (PathIntegratorRandCore.h:65-363). Generic over the colour policy (pt_spectral.h):
+// RgbColors is the reference's RGB integrator, HeroColors the hero-wavelength spectral one.
 #pragma once
 
 #include "pt_device.h"
+#include "pt_spectral.h"
 
 namespace pt {
 
 // PathIntegratorRandCore.h:65-108. Returns the NEE term (before multiplication by throughput).
-PT_DEV V3 env_nee(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const EnvironmentMapGpu* env, V3 pos,
-                  const Ctx& c, Rng& rng, TraceStats* st)
+template <class Pol>
+PT_DEV typename Pol::Col env_nee_t(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const EnvironmentMapGpu* env,
+                                   V3 pos, const Ctx& c, const typename Pol::Col& base,
+                                   const typename Pol::Col& rad, const Pol& pol, Rng& rng, TraceStats* st)
 {
+    using C = typename Pol::Col;
+    const C zero = ColN<C>::splat(0.0f);
     if (skips_env_nee(c.m)) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const float u1 = rand01(rng);
     const float u2 = rand01(rng);
     float lightPdf = 0.0f;
     const V3 wi = env_sample(env, u1, u2, lightPdf);
     if (lightPdf <= kMinPdf) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     // LightSamplingCore.h:187-197: origin offset along the normal, closest-hit -> any hit
     const V3 o = add(pos, scale(c.n, kShadowBias));
     if (trace_any(s, o, wi, kShadowBias, kRayTMax, st)) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const float cosT = max2(0.0f, dot(c.n, wi));
     if (cosT <= 0.0f) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
-    const V3 L = env_eval(env, p, wi);
-    const V3 f = brdf_eval(c, wi);
+    const C L = pol.radiance(env_eval(env, p, wi));
+    const C f = brdf_eval_t(c, wi, base, rad);
     const float bpdf = brdf_pdf(c, wi);
     const float mis = mis_balance(lightPdf, bpdf);
     const float sc = mis * cosT / lightPdf;
@@ -38,10 +44,15 @@ PT_DEV V3 env_nee(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const En
 }
 
 // PathIntegratorRandCore.h:110-167
-PT_DEV V3 emissive_nee(const MeshAccelSceneGpu& s, V3 pos, const Ctx& c, Rng& rng, TraceStats* st)
+template <class Pol>
+PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, const Ctx& c,
+                                        const typename Pol::Col& base, const typename Pol::Col& rad,
+                                        const Pol& pol, Rng& rng, TraceStats* st)
 {
+    using C = typename Pol::Col;
+    const C zero = ColN<C>::splat(0.0f);
     if (s.emissiveTriangleCount == 0) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const float uTri = rand01(rng);
     const float u1 = rand01(rng);
@@ -49,39 +60,40 @@ PT_DEV V3 emissive_nee(const MeshAccelSceneGpu& s, V3 pos, const Ctx& c, Rng& rn
     V3 lp, ln, lr;
     float areaPdf = 0.0f;
     if (!sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const V3 toL = sub(lp, pos);
     const float d2 = dot(toL, toL);
     if (d2 <= kMinPdf) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const V3 wi = normalize(toL);
     const float cosT = max2(0.0f, dot(c.n, wi));
     const float cosL = max2(0.0f, dot(ln, scale(wi, -1.0f)));
     if (cosT <= 0.0f || cosL <= 0.0f) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const float lightPdf = areaPdf * d2 / cosL;
     // The result is zero whenever lightPdf <= kMinPdf, occluded or not, so the shadow ray
     // (LightSamplingCore.h:360-369) is only traced when it can matter.
     if (lightPdf <= kMinPdf) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
     const V3 o = add(pos, scale(wi, kShadowBias));
     if (trace_any(s, o, wi, kShadowBias, kRayTMax, st)) {
-        return mk(0.0f, 0.0f, 0.0f);
+        return zero;
     }
-    const V3 f = brdf_eval(c, wi);
+    const C f = brdf_eval_t(c, wi, base, rad);
     const float bpdf = brdf_pdf(c, wi);
     const float mis = mis_balance(lightPdf, bpdf);
     const float sc = mis * cosT / lightPdf;
-    return mul(f, scale(lr, sc));
+    return mul(f, scale(pol.radiance(lr), sc));
 }
 
 // PathIntegratorRandCore.h:169-204 (the step direction is computed and discarded there;
 // it has no effect on the result and is not computed here)
-PT_DEV V3 subsurface_tier_b(V3 T, const Mat& m, const Sample& smp, Rng& rng)
+template <class C>
+PT_DEV C subsurface_tier_b_t(C T, const Mat& m, const C& albedo, const Sample& smp, Rng& rng)
 {
     if (!smp.sss || smp.steps <= 0) {
         return T;
@@ -90,7 +102,6 @@ PT_DEV V3 subsurface_tier_b(V3 T, const Mat& m, const Sample& smp, Rng& rng)
     if (mr <= kMinRadius) {
         return T;
     }
-    const V3 albedo = mk(m.r, m.g, m.b);
     for (int step = 0; step < smp.steps; ++step) {
         (void)rand01(rng);
         (void)rand01(rng);
@@ -107,12 +118,15 @@ PT_DEV V3 subsurface_tier_b(V3 T, const Mat& m, const Sample& smp, Rng& rng)
 }
 
 // PathIntegratorRandCore.h:206-363
-PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const EnvironmentMapGpu* env, V3 ro,
-                     V3 rd, Rng& rng, TraceStats* st)
+template <class Pol>
+PT_DEV typename Pol::Col trace_path_t(const MeshAccelSceneGpu& s, const RenderParamsGpu& p,
+                                      const EnvironmentMapGpu* env, V3 ro, V3 rd, const Pol& pol, Rng& rng,
+                                      TraceStats* st)
 {
+    using C = typename Pol::Col;
     Hit hit = trace_closest(s, ro, rd, 0.0f, kRayTMax, st);
     if (!hit.hit) {
-        return env_eval(env, p, rd);
+        return pol.radiance(env_eval(env, p, rd));
     }
     int maxDepth = 32;
     if (p.maxPathDepth > 0) {
@@ -123,8 +137,8 @@ PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const
     const int rrMin = p.russianRouletteMinDepth >= 0 ? p.russianRouletteMinDepth : 3;
     const int flags = p.brdfDebugFlags;
 
-    V3 L = mk(0.0f, 0.0f, 0.0f);
-    V3 T = mk(1.0f, 1.0f, 1.0f);
+    C L = ColN<C>::splat(0.0f);
+    C T = ColN<C>::splat(1.0f);
     V3 origin = ro;
     V3 dir = rd;
     float eta = 1.0f;
@@ -138,9 +152,12 @@ PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const
         c.eta = eta;
         c.flags = 0;
 
-        L = add(L, mul(T, mk(c.m.r * c.m.emission, c.m.g * c.m.emission, c.m.b * c.m.emission)));
-        L = add(L, mul(T, env_nee(s, p, env, pos, c, rng, st)));
-        L = add(L, mul(T, emissive_nee(s, pos, c, rng, st)));
+        const C base = pol.base(c.m);
+        const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
+
+        L = add(L, mul(T, pol.emission(c.m)));
+        L = add(L, mul(T, env_nee_t(s, p, env, pos, c, base, rad, pol, rng, st)));
+        L = add(L, mul(T, emissive_nee_t(s, pos, c, base, rad, pol, rng, st)));
 
         c.flags = flags;
         const float u1 = rand01(rng);
@@ -149,13 +166,14 @@ PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const
         if (!smp.valid || smp.pdf <= kMinPdf) {
             break;
         }
-        const V3 f = brdf_eval(c, smp.dir);
-        T = apply_throughput(T, c, smp, f);
-        T = subsurface_tier_b(T, c.m, smp, rng);
+        const C f = brdf_eval_t(c, smp.dir, base, rad);
+        T = apply_throughput_t(T, c, smp, f, base);
+        T = subsurface_tier_b_t(T, c.m, base, smp, rng);
         if ((flags & 16) != 0) {
-            T = smp.transmitted ? mul(T, mk(0.2f, 1.0f, 0.2f)) : mul(T, mk(1.0f, 0.2f, 0.2f));
+            T = smp.transmitted ? mul(T, pol.reflectance(mk(0.2f, 1.0f, 0.2f)))
+                                : mul(T, pol.reflectance(mk(1.0f, 0.2f, 0.2f)));
         }
-        const float tl = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+        const float tl = pol.lum(T);
         if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
             break;
         }
@@ -173,7 +191,7 @@ PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const
         dir = smp.dir;
         hit = trace_closest(s, origin, dir, bias, kRayTMax, st);
         if (!hit.hit) {
-            const V3 envL = env_eval(env, p, dir);
+            const C envL = pol.radiance(env_eval(env, p, dir));
             const float lpdf = env_pdf(env, dir);
             const float mis = mis_balance(smp.pdf, lpdf);
             L = add(L, mul(T, scale(envL, mis)));
@@ -181,6 +199,28 @@ PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const
         }
     }
     return L;
+}
+
+PT_DEV V3 env_nee(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const EnvironmentMapGpu* env, V3 pos,
+                  const Ctx& c, Rng& rng, TraceStats* st)
+{
+    return env_nee_t(s, p, env, pos, c, mat_base(c.m), mat_radii(c.m), RgbColors{}, rng, st);
+}
+
+PT_DEV V3 emissive_nee(const MeshAccelSceneGpu& s, V3 pos, const Ctx& c, Rng& rng, TraceStats* st)
+{
+    return emissive_nee_t(s, pos, c, mat_base(c.m), mat_radii(c.m), RgbColors{}, rng, st);
+}
+
+PT_DEV V3 subsurface_tier_b(V3 T, const Mat& m, const Sample& smp, Rng& rng)
+{
+    return subsurface_tier_b_t(T, m, mat_base(m), smp, rng);
+}
+
+PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const EnvironmentMapGpu* env, V3 ro,
+                     V3 rd, Rng& rng, TraceStats* st)
+{
+    return trace_path_t(s, p, env, ro, rd, RgbColors{}, rng, st);
 }
 
 } // namespace pt

@@ -16,4 +16,11 @@ constexpr int kMaxSahDepth = 48;
 // box and the root reference (bits of an int in rootOut[6]); levelsOut the number of levels.
 bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
                       float rootOut[7], int* levelsOut);
+
+#include "pt_spectral.h"
+// Spectral sampling state set through the C ABI (pt_spectral.hip): whether the launchers
+// render hero wavelengths, and the model (coefficient table, white normalisation, wavelength
+// range) the kernels then read.
+bool pt_spectral_enabled();
+pt::SpecModel pt_spectral_model();
 #endif

@@ -15,10 +15,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// PathTracer.cu:69-105
+// PathTracer.cu:69-105. SPEC: hero-wavelength sample (one more draw after the lens draws picks
+// the four wavelengths; the spectral radiance is converted to linear sRGB before the clamp).
+template <bool SPEC>
 PT_DEV V3 pixel_radiance(int idx, int x, int y, int width, int height, bool jitter, int sampleIndex,
                          const CameraGpu& cam, const MeshAccelSceneGpu& s, const EnvironmentMapGpu* env,
-                         const RenderParamsGpu& p, uint32_t seed, TraceStats* st)
+                         const RenderParamsGpu& p, uint32_t seed, TraceStats* st, const SpecModel& sm)
 {
     Rng rng;
     rng_init(rng, idx, sampleIndex, seed);
@@ -30,7 +32,13 @@ PT_DEV V3 pixel_radiance(int idx, int x, int y, int width, int height, bool jitt
     const float l2 = rand01(rng);
     V3 ro, rd;
     camera_ray(cam, u, v, l1, l2, ro, rd);
-    return clamp_firefly(trace_path(s, p, env, ro, rd, rng, st));
+    if constexpr (SPEC) {
+        const HeroColors pol = hero_colors(sm, rand01(rng));
+        return clamp_firefly_spectral(pol.to_rgb(trace_path_t(s, p, env, ro, rd, pol, rng, st)));
+    } else {
+        (void)sm;
+        return clamp_firefly(trace_path(s, p, env, ro, rd, rng, st));
+    }
 }
 
 PT_DEV const EnvironmentMapGpu* env_or_null(const EnvironmentMapGpu* env) { return env; }
@@ -67,11 +75,12 @@ PT_DEV bool accumulate(uint32_t n, V3 r, float4& acc, float& mean, float& m2, co
 }
 
 // PathTracer.cu:186-273
+template <bool SPEC>
 __global__ __launch_bounds__(kBlock) void sample_adaptive_kernel(
     float4* __restrict__ acc, uint32_t* __restrict__ counts, float* __restrict__ lumMean, float* __restrict__ m2,
     uint8_t* __restrict__ converged, const int* __restrict__ active, int activeCount, const CameraGpu* camera,
     const MeshAccelSceneGpu* scene, const EnvironmentMapGpu* env, const RenderParamsGpu* params, int width,
-    int height, uint32_t seed)
+    int height, uint32_t seed, SpecModel sm)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= activeCount) {
@@ -85,8 +94,8 @@ __global__ __launch_bounds__(kBlock) void sample_adaptive_kernel(
     const RenderParamsGpu p = *params;
     const CameraGpu cam = *camera;
     const uint32_t n = counts[idx];
-    const V3 r = pixel_radiance(idx, idx % width, idx / width, width, height, true, static_cast<int>(n), cam, s, env,
-                                p, seed, nullptr);
+    const V3 r = pixel_radiance<SPEC>(idx, idx % width, idx / width, width, height, true, static_cast<int>(n), cam, s,
+                                      env, p, seed, nullptr, sm);
     float4 a = acc[idx];
     float mean = lumMean[idx];
     float mm = m2[idx];
@@ -105,12 +114,12 @@ __global__ __launch_bounds__(kBlock) void sample_adaptive_kernel(
 // passes run out. Per-pixel results equal passCount calls of the single-pass kernel.
 // STATS adds the traversal counters of all samples into totals[0..3] (nodes, triangles,
 // rays, closest hits) and totals[4] (samples).
-template <bool STATS>
+template <bool STATS, bool SPEC>
 __global__ __launch_bounds__(kBlock) void render_passes_kernel(
     float4* __restrict__ acc, uint32_t* __restrict__ counts, float* __restrict__ lumMean, float* __restrict__ m2,
     uint8_t* __restrict__ converged, const int* __restrict__ active, int activeCount, const CameraGpu* camera,
     const MeshAccelSceneGpu* scene, const EnvironmentMapGpu* env, const RenderParamsGpu* params, int width,
-    int height, uint32_t firstSeed, int passCount, int* workCounter, unsigned long long* totals)
+    int height, uint32_t firstSeed, int passCount, int* workCounter, unsigned long long* totals, SpecModel sm)
 {
     const MeshAccelSceneGpu s = *scene;
     const RenderParamsGpu p = *params;
@@ -134,8 +143,8 @@ __global__ __launch_bounds__(kBlock) void render_passes_kernel(
         const int y = idx / width;
         for (int pass = 0; pass < passCount && !conv; ++pass) {
             TraceStats st{0u, 0u, 0u, 0u};
-            const V3 r = pixel_radiance(idx, x, y, width, height, true, static_cast<int>(n), cam, s, env, p,
-                                        firstSeed + static_cast<uint32_t>(pass), STATS ? &st : nullptr);
+            const V3 r = pixel_radiance<SPEC>(idx, x, y, width, height, true, static_cast<int>(n), cam, s, env, p,
+                                              firstSeed + static_cast<uint32_t>(pass), STATS ? &st : nullptr, sm);
             if (STATS) {
                 tot[0] += st.nodes;
                 tot[1] += st.tris;
@@ -163,13 +172,15 @@ __global__ __launch_bounds__(kBlock) void render_passes_kernel(
 
 // Traversal statistics for one pass (same sample as sample_adaptive_kernel, no writes to the
 // accumulator): internal nodes fetched, triangles tested, rays, closest hits.
+template <bool SPEC>
 __global__ __launch_bounds__(kBlock) void count_traversal_kernel(const uint32_t* __restrict__ counts,
                                                                  const int* __restrict__ active, int activeCount,
                                                                  const CameraGpu* camera,
                                                                  const MeshAccelSceneGpu* scene,
                                                                  const EnvironmentMapGpu* env,
                                                                  const RenderParamsGpu* params, int width,
-                                                                 int height, uint32_t seed, uint32_t* out)
+                                                                 int height, uint32_t seed, uint32_t* out,
+                                                                 SpecModel sm)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= activeCount) {
@@ -180,19 +191,20 @@ __global__ __launch_bounds__(kBlock) void count_traversal_kernel(const uint32_t*
     if (idx >= 0) {
         TraceStats st{0u, 0u, 0u, 0u};
         const MeshAccelSceneGpu s = *scene;
-        (void)pixel_radiance(idx, idx % width, idx / width, width, height, true, static_cast<int>(counts[idx]),
-                             *camera, s, env, *params, seed, &st);
+        (void)pixel_radiance<SPEC>(idx, idx % width, idx / width, width, height, true,
+                                   static_cast<int>(counts[idx]), *camera, s, env, *params, seed, &st, sm);
         o = make_uint4(st.nodes, st.tris, st.rays, st.hits);
     }
     reinterpret_cast<uint4*>(out)[t] = o;
 }
 
 // PathTracer.cu:132-184, preview branch: overwrite, no jitter, sample index 0.
+template <bool SPEC>
 __global__ __launch_bounds__(kBlock) void preview_kernel(float4* __restrict__ buf, const CameraGpu* camera,
                                                          const MeshAccelSceneGpu* scene,
                                                          const EnvironmentMapGpu* env,
                                                          const RenderParamsGpu* params, int width, int height,
-                                                         uint32_t seed)
+                                                         uint32_t seed, SpecModel sm)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= width * height) {
@@ -202,7 +214,8 @@ __global__ __launch_bounds__(kBlock) void preview_kernel(float4* __restrict__ bu
                                                                             10.0f / 255.0f, 1.0f, 32, 3, 16,
                                                                             1024, 0, 0, 0.02f};
     const MeshAccelSceneGpu s = *scene;
-    const V3 r = pixel_radiance(i, i % width, i / width, width, height, false, 0, *camera, s, env, p, seed, nullptr);
+    const V3 r =
+        pixel_radiance<SPEC>(i, i % width, i / width, width, height, false, 0, *camera, s, env, p, seed, nullptr, sm);
     buf[i] = make_float4(r.x, r.y, r.z, 1.0f);
 }
 
@@ -604,9 +617,12 @@ bool pathTracerAdaptiveSample(PtFloat4* d_buffer, uint32_t* d_samples, float* d_
         ptSetError("pathTracerAdaptiveSample: no scene");
         return false;
     }
-    hipLaunchKernelGGL(sample_adaptive_kernel, dim3(grid1(activeCount, kBlock)), dim3(kBlock), 0, stream,
-                       reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_activeIndices,
-                       activeCount, d_camera, d_scene, d_env, d_params, width, height, globalSeed);
+    const bool spec = pt_spectral_enabled();
+    const SpecModel sm = pt_spectral_model();
+    hipLaunchKernelGGL(spec ? sample_adaptive_kernel<true> : sample_adaptive_kernel<false>,
+                       dim3(grid1(activeCount, kBlock)), dim3(kBlock), 0, stream, reinterpret_cast<float4*>(d_buffer),
+                       d_samples, d_lumMean, d_m2, d_converged, d_activeIndices, activeCount, d_camera, d_scene, d_env,
+                       d_params, width, height, globalSeed, sm);
     return launched("pathTracerAdaptiveSample");
 }
 
@@ -639,17 +655,13 @@ static bool render_passes_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d
     const long long wanted = static_cast<long long>(cus) * 16;
     const unsigned blocks = static_cast<unsigned>(
         std::min<long long>(wanted, (static_cast<long long>(activeCount) + kBlock - 1) / kBlock));
-    if (d_totals != nullptr) {
-        hipLaunchKernelGGL(render_passes_kernel<true>, dim3(blocks), dim3(kBlock), 0, stream,
-                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged,
-                           d_activeIndices, activeCount, d_camera, d_scene, d_env, d_params, width, height,
-                           firstGlobalSeed, passCount, counter, d_totals);
-    } else {
-        hipLaunchKernelGGL(render_passes_kernel<false>, dim3(blocks), dim3(kBlock), 0, stream,
-                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged,
-                           d_activeIndices, activeCount, d_camera, d_scene, d_env, d_params, width, height,
-                           firstGlobalSeed, passCount, counter, nullptr);
-    }
+    const bool spec = pt_spectral_enabled();
+    const SpecModel sm = pt_spectral_model();
+    auto kernel = d_totals != nullptr ? (spec ? render_passes_kernel<true, true> : render_passes_kernel<true, false>)
+                                      : (spec ? render_passes_kernel<false, true> : render_passes_kernel<false, false>);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, stream, reinterpret_cast<float4*>(d_buffer), d_samples,
+                       d_lumMean, d_m2, d_converged, d_activeIndices, activeCount, d_camera, d_scene, d_env, d_params,
+                       width, height, firstGlobalSeed, passCount, counter, d_totals, sm);
     return launched("pathTracerAdaptiveRenderPasses");
 }
 
@@ -707,9 +719,11 @@ bool pathTracerPreviewSample(PtFloat4* d_buffer, int width, int height, const Ca
         ptSetError("pathTracerPreviewSample: no scene");
         return false;
     }
-    hipLaunchKernelGGL(preview_kernel, dim3(grid1(static_cast<long long>(width) * height, kBlock)), dim3(kBlock), 0,
-                       stream, reinterpret_cast<float4*>(d_buffer), d_camera, d_scene, d_env, d_params, width, height,
-                       globalSeed);
+    const bool spec = pt_spectral_enabled();
+    hipLaunchKernelGGL(spec ? preview_kernel<true> : preview_kernel<false>,
+                       dim3(grid1(static_cast<long long>(width) * height, kBlock)), dim3(kBlock), 0, stream,
+                       reinterpret_cast<float4*>(d_buffer), d_camera, d_scene, d_env, d_params, width, height,
+                       globalSeed, pt_spectral_model());
     return launched("pathTracerPreviewSample");
 }
 
@@ -781,9 +795,11 @@ bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndi
         ptSetError("pathTracerCountTraversal: invalid arguments");
         return false;
     }
-    hipLaunchKernelGGL(count_traversal_kernel, dim3(grid1(activeCount, kBlock)), dim3(kBlock), 0, stream, d_samples,
-                       d_activeIndices, activeCount, d_camera, d_scene, d_env, d_params, width, height, globalSeed,
-                       d_stats);
+    const bool spec = pt_spectral_enabled();
+    hipLaunchKernelGGL(spec ? count_traversal_kernel<true> : count_traversal_kernel<false>,
+                       dim3(grid1(activeCount, kBlock)), dim3(kBlock), 0, stream, d_samples, d_activeIndices,
+                       activeCount, d_camera, d_scene, d_env, d_params, width, height, globalSeed, d_stats,
+                       pt_spectral_model());
     return launched("pathTracerCountTraversal");
 }
 

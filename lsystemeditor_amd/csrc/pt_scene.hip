@@ -839,11 +839,6 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
 
 float ptSceneLastBuildMs(const PtScene* scene) { return scene != nullptr ? scene->buildMs : 0.0f; }
 
-hipError_t spectralUploadStateToSymbol(const SpectralStateGpu* hostState)
-{
-    return hostState == nullptr ? hipErrorInvalidValue : hipSuccess;
-}
-
 } // extern "C"
 
 // ---- environment map (EnvironmentMap.cpp:76-127 importance tables + upload) -----------------

@@ -44,6 +44,8 @@ struct Paths {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmin;
     float *cex, *cey, *cez, *cmx, *cmy, *cmz;
     float *rx, *ry, *rz;
+    // fourth hero wavelength of T, L and the pending NEE values, and the hero draw (spectral mode)
+    float *Tw, *Lw, *cew, *cmw, *hero;
     uint32_t *pixel, *sample, *seed, *draws, *flags;
     // Hit records in shade-queue order, double-buffered by bounce parity: (path, triangle or -1,
     // material or -1, t bits) and (u, v). The shade kernel reads its whole input with one coalesced
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
                                                           const uint8_t* __restrict__ converged,
                                                           const CameraGpu* camera, const RenderParamsGpu* params,
                                                           int width, int height, uint32_t firstSeed, int passBase,
-                                                          int passCount, int groupSize, Paths P)
+                                                          int passCount, int groupSize, Paths P, bool spectral)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int activeCount = *activeCountPtr;
@@ -388,6 +390,11 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         const float l2 = rand01(rng);
         V3 ro, rd;
         camera_ray(cam, u, v, l1, l2, ro, rd);
+        if (spectral) {
+            P.hero[p] = rand01(rng);  // the hero-wavelength draw follows the lens draws
+            P.Tw[p] = 1.0f;
+            P.Lw[p] = 0.0f;
+        }
         P.pixel[p] = static_cast<uint32_t>(oIdx);
         P.sample[p] = oN + static_cast<uint32_t>(j);
         P.seed[p] = seed;
@@ -425,6 +432,36 @@ PT_DEV void push_shadow(Paths& P, int* counter, bool want, V3 o, V3 d, int path,
 #ifndef PT_SHADE_ATTR
 #define PT_SHADE_ATTR
 #endif
+PT_DEV V3 ld_col(const float* x, const float* y, const float* z, const float*, int p, const V3*)
+{
+    return mk(x[p], y[p], z[p]);
+}
+PT_DEV S4 ld_col(const float* x, const float* y, const float* z, const float* w, int p, const S4*)
+{
+    return mk4(x[p], y[p], z[p], w[p]);
+}
+PT_DEV void st_col(float* x, float* y, float* z, float*, int p, V3 v)
+{
+    x[p] = v.x;
+    y[p] = v.y;
+    z[p] = v.z;
+}
+PT_DEV void st_col(float* x, float* y, float* z, float* w, int p, S4 v)
+{
+    x[p] = v.x;
+    y[p] = v.y;
+    z[p] = v.z;
+    w[p] = v.w;
+}
+PT_DEV RgbColors make_colors(const Paths&, int, const SpecModel&, const RgbColors*) { return RgbColors{}; }
+PT_DEV HeroColors make_colors(const Paths& P, int p, const SpecModel& sm, const HeroColors*)
+{
+    return hero_colors(sm, P.hero[p]);
+}
+PT_DEV V3 finish_sample(const RgbColors&, V3 L) { return clamp_firefly(L); }  // PathTracer.cu:101-104
+PT_DEV V3 finish_sample(const HeroColors& pol, S4 L) { return clamp_firefly_spectral(pol.to_rgb(L)); }
+
+template <class Pol>
 __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const MeshAccelSceneGpu* scene,
                                                             const RenderParamsGpu* params,
                                                             const EnvironmentMapGpu* env, Paths P, int bounce,
@@ -432,8 +469,10 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                                                             const float2* __restrict__ inB,
                                                             const int* __restrict__ inCountPtr, int* extOut,
                                                             int4* __restrict__ outA, int* counters, float* capture,
-                                                            int captureSpp)
+                                                            int captureSpp, SpecModel sm)
 {
+    using C = typename Pol::Col;
+    constexpr bool kSpectral = ColN<C>::n == 4;
     const MeshAccelSceneGpu s = *scene;
     const RenderParamsGpu prm = *params;
     const int n = *inCountPtr;
@@ -455,13 +494,14 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
         V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO;
         if (live) {
             uint32_t flags = P.flags[p];
-            V3 L = mk(P.Lx[p], P.Ly[p], P.Lz[p]);
+            const Pol pol = make_colors(P, p, sm, static_cast<const Pol*>(nullptr));
+            C L = ld_col(P.Lx, P.Ly, P.Lz, P.Lw, p, static_cast<const C*>(nullptr));
             // shadow results of the previous bounce, in the reference's order (env, then emissive)
             if ((flags & kFlagEnv) != 0u && P.occ[2 * p] == 0u) {
-                L = add(L, mk(P.cex[p], P.cey[p], P.cez[p]));
+                L = add(L, ld_col(P.cex, P.cey, P.cez, P.cew, p, static_cast<const C*>(nullptr)));
             }
             if ((flags & kFlagEmis) != 0u && P.occ[2 * p + 1] == 0u) {
-                L = add(L, mk(P.cmx[p], P.cmy[p], P.cmz[p]));
+                L = add(L, ld_col(P.cmx, P.cmy, P.cmz, P.cmw, p, static_cast<const C*>(nullptr)));
             }
             uint32_t newFlags = 0u;
             if ((flags & kFlagExt) != 0u) {
@@ -470,12 +510,12 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                 const uint32_t tri = static_cast<uint32_t>(ha.y);
                 if (tri == 0xFFFFFFFFu) {
                     if (bounce == 0) {
-                        L = env_eval(env, prm, dir);  // PathIntegratorRandCore.h:215-217
+                        L = pol.radiance(env_eval(env, prm, dir));  // PathIntegratorRandCore.h:215-217
                     } else {                            // :338-345
-                        const V3 envL = env_eval(env, prm, dir);
+                        const C envL = pol.radiance(env_eval(env, prm, dir));
                         const float lpdf = env_pdf(env, dir);
                         const float mis = mis_balance(P.pdf[p], lpdf);
-                        const V3 T = mk(P.Tx[p], P.Ty[p], P.Tz[p]);
+                        const C T = ld_col(P.Tx, P.Ty, P.Tz, P.Tw, p, static_cast<const C*>(nullptr));
                         L = add(L, mul(T, scale(envL, mis)));
                     }
                 } else if (bounce < maxDepth) {
@@ -488,7 +528,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     Rng rng;
                     rng_resume(rng, static_cast<int>(P.pixel[p]), static_cast<int>(P.sample[p]), P.seed[p],
                                P.draws[p]);
-                    V3 T = mk(P.Tx[p], P.Ty[p], P.Tz[p]);
+                    C T = ld_col(P.Tx, P.Ty, P.Tz, P.Tw, p, static_cast<const C*>(nullptr));
                     const V3 pos = add(origin, scale(dir, t));
                     Ctx c;
                     c.n = h.normal;
@@ -499,7 +539,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     c.w = lobe_weights(c.m);
                     c.eta = P.eta[p];
                     c.flags = 0;
-                    L = add(L, mul(T, mk(c.m.r * c.m.emission, c.m.g * c.m.emission, c.m.b * c.m.emission)));
+                    const C base = pol.base(c.m);
+                    const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
+                    L = add(L, mul(T, pol.emission(c.m)));
                     // environment NEE (:65-108): value computed now, added if the shadow ray is clear
                     if (!skips_env_nee(c.m)) {
                         const float u1 = rand01(rng);
@@ -509,15 +551,13 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                         if (lightPdf > kMinPdf) {
                             const float cosT = max2(0.0f, dot(c.n, wi));
                             if (cosT > 0.0f) {
-                                const V3 Le = env_eval(env, prm, wi);
-                                const V3 f = brdf_eval(c, wi);
+                                const C Le = pol.radiance(env_eval(env, prm, wi));
+                                const C f = brdf_eval_t(c, wi, base, rad);
                                 const float bpdf = brdf_pdf(c, wi);
                                 const float mis = mis_balance(lightPdf, bpdf);
                                 const float sc = mis * cosT / lightPdf;
-                                const V3 cv = mul(T, mul(f, scale(Le, sc)));
-                                P.cex[p] = cv.x;
-                                P.cey[p] = cv.y;
-                                P.cez[p] = cv.z;
+                                const C cv = mul(T, mul(f, scale(Le, sc)));
+                                st_col(P.cex, P.cey, P.cez, P.cew, p, cv);
                                 wantEnv = true;
                                 envO = add(pos, scale(c.n, kShadowBias));
                                 envD = wi;
@@ -541,14 +581,12 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                                 if (cosT > 0.0f && cosL > 0.0f) {
                                     const float lightPdf = areaPdf * d2 / cosL;
                                     if (lightPdf > kMinPdf) {
-                                        const V3 f = brdf_eval(c, wi);
+                                        const C f = brdf_eval_t(c, wi, base, rad);
                                         const float bpdf = brdf_pdf(c, wi);
                                         const float mis = mis_balance(lightPdf, bpdf);
                                         const float sc = mis * cosT / lightPdf;
-                                        const V3 cv = mul(T, mul(f, scale(lr, sc)));
-                                        P.cmx[p] = cv.x;
-                                        P.cmy[p] = cv.y;
-                                        P.cmz[p] = cv.z;
+                                        const C cv = mul(T, mul(f, scale(pol.radiance(lr), sc)));
+                                        st_col(P.cmx, P.cmy, P.cmz, P.cmw, p, cv);
                                         wantEmis = true;
                                         emO = add(pos, scale(wi, kShadowBias));
                                         emD = wi;
@@ -564,12 +602,12 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     const Sample smp = brdf_sample(c, u1, u2);
                     bool cont = smp.valid && smp.pdf > kMinPdf;
                     if (cont) {
-                        const V3 f = brdf_eval(c, smp.dir);
-                        T = apply_throughput(T, c, smp, f);
+                        const C f = brdf_eval_t(c, smp.dir, base, rad);
+                        T = apply_throughput_t(T, c, smp, f, base);
                         if (smp.sss && smp.steps > 0) {
                             const float mr = max_radius(c.m);
                             if (mr > kMinRadius) {
-                                const V3 albedo = mk(c.m.r, c.m.g, c.m.b);
+                                const C albedo = base;
                                 for (int step = 0; step < smp.steps; ++step) {
                                     (void)rand01(rng);
                                     (void)rand01(rng);
@@ -584,9 +622,10 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                             }
                         }
                         if ((c.flags & 16) != 0) {
-                            T = smp.transmitted ? mul(T, mk(0.2f, 1.0f, 0.2f)) : mul(T, mk(1.0f, 0.2f, 0.2f));
+                            T = smp.transmitted ? mul(T, pol.reflectance(mk(0.2f, 1.0f, 0.2f)))
+                                                : mul(T, pol.reflectance(mk(1.0f, 0.2f, 0.2f)));
                         }
-                        const float tl = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+                        const float tl = pol.lum(T);
                         if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
                             cont = false;
                         }
@@ -612,9 +651,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                         P.tmin[p] = bias;
                         P.eta[p] = smp.nextEta;
                         P.pdf[p] = smp.pdf;
-                        P.Tx[p] = T.x;
-                        P.Ty[p] = T.y;
-                        P.Tz[p] = T.z;
+                        st_col(P.Tx, P.Ty, P.Tz, P.Tw, p, T);
                         wantExt = true;
                     }
                     P.draws[p] = rng_draws(rng);
@@ -623,21 +660,21 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
             newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
             P.flags[p] = newFlags;
             if (newFlags == 0u) {
-                if (capture != nullptr && P.sample[p] < static_cast<uint32_t>(captureSpp)) {
-                    // unclamped radiance of (pixel, sample) for the adjoint
-                    const size_t slot = 3 * (static_cast<size_t>(P.pixel[p]) * captureSpp + P.sample[p]);
-                    capture[slot + 0] = L.x;
-                    capture[slot + 1] = L.y;
-                    capture[slot + 2] = L.z;
+                if constexpr (!kSpectral) {
+                    if (capture != nullptr && P.sample[p] < static_cast<uint32_t>(captureSpp)) {
+                        // unclamped radiance of (pixel, sample) for the adjoint
+                        const size_t slot = 3 * (static_cast<size_t>(P.pixel[p]) * captureSpp + P.sample[p]);
+                        capture[slot + 0] = L.x;
+                        capture[slot + 1] = L.y;
+                        capture[slot + 2] = L.z;
+                    }
                 }
-                const V3 r = clamp_firefly(L);  // PathTracer.cu:101-104
+                const V3 r = finish_sample(pol, L);
                 P.rx[p] = r.x;
                 P.ry[p] = r.y;
                 P.rz[p] = r.z;
             } else {
-                P.Lx[p] = L.x;
-                P.Ly[p] = L.y;
-                P.Lz[p] = L.z;
+                st_col(P.Lx, P.Ly, P.Lz, P.Lw, p, L);
             }
         }
         const int ext = wave_append(&counters[0], wantExt);
@@ -840,7 +877,8 @@ size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spil
     Paths& P = w.P;
     const size_t c = static_cast<size_t>(cap);
     float** fs[] = {&P.Tx, &P.Ty, &P.Tz, &P.Lx, &P.Ly, &P.Lz, &P.eta, &P.pdf, &P.ox, &P.oy, &P.oz, &P.dx, &P.dy,
-                    &P.dz, &P.tmin, &P.cex, &P.cey, &P.cez, &P.cmx, &P.cmy, &P.cmz, &P.rx, &P.ry, &P.rz};
+                    &P.dz, &P.tmin, &P.cex, &P.cey, &P.cez, &P.cmx, &P.cmy, &P.cmz, &P.rx, &P.ry, &P.rz,
+                    &P.Tw, &P.Lw, &P.cew, &P.cmw, &P.hero};
     for (float** f : fs) {
         *f = carve<float>(cur, c);
     }
@@ -909,7 +947,8 @@ unsigned blocks_for(long long n) { return static_cast<unsigned>(std::max<long lo
 // trace of the shadow queue (from bounce 1 on) and the shade launch that fills the next queues.
 // Launch counts depend only on the host-known depth and path bound (no readbacks).
 void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, const RenderParamsGpu* d_params,
-                    const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks)
+                    const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks,
+                    bool spec, const SpecModel& sm)
 {
     const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
     const unsigned shadowBlocks =
@@ -941,9 +980,10 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
         }
         const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
         prof_mark(stream, 2, false, b);
-        hipLaunchKernelGGL(wf_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
-                           P, b, maxPathDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
-                           P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp);
+        hipLaunchKernelGGL(spec ? wf_shade_kernel<HeroColors> : wf_shade_kernel<RgbColors>, dim3(traceBlocks),
+                           dim3(kWfBlock), 0, stream, d_scene, d_params, d_env, P, b, maxPathDepth, P.hitA[b & 1],
+                           P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, gCapture,
+                           gCaptureSpp, sm);
         prof_mark(stream, 2, true, b);
     }
 }
@@ -984,6 +1024,8 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     }
     const int persistentBlocks = cus * 8;
+    const bool spec = pt_spectral_enabled();
+    const SpecModel sm = pt_spectral_model();
     const int spillThreads = persistentBlocks * kWfBlock;
     const int cap = std::max(4 * activeCount, std::min(target, 64 * activeCount));
     if (!ensure_workspace(cap, activeCount, maxPathDepth, spillThreads)) {
@@ -1022,9 +1064,9 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur,
                            d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
-                           passCount, group, P);
+                           passCount, group, P, spec);
         prof_mark(stream, 3, true);
-        launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks);
+        launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks, spec, sm);
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
                            reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
@@ -1188,6 +1230,8 @@ extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_s
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     }
     const int persistentBlocks = cus * 8;
+    const bool spec = pt_spectral_enabled();
+    const SpecModel sm = pt_spectral_model();
     const int spillThreads = persistentBlocks * kWfBlock;
     const int cap = std::max(pixels, std::min(target, samplesPerPixel * pixels));
     const bool fits = gWs.mem != nullptr && gWs.capacity >= cap && gWs.pixels >= pixels &&
@@ -1214,9 +1258,9 @@ extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_s
         hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
                            counterInts);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, d_samples,
-                           d_converged, d_camera, d_params, width, height, 1u, passBase, samplesPerPixel, g, P);
+                           d_converged, d_camera, d_params, width, height, 1u, passBase, samplesPerPixel, g, P, spec);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, static_cast<long long>(g) * pixels,
-                       persistentBlocks);
+                       persistentBlocks, spec, sm);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, P,
                            reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
     }

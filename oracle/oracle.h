@@ -253,6 +253,31 @@ void orc_render_detached(float* img3, int width, int height, const OCamera* cam,
 void orc_render_adjoint(const double* gradImg3, int width, int height, const OCamera* cam, const OScene* scene,
                         const OEnv* env, const OParams* params, int spp, double* grad, int threads);
 
+/* ---- spectral (Spectral/Rgb2Spec.h, SpectralCore.h, SpectralState.cpp) ----------------------- */
+typedef struct {
+    const float* scale;  /* res entries */
+    const float* data;   /* 3 * 3 * res^3 coefficients */
+    int res;
+    float white[3];      /* white normalisation */
+    float lambdaMin, lambdaRange; /* hero-wavelength range */
+} OSpecModel;
+float orc_cie_interp(int which, float lambdaNm); /* 0 x, 1 y, 2 z, 3 D65 */
+void orc_cie_rgb_response(float lambdaNm, float* out3);
+void orc_rgb2spec_coeffs(const OSpecModel* m, float r, float g, float b, float* out3);
+float orc_rgb2spec_sigmoid(const float* c3, float lambdaNm);
+float orc_rgb2spec_reflectance(const OSpecModel* m, float r, float g, float b, float lambdaNm);
+float orc_rgb2spec_radiance(const OSpecModel* m, float r, float g, float b, float lambdaNm);
+void orc_spectral_white_norm(const OSpecModel* m, float* out3);
+void orc_spectral_to_rgb(const OSpecModel* m, float L, float lambdaNm, float pdf, float* out3);
+void orc_hero_lambdas(const OSpecModel* m, float u, float* out4);
+OVec3 orc_hero_to_rgb(const OSpecModel* m, const float* L4, const float* lambda4);
+OVec3 orc_clamp_firefly_spectral(OVec3 r);
+void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* scene, const OParams* params, const OEnv* env,
+                             ORng* rng, const OSpecModel* model, const float* lambda4, float* L4);
+/* Hero-wavelength sampling for orc_pixel_radiance and everything built on it (NULL: RGB). The
+ * hero draw follows the lens draws. Process-wide; not for concurrent use with other settings. */
+void orc_set_spectral(const OSpecModel* model);
+
 /* UV debug view: pinhole primary ray, fractional hit UV as red/green. */
 /* Region reset (acc w = 1) listing the region row-major as the active set; returns its size. */
 int orc_clear_region(float* acc4, uint32_t* counts, float* mean, float* m2, uint8_t* converged, int* active,

@@ -77,6 +77,10 @@ class Env(C.Structure):
                 ("width", C.c_int), ("height", C.c_int), ("valid", C.c_int)]
 
 
+class Rng(C.Structure):
+    _fields_ = [("key", C.c_uint32 * 2), ("ctr", C.c_uint32 * 4), ("buf", C.c_uint32 * 4), ("avail", C.c_int)]
+
+
 class TraceStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "closestRays", "shadowRays", "nodeFetches", "triTests", "hits")]
 
@@ -157,6 +161,21 @@ def lib():
                                           C.c_int]),
             "orc_clear_region": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "orc_rebuild_active": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
+            "orc_cie_interp": (f, [C.c_int, f]),
+            "orc_cie_rgb_response": (None, [f, vp]),
+            "orc_rgb2spec_coeffs": (None, [C.POINTER(SpecModel), f, f, f, vp]),
+            "orc_rgb2spec_sigmoid": (f, [vp, f]),
+            "orc_rgb2spec_reflectance": (f, [C.POINTER(SpecModel), f, f, f, f]),
+            "orc_rgb2spec_radiance": (f, [C.POINTER(SpecModel), f, f, f, f]),
+            "orc_spectral_white_norm": (None, [C.POINTER(SpecModel), vp]),
+            "orc_spectral_to_rgb": (None, [C.POINTER(SpecModel), f, f, f, vp]),
+            "orc_hero_lambdas": (None, [C.POINTER(SpecModel), f, vp]),
+            "orc_hero_to_rgb": (Vec3, [C.POINTER(SpecModel), vp, vp]),
+            "orc_clamp_firefly_spectral": (Vec3, [Vec3]),
+            "orc_trace_path_spectral": (None, [Vec3, Vec3, C.POINTER(Scene), vp, C.POINTER(Env), vp,
+                                               C.POINTER(SpecModel), vp, vp]),
+            "orc_set_spectral": (None, [C.POINTER(SpecModel)]),
+            "orc_rng_init": (None, [C.POINTER(Rng), C.c_int, C.c_int, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -189,6 +208,11 @@ def ref_policy():
     L.ref_normalize.argtypes = [C.c_void_p, C.c_void_p]
     L.ref_rotate_ypr.restype = None
     L.ref_rotate_ypr.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]
+    L.ref_rgb2spec_coeffs.restype = None
+    L.ref_rgb2spec_coeffs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    for n in ("ref_rgb2spec_reflectance", "ref_rgb2spec_radiance"):
+        getattr(L, n).restype = C.c_float
+        getattr(L, n).argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_float] * 4
     return L
 
 
@@ -262,6 +286,33 @@ class OracleScene:
 
     def trace(self, ro, rd, tmin=0.0, tmax=1.0e6) -> Hit:
         return lib().orc_trace_ray(v3(ro), v3(rd), C.byref(self.scene), tmin, tmax)
+
+
+class SpecModel(C.Structure):
+    _fields_ = [("scale", C.c_void_p), ("data", C.c_void_p), ("res", C.c_int), ("white", C.c_float * 3),
+                ("lambdaMin", C.c_float), ("lambdaRange", C.c_float)]
+
+
+class OracleSpectral:
+    """An rgb2spec table as the oracle's OSpecModel (white normalisation computed by the oracle,
+    SpectralState.cpp:15-46), with the hero-wavelength range. Keeps the arrays alive."""
+
+    def __init__(self, scale: np.ndarray, data: np.ndarray, lambda_min: float = 360.0, lambda_max: float = 830.0):
+        self.scale = np.ascontiguousarray(scale, dtype=np.float32)
+        self.data = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+        rng = float(np.float32(np.float32(lambda_max) - np.float32(lambda_min)))
+        self.model = SpecModel(ptr(self.scale), ptr(self.data), int(self.scale.shape[0]), (C.c_float * 3)(1, 1, 1),
+                               float(lambda_min), rng)
+        w = np.zeros(3, np.float32)
+        lib().orc_spectral_white_norm(C.byref(self.model), ptr(w))
+        self.model.white = (C.c_float * 3)(*w.tolist())
+
+    def __enter__(self):
+        lib().orc_set_spectral(C.byref(self.model))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_spectral(None)
 
 
 class OracleEnv:

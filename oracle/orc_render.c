@@ -253,6 +253,253 @@ OVec3 orc_trace_path(OVec3 ro, OVec3 rd, const OScene* s, const OParams* params,
     return L;
 }
 
+/* ---- hero-wavelength integrator -----------------------------------------------------------
+ * Same path as orc_trace_path (sampling, pdfs and Russian roulette use the RGB material); every
+ * value is evaluated per wavelength k as one channel of the RGB model applied to a grey material
+ * whose colour is the rgb2spec reflectance of the base colour at lambda_k and whose scatter
+ * radius is the radiance-upsampled scatter radius at lambda_k. Light values are upsampled as
+ * radiance. The tier-B walk keeps the RGB radius and multiplies each wavelength by its albedo. */
+static OMaterial grey_material(const OMaterial* m, float base, float radius)
+{
+    OMaterial g = *m;
+    g.r = base;
+    g.g = base;
+    g.b = base;
+    g.scatterRadiusR = radius;
+    g.scatterRadiusG = radius;
+    g.scatterRadiusB = radius;
+    return g;
+}
+
+typedef struct {
+    const OSpecModel* model;
+    float lambda[4];
+    OMaterial grey[4];
+    OBrdfCtx ctx[4];
+} SpecVertex;
+
+static void spec_vertex(SpecVertex* v, const OMaterial* m, OVec3 n, OVec3 wo, float eta, int flags)
+{
+    for (int k = 0; k < 4; ++k) {
+        const float b = orc_rgb2spec_reflectance(v->model, m->r, m->g, m->b, v->lambda[k]);
+        const float r = orc_rgb2spec_radiance(v->model, fmax2(0.0f, m->scatterRadiusR), fmax2(0.0f, m->scatterRadiusG),
+                                              fmax2(0.0f, m->scatterRadiusB), v->lambda[k]);
+        v->grey[k] = grey_material(m, b, r);
+        v->ctx[k] = make_ctx(n, wo, &v->grey[k], eta, flags);
+    }
+}
+
+/* f_k * (light_k * scale) for a light sample of RGB radiance rgb */
+static void spec_light_term(const SpecVertex* v, OVec3 wi, OVec3 rgb, float scale, float* out4)
+{
+    for (int k = 0; k < 4; ++k) {
+        const float f = orc_brdf_eval(&v->ctx[k], wi).x;
+        const float Le = orc_rgb2spec_radiance(v->model, rgb.x, rgb.y, rgb.z, v->lambda[k]);
+        out4[k] = f * (Le * scale);
+    }
+}
+
+static int spec_env_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta,
+                        const OScene* s, const OEnv* env, const OParams* params, ORng* rng, float* out4)
+{
+    out4[0] = out4[1] = out4[2] = out4[3] = 0.0f;
+    if (orc_skips_env_nee(m)) {
+        return 0;
+    }
+    const float u1 = orc_rand01(rng);
+    const float u2 = orc_rand01(rng);
+    float lightPdf = 0.0f;
+    const OVec3 wi = orc_env_sample(env, params, u1, u2, &lightPdf);
+    if (lightPdf <= K_MIN_PDF || occluded(p, n, wi, s)) {
+        return 0;
+    }
+    const float cosT = fmax2(0.0f, vdot(n, wi));
+    if (cosT <= 0.0f) {
+        return 0;
+    }
+    const OBrdfCtx rgb = make_ctx(n, wo, m, eta, 0);
+    const float mis = orc_mis_balance(lightPdf, orc_brdf_pdf(&rgb, wi));
+    spec_light_term(v, wi, orc_env_eval(env, params, wi), mis * cosT / lightPdf, out4);
+    return 1;
+}
+
+static int spec_emissive_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta,
+                             const OScene* s, ORng* rng, float* out4)
+{
+    out4[0] = out4[1] = out4[2] = out4[3] = 0.0f;
+    if (s == NULL || s->emissiveCount == 0) {
+        return 0;
+    }
+    const float uTri = orc_rand01(rng);
+    const float u1 = orc_rand01(rng);
+    const float u2 = orc_rand01(rng);
+    OVec3 lp, ln, lr;
+    float areaPdf = 0.0f;
+    if (!orc_light_sample_emissive(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf)) {
+        return 0;
+    }
+    const OVec3 toL = vsub(lp, p);
+    const float d2 = vdot(toL, toL);
+    if (d2 <= K_MIN_PDF) {
+        return 0;
+    }
+    const OVec3 wi = vnorm(toL);
+    const float cosT = fmax2(0.0f, vdot(n, wi));
+    const float cosL = fmax2(0.0f, vdot(ln, vscale(wi, -1.0f)));
+    if (cosT <= 0.0f || cosL <= 0.0f || occluded_from(p, wi, s)) {
+        return 0;
+    }
+    const float lightPdf = areaPdf * d2 / cosL;
+    if (lightPdf <= K_MIN_PDF) {
+        return 0;
+    }
+    const OBrdfCtx rgb = make_ctx(n, wo, m, eta, 0);
+    const float mis = orc_mis_balance(lightPdf, orc_brdf_pdf(&rgb, wi));
+    spec_light_term(v, wi, lr, mis * cosT / lightPdf, out4);
+    return 1;
+}
+
+void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* s, const OParams* params, const OEnv* env,
+                             ORng* rng, const OSpecModel* model, const float* lambda4, float* L4)
+{
+    for (int k = 0; k < 4; ++k) {
+        L4[k] = 0.0f;
+    }
+    const OHit first = trace(ro, rd, s, 0.0f, K_RAY_TMAX, 1);
+    if (!first.hit) {
+        const OVec3 bg = orc_env_eval(env, params, rd);
+        for (int k = 0; k < 4; ++k) {
+            L4[k] = orc_rgb2spec_radiance(model, bg.x, bg.y, bg.z, lambda4[k]);
+        }
+        return;
+    }
+    int maxDepth = 32;
+    int rrMin = 3;
+    int flags = 0;
+    if (params != NULL) {
+        if (params->maxPathDepth > 0) {
+            maxDepth = params->maxPathDepth;
+        } else if (params->maxPathDepth == 0) {
+            maxDepth = 256;
+        }
+        if (params->russianRouletteMinDepth >= 0) {
+            rrMin = params->russianRouletteMinDepth;
+        }
+        flags = params->brdfDebugFlags;
+    }
+    float T[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    OVec3 origin = ro;
+    OVec3 dir = rd;
+    OHit hit = first;
+    float eta = 1.0f;
+    SpecVertex v;
+    v.model = model;
+    for (int k = 0; k < 4; ++k) {
+        v.lambda[k] = lambda4[k];
+    }
+
+    for (int depth = 0; depth < maxDepth; ++depth) {
+        const OVec3 p = vray(origin, dir, hit.t);
+        const OVec3 n = hit.normal;
+        const OVec3 wo = vscale(dir, -1.0f);
+        const OMaterial src = orc_fetch_material(s, hit.triangleIndex);
+        const OMaterial m = orc_resolve_material(&src, hit.uv, s);
+        spec_vertex(&v, &m, n, wo, eta, 0);
+
+        for (int k = 0; k < 4; ++k) {
+            const float Le = orc_rgb2spec_radiance(model, m.r * m.emission, m.g * m.emission, m.b * m.emission,
+                                                   v.lambda[k]);
+            L4[k] = L4[k] + T[k] * Le;
+        }
+        float c[4];
+        spec_env_nee(&v, p, n, wo, &m, eta, s, env, params, rng, c);
+        for (int k = 0; k < 4; ++k) {
+            L4[k] = L4[k] + T[k] * c[k];
+        }
+        spec_emissive_nee(&v, p, n, wo, &m, eta, s, rng, c);
+        for (int k = 0; k < 4; ++k) {
+            L4[k] = L4[k] + T[k] * c[k];
+        }
+
+        const OBrdfCtx ctx = make_ctx(n, wo, &m, eta, flags);
+        const float u1 = orc_rand01(rng);
+        const float u2 = orc_rand01(rng);
+        const OSample smp = orc_brdf_sample(&ctx, u1, u2);
+        if (!smp.valid || smp.pdf <= K_MIN_PDF) {
+            break;
+        }
+        for (int k = 0; k < 4; ++k) {
+            OBrdfCtx ck = v.ctx[k];
+            ck.debugFlags = flags;
+            const float f = orc_brdf_eval(&ck, smp.direction).x;
+            T[k] = orc_apply_throughput(v3(T[k], T[k], T[k]), &ck, &smp, v3(f, f, f)).x;
+        }
+        /* tier-B walk (PathIntegratorRandCore.h:169-204) with the RGB radius */
+        if (smp.subsurfaceScatter && smp.subsurfaceInternalSteps > 0) {
+            const float maxR = orc_max_scatter_radius(&m);
+            if (maxR > 1.0e-5f) {
+                for (int step = 0; step < smp.subsurfaceInternalSteps; ++step) {
+                    (void)orc_rand01(rng);
+                    (void)orc_rand01(rng);
+                    const float u3 = orc_rand01(rng);
+                    const float mfp = maxR * (-orc_logf(fmax2(1.0e-8f, u3)));
+                    const float surv = orc_expf(-mfp / maxR);
+                    for (int k = 0; k < 4; ++k) {
+                        T[k] = T[k] * (v.grey[k].r * surv);
+                    }
+                }
+                const float ed = vlen(smp.subsurfaceExitOffset);
+                if (ed > 0.0f) {
+                    const float a = orc_expf(-ed / maxR);
+                    for (int k = 0; k < 4; ++k) {
+                        T[k] = T[k] * a;
+                    }
+                }
+            }
+        }
+        if ((flags & (1 << 4)) != 0) {
+            const OVec3 tint = smp.transmitted ? v3(0.2f, 1.0f, 0.2f) : v3(1.0f, 0.2f, 0.2f);
+            for (int k = 0; k < 4; ++k) {
+                T[k] = T[k] * orc_rgb2spec_reflectance(model, tint.x, tint.y, tint.z, v.lambda[k]);
+            }
+        }
+        const float tl = 0.25f * (((T[0] + T[1]) + T[2]) + T[3]);
+        if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
+            break;
+        }
+        if (depth >= rrMin) {
+            const float surv = fmin2(0.95f, fmax2(tl, 0.05f));
+            if (orc_rand01(rng) > surv) {
+                break;
+            }
+            for (int k = 0; k < 4; ++k) {
+                T[k] = T[k] * (1.0f / surv);
+            }
+        }
+        eta = smp.nextMediumEta;
+        const float bias = fmax2(K_SHADOW_BIAS, K_REL_SHADOW_BIAS * hit.t);
+        const float sign = vdot(n, smp.direction) >= 0.0f ? 1.0f : -1.0f;
+        origin = vadd(p, vscale(n, sign * bias));
+        dir = smp.direction;
+        hit = trace(origin, dir, s, bias, K_RAY_TMAX, 1);
+        if (!hit.hit) {
+            const OVec3 bg = orc_env_eval(env, params, dir);
+            const float lpdf = orc_env_pdf(env, params, dir);
+            const float mis = orc_mis_balance(smp.pdf, lpdf);
+            for (int k = 0; k < 4; ++k) {
+                const float envL = orc_rgb2spec_radiance(model, bg.x, bg.y, bg.z, v.lambda[k]);
+                L4[k] = L4[k] + T[k] * (envL * mis);
+            }
+            break;
+        }
+    }
+}
+
+/* Spectral model used by orc_pixel_radiance (NULL: RGB). */
+static const OSpecModel* gSpecModel = NULL;
+
+void orc_set_spectral(const OSpecModel* model) { gSpecModel = model; }
+
 /* PathTracer.cu:49-67 */
 OVec3 orc_clamp_firefly(OVec3 r)
 {
@@ -283,6 +530,12 @@ OVec3 orc_pixel_radiance(int idx, int x, int y, int width, int height, int jitte
     const float l2 = orc_rand01(&rng);
     OVec3 ro, rd;
     orc_camera_ray(cam, u, v, l1, l2, &ro, &rd);
+    if (gSpecModel != NULL) {
+        float lambda[4], L4[4];
+        orc_hero_lambdas(gSpecModel, orc_rand01(&rng), lambda);
+        orc_trace_path_spectral(ro, rd, s, params, env, &rng, gSpecModel, lambda, L4);
+        return orc_clamp_firefly_spectral(orc_hero_to_rgb(gSpecModel, L4, lambda));
+    }
     return orc_clamp_firefly(orc_trace_path(ro, rd, s, params, env, &rng));
 }
 

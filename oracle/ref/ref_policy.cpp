@@ -6,6 +6,7 @@
 #include "PathTracerPreviewLevels.h"
 #include "PathTracerSampleBudget.h"
 #include "Sampling/MisCore.h"
+#include "Spectral/Rgb2Spec.h"
 
 extern "C" {
 
@@ -51,6 +52,31 @@ void ref_rotate_ypr(const float* in3, float yaw, float pitch, float roll, float*
     out3[0] = r.x;
     out3[1] = r.y;
     out3[2] = r.z;
+}
+
+static Rgb2SpecGpu ref_model(const float* scale, const float* data, int res)
+{
+    Rgb2SpecGpu m;
+    m.scale = scale;
+    m.data = data;
+    m.res = res;
+    return m;
+}
+
+void ref_rgb2spec_coeffs(const float* scale, const float* data, int res, const float* rgb3, float* out3)
+{
+    float rgb[3] = {rgb3[0], rgb3[1], rgb3[2]};
+    rgb2specFetchCoefficients(ref_model(scale, data, res), rgb, out3);
+}
+
+float ref_rgb2spec_reflectance(const float* scale, const float* data, int res, float r, float g, float b, float lambda)
+{
+    return rgb2specEvalReflectance(ref_model(scale, data, res), r, g, b, lambda);
+}
+
+float ref_rgb2spec_radiance(const float* scale, const float* data, int res, float r, float g, float b, float lambda)
+{
+    return rgb2specEvalRadiance(ref_model(scale, data, res), r, g, b, lambda);
 }
 
 }
