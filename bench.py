@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-loop", action="store_true", help="skip the per-pass (reference cadence) timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-render (C2) measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-scene (C3) measurement")
+    ap.add_argument("--no-spectral", action="store_true", help="skip the hero-wavelength (spectral) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device tensor copy (scale reference for PMC byte counters)")
     return ap.parse_args()
@@ -70,7 +71,8 @@ def read_traffic(name: str):
 
 def cpu_baseline(spec, threads: int):
     """The oracle (CPU restatement, C with pthreads) on a bounded sample of the same workload:
-    a centred 256x256 window of the full-resolution frame for its first 16 passes."""
+    a centred 512x512 window of the full-resolution frame for its first 32 passes (about ten
+    seconds of CPU work)."""
     import ctypes as C
     from lsystemeditor_amd import abi
     from oracle import oracle as O
@@ -79,14 +81,14 @@ def cpu_baseline(spec, threads: int):
     sc = O.OracleScene(abi.host_to_gpu_triangles(spec.triangles), spec.materials)
     build_s = time.perf_counter() - t0
     w, h = spec.width, spec.height
-    win = 256
+    win = 512
     x0, y0 = (w - win) // 2, (h - win) // 2
     idx = ((np.arange(y0, y0 + win)[:, None] * w) + np.arange(x0, x0 + win)[None, :]).astype(np.int32).ravel()
     acc = O.Accumulator(w, h)
     acc.clear()
     env = O.Env()
     L = O.lib()
-    passes = 16
+    passes = 32
     t0 = time.perf_counter()
     for p in range(passes):
         L.orc_adaptive_sample(O.ptr(acc.acc), O.ptr(acc.counts), O.ptr(acc.mean), O.ptr(acc.m2), O.ptr(acc.conv),
@@ -207,6 +209,58 @@ def batch_bench(bvh: str) -> dict:
             "bvh_build_ms_total": round(br.build_ms, 1), "scene_generation_s": round(gen_s, 1)}
 
 
+def spectral_bench(scene, spec, inputs, acc, passes: int, bvh: str) -> dict:
+    """Hero-wavelength rendering: the C1 frame and the C0 workload (four wavelengths in 380..720 nm)
+    with the project's rgb2spec table; Msamples/s beside the RGB figures."""
+    import torch
+
+    from lsystemeditor_amd import abi, render, rgb2spec, scenes
+    res = 16
+    t0 = time.perf_counter()
+    table = abi.Spectral(*rgb2spec.table(res))
+    table_s = time.perf_counter() - t0
+    out = {"hero_wavelengths": 4, "rgb2spec_table_res": res, "rgb2spec_table_build_s": round(table_s, 2)}
+    try:
+        table.enable(360.0, 830.0)
+        render.render_wavefront(scene, acc, inputs, passes=passes)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        render.render_wavefront(scene, acc, inputs, passes=passes)
+        torch.cuda.synchronize()
+        n = int(acc.counts.to(torch.int64).sum().item())
+        out["c1_msamples_s"] = n / (time.perf_counter() - t1) / 1e6
+        out["c1_samples"] = n
+        c0 = scenes.config0()
+        sc0 = abi.Scene(c0.triangles, c0.materials, bvh=bvh)
+        in0 = render.DeviceInputs(c0.camera, c0.params)
+        acc0 = render.Accumulator.allocate(c0.width, c0.height)
+        table.enable(380.0, 720.0)
+        render.render_wavefront(sc0, acc0, in0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            render.render_wavefront(sc0, acc0, in0)
+        torch.cuda.synchronize()
+        n0 = int(acc0.counts.to(torch.int64).sum().item())
+        out["c0_workload"] = "C0: 256x256, 16 spp, depth 8, 4 hero wavelengths in 380-720 nm"
+        out["c0_msamples_s"] = n0 * reps / (time.perf_counter() - t2) / 1e6
+        abi.Spectral.disable()
+        render.render_wavefront(sc0, acc0, in0)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(reps):
+            render.render_wavefront(sc0, acc0, in0)
+        torch.cuda.synchronize()
+        out["c0_rgb_msamples_s"] = n0 * reps / (time.perf_counter() - t3) / 1e6
+    finally:
+        abi.Spectral.disable()
+        table.close()
+    print(f"[bench] spectral: C1 {out['c1_msamples_s']:.1f} Msamples/s, C0 {out['c0_msamples_s']:.1f} "
+          f"(RGB {out['c0_rgb_msamples_s']:.1f})", file=sys.stderr, flush=True)
+    return out
+
+
 def profile_read():
     import ctypes as C
     from lsystemeditor_amd import abi
@@ -306,6 +360,9 @@ def main():
         print(f"[bench] persistent megakernel {extra['persistent_megakernel_msamples_s']:.1f}, reference cadence "
               f"(one launch per pass) {extra['reference_cadence_msamples_s']:.1f} Msamples/s", file=sys.stderr,
               flush=True)
+
+    if not args.no_spectral and args.config == "c1":
+        extra["spectral"] = spectral_bench(scene, spec, inputs, acc, passes, args.bvh)
 
     if not args.no_batch and args.config == "c1":
         extra["batch"] = batch_bench(args.bvh)

@@ -14,6 +14,7 @@
 //               handles hits and misses, samples the lights and the BSDF, and appends the
 //               next rays to queues (wave-aggregated atomics).
 #include "pt_device.h"
+#include "pt_integrator.h"
 #include "pt_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -38,24 +39,29 @@ enum : uint32_t {
     kFlagEmis = 4u,  // emissive NEE shadow ray pending
 };
 
-// Path state, structure of arrays over path slots.
-struct Paths {
-    float *Tx, *Ty, *Tz, *Lx, *Ly, *Lz, *eta, *pdf;
+// Path state, structure of arrays over the slots of one bounce's shade queue. Two buffers
+// alternate by bounce parity: the shade kernel reads item i of its queue at slot i of one buffer
+// and writes the paths that go on at their slot of the next queue in the other, so every state
+// access is coalesced (the generation kernel fills buffer 0 in path order).
+struct PathState {
+    float *Tx, *Ty, *Tz, *Tw, *Lx, *Ly, *Lz, *Lw, *eta, *pdf;
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmin;
-    float *cex, *cey, *cez, *cmx, *cmy, *cmz;
-    float *rx, *ry, *rz;
-    // fourth hero wavelength of T, L and the pending NEE values, and the hero draw (spectral mode)
-    float *Tw, *Lw, *cew, *cmw, *hero;
+    float *cex, *cey, *cez, *cew, *cmx, *cmy, *cmz, *cmw; // pending NEE contributions
+    float* hero;                                          // hero-wavelength draw (spectral mode)
     uint32_t *pixel, *sample, *seed, *draws, *flags;
-    // Hit records in shade-queue order, double-buffered by bounce parity: (path, triangle or -1,
-    // material or -1, t bits) and (u, v). The shade kernel reads its whole input with one coalesced
-    // load; the extension trace writes each ray's result at the path's shade slot.
+    uint32_t* slot0; // the path's generation slot, where its final radiance goes
+    uint8_t* occ;    // 2 per slot: environment, emissive shadow results
+};
+
+struct Paths {
+    PathState S[2];
+    float *rx, *ry, *rz; // final clamped radiance per generation slot
+    // Hit records in shade-queue order, double-buffered by bounce parity: (slot, triangle or -1,
+    // material or -1, t bits) and (u, v), written by the extension trace at the ray's slot.
     int4* hitA[2];
     float2* hitB[2];
-    int* shadeSlot; // per path: its index in the next shade queue
-    uint8_t* occ;   // 2 per path: environment, emissive
-    int* extQ[2];   // extension ray queues (path ids)
-    float4* shadowQ; // 2 per ray: (origin, path*4 + type), (direction, -)
+    int* extQ[2];   // extension ray queues (slots of the next shade queue)
+    float4* shadowQ; // 2 per ray: (origin, slot*4 + type), (direction, -)
     int* counters;  // per bounce: [0] ext, [1] shade, [2] shadow rays produced, [3] paths (gen, bounce 0),
                     // [4] extension and [5] shadow fetch positions of the trace kernels
     int* pxBase;
@@ -80,6 +86,63 @@ PT_DEV int wave_append(int* counter, bool want)
     }
     base = __shfl(base, leader, 64);
     return want ? base + __popcll(mask & ((1ull << lane_id()) - 1ull)) : -1;
+}
+
+// Queue appends of one shade trip, aggregated over the block: one 64-bit atomic for the
+// extension and shade queues (adjacent counters) and one for the shadow-ray queue, per 256
+// items, instead of one atomic per wave and queue. Lanes get their slots in lane order within a
+// wave and wave order within the block. Every thread of the block must call it.
+struct BlockAppendLds {
+    unsigned long long waveBase[kWfBlock / 64];
+    int shadowBase[kWfBlock / 64];
+    int counts[kWfBlock / 64][3];
+};
+
+PT_DEV void block_append(BlockAppendLds& lds, int* counters, bool wantExt, bool wantNext, bool wantEnv,
+                         bool wantEmis, int& extSlot, int& nextSlot, int& envSlot, int& emisSlot)
+{
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const unsigned long long below = (1ull << lane_id()) - 1ull;
+    const unsigned long long mExt = __ballot(wantExt);
+    const unsigned long long mNext = __ballot(wantNext);
+    const unsigned long long mEnv = __ballot(wantEnv);
+    const unsigned long long mEmis = __ballot(wantEmis);
+    if (lane_id() == 0) {
+        lds.counts[wave][0] = __popcll(mExt);
+        lds.counts[wave][1] = __popcll(mNext);
+        lds.counts[wave][2] = __popcll(mEnv) + __popcll(mEmis);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int e = 0, n = 0, sh = 0;
+        for (int w = 0; w < kWfBlock / 64; ++w) {
+            e += lds.counts[w][0];
+            n += lds.counts[w][1];
+            sh += lds.counts[w][2];
+        }
+        unsigned long long base = 0ull;
+        if (e + n > 0) {
+            base = atomicAdd(reinterpret_cast<unsigned long long*>(counters),
+                             (static_cast<unsigned long long>(n) << 32) | static_cast<unsigned long long>(e));
+        }
+        int sbase = sh > 0 ? atomicAdd(&counters[2], sh) : 0;
+        unsigned long long eb = base & 0xFFFFFFFFull, nb = base >> 32;
+        for (int w = 0; w < kWfBlock / 64; ++w) {
+            lds.waveBase[w] = (nb << 32) | eb;
+            lds.shadowBase[w] = sbase;
+            eb += static_cast<unsigned long long>(lds.counts[w][0]);
+            nb += static_cast<unsigned long long>(lds.counts[w][1]);
+            sbase += lds.counts[w][2];
+        }
+    }
+    __syncthreads();
+    const unsigned long long wb = lds.waveBase[wave];
+    const int sb = lds.shadowBase[wave];
+    extSlot = static_cast<int>(wb & 0xFFFFFFFFull) + __popcll(mExt & below);
+    nextSlot = static_cast<int>(wb >> 32) + __popcll(mNext & below);
+    envSlot = sb + __popcll(mEnv & below);
+    emisSlot = sb + __popcll(mEnv) + __popcll(mEmis & below);
+    __syncthreads();  // the LDS record is reused by the next trip
 }
 
 // ----------------------------------------------------------------------------------------
@@ -140,7 +203,7 @@ constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf
 //   ANY_HIT = true:  shadow rays of shadowQ, result to occ.
 template <bool ANY_HIT, bool STATS>
 __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
-                                                            const int* __restrict__ queue,
+                                                            PathState PS, const int* __restrict__ queue,
                                                             const int* __restrict__ countPtr, int* fetchCounter,
                                                             unsigned long long* stats, int4* __restrict__ hitA,
                                                             float2* __restrict__ hitB)
@@ -157,8 +220,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
     int item = -1;        // queue index of the lane's current ray
     bool exhausted = false;
-    uint32_t tag = 0u;    // path id (extension) or path * 4 + type (shadow)
-    int slot = 0;         // extension: the path's shade slot
+    uint32_t tag = 0u;    // slot (extension) or slot * 4 + type (shadow)
     int bestMat = -1;
     Ray r;
     r.o = mk(0.0f, 0.0f, 0.0f);
@@ -190,11 +252,10 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                         r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
                         tMin = kShadowBias;
                     } else {
-                        const int p = queue != nullptr ? queue[idx] : idx;
-                        tag = static_cast<uint32_t>(p);
-                        slot = P.shadeSlot[p];
-                        r = make_ray(mk(P.ox[p], P.oy[p], P.oz[p]), mk(P.dx[p], P.dy[p], P.dz[p]));
-                        tMin = P.tmin[p];
+                        const int q = queue != nullptr ? queue[idx] : idx;
+                        tag = static_cast<uint32_t>(q);
+                        r = make_ray(mk(PS.ox[q], PS.oy[q], PS.oz[q]), mk(PS.dx[q], PS.dy[q], PS.dz[q]));
+                        tMin = PS.tmin[q];
                     }
                     bestT = kRayTMax;
                     bestTri = 0xFFFFFFFFu;
@@ -293,11 +354,11 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
         if (finished) {
             const bool hit = bestT < kRayTMax;
             if (ANY_HIT) {
-                P.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
+                PS.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
             } else {
-                hitA[slot] = make_int4(static_cast<int>(tag), hit ? static_cast<int>(bestTri) : -1, hit ? bestMat : -1,
-                                       __float_as_int(hit ? bestT : 0.0f));
-                hitB[slot] = make_float2(bestU, bestV);
+                hitA[tag] = make_int4(static_cast<int>(tag), hit ? static_cast<int>(bestTri) : -1, hit ? bestMat : -1,
+                                      __float_as_int(hit ? bestT : 0.0f));
+                hitB[tag] = make_float2(bestU, bestV);
             }
             item = -1;
         }
@@ -390,42 +451,40 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         const float l2 = rand01(rng);
         V3 ro, rd;
         camera_ray(cam, u, v, l1, l2, ro, rd);
+        PathState& O = P.S[0];
         if (spectral) {
-            P.hero[p] = rand01(rng);  // the hero-wavelength draw follows the lens draws
-            P.Tw[p] = 1.0f;
-            P.Lw[p] = 0.0f;
+            O.hero[p] = rand01(rng);  // the hero-wavelength draw follows the lens draws
+            O.Tw[p] = 1.0f;
+            O.Lw[p] = 0.0f;
         }
-        P.pixel[p] = static_cast<uint32_t>(oIdx);
-        P.sample[p] = oN + static_cast<uint32_t>(j);
-        P.seed[p] = seed;
-        P.draws[p] = rng_draws(rng);
-        P.ox[p] = ro.x;
-        P.oy[p] = ro.y;
-        P.oz[p] = ro.z;
-        P.dx[p] = rd.x;
-        P.dy[p] = rd.y;
-        P.dz[p] = rd.z;
-        P.tmin[p] = 0.0f;
-        P.Tx[p] = 1.0f;
-        P.Ty[p] = 1.0f;
-        P.Tz[p] = 1.0f;
-        P.Lx[p] = 0.0f;
-        P.Ly[p] = 0.0f;
-        P.Lz[p] = 0.0f;
-        P.eta[p] = 1.0f;
-        P.pdf[p] = 0.0f;
-        P.flags[p] = kFlagExt;
-        P.shadeSlot[p] = p; // bounce 0 shades in path order
+        O.pixel[p] = static_cast<uint32_t>(oIdx);
+        O.sample[p] = oN + static_cast<uint32_t>(j);
+        O.seed[p] = seed;
+        O.draws[p] = rng_draws(rng);
+        O.slot0[p] = static_cast<uint32_t>(p);
+        O.ox[p] = ro.x;
+        O.oy[p] = ro.y;
+        O.oz[p] = ro.z;
+        O.dx[p] = rd.x;
+        O.dy[p] = rd.y;
+        O.dz[p] = rd.z;
+        O.tmin[p] = 0.0f;
+        O.Tx[p] = 1.0f;
+        O.Ty[p] = 1.0f;
+        O.Tz[p] = 1.0f;
+        O.Lx[p] = 0.0f;
+        O.Ly[p] = 0.0f;
+        O.Lz[p] = 0.0f;
+        O.eta[p] = 1.0f;
+        O.pdf[p] = 0.0f;
+        O.flags[p] = kFlagExt;
     }
 }
 
-PT_DEV void push_shadow(Paths& P, int* counter, bool want, V3 o, V3 d, int path, uint32_t type)
+PT_DEV void store_shadow(Paths& P, int slot, V3 o, V3 d, int item, uint32_t type)
 {
-    const int slot = wave_append(counter, want);
-    if (want) {
-        P.shadowQ[2 * slot] = make_float4(o.x, o.y, o.z, __uint_as_float((static_cast<uint32_t>(path) << 2) | type));
-        P.shadowQ[2 * slot + 1] = make_float4(d.x, d.y, d.z, 0.0f);
-    }
+    P.shadowQ[2 * slot] = make_float4(o.x, o.y, o.z, __uint_as_float((static_cast<uint32_t>(item) << 2) | type));
+    P.shadowQ[2 * slot + 1] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
 // One bounce of PathIntegratorRandCore.h:241-347 for every path in the shade queue.
@@ -453,26 +512,22 @@ PT_DEV void st_col(float* x, float* y, float* z, float* w, int p, S4 v)
     z[p] = v.z;
     w[p] = v.w;
 }
-PT_DEV RgbColors make_colors(const Paths&, int, const SpecModel&, const RgbColors*) { return RgbColors{}; }
-PT_DEV HeroColors make_colors(const Paths& P, int p, const SpecModel& sm, const HeroColors*)
-{
-    return hero_colors(sm, P.hero[p]);
-}
+PT_DEV RgbColors make_colors(float, const SpecModel&, const RgbColors*) { return RgbColors{}; }
+PT_DEV HeroColors make_colors(float hero, const SpecModel& sm, const HeroColors*) { return hero_colors(sm, hero); }
 PT_DEV V3 finish_sample(const RgbColors&, V3 L) { return clamp_firefly(L); }  // PathTracer.cu:101-104
 PT_DEV V3 finish_sample(const HeroColors& pol, S4 L) { return clamp_firefly_spectral(pol.to_rgb(L)); }
 
 template <class Pol>
-__global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const MeshAccelSceneGpu* scene,
-                                                            const RenderParamsGpu* params,
-                                                            const EnvironmentMapGpu* env, Paths P, int bounce,
-                                                            int maxDepth, const int4* __restrict__ inA,
-                                                            const float2* __restrict__ inB,
-                                                            const int* __restrict__ inCountPtr, int* extOut,
-                                                            int4* __restrict__ outA, int* counters, float* capture,
-                                                            int captureSpp, SpecModel sm)
+__global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
+    const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
+    PathState O, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA, const float2* __restrict__ inB,
+    const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA, int* counters, float* capture,
+    int captureSpp, SpecModel sm)
 {
     using C = typename Pol::Col;
     constexpr bool kSpectral = ColN<C>::n == 4;
+    const C* ct = nullptr;
+    __shared__ BlockAppendLds appendLds;
     const MeshAccelSceneGpu s = *scene;
     const RenderParamsGpu prm = *params;
     const int n = *inCountPtr;
@@ -483,39 +538,44 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
     for (int it = 0; it < trips; ++it) {
         const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
         const bool live = i < n;
-        int4 ha = make_int4(0, -1, -1, 0);
-        float2 hb = make_float2(0.0f, 0.0f);
-        if (live) {
-            ha = inA[i];
-            hb = inB[i];
-        }
-        const int p = ha.x;
+        // what goes to the item's slot of the next shade queue
         bool wantExt = false, wantEnv = false, wantEmis = false;
-        V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO;
+        V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO, nextO = envO, nextD = envO;
+        C L = ColN<C>::splat(0.0f), T = L, ce = L, cm = L;
+        float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f, hero = 0.0f;
+        uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u;
         if (live) {
-            uint32_t flags = P.flags[p];
-            const Pol pol = make_colors(P, p, sm, static_cast<const Pol*>(nullptr));
-            C L = ld_col(P.Lx, P.Ly, P.Lz, P.Lw, p, static_cast<const C*>(nullptr));
+            const int4 ha = inA[i];
+            const float2 hb = inB[i];
+            const uint32_t flags = I.flags[i];
+            pixel = I.pixel[i];
+            sample = I.sample[i];
+            seed = I.seed[i];
+            slot0 = I.slot0[i];
+            if constexpr (kSpectral) {
+                hero = I.hero[i];
+            }
+            const Pol pol = make_colors(hero, sm, static_cast<const Pol*>(nullptr));
+            L = ld_col(I.Lx, I.Ly, I.Lz, I.Lw, i, ct);
             // shadow results of the previous bounce, in the reference's order (env, then emissive)
-            if ((flags & kFlagEnv) != 0u && P.occ[2 * p] == 0u) {
-                L = add(L, ld_col(P.cex, P.cey, P.cez, P.cew, p, static_cast<const C*>(nullptr)));
+            if ((flags & kFlagEnv) != 0u && I.occ[2 * i] == 0u) {
+                L = add(L, ld_col(I.cex, I.cey, I.cez, I.cew, i, ct));
             }
-            if ((flags & kFlagEmis) != 0u && P.occ[2 * p + 1] == 0u) {
-                L = add(L, ld_col(P.cmx, P.cmy, P.cmz, P.cmw, p, static_cast<const C*>(nullptr)));
+            if ((flags & kFlagEmis) != 0u && I.occ[2 * i + 1] == 0u) {
+                L = add(L, ld_col(I.cmx, I.cmy, I.cmz, I.cmw, i, ct));
             }
-            uint32_t newFlags = 0u;
             if ((flags & kFlagExt) != 0u) {
-                const V3 origin = mk(P.ox[p], P.oy[p], P.oz[p]);
-                const V3 dir = mk(P.dx[p], P.dy[p], P.dz[p]);
+                const V3 origin = mk(I.ox[i], I.oy[i], I.oz[i]);
+                const V3 dir = mk(I.dx[i], I.dy[i], I.dz[i]);
                 const uint32_t tri = static_cast<uint32_t>(ha.y);
                 if (tri == 0xFFFFFFFFu) {
                     if (bounce == 0) {
                         L = pol.radiance(env_eval(env, prm, dir));  // PathIntegratorRandCore.h:215-217
-                    } else {                            // :338-345
+                    } else {                                         // :338-345
                         const C envL = pol.radiance(env_eval(env, prm, dir));
                         const float lpdf = env_pdf(env, dir);
-                        const float mis = mis_balance(P.pdf[p], lpdf);
-                        const C T = ld_col(P.Tx, P.Ty, P.Tz, P.Tw, p, static_cast<const C*>(nullptr));
+                        const float mis = mis_balance(I.pdf[i], lpdf);
+                        T = ld_col(I.Tx, I.Ty, I.Tz, I.Tw, i, ct);
                         L = add(L, mul(T, scale(envL, mis)));
                     }
                 } else if (bounce < maxDepth) {
@@ -526,9 +586,8 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     h.t = t;
                     finish_hit(s, ray, tri, hb.x, hb.y, h);
                     Rng rng;
-                    rng_resume(rng, static_cast<int>(P.pixel[p]), static_cast<int>(P.sample[p]), P.seed[p],
-                               P.draws[p]);
-                    C T = ld_col(P.Tx, P.Ty, P.Tz, P.Tw, p, static_cast<const C*>(nullptr));
+                    rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws[i]);
+                    T = ld_col(I.Tx, I.Ty, I.Tz, I.Tw, i, ct);
                     const V3 pos = add(origin, scale(dir, t));
                     Ctx c;
                     c.n = h.normal;
@@ -537,7 +596,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     c.m = resolve_material_index(s, mi, s.materials != nullptr && tri < s.triangleCount &&
                                                             mi < s.materialCount, h.uvx, h.uvy);
                     c.w = lobe_weights(c.m);
-                    c.eta = P.eta[p];
+                    c.eta = I.eta[i];
                     c.flags = 0;
                     const C base = pol.base(c.m);
                     const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
@@ -556,8 +615,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                                 const float bpdf = brdf_pdf(c, wi);
                                 const float mis = mis_balance(lightPdf, bpdf);
                                 const float sc = mis * cosT / lightPdf;
-                                const C cv = mul(T, mul(f, scale(Le, sc)));
-                                st_col(P.cex, P.cey, P.cez, P.cew, p, cv);
+                                ce = mul(T, mul(f, scale(Le, sc)));
                                 wantEnv = true;
                                 envO = add(pos, scale(c.n, kShadowBias));
                                 envD = wi;
@@ -585,8 +643,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                                         const float bpdf = brdf_pdf(c, wi);
                                         const float mis = mis_balance(lightPdf, bpdf);
                                         const float sc = mis * cosT / lightPdf;
-                                        const C cv = mul(T, mul(f, scale(pol.radiance(lr), sc)));
-                                        st_col(P.cmx, P.cmy, P.cmz, P.cmw, p, cv);
+                                        cm = mul(T, mul(f, scale(pol.radiance(lr), sc)));
                                         wantEmis = true;
                                         emO = add(pos, scale(wi, kShadowBias));
                                         emD = wi;
@@ -604,23 +661,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                     if (cont) {
                         const C f = brdf_eval_t(c, smp.dir, base, rad);
                         T = apply_throughput_t(T, c, smp, f, base);
-                        if (smp.sss && smp.steps > 0) {
-                            const float mr = max_radius(c.m);
-                            if (mr > kMinRadius) {
-                                const C albedo = base;
-                                for (int step = 0; step < smp.steps; ++step) {
-                                    (void)rand01(rng);
-                                    (void)rand01(rng);
-                                    const float u3 = rand01(rng);
-                                    const float mfp = mr * (-logf_pt(max2(1.0e-8f, u3)));
-                                    T = mul(T, scale(albedo, expf_pt(-mfp / mr)));
-                                }
-                                const float ed = length(smp.exitOffset);
-                                if (ed > 0.0f) {
-                                    T = scale(T, expf_pt(-ed / mr));
-                                }
-                            }
-                        }
+                        T = subsurface_tier_b_t(T, c.m, base, smp, rng);
                         if ((c.flags & 16) != 0) {
                             T = smp.transmitted ? mul(T, pol.reflectance(mk(0.2f, 1.0f, 0.2f)))
                                                 : mul(T, pol.reflectance(mk(1.0f, 0.2f, 0.2f)));
@@ -639,56 +680,78 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(const 
                         }
                     }
                     if (cont) {
-                        const float bias = max2(kShadowBias, kRelShadowBias * t);
+                        nextTmin = max2(kShadowBias, kRelShadowBias * t);
                         const float sgn = dot(c.n, smp.dir) >= 0.0f ? 1.0f : -1.0f;
-                        const V3 o2 = add(pos, scale(c.n, sgn * bias));
-                        P.ox[p] = o2.x;
-                        P.oy[p] = o2.y;
-                        P.oz[p] = o2.z;
-                        P.dx[p] = smp.dir.x;
-                        P.dy[p] = smp.dir.y;
-                        P.dz[p] = smp.dir.z;
-                        P.tmin[p] = bias;
-                        P.eta[p] = smp.nextEta;
-                        P.pdf[p] = smp.pdf;
-                        st_col(P.Tx, P.Ty, P.Tz, P.Tw, p, T);
+                        nextO = add(pos, scale(c.n, sgn * nextTmin));
+                        nextD = smp.dir;
+                        nextEta = smp.nextEta;
+                        nextPdf = smp.pdf;
                         wantExt = true;
                     }
-                    P.draws[p] = rng_draws(rng);
+                    draws = rng_draws(rng);
                 }
             }
             newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
-            P.flags[p] = newFlags;
             if (newFlags == 0u) {
                 if constexpr (!kSpectral) {
-                    if (capture != nullptr && P.sample[p] < static_cast<uint32_t>(captureSpp)) {
+                    if (capture != nullptr && sample < static_cast<uint32_t>(captureSpp)) {
                         // unclamped radiance of (pixel, sample) for the adjoint
-                        const size_t slot = 3 * (static_cast<size_t>(P.pixel[p]) * captureSpp + P.sample[p]);
+                        const size_t slot = 3 * (static_cast<size_t>(pixel) * captureSpp + sample);
                         capture[slot + 0] = L.x;
                         capture[slot + 1] = L.y;
                         capture[slot + 2] = L.z;
                     }
                 }
                 const V3 r = finish_sample(pol, L);
-                P.rx[p] = r.x;
-                P.ry[p] = r.y;
-                P.rz[p] = r.z;
-            } else {
-                st_col(P.Lx, P.Ly, P.Lz, P.Lw, p, L);
+                P.rx[slot0] = r.x;
+                P.ry[slot0] = r.y;
+                P.rz[slot0] = r.z;
             }
         }
-        const int ext = wave_append(&counters[0], wantExt);
+        const bool next = wantExt || wantEnv || wantEmis;
+        int ext, sh, envSlot, emisSlot;
+        block_append(appendLds, counters, wantExt, next, wantEnv, wantEmis, ext, sh, envSlot, emisSlot);
         if (wantExt) {
-            extOut[ext] = p;
+            extOut[ext] = sh;
         }
-        const int sh = wave_append(&counters[1], wantExt || wantEnv || wantEmis);
-        if (wantExt || wantEnv || wantEmis) {
-            // next bounce's record; the extension trace fills in its hit
-            outA[sh] = make_int4(p, -1, -1, 0);
-            P.shadeSlot[p] = sh;
+        if (next) {
+            // the next bounce's record (the extension trace fills in its hit) and state
+            outA[sh] = make_int4(sh, -1, -1, 0);
+            O.flags[sh] = newFlags;
+            O.pixel[sh] = pixel;
+            O.sample[sh] = sample;
+            O.seed[sh] = seed;
+            O.slot0[sh] = slot0;
+            if constexpr (kSpectral) {
+                O.hero[sh] = hero;
+            }
+            st_col(O.Lx, O.Ly, O.Lz, O.Lw, sh, L);
+            if (wantEnv) {
+                st_col(O.cex, O.cey, O.cez, O.cew, sh, ce);
+            }
+            if (wantEmis) {
+                st_col(O.cmx, O.cmy, O.cmz, O.cmw, sh, cm);
+            }
+            if (wantExt) {
+                O.ox[sh] = nextO.x;
+                O.oy[sh] = nextO.y;
+                O.oz[sh] = nextO.z;
+                O.dx[sh] = nextD.x;
+                O.dy[sh] = nextD.y;
+                O.dz[sh] = nextD.z;
+                O.tmin[sh] = nextTmin;
+                O.eta[sh] = nextEta;
+                O.pdf[sh] = nextPdf;
+                O.draws[sh] = draws;
+                st_col(O.Tx, O.Ty, O.Tz, O.Tw, sh, T);
+            }
         }
-        push_shadow(P, &counters[2], wantEnv, envO, envD, p, 1u);
-        push_shadow(P, &counters[2], wantEmis, emO, emD, p, 2u);
+        if (wantEnv) {
+            store_shadow(P, envSlot, envO, envD, sh, 1u);
+        }
+        if (wantEmis) {
+            store_shadow(P, emisSlot, emO, emD, sh, 2u);
+        }
     }
 }
 
@@ -876,24 +939,28 @@ size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spil
     char* cur = base;
     Paths& P = w.P;
     const size_t c = static_cast<size_t>(cap);
-    float** fs[] = {&P.Tx, &P.Ty, &P.Tz, &P.Lx, &P.Ly, &P.Lz, &P.eta, &P.pdf, &P.ox, &P.oy, &P.oz, &P.dx, &P.dy,
-                    &P.dz, &P.tmin, &P.cex, &P.cey, &P.cez, &P.cmx, &P.cmy, &P.cmz, &P.rx, &P.ry, &P.rz,
-                    &P.Tw, &P.Lw, &P.cew, &P.cmw, &P.hero};
-    for (float** f : fs) {
-        *f = carve<float>(cur, c);
+    for (PathState& S : P.S) {
+        float** fs[] = {&S.Tx,  &S.Ty,  &S.Tz,  &S.Tw,  &S.Lx,  &S.Ly,  &S.Lz,  &S.Lw,  &S.eta,
+                        &S.pdf, &S.ox,  &S.oy,  &S.oz,  &S.dx,  &S.dy,  &S.dz,  &S.tmin, &S.cex,
+                        &S.cey, &S.cez, &S.cew, &S.cmx, &S.cmy, &S.cmz, &S.cmw, &S.hero};
+        for (float** f : fs) {
+            *f = carve<float>(cur, c);
+        }
+        uint32_t** us[] = {&S.pixel, &S.sample, &S.seed, &S.draws, &S.flags, &S.slot0};
+        for (uint32_t** u : us) {
+            *u = carve<uint32_t>(cur, c);
+        }
+        S.occ = carve<uint8_t>(cur, 2 * c);
     }
-    uint32_t** us[] = {&P.pixel, &P.sample, &P.seed, &P.draws, &P.flags};
-    for (uint32_t** u : us) {
-        *u = carve<uint32_t>(cur, c);
-    }
-    P.occ = carve<uint8_t>(cur, 2 * c);
+    P.rx = carve<float>(cur, c);
+    P.ry = carve<float>(cur, c);
+    P.rz = carve<float>(cur, c);
     P.extQ[0] = carve<int>(cur, c);
     P.extQ[1] = carve<int>(cur, c);
     P.hitA[0] = carve<int4>(cur, c);
     P.hitA[1] = carve<int4>(cur, c);
     P.hitB[0] = carve<float2>(cur, c);
     P.hitB[1] = carve<float2>(cur, c);
-    P.shadeSlot = carve<int>(cur, c);
     P.shadowQ = carve<float4>(cur, 4 * c);
     P.counters = carve<int>(cur, kCountersPerBounce * static_cast<size_t>(depth + 2));
     P.pxBase = carve<int>(cur, static_cast<size_t>(pixels));
@@ -961,29 +1028,32 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
         prof_mark(stream, 0, false, b);
         if (gProf.counted) {
             hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, extQueue, extCount, &cIn[4], gProf.dStats, P.hitA[b & 1], P.hitB[b & 1]);
+                               d_scene, P, P.S[b & 1], extQueue, extCount, &cIn[4], gProf.dStats, P.hitA[b & 1],
+                               P.hitB[b & 1]);
         } else {
             hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, extQueue, extCount, &cIn[4], nullptr, P.hitA[b & 1], P.hitB[b & 1]);
+                               d_scene, P, P.S[b & 1], extQueue, extCount, &cIn[4], nullptr, P.hitA[b & 1],
+                               P.hitB[b & 1]);
         }
         prof_mark(stream, 0, true, b);
         if (b > 0) {
             prof_mark(stream, 1, false, b);
             if (gProf.counted) {
                 hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, nullptr, &cIn[2], &cIn[5], gProf.dStats, nullptr, nullptr);
+                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], &cIn[5], gProf.dStats, nullptr,
+                                   nullptr);
             } else {
                 hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, nullptr, &cIn[2], &cIn[5], nullptr, nullptr, nullptr);
+                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], &cIn[5], nullptr, nullptr, nullptr);
             }
             prof_mark(stream, 1, true, b);
         }
         const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
         prof_mark(stream, 2, false, b);
         hipLaunchKernelGGL(spec ? wf_shade_kernel<HeroColors> : wf_shade_kernel<RgbColors>, dim3(traceBlocks),
-                           dim3(kWfBlock), 0, stream, d_scene, d_params, d_env, P, b, maxPathDepth, P.hitA[b & 1],
-                           P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, gCapture,
-                           gCaptureSpp, sm);
+                           dim3(kWfBlock), 0, stream, d_scene, d_params, d_env, P.S[b & 1], P.S[(b + 1) & 1], P, b,
+                           maxPathDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
+                           P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp, sm);
         prof_mark(stream, 2, true, b);
     }
 }
