@@ -32,6 +32,10 @@ constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two
 constexpr int kSpillEntries = 64;  // stack entries per thread in LDS ring + global spill (tree depth <= 48)
 constexpr int kCountersPerBounce = 8;
 constexpr int kRefillLanes = 32;   // idle lanes that trigger a wave's ray refill
+// Queue entries a trace wave claims per atomic. Every wave of the grid refills from one counter;
+// claiming in chunks keeps that counter's atomic traffic low (one request per two rays per lane)
+// while leaving enough entries unclaimed for the waves to balance uneven ray costs.
+constexpr int kFetchChunk = 128;
 
 enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
@@ -192,9 +196,9 @@ struct LdsStack {
 
 constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf refs are ~slot >= -2^28)
 
-// Persistent traversal: every lane takes the next ray from the queue as soon as its current ray
-// finishes (one atomic per wave for all lanes that need a ray), so the lanes of a wave stay busy
-// while per-ray node counts vary widely. One loop iteration advances a lane by one node: an
+// Persistent traversal: every lane takes the next ray from its wave's claimed range of the queue as
+// soon as its current ray finishes (the wave claims kFetchChunk entries per atomic), so the lanes
+// of a wave stay busy while per-ray node counts vary widely. One loop iteration advances a lane by one node: an
 // internal node (both child boxes tested, near child first, far child deferred on the LDS stack
 // with its entry distance), or a leaf triangle, followed by pops that skip deferred children whose
 // entry distance exceeds the current closest t. Visit order and culling equal pt::traverse.
@@ -219,6 +223,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     TraceStats ts{0u, 0u, 0u, 0u};
     const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
     int item = -1;        // queue index of the lane's current ray
+    int wNext = 0, wEnd = 0; // the wave's unclaimed range of queue entries (same in every lane)
     bool exhausted = false;
     uint32_t tag = 0u;    // slot (extension) or slot * 4 + type (shadow)
     int bestMat = -1;
@@ -235,14 +240,21 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
         const unsigned long long m = __ballot(need);
         const unsigned long long busy = __ballot(item >= 0);
         if (m != 0ull && (__popcll(m) >= kRefillLanes || busy == 0ull)) {
-            const int leader = __ffsll(static_cast<long long>(m)) - 1;
-            int base = 0;
-            if (lane_id() == leader) {
-                base = atomicAdd(fetchCounter, __popcll(m));
+            if (wNext >= wEnd) {
+                // a new range of kFetchChunk queue entries for this wave: one atomic per chunk
+                int base = 0;
+                if (lane_id() == 0) {
+                    base = atomicAdd(fetchCounter, kFetchChunk);
+                }
+                base = __shfl(base, 0, 64);
+                wNext = base;
+                wEnd = base + kFetchChunk;
             }
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const int idx = base + __popcll(m & lowerLanes);
+            const int take = min(__popcll(m), wEnd - wNext);
+            const int rank = __popcll(m & lowerLanes);
+            const int idx = wNext + rank;
+            wNext += take;
+            if (need && rank < take) {
                 if (idx < n) {
                     item = idx;
                     if (ANY_HIT) {
