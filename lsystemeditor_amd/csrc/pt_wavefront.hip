@@ -30,12 +30,18 @@ namespace {
 constexpr int kWfBlock = 256;
 constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
 constexpr int kSpillEntries = 64;  // stack entries per thread in LDS ring + global spill (tree depth <= 48)
-constexpr int kCountersPerBounce = 8;
+constexpr int kFetchParts = 16;   // ray-queue partitions with a fetch counter each (trace kernels)
+constexpr int kFetchStride = 16;  // ints between fetch counters: one 64-byte line each
+// Per bounce: [0] ext, [1] shade, [2] shadow rays produced, [3] generated paths, then the
+// extension and the shadow queue's fetch counters (kFetchParts each, kFetchStride apart).
+constexpr int kFetchOffset = 16;
+constexpr int kCountersPerBounce = kFetchOffset + 2 * kFetchParts * kFetchStride;
 constexpr int kRefillLanes = 32;   // idle lanes that trigger a wave's ray refill
-// Queue entries a trace wave claims per atomic. Every wave of the grid refills from one counter;
-// claiming in chunks keeps that counter's atomic traffic low (one request per two rays per lane)
-// while leaving enough entries unclaimed for the waves to balance uneven ray costs.
-constexpr int kFetchChunk = 128;
+// Queue entries a trace wave claims per atomic. The queue is split into kFetchParts contiguous
+// parts with a counter each; a wave starts on part blockIdx % kFetchParts and moves to the next
+// part when its own runs dry. Chunks and parts keep every counter's atomic traffic low while
+// leaving enough entries unclaimed for the waves to balance uneven ray costs.
+constexpr int kFetchChunk = 64;
 
 enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
@@ -66,8 +72,7 @@ struct Paths {
     float2* hitB[2];
     int* extQ[2];   // extension ray queues (slots of the next shade queue)
     float4* shadowQ; // 2 per ray: (origin, slot*4 + type), (direction, -)
-    int* counters;  // per bounce: [0] ext, [1] shade, [2] shadow rays produced, [3] paths (gen, bounce 0),
-                    // [4] extension and [5] shadow fetch positions of the trace kernels
+    int* counters;  // kCountersPerBounce per bounce (layout at kCountersPerBounce)
     int* pxBase;
     int* pxK;
     int2* spill;
@@ -208,7 +213,7 @@ constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf
 template <bool ANY_HIT, bool STATS>
 __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
                                                             PathState PS, const int* __restrict__ queue,
-                                                            const int* __restrict__ countPtr, int* fetchCounter,
+                                                            const int* __restrict__ countPtr, int* fetchCounters,
                                                             unsigned long long* stats, int4* __restrict__ hitA,
                                                             float2* __restrict__ hitB)
 {
@@ -224,6 +229,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
     int item = -1;        // queue index of the lane's current ray
     int wNext = 0, wEnd = 0; // the wave's unclaimed range of queue entries (same in every lane)
+    int part = static_cast<int>(blockIdx.x % kFetchParts);
+    int tried = 0;           // parts found drained
     bool exhausted = false;
     uint32_t tag = 0u;    // slot (extension) or slot * 4 + type (shadow)
     int bestMat = -1;
@@ -240,22 +247,35 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
         const unsigned long long m = __ballot(need);
         const unsigned long long busy = __ballot(item >= 0);
         if (m != 0ull && (__popcll(m) >= kRefillLanes || busy == 0ull)) {
-            if (wNext >= wEnd) {
-                // a new range of kFetchChunk queue entries for this wave: one atomic per chunk
-                int base = 0;
-                if (lane_id() == 0) {
-                    base = atomicAdd(fetchCounter, kFetchChunk);
+            if (wNext >= wEnd && tried < kFetchParts) {
+                // a new range of at most kFetchChunk entries from the current part
+                while (true) {
+                    const int lo = static_cast<int>((static_cast<long long>(n) * part) / kFetchParts);
+                    const int hi = static_cast<int>((static_cast<long long>(n) * (part + 1)) / kFetchParts);
+                    int base = 0;
+                    if (lane_id() == 0) {
+                        base = atomicAdd(&fetchCounters[kFetchStride * part], kFetchChunk);
+                    }
+                    base = __shfl(base, 0, 64);
+                    const int start = lo + base;
+                    if (start < hi) {
+                        wNext = start;
+                        wEnd = min(start + kFetchChunk, hi);
+                        break;
+                    }
+                    part = part + 1 == kFetchParts ? 0 : part + 1;
+                    if (++tried == kFetchParts) {
+                        wNext = wEnd = n;  // every part is drained
+                        break;
+                    }
                 }
-                base = __shfl(base, 0, 64);
-                wNext = base;
-                wEnd = base + kFetchChunk;
             }
             const int take = min(__popcll(m), wEnd - wNext);
             const int rank = __popcll(m & lowerLanes);
             const int idx = wNext + rank;
             wNext += take;
             if (need && rank < take) {
-                if (idx < n) {
+                if (idx < wEnd) {
                     item = idx;
                     if (ANY_HIT) {
                         const float4 a = P.shadowQ[2 * idx];
@@ -287,6 +307,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 } else {
                     exhausted = true;
                 }
+            } else if (need && tried >= kFetchParts) {
+                exhausted = true;
             }
         }
         if (__ballot(item >= 0) == 0ull) {
@@ -1034,17 +1056,19 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
         static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
     for (int b = 0; b <= maxPathDepth; ++b) {
         int* cIn = P.counters + kCountersPerBounce * b;        // produced by bounce b-1 (or gen)
+        int* extFetch = cIn + kFetchOffset;
+        int* shadowFetch = extFetch + kFetchParts * kFetchStride;
         int* cOut = P.counters + kCountersPerBounce * (b + 1); // produced by this bounce
         const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
         const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
         prof_mark(stream, 0, false, b);
         if (gProf.counted) {
             hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, P.S[b & 1], extQueue, extCount, &cIn[4], gProf.dStats, P.hitA[b & 1],
+                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, gProf.dStats, P.hitA[b & 1],
                                P.hitB[b & 1]);
         } else {
             hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, P.S[b & 1], extQueue, extCount, &cIn[4], nullptr, P.hitA[b & 1],
+                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, nullptr, P.hitA[b & 1],
                                P.hitB[b & 1]);
         }
         prof_mark(stream, 0, true, b);
@@ -1052,11 +1076,11 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
             prof_mark(stream, 1, false, b);
             if (gProf.counted) {
                 hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], &cIn[5], gProf.dStats, nullptr,
+                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, gProf.dStats, nullptr,
                                    nullptr);
             } else {
                 hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], &cIn[5], nullptr, nullptr, nullptr);
+                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, nullptr, nullptr, nullptr);
             }
             prof_mark(stream, 1, true, b);
         }
