@@ -185,6 +185,8 @@ PT_DEV V3 clamp_vjp(V3 C, V3 g)
     return mk(C.x > 0.0f ? gr.x : 0.0f, C.y > 0.0f ? gr.y : 0.0f, C.z > 0.0f ? gr.z : 0.0f);
 }
 
+// lg points at this lane's copy of the block's per-material FP64 sums (lanes of a wave spread over
+// several copies, so one LDS atomic instruction rarely has many lanes on the same address).
 PT_DEV void flush(double* lg, int mi, const float acc[kGradParams])
 {
     for (int k = 0; k < kGradParams; ++k) {
@@ -350,12 +352,15 @@ __global__ __launch_bounds__(kAdBlock) void adjoint_kernel(const float* __restri
                                                            const EnvironmentMapGpu* env,
                                                            const RenderParamsGpu* params, int materialCount,
                                                            const float* __restrict__ sampleRadiance,
-                                                           double* __restrict__ grad)
+                                                           double* __restrict__ grad, int copies)
 {
-    __shared__ double lg[kMaxGradMaterials * kGradParams];
-    for (int k = static_cast<int>(threadIdx.x); k < kMaxGradMaterials * kGradParams; k += kAdBlock) {
-        lg[k] = 0.0;
+    // `copies` replicas of the per-material sums in dynamic LDS; lane l adds to replica l % copies
+    extern __shared__ double lgAll[];
+    const int perCopy = materialCount * kGradParams;
+    for (int k = static_cast<int>(threadIdx.x); k < copies * perCopy; k += kAdBlock) {
+        lgAll[k] = 0.0;
     }
+    double* lg = lgAll + (static_cast<int>(threadIdx.x) % copies) * perCopy;
     __syncthreads();
     const MeshAccelSceneGpu s = *scene;
     const RenderParamsGpu p = *params;
@@ -396,9 +401,13 @@ __global__ __launch_bounds__(kAdBlock) void adjoint_kernel(const float* __restri
         }
     }
     __syncthreads();
-    for (int k = static_cast<int>(threadIdx.x); k < materialCount * kGradParams; k += kAdBlock) {
-        if (lg[k] != 0.0) {
-            atomicAdd(&grad[k], lg[k]);
+    for (int k = static_cast<int>(threadIdx.x); k < perCopy; k += kAdBlock) {
+        double sum = 0.0;
+        for (int c = 0; c < copies; ++c) {
+            sum += lgAll[c * perCopy + k];
+        }
+        if (sum != 0.0) {
+            atomicAdd(&grad[k], sum);
         }
     }
 }
@@ -615,9 +624,15 @@ extern "C" bool pathTracerAdjointRender(const float* d_gradImage, int width, int
     }
     const long long total = static_cast<long long>(width) * height * samplesPerPixel;
     const long long blocks = std::min<long long>((total + kAdBlock - 1) / kAdBlock, static_cast<long long>(cus) * 16);
-    hipLaunchKernelGGL(adjoint_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kAdBlock), 0, stream, d_gradImage,
+    // LDS replicas of the per-material sums: up to 16, within 24 KiB per block
+    int copies = 16;
+    while (copies > 1 && static_cast<size_t>(copies) * materialCount * kGradParams * sizeof(double) > 24576) {
+        copies >>= 1;
+    }
+    const size_t lds = static_cast<size_t>(copies) * materialCount * kGradParams * sizeof(double);
+    hipLaunchKernelGGL(adjoint_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kAdBlock), lds, stream, d_gradImage,
                        width, height, samplesPerPixel, d_camera, d_scene, d_env, d_params, materialCount,
-                       d_sampleRadiance, d_grad);
+                       d_sampleRadiance, d_grad, copies);
     if (hipGetLastError() != hipSuccess) {
         ptSetError("pathTracerAdjointRender: launch failed");
         return false;

@@ -426,6 +426,21 @@ constexpr float kPi = 3.14159265f;
 constexpr float kInvPi = 1.0f / kPi;
 constexpr float kMinRadius = 1.0e-5f;
 constexpr int kStackSize = 64; // traversal stack entries (tree depth <= 48 for every build mode)
+constexpr int kRingEntries = 8; // entries of a thread's traversal stack kept in LDS
+constexpr int kMaxTraceBlock = 256; // threads per block of every kernel that traverses
+
+typedef int LdsNativeInt2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) LdsNativeInt2 LdsStackEntry;
+
+// The block's LDS area for the first kRingEntries entries of every thread's traversal stack
+// (entry e of thread t at e * kMaxTraceBlock + t, so a wave's accesses are conflict-free). One
+// allocation per kernel, shared by every traversal in it: a thread runs one traversal at a time.
+PT_DEV LdsStackEntry* traversal_ring()
+{
+    __shared__ LdsNativeInt2 ring[kRingEntries * kMaxTraceBlock];
+    const int tid = static_cast<int>(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    return (LdsStackEntry*)(ring + tid);
+}
 
 // Loads through the global address space: global_load instructions (vector memory counter
 // only) instead of flat ones, which also count against the LDS counter.
@@ -584,8 +599,10 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
     if (!slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], tMin, tMax, te)) {
         return false;
     }
-    int stackRef[kStackSize];
-    float stackT[kStackSize];
+    // stack entries below kRingEntries live in LDS, deeper ones in a private array
+    LdsStackEntry* ring = traversal_ring();
+    int stackRef[kStackSize - kRingEntries];
+    float stackT[kStackSize - kRingEntries];
     int sp = 0;
     int cur = s.rootRef;
     while (true) {
@@ -625,9 +642,15 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
                 // The median-split tree of at most 2^28 triangles is at most 28 levels deep, so
                 // the deferred-child stack never holds more than 28 entries; the guard only
                 // keeps the private array in bounds.
-                if (sp < kStackSize) {
-                    stackRef[sp] = leftFirst ? right : left;
-                    stackT[sp] = leftFirst ? re : le;
+                if (sp < kRingEntries) {
+                    LdsNativeInt2 e;
+                    e.x = leftFirst ? right : left;
+                    e.y = __float_as_int(leftFirst ? re : le);
+                    ring[sp * kMaxTraceBlock] = e;
+                    ++sp;
+                } else if (sp < kStackSize) {
+                    stackRef[sp - kRingEntries] = leftFirst ? right : left;
+                    stackT[sp - kRingEntries] = leftFirst ? re : le;
                     ++sp;
                 }
                 cur = leftFirst ? left : right;
@@ -645,8 +668,18 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
         bool found = false;
         while (sp > 0) {
             --sp;
-            if (!(stackT[sp] > bestT)) {
-                cur = stackRef[sp];
+            int ref;
+            float t;
+            if (sp < kRingEntries) {
+                const LdsNativeInt2 e = ring[sp * kMaxTraceBlock];
+                ref = e.x;
+                t = __int_as_float(e.y);
+            } else {
+                ref = stackRef[sp - kRingEntries];
+                t = stackT[sp - kRingEntries];
+            }
+            if (!(t > bestT)) {
+                cur = ref;
                 found = true;
                 break;
             }
