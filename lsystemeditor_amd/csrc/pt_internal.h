@@ -23,4 +23,11 @@ bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, 
 // range) the kernels then read.
 bool pt_spectral_enabled();
 pt::SpecModel pt_spectral_model();
+
+// Adjoint of the fixed-spp render (pathTracerAdjointRender after argument checks), scheduled on
+// the wavefront machinery (pt_wavefront.hip).
+bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int spp, const CameraGpu* d_camera,
+                          const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                          const RenderParamsGpu* d_params, int materialCount, const float* d_sampleRadiance,
+                          double* d_grad, hipStream_t stream);
 #endif

@@ -14,6 +14,7 @@
 //               handles hits and misses, samples the lights and the BSDF, and appends the
 //               next rays to queues (wave-aggregated atomics).
 #include "pt_device.h"
+#include "pt_adjoint_math.h"
 #include "pt_integrator.h"
 #include "pt_internal.h"
 
@@ -414,7 +415,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
                                                           const uint8_t* __restrict__ converged,
                                                           const CameraGpu* camera, const RenderParamsGpu* params,
                                                           int width, int height, uint32_t firstSeed, int passBase,
-                                                          int passCount, int groupSize, Paths P, bool spectral)
+                                                          int passCount, int groupSize, Paths P, bool spectral,
+                                                          int sampleLimit)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int activeCount = *activeCountPtr;
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         if (idx >= 0 && converged[idx] == 0) {
             n = counts[idx];
             k = min(groupSize, passCount - passBase);
-            const int maxS = params->maxSamplesPerPixel;
+            const int maxS = sampleLimit >= 0 ? sampleLimit : params->maxSamplesPerPixel;
             if (maxS > 0) {
                 k = min(k, maxS - static_cast<int>(n));
             }
@@ -789,6 +791,397 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
     }
 }
 
+// ----------------------------------------------------------------------------------------
+// Adjoint (path replay) of the fixed-spp render on the same schedule. Every path is the primal
+// path (same RNG use and decisions as wf_shade_kernel); per vertex the adjoint accumulates the
+// derivatives of that vertex's contributions with respect to its material: emission, the
+// next-event BSDF values and the BSDF value of the sampled continuation, which scales everything
+// the path gathers afterwards, R = C - (what the path has gathered so far). A next-event
+// contribution only counts when its shadow ray is clear, which the next bounce learns; since the
+// BSDF derivative is linear in its weight vector, the vertex's continuation term is taken with R
+// excluding its own next-event values and the two corrections (next-event derivative minus the
+// continuation term of the next-event value) wait with the shadow rays as pending terms.
+// Derivatives are FP32 per vertex; the sums are FP64 (replicated LDS per block, then the block's
+// row of a partial-sum buffer, reduced once per adjoint).
+// ----------------------------------------------------------------------------------------
+struct AdjointState {
+    float *gx, *gy, *gz;             // dLoss/dRadiance of the path's sample (after the clamp VJP)
+    float* pe[kGradParams];          // pending terms of the previous vertex's environment NEE
+    float* pm[kGradParams];          // pending terms of its emissive NEE
+    int* pmi;                        // that vertex's material (-1: nothing pending)
+};
+
+// gw and R = C for the paths of a group (slot p = generation order).
+__global__ __launch_bounds__(kWfBlock) void wf_adjoint_init_kernel(PathState S, AdjointState A,
+                                                                   const int* __restrict__ countPtr,
+                                                                   const float* __restrict__ gradImg,
+                                                                   const float* __restrict__ radiance, int spp,
+                                                                   float invSpp)
+{
+    const int p = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= *countPtr) {
+        return;
+    }
+    const uint32_t px = S.pixel[p];
+    const size_t q = 3 * (static_cast<size_t>(px) * spp + S.sample[p]);
+    const V3 C = mk(radiance[q], radiance[q + 1], radiance[q + 2]);
+    const V3 gs = mk(gradImg[3 * px] * invSpp, gradImg[3 * px + 1] * invSpp, gradImg[3 * px + 2] * invSpp);
+    const V3 gw = clamp_vjp(C, gs);
+    A.gx[p] = gw.x;
+    A.gy[p] = gw.y;
+    A.gz[p] = gw.z;
+    A.pmi[p] = -1;
+    S.Lx[p] = C.x;
+    S.Ly[p] = C.y;
+    S.Lz[p] = C.z;
+    if (gw.x == 0.0f && gw.y == 0.0f && gw.z == 0.0f) {
+        S.flags[p] = 0u;  // nothing to differentiate: the path stops at its first shade
+    }
+}
+
+PT_DEV V3 safe_ratio(V3 g, V3 x, V3 f)
+{
+    return mk(f.x != 0.0f ? g.x * x.x / f.x : 0.0f, f.y != 0.0f ? g.y * x.y / f.y : 0.0f,
+              f.z != 0.0f ? g.z * x.z / f.z : 0.0f);
+}
+
+__global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
+    const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
+    PathState O, AdjointState AI, AdjointState AO, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA,
+    const float2* __restrict__ inB, const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA,
+    int* counters, int materialCount, int copies, double* __restrict__ partial)
+{
+    __shared__ BlockAppendLds appendLds;
+    extern __shared__ double lgAll[];
+    const int perCopy = materialCount * kGradParams;
+    for (int k = static_cast<int>(threadIdx.x); k < copies * perCopy; k += kWfBlock) {
+        lgAll[k] = 0.0;
+    }
+    double* lg = lgAll + (static_cast<int>(threadIdx.x) % copies) * perCopy;
+    __syncthreads();
+    const MeshAccelSceneGpu s = *scene;
+    const RenderParamsGpu prm = *params;
+    const int n = *inCountPtr;
+    const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    const int trips = (n + stride - 1) / stride;
+    for (int it = 0; it < trips; ++it) {
+        const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
+        const bool live = i < n;
+        bool wantExt = false, wantEnv = false, wantEmis = false;
+        V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO, nextO = envO, nextD = envO;
+        V3 R = envO, T = envO, ce = envO, cm = envO, gw = envO;
+        float e1[kGradParams], e2[kGradParams];
+        for (int k = 0; k < kGradParams; ++k) {
+            e1[k] = 0.0f;
+            e2[k] = 0.0f;
+        }
+        float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f;
+        uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u;
+        int pendMat = -1;
+        if (live) {
+            const int4 ha = inA[i];
+            const float2 hb = inB[i];
+            const uint32_t flags = I.flags[i];
+            pixel = I.pixel[i];
+            sample = I.sample[i];
+            seed = I.seed[i];
+            slot0 = I.slot0[i];
+            gw = mk(AI.gx[i], AI.gy[i], AI.gz[i]);
+            R = mk(I.Lx[i], I.Ly[i], I.Lz[i]);
+            // the previous vertex's next-event terms, in the forward's order (env, then emissive)
+            const int pmi = AI.pmi[i];
+            float prev[kGradParams];
+            for (int k = 0; k < kGradParams; ++k) {
+                prev[k] = 0.0f;
+            }
+            if ((flags & kFlagEnv) != 0u && I.occ[2 * i] == 0u) {
+                R = sub(R, mk(I.cex[i], I.cey[i], I.cez[i]));
+                if (pmi >= 0) {
+                    for (int k = 0; k < kGradParams; ++k) {
+                        prev[k] += AI.pe[k][i];
+                    }
+                }
+            }
+            if ((flags & kFlagEmis) != 0u && I.occ[2 * i + 1] == 0u) {
+                R = sub(R, mk(I.cmx[i], I.cmy[i], I.cmz[i]));
+                if (pmi >= 0) {
+                    for (int k = 0; k < kGradParams; ++k) {
+                        prev[k] += AI.pm[k][i];
+                    }
+                }
+            }
+            if (pmi >= 0) {
+                flush(lg, pmi, prev);
+            }
+            const uint32_t tri = static_cast<uint32_t>(ha.y);
+            if ((flags & kFlagExt) != 0u && tri != 0xFFFFFFFFu && bounce < maxDepth) {
+                const V3 origin = mk(I.ox[i], I.oy[i], I.oz[i]);
+                const V3 dir = mk(I.dx[i], I.dy[i], I.dz[i]);
+                const float t = __int_as_float(ha.w);
+                const Ray ray = make_ray(origin, dir);
+                Hit h;
+                h.hit = true;
+                h.t = t;
+                finish_hit(s, ray, tri, hb.x, hb.y, h);
+                Rng rng;
+                rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws[i]);
+                T = mk(I.Tx[i], I.Ty[i], I.Tz[i]);
+                const V3 pos = add(origin, scale(dir, t));
+                Ctx c;
+                c.n = h.normal;
+                c.wo = scale(dir, -1.0f);
+                const uint32_t mi = static_cast<uint32_t>(ha.z);
+                const bool valid = s.materials != nullptr && tri < s.triangleCount && mi < s.materialCount;
+                c.m = resolve_material_index(s, mi, valid, h.uvx, h.uvy);
+                c.w = lobe_weights(c.m);
+                c.eta = I.eta[i];
+                c.flags = 0;
+                int gmi = -1;
+                Live lm{false, false, false, false, false, false, false};
+                if (valid && mi < static_cast<uint32_t>(materialCount)) {
+                    gmi = static_cast<int>(mi);
+                    lm = live_of(s.materials[mi], c.m, s);
+                }
+                float acc[kGradParams];
+                for (int k = 0; k < kGradParams; ++k) {
+                    acc[k] = 0.0f;
+                }
+                // emission
+                R = sub(R, mul(T, mk(c.m.r * c.m.emission, c.m.g * c.m.emission, c.m.b * c.m.emission)));
+                if (lm.emis) {
+                    acc[6] += gw.x * T.x * c.m.r + gw.y * T.y * c.m.g + gw.z * T.z * c.m.b;
+                }
+                if (lm.rgb) {
+                    acc[0] += gw.x * T.x * c.m.emission;
+                    acc[1] += gw.y * T.y * c.m.emission;
+                    acc[2] += gw.z * T.z * c.m.emission;
+                }
+                // environment NEE
+                if (!skips_env_nee(c.m)) {
+                    const float u1 = rand01(rng);
+                    const float u2 = rand01(rng);
+                    float lightPdf = 0.0f;
+                    const V3 wi = env_sample(env, u1, u2, lightPdf);
+                    if (lightPdf > kMinPdf) {
+                        const float cosT = max2(0.0f, dot(c.n, wi));
+                        if (cosT > 0.0f) {
+                            const V3 f = brdf_eval(c, wi);
+                            const float mis = mis_balance(lightPdf, brdf_pdf(c, wi));
+                            const V3 w = scale(env_eval(env, prm, wi), mis * cosT / lightPdf);
+                            ce = mul(T, mul(f, w));
+                            if (gmi >= 0) {
+                                brdf_grad(c, wi, mul(gw, mul(T, w)), lm, e1);
+                            }
+                            wantEnv = true;
+                            envO = add(pos, scale(c.n, kShadowBias));
+                            envD = wi;
+                        }
+                    }
+                }
+                // emissive-triangle NEE
+                if (s.emissiveTriangleCount != 0u) {
+                    const float uTri = rand01(rng);
+                    const float u1 = rand01(rng);
+                    const float u2 = rand01(rng);
+                    V3 lp, ln, lr;
+                    float areaPdf = 0.0f;
+                    if (sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
+                        const V3 toL = sub(lp, pos);
+                        const float d2 = dot(toL, toL);
+                        if (d2 > kMinPdf) {
+                            const V3 wi = normalize(toL);
+                            const float cosT = max2(0.0f, dot(c.n, wi));
+                            const float cosL = max2(0.0f, dot(ln, scale(wi, -1.0f)));
+                            if (cosT > 0.0f && cosL > 0.0f) {
+                                const float lightPdf = areaPdf * d2 / cosL;
+                                if (lightPdf > kMinPdf) {
+                                    const V3 f = brdf_eval(c, wi);
+                                    const float mis = mis_balance(lightPdf, brdf_pdf(c, wi));
+                                    const V3 w = scale(lr, mis * cosT / lightPdf);
+                                    cm = mul(T, mul(f, w));
+                                    if (gmi >= 0) {
+                                        brdf_grad(c, wi, mul(gw, mul(T, w)), lm, e2);
+                                    }
+                                    wantEmis = true;
+                                    emO = add(pos, scale(wi, kShadowBias));
+                                    emD = wi;
+                                }
+                            }
+                        }
+                    }
+                }
+                // BSDF sampling: the continuation term, then throughput and Russian roulette
+                c.flags = prm.brdfDebugFlags;
+                const float u1 = rand01(rng);
+                const float u2 = rand01(rng);
+                const Sample smp = brdf_sample(c, u1, u2);
+                bool cont = smp.valid && smp.pdf > kMinPdf;
+                if (cont) {
+                    const V3 f = brdf_eval(c, smp.dir);
+                    if (gmi >= 0) {
+                        brdf_grad(c, smp.dir, safe_ratio(gw, R, f), lm, acc);
+                        if (wantEnv) {
+                            float g[kGradParams] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                            brdf_grad(c, smp.dir, safe_ratio(gw, ce, f), lm, g);
+                            for (int k = 0; k < kGradParams; ++k) {
+                                e1[k] -= g[k];
+                            }
+                        }
+                        if (wantEmis) {
+                            float g[kGradParams] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                            brdf_grad(c, smp.dir, safe_ratio(gw, cm, f), lm, g);
+                            for (int k = 0; k < kGradParams; ++k) {
+                                e2[k] -= g[k];
+                            }
+                        }
+                    }
+                    T = apply_throughput(T, c, smp, f);
+                    T = subsurface_tier_b(T, c.m, smp, rng);
+                    if ((c.flags & 16) != 0) {
+                        T = smp.transmitted ? mul(T, mk(0.2f, 1.0f, 0.2f)) : mul(T, mk(1.0f, 0.2f, 0.2f));
+                    }
+                    const float tl = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+                    if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
+                        cont = false;
+                    }
+                    if (cont && bounce >= rrMin) {
+                        const float surv = min2(0.95f, max2(tl, 0.05f));
+                        if (rand01(rng) > surv) {
+                            cont = false;
+                        } else {
+                            T = scale(T, 1.0f / surv);
+                        }
+                    }
+                }
+                if (gmi >= 0) {
+                    flush(lg, gmi, acc);
+                    if (wantEnv || wantEmis) {
+                        pendMat = gmi;
+                    }
+                }
+                if (cont) {
+                    nextTmin = max2(kShadowBias, kRelShadowBias * t);
+                    const float sgn = dot(c.n, smp.dir) >= 0.0f ? 1.0f : -1.0f;
+                    nextO = add(pos, scale(c.n, sgn * nextTmin));
+                    nextD = smp.dir;
+                    nextEta = smp.nextEta;
+                    nextPdf = smp.pdf;
+                    wantExt = true;
+                }
+                draws = rng_draws(rng);
+            }
+            newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
+        }
+        const bool next = wantExt || wantEnv || wantEmis;
+        int ext, sh, envSlot, emisSlot;
+        block_append(appendLds, counters, wantExt, next, wantEnv, wantEmis, ext, sh, envSlot, emisSlot);
+        if (wantExt) {
+            extOut[ext] = sh;
+        }
+        if (next) {
+            outA[sh] = make_int4(sh, -1, -1, 0);
+            O.flags[sh] = newFlags;
+            O.pixel[sh] = pixel;
+            O.sample[sh] = sample;
+            O.seed[sh] = seed;
+            O.slot0[sh] = slot0;
+            O.Lx[sh] = R.x;
+            O.Ly[sh] = R.y;
+            O.Lz[sh] = R.z;
+            AO.gx[sh] = gw.x;
+            AO.gy[sh] = gw.y;
+            AO.gz[sh] = gw.z;
+            AO.pmi[sh] = pendMat;
+            if (wantEnv) {
+                O.cex[sh] = ce.x;
+                O.cey[sh] = ce.y;
+                O.cez[sh] = ce.z;
+                for (int k = 0; k < kGradParams; ++k) {
+                    AO.pe[k][sh] = e1[k];
+                }
+            }
+            if (wantEmis) {
+                O.cmx[sh] = cm.x;
+                O.cmy[sh] = cm.y;
+                O.cmz[sh] = cm.z;
+                for (int k = 0; k < kGradParams; ++k) {
+                    AO.pm[k][sh] = e2[k];
+                }
+            }
+            if (wantExt) {
+                O.ox[sh] = nextO.x;
+                O.oy[sh] = nextO.y;
+                O.oz[sh] = nextO.z;
+                O.dx[sh] = nextD.x;
+                O.dy[sh] = nextD.y;
+                O.dz[sh] = nextD.z;
+                O.tmin[sh] = nextTmin;
+                O.eta[sh] = nextEta;
+                O.pdf[sh] = nextPdf;
+                O.draws[sh] = draws;
+                O.Tx[sh] = T.x;
+                O.Ty[sh] = T.y;
+                O.Tz[sh] = T.z;
+            }
+        }
+        if (wantEnv) {
+            store_shadow(P, envSlot, envO, envD, sh, 1u);
+        }
+        if (wantEmis) {
+            store_shadow(P, emisSlot, emO, emD, sh, 2u);
+        }
+    }
+    // this block's sums into its row of the partial-sum buffer (rows are block-private)
+    __syncthreads();
+    for (int k = static_cast<int>(threadIdx.x); k < perCopy; k += kWfBlock) {
+        double sum = 0.0;
+        for (int c = 0; c < copies; ++c) {
+            sum += lgAll[c * perCopy + k];
+        }
+        partial[static_cast<size_t>(blockIdx.x) * perCopy + k] += sum;
+    }
+}
+
+// grad[k] += sum over block rows of partial[row][k]
+__global__ void wf_adjoint_reduce_kernel(const double* __restrict__ partial, int rows, int perRow,
+                                         double* __restrict__ grad)
+{
+    const int k = static_cast<int>(blockIdx.x);
+    double sum = 0.0;
+    for (int r = static_cast<int>(threadIdx.x); r < rows; r += static_cast<int>(blockDim.x)) {
+        sum += partial[static_cast<size_t>(r) * perRow + k];
+    }
+    __shared__ double red[kWfBlock];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = kWfBlock / 2; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+            red[threadIdx.x] += red[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        grad[k] += red[0];
+    }
+}
+
+__global__ void wf_fill_kernel(uint32_t* p, int n, uint32_t v)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) {
+        p[i] = v;
+    }
+}
+
+__global__ void wf_zero_f64_kernel(double* p, long long n)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = 0.0;
+    }
+}
+
 // Applies each active pixel's group samples in order (PathTracer.cu:233-272).
 __global__ __launch_bounds__(kWfBlock) void wf_accumulate_kernel(const int* __restrict__ active,
                                                                  const int* __restrict__ activeCountPtr, Paths P,
@@ -1047,9 +1440,10 @@ unsigned blocks_for(long long n) { return static_cast<unsigned>(std::max<long lo
 // One group's bounce loop: per bounce, the closest-hit trace of the extension queue, the any-hit
 // trace of the shadow queue (from bounce 1 on) and the shade launch that fills the next queues.
 // Launch counts depend only on the host-known depth and path bound (no readbacks).
-void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, const RenderParamsGpu* d_params,
-                    const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks,
-                    bool spec, const SpecModel& sm)
+// shade(b, shadeCount, cOut, traceBlocks) launches bounce b's shade kernel.
+template <class ShadeLaunch>
+void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, int maxPathDepth,
+                         long long maxPaths, int persistentBlocks, ShadeLaunch shade)
 {
     const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
     const unsigned shadowBlocks =
@@ -1086,12 +1480,23 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
         }
         const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
         prof_mark(stream, 2, false, b);
-        hipLaunchKernelGGL(spec ? wf_shade_kernel<HeroColors> : wf_shade_kernel<RgbColors>, dim3(traceBlocks),
-                           dim3(kWfBlock), 0, stream, d_scene, d_params, d_env, P.S[b & 1], P.S[(b + 1) & 1], P, b,
-                           maxPathDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
-                           P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp, sm);
+        shade(b, shadeCount, cOut, traceBlocks);
         prof_mark(stream, 2, true, b);
     }
+}
+
+void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, const RenderParamsGpu* d_params,
+                    const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks,
+                    bool spec, const SpecModel& sm)
+{
+    launch_bounces_with(stream, P, d_scene, maxPathDepth, maxPaths, persistentBlocks,
+                        [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
+                            hipLaunchKernelGGL(spec ? wf_shade_kernel<HeroColors> : wf_shade_kernel<RgbColors>,
+                                               dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
+                                               P.S[b & 1], P.S[(b + 1) & 1], P, b, maxPathDepth, P.hitA[b & 1],
+                                               P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1],
+                                               cOut, gCapture, gCaptureSpp, sm);
+                        });
 }
 
 extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
@@ -1170,7 +1575,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur,
                            d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
-                           passCount, group, P, spec);
+                           passCount, group, P, spec, -1);
         prof_mark(stream, 3, true);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks, spec, sm);
         prof_mark(stream, 3, false);
@@ -1364,7 +1769,8 @@ extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_s
         hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
                            counterInts);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, d_samples,
-                           d_converged, d_camera, d_params, width, height, 1u, passBase, samplesPerPixel, g, P, spec);
+                           d_converged, d_camera, d_params, width, height, 1u, passBase, samplesPerPixel, g, P, spec,
+                           -1);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, static_cast<long long>(g) * pixels,
                        persistentBlocks, spec, sm);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, P,
@@ -1377,3 +1783,178 @@ extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_s
     }
     return true;
 }
+
+// ---- adjoint on the wavefront schedule ------------------------------------------------------
+
+namespace {
+
+struct AdjointWs {
+    void* mem = nullptr;
+    int capacity = 0;
+    int pixels = 0;
+    int rows = 0;
+    AdjointState A[2]{};
+    uint32_t* counts = nullptr;  // per pixel: samples taken so far (the generation kernel's n)
+    uint8_t* conv = nullptr;     // per pixel: 0
+    double* partial = nullptr;   // rows x kMaxGradMaterials x kGradParams
+};
+AdjointWs gAd;
+
+bool ensure_adjoint(int cap, int pixels, int rows)
+{
+    if (gAd.mem != nullptr && gAd.capacity >= cap && gAd.pixels >= pixels && gAd.rows >= rows) {
+        return true;
+    }
+    cap = std::max(cap, gAd.capacity);
+    pixels = std::max(pixels, gAd.pixels);
+    rows = std::max(rows, gAd.rows);
+    auto lay = [&](char* base) {
+        char* cur = base;
+        for (AdjointState& A : gAd.A) {
+            A.gx = carve<float>(cur, cap);
+            A.gy = carve<float>(cur, cap);
+            A.gz = carve<float>(cur, cap);
+            for (int k = 0; k < kGradParams; ++k) {
+                A.pe[k] = carve<float>(cur, cap);
+                A.pm[k] = carve<float>(cur, cap);
+            }
+            A.pmi = carve<int>(cur, cap);
+        }
+        gAd.counts = carve<uint32_t>(cur, pixels);
+        gAd.conv = carve<uint8_t>(cur, pixels);
+        gAd.partial = carve<double>(cur, static_cast<size_t>(rows) * kMaxGradMaterials * kGradParams);
+        return static_cast<size_t>(cur - base) + 256;
+    };
+    const size_t bytes = lay(nullptr);
+    if (gAd.mem != nullptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(gAd.mem);
+        gAd.mem = nullptr;
+    }
+    void* mem = nullptr;
+    if (hipMalloc(&mem, bytes) != hipSuccess) {
+        ptSetError("adjoint workspace allocation of %zu bytes failed", bytes);
+        gAd = AdjointWs{};
+        return false;
+    }
+    gAd.mem = mem;
+    gAd.capacity = cap;
+    gAd.pixels = pixels;
+    gAd.rows = rows;
+    lay(static_cast<char*>(mem));
+    return true;
+}
+
+} // namespace
+
+bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int spp, const CameraGpu* d_camera,
+                          const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                          const RenderParamsGpu* d_params, int materialCount, const float* d_sampleRadiance,
+                          double* d_grad, hipStream_t stream)
+{
+    RenderParamsGpu hp;
+    if (hipMemcpyAsync(&hp, d_params, sizeof(hp), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        ptSetError("pathTracerAdjointRender: reading the render parameters failed");
+        return false;
+    }
+    const int maxDepth = hp.maxPathDepth > 0 ? hp.maxPathDepth : (hp.maxPathDepth == 0 ? 256 : 32);
+    const int pixels = width * height;
+    // the primal per-sample radiance: the caller's (stored by the forward) or a fixed-spp render
+    const float* radiance = d_sampleRadiance;
+    float* tmp = nullptr;
+    if (radiance == nullptr) {
+        const size_t nRad = static_cast<size_t>(pixels) * spp * 3;
+        const size_t accBytes = static_cast<size_t>(pixels) * (16 + 4 + 4 + 4 + 1);
+        if (hipMalloc(&tmp, nRad * sizeof(float) + accBytes + 256) != hipSuccess) {
+            ptSetError("pathTracerAdjointRender: radiance buffer allocation failed");
+            return false;
+        }
+        char* cur = reinterpret_cast<char*>(tmp) + nRad * sizeof(float);
+        PtFloat4* acc = carve<PtFloat4>(cur, pixels);
+        uint32_t* cnt = carve<uint32_t>(cur, pixels);
+        float* mean = carve<float>(cur, pixels);
+        float* m2 = carve<float>(cur, pixels);
+        uint8_t* cv = carve<uint8_t>(cur, pixels);
+        float* savedCapture = gCapture;
+        const int savedSpp = gCaptureSpp;
+        gCapture = tmp;
+        gCaptureSpp = spp;
+        const bool ok = hipMemsetAsync(cnt, 0, pixels * sizeof(uint32_t), stream) == hipSuccess &&
+                        hipMemsetAsync(cv, 0, pixels, stream) == hipSuccess &&
+                        hipMemsetAsync(mean, 0, pixels * sizeof(float), stream) == hipSuccess &&
+                        hipMemsetAsync(m2, 0, pixels * sizeof(float), stream) == hipSuccess &&
+                        hipMemsetAsync(acc, 0, pixels * sizeof(PtFloat4), stream) == hipSuccess &&
+                        pathTracerRenderFixedWavefront(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene,
+                                                       d_env, d_params, spp, maxDepth, 0, stream);
+        gCapture = savedCapture;
+        gCaptureSpp = savedSpp;
+        if (!ok) {
+            (void)hipFree(tmp);
+            return false;
+        }
+        radiance = tmp;
+    }
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) == hipSuccess) {
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    const int persistentBlocks = cus * 8;
+    const int spillThreads = persistentBlocks * kWfBlock;
+    const int target = 1 << 24;
+    const int cap = std::max(pixels, std::min(target, spp * pixels));
+    bool ok = ensure_workspace(cap, pixels, maxDepth, spillThreads) && ensure_adjoint(cap, pixels, persistentBlocks);
+    if (ok) {
+        Paths P = gWs.P;
+        int* list = gWs.lists[0];
+        int* count = gWs.listCount;
+        const int counterInts = kCountersPerBounce * (maxDepth + 2);
+        const int group = std::max(1, std::min(spp, std::min(gWs.capacity, target) / pixels));
+        const int perCopy = materialCount * kGradParams;
+        int copies = 16;
+        while (copies > 1 && static_cast<size_t>(copies) * perCopy * sizeof(double) > 24576) {
+            copies >>= 1;
+        }
+        const size_t lds = static_cast<size_t>(copies) * perCopy * sizeof(double);
+        const float invSpp = 1.0f / static_cast<float>(spp);
+        hipLaunchKernelGGL(wf_iota_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, pixels);
+        hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream,
+                           reinterpret_cast<uint32_t*>(gAd.conv), (pixels + 3) / 4, 0u);
+        hipLaunchKernelGGL(wf_zero_f64_kernel, dim3(blocks_for(static_cast<long long>(persistentBlocks) * perCopy)),
+                           dim3(kWfBlock), 0, stream, gAd.partial, static_cast<long long>(persistentBlocks) * perCopy);
+        for (int passBase = 0; passBase < spp; passBase += group) {
+            const int g = std::min(group, spp - passBase);
+            hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, gAd.counts, pixels,
+                               static_cast<uint32_t>(passBase));
+            hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
+                               counterInts);
+            hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count,
+                               gAd.counts, gAd.conv, d_camera, d_params, width, height, 1u, passBase, spp, g, P, false,
+                               0);
+            hipLaunchKernelGGL(wf_adjoint_init_kernel, dim3(blocks_for(static_cast<long long>(g) * pixels)),
+                               dim3(kWfBlock), 0, stream, P.S[0], gAd.A[0], &P.counters[3], d_gradImage, radiance,
+                               spp, invSpp);
+            launch_bounces_with(stream, P, d_scene, maxDepth, static_cast<long long>(g) * pixels, persistentBlocks,
+                                [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
+                                    hipLaunchKernelGGL(wf_adjoint_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), lds,
+                                                       stream, d_scene, d_params, d_env, P.S[b & 1],
+                                                       P.S[(b + 1) & 1], gAd.A[b & 1], gAd.A[(b + 1) & 1], P, b,
+                                                       maxDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount,
+                                                       P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, materialCount,
+                                                       copies, gAd.partial);
+                                });
+        }
+        hipLaunchKernelGGL(wf_adjoint_reduce_kernel, dim3(static_cast<unsigned>(perCopy)), dim3(kWfBlock), 0, stream,
+                           gAd.partial, persistentBlocks, perCopy, d_grad);
+        if (hipGetLastError() != hipSuccess) {
+            ptSetError("pathTracerAdjointRender: launch failed");
+            ok = false;
+        }
+    }
+    if (tmp != nullptr) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(tmp);
+    }
+    return ok;
+}
+
