@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-render (C2) measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-scene (C3) measurement")
     ap.add_argument("--no-spectral", action="store_true", help="skip the hero-wavelength (spectral) measurement")
+    ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-rendering (C4) measurement")
+    ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device tensor copy (scale reference for PMC byte counters)")
     return ap.parse_args()
@@ -261,6 +263,33 @@ def spectral_bench(scene, spec, inputs, acc, passes: int, bvh: str) -> dict:
     return out
 
 
+def inverse_bench(bvh: str, steps: int) -> dict:
+    """C4: 32 bushes (iterations 3-6, 256x256, 16 spp, depth 8), 200 Adam steps (lr 1e-2) through
+    the adjoint on base colour, roughness and metallic of bark, leaf and ground; FD spot checks
+    every 50 steps."""
+    from lsystemeditor_amd import inverse
+    t0 = time.perf_counter()
+    specs = inverse.c4_specs()
+    gen_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    prob = inverse.InverseProblem(specs, bvh=bvh)
+    setup_s = time.perf_counter() - t1
+    res = inverse.optimise(prob, steps=steps, lr=1e-2, check_every=50,
+                           log=lambda m: print(f"[bench] C4 {m}", file=sys.stderr, flush=True))
+    out = {"workload": "C4: 32 bushes, seeds 0-31, iterations 3-6, 256x256, 16 spp, depth 8; Adam lr 1e-2 on "
+                       "base rgb, roughness, metallic of bark, leaf, ground (targets: seeded +-20% perturbation)",
+           "scenes": len(specs), "params": int(prob.mask.sum()) * len(specs),
+           "triangles": int(sum(s.triangles.shape[0] for s in specs)),
+           "scene_generation_s": round(gen_s, 2), "setup_s": round(setup_s, 2), **res}
+    e0, e1 = res["param_mae_initial"], res["param_mae_final"]
+    print(f"[bench] C4 {steps} steps in {res['wall_s']:.1f} s ({res['grad_samples_per_s'] / 1e6:.1f} grad-Msamples/s), "
+          f"parameter error {e0['all']:.4f} -> {e1['all']:.4f} (base colour {e0['base_color']:.4f} -> "
+          f"{e1['base_color']:.4f}), step loss {res['loss_first_steps']:.4g} -> {res['loss_last_steps']:.4g}",
+          file=sys.stderr, flush=True)
+    del prob
+    return out
+
+
 def profile_read():
     import ctypes as C
     from lsystemeditor_amd import abi
@@ -369,6 +398,9 @@ def main():
 
     if not args.no_adjoint and args.config == "c1":
         extra["adjoint"] = adjoint_bench(spec, args.bvh, args.cpu_threads or min(16, os.cpu_count() or 1))
+
+    if not args.no_inverse and args.config == "c1":
+        extra["inverse"] = inverse_bench(args.bvh, args.inverse_steps)
 
     cpu = None
     if not args.no_cpu:

@@ -326,12 +326,24 @@ bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndi
  * (untextured; roughness/metallic/specular strictly inside (0, 1); emission > 0).
  * materialCount <= 64. d_sampleRadiance (optional) holds the primal, unclamped radiance of every
  * (pixel, sample), 3 floats at (pixel * samplesPerPixel + sample), as
- * pathTracerWavefrontCaptureRadiance records it during the forward; without it each primal path
- * is traced again before its replay. No reference counterpart. */
+ * pathTracerWavefrontCaptureRadiance records it during the forward; without it the fixed-spp
+ * render of the same samples runs first. Reads the render parameters from the device (one stream
+ * synchronisation) and uses the wavefront workspace: calls on one device must not overlap.
+ * No reference counterpart. */
 bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
                              const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
                              const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, int materialCount,
                              const float* d_sampleRadiance, double* d_grad, hipStream_t stream);
+
+/* pathTracerAdjointRender over another sample set: sample k of every pixel uses seed
+ * firstSeed + k (pathTracerAdjointRender is firstSeed = 1). Replaying independent samples each
+ * optimisation step keeps the gradient's sampling error from acting as a fixed bias. No
+ * reference counterpart. */
+bool pathTracerAdjointRenderSeeded(const float* d_gradImage, int width, int height, int samplesPerPixel,
+                                   const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                   const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                   int materialCount, const float* d_sampleRadiance, double* d_grad,
+                                   unsigned int firstSeed, hipStream_t stream);
 
 /* While d_out is set, pathTracerAdaptiveRenderWavefront writes the unclamped radiance of every
  * finished sample with index < samplesPerPixel to d_out[3 * (pixel * samplesPerPixel + sample)]

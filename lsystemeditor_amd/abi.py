@@ -84,6 +84,7 @@ SIGNATURES = {
     "ptSceneSetBuildMode": (_b, [_p, _i]),
     "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
     "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
+    "pathTracerAdjointRenderSeeded": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _u, _p]),
     "pathTracerWavefrontCaptureRadiance": (_b, [_p, _i]),
     "pathTracerRenderFixedWavefront": (_b, [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _i, _i, _i, _p]),
     "pathTracerWavefrontReserve": (_b, [_i, _i, _i]),

@@ -209,11 +209,11 @@ extern "C" bool pathTracerDetachedRender(float* d_image, int width, int height, 
     return true;
 }
 
-extern "C" bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
-                                        const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
-                                        const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
-                                        int materialCount, const float* d_sampleRadiance, double* d_grad,
-                                        hipStream_t stream)
+extern "C" bool pathTracerAdjointRenderSeeded(const float* d_gradImage, int width, int height, int samplesPerPixel,
+                                              const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                              const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                              int materialCount, const float* d_sampleRadiance, double* d_grad,
+                                              unsigned int firstSeed, hipStream_t stream)
 {
     if (d_gradImage == nullptr || d_camera == nullptr || d_scene == nullptr || d_params == nullptr ||
         d_grad == nullptr || width <= 0 || height <= 0 || samplesPerPixel <= 0 || materialCount <= 0 ||
@@ -226,5 +226,15 @@ extern "C" bool pathTracerAdjointRender(const float* d_gradImage, int width, int
         return false;
     }
     return pt_adjoint_wavefront(d_gradImage, width, height, samplesPerPixel, d_camera, d_scene, d_env, d_params,
-                                materialCount, d_sampleRadiance, d_grad, stream);
+                                materialCount, d_sampleRadiance, d_grad, firstSeed, stream);
+}
+
+extern "C" bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
+                                        const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                        const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                        int materialCount, const float* d_sampleRadiance, double* d_grad,
+                                        hipStream_t stream)
+{
+    return pathTracerAdjointRenderSeeded(d_gradImage, width, height, samplesPerPixel, d_camera, d_scene, d_env,
+                                         d_params, materialCount, d_sampleRadiance, d_grad, 1u, stream);
 }

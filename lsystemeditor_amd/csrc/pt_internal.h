@@ -29,5 +29,5 @@ pt::SpecModel pt_spectral_model();
 bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int spp, const CameraGpu* d_camera,
                           const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
                           const RenderParamsGpu* d_params, int materialCount, const float* d_sampleRadiance,
-                          double* d_grad, hipStream_t stream);
+                          double* d_grad, unsigned int firstSeed, hipStream_t stream);
 #endif

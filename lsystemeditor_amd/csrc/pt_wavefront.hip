@@ -1718,12 +1718,12 @@ extern "C" bool pathTracerWavefrontReserve(int maxPathsInFlight, int pixels, int
     return ensure_workspace(std::max(maxPathsInFlight, 4), pixels, maxPathDepth, cus * 8 * kWfBlock);
 }
 
-extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
-                                               uint8_t* d_converged, int width, int height,
-                                               const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
-                                               const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
-                                               int samplesPerPixel, int maxPathDepth, int maxPathsInFlight,
-                                               hipStream_t stream)
+namespace {
+
+bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2, uint8_t* d_converged,
+                       int width, int height, const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                       const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, int samplesPerPixel,
+                       int maxPathDepth, int maxPathsInFlight, unsigned int firstSeed, hipStream_t stream)
 {
     if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
         d_converged == nullptr || d_camera == nullptr || d_scene == nullptr || d_params == nullptr || width <= 0 ||
@@ -1769,8 +1769,8 @@ extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_s
         hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
                            counterInts);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, d_samples,
-                           d_converged, d_camera, d_params, width, height, 1u, passBase, samplesPerPixel, g, P, spec,
-                           -1);
+                           d_converged, d_camera, d_params, width, height, firstSeed, passBase, samplesPerPixel, g, P,
+                           spec, -1);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, static_cast<long long>(g) * pixels,
                        persistentBlocks, spec, sm);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, P,
@@ -1782,6 +1782,19 @@ extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_s
         return false;
     }
     return true;
+}
+
+} // namespace
+
+extern "C" bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                               uint8_t* d_converged, int width, int height,
+                                               const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                               const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                               int samplesPerPixel, int maxPathDepth, int maxPathsInFlight,
+                                               hipStream_t stream)
+{
+    return render_fixed_impl(d_buffer, d_samples, d_lumMean, d_m2, d_converged, width, height, d_camera, d_scene,
+                             d_env, d_params, samplesPerPixel, maxPathDepth, maxPathsInFlight, 1u, stream);
 }
 
 // ---- adjoint on the wavefront schedule ------------------------------------------------------
@@ -1850,7 +1863,7 @@ bool ensure_adjoint(int cap, int pixels, int rows)
 bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int spp, const CameraGpu* d_camera,
                           const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
                           const RenderParamsGpu* d_params, int materialCount, const float* d_sampleRadiance,
-                          double* d_grad, hipStream_t stream)
+                          double* d_grad, unsigned int firstSeed, hipStream_t stream)
 {
     RenderParamsGpu hp;
     if (hipMemcpyAsync(&hp, d_params, sizeof(hp), hipMemcpyDeviceToHost, stream) != hipSuccess ||
@@ -1885,8 +1898,8 @@ bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int s
                         hipMemsetAsync(mean, 0, pixels * sizeof(float), stream) == hipSuccess &&
                         hipMemsetAsync(m2, 0, pixels * sizeof(float), stream) == hipSuccess &&
                         hipMemsetAsync(acc, 0, pixels * sizeof(PtFloat4), stream) == hipSuccess &&
-                        pathTracerRenderFixedWavefront(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene,
-                                                       d_env, d_params, spp, maxDepth, 0, stream);
+                        render_fixed_impl(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene, d_env, d_params,
+                                          spp, maxDepth, 0, firstSeed, stream);
         gCapture = savedCapture;
         gCaptureSpp = savedSpp;
         if (!ok) {
@@ -1929,8 +1942,8 @@ bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int s
             hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
                                counterInts);
             hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count,
-                               gAd.counts, gAd.conv, d_camera, d_params, width, height, 1u, passBase, spp, g, P, false,
-                               0);
+                               gAd.counts, gAd.conv, d_camera, d_params, width, height, firstSeed, passBase, spp, g, P,
+                               false, 0);
             hipLaunchKernelGGL(wf_adjoint_init_kernel, dim3(blocks_for(static_cast<long long>(g) * pixels)),
                                dim3(kWfBlock), 0, stream, P.S[0], gAd.A[0], &P.counters[3], d_gradImage, radiance,
                                spp, invSpp);

@@ -52,30 +52,37 @@ class DiffRenderer:
         abi.check(abi.lib().ptSceneUpdateMaterials(self.scene.handle, mats.ctypes.data, mats.shape[0],
                                                    render._stream_handle(self.stream)), "ptSceneUpdateMaterials")
 
-    def forward(self, theta: torch.Tensor) -> torch.Tensor:
-        """(height * width, 3) float32 image of the fixed-spp render at parameters theta."""
+    def forward(self, theta: torch.Tensor, seed_base: int = 0) -> torch.Tensor:
+        """(height * width, 3) float32 image of the fixed-spp render at parameters theta. Sample k
+        of a pixel uses seed seed_base + k + 1; the adjoint's samples are those of seed_base 0,
+        whose per-sample radiance this render then stores for it."""
         self.set_theta(theta)
-        abi.check(abi.lib().pathTracerWavefrontCaptureRadiance(self.radiance.data_ptr(), self.spp),
-                  "pathTracerWavefrontCaptureRadiance")
+        capture = seed_base == 0
+        if capture:
+            abi.check(abi.lib().pathTracerWavefrontCaptureRadiance(self.radiance.data_ptr(), self.spp),
+                      "pathTracerWavefrontCaptureRadiance")
         try:
-            render.render_wavefront(self.scene, self.acc, self.inputs, passes=self.spp, stream=self.stream)
+            render.render_wavefront(self.scene, self.acc, self.inputs, first_iter=seed_base, passes=self.spp,
+                                    stream=self.stream)
         finally:
-            abi.lib().pathTracerWavefrontCaptureRadiance(None, 0)
-        self.radiance_valid = True
+            if capture:
+                abi.lib().pathTracerWavefrontCaptureRadiance(None, 0)
+        self.radiance_valid = capture
         self.last_theta = theta.detach().to("cpu", torch.float32).clone()
         return self.acc.acc[:, :3].clone()
 
-    def adjoint(self, grad_image: torch.Tensor, reuse_forward: bool = True) -> torch.Tensor:
+    def adjoint(self, grad_image: torch.Tensor, reuse_forward: bool = True, seed_base: int = 0) -> torch.Tensor:
         """dLoss/dtheta (materials, 7) float64 for dLoss/dImage = grad_image (height * width, 3),
-        at the parameters of the last forward (whose per-sample radiance is reused unless
-        reuse_forward is False)."""
+        at the current parameters, replaying the samples of seed base `seed_base` (sample k uses
+        seed seed_base + k + 1). For seed base 0 the last forward's per-sample radiance is reused
+        unless reuse_forward is False; other seed bases render their samples first."""
         g = grad_image.detach().to("cuda", torch.float32).contiguous()
         out = torch.zeros((self.material_count, 7), dtype=torch.float64, device="cuda")
-        rad = self.radiance.data_ptr() if (reuse_forward and self.radiance_valid) else None
-        abi.check(abi.lib().pathTracerAdjointRender(
+        rad = self.radiance.data_ptr() if (reuse_forward and self.radiance_valid and seed_base == 0) else None
+        abi.check(abi.lib().pathTracerAdjointRenderSeeded(
             g.data_ptr(), self.width, self.height, self.spp, self.inputs.camera.data_ptr(), self.scene.device,
             self.inputs.env_ptr, self.inputs.params.data_ptr(), self.material_count, rad, out.data_ptr(),
-            render._stream_handle(self.stream)), "pathTracerAdjointRender")
+            seed_base + 1, render._stream_handle(self.stream)), "pathTracerAdjointRenderSeeded")
         return out
 
     def detached(self, eval_theta: torch.Tensor) -> torch.Tensor:
@@ -150,3 +157,33 @@ class RenderFunction(torch.autograd.Function):
 
 def render_image(theta: torch.Tensor, renderer: DiffRenderer) -> torch.Tensor:
     return RenderFunction.apply(theta, renderer)
+
+
+class DecorrelatedRenderFunction(torch.autograd.Function):
+    """image = render(theta) with the samples of seed base `seed_base`; the backward replays the
+    independent samples of seed base `adjoint_base` (rendered again inside the adjoint). The loss
+    residual and the path derivatives then come from independent samples, so the gradient of a
+    loss that is nonlinear in the image (e.g. L2) carries no sample covariance term. Costs one more
+    render per step."""
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, renderer: DiffRenderer, seed_base: int, adjoint_base: int) -> torch.Tensor:
+        if seed_base == adjoint_base:
+            raise ValueError("the forward's and the adjoint's seed bases must differ")
+        ctx.renderer = renderer
+        ctx.theta = theta.detach().clone()
+        ctx.adjoint_base = adjoint_base
+        return renderer.forward(theta, seed_base=seed_base)
+
+    @staticmethod
+    def backward(ctx, grad_image: torch.Tensor):
+        r = ctx.renderer
+        if not torch.equal(r.last_theta, ctx.theta.cpu()):
+            r.set_theta(ctx.theta)
+        grad = r.adjoint(grad_image, reuse_forward=False, seed_base=ctx.adjoint_base)
+        return grad.to(ctx.theta.device, ctx.theta.dtype), None, None, None
+
+
+def render_image_decorrelated(theta: torch.Tensor, renderer: DiffRenderer, seed_base: int,
+                              adjoint_base: int) -> torch.Tensor:
+    return DecorrelatedRenderFunction.apply(theta, renderer, seed_base, adjoint_base)

@@ -252,6 +252,13 @@ void orc_render_detached(float* img3, int width, int height, const OCamera* cam,
                          const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp, int threads);
 void orc_render_adjoint(const double* gradImg3, int width, int height, const OCamera* cam, const OScene* scene,
                         const OEnv* env, const OParams* params, int spp, double* grad, int threads);
+/* The detached image and the adjoint over the samples of seeds firstSeed .. firstSeed + spp - 1. */
+void orc_render_detached_seeded(float* img3, int width, int height, const OCamera* cam, const OScene* scene,
+                                const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp,
+                                uint32_t firstSeed, int threads);
+void orc_render_adjoint_seeded(const double* gradImg3, int width, int height, const OCamera* cam,
+                               const OScene* scene, const OEnv* env, const OParams* params, int spp,
+                               uint32_t firstSeed, double* grad, int threads);
 
 /* ---- spectral (Spectral/Rgb2Spec.h, SpectralCore.h, SpectralState.cpp) ----------------------- */
 typedef struct {

@@ -159,6 +159,10 @@ def lib():
                                            C.c_int]),
             "orc_render_adjoint": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), C.POINTER(Env), vp, C.c_int, vp,
                                           C.c_int]),
+            "orc_render_detached_seeded": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, C.POINTER(Env), vp,
+                                                  C.c_int, C.c_uint32, C.c_int]),
+            "orc_render_adjoint_seeded": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), C.POINTER(Env), vp,
+                                                 C.c_int, C.c_uint32, vp, C.c_int]),
             "orc_clear_region": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "orc_rebuild_active": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
             "orc_cie_interp": (f, [C.c_int, f]),

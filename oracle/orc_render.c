@@ -1343,6 +1343,7 @@ typedef struct {
     const double* gradImg;
     double* grad;
     int w, h, spp, y0, y1, gradCount;
+    uint32_t firstSeed;
     const OCamera* cam;
     const OScene* s;
     const OMaterial* evalMats;
@@ -1358,7 +1359,7 @@ static void ad_rows(const AdJob* j)
             float acc[3] = {0.0f, 0.0f, 0.0f};
             for (int k = 0; k < j->spp; ++k) {
                 ORng rng;
-                orc_rng_init(&rng, idx, k, (uint32_t)k + 1u);
+                orc_rng_init(&rng, idx, k, j->firstSeed + (uint32_t)k);
                 const float ju = orc_rand01(&rng);
                 const float jv = orc_rand01(&rng);
                 const float u = ((float)x + ju) / (float)j->w;
@@ -1445,8 +1446,17 @@ static void ad_run(AdJob base, int threads, double* gradOut)
 void orc_render_detached(float* img3, int width, int height, const OCamera* cam, const OScene* s,
                          const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp, int threads)
 {
+    orc_render_detached_seeded(img3, width, height, cam, s, evalMats, env, params, spp, 1u, threads);
+}
+
+/* The same image over the samples of seeds firstSeed .. firstSeed + spp - 1. */
+void orc_render_detached_seeded(float* img3, int width, int height, const OCamera* cam, const OScene* s,
+                                const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp,
+                                uint32_t firstSeed, int threads)
+{
     AdJob j;
     memset(&j, 0, sizeof(j));
+    j.firstSeed = firstSeed;
     j.img = img3;
     j.w = width;
     j.h = height;
@@ -1460,12 +1470,15 @@ void orc_render_detached(float* img3, int width, int height, const OCamera* cam,
 }
 
 /* Adjoint of the same render: grad[7 * materialCount] += sum over pixels and samples of
- * gradImg . d(pixel)/d(theta), with gradImg = dLoss/dImage (3 doubles per pixel). */
-void orc_render_adjoint(const double* gradImg, int width, int height, const OCamera* cam, const OScene* s,
-                        const OEnv* env, const OParams* params, int spp, double* grad, int threads)
+ * gradImg . d(pixel)/d(theta), with gradImg = dLoss/dImage (3 doubles per pixel). Sample k of a
+ * pixel uses sample index k and seed firstSeed + k (orc_render_adjoint: firstSeed = 1). */
+void orc_render_adjoint_seeded(const double* gradImg, int width, int height, const OCamera* cam, const OScene* s,
+                               const OEnv* env, const OParams* params, int spp, uint32_t firstSeed, double* grad,
+                               int threads)
 {
     AdJob j;
     memset(&j, 0, sizeof(j));
+    j.firstSeed = firstSeed;
     j.gradImg = gradImg;
     j.w = width;
     j.h = height;
@@ -1476,4 +1489,10 @@ void orc_render_adjoint(const double* gradImg, int width, int height, const OCam
     j.params = params;
     j.gradCount = 7 * (int)s->materialCount;
     ad_run(j, threads, grad);
+}
+
+void orc_render_adjoint(const double* gradImg, int width, int height, const OCamera* cam, const OScene* s,
+                        const OEnv* env, const OParams* params, int spp, double* grad, int threads)
+{
+    orc_render_adjoint_seeded(gradImg, width, height, cam, s, env, params, spp, 1u, grad, threads);
 }

@@ -83,6 +83,32 @@ def test_adjoint_matches_oracle_fp64(D, bvh):
     assert live[:, :6].sum() >= 6 * (r.material_count - 1)
 
 
+def test_seeded_adjoint_matches_oracle_fp64(D):
+    """pathTracerAdjointRenderSeeded over seeds 37..44 (render of those samples inside the call)
+    against the oracle's adjoint of the same sample set."""
+    w = h = 32
+    spp = 8
+    r, ref, cam, params = setup(D, w, h, spp)
+    img = r.forward(r.theta0().cuda(), seed_base=36)
+    grad_img = (2.0 * (img - (img * 0.8 + 0.02))).contiguous()
+    g_gpu = r.adjoint(grad_img, reuse_forward=False, seed_base=36).cpu().numpy()
+    g_cpu = np.zeros(7 * r.material_count)
+    gi = np.ascontiguousarray(grad_img.cpu().numpy().astype(np.float64))
+    O.lib().orc_render_adjoint_seeded(O.ptr(gi), w, h, O.ptr(cam), C.byref(ref.scene), C.byref(O.Env()),
+                                      O.ptr(params), spp, 37, O.ptr(g_cpu), 8)
+    g_cpu = g_cpu.reshape(-1, 7)
+    scale = np.abs(g_cpu).max()
+    assert scale > 0
+    err = np.abs(g_gpu - g_cpu)
+    assert np.all(err <= 1e-4 * np.abs(g_cpu) + 1e-6 * scale), f"max rel err {(err / (np.abs(g_cpu) + 1e-30)).max()}"
+    # the forward of seed base 36 is the oracle's detached image of seeds 37..44
+    want = np.zeros((w * h, 3), np.float32)
+    O.lib().orc_render_detached_seeded(O.ptr(want), w, h, O.ptr(cam), C.byref(ref.scene), None, C.byref(O.Env()),
+                                       O.ptr(params), spp, 37, 8)
+    got = img.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def test_autograd_backward_equals_adjoint(D):
     import torch
     w = h = 16

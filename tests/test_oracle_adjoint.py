@@ -2,7 +2,8 @@
 forward integrator at the primal parameters (bit for bit), the analytic BSDF derivative matches
 central differences of orc_brdf_eval, the firefly-clamp VJP matches differences of
 orc_clamp_firefly, and the FP64 adjoint gradient of an image loss matches central finite
-differences of the detached estimator (h = 1e-3) on a small scene."""
+differences of the detached estimator (h = 1e-3) on a small scene, for the default sample set
+(seeds 1..spp) and another one."""
 import ctypes as C
 
 import numpy as np
@@ -97,25 +98,46 @@ def test_clamp_vjp_matches_differences():
                 assert abs(out[k] - fd) <= 1e-3 * max(1.0, abs(fd)), (C3, k, out[k], fd)
 
 
-def render(sc, cam, params, w, h, spp, eval_mats=None):
+def render(sc, cam, params, w, h, spp, eval_mats=None, first_seed=1):
     img = np.zeros((w * h, 3), np.float32)
-    O.lib().orc_render_detached(O.ptr(img), w, h, O.ptr(cam), C.byref(sc.scene),
-                                None if eval_mats is None else O.ptr(eval_mats), C.byref(O.Env()), O.ptr(params),
-                                spp, 8)
+    O.lib().orc_render_detached_seeded(O.ptr(img), w, h, O.ptr(cam), C.byref(sc.scene),
+                                       None if eval_mats is None else O.ptr(eval_mats), C.byref(O.Env()),
+                                       O.ptr(params), spp, first_seed, 8)
     return img
 
 
-def test_adjoint_matches_finite_differences_of_the_detached_estimator():
+def test_seeded_entry_points_at_seed_one_are_the_plain_ones():
+    w = h = 10
+    tris, mats, cam, params = small_scene(w, h, 4)
+    sc = O.OracleScene(abi.host_to_gpu_triangles(tris), mats)
+    plain = np.zeros((w * h, 3), np.float32)
+    O.lib().orc_render_detached(O.ptr(plain), w, h, O.ptr(cam), C.byref(sc.scene), None, C.byref(O.Env()),
+                                O.ptr(params), 4, 8)
+    assert np.array_equal(render(sc, cam, params, w, h, 4).view(np.uint32), plain.view(np.uint32))
+    other = render(sc, cam, params, w, h, 4, first_seed=5)
+    assert not np.array_equal(other, plain)
+    gi = np.ascontiguousarray(np.full((w * h, 3), 0.25))
+    a = np.zeros(7 * sc.mats.shape[0])
+    b = np.zeros_like(a)
+    O.lib().orc_render_adjoint(O.ptr(gi), w, h, O.ptr(cam), C.byref(sc.scene), C.byref(O.Env()), O.ptr(params), 4,
+                               O.ptr(a), 1)
+    O.lib().orc_render_adjoint_seeded(O.ptr(gi), w, h, O.ptr(cam), C.byref(sc.scene), C.byref(O.Env()),
+                                      O.ptr(params), 4, 1, O.ptr(b), 1)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("first_seed", [1, 37])
+def test_adjoint_matches_finite_differences_of_the_detached_estimator(first_seed):
     w = h = 12
     spp = 4
     tris, mats, cam, params = small_scene(w, h, spp)
     sc = O.OracleScene(abi.host_to_gpu_triangles(tris), mats)
-    img = render(sc, cam, params, w, h, spp)
+    img = render(sc, cam, params, w, h, spp, first_seed=first_seed)
     target = (img * np.float32(0.8) + np.float32(0.02)).astype(np.float32)
     grad_img = np.ascontiguousarray(2.0 * (img.astype(np.float64) - target))
     grad = np.zeros(7 * sc.mats.shape[0])
-    O.lib().orc_render_adjoint(O.ptr(grad_img), w, h, O.ptr(cam), C.byref(sc.scene), C.byref(O.Env()),
-                               O.ptr(params), spp, O.ptr(grad), 8)
+    O.lib().orc_render_adjoint_seeded(O.ptr(grad_img), w, h, O.ptr(cam), C.byref(sc.scene), C.byref(O.Env()),
+                                      O.ptr(params), spp, first_seed, O.ptr(grad), 8)
     grad = grad.reshape(-1, 7)
     scale = np.abs(grad).max()
     assert scale > 0
@@ -131,7 +153,7 @@ def test_adjoint_matches_finite_differences_of_the_detached_estimator():
             for sgn in (+1, -1):
                 em = np.ascontiguousarray(sc.mats.copy())
                 em[mi, WORD[name]] = np.float32(v + sgn * hstep)
-                im = render(sc, cam, params, w, h, spp, em)
+                im = render(sc, cam, params, w, h, spp, em, first_seed)
                 losses.append(float(((im.astype(np.float64) - target) ** 2).sum()))
             fd = (losses[0] - losses[1]) / (2 * hstep)
             assert abs(grad[mi, k] - fd) <= 2e-3 * abs(fd) + 2e-4 * scale, (mi, name, grad[mi, k], fd)
