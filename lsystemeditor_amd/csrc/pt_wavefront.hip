@@ -225,6 +225,12 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
         return; // deep bounces of a group often have no rays left
     }
     const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    // Waves beyond what the queue can feed one chunk each leave at once: otherwise every wave of
+    // the persistent grid probes every part's counter on its way out, a fixed cost per launch
+    // (about a millisecond per 1M-path render) that small queues cannot amortise.
+    if (static_cast<long long>(gtid >> 6) * kFetchChunk >= n) {
+        return;
+    }
     LdsStack st{(LdsInt2*)(ring + threadIdx.x), (GlobalInt2*)(P.spill + gtid), P.spillThreads, 0, 0};
     TraceStats ts{0u, 0u, 0u, 0u};
     const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
