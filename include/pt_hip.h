@@ -350,6 +350,52 @@ bool pathTracerAdjointRenderSeeded(const float* d_gradImage, int width, int heig
  * (width * height * samplesPerPixel * 3 floats). NULL turns it off. No reference counterpart. */
 bool pathTracerWavefrontCaptureRadiance(float* d_out, int samplesPerPixel);
 
+/* ---- batches of scenes (no reference counterpart) --------------------------------------------
+ * A batch renders many scenes in one wavefront: every bounce's trace and shade launches serve the
+ * paths of all scenes, so the cost of a launch's slowest rays is paid once per bounce for the
+ * whole batch instead of once per scene. Each scene's results equal those of its own render. */
+
+/* One scene of a batch: the device pointers the single-scene launchers take (env may be NULL:
+ * solid background) and the image size. */
+typedef struct {
+    const MeshAccelSceneGpu* scene;
+    const CameraGpu* camera;
+    const RenderParamsGpu* params;
+    const EnvironmentMapGpu* env;
+    int width, height;
+} PtBatchSceneGpu;
+
+typedef struct PtBatch PtBatch;
+
+/* Batch of sceneCount (1..4096) scenes from a host array. The scenes' pixels are concatenated in
+ * order (scene i starts at pixel offset sum_{j<i} width_j * height_j of the batch's per-pixel
+ * buffers) and so are their materials' gradient rows (scene i's material m is row
+ * sum_{j<i} materialCount_j + m). Reads every scene's description and parameters once
+ * (synchronously) to size the batch; the renders read the records' current contents, so material,
+ * camera and parameter updates need no new batch, while a rebuilt scene or a larger
+ * maxPathDepth does. NULL on failure. */
+PtBatch* ptBatchCreate(const PtBatchSceneGpu* scenes, int sceneCount);
+void ptBatchDestroy(PtBatch* batch);
+/* Total pixels, gradient rows, the largest resolved path depth, and (arrays of sceneCount ints,
+ * each optional) the scenes' pixel offsets and first gradient rows. */
+bool ptBatchInfo(const PtBatch* batch, int* totalPixels, int* gradRows, int* maxPathDepth, int* pixelOffsets,
+                 int* gradRowOffsets);
+
+/* pathTracerRenderFixedWavefront over a batch: per-pixel buffers hold the batch's totalPixels
+ * pixels; every scene takes samplesPerPixel samples, sample k with seed firstSeed + k (1 gives the
+ * plain fixed render), each with its own parameters. Capturable into a HIP graph once the
+ * workspace is reserved (pathTracerWavefrontReserve with the batch's pixels as path bound).
+ * Calls on one device must not overlap (shared workspace). */
+bool pathTracerRenderFixedBatch(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                uint8_t* d_converged, const PtBatch* batch, int samplesPerPixel, int maxPathsInFlight,
+                                unsigned int firstSeed, hipStream_t stream);
+
+/* pathTracerAdjointRenderSeeded over a batch: d_gradImage holds dLoss/dPixel for the batch's
+ * pixels (3 floats each), d_grad accumulates 7 doubles per gradient row (at most 512 rows). The
+ * samples (seeds firstSeed..) are rendered first, then replayed. RGB mode only; not capturable. */
+bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtBatch* batch, int samplesPerPixel,
+                                  unsigned int firstSeed, double* d_grad, hipStream_t stream);
+
 /* The detached estimator of the same render as an image (3 floats per pixel, running mean of the
  * clamped radiance): paths sampled with the scene's materials (directions, pdfs, MIS weights,
  * Russian roulette), values evaluated with d_evalMaterials (same count and layout). With

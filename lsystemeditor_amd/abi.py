@@ -85,6 +85,11 @@ SIGNATURES = {
     "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
     "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
     "pathTracerAdjointRenderSeeded": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _u, _p]),
+    "ptBatchCreate": (_p, [_p, _i]),
+    "ptBatchDestroy": (None, [_p]),
+    "ptBatchInfo": (_b, [_p, _p, _p, _p, _p, _p]),
+    "pathTracerRenderFixedBatch": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _u, _p]),
+    "pathTracerAdjointRenderBatch": (_b, [_p, _p, _i, _u, _p, _p]),
     "pathTracerWavefrontCaptureRadiance": (_b, [_p, _i]),
     "pathTracerRenderFixedWavefront": (_b, [_p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _i, _i, _i, _p]),
     "pathTracerWavefrontReserve": (_b, [_i, _i, _i]),
@@ -389,6 +394,12 @@ class Environment:
             self.close()
         except Exception:
             pass
+
+
+class PtBatchSceneGpu(C.Structure):
+    """One scene of a batch (ptBatchCreate): device pointers and image size."""
+    _fields_ = [("scene", C.c_void_p), ("camera", C.c_void_p), ("params", C.c_void_p), ("env", C.c_void_p),
+                ("width", C.c_int), ("height", C.c_int)]
 
 
 class SpectralStateGpu(C.Structure):

@@ -43,6 +43,8 @@ constexpr int kRefillLanes = 32;   // idle lanes that trigger a wave's ray refil
 // part when its own runs dry. Chunks and parts keep every counter's atomic traffic low while
 // leaving enough entries unclaimed for the waves to balance uneven ray costs.
 constexpr int kFetchChunk = 64;
+constexpr int kMaxBatchScenes = 4096;   // scenes per batch
+constexpr int kMaxBatchGradRows = 512;  // material rows of a batch adjoint (LDS sums)
 
 enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
@@ -61,6 +63,7 @@ struct PathState {
     float* hero;                                          // hero-wavelength draw (spectral mode)
     uint32_t *pixel, *sample, *seed, *draws, *flags;
     uint32_t* slot0; // the path's generation slot, where its final radiance goes
+    uint32_t* scene; // batch renders: the path's scene (pixel is then the scene-local pixel)
     uint8_t* occ;    // 2 per slot: environment, emissive shadow results
 };
 
@@ -79,6 +82,35 @@ struct Paths {
     int2* spill;
     int spillThreads;
 };
+
+// One scene of a batch render, expanded on the device at every call from the caller's pointers
+// (wf_batch_expand_kernel): the kernels of a batch read a path's scene, parameters, camera and
+// environment here, indexed by the path's scene id.
+struct BatchScene {
+    MeshAccelSceneGpu scene;
+    RenderParamsGpu params;
+    CameraGpu camera;
+    EnvironmentMapGpu env;  // valid = 0 where the scene has none (solid background)
+    int width, height;
+    int pixelOffset;        // first pixel of the scene in the batch's per-pixel buffers
+    int gradRow;            // first gradient row of the scene's materials (adjoint)
+    int maxDepth;           // the scene's resolved path depth
+};
+
+// The batch scene of global pixel gp: the last scene whose pixelOffset <= gp.
+PT_DEV int batch_scene_of(const BatchScene* tab, int count, int gp)
+{
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].pixelOffset <= gp) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
 
 PT_DEV int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
 
@@ -202,6 +234,21 @@ struct LdsStack {
 
 constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf refs are ~slot >= -2^28)
 
+// What a trace kernel reads of a scene: the tree, the leaf triangles and the root.
+struct TraceScene {
+    const PtFloat4* nodes;
+    const PtFloat4* tris;
+    uint32_t triCount;
+    int rootRef;
+    float rmin[3], rmax[3];
+};
+
+PT_DEV TraceScene trace_scene(const MeshAccelSceneGpu& m)
+{
+    return TraceScene{m.nodes, m.triPos, m.triangleCount, m.rootRef, {m.rootMin[0], m.rootMin[1], m.rootMin[2]},
+                      {m.rootMax[0], m.rootMax[1], m.rootMax[2]}};
+}
+
 // Persistent traversal: every lane takes the next ray from its wave's claimed range of the queue as
 // soon as its current ray finishes (the wave claims kFetchChunk entries per atomic), so the lanes
 // of a wave stay busy while per-ray node counts vary widely. One loop iteration advances a lane by one node: an
@@ -211,15 +258,18 @@ constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf
 //   ANY_HIT = false: extension rays of the extension queue (path ids; identity when queue is
 //                    null), results to the hit record at the path's shade slot.
 //   ANY_HIT = true:  shadow rays of shadowQ, result to occ.
-template <bool ANY_HIT, bool STATS>
+//   BATCH:           rays of a batch render; each ray traverses its own scene's tree (the scene
+//                    id of its path, or the one its shadow-queue entry carries).
+template <bool ANY_HIT, bool STATS, bool BATCH>
 __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
                                                             PathState PS, const int* __restrict__ queue,
                                                             const int* __restrict__ countPtr, int* fetchCounters,
                                                             unsigned long long* stats, int4* __restrict__ hitA,
-                                                            float2* __restrict__ hitB)
+                                                            float2* __restrict__ hitB,
+                                                            const BatchScene* __restrict__ tab)
 {
     __shared__ int2 ring[kLdsEntries * kWfBlock];
-    const MeshAccelSceneGpu s = *scene;
+    TraceScene s = BATCH ? TraceScene{} : trace_scene(*scene);
     const int n = *countPtr;
     if (n == 0) {
         return; // deep bounces of a group often have no rays left
@@ -290,11 +340,17 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                         tag = __float_as_uint(a.w);
                         r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
                         tMin = kShadowBias;
+                        if constexpr (BATCH) {
+                            s = trace_scene(tab[__float_as_uint(b.w)].scene);
+                        }
                     } else {
                         const int q = queue != nullptr ? queue[idx] : idx;
                         tag = static_cast<uint32_t>(q);
                         r = make_ray(mk(PS.ox[q], PS.oy[q], PS.oz[q]), mk(PS.dx[q], PS.dy[q], PS.dz[q]));
                         tMin = PS.tmin[q];
+                        if constexpr (BATCH) {
+                            s = trace_scene(tab[PS.scene[q]].scene);
+                        }
                     }
                     bestT = kRayTMax;
                     bestTri = 0xFFFFFFFFu;
@@ -307,8 +363,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                         ts.rays += 1u;
                     }
                     float te;
-                    cur = (s.triangleCount != 0u && slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
-                                                          s.rootMax[1], s.rootMax[2], tMin, kRayTMax, te))
+                    cur = (s.triCount != 0u && slab(r, s.rmin[0], s.rmin[1], s.rmin[2], s.rmax[0], s.rmax[1],
+                                                    s.rmax[2], tMin, kRayTMax, te))
                         ? s.rootRef
                         : kDone;
                 } else {
@@ -332,9 +388,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
         if (!finished) {
             if (cur < 0) {
                 const uint32_t slot = static_cast<uint32_t>(~cur);
-                const float4 a = ld4(s.triPos, 3u * slot + 0u);
-                const float4 b = ld4(s.triPos, 3u * slot + 1u);
-                const float4 c = ld4(s.triPos, 3u * slot + 2u);
+                const float4 a = ld4(s.tris, 3u * slot + 0u);
+                const float4 b = ld4(s.tris, 3u * slot + 1u);
+                const float4 c = ld4(s.tris, 3u * slot + 2u);
                 // all 48 bytes in one round trip: keeps the compiler from deferring v0's load
                 // past the determinant test
                 asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w));
@@ -414,6 +470,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
 
 // ----------------------------------------------------------------------------------------
 // Generation: camera rays for G consecutive samples of every active pixel (PathTracer.cu:69-99).
+// In a batch (tab != null) active entries are batch pixels; each path keeps its scene-local pixel,
+// which keys its random numbers as in a render of that scene alone, and its scene id.
 // ----------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict__ active,
                                                           const int* __restrict__ activeCountPtr,
@@ -422,19 +480,29 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
                                                           const CameraGpu* camera, const RenderParamsGpu* params,
                                                           int width, int height, uint32_t firstSeed, int passBase,
                                                           int passCount, int groupSize, Paths P, bool spectral,
-                                                          int sampleLimit)
+                                                          int sampleLimit, const BatchScene* __restrict__ tab,
+                                                          int tabCount)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int activeCount = *activeCountPtr;
     int idx = -1;
     int k = 0;
     uint32_t n = 0u;
+    int sid = 0;
+    int local = 0;
     if (t < activeCount) {
         idx = active[t];
+        local = idx;
+        if (tab != nullptr && idx >= 0) {
+            sid = batch_scene_of(tab, tabCount, idx);
+            local = idx - tab[sid].pixelOffset;
+        }
         if (idx >= 0 && converged[idx] == 0) {
             n = counts[idx];
             k = min(groupSize, passCount - passBase);
-            const int maxS = sampleLimit >= 0 ? sampleLimit : params->maxSamplesPerPixel;
+            const int maxS = sampleLimit >= 0 ? sampleLimit
+                                              : (tab != nullptr ? tab[sid].params.maxSamplesPerPixel
+                                                                : params->maxSamplesPerPixel);
             if (maxS > 0) {
                 k = min(k, maxS - static_cast<int>(n));
             }
@@ -462,7 +530,6 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
     // The wave's samples occupy the contiguous slots [waveBase, waveBase + total), pixel by pixel.
     // Lanes take consecutive slots so every store of the path state is coalesced; a slot's pixel
     // is the lane whose inclusive prefix first exceeds the slot offset.
-    const CameraGpu cam = *camera;
     for (int q0 = 0; q0 < total; q0 += 64) {
         const int q = q0 + lane_id();
         int owner = 0;
@@ -472,7 +539,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
                 owner += step;
             }
         }
-        const int oIdx = __shfl(idx, owner, 64);
+        const int oIdx = __shfl(local, owner, 64);
+        const int oSid = __shfl(sid, owner, 64);
         const uint32_t oN = __shfl(n, owner, 64);
         const int oExcl = __shfl(incl - k, owner, 64);
         if (q >= total) {
@@ -480,20 +548,26 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         }
         const int j = q - oExcl;
         const int p = waveBase + q;
-        const int x = oIdx % width;
-        const int y = oIdx / width;
+        const int w = tab != nullptr ? tab[oSid].width : width;
+        const int h = tab != nullptr ? tab[oSid].height : height;
+        const CameraGpu cam = tab != nullptr ? tab[oSid].camera : *camera;
+        const int x = oIdx % w;
+        const int y = oIdx / w;
         const uint32_t seed = firstSeed + static_cast<uint32_t>(passBase + j);
         Rng rng;
         rng_init(rng, oIdx, static_cast<int>(oN) + j, seed);
         const float ju = rand01(rng);
         const float jv = rand01(rng);
-        const float u = (static_cast<float>(x) + ju) / static_cast<float>(width);
-        const float v = (static_cast<float>(y) + jv) / static_cast<float>(height);
+        const float u = (static_cast<float>(x) + ju) / static_cast<float>(w);
+        const float v = (static_cast<float>(y) + jv) / static_cast<float>(h);
         const float l1 = rand01(rng);
         const float l2 = rand01(rng);
         V3 ro, rd;
         camera_ray(cam, u, v, l1, l2, ro, rd);
         PathState& O = P.S[0];
+        if (tab != nullptr) {
+            O.scene[p] = static_cast<uint32_t>(oSid);
+        }
         if (spectral) {
             O.hero[p] = rand01(rng);  // the hero-wavelength draw follows the lens draws
             O.Tw[p] = 1.0f;
@@ -523,10 +597,12 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
     }
 }
 
-PT_DEV void store_shadow(Paths& P, int slot, V3 o, V3 d, int item, uint32_t type)
+// A shadow-queue entry: origin and slot * 4 + type, direction and the batch scene id (0 outside
+// batches).
+PT_DEV void store_shadow(Paths& P, int slot, V3 o, V3 d, int item, uint32_t type, uint32_t sceneId)
 {
     P.shadowQ[2 * slot] = make_float4(o.x, o.y, o.z, __uint_as_float((static_cast<uint32_t>(item) << 2) | type));
-    P.shadowQ[2 * slot + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    P.shadowQ[2 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(sceneId));
 }
 
 // One bounce of PathIntegratorRandCore.h:241-347 for every path in the shade queue.
@@ -559,21 +635,28 @@ PT_DEV HeroColors make_colors(float hero, const SpecModel& sm, const HeroColors*
 PT_DEV V3 finish_sample(const RgbColors&, V3 L) { return clamp_firefly(L); }  // PathTracer.cu:101-104
 PT_DEV V3 finish_sample(const HeroColors& pol, S4 L) { return clamp_firefly_spectral(pol.to_rgb(L)); }
 
-template <class Pol>
+// BATCH: paths of a batch render. Each path uses its scene's tree, materials, parameters and
+// environment; the lanes of a wave take one scene at a time (its data then stays wave-uniform),
+// and each path's depth limit is its scene's.
+template <class Pol, bool BATCH>
 __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
     const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
     PathState O, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA, const float2* __restrict__ inB,
     const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA, int* counters, float* capture,
-    int captureSpp, SpecModel sm)
+    int captureSpp, SpecModel sm, const BatchScene* __restrict__ tab)
 {
     using C = typename Pol::Col;
     constexpr bool kSpectral = ColN<C>::n == 4;
     const C* ct = nullptr;
     __shared__ BlockAppendLds appendLds;
-    const MeshAccelSceneGpu s = *scene;
-    const RenderParamsGpu prm = *params;
     const int n = *inCountPtr;
-    const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
+    // one scene: its description and parameters are read once, before the loop
+    MeshAccelSceneGpu scene1{};
+    RenderParamsGpu params1{};
+    if constexpr (!BATCH) {
+        scene1 = *scene;
+        params1 = *params;
+    }
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
     // uniform trip count so the wave-level appends see every lane of the wave
     const int trips = (n + stride - 1) / stride;
@@ -585,8 +668,10 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
         V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO, nextO = envO, nextD = envO;
         C L = ColN<C>::splat(0.0f), T = L, ce = L, cm = L;
         float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f, hero = 0.0f;
-        uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u;
-        if (live) {
+        uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u, sid = 0u;
+        auto body = [&](const MeshAccelSceneGpu& s, const RenderParamsGpu& prm, const EnvironmentMapGpu* env,
+                        int depthLimit, uint32_t pixelOffset) {
+            const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
             const int4 ha = inA[i];
             const float2 hb = inB[i];
             const uint32_t flags = I.flags[i];
@@ -620,7 +705,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                         T = ld_col(I.Tx, I.Ty, I.Tz, I.Tw, i, ct);
                         L = add(L, mul(T, scale(envL, mis)));
                     }
-                } else if (bounce < maxDepth) {
+                } else if (bounce < depthLimit) {
                     const float t = __int_as_float(ha.w);
                     const Ray ray = make_ray(origin, dir);
                     Hit h;
@@ -738,7 +823,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                 if constexpr (!kSpectral) {
                     if (capture != nullptr && sample < static_cast<uint32_t>(captureSpp)) {
                         // unclamped radiance of (pixel, sample) for the adjoint
-                        const size_t slot = 3 * (static_cast<size_t>(pixel) * captureSpp + sample);
+                        const size_t slot = 3 * ((static_cast<size_t>(pixelOffset) + pixel) * captureSpp + sample);
                         capture[slot + 0] = L.x;
                         capture[slot + 1] = L.y;
                         capture[slot + 2] = L.z;
@@ -749,6 +834,24 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                 P.ry[slot0] = r.y;
                 P.rz[slot0] = r.z;
             }
+        };
+        if constexpr (BATCH) {
+            sid = live ? I.scene[i] : 0u;
+            bool todo = live;
+            while (__ballot(todo) != 0ull) {
+                if (todo) {
+                    const uint32_t one = __builtin_amdgcn_readfirstlane(sid);
+                    if (sid == one) {
+                        const BatchScene* B = tab + one;
+                        const MeshAccelSceneGpu sc = B->scene;
+                        const RenderParamsGpu pc = B->params;
+                        body(sc, pc, &B->env, B->maxDepth, static_cast<uint32_t>(B->pixelOffset));
+                        todo = false;
+                    }
+                }
+            }
+        } else if (live) {
+            body(scene1, params1, env, maxDepth, 0u);
         }
         const bool next = wantExt || wantEnv || wantEmis;
         int ext, sh, envSlot, emisSlot;
@@ -764,6 +867,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
             O.sample[sh] = sample;
             O.seed[sh] = seed;
             O.slot0[sh] = slot0;
+            if constexpr (BATCH) {
+                O.scene[sh] = sid;
+            }
             if constexpr (kSpectral) {
                 O.hero[sh] = hero;
             }
@@ -789,10 +895,10 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
             }
         }
         if (wantEnv) {
-            store_shadow(P, envSlot, envO, envD, sh, 1u);
+            store_shadow(P, envSlot, envO, envD, sh, 1u, sid);
         }
         if (wantEmis) {
-            store_shadow(P, emisSlot, emO, emD, sh, 2u);
+            store_shadow(P, emisSlot, emO, emD, sh, 2u, sid);
         }
     }
 }
@@ -822,13 +928,13 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_init_kernel(PathState S, 
                                                                    const int* __restrict__ countPtr,
                                                                    const float* __restrict__ gradImg,
                                                                    const float* __restrict__ radiance, int spp,
-                                                                   float invSpp)
+                                                                   float invSpp, const BatchScene* __restrict__ tab)
 {
     const int p = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (p >= *countPtr) {
         return;
     }
-    const uint32_t px = S.pixel[p];
+    const uint32_t px = S.pixel[p] + (tab != nullptr ? static_cast<uint32_t>(tab[S.scene[p]].pixelOffset) : 0u);
     const size_t q = 3 * (static_cast<size_t>(px) * spp + S.sample[p]);
     const V3 C = mk(radiance[q], radiance[q + 1], radiance[q + 2]);
     const V3 gs = mk(gradImg[3 * px] * invSpp, gradImg[3 * px + 1] * invSpp, gradImg[3 * px + 2] * invSpp);
@@ -851,11 +957,14 @@ PT_DEV V3 safe_ratio(V3 g, V3 x, V3 f)
               f.z != 0.0f ? g.z * x.z / f.z : 0.0f);
 }
 
+// BATCH as in wf_shade_kernel; a scene's material m sums into gradient row gradRow + m, and
+// materialCount is the batch's row count.
+template <bool BATCH>
 __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
     const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
     PathState O, AdjointState AI, AdjointState AO, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA,
     const float2* __restrict__ inB, const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA,
-    int* counters, int materialCount, int copies, double* __restrict__ partial)
+    int* counters, int materialCount, int copies, double* __restrict__ partial, const BatchScene* __restrict__ tab)
 {
     __shared__ BlockAppendLds appendLds;
     extern __shared__ double lgAll[];
@@ -865,10 +974,13 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
     }
     double* lg = lgAll + (static_cast<int>(threadIdx.x) % copies) * perCopy;
     __syncthreads();
-    const MeshAccelSceneGpu s = *scene;
-    const RenderParamsGpu prm = *params;
+    MeshAccelSceneGpu scene1{};
+    RenderParamsGpu params1{};
+    if constexpr (!BATCH) {
+        scene1 = *scene;
+        params1 = *params;
+    }
     const int n = *inCountPtr;
-    const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
     const int trips = (n + stride - 1) / stride;
     for (int it = 0; it < trips; ++it) {
@@ -883,9 +995,11 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             e2[k] = 0.0f;
         }
         float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f;
-        uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u;
+        uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u, sid = 0u;
         int pendMat = -1;
-        if (live) {
+        auto body = [&](const MeshAccelSceneGpu& s, const RenderParamsGpu& prm, const EnvironmentMapGpu* env,
+                        int depthLimit, int gradRow) {
+            const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
             const int4 ha = inA[i];
             const float2 hb = inB[i];
             const uint32_t flags = I.flags[i];
@@ -921,7 +1035,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 flush(lg, pmi, prev);
             }
             const uint32_t tri = static_cast<uint32_t>(ha.y);
-            if ((flags & kFlagExt) != 0u && tri != 0xFFFFFFFFu && bounce < maxDepth) {
+            if ((flags & kFlagExt) != 0u && tri != 0xFFFFFFFFu && bounce < depthLimit) {
                 const V3 origin = mk(I.ox[i], I.oy[i], I.oz[i]);
                 const V3 dir = mk(I.dx[i], I.dy[i], I.dz[i]);
                 const float t = __int_as_float(ha.w);
@@ -945,8 +1059,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 c.flags = 0;
                 int gmi = -1;
                 Live lm{false, false, false, false, false, false, false};
-                if (valid && mi < static_cast<uint32_t>(materialCount)) {
-                    gmi = static_cast<int>(mi);
+                if (valid && gradRow + static_cast<int>(mi) < materialCount) {
+                    gmi = gradRow + static_cast<int>(mi);
                     lm = live_of(s.materials[mi], c.m, s);
                 }
                 float acc[kGradParams];
@@ -1078,6 +1192,24 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 draws = rng_draws(rng);
             }
             newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
+        };
+        if constexpr (BATCH) {
+            sid = live ? I.scene[i] : 0u;
+            bool todo = live;
+            while (__ballot(todo) != 0ull) {
+                if (todo) {
+                    const uint32_t one = __builtin_amdgcn_readfirstlane(sid);
+                    if (sid == one) {
+                        const BatchScene* B = tab + one;
+                        const MeshAccelSceneGpu sc = B->scene;
+                        const RenderParamsGpu pc = B->params;
+                        body(sc, pc, &B->env, B->maxDepth, B->gradRow);
+                        todo = false;
+                    }
+                }
+            }
+        } else if (live) {
+            body(scene1, params1, env, maxDepth, 0);
         }
         const bool next = wantExt || wantEnv || wantEmis;
         int ext, sh, envSlot, emisSlot;
@@ -1092,6 +1224,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             O.sample[sh] = sample;
             O.seed[sh] = seed;
             O.slot0[sh] = slot0;
+            if constexpr (BATCH) {
+                O.scene[sh] = sid;
+            }
             O.Lx[sh] = R.x;
             O.Ly[sh] = R.y;
             O.Lz[sh] = R.z;
@@ -1132,10 +1267,10 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             }
         }
         if (wantEnv) {
-            store_shadow(P, envSlot, envO, envD, sh, 1u);
+            store_shadow(P, envSlot, envO, envD, sh, 1u, sid);
         }
         if (wantEmis) {
-            store_shadow(P, emisSlot, emO, emD, sh, 2u);
+            store_shadow(P, emisSlot, emO, emD, sh, 2u, sid);
         }
     }
     // this block's sums into its row of the partial-sum buffer (rows are block-private)
@@ -1194,7 +1329,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_accumulate_kernel(const int* __re
                                                                  float4* __restrict__ acc, uint32_t* __restrict__ counts,
                                                                  float* __restrict__ lumMean, float* __restrict__ m2,
                                                                  uint8_t* __restrict__ converged,
-                                                                 const RenderParamsGpu* params)
+                                                                 const RenderParamsGpu* params,
+                                                                 const BatchScene* __restrict__ tab, int tabCount)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= *activeCountPtr) {
@@ -1206,7 +1342,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_accumulate_kernel(const int* __re
     }
     const int idx = active[t];
     const int base = P.pxBase[t];
-    const RenderParamsGpu prm = *params;
+    const RenderParamsGpu prm = tab != nullptr ? tab[batch_scene_of(tab, tabCount, idx)].params : *params;
     uint32_t n = counts[idx];
     float4 a = acc[idx];
     float mean = lumMean[idx];
@@ -1379,7 +1515,7 @@ size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spil
         for (float** f : fs) {
             *f = carve<float>(cur, c);
         }
-        uint32_t** us[] = {&S.pixel, &S.sample, &S.seed, &S.draws, &S.flags, &S.slot0};
+        uint32_t** us[] = {&S.pixel, &S.sample, &S.seed, &S.draws, &S.flags, &S.slot0, &S.scene};
         for (uint32_t** u : us) {
             *u = carve<uint32_t>(cur, c);
         }
@@ -1449,7 +1585,8 @@ unsigned blocks_for(long long n) { return static_cast<unsigned>(std::max<long lo
 // shade(b, shadeCount, cOut, traceBlocks) launches bounce b's shade kernel.
 template <class ShadeLaunch>
 void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, int maxPathDepth,
-                         long long maxPaths, int persistentBlocks, ShadeLaunch shade)
+                         long long maxPaths, int persistentBlocks, ShadeLaunch shade,
+                         const BatchScene* tab = nullptr)
 {
     const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
     const unsigned shadowBlocks =
@@ -1462,25 +1599,34 @@ void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* 
         const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
         const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
         prof_mark(stream, 0, false, b);
-        if (gProf.counted) {
-            hipLaunchKernelGGL((wf_trace_kernel<false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, gProf.dStats, P.hitA[b & 1],
-                               P.hitB[b & 1]);
-        } else {
-            hipLaunchKernelGGL((wf_trace_kernel<false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+        if (tab != nullptr) {
+            hipLaunchKernelGGL((wf_trace_kernel<false, false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
                                d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, nullptr, P.hitA[b & 1],
-                               P.hitB[b & 1]);
+                               P.hitB[b & 1], tab);
+        } else if (gProf.counted) {
+            hipLaunchKernelGGL((wf_trace_kernel<false, true, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, gProf.dStats, P.hitA[b & 1],
+                               P.hitB[b & 1], nullptr);
+        } else {
+            hipLaunchKernelGGL((wf_trace_kernel<false, false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
+                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, nullptr, P.hitA[b & 1],
+                               P.hitB[b & 1], nullptr);
         }
         prof_mark(stream, 0, true, b);
         if (b > 0) {
             prof_mark(stream, 1, false, b);
-            if (gProf.counted) {
-                hipLaunchKernelGGL((wf_trace_kernel<true, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+            if (tab != nullptr) {
+                hipLaunchKernelGGL((wf_trace_kernel<true, false, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
+                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, nullptr, nullptr, nullptr,
+                                   tab);
+            } else if (gProf.counted) {
+                hipLaunchKernelGGL((wf_trace_kernel<true, true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
                                    d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, gProf.dStats, nullptr,
-                                   nullptr);
+                                   nullptr, nullptr);
             } else {
-                hipLaunchKernelGGL((wf_trace_kernel<true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, nullptr, nullptr, nullptr);
+                hipLaunchKernelGGL((wf_trace_kernel<true, false, false>), dim3(shadowBlocks), dim3(kWfBlock), 0,
+                                   stream, d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, nullptr, nullptr,
+                                   nullptr, nullptr);
             }
             prof_mark(stream, 1, true, b);
         }
@@ -1493,16 +1639,19 @@ void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* 
 
 void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, const RenderParamsGpu* d_params,
                     const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks,
-                    bool spec, const SpecModel& sm)
+                    bool spec, const SpecModel& sm, const BatchScene* tab = nullptr)
 {
-    launch_bounces_with(stream, P, d_scene, maxPathDepth, maxPaths, persistentBlocks,
-                        [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
-                            hipLaunchKernelGGL(spec ? wf_shade_kernel<HeroColors> : wf_shade_kernel<RgbColors>,
-                                               dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
-                                               P.S[b & 1], P.S[(b + 1) & 1], P, b, maxPathDepth, P.hitA[b & 1],
-                                               P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1],
-                                               cOut, gCapture, gCaptureSpp, sm);
-                        });
+    const auto kernel = tab != nullptr ? (spec ? wf_shade_kernel<HeroColors, true> : wf_shade_kernel<RgbColors, true>)
+                                       : (spec ? wf_shade_kernel<HeroColors, false> : wf_shade_kernel<RgbColors, false>);
+    launch_bounces_with(
+        stream, P, d_scene, maxPathDepth, maxPaths, persistentBlocks,
+        [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
+            hipLaunchKernelGGL(kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
+                               P.S[b & 1], P.S[(b + 1) & 1], P, b, maxPathDepth, P.hitA[b & 1], P.hitB[b & 1],
+                               shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp, sm,
+                               tab);
+        },
+        tab);
 }
 
 extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
@@ -1581,12 +1730,12 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur,
                            d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
-                           passCount, group, P, spec, -1);
+                           passCount, group, P, spec, -1, nullptr, 0);
         prof_mark(stream, 3, true);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks, spec, sm);
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
-                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
+                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params, nullptr, 0);
         prof_mark(stream, 3, true);
         hipLaunchKernelGGL(wf_zero_kernel, dim3(1), dim3(kWfBlock), 0, stream, countNext, 1);
         hipLaunchKernelGGL(wf_compact_kernel, dim3(std::min<unsigned>(blocks_for(active), 4096u)), dim3(kWfBlock), 0,
@@ -1726,21 +1875,27 @@ extern "C" bool pathTracerWavefrontReserve(int maxPathsInFlight, int pixels, int
 
 namespace {
 
+// One scene (tab == null), or the batch whose expanded table is tab (tabCount scenes,
+// batchPixels pixels in all; the single-scene arguments are then unused).
 bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2, uint8_t* d_converged,
                        int width, int height, const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
                        const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, int samplesPerPixel,
-                       int maxPathDepth, int maxPathsInFlight, unsigned int firstSeed, hipStream_t stream)
+                       int maxPathDepth, int maxPathsInFlight, unsigned int firstSeed, hipStream_t stream,
+                       const BatchScene* tab = nullptr, int tabCount = 0, int batchPixels = 0)
 {
+    const bool single = tab == nullptr;
     if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
-        d_converged == nullptr || d_camera == nullptr || d_scene == nullptr || d_params == nullptr || width <= 0 ||
-        height <= 0 || samplesPerPixel <= 0 || maxPathDepth < 0 || maxPathDepth > 256) {
-        ptSetError("pathTracerRenderFixedWavefront: invalid arguments");
+        d_converged == nullptr || samplesPerPixel <= 0 || maxPathDepth < 0 || maxPathDepth > 256 ||
+        (single && (d_camera == nullptr || d_scene == nullptr || d_params == nullptr || width <= 0 || height <= 0)) ||
+        (!single && (tabCount <= 0 || batchPixels <= 0))) {
+        ptSetError(single ? "pathTracerRenderFixedWavefront: invalid arguments"
+                          : "pathTracerRenderFixedBatch: invalid arguments");
         return false;
     }
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(stream, &capture);
     const bool capturing = capture == hipStreamCaptureStatusActive;
-    const int pixels = width * height;
+    const int pixels = single ? width * height : batchPixels;
     const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 24);
     int device = 0, cus = 256;
     if (hipGetDevice(&device) == hipSuccess) {
@@ -1776,15 +1931,16 @@ bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean
                            counterInts);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, d_samples,
                            d_converged, d_camera, d_params, width, height, firstSeed, passBase, samplesPerPixel, g, P,
-                           spec, -1);
+                           spec, -1, tab, tabCount);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, static_cast<long long>(g) * pixels,
-                       persistentBlocks, spec, sm);
+                       persistentBlocks, spec, sm, tab);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, P,
-                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params);
+                           reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params, tab,
+                           tabCount);
     }
     gCapturing = false;
     if (hipGetLastError() != hipSuccess) {
-        ptSetError("pathTracerRenderFixedWavefront: launch failed");
+        ptSetError(single ? "pathTracerRenderFixedWavefront: launch failed" : "pathTracerRenderFixedBatch: launch failed");
         return false;
     }
     return true;
@@ -1811,22 +1967,46 @@ struct AdjointWs {
     void* mem = nullptr;
     int capacity = 0;
     int pixels = 0;
-    int rows = 0;
+    size_t partialDoubles = 0;
     AdjointState A[2]{};
     uint32_t* counts = nullptr;  // per pixel: samples taken so far (the generation kernel's n)
     uint8_t* conv = nullptr;     // per pixel: 0
-    double* partial = nullptr;   // rows x kMaxGradMaterials x kGradParams
+    double* partial = nullptr;   // one row of gradient sums per block of the shade grid
+    // the primal per-sample radiance and the accumulators of the render that records it, when
+    // the caller has none (grow-only: adjoint calls on a device do not overlap)
+    void* tmp = nullptr;
+    size_t tmpBytes = 0;
 };
 AdjointWs gAd;
 
-bool ensure_adjoint(int cap, int pixels, int rows)
+bool ensure_adjoint_tmp(size_t bytes)
 {
-    if (gAd.mem != nullptr && gAd.capacity >= cap && gAd.pixels >= pixels && gAd.rows >= rows) {
+    if (gAd.tmp != nullptr && gAd.tmpBytes >= bytes) {
+        return true;
+    }
+    if (gAd.tmp != nullptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(gAd.tmp);
+        gAd.tmp = nullptr;
+        gAd.tmpBytes = 0;
+    }
+    if (hipMalloc(&gAd.tmp, bytes) != hipSuccess) {
+        gAd.tmp = nullptr;
+        ptSetError("pathTracerAdjointRender: radiance buffer allocation of %zu bytes failed", bytes);
+        return false;
+    }
+    gAd.tmpBytes = bytes;
+    return true;
+}
+
+bool ensure_adjoint(int cap, int pixels, size_t partialDoubles)
+{
+    if (gAd.mem != nullptr && gAd.capacity >= cap && gAd.pixels >= pixels && gAd.partialDoubles >= partialDoubles) {
         return true;
     }
     cap = std::max(cap, gAd.capacity);
     pixels = std::max(pixels, gAd.pixels);
-    rows = std::max(rows, gAd.rows);
+    partialDoubles = std::max(partialDoubles, gAd.partialDoubles);
     auto lay = [&](char* base) {
         char* cur = base;
         for (AdjointState& A : gAd.A) {
@@ -1841,7 +2021,7 @@ bool ensure_adjoint(int cap, int pixels, int rows)
         }
         gAd.counts = carve<uint32_t>(cur, pixels);
         gAd.conv = carve<uint8_t>(cur, pixels);
-        gAd.partial = carve<double>(cur, static_cast<size_t>(rows) * kMaxGradMaterials * kGradParams);
+        gAd.partial = carve<double>(cur, partialDoubles);
         return static_cast<size_t>(cur - base) + 256;
     };
     const size_t bytes = lay(nullptr);
@@ -1853,14 +2033,130 @@ bool ensure_adjoint(int cap, int pixels, int rows)
     void* mem = nullptr;
     if (hipMalloc(&mem, bytes) != hipSuccess) {
         ptSetError("adjoint workspace allocation of %zu bytes failed", bytes);
-        gAd = AdjointWs{};
+        gAd.capacity = 0;
+        gAd.pixels = 0;
+        gAd.partialDoubles = 0;
         return false;
     }
     gAd.mem = mem;
     gAd.capacity = cap;
     gAd.pixels = pixels;
-    gAd.rows = rows;
+    gAd.partialDoubles = partialDoubles;
     lay(static_cast<char*>(mem));
+    return true;
+}
+
+} // namespace
+
+namespace {
+
+// maxPathDepth as the integrator resolves it (PathIntegratorRandCore.h:219-227)
+__host__ __device__ inline int resolve_depth(int maxPathDepth)
+{
+    return maxPathDepth > 0 ? maxPathDepth : (maxPathDepth == 0 ? 256 : 32);
+}
+
+// Renders the samples of seeds firstSeed.. and records their per-sample radiance into the
+// adjoint's grow-only buffer (one scene, or the batch of tab); returns the radiance or null.
+const float* adjoint_primal(int width, int height, int pixels, int spp, int maxDepth, const CameraGpu* d_camera,
+                            const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                            const RenderParamsGpu* d_params, unsigned int firstSeed, hipStream_t stream,
+                            const BatchScene* tab, int tabCount)
+{
+    const size_t nRad = static_cast<size_t>(pixels) * spp * 3;
+    const size_t bytes = nRad * sizeof(float) + static_cast<size_t>(pixels) * (16 + 4 + 4 + 4 + 1) + 6 * 256;
+    if (!ensure_adjoint_tmp(bytes)) {
+        return nullptr;
+    }
+    float* rad = static_cast<float*>(gAd.tmp);
+    char* cur = reinterpret_cast<char*>(rad + nRad);
+    PtFloat4* acc = carve<PtFloat4>(cur, pixels);
+    uint32_t* cnt = carve<uint32_t>(cur, pixels);
+    float* mean = carve<float>(cur, pixels);
+    float* m2 = carve<float>(cur, pixels);
+    uint8_t* cv = carve<uint8_t>(cur, pixels);
+    float* savedCapture = gCapture;
+    const int savedSpp = gCaptureSpp;
+    gCapture = rad;
+    gCaptureSpp = spp;
+    const bool ok = hipMemsetAsync(cnt, 0, pixels * sizeof(uint32_t), stream) == hipSuccess &&
+                    hipMemsetAsync(cv, 0, pixels, stream) == hipSuccess &&
+                    hipMemsetAsync(mean, 0, pixels * sizeof(float), stream) == hipSuccess &&
+                    hipMemsetAsync(m2, 0, pixels * sizeof(float), stream) == hipSuccess &&
+                    hipMemsetAsync(acc, 0, pixels * sizeof(PtFloat4), stream) == hipSuccess &&
+                    render_fixed_impl(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene, d_env, d_params, spp,
+                                      maxDepth, 0, firstSeed, stream, tab, tabCount, pixels);
+    gCapture = savedCapture;
+    gCaptureSpp = savedSpp;
+    return ok ? rad : nullptr;
+}
+
+// The replay of every sample (seeds firstSeed..) against the primal radiance: gradient rows
+// (7 doubles each) summed into d_grad.
+bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels, int spp, int maxDepth,
+                    const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
+                    const RenderParamsGpu* d_params, int gradRows, const float* radiance, double* d_grad,
+                    unsigned int firstSeed, hipStream_t stream, const BatchScene* tab, int tabCount)
+{
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) == hipSuccess) {
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    const int persistentBlocks = cus * 8;
+    const int spillThreads = persistentBlocks * kWfBlock;
+    const int target = 1 << 24;
+    const int cap = std::max(pixels, std::min(target, spp * pixels));
+    const int perCopy = gradRows * kGradParams;
+    if (!ensure_workspace(cap, pixels, maxDepth, spillThreads) ||
+        !ensure_adjoint(cap, pixels, static_cast<size_t>(persistentBlocks) * perCopy)) {
+        return false;
+    }
+    Paths P = gWs.P;
+    int* list = gWs.lists[0];
+    int* count = gWs.listCount;
+    const int counterInts = kCountersPerBounce * (maxDepth + 2);
+    const int group = std::max(1, std::min(spp, std::min(gWs.capacity, target) / pixels));
+    // LDS copies of the per-row sums: as many as fit 24 KB (at least one)
+    int copies = 16;
+    while (copies > 1 && static_cast<size_t>(copies) * perCopy * sizeof(double) > 24576) {
+        copies >>= 1;
+    }
+    const size_t lds = static_cast<size_t>(copies) * perCopy * sizeof(double);
+    const float invSpp = 1.0f / static_cast<float>(spp);
+    hipLaunchKernelGGL(wf_iota_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, pixels);
+    hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream,
+                       reinterpret_cast<uint32_t*>(gAd.conv), (pixels + 3) / 4, 0u);
+    hipLaunchKernelGGL(wf_zero_f64_kernel, dim3(blocks_for(static_cast<long long>(persistentBlocks) * perCopy)),
+                       dim3(kWfBlock), 0, stream, gAd.partial, static_cast<long long>(persistentBlocks) * perCopy);
+    const auto shadeKernel = tab != nullptr ? wf_adjoint_shade_kernel<true> : wf_adjoint_shade_kernel<false>;
+    for (int passBase = 0; passBase < spp; passBase += group) {
+        const int g = std::min(group, spp - passBase);
+        hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, gAd.counts, pixels,
+                           static_cast<uint32_t>(passBase));
+        hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
+                           counterInts);
+        hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count,
+                           gAd.counts, gAd.conv, d_camera, d_params, width, height, firstSeed, passBase, spp, g, P,
+                           false, 0, tab, tabCount);
+        hipLaunchKernelGGL(wf_adjoint_init_kernel, dim3(blocks_for(static_cast<long long>(g) * pixels)),
+                           dim3(kWfBlock), 0, stream, P.S[0], gAd.A[0], &P.counters[3], d_gradImage, radiance, spp,
+                           invSpp, tab);
+        launch_bounces_with(
+            stream, P, d_scene, maxDepth, static_cast<long long>(g) * pixels, persistentBlocks,
+            [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
+                hipLaunchKernelGGL(shadeKernel, dim3(traceBlocks), dim3(kWfBlock), lds, stream, d_scene, d_params,
+                                   d_env, P.S[b & 1], P.S[(b + 1) & 1], gAd.A[b & 1], gAd.A[(b + 1) & 1], P, b,
+                                   maxDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
+                                   P.hitA[(b + 1) & 1], cOut, perCopy / kGradParams, copies, gAd.partial, tab);
+            },
+            tab);
+    }
+    hipLaunchKernelGGL(wf_adjoint_reduce_kernel, dim3(static_cast<unsigned>(perCopy)), dim3(kWfBlock), 0, stream,
+                       gAd.partial, persistentBlocks, perCopy, d_grad);
+    if (hipGetLastError() != hipSuccess) {
+        ptSetError("pathTracerAdjointRender: launch failed");
+        return false;
+    }
     return true;
 }
 
@@ -1877,103 +2173,243 @@ bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int s
         ptSetError("pathTracerAdjointRender: reading the render parameters failed");
         return false;
     }
-    const int maxDepth = hp.maxPathDepth > 0 ? hp.maxPathDepth : (hp.maxPathDepth == 0 ? 256 : 32);
+    const int maxDepth = resolve_depth(hp.maxPathDepth);
     const int pixels = width * height;
     // the primal per-sample radiance: the caller's (stored by the forward) or a fixed-spp render
     const float* radiance = d_sampleRadiance;
-    float* tmp = nullptr;
     if (radiance == nullptr) {
-        const size_t nRad = static_cast<size_t>(pixels) * spp * 3;
-        const size_t accBytes = static_cast<size_t>(pixels) * (16 + 4 + 4 + 4 + 1);
-        if (hipMalloc(&tmp, nRad * sizeof(float) + accBytes + 256) != hipSuccess) {
-            ptSetError("pathTracerAdjointRender: radiance buffer allocation failed");
+        radiance = adjoint_primal(width, height, pixels, spp, maxDepth, d_camera, d_scene, d_env, d_params, firstSeed,
+                                  stream, nullptr, 0);
+        if (radiance == nullptr) {
             return false;
         }
-        char* cur = reinterpret_cast<char*>(tmp) + nRad * sizeof(float);
-        PtFloat4* acc = carve<PtFloat4>(cur, pixels);
-        uint32_t* cnt = carve<uint32_t>(cur, pixels);
-        float* mean = carve<float>(cur, pixels);
-        float* m2 = carve<float>(cur, pixels);
-        uint8_t* cv = carve<uint8_t>(cur, pixels);
-        float* savedCapture = gCapture;
-        const int savedSpp = gCaptureSpp;
-        gCapture = tmp;
-        gCaptureSpp = spp;
-        const bool ok = hipMemsetAsync(cnt, 0, pixels * sizeof(uint32_t), stream) == hipSuccess &&
-                        hipMemsetAsync(cv, 0, pixels, stream) == hipSuccess &&
-                        hipMemsetAsync(mean, 0, pixels * sizeof(float), stream) == hipSuccess &&
-                        hipMemsetAsync(m2, 0, pixels * sizeof(float), stream) == hipSuccess &&
-                        hipMemsetAsync(acc, 0, pixels * sizeof(PtFloat4), stream) == hipSuccess &&
-                        render_fixed_impl(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene, d_env, d_params,
-                                          spp, maxDepth, 0, firstSeed, stream);
-        gCapture = savedCapture;
-        gCaptureSpp = savedSpp;
-        if (!ok) {
-            (void)hipFree(tmp);
-            return false;
-        }
-        radiance = tmp;
     }
-    int device = 0, cus = 256;
-    if (hipGetDevice(&device) == hipSuccess) {
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    }
-    const int persistentBlocks = cus * 8;
-    const int spillThreads = persistentBlocks * kWfBlock;
-    const int target = 1 << 24;
-    const int cap = std::max(pixels, std::min(target, spp * pixels));
-    bool ok = ensure_workspace(cap, pixels, maxDepth, spillThreads) && ensure_adjoint(cap, pixels, persistentBlocks);
-    if (ok) {
-        Paths P = gWs.P;
-        int* list = gWs.lists[0];
-        int* count = gWs.listCount;
-        const int counterInts = kCountersPerBounce * (maxDepth + 2);
-        const int group = std::max(1, std::min(spp, std::min(gWs.capacity, target) / pixels));
-        const int perCopy = materialCount * kGradParams;
-        int copies = 16;
-        while (copies > 1 && static_cast<size_t>(copies) * perCopy * sizeof(double) > 24576) {
-            copies >>= 1;
-        }
-        const size_t lds = static_cast<size_t>(copies) * perCopy * sizeof(double);
-        const float invSpp = 1.0f / static_cast<float>(spp);
-        hipLaunchKernelGGL(wf_iota_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, pixels);
-        hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream,
-                           reinterpret_cast<uint32_t*>(gAd.conv), (pixels + 3) / 4, 0u);
-        hipLaunchKernelGGL(wf_zero_f64_kernel, dim3(blocks_for(static_cast<long long>(persistentBlocks) * perCopy)),
-                           dim3(kWfBlock), 0, stream, gAd.partial, static_cast<long long>(persistentBlocks) * perCopy);
-        for (int passBase = 0; passBase < spp; passBase += group) {
-            const int g = std::min(group, spp - passBase);
-            hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, gAd.counts, pixels,
-                               static_cast<uint32_t>(passBase));
-            hipLaunchKernelGGL(wf_zero_kernel, dim3(blocks_for(counterInts)), dim3(kWfBlock), 0, stream, P.counters,
-                               counterInts);
-            hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count,
-                               gAd.counts, gAd.conv, d_camera, d_params, width, height, firstSeed, passBase, spp, g, P,
-                               false, 0);
-            hipLaunchKernelGGL(wf_adjoint_init_kernel, dim3(blocks_for(static_cast<long long>(g) * pixels)),
-                               dim3(kWfBlock), 0, stream, P.S[0], gAd.A[0], &P.counters[3], d_gradImage, radiance,
-                               spp, invSpp);
-            launch_bounces_with(stream, P, d_scene, maxDepth, static_cast<long long>(g) * pixels, persistentBlocks,
-                                [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
-                                    hipLaunchKernelGGL(wf_adjoint_shade_kernel, dim3(traceBlocks), dim3(kWfBlock), lds,
-                                                       stream, d_scene, d_params, d_env, P.S[b & 1],
-                                                       P.S[(b + 1) & 1], gAd.A[b & 1], gAd.A[(b + 1) & 1], P, b,
-                                                       maxDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount,
-                                                       P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, materialCount,
-                                                       copies, gAd.partial);
-                                });
-        }
-        hipLaunchKernelGGL(wf_adjoint_reduce_kernel, dim3(static_cast<unsigned>(perCopy)), dim3(kWfBlock), 0, stream,
-                           gAd.partial, persistentBlocks, perCopy, d_grad);
-        if (hipGetLastError() != hipSuccess) {
-            ptSetError("pathTracerAdjointRender: launch failed");
-            ok = false;
-        }
-    }
-    if (tmp != nullptr) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipFree(tmp);
-    }
-    return ok;
+    return adjoint_replay(d_gradImage, width, height, pixels, spp, maxDepth, d_camera, d_scene, d_env, d_params,
+                          materialCount, radiance, d_grad, firstSeed, stream, nullptr, 0);
 }
 
+// ---- batches of scenes --------------------------------------------------------------------
+
+namespace {
+
+// The caller's pointers for one scene plus its place in the batch (device copy of the table).
+struct BatchRef {
+    const MeshAccelSceneGpu* scene;
+    const CameraGpu* camera;
+    const RenderParamsGpu* params;
+    const EnvironmentMapGpu* env;
+    int width, height, pixelOffset, gradRow, maxDepth;
+};
+
+// Copies every scene's description, parameters, camera and environment into the batch table, so
+// the render reads what the caller's device records hold at this call.
+__global__ void wf_batch_expand_kernel(const BatchRef* __restrict__ refs, int count, BatchScene* __restrict__ tab)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= count) {
+        return;
+    }
+    const BatchRef r = refs[i];
+    BatchScene b;
+    b.scene = *r.scene;
+    b.params = *r.params;
+    b.camera = *r.camera;
+    if (r.env != nullptr) {
+        b.env = *r.env;
+    } else {
+        b.env.pixels = nullptr;
+        b.env.marginalCdf = nullptr;
+        b.env.rowCdf = nullptr;
+        b.env.width = 0;
+        b.env.height = 0;
+        b.env.valid = 0;
+    }
+    b.width = r.width;
+    b.height = r.height;
+    b.pixelOffset = r.pixelOffset;
+    b.gradRow = r.gradRow;
+    b.maxDepth = min(resolve_depth(b.params.maxPathDepth), r.maxDepth);
+    tab[i] = b;
+}
+
+} // namespace
+
+struct PtBatch {
+    int count = 0;
+    int totalPixels = 0;
+    int gradRows = 0;
+    int maxDepth = 0;
+    std::vector<int> pixelOffsets;
+    std::vector<int> gradRowOffsets;
+    BatchRef* refs = nullptr;  // device
+    BatchScene* tab = nullptr; // device, filled at every call
+};
+
+namespace {
+
+void batch_free(PtBatch* b)
+{
+    if (b->refs != nullptr) {
+        (void)hipFree(b->refs);
+    }
+    if (b->tab != nullptr) {
+        (void)hipFree(b->tab);
+    }
+    delete b;
+}
+
+bool batch_expand(const PtBatch* b, hipStream_t stream)
+{
+    hipLaunchKernelGGL(wf_batch_expand_kernel, dim3(static_cast<unsigned>((b->count + kWfBlock - 1) / kWfBlock)),
+                       dim3(kWfBlock), 0, stream, b->refs, b->count, b->tab);
+    return hipGetLastError() == hipSuccess;
+}
+
+} // namespace
+
+extern "C" PtBatch* ptBatchCreate(const PtBatchSceneGpu* scenes, int sceneCount)
+{
+    if (scenes == nullptr || sceneCount <= 0 || sceneCount > kMaxBatchScenes) {
+        ptSetError("ptBatchCreate: invalid arguments (1..%d scenes)", kMaxBatchScenes);
+        return nullptr;
+    }
+    std::vector<BatchRef> refs(static_cast<size_t>(sceneCount));
+    auto* b = new PtBatch();
+    b->count = sceneCount;
+    long long pixels = 0;
+    long long rows = 0;
+    for (int i = 0; i < sceneCount; ++i) {
+        const PtBatchSceneGpu& e = scenes[i];
+        if (e.scene == nullptr || e.camera == nullptr || e.params == nullptr || e.width <= 0 || e.height <= 0) {
+            ptSetError("ptBatchCreate: scene %d: invalid arguments", i);
+            batch_free(b);
+            return nullptr;
+        }
+        MeshAccelSceneGpu desc;
+        RenderParamsGpu prm;
+        if (hipMemcpy(&desc, e.scene, sizeof(desc), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&prm, e.params, sizeof(prm), hipMemcpyDeviceToHost) != hipSuccess) {
+            ptSetError("ptBatchCreate: scene %d: reading its description or parameters failed", i);
+            batch_free(b);
+            return nullptr;
+        }
+        const int depth = resolve_depth(prm.maxPathDepth);
+        b->maxDepth = std::max(b->maxDepth, depth);
+        refs[static_cast<size_t>(i)] = BatchRef{e.scene, e.camera, e.params, e.env, e.width, e.height,
+                                               static_cast<int>(pixels), static_cast<int>(rows), 0};
+        b->pixelOffsets.push_back(static_cast<int>(pixels));
+        b->gradRowOffsets.push_back(static_cast<int>(rows));
+        pixels += static_cast<long long>(e.width) * e.height;
+        rows += desc.materialCount;
+        if (pixels > (1ll << 30)) {
+            ptSetError("ptBatchCreate: more than 2^30 pixels in one batch");
+            batch_free(b);
+            return nullptr;
+        }
+    }
+    if (b->maxDepth > 256) {
+        ptSetError("ptBatchCreate: path depth above 256");
+        batch_free(b);
+        return nullptr;
+    }
+    for (BatchRef& r : refs) {
+        r.maxDepth = b->maxDepth;
+    }
+    b->totalPixels = static_cast<int>(pixels);
+    b->gradRows = static_cast<int>(rows);
+    if (hipMalloc(&b->refs, refs.size() * sizeof(BatchRef)) != hipSuccess ||
+        hipMalloc(&b->tab, refs.size() * sizeof(BatchScene)) != hipSuccess ||
+        hipMemcpy(b->refs, refs.data(), refs.size() * sizeof(BatchRef), hipMemcpyHostToDevice) != hipSuccess) {
+        ptSetError("ptBatchCreate: device table allocation failed");
+        batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" void ptBatchDestroy(PtBatch* batch)
+{
+    if (batch != nullptr) {
+        (void)hipDeviceSynchronize();
+        batch_free(batch);
+    }
+}
+
+extern "C" bool ptBatchInfo(const PtBatch* batch, int* totalPixels, int* gradRows, int* maxPathDepth,
+                            int* pixelOffsets, int* gradRowOffsets)
+{
+    if (batch == nullptr) {
+        ptSetError("ptBatchInfo: no batch");
+        return false;
+    }
+    if (totalPixels != nullptr) {
+        *totalPixels = batch->totalPixels;
+    }
+    if (gradRows != nullptr) {
+        *gradRows = batch->gradRows;
+    }
+    if (maxPathDepth != nullptr) {
+        *maxPathDepth = batch->maxDepth;
+    }
+    for (int i = 0; i < batch->count; ++i) {
+        if (pixelOffsets != nullptr) {
+            pixelOffsets[i] = batch->pixelOffsets[static_cast<size_t>(i)];
+        }
+        if (gradRowOffsets != nullptr) {
+            gradRowOffsets[i] = batch->gradRowOffsets[static_cast<size_t>(i)];
+        }
+    }
+    return true;
+}
+
+extern "C" bool pathTracerRenderFixedBatch(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
+                                           uint8_t* d_converged, const PtBatch* batch, int samplesPerPixel,
+                                           int maxPathsInFlight, unsigned int firstSeed, hipStream_t stream)
+{
+    if (batch == nullptr) {
+        ptSetError("pathTracerRenderFixedBatch: no batch");
+        return false;
+    }
+    if (!batch_expand(batch, stream)) {
+        ptSetError("pathTracerRenderFixedBatch: launch failed");
+        return false;
+    }
+    return render_fixed_impl(d_buffer, d_samples, d_lumMean, d_m2, d_converged, 0, 0, nullptr, nullptr, nullptr,
+                             nullptr, samplesPerPixel, batch->maxDepth, maxPathsInFlight, firstSeed, stream,
+                             batch->tab, batch->count, batch->totalPixels);
+}
+
+extern "C" bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtBatch* batch, int samplesPerPixel,
+                                             unsigned int firstSeed, double* d_grad, hipStream_t stream)
+{
+    if (d_gradImage == nullptr || batch == nullptr || d_grad == nullptr || samplesPerPixel <= 0) {
+        ptSetError("pathTracerAdjointRenderBatch: invalid arguments");
+        return false;
+    }
+    if (batch->gradRows <= 0 || batch->gradRows > kMaxBatchGradRows) {
+        ptSetError("pathTracerAdjointRenderBatch: %d material rows (1..%d)", batch->gradRows, kMaxBatchGradRows);
+        return false;
+    }
+    if (pt_spectral_enabled()) {
+        ptSetError("pathTracerAdjointRenderBatch: RGB rendering only (hero-wavelength sampling is on)");
+        return false;
+    }
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(stream, &capture);
+    if (capture == hipStreamCaptureStatusActive) {
+        ptSetError("pathTracerAdjointRenderBatch: not capturable");
+        return false;
+    }
+    if (!batch_expand(batch, stream)) {
+        ptSetError("pathTracerAdjointRenderBatch: launch failed");
+        return false;
+    }
+    const float* radiance = adjoint_primal(0, 0, batch->totalPixels, samplesPerPixel, batch->maxDepth, nullptr,
+                                           nullptr, nullptr, nullptr, firstSeed, stream, batch->tab, batch->count);
+    if (radiance == nullptr) {
+        return false;
+    }
+    return adjoint_replay(d_gradImage, 0, 0, batch->totalPixels, samplesPerPixel, batch->maxDepth, nullptr, nullptr,
+                          nullptr, nullptr, batch->gradRows, radiance, d_grad, firstSeed, stream, batch->tab,
+                          batch->count);
+}

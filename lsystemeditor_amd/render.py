@@ -246,3 +246,68 @@ def render_fixed(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, spp: 
         acc.acc.data_ptr(), acc.counts.data_ptr(), acc.mean.data_ptr(), acc.m2.data_ptr(), acc.conv.data_ptr(),
         acc.width, acc.height, inputs.camera.data_ptr(), scene.device, inputs.env_ptr, inputs.params.data_ptr(),
         spp, resolved_depth(inputs.params_host), max_paths, _stream_handle(stream)), "pathTracerRenderFixedWavefront")
+
+
+class SceneBatch:
+    """Scenes rendered in one wavefront (ptBatchCreate): their pixels concatenated in one
+    accumulator, each scene's result equal to that of its own fixed-spp render."""
+
+    def __init__(self, scenes: list, inputs: list, sizes: list[tuple[int, int]]):
+        if not (len(scenes) == len(inputs) == len(sizes)) or not scenes:
+            raise ValueError("one input set and size per scene, at least one scene")
+        n = len(scenes)
+        entries = (abi.PtBatchSceneGpu * n)()
+        for e, sc, inp, (w, h) in zip(entries, scenes, inputs, sizes):
+            e.scene = sc.device
+            e.camera = inp.camera.data_ptr()
+            e.params = inp.params.data_ptr()
+            e.env = inp.env_ptr
+            e.width = w
+            e.height = h
+        L = abi.lib()
+        self.handle = L.ptBatchCreate(entries, n)
+        if not self.handle:
+            raise RuntimeError(f"ptBatchCreate failed: {abi.last_error()}")
+        self.scenes, self.inputs, self.sizes = list(scenes), list(inputs), list(sizes)
+        tp, rows, depth = C.c_int(), C.c_int(), C.c_int()
+        po = (C.c_int * n)()
+        ro = (C.c_int * n)()
+        abi.check(L.ptBatchInfo(self.handle, C.byref(tp), C.byref(rows), C.byref(depth), po, ro), "ptBatchInfo")
+        self.total_pixels, self.grad_rows, self.max_depth = tp.value, rows.value, depth.value
+        self.pixel_offsets = list(po)
+        self.grad_row_offsets = list(ro)
+        self.acc = Accumulator.allocate(self.total_pixels, 1)
+
+    def __len__(self) -> int:
+        return len(self.scenes)
+
+    def reserve(self, max_paths: int = 0) -> None:
+        """Sizes the shared workspace (needed before graph capture)."""
+        paths = max(max_paths if max_paths > 0 else (1 << 24), self.total_pixels)
+        abi.check(abi.lib().pathTracerWavefrontReserve(paths, self.total_pixels, self.max_depth),
+                  "pathTracerWavefrontReserve")
+
+    def render_fixed(self, spp: int, first_seed: int = 1, max_paths: int = 0, clear: bool = True,
+                     stream=None) -> None:
+        """Every scene's fixed-spp render (sample k with seed first_seed + k) in one wavefront."""
+        if clear:
+            self.acc.clear(stream)
+        a = self.acc
+        abi.check(abi.lib().pathTracerRenderFixedBatch(
+            a.acc.data_ptr(), a.counts.data_ptr(), a.mean.data_ptr(), a.m2.data_ptr(), a.conv.data_ptr(), self.handle,
+            spp, max_paths, first_seed, _stream_handle(stream)), "pathTracerRenderFixedBatch")
+
+    def pixels(self, i: int) -> slice:
+        w, h = self.sizes[i]
+        return slice(self.pixel_offsets[i], self.pixel_offsets[i] + w * h)
+
+    def close(self) -> None:
+        if self.handle:
+            abi.lib().ptBatchDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
