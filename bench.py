@@ -174,8 +174,9 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
 
 def batch_bench(bvh: str) -> dict:
     """C3: 64 bushes (seeds 0..63, iterations 3..7, 128^2..1024^2, 64 spp, depth 8) rendered by one
-    graph replay, against the same 64 renders issued one by one through the adaptive entry point.
-    The paths-in-flight bound is autotuned on the batch's largest scene."""
+    batch wavefront replayed from a graph, against the per-scene graph (every scene's own render
+    back to back) and the same 64 renders issued one by one through the adaptive entry point.
+    Paths-in-flight bounds are autotuned (per-scene: on the largest scene; batch: on the batch)."""
     import torch
     from lsystemeditor_amd import batch
 
@@ -199,15 +200,29 @@ def batch_bench(bvh: str) -> dict:
     br.replay()
     torch.cuda.synchronize()
     t_graph = time.perf_counter() - t0
+    # one batch wavefront over all 64 scenes, its paths-in-flight bound autotuned, graph-captured
+    bbest, btune = batch.autotune_batch_paths(br)
+    br.max_paths = bbest
+    br.capture_batched()
+    br.replay_batched()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br.replay_batched()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
     tri = sum(s.triangles.shape[0] for s in specs)
-    print(f"[bench] C3 {len(specs)} scenes, {tri} triangles, {samples / 1e6:.0f} Msamples: graph "
-          f"{samples / t_graph / 1e6:.1f} Msamples/s, sequential {samples / t_seq / 1e6:.1f} Msamples/s "
-          f"(paths in flight {best})", file=sys.stderr, flush=True)
+    print(f"[bench] C3 {len(specs)} scenes, {tri} triangles, {samples / 1e6:.0f} Msamples: batch wavefront "
+          f"{samples / t_batch / 1e6:.1f} Msamples/s (paths in flight {bbest}), per-scene graph "
+          f"{samples / t_graph / 1e6:.1f}, sequential {samples / t_seq / 1e6:.1f} Msamples/s",
+          file=sys.stderr, flush=True)
     return {"workload": "C3: 64 bushes, seeds 0-63, iterations 3-7, resolutions 128^2-1024^2, 64 spp, depth 8",
             "scenes": len(specs), "triangles": int(tri), "samples": samples,
+            "batch_wavefront_msamples_s": samples / t_batch / 1e6,
             "graph_msamples_s": samples / t_graph / 1e6, "sequential_msamples_s": samples / t_seq / 1e6,
-            "speedup_vs_sequential": t_seq / t_graph, "autotuned_paths_in_flight": best,
-            "autotune_ms": {str(k): round(v, 2) for k, v in tune.items()},
+            "speedup_vs_sequential": t_seq / t_batch, "per_scene_graph_speedup_vs_sequential": t_seq / t_graph,
+            "autotuned_paths_in_flight": best, "autotune_ms": {str(k): round(v, 2) for k, v in tune.items()},
+            "batch_autotuned_paths_in_flight": bbest,
+            "batch_autotune_ms": {str(k): round(v, 2) for k, v in btune.items()},
             "bvh_build_ms_total": round(br.build_ms, 1), "scene_generation_s": round(gen_s, 1)}
 
 

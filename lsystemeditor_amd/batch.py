@@ -1,6 +1,12 @@
-"""Batched rendering of many L-system scenes of uneven size: one HIP graph replays every scene's
-fixed-spp wavefront render back to back (no host synchronisation between or inside scenes), with
-the wavefront group size (paths in flight) picked by a timing autotuner.
+"""Batched rendering of many L-system scenes of uneven size. Two schedules:
+
+- batch wavefront (render_batched / capture_batched): the scenes form one batch
+  (ptBatchCreate) and every bounce's trace and shade launches serve the paths of all scenes, so
+  the latency tail of a launch (its slowest rays) is paid once per bounce for the whole batch;
+- per-scene sequence (render_all / capture): one HIP graph replays every scene's own fixed-spp
+  wavefront render back to back.
+
+Both need no host synchronisation; the paths-in-flight bound is picked by a timing autotuner.
 
 Workload C3: 64 stochastic bushes, seeds 0..63, iterations uniform in 3..7 (3 k .. 2 M triangles),
 resolutions drawn from 128^2 .. 1024^2, 64 spp, max depth 8.
@@ -49,6 +55,8 @@ class BatchRenderer:
         self.accs = [render.Accumulator.allocate(s.width, s.height) for s in specs]
         self.max_paths = max_paths
         self.graph = None
+        self.batch = None
+        self.batch_graph = None
         torch.cuda.synchronize()
 
     @property
@@ -83,6 +91,37 @@ class BatchRenderer:
     def replay(self):
         self.graph.replay()
 
+    def _batch(self) -> render.SceneBatch:
+        if self.batch is None:
+            self.batch = render.SceneBatch(self.scenes, self.inputs, [(s.width, s.height) for s in self.specs])
+        return self.batch
+
+    def render_batched(self, max_paths: int | None = None):
+        """Every scene's fixed-spp render in one batch wavefront (spp of the first scene; all
+        scenes of a batch take the same count)."""
+        b = self._batch()
+        mp = self.max_paths if max_paths is None else max_paths
+        b.reserve(mp)
+        with torch.cuda.stream(self.stream):
+            b.render_fixed(self.specs[0].spp, max_paths=mp, stream=self.stream)
+
+    def capture_batched(self):
+        """Records render_batched into one graph."""
+        b = self._batch()
+        b.reserve(self.max_paths)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            b.render_fixed(self.specs[0].spp, max_paths=self.max_paths, stream=torch.cuda.current_stream())
+        self.batch_graph = g
+
+    def replay_batched(self):
+        self.batch_graph.replay()
+
+    def batched_image(self, i: int) -> torch.Tensor:
+        """Scene i's (pixels, 4) accumulator rows of the last batch render."""
+        return self._batch().acc.acc[self._batch().pixels(i)]
+
     def render_sequential_adaptive_api(self):
         """The reference-style sequence: one plain render per scene through the adaptive
         wavefront entry point, with its host synchronisation per group."""
@@ -110,5 +149,20 @@ def autotune_paths(spec: scenes.SceneSpec, candidates=(1 << 21, 1 << 22, 1 << 23
             render.render_fixed(sc, acc, inp, spec.spp, max_paths=c)
         torch.cuda.synchronize()
         times[c] = (time.perf_counter() - t0) / reps * 1e3
+    best = min(times, key=times.get)
+    return best, times
+
+
+def autotune_batch_paths(br: BatchRenderer, candidates=(1 << 23, 1 << 24, 1 << 25, 1 << 26)) -> tuple[int, dict]:
+    """Times render_batched for each paths-in-flight candidate (at least the batch's pixel
+    count); returns the fastest and all timings (ms)."""
+    times = {}
+    for c in candidates:
+        br.render_batched(max_paths=c)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        br.render_batched(max_paths=c)
+        torch.cuda.synchronize()
+        times[c] = (time.perf_counter() - t0) * 1e3
     best = min(times, key=times.get)
     return best, times

@@ -35,6 +35,7 @@ class DiffRenderer:
         # per-sample primal radiance of the last forward, so the adjoint replays without re-tracing
         self.radiance = torch.empty((width * height * spp, 3), dtype=torch.float32, device="cuda")
         self.radiance_valid = False
+        self.uploaded: torch.Tensor | None = None  # the parameters the scene's materials hold
 
     @property
     def material_count(self) -> int:
@@ -46,11 +47,12 @@ class DiffRenderer:
 
     def set_theta(self, theta: torch.Tensor) -> None:
         self.radiance_valid = False
-        t = theta.detach().to("cpu", torch.float32).numpy()
+        t = theta.detach().to("cpu", torch.float32)
         mats = self.base.copy()
-        mats[:, list(PARAM_WORDS)] = t
+        mats[:, list(PARAM_WORDS)] = t.numpy()
         abi.check(abi.lib().ptSceneUpdateMaterials(self.scene.handle, mats.ctypes.data, mats.shape[0],
                                                    render._stream_handle(self.stream)), "ptSceneUpdateMaterials")
+        self.uploaded = t.clone()
 
     def forward(self, theta: torch.Tensor, seed_base: int = 0) -> torch.Tensor:
         """(height * width, 3) float32 image of the fixed-spp render at parameters theta. Sample k
@@ -187,3 +189,67 @@ class DecorrelatedRenderFunction(torch.autograd.Function):
 def render_image_decorrelated(theta: torch.Tensor, renderer: DiffRenderer, seed_base: int,
                               adjoint_base: int) -> torch.Tensor:
     return DecorrelatedRenderFunction.apply(theta, renderer, seed_base, adjoint_base)
+
+
+class BatchDiffRenderer:
+    """The fixed-spp renders and adjoints of several DiffRenderers (one spp for all) as one batch
+    wavefront (pathTracerRenderFixedBatch / pathTracerAdjointRenderBatch): images concatenated
+    in renderer order, each equal to that renderer's own render; gradients per renderer."""
+
+    def __init__(self, renderers: list[DiffRenderer]):
+        if not renderers or len({r.spp for r in renderers}) != 1:
+            raise ValueError("a batch needs at least one renderer, all with the same spp")
+        self.renderers = list(renderers)
+        self.spp = renderers[0].spp
+        self.batch = render.SceneBatch([r.scene for r in renderers], [r.inputs for r in renderers],
+                                       [(r.width, r.height) for r in renderers])
+
+    def pixels(self, i: int) -> slice:
+        return self.batch.pixels(i)
+
+    def set_thetas(self, thetas) -> None:
+        """Uploads every renderer's parameters that differ from what its scene holds."""
+        for r, th in zip(self.renderers, thetas):
+            t = th.detach().to("cpu", torch.float32)
+            if r.uploaded is None or not torch.equal(r.uploaded, t):
+                r.set_theta(t)
+
+    def forward(self, thetas, seed_base: int = 0) -> torch.Tensor:
+        """(total pixels, 3) images at parameters thetas; sample k uses seed seed_base + k + 1."""
+        self.set_thetas(thetas)
+        self.batch.render_fixed(self.spp, first_seed=seed_base + 1)
+        return self.batch.acc.acc[:, :3].clone()
+
+    def adjoint(self, grad_image: torch.Tensor, seed_base: int) -> list[torch.Tensor]:
+        """Per renderer (materials, 7) float64 dLoss/dtheta at the uploaded parameters for
+        dLoss/dImage = grad_image (total pixels, 3), replaying the samples of seed base seed_base."""
+        g = grad_image.detach().to("cuda", torch.float32).contiguous()
+        out = torch.zeros((self.batch.grad_rows, 7), dtype=torch.float64, device="cuda")
+        abi.check(abi.lib().pathTracerAdjointRenderBatch(g.data_ptr(), self.batch.handle, self.spp, seed_base + 1,
+                                                         out.data_ptr(), None), "pathTracerAdjointRenderBatch")
+        return [out[o:o + r.material_count] for o, r in zip(self.batch.grad_row_offsets, self.renderers)]
+
+
+class BatchDecorrelatedRenderFunction(torch.autograd.Function):
+    """DecorrelatedRenderFunction over a batch: images = render(thetas) with the samples of
+    seed base `seed_base`; the backward replays the independent samples of `adjoint_base`."""
+
+    @staticmethod
+    def forward(ctx, br: BatchDiffRenderer, seed_base: int, adjoint_base: int, *thetas):
+        if seed_base == adjoint_base:
+            raise ValueError("the forward's and the adjoint's seed bases must differ")
+        ctx.br = br
+        ctx.thetas = [t.detach().clone() for t in thetas]
+        ctx.adjoint_base = adjoint_base
+        return br.forward(thetas, seed_base=seed_base)
+
+    @staticmethod
+    def backward(ctx, grad_image: torch.Tensor):
+        br = ctx.br
+        br.set_thetas(ctx.thetas)
+        grads = br.adjoint(grad_image, ctx.adjoint_base)
+        return (None, None, None, *[g.to(t.device, t.dtype) for g, t in zip(grads, ctx.thetas)])
+
+
+def render_batch_decorrelated(thetas, br: BatchDiffRenderer, seed_base: int, adjoint_base: int) -> torch.Tensor:
+    return BatchDecorrelatedRenderFunction.apply(br, seed_base, adjoint_base, *thetas)

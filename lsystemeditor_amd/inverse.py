@@ -77,15 +77,17 @@ class InverseProblem:
             self.theta.append(torch.from_numpy(th0.copy()).requires_grad_(True))
             self.renderers.append(r)
         self.samples_per_render = [s.width * s.height * s.spp for s in specs]
+        # every scene's loss render and adjoint in one batch wavefront
+        self.batch = diff.BatchDiffRenderer(self.renderers)
+        self.target = torch.cat(self.targets)
 
     def loss(self, step: int = 0) -> torch.Tensor:
         """Summed L2 image loss of step `step`: the loss render uses seed base (2 step + 1) spp,
-        the adjoint replays seed base (2 step + 2) spp (the target is seed base 0)."""
-        total = torch.zeros((), dtype=torch.float64, device="cuda")
-        for th, r, tgt in zip(self.theta, self.renderers, self.targets):
-            img = diff.render_image_decorrelated(th, r, (2 * step + 1) * r.spp, (2 * step + 2) * r.spp)
-            total = total + ((img - tgt) ** 2).sum().double()
-        return total
+        the adjoint replays seed base (2 step + 2) spp (the target is seed base 0); all scenes in
+        one batch render and one batch adjoint."""
+        spp = self.batch.spp
+        img = diff.render_batch_decorrelated(self.theta, self.batch, (2 * step + 1) * spp, (2 * step + 2) * spp)
+        return ((img - self.target) ** 2).sum().double()
 
     def param_error(self, params=OPT_PARAMS) -> float:
         """Mean absolute error against the targets over the optimised groups and the given

@@ -179,3 +179,25 @@ def test_batch_rejects_bad_arguments(cuda):
     b = _batch(items)
     assert not L.pathTracerAdjointRenderBatch(None, b.handle, 4, 1, None, None)
     b.close()
+
+
+def test_batch_autograd_equals_per_scene_autograd(cuda):
+    """render_batch_decorrelated: images and parameter gradients equal the per-scene
+    DecorrelatedRenderFunction's."""
+    import torch
+    from lsystemeditor_amd import diff, inverse
+    specs = inverse.c4_specs(count=3, res=24, spp=4, iterations=(3, 4))
+    rs = [diff.DiffRenderer(abi.Scene(s.triangles, s.materials, bvh="sah"), s.materials, s.camera, s.params, s.width,
+                            s.height, s.spp) for s in specs]
+    thetas = [r.theta0().requires_grad_(True) for r in rs]
+    targets = [torch.full((s.width * s.height, 3), 0.3, device="cuda") for s in specs]
+    br = diff.BatchDiffRenderer(rs)
+    img = diff.render_batch_decorrelated(thetas, br, 8, 12)
+    ((img - torch.cat(targets)) ** 2).sum().backward()
+    got = [t.grad.clone() for t in thetas]
+    for i, (r, tgt) in enumerate(zip(rs, targets)):
+        th = r.theta0().requires_grad_(True)
+        im = diff.render_image_decorrelated(th, r, 8, 12)
+        assert torch.equal(im, img[br.pixels(i)]), i
+        ((im - tgt) ** 2).sum().backward()
+        assert torch.allclose(got[i], th.grad, rtol=1e-6, atol=1e-7 * float(th.grad.abs().max())), i
