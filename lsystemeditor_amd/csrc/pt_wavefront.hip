@@ -635,9 +635,9 @@ PT_DEV HeroColors make_colors(float hero, const SpecModel& sm, const HeroColors*
 PT_DEV V3 finish_sample(const RgbColors&, V3 L) { return clamp_firefly(L); }  // PathTracer.cu:101-104
 PT_DEV V3 finish_sample(const HeroColors& pol, S4 L) { return clamp_firefly_spectral(pol.to_rgb(L)); }
 
-// BATCH: paths of a batch render. Each path uses its scene's tree, materials, parameters and
-// environment; the lanes of a wave take one scene at a time (its data then stays wave-uniform),
-// and each path's depth limit is its scene's.
+// BATCH: paths of a batch render. Each path reads its scene's tree, materials, parameters and
+// environment through its own pointer into the batch table (lanes of a wave nearly always share
+// a scene, so these reads share cache lines), and its depth limit is its scene's.
 template <class Pol, bool BATCH>
 __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
     const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
@@ -837,18 +837,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
         };
         if constexpr (BATCH) {
             sid = live ? I.scene[i] : 0u;
-            bool todo = live;
-            while (__ballot(todo) != 0ull) {
-                if (todo) {
-                    const uint32_t one = __builtin_amdgcn_readfirstlane(sid);
-                    if (sid == one) {
-                        const BatchScene* B = tab + one;
-                        const MeshAccelSceneGpu sc = B->scene;
-                        const RenderParamsGpu pc = B->params;
-                        body(sc, pc, &B->env, B->maxDepth, static_cast<uint32_t>(B->pixelOffset));
-                        todo = false;
-                    }
-                }
+            if (live) {
+                const BatchScene* B = tab + sid;
+                body(B->scene, B->params, &B->env, B->maxDepth, static_cast<uint32_t>(B->pixelOffset));
             }
         } else if (live) {
             body(scene1, params1, env, maxDepth, 0u);
@@ -1195,18 +1186,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
         };
         if constexpr (BATCH) {
             sid = live ? I.scene[i] : 0u;
-            bool todo = live;
-            while (__ballot(todo) != 0ull) {
-                if (todo) {
-                    const uint32_t one = __builtin_amdgcn_readfirstlane(sid);
-                    if (sid == one) {
-                        const BatchScene* B = tab + one;
-                        const MeshAccelSceneGpu sc = B->scene;
-                        const RenderParamsGpu pc = B->params;
-                        body(sc, pc, &B->env, B->maxDepth, B->gradRow);
-                        todo = false;
-                    }
-                }
+            if (live) {
+                const BatchScene* B = tab + sid;
+                body(B->scene, B->params, &B->env, B->maxDepth, B->gradRow);
             }
         } else if (live) {
             body(scene1, params1, env, maxDepth, 0);
