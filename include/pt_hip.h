@@ -328,7 +328,9 @@ bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndi
  * (pixel, sample), 3 floats at (pixel * samplesPerPixel + sample), as
  * pathTracerWavefrontCaptureRadiance records it during the forward; without it the fixed-spp
  * render of the same samples runs first. Reads the render parameters from the device (one stream
- * synchronisation) and uses the wavefront workspace: calls on one device must not overlap.
+ * synchronisation) and uses the wavefront workspace: calls on one device must not overlap. Paths
+ * run in groups of up to 16M, or of the reserved workspace's size when that is larger
+ * (pathTracerWavefrontReserve); the grouping changes only the order of the FP64 sums.
  * No reference counterpart. */
 bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
                              const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,

@@ -2067,7 +2067,8 @@ const float* adjoint_primal(int width, int height, int pixels, int spp, int maxD
                     hipMemsetAsync(m2, 0, pixels * sizeof(float), stream) == hipSuccess &&
                     hipMemsetAsync(acc, 0, pixels * sizeof(PtFloat4), stream) == hipSuccess &&
                     render_fixed_impl(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene, d_env, d_params, spp,
-                                      maxDepth, 0, firstSeed, stream, tab, tabCount, pixels);
+                                      maxDepth, std::max(1 << 24, gWs.capacity), firstSeed, stream, tab, tabCount,
+                                      pixels);
     gCapture = savedCapture;
     gCaptureSpp = savedSpp;
     return ok ? rad : nullptr;
@@ -2086,7 +2087,8 @@ bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels,
     }
     const int persistentBlocks = cus * 8;
     const int spillThreads = persistentBlocks * kWfBlock;
-    const int target = 1 << 24;
+    // paths per group: 16M, or as many as a larger reserved workspace holds
+    const int target = std::max(1 << 24, gWs.capacity);
     const int cap = std::max(pixels, std::min(target, spp * pixels));
     const int perCopy = gradRows * kGradParams;
     if (!ensure_workspace(cap, pixels, maxDepth, spillThreads) ||

@@ -203,6 +203,10 @@ class BatchDiffRenderer:
         self.spp = renderers[0].spp
         self.batch = render.SceneBatch([r.scene for r in renderers], [r.inputs for r in renderers],
                                        [(r.width, r.height) for r in renderers])
+        # one group of paths for the whole batch where it fits 64M paths: every launch's slowest
+        # rays then cost once per bounce per render (the adjoint uses the reserved size too)
+        self.max_paths = int(min(max(self.batch.total_pixels * self.spp, 1 << 24), 1 << 26))
+        self.batch.reserve(self.max_paths)
 
     def pixels(self, i: int) -> slice:
         return self.batch.pixels(i)
@@ -217,7 +221,7 @@ class BatchDiffRenderer:
     def forward(self, thetas, seed_base: int = 0) -> torch.Tensor:
         """(total pixels, 3) images at parameters thetas; sample k uses seed seed_base + k + 1."""
         self.set_thetas(thetas)
-        self.batch.render_fixed(self.spp, first_seed=seed_base + 1)
+        self.batch.render_fixed(self.spp, first_seed=seed_base + 1, max_paths=self.max_paths)
         return self.batch.acc.acc[:, :3].clone()
 
     def adjoint(self, grad_image: torch.Tensor, seed_base: int) -> list[torch.Tensor]:
