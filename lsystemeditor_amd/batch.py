@@ -153,7 +153,7 @@ def autotune_paths(spec: scenes.SceneSpec, candidates=(1 << 21, 1 << 22, 1 << 23
     return best, times
 
 
-def autotune_batch_paths(br: BatchRenderer, candidates=(1 << 23, 1 << 24, 1 << 25, 1 << 26)) -> tuple[int, dict]:
+def autotune_batch_paths(br: BatchRenderer, candidates=(1 << 24, 1 << 25, 1 << 26, 1 << 27)) -> tuple[int, dict]:
     """Times render_batched for each paths-in-flight candidate (at least the batch's pixel
     count); returns the fastest and all timings (ms)."""
     times = {}
