@@ -28,7 +28,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Msamples/s and grad-samples/s on 1 GPU vs plain fwd at equal RMSE; HBM GB/s %"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of the traversal kernels (the dominant kernels), per unit of work:
-NODE_BYTES = 64        # internal node fetched: both child boxes + child references
+NODE_BYTES = 64        # binary internal node fetched: both child boxes + child references
+WIDE_NODE_BYTES = 112  # 4-wide node fetched: four child boxes + four child references
 TRI_BYTES = 48         # leaf triangle tested: v0, two edges, original index
 EXT_RAY_BYTES = 28 + 16  # extension ray: origin, direction, tMin read; t, u, v, triangle written
 SHADOW_RAY_BYTES = 32 + 1  # shadow ray record read; occlusion byte written
@@ -40,7 +41,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=["c1", "c0"], default="c1")
-    ap.add_argument("--bvh", choices=["median", "sah"], default="sah",
+    ap.add_argument("--bvh", choices=["median", "sah", "sah-binary"], default="sah",
                     help="BVH build: the reference's median split, or binned SAH (same results, faster traversal)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -350,7 +351,8 @@ def main():
     torch.cuda.synchronize()
     prof = profile_read()
     ext_nodes, ext_tris, ext_rays, sh_nodes, sh_tris, sh_rays = (int(v) for v in prof[8:14])
-    trace_bytes = (ext_nodes + sh_nodes) * NODE_BYTES + (ext_tris + sh_tris) * TRI_BYTES \
+    node_bytes = WIDE_NODE_BYTES if args.bvh == "sah" else NODE_BYTES  # "sah" traces the 4-wide tree
+    trace_bytes = (ext_nodes + sh_nodes) * node_bytes + (ext_tris + sh_tris) * TRI_BYTES \
         + ext_rays * EXT_RAY_BYTES + sh_rays * SHADOW_RAY_BYTES
 
     # timed region: K steps, kernel durations from hipEvents on the render stream
@@ -462,7 +464,7 @@ def main():
                      "bytes_per_step": trace_bytes,
                      "model": {"ext_nodes": ext_nodes, "ext_triangles": ext_tris, "ext_rays": ext_rays,
                                "shadow_nodes": sh_nodes, "shadow_triangles": sh_tris, "shadow_rays": sh_rays,
-                               "node_bytes": NODE_BYTES, "tri_bytes": TRI_BYTES, "ext_ray_bytes": EXT_RAY_BYTES,
+                               "node_bytes": node_bytes, "tri_bytes": TRI_BYTES, "ext_ray_bytes": EXT_RAY_BYTES,
                                "shadow_ray_bytes": SHADOW_RAY_BYTES}},
         "cpu_baseline": cpu,
     }

@@ -128,6 +128,12 @@ typedef struct {
     int rootRef;                          /* 0 = internal node 0; < 0 = single leaf ~rootRef */
     float rootMin[3];
     float rootMax[3];
+    const PtFloat4* wideNodes;           /* 4-wide tree of the SAH build (NULL otherwise): 8 x float4 per
+                                           node, min x, min y, min z, max x, max y, max z of the four
+                                           children, then their references (int bits: >= 0 wide node,
+                                           < 0 leaf slot ~ref, INT_MIN none), one unused */
+    uint32_t wideNodeCount;
+    int wideRootRef;                      /* as rootRef, in the wide tree */
 } MeshAccelSceneGpu;
 
 /* ---- error reporting ------------------------------------------------------------------ */
@@ -445,8 +451,15 @@ bool ptSceneUpdateMaterials(PtScene* scene, const MaterialGpu* materials, uint32
 /* BVH build used by the next ptSceneBuild (no reference counterpart): 0 = the reference's
  * median split (default; node arrays equal the reference's), 1 = binned SAH over Morton-ordered
  * triangles (same device layout, geometry-dependent topology, depth <= 48; ptSceneReadBvh then
- * reports 0 nodes). Closest hits are the same except for exact ties between triangles. */
+ * reports 0 nodes) plus its 4-wide collapse, which the wavefront trace kernels traverse;
+ * 2 = the binned SAH tree alone. Closest hits are the same except for exact ties between
+ * triangles. */
 bool ptSceneSetBuildMode(PtScene* scene, int mode);
+
+/* The 4-wide tree of a scene built in mode 1: copies its nodes (8 x float4 each, layout of
+ * MeshAccelSceneGpu::wideNodes) into a host buffer of *count entries (nodes may be NULL to ask
+ * for the count) and reports the root reference. No reference counterpart. */
+bool ptSceneReadWide(const PtScene* scene, float* nodes, int* count, int* rootRef);
 
 /* Copies the device layout: internal nodes (nodeCount x 16 floats) and leaf slots
  * (triangleCount x 12 floats); NULL skips one. levels receives the SAH build's level count

@@ -24,7 +24,7 @@ TEXTURE_WORDS = 16         # TextureDescGpu, 64 bytes
 CAMERA_WORDS = 20          # CameraGpu, 80 bytes
 PARAMS_WORDS = 11          # RenderParamsGpu, 44 bytes
 BVH_NODE_WORDS = 12        # MeshBvhNode, 48 bytes
-SCENE_DESC_BYTES = 112     # MeshAccelSceneGpu
+SCENE_DESC_BYTES = 128     # MeshAccelSceneGpu
 
 
 class LibraryMissing(ImportError):
@@ -95,6 +95,7 @@ SIGNATURES = {
     "pathTracerWavefrontReserve": (_b, [_i, _i, _i]),
     "pathTracerDetachedRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "ptSceneReadLayout": (_b, [_p, _p, _p, _p]),
+    "ptSceneReadWide": (_b, [_p, _p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
     "ptEnvironmentDevice": (_p, [_p]),
     "ptEnvironmentReadTables": (_b, [_p, _p, _p]),
@@ -292,7 +293,7 @@ def pack_params(background=(10.0 / 255.0, 10.0 / 255.0, 10.0 / 255.0), environme
 class Scene:
     """A scene built on the GPU (ptSceneBuild): BVH, device layout, emissive list."""
 
-    BUILD_MODES = {"median": 0, "sah": 1}
+    BUILD_MODES = {"median": 0, "sah": 1, "sah-binary": 2}
 
     def __init__(self, host_triangles: np.ndarray, materials: np.ndarray, textures: np.ndarray | None = None,
                  stream: int = 0, bvh: str = "median"):
@@ -332,6 +333,16 @@ class Scene:
         check(lib().ptSceneReadLayout(self.handle, nodes.ctypes.data if nodes.size else None,
                                       slots.ctypes.data if slots.size else None, C.byref(levels)), "ptSceneReadLayout")
         return nodes, slots, levels.value
+
+    def read_wide(self) -> tuple[np.ndarray, int]:
+        """4-wide tree (build mode "sah"): nodes (count, 32) float32 and the root reference."""
+        count, root = C.c_int(0), C.c_int(0)
+        check(lib().ptSceneReadWide(self.handle, None, C.byref(count), C.byref(root)), "ptSceneReadWide")
+        nodes = np.zeros((count.value, 32), np.float32)
+        if count.value:
+            check(lib().ptSceneReadWide(self.handle, nodes.ctypes.data, C.byref(count), C.byref(root)),
+                  "ptSceneReadWide")
+        return nodes, root.value
 
     def description(self) -> bytes:
         buf = (C.c_uint8 * SCENE_DESC_BYTES)()

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdarg>
+#include <vector>
 
 // Records the message returned by pathTracerLastError() (thread-local).
 void ptSetError(const char* fmt, ...);
@@ -14,10 +15,18 @@ constexpr int kMaxSahDepth = 48;
 // Binned-SAH BVH over device triangles (25-word records): internal nodes (n - 1 x 64 B), leaf
 // slots (3 x float4 each) and shading records (4 x float4 per triangle). rootOut receives the root
 // box and the root reference (bits of an int in rootOut[6]); levelsOut the number of levels.
+// levelStarts (optional) receives the first internal node of every level.
 bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
-                      float rootOut[7], int* levelsOut);
+                      float rootOut[7], int* levelsOut, std::vector<int>* levelStarts);
+// 4-wide tree collapsed from a level-numbered binary tree of internalCount internal nodes
+// (pt_sah.hip): wideOut (device, 8 float4 per node) and its node count; the wide root is node 0
+// where the binary root is internal.
+bool pt_build_wide(const float4* nodes, int internalCount, const std::vector<int>& levelStarts, float4** wideOut,
+                   int* wideCount, hipStream_t stream);
 
 #include "pt_spectral.h"
+// Whether the scene description at d was built with a 4-wide tree (ptSceneBuild registers it).
+bool pt_scene_is_wide(const MeshAccelSceneGpu* d);
 // Spectral sampling state set through the C ABI (pt_spectral.hip): whether the launchers
 // render hero wavelengths, and the model (coefficient table, white normalisation, wavelength
 // range) the kernels then read.

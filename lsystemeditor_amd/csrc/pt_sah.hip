@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace {
 
@@ -509,8 +510,11 @@ void dfree(T*& p)
 } // namespace
 
 bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
-                      float rootOut[7], int* levelsOut)
+                      float rootOut[7], int* levelsOut, std::vector<int>* levelStarts)
 {
+    if (levelStarts != nullptr) {
+        levelStarts->clear();
+    }
     float4* bounds = nullptr;
     float4* cent = nullptr;
     uint32_t* gcb = nullptr;
@@ -605,6 +609,9 @@ bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, 
                 levelOk = false;
                 break;
             }
+            if (levelStarts != nullptr) {
+                levelStarts->push_back(levelBase);
+            }
             hipLaunchKernelGGL(emit_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, split, rank,
                                levelBase, reinterpret_cast<float*>(nodes), rootBox, segsNext);
             hipLaunchKernelGGL(flag_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, split, cent, n,
@@ -662,4 +669,138 @@ bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, 
         (void)hipFree(tmp);
     }
     return ok;
+}
+
+// ---- 4-wide tree -------------------------------------------------------------------------
+//
+// Collapses the level-numbered binary tree: every internal node on an even level becomes a wide
+// node holding up to four children, its grandchildren where its children are internal and its
+// children where they are leaves. The boxes are the binary tree's own floats, so a traversal's
+// box tests give the same entry distances. Wide node (8 float4, 7 used): min x / min y / min z /
+// max x / max y / max z of the four children, then their references (int bits: >= 0 wide node,
+// < 0 leaf slot ~ref, kWideEmpty unused).
+
+namespace {
+
+constexpr int kWideEmpty = static_cast<int>(0x80000000u);
+
+struct ChildBox {
+    float mn[3], mx[3];
+    int ref;
+};
+
+__device__ void binary_children(const float4* __restrict__ nodes, int node, ChildBox out[2])
+{
+    const float4 n0 = nodes[4 * node + 0];
+    const float4 n1 = nodes[4 * node + 1];
+    const float4 n2 = nodes[4 * node + 2];
+    const float4 n3 = nodes[4 * node + 3];
+    out[0] = ChildBox{{n0.x, n0.y, n0.z}, {n0.w, n1.x, n1.y}, __float_as_int(n3.x)};
+    out[1] = ChildBox{{n1.z, n1.w, n2.x}, {n2.y, n2.z, n2.w}, __float_as_int(n3.y)};
+}
+
+// levelStart[L]: first internal node of level L (L < levels; levelStart[levels] = internal count);
+// wideStart[L]: first wide node made from level L (even L).
+__global__ void wide_collapse_kernel(const float4* __restrict__ nodes, int internalCount,
+                                     const int* __restrict__ levelStart, const int* __restrict__ wideStart, int levels,
+                                     float4* __restrict__ wide)
+{
+    const int node = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (node >= internalCount) {
+        return;
+    }
+    int level = 0;
+    while (level + 1 < levels && levelStart[level + 1] <= node) {
+        ++level;
+    }
+    if ((level & 1) != 0) {
+        return;
+    }
+    ChildBox kids[2];
+    binary_children(nodes, node, kids);
+    ChildBox c[4];
+    int k = 0;
+    for (int side = 0; side < 2; ++side) {
+        if (kids[side].ref >= 0) {
+            ChildBox g[2];
+            binary_children(nodes, kids[side].ref, g);
+            for (int q = 0; q < 2; ++q) {
+                if (g[q].ref >= 0) {  // internal on level + 2: a wide node
+                    g[q].ref = wideStart[level + 2] + (g[q].ref - levelStart[level + 2]);
+                }
+                c[k++] = g[q];
+            }
+        } else {
+            c[k++] = kids[side];
+        }
+    }
+    for (; k < 4; ++k) {
+        c[k] = ChildBox{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, kWideEmpty};
+    }
+    const int w = wideStart[level] + (node - levelStart[level]);
+    float4* o = wide + 8 * static_cast<size_t>(w);
+    o[0] = make_float4(c[0].mn[0], c[1].mn[0], c[2].mn[0], c[3].mn[0]);
+    o[1] = make_float4(c[0].mn[1], c[1].mn[1], c[2].mn[1], c[3].mn[1]);
+    o[2] = make_float4(c[0].mn[2], c[1].mn[2], c[2].mn[2], c[3].mn[2]);
+    o[3] = make_float4(c[0].mx[0], c[1].mx[0], c[2].mx[0], c[3].mx[0]);
+    o[4] = make_float4(c[0].mx[1], c[1].mx[1], c[2].mx[1], c[3].mx[1]);
+    o[5] = make_float4(c[0].mx[2], c[1].mx[2], c[2].mx[2], c[3].mx[2]);
+    o[6] = make_float4(__int_as_float(c[0].ref), __int_as_float(c[1].ref), __int_as_float(c[2].ref),
+                       __int_as_float(c[3].ref));
+    o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+} // namespace
+
+bool pt_build_wide(const float4* nodes, int internalCount, const std::vector<int>& levelStarts, float4** wideOut,
+                   int* wideCount, hipStream_t stream)
+{
+    *wideOut = nullptr;
+    *wideCount = 0;
+    if (internalCount <= 0) {
+        return true;  // a single leaf (or nothing): no internal node to collapse
+    }
+    const int levels = static_cast<int>(levelStarts.size());
+    std::vector<int> starts(levelStarts);
+    starts.push_back(internalCount);
+    std::vector<int> wideStart(static_cast<size_t>(levels) + 1, 0);
+    int count = 0;
+    for (int L = 0; L < levels; ++L) {
+        wideStart[static_cast<size_t>(L)] = count;
+        if ((L & 1) == 0) {
+            count += starts[static_cast<size_t>(L) + 1] - starts[static_cast<size_t>(L)];
+        }
+    }
+    wideStart[static_cast<size_t>(levels)] = count;
+    int* dStarts = nullptr;
+    int* dWide = nullptr;
+    float4* wide = nullptr;
+    bool ok = hipMalloc(&dStarts, starts.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&dWide, wideStart.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&wide, 8 * static_cast<size_t>(count) * sizeof(float4)) == hipSuccess &&
+              hipMemcpyAsync(dStarts, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice, stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(dWide, wideStart.data(), wideStart.size() * sizeof(int), hipMemcpyHostToDevice,
+                             stream) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(wide_collapse_kernel, dim3(static_cast<unsigned>((internalCount + 255) / 256)), dim3(256), 0,
+                           stream, nodes, internalCount, dStarts, dWide, levels, wide);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
+    }
+    if (dStarts != nullptr) {
+        (void)hipFree(dStarts);
+    }
+    if (dWide != nullptr) {
+        (void)hipFree(dWide);
+    }
+    if (!ok) {
+        if (wide != nullptr) {
+            (void)hipFree(wide);
+        }
+        ptSetError("wide BVH collapse failed");
+        return false;
+    }
+    *wideOut = wide;
+    *wideCount = count;
+    return true;
 }

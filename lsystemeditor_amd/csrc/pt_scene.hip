@@ -26,6 +26,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -394,7 +396,31 @@ MaterialGpu default_material()
     return m;
 }
 
+// Scene descriptions (device addresses) that carry a 4-wide tree: the launchers run the wide
+// trace kernels for these.
+std::mutex gWideMutex;
+std::unordered_set<const MeshAccelSceneGpu*> gWideScenes;
+
+void wide_registry(const MeshAccelSceneGpu* d, bool add)
+{
+    if (d == nullptr) {
+        return;
+    }
+    std::lock_guard<std::mutex> lock(gWideMutex);
+    if (add) {
+        gWideScenes.insert(d);
+    } else {
+        gWideScenes.erase(d);
+    }
+}
+
 } // namespace
+
+bool pt_scene_is_wide(const MeshAccelSceneGpu* d)
+{
+    std::lock_guard<std::mutex> lock(gWideMutex);
+    return gWideScenes.count(d) != 0;
+}
 
 struct PtScene {
     MeshAccelSceneGpu desc{};
@@ -410,7 +436,9 @@ struct PtScene {
     MeshBvhNode* refNodes = nullptr;
     uint32_t refCount = 0;
     float buildMs = 0.0f;
-    int buildMode = 0; // 0: median split (reference topology), 1: binned SAH
+    int buildMode = 0; // 0: median split (reference topology), 1: binned SAH + 4-wide tree, 2: binned SAH
+    float4* wide = nullptr; // 4-wide tree (mode 1)
+    int wideCount = 0;
     int levels = 0;
     // host copies for ptSceneUpdateMaterials: materials, and per emissive entry its area and
     // material index
@@ -421,7 +449,11 @@ struct PtScene {
 
     void clear()
     {
+        wide_registry(d_desc, false);
         release(d_desc);
+        release(wide);
+        wide = nullptr;
+        wideCount = 0;
         release(nodes);
         release(triPos);
         release(triShade);
@@ -662,13 +694,18 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
              alloc(&sc->triShade, 4 * static_cast<size_t>(n)) && alloc(&sc->triPos, 3 * static_cast<size_t>(n)) &&
              alloc(&sc->nodes, 4 * static_cast<size_t>(n > 1 ? n - 1 : 1)) &&
              alloc(&sc->refNodes, 2 * static_cast<size_t>(n) - 1);
-        if (ok && sc->buildMode == 1) {
+        if (ok && (sc->buildMode == 1 || sc->buildMode == 2)) {
             hipEvent_t e0 = nullptr, e1 = nullptr;
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
             (void)hipEventRecord(e0, stream);
             float root[7] = {0};
-            ok = pt_build_bvh_sah(dTris, n, sc->nodes, sc->triPos, sc->triShade, stream, root, &sc->levels);
+            std::vector<int> levelStarts;
+            ok = pt_build_bvh_sah(dTris, n, sc->nodes, sc->triPos, sc->triShade, stream, root, &sc->levels,
+                                  &levelStarts);
+            if (ok && sc->buildMode == 1) {
+                ok = pt_build_wide(sc->nodes, n - 1, levelStarts, &sc->wide, &sc->wideCount, stream);
+            }
             (void)hipEventRecord(e1, stream);
             (void)hipEventSynchronize(e1);
             (void)hipEventElapsedTime(&sc->buildMs, e0, e1);
@@ -710,10 +747,47 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
     if (n == 0) {
         d.rootRef = 0;
     }
+    // the wide tree: node 0 where the binary root is internal, the same leaf otherwise
+    const bool wide = sc->buildMode == 1 && n > 0;
+    d.wideNodes = wide && sc->wideCount > 0 ? reinterpret_cast<const PtFloat4*>(sc->wide) : nullptr;
+    d.wideNodeCount = wide ? static_cast<uint32_t>(sc->wideCount) : 0u;
+    d.wideRootRef = wide ? (d.rootRef >= 0 ? 0 : d.rootRef) : 0;
     if (!upload(&sc->d_desc, &d, 1, stream) || hipStreamSynchronize(stream) != hipSuccess) {
         sc->clear();
         ptSetError("ptSceneBuild: scene upload failed");
         return false;
+    }
+    if (wide) {
+        wide_registry(sc->d_desc, true);
+    }
+    return true;
+}
+
+bool ptSceneReadWide(const PtScene* scene, float* nodes, int* count, int* rootRef)
+{
+    if (scene == nullptr || scene->d_desc == nullptr || count == nullptr) {
+        ptSetError("ptSceneReadWide: invalid arguments");
+        return false;
+    }
+    if (scene->buildMode != 1) {
+        ptSetError("ptSceneReadWide: the scene has no 4-wide tree (build mode 1)");
+        return false;
+    }
+    if (nodes != nullptr) {
+        if (*count < scene->wideCount) {
+            ptSetError("ptSceneReadWide: buffer of %d nodes for %d", *count, scene->wideCount);
+            return false;
+        }
+        if (scene->wideCount > 0 &&
+            hipMemcpy(nodes, scene->wide, static_cast<size_t>(scene->wideCount) * 8 * sizeof(float4),
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            ptSetError("ptSceneReadWide: copy failed");
+            return false;
+        }
+    }
+    *count = scene->wideCount;
+    if (rootRef != nullptr) {
+        *rootRef = scene->desc.wideRootRef;
     }
     return true;
 }
@@ -775,7 +849,7 @@ bool ptSceneUpdateMaterials(PtScene* sc, const MaterialGpu* materials, uint32_t 
 
 bool ptSceneSetBuildMode(PtScene* scene, int mode)
 {
-    if (scene == nullptr || mode < 0 || mode > 1) {
+    if (scene == nullptr || mode < 0 || mode > 2) {
         ptSetError("ptSceneSetBuildMode: invalid arguments");
         return false;
     }

@@ -30,7 +30,9 @@ namespace {
 
 constexpr int kWfBlock = 256;
 constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
-constexpr int kSpillEntries = 64;  // stack entries per thread in LDS ring + global spill (tree depth <= 48)
+// stack entries per thread in LDS ring + global spill: binary trees push one entry per level
+// (depth <= 48), 4-wide trees up to three (depth <= 24, so at most 72)
+constexpr int kSpillEntries = 96;
 constexpr int kFetchParts = 16;   // ray-queue partitions with a fetch counter each (trace kernels)
 constexpr int kFetchStride = 16;  // ints between fetch counters: one 64-byte line each
 // Per bounce: [0] ext, [1] shade, [2] shadow rays produced, [3] generated paths, then the
@@ -233,20 +235,42 @@ struct LdsStack {
 };
 
 constexpr int kDone = static_cast<int>(0x80000000u); // traversal finished (leaf refs are ~slot >= -2^28)
+constexpr float kInfDist = __builtin_huge_valf();     // a 4-wide child that was missed
 
 // What a trace kernel reads of a scene: the tree, the leaf triangles and the root.
 struct TraceScene {
     const PtFloat4* nodes;
     const PtFloat4* tris;
+    const PtFloat4* wide;
     uint32_t triCount;
     int rootRef;
+    int wideRoot;
     float rmin[3], rmax[3];
 };
 
 PT_DEV TraceScene trace_scene(const MeshAccelSceneGpu& m)
 {
-    return TraceScene{m.nodes, m.triPos, m.triangleCount, m.rootRef, {m.rootMin[0], m.rootMin[1], m.rootMin[2]},
+    return TraceScene{m.nodes,
+                      m.triPos,
+                      m.wideNodes,
+                      m.triangleCount,
+                      m.rootRef,
+                      m.wideRootRef,
+                      {m.rootMin[0], m.rootMin[1], m.rootMin[2]},
                       {m.rootMax[0], m.rootMax[1], m.rootMax[2]}};
+}
+
+// Orders two (entry distance, reference) pairs by distance.
+PT_DEV void order2(float& da, int& ra, float& db, int& rb)
+{
+    if (db < da) {
+        const float d = da;
+        const int r = ra;
+        da = db;
+        ra = rb;
+        db = d;
+        rb = r;
+    }
 }
 
 // Persistent traversal: every lane takes the next ray from its wave's claimed range of the queue as
@@ -260,7 +284,10 @@ PT_DEV TraceScene trace_scene(const MeshAccelSceneGpu& m)
 //   ANY_HIT = true:  shadow rays of shadowQ, result to occ.
 //   BATCH:           rays of a batch render; each ray traverses its own scene's tree (the scene
 //                    id of its path, or the one its shadow-queue entry carries).
-template <bool ANY_HIT, bool STATS, bool BATCH>
+//   WIDE:            traverse the scene's 4-wide tree: the four child boxes of a node are tested
+//                    at once (the binary tree's box floats, so the same entry distances), the
+//                    nearest hit child is visited next and the others are pushed farthest first.
+template <bool ANY_HIT, bool STATS, bool BATCH, bool WIDE>
 __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
                                                             PathState PS, const int* __restrict__ queue,
                                                             const int* __restrict__ countPtr, int* fetchCounters,
@@ -365,7 +392,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     float te;
                     cur = (s.triCount != 0u && slab(r, s.rmin[0], s.rmin[1], s.rmin[2], s.rmax[0], s.rmax[1],
                                                     s.rmax[2], tMin, kRayTMax, te))
-                        ? s.rootRef
+                        ? (WIDE ? s.wideRoot : s.rootRef)
                         : kDone;
                 } else {
                     exhausted = true;
@@ -409,6 +436,49 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     }
                 }
                 pop = !finished;
+            } else if constexpr (WIDE) {
+                const uint32_t base = 8u * static_cast<uint32_t>(cur);
+                const float4 mnx = ld4(s.wide, base + 0u);
+                const float4 mny = ld4(s.wide, base + 1u);
+                const float4 mnz = ld4(s.wide, base + 2u);
+                const float4 mxx = ld4(s.wide, base + 3u);
+                const float4 mxy = ld4(s.wide, base + 4u);
+                const float4 mxz = ld4(s.wide, base + 5u);
+                const float4 refs = ld4(s.wide, base + 6u);
+                if (STATS) {
+                    ts.nodes += 1u;
+                }
+                int r0 = __float_as_int(refs.x), r1 = __float_as_int(refs.y), r2 = __float_as_int(refs.z),
+                    r3 = __float_as_int(refs.w);
+                float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
+                float d0 = (r0 != kDone && slab(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, bestT, e0))
+                    ? e0 : kInfDist;
+                float d1 = (r1 != kDone && slab(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, bestT, e1))
+                    ? e1 : kInfDist;
+                float d2 = (r2 != kDone && slab(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, bestT, e2))
+                    ? e2 : kInfDist;
+                float d3 = (r3 != kDone && slab(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, bestT, e3))
+                    ? e3 : kInfDist;
+                // sorting network for four
+                order2(d0, r0, d1, r1);
+                order2(d2, r2, d3, r3);
+                order2(d0, r0, d2, r2);
+                order2(d1, r1, d3, r3);
+                order2(d1, r1, d2, r2);
+                if (d0 == kInfDist) {
+                    pop = true;
+                } else {
+                    if (d3 != kInfDist && st.sp < kSpillEntries) {
+                        st.push(r3, d3);
+                    }
+                    if (d2 != kInfDist && st.sp < kSpillEntries) {
+                        st.push(r2, d2);
+                    }
+                    if (d1 != kInfDist && st.sp < kSpillEntries) {
+                        st.push(r1, d1);
+                    }
+                    cur = r0;
+                }
             } else {
                 const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
                 const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
@@ -1565,14 +1635,32 @@ unsigned blocks_for(long long n) { return static_cast<unsigned>(std::max<long lo
 // trace of the shadow queue (from bounce 1 on) and the shade launch that fills the next queues.
 // Launch counts depend only on the host-known depth and path bound (no readbacks).
 // shade(b, shadeCount, cOut, traceBlocks) launches bounce b's shade kernel.
+// The trace kernel of a launch: counted (profiling) or not, batch or single scene, binary or
+// 4-wide tree.
+template <bool ANY_HIT>
+auto trace_kernel_for(bool counted, bool batch, bool wide)
+{
+    if (batch) {
+        return wide ? wf_trace_kernel<ANY_HIT, false, true, true> : wf_trace_kernel<ANY_HIT, false, true, false>;
+    }
+    if (counted) {
+        return wide ? wf_trace_kernel<ANY_HIT, true, false, true> : wf_trace_kernel<ANY_HIT, true, false, false>;
+    }
+    return wide ? wf_trace_kernel<ANY_HIT, false, false, true> : wf_trace_kernel<ANY_HIT, false, false, false>;
+}
+
 template <class ShadeLaunch>
 void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, int maxPathDepth,
-                         long long maxPaths, int persistentBlocks, ShadeLaunch shade,
-                         const BatchScene* tab = nullptr)
+                         long long maxPaths, int persistentBlocks, ShadeLaunch shade, const BatchScene* tab,
+                         bool wide)
 {
     const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
     const unsigned shadowBlocks =
         static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(2 * maxPaths)));
+    const bool counted = gProf.counted && tab == nullptr;
+    const auto extKernel = trace_kernel_for<false>(counted, tab != nullptr, wide);
+    const auto shadowKernel = trace_kernel_for<true>(counted, tab != nullptr, wide);
+    unsigned long long* stats = counted ? gProf.dStats : nullptr;
     for (int b = 0; b <= maxPathDepth; ++b) {
         int* cIn = P.counters + kCountersPerBounce * b;        // produced by bounce b-1 (or gen)
         int* extFetch = cIn + kFetchOffset;
@@ -1581,35 +1669,13 @@ void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* 
         const int* extQueue = b == 0 ? nullptr : P.extQ[b & 1];
         const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
         prof_mark(stream, 0, false, b);
-        if (tab != nullptr) {
-            hipLaunchKernelGGL((wf_trace_kernel<false, false, true>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, nullptr, P.hitA[b & 1],
-                               P.hitB[b & 1], tab);
-        } else if (gProf.counted) {
-            hipLaunchKernelGGL((wf_trace_kernel<false, true, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, gProf.dStats, P.hitA[b & 1],
-                               P.hitB[b & 1], nullptr);
-        } else {
-            hipLaunchKernelGGL((wf_trace_kernel<false, false, false>), dim3(traceBlocks), dim3(kWfBlock), 0, stream,
-                               d_scene, P, P.S[b & 1], extQueue, extCount, extFetch, nullptr, P.hitA[b & 1],
-                               P.hitB[b & 1], nullptr);
-        }
+        hipLaunchKernelGGL(extKernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, P, P.S[b & 1], extQueue,
+                           extCount, extFetch, stats, P.hitA[b & 1], P.hitB[b & 1], tab);
         prof_mark(stream, 0, true, b);
         if (b > 0) {
             prof_mark(stream, 1, false, b);
-            if (tab != nullptr) {
-                hipLaunchKernelGGL((wf_trace_kernel<true, false, true>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, nullptr, nullptr, nullptr,
-                                   tab);
-            } else if (gProf.counted) {
-                hipLaunchKernelGGL((wf_trace_kernel<true, true, false>), dim3(shadowBlocks), dim3(kWfBlock), 0, stream,
-                                   d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, gProf.dStats, nullptr,
-                                   nullptr, nullptr);
-            } else {
-                hipLaunchKernelGGL((wf_trace_kernel<true, false, false>), dim3(shadowBlocks), dim3(kWfBlock), 0,
-                                   stream, d_scene, P, P.S[b & 1], nullptr, &cIn[2], shadowFetch, nullptr, nullptr,
-                                   nullptr, nullptr);
-            }
+            hipLaunchKernelGGL(shadowKernel, dim3(shadowBlocks), dim3(kWfBlock), 0, stream, d_scene, P, P.S[b & 1],
+                               nullptr, &cIn[2], shadowFetch, stats, nullptr, nullptr, tab);
             prof_mark(stream, 1, true, b);
         }
         const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
@@ -1621,7 +1687,7 @@ void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* 
 
 void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, const RenderParamsGpu* d_params,
                     const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks,
-                    bool spec, const SpecModel& sm, const BatchScene* tab = nullptr)
+                    bool spec, const SpecModel& sm, const BatchScene* tab, bool wide)
 {
     const auto kernel = tab != nullptr ? (spec ? wf_shade_kernel<HeroColors, true> : wf_shade_kernel<RgbColors, true>)
                                        : (spec ? wf_shade_kernel<HeroColors, false> : wf_shade_kernel<RgbColors, false>);
@@ -1633,7 +1699,7 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
                                shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp, sm,
                                tab);
         },
-        tab);
+        tab, wide);
 }
 
 extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
@@ -1674,6 +1740,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
     const int persistentBlocks = cus * 8;
     const bool spec = pt_spectral_enabled();
     const SpecModel sm = pt_spectral_model();
+    const bool wide = pt_scene_is_wide(d_scene);
     const int spillThreads = persistentBlocks * kWfBlock;
     const int cap = std::max(4 * activeCount, std::min(target, 64 * activeCount));
     if (!ensure_workspace(cap, activeCount, maxPathDepth, spillThreads)) {
@@ -1714,7 +1781,8 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
                            d_samples, d_converged, d_camera, d_params, width, height, firstGlobalSeed, passBase,
                            passCount, group, P, spec, -1, nullptr, 0);
         prof_mark(stream, 3, true);
-        launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks, spec, sm);
+        launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, maxPaths, persistentBlocks, spec, sm, nullptr,
+                       wide);
         prof_mark(stream, 3, false);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(active)), dim3(kWfBlock), 0, stream, list, countCur, P,
                            reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params, nullptr, 0);
@@ -1863,9 +1931,10 @@ bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean
                        int width, int height, const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
                        const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params, int samplesPerPixel,
                        int maxPathDepth, int maxPathsInFlight, unsigned int firstSeed, hipStream_t stream,
-                       const BatchScene* tab = nullptr, int tabCount = 0, int batchPixels = 0)
+                       const BatchScene* tab = nullptr, int tabCount = 0, int batchPixels = 0, bool batchWide = false)
 {
     const bool single = tab == nullptr;
+    const bool wide = single ? pt_scene_is_wide(d_scene) : batchWide;
     if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
         d_converged == nullptr || samplesPerPixel <= 0 || maxPathDepth < 0 || maxPathDepth > 256 ||
         (single && (d_camera == nullptr || d_scene == nullptr || d_params == nullptr || width <= 0 || height <= 0)) ||
@@ -1915,7 +1984,7 @@ bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean
                            d_converged, d_camera, d_params, width, height, firstSeed, passBase, samplesPerPixel, g, P,
                            spec, -1, tab, tabCount);
         launch_bounces(stream, P, d_scene, d_params, d_env, maxPathDepth, static_cast<long long>(g) * pixels,
-                       persistentBlocks, spec, sm, tab);
+                       persistentBlocks, spec, sm, tab, wide);
         hipLaunchKernelGGL(wf_accumulate_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count, P,
                            reinterpret_cast<float4*>(d_buffer), d_samples, d_lumMean, d_m2, d_converged, d_params, tab,
                            tabCount);
@@ -2043,7 +2112,7 @@ __host__ __device__ inline int resolve_depth(int maxPathDepth)
 const float* adjoint_primal(int width, int height, int pixels, int spp, int maxDepth, const CameraGpu* d_camera,
                             const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
                             const RenderParamsGpu* d_params, unsigned int firstSeed, hipStream_t stream,
-                            const BatchScene* tab, int tabCount)
+                            const BatchScene* tab, int tabCount, bool batchWide)
 {
     const size_t nRad = static_cast<size_t>(pixels) * spp * 3;
     const size_t bytes = nRad * sizeof(float) + static_cast<size_t>(pixels) * (16 + 4 + 4 + 4 + 1) + 6 * 256;
@@ -2068,7 +2137,7 @@ const float* adjoint_primal(int width, int height, int pixels, int spp, int maxD
                     hipMemsetAsync(acc, 0, pixels * sizeof(PtFloat4), stream) == hipSuccess &&
                     render_fixed_impl(acc, cnt, mean, m2, cv, width, height, d_camera, d_scene, d_env, d_params, spp,
                                       maxDepth, std::max(1 << 24, gWs.capacity), firstSeed, stream, tab, tabCount,
-                                      pixels);
+                                      pixels, batchWide);
     gCapture = savedCapture;
     gCaptureSpp = savedSpp;
     return ok ? rad : nullptr;
@@ -2079,8 +2148,9 @@ const float* adjoint_primal(int width, int height, int pixels, int spp, int maxD
 bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels, int spp, int maxDepth,
                     const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
                     const RenderParamsGpu* d_params, int gradRows, const float* radiance, double* d_grad,
-                    unsigned int firstSeed, hipStream_t stream, const BatchScene* tab, int tabCount)
+                    unsigned int firstSeed, hipStream_t stream, const BatchScene* tab, int tabCount, bool batchWide)
 {
+    const bool wide = tab != nullptr ? batchWide : pt_scene_is_wide(d_scene);
     int device = 0, cus = 256;
     if (hipGetDevice(&device) == hipSuccess) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
@@ -2133,7 +2203,7 @@ bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels,
                                    maxDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
                                    P.hitA[(b + 1) & 1], cOut, perCopy / kGradParams, copies, gAd.partial, tab);
             },
-            tab);
+            tab, wide);
     }
     hipLaunchKernelGGL(wf_adjoint_reduce_kernel, dim3(static_cast<unsigned>(perCopy)), dim3(kWfBlock), 0, stream,
                        gAd.partial, persistentBlocks, perCopy, d_grad);
@@ -2163,13 +2233,13 @@ bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int s
     const float* radiance = d_sampleRadiance;
     if (radiance == nullptr) {
         radiance = adjoint_primal(width, height, pixels, spp, maxDepth, d_camera, d_scene, d_env, d_params, firstSeed,
-                                  stream, nullptr, 0);
+                                  stream, nullptr, 0, false);
         if (radiance == nullptr) {
             return false;
         }
     }
     return adjoint_replay(d_gradImage, width, height, pixels, spp, maxDepth, d_camera, d_scene, d_env, d_params,
-                          materialCount, radiance, d_grad, firstSeed, stream, nullptr, 0);
+                          materialCount, radiance, d_grad, firstSeed, stream, nullptr, 0, false);
 }
 
 // ---- batches of scenes --------------------------------------------------------------------
@@ -2219,6 +2289,7 @@ __global__ void wf_batch_expand_kernel(const BatchRef* __restrict__ refs, int co
 } // namespace
 
 struct PtBatch {
+    bool wide = true;  // every scene has a 4-wide tree
     int count = 0;
     int totalPixels = 0;
     int gradRows = 0;
@@ -2279,6 +2350,7 @@ extern "C" PtBatch* ptBatchCreate(const PtBatchSceneGpu* scenes, int sceneCount)
         }
         const int depth = resolve_depth(prm.maxPathDepth);
         b->maxDepth = std::max(b->maxDepth, depth);
+        b->wide = b->wide && desc.wideNodes != nullptr && pt_scene_is_wide(e.scene);
         refs[static_cast<size_t>(i)] = BatchRef{e.scene, e.camera, e.params, e.env, e.width, e.height,
                                                static_cast<int>(pixels), static_cast<int>(rows), 0};
         b->pixelOffsets.push_back(static_cast<int>(pixels));
@@ -2360,7 +2432,7 @@ extern "C" bool pathTracerRenderFixedBatch(PtFloat4* d_buffer, uint32_t* d_sampl
     }
     return render_fixed_impl(d_buffer, d_samples, d_lumMean, d_m2, d_converged, 0, 0, nullptr, nullptr, nullptr,
                              nullptr, samplesPerPixel, batch->maxDepth, maxPathsInFlight, firstSeed, stream,
-                             batch->tab, batch->count, batch->totalPixels);
+                             batch->tab, batch->count, batch->totalPixels, batch->wide);
 }
 
 extern "C" bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtBatch* batch, int samplesPerPixel,
@@ -2389,11 +2461,12 @@ extern "C" bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtB
         return false;
     }
     const float* radiance = adjoint_primal(0, 0, batch->totalPixels, samplesPerPixel, batch->maxDepth, nullptr,
-                                           nullptr, nullptr, nullptr, firstSeed, stream, batch->tab, batch->count);
+                                           nullptr, nullptr, nullptr, firstSeed, stream, batch->tab, batch->count,
+                                           batch->wide);
     if (radiance == nullptr) {
         return false;
     }
     return adjoint_replay(d_gradImage, 0, 0, batch->totalPixels, samplesPerPixel, batch->maxDepth, nullptr, nullptr,
                           nullptr, nullptr, batch->gradRows, radiance, d_grad, firstSeed, stream, batch->tab,
-                          batch->count);
+                          batch->count, batch->wide);
 }

@@ -113,3 +113,109 @@ def test_sah_render_matches_oracle(cuda, scheduler):
     else:
         render.render_wavefront(sc, dev, inputs)
     compare_accumulators(dev.host(), acc_o, f"sah/{scheduler}")
+
+
+def check_wide(tris: np.ndarray, wide: np.ndarray, slots: np.ndarray, root_ref: int, root_box):
+    """The 4-wide tree: every triangle in exactly one leaf, every wide node reached once, every
+    child box the exact union of the boxes below it (leaf boxes: the triangle's); returns depth."""
+    n = tris.shape[0]
+    v = tris[:, 0:9].reshape(n, 3, 3)
+    lo, hi = v.min(1), v.max(1)
+    orig = slots[:, 3].view(np.int32)
+    refs = wide[:, 24:28].view(np.int32)
+    seen_leaf = np.zeros(n, bool)
+    seen_node = np.zeros(wide.shape[0], bool)
+    empty = np.int32(-2 ** 31)
+    max_depth = 0
+    stack = [(root_ref, np.asarray(root_box, np.float32), 0)]
+    while stack:
+        ref, box, depth = stack.pop()
+        max_depth = max(max_depth, depth)
+        if ref < 0:
+            s = ~ref
+            assert not seen_leaf[s]
+            seen_leaf[s] = True
+            t = orig[s]
+            assert np.array_equal(box, np.concatenate([lo[t], hi[t]]).astype(np.float32))
+            continue
+        assert not seen_node[ref], ref
+        seen_node[ref] = True
+        kids = [c for c in range(4) if refs[ref, c] != empty]
+        assert len(kids) >= 2, f"wide node {ref} has {len(kids)} children"
+        assert all(refs[ref, c] == empty for c in range(len(kids), 4)), "children first, then empty slots"
+        cb = [np.array([wide[ref, 0 + c], wide[ref, 4 + c], wide[ref, 8 + c], wide[ref, 12 + c], wide[ref, 16 + c],
+                        wide[ref, 20 + c]], np.float32) for c in kids]
+        union = np.concatenate([np.min([b[0:3] for b in cb], 0), np.max([b[3:6] for b in cb], 0)])
+        assert np.array_equal(union, box), f"wide node {ref}: box is not the union of its children"
+        for c, b in zip(kids, cb):
+            stack.append((int(refs[ref, c]), b, depth + 1))
+    assert seen_leaf.all() and seen_node.all()
+    return max_depth
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 64, 1000, 4097])
+def test_wide_tree_is_valid(cuda, n):
+    tris = random_soup(n, 300 + n)
+    sc = abi.Scene(tris, abi.pack_material()[None], bvh="sah")
+    nodes, slots, levels = sc.read_layout()
+    wide, wroot = sc.read_wide()
+    desc = np.frombuffer(sc.description(), np.uint8)
+    root_ref = int(np.frombuffer(desc[84:88].tobytes(), np.int32)[0])
+    root_box = np.frombuffer(desc[88:112].tobytes(), np.float32)
+    assert int(np.frombuffer(desc[124:128].tobytes(), np.int32)[0]) == wroot
+    assert int(np.frombuffer(desc[120:124].tobytes(), np.uint32)[0]) == wide.shape[0]
+    if n == 1:
+        assert wide.shape[0] == 0 and wroot == root_ref == -1
+        return
+    assert wroot == 0
+    depth = check_wide(tris, wide, slots, wroot, root_box)
+    assert depth <= (levels + 1) // 2 + 1
+
+
+def test_wide_tree_c0(cuda):
+    s = scenes.config0()
+    sc = abi.Scene(s.triangles, s.materials, bvh="sah")
+    _, slots, levels = sc.read_layout()
+    wide, wroot = sc.read_wide()
+    desc = np.frombuffer(sc.description(), np.uint8)
+    root_box = np.frombuffer(desc[88:112].tobytes(), np.float32)
+    assert check_wide(s.triangles, wide, slots, wroot, root_box) <= 25
+
+
+@pytest.mark.parametrize("kind", ["zoo", "plain"])
+def test_wide_wavefront_render_equals_binary_and_median(cuda, kind):
+    """The wavefront render on the 4-wide tree equals the binary SAH tree's and the median tree's,
+    bit for bit (fixed and adaptive schedules)."""
+    from lsystemeditor_amd import render
+    w, h = 48, 40
+    tris, mats, tex, cam, params = dense_plant(w, h, spp=12, min_spp=4, threshold=0.05, materials=kind)
+    out = {}
+    for bvh in ("sah", "sah-binary", "median"):
+        sc = abi.Scene(tris, mats, tex, bvh=bvh)
+        inputs = render.DeviceInputs(cam, params)
+        a = render.Accumulator.allocate(w, h)
+        render.render_wavefront(sc, a, inputs)
+        f = render.Accumulator.allocate(w, h)
+        render.render_fixed(sc, f, inputs, 8)
+        out[bvh] = (a.host(), f.host())
+    for bvh in ("sah-binary", "median"):
+        for i in range(2):
+            for k in ("acc", "counts", "mean", "m2", "conv"):
+                g, wnt = np.asarray(out["sah"][i][k]), np.asarray(out[bvh][i][k])
+                assert np.array_equal(g.view(np.uint8), wnt.view(np.uint8)), (bvh, i, k)
+
+
+def test_wide_batch_render_equals_binary(cuda):
+    """A batch of C4-type bushes on their 4-wide trees equals the same batch on binary trees."""
+    from lsystemeditor_amd import inverse, render
+    specs = inverse.c4_specs(count=3, res=48, spp=4, iterations=(3, 5))
+    res = {}
+    for bvh in ("sah", "sah-binary"):
+        scs = [abi.Scene(s.triangles, s.materials, bvh=bvh) for s in specs]
+        inps = [render.DeviceInputs(s.camera, s.params) for s in specs]
+        b = render.SceneBatch(scs, inps, [(s.width, s.height) for s in specs])
+        b.render_fixed(4)
+        res[bvh] = b.acc.host()
+        b.close()
+    for k in ("acc", "counts", "mean", "m2", "conv"):
+        assert np.array_equal(np.asarray(res["sah"][k]).view(np.uint8), np.asarray(res["sah-binary"][k]).view(np.uint8)), k
