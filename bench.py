@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=["c1", "c0"], default="c1")
+    ap.add_argument("--max-paths", type=int, default=0,
+                    help="paths in flight per wavefront group of the C1 render (0 = the library default)")
     ap.add_argument("--bvh", choices=["median", "sah", "sah-binary"], default="sah",
                     help="BVH build: the reference's median split, or binned SAH (same results, faster traversal)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
@@ -339,7 +341,7 @@ def main():
     L = abi.lib()
 
     def step():
-        render.render_wavefront(scene, acc, inputs, passes=passes)
+        render.render_wavefront(scene, acc, inputs, passes=passes, max_paths=args.max_paths)
 
     for _ in range(args.warmup):
         step()

@@ -254,10 +254,10 @@ bool pathTracerAdaptiveRenderPassesCounted(PtFloat4* d_buffer, uint32_t* d_sampl
  * samples of all active pixels are traced together, one bounce per launch: closest-hit and
  * any-hit traversal kernels with an LDS stack, and a shading kernel that appends the next rays
  * to queues. maxPathDepth is the depth the params resolve to (maxPathDepth > 0 ? that : 0 -> 256;
- * PathIntegratorRandCore.h:219-227). maxPathsInFlight bounds the paths of one group (0 = 16M;
+ * PathIntegratorRandCore.h:219-227). maxPathsInFlight bounds the paths of one group (0 = 64M;
  * at least 4 per active pixel are always allowed). The first group takes every sample that no
  * pixel can skip (up to minSamples rounded up to 4); later groups take 4 samples per active
- * pixel, widened speculatively up to 64 (samples drawn after a pixel converged are discarded,
+ * pixel, widened speculatively up to 256 (samples drawn after a pixel converged are discarded,
  * so results do not depend on the grouping).
  * Uses a library-owned device workspace (grown on demand); calls on one device must not overlap.
  * The caller's active list is read, not modified. Synchronises the stream once per group. */
@@ -271,7 +271,7 @@ bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, 
 /* Fixed-spp wavefront render with no host synchronisation, so it can be captured into a HIP
  * graph: every pixel takes samplesPerPixel samples (sample index k, seed k + 1, as the adaptive
  * schedule with minSamples = maxSamplesPerPixel = samplesPerPixel), in groups of up to
- * maxPathsInFlight paths (0 = 16M), accumulated into the cleared accumulator as the reference
+ * maxPathsInFlight paths (0 = 64M), accumulated into the cleared accumulator as the reference
  * does. While a stream is capturing, the workspace is not allocated: reserve it first with
  * pathTracerWavefrontReserve. Calls on one device must not overlap (shared workspace).
  * No reference counterpart. */

@@ -66,7 +66,7 @@ class BatchRenderer:
     def _reserve(self):
         depth = max(render.resolved_depth(i.params_host) for i in self.inputs)
         pixels = max(s.width * s.height for s in self.specs)
-        paths = self.max_paths if self.max_paths > 0 else (1 << 24)
+        paths = self.max_paths if self.max_paths > 0 else (1 << 26)
         paths = max(paths, pixels)
         abi.check(abi.lib().pathTracerWavefrontReserve(paths, pixels, depth), "pathTracerWavefrontReserve")
 
@@ -130,7 +130,7 @@ class BatchRenderer:
                 render.render_wavefront(sc, acc, inp, passes=spec.spp, stream=self.stream)
 
 
-def autotune_paths(spec: scenes.SceneSpec, candidates=(1 << 21, 1 << 22, 1 << 23, 1 << 24), bvh: str = "sah",
+def autotune_paths(spec: scenes.SceneSpec, candidates=(1 << 22, 1 << 24, 1 << 25, 1 << 26), bvh: str = "sah",
                    reps: int = 2) -> tuple[int, dict]:
     """Times the fixed-spp render of one scene for each paths-in-flight candidate; returns the
     fastest and all timings (ms)."""

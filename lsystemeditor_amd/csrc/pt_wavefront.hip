@@ -46,6 +46,10 @@ constexpr int kRefillLanes = 32;   // idle lanes that trigger a wave's ray refil
 // leaving enough entries unclaimed for the waves to balance uneven ray costs.
 constexpr int kFetchChunk = 64;
 constexpr int kMaxBatchScenes = 4096;   // scenes per batch
+// Paths in flight per group when the caller gives no bound: 64M paths of state (about 27 GB) are
+// a tenth of the device's HBM, and groups this wide keep the number of groups (each paying the
+// latency tail of its deep bounces) small.
+constexpr int kDefaultPathsInFlight = 1 << 26;
 constexpr int kMaxBatchGradRows = 512;  // material rows of a batch adjoint (LDS sums)
 
 enum : uint32_t {
@@ -1719,7 +1723,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
     }
     // maxPathDepth must equal the depth the params resolve to (PathIntegratorRandCore.h:219-227);
     // it is passed by value because the params live in device memory.
-    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 24);
+    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : kDefaultPathsInFlight;
     // Samples every active pixel is sure to take: convergence is only tested at counts that are
     // multiples of 4 and at least minSamples, or reached at maxSamplesPerPixel. A first group of
     // that size starts no speculative sample.
@@ -1761,8 +1765,10 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
     int passBase = 0;
     const int counterInts = kCountersPerBounce * (maxPathDepth + 2);
     while (active > 0 && passBase < passCount) {
-        // group size: 4 samples per pixel (one convergence interval), widened speculatively
-        // while few pixels remain so the GPU stays full
+        // group size: 4 samples per pixel (one convergence interval), widened speculatively up
+        // to 256 while the group fits the paths-in-flight bound: pixels of an adaptive render
+        // mostly converge at minSamples or run to maxSamplesPerPixel, so few speculative samples
+        // are discarded, and every group saved is one set of deep-bounce launch tails saved
         int group = 0;
         if (passBase < safeEnd) {
             group = safeEnd - passBase;
@@ -1770,7 +1776,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
             group = std::min(group, fit >= 4 ? fit & ~3 : fit);
         } else {
             const int budget = gSpeculativeBudget > 0 ? std::min(gWs.capacity, gSpeculativeBudget) : gWs.capacity;
-            group = 4 * std::max(1, std::min(16, budget / std::max(1, 4 * active)));
+            group = 4 * std::max(1, std::min(64, budget / std::max(1, 4 * active)));
         }
         group = std::min(group, passCount - passBase);
         const long long maxPaths = static_cast<long long>(group) * active;
@@ -1947,7 +1953,7 @@ bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean
     (void)hipStreamIsCapturing(stream, &capture);
     const bool capturing = capture == hipStreamCaptureStatusActive;
     const int pixels = single ? width * height : batchPixels;
-    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : (1 << 24);
+    const int target = maxPathsInFlight > 0 ? maxPathsInFlight : kDefaultPathsInFlight;
     int device = 0, cus = 256;
     if (hipGetDevice(&device) == hipSuccess) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);

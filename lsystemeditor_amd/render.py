@@ -283,7 +283,7 @@ class SceneBatch:
 
     def reserve(self, max_paths: int = 0) -> None:
         """Sizes the shared workspace (needed before graph capture)."""
-        paths = max(max_paths if max_paths > 0 else (1 << 24), self.total_pixels)
+        paths = max(max_paths if max_paths > 0 else (1 << 26), self.total_pixels)
         abi.check(abi.lib().pathTracerWavefrontReserve(paths, self.total_pixels, self.max_depth),
                   "pathTracerWavefrontReserve")
 
