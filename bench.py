@@ -76,8 +76,8 @@ def read_traffic(name: str):
 
 def cpu_baseline(spec, threads: int):
     """The oracle (CPU restatement, C with pthreads) on a bounded sample of the same workload:
-    a centred 512x512 window of the full-resolution frame for its first 32 passes (about ten
-    seconds of CPU work)."""
+    a centred 512x512 window of the full-resolution frame for its first 64 passes (about ten
+    seconds of CPU work on 16 host threads)."""
     import ctypes as C
     from lsystemeditor_amd import abi
     from oracle import oracle as O
@@ -93,7 +93,7 @@ def cpu_baseline(spec, threads: int):
     acc.clear()
     env = O.Env()
     L = O.lib()
-    passes = 32
+    passes = 64
     t0 = time.perf_counter()
     for p in range(passes):
         L.orc_adaptive_sample(O.ptr(acc.acc), O.ptr(acc.counts), O.ptr(acc.mean), O.ptr(acc.m2), O.ptr(acc.conv),
