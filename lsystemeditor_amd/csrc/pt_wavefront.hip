@@ -1379,7 +1379,22 @@ __global__ void wf_zero_f64_kernel(double* p, long long n)
     }
 }
 
-// Applies each active pixel's group samples in order (PathTracer.cu:233-272).
+// Applies each active pixel's group samples in order (PathTracer.cu:233-272). The samples of the
+// 64 pixels of a wave occupy one contiguous slot range (the generation kernel's wave-level
+// scan), which the wave streams through LDS in chunks with coalesced reads; each lane then runs
+// its pixel's Welford updates in sample order from LDS. (Reading each pixel's samples straight
+// from global memory puts 64 far-apart streams per wave in flight, and the L2 evicts them
+// between consecutive samples.)
+constexpr int kAccChunk = 512;                          // samples per wave and chunk
+constexpr int kAccSkew = kAccChunk + kAccChunk / 32;    // one skip word per 32 against bank conflicts
+
+PT_DEV void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(kWfBlock) void wf_accumulate_kernel(const int* __restrict__ active,
                                                                  const int* __restrict__ activeCountPtr, Paths P,
                                                                  float4* __restrict__ acc, uint32_t* __restrict__ counts,
@@ -1388,49 +1403,82 @@ __global__ __launch_bounds__(kWfBlock) void wf_accumulate_kernel(const int* __re
                                                                  const RenderParamsGpu* params,
                                                                  const BatchScene* __restrict__ tab, int tabCount)
 {
+    __shared__ float stage[kWfBlock / 64][3][kAccSkew];
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= *activeCountPtr) {
-        return;
+    const int n = *activeCountPtr;
+    const int waveFirst = t - lane_id();
+    if (waveFirst >= n) {
+        return;  // the whole wave is past the list
     }
-    const int k = P.pxK[t];
-    if (k == 0) {
-        return;
-    }
-    const int idx = active[t];
-    const int base = P.pxBase[t];
-    const RenderParamsGpu prm = tab != nullptr ? tab[batch_scene_of(tab, tabCount, idx)].params : *params;
-    uint32_t n = counts[idx];
-    float4 a = acc[idx];
-    float mean = lumMean[idx];
-    float mm = m2[idx];
+    float* sx = stage[threadIdx.x >> 6][0];
+    float* sy = stage[threadIdx.x >> 6][1];
+    float* sz = stage[threadIdx.x >> 6][2];
+    const bool valid = t < n;
+    const int k = valid ? P.pxK[t] : 0;
+    const int base = valid ? P.pxBase[t] : 0;
+    const int lastLane = min(63, n - 1 - waveFirst);
+    const int w0 = __shfl(base, 0, 64);
+    const int w1 = __shfl(base + k, lastLane, 64);
+    const bool mine = valid && k > 0;
+    int idx = 0;
+    RenderParamsGpu prm{};
+    uint32_t cnt = 0u;
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float mean = 0.0f, mm = 0.0f;
     bool conv = false;
-    for (int j = 0; j < k && !conv; ++j) {
-        const int p = base + j;
-        const V3 r = mk(P.rx[p], P.ry[p], P.rz[p]);
-        const float sl = r.x * 0.2126f + r.y * 0.7152f + r.z * 0.0722f;
-        const float delta = sl - mean;
-        const float nm = mean + delta / static_cast<float>(n + 1u);
-        const float nM2 = mm + delta * (sl - nm);
-        mean = nm;
-        mm = nM2;
-        const float inv = 1.0f / static_cast<float>(n + 1u);
-        a = make_float4((a.x * static_cast<float>(n) + r.x) * inv, (a.y * static_cast<float>(n) + r.y) * inv,
-                        (a.z * static_cast<float>(n) + r.z) * inv, 1.0f);
-        const uint32_t nc = n + 1u;
-        n = nc;
-        if (prm.maxSamplesPerPixel > 0 && nc >= static_cast<uint32_t>(prm.maxSamplesPerPixel)) {
-            conv = true;
-        } else if ((nc % 4u) == 0u && nc >= static_cast<uint32_t>(prm.minSamples) && nm >= 1.0e-5f) {
-            const float sd = sqrtf(nM2 / static_cast<float>(nc));
-            conv = (sd / fmaxf(nm, 1.0e-4f)) < prm.relativeErrorThreshold;
+    if (mine) {
+        idx = active[t];
+        prm = tab != nullptr ? tab[batch_scene_of(tab, tabCount, idx)].params : *params;
+        cnt = counts[idx];
+        a = acc[idx];
+        mean = lumMean[idx];
+        mm = m2[idx];
+    }
+    for (int c0 = w0; c0 < w1; c0 += kAccChunk) {
+        const int len = min(kAccChunk, w1 - c0);
+        wave_sync();  // the previous chunk has been read
+        for (int e = lane_id(); e < len; e += 64) {
+            const int q = e + (e >> 5);
+            sx[q] = P.rx[c0 + e];
+            sy[q] = P.ry[c0 + e];
+            sz[q] = P.rz[c0 + e];
+        }
+        wave_sync();
+        if (mine && !conv) {
+            const int lo = max(base, c0);
+            const int hi = min(base + k, c0 + len);
+            for (int p = lo; p < hi && !conv; ++p) {
+                const int e = p - c0;
+                const int q = e + (e >> 5);
+                const V3 r = mk(sx[q], sy[q], sz[q]);
+                const float sl = r.x * 0.2126f + r.y * 0.7152f + r.z * 0.0722f;
+                const float delta = sl - mean;
+                const float nm = mean + delta / static_cast<float>(cnt + 1u);
+                const float nM2 = mm + delta * (sl - nm);
+                mean = nm;
+                mm = nM2;
+                const float inv = 1.0f / static_cast<float>(cnt + 1u);
+                a = make_float4((a.x * static_cast<float>(cnt) + r.x) * inv, (a.y * static_cast<float>(cnt) + r.y) * inv,
+                                (a.z * static_cast<float>(cnt) + r.z) * inv, 1.0f);
+                const uint32_t nc = cnt + 1u;
+                cnt = nc;
+                if (prm.maxSamplesPerPixel > 0 && nc >= static_cast<uint32_t>(prm.maxSamplesPerPixel)) {
+                    conv = true;
+                } else if ((nc % 4u) == 0u && nc >= static_cast<uint32_t>(prm.minSamples) && nm >= 1.0e-5f) {
+                    const float sd = sqrtf(nM2 / static_cast<float>(nc));
+                    conv = (sd / fmaxf(nm, 1.0e-4f)) < prm.relativeErrorThreshold;
+                }
+            }
         }
     }
-    acc[idx] = a;
-    counts[idx] = n;
-    lumMean[idx] = mean;
-    m2[idx] = mm;
-    if (conv) {
-        converged[idx] = 1u;
+    if (mine) {
+        acc[idx] = a;
+        counts[idx] = cnt;
+        lumMean[idx] = mean;
+        m2[idx] = mm;
+        if (conv) {
+            converged[idx] = 1u;
+        }
     }
 }
 
