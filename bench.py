@@ -460,7 +460,10 @@ def main():
                    "samples_started_per_step": started // args.steps, **extra},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"wf_trace_{spec.name.lower()}_traffic.txt"),
-                     "kernel": "wf_trace_ext_kernel + wf_trace_shadow_kernel",
+                     "kernel": ("wf_trace_kernel<false,false,false,W> (extension rays) + "
+                                "wf_trace_kernel<true,false,false,W> (shadow rays), W = 4-wide tree"
+                                if args.bvh == "sah" else
+                                "wf_trace_kernel<false,false,false,false> + wf_trace_kernel<true,false,false,false>"),
                      "kernel_ms_per_step": trace_ms_per_step,
                      "launches_per_step": (n_ext + n_sh) / args.steps,
                      "bytes_per_step": trace_bytes,
