@@ -319,8 +319,8 @@ def profile_read():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
-                                   "not an implemented path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the renderer runs on one GPU only; sharding across GPUs is described, "
+                                   "not implemented", "n_gpus": args.gpus}))
         return 2
     import torch
 
