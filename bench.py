@@ -301,8 +301,8 @@ def spectral_bench(scene, spec, inputs, acc, passes: int, bvh: str) -> dict:
 
 def inverse_bench(bvh: str, steps: int) -> dict:
     """C4: 32 bushes (iterations 3-6, 256x256, 16 spp, depth 8), 200 Adam steps (lr 1e-2) through
-    the adjoint on base colour, roughness and metallic of bark, leaf and ground; FD spot checks
-    every 50 steps."""
+    the adjoint on 16-bin base-colour spectra, roughness and metallic of bark, leaf and ground; FD
+    spot checks every 50 steps."""
     from lsystemeditor_amd import inverse
     t0 = time.perf_counter()
     specs = inverse.c4_specs()
@@ -313,13 +313,14 @@ def inverse_bench(bvh: str, steps: int) -> dict:
     res = inverse.optimise(prob, steps=steps, lr=1e-2, check_every=50,
                            log=lambda m: print(f"[bench] C4 {m}", file=sys.stderr, flush=True))
     out = {"workload": "C4: 32 bushes, seeds 0-31, iterations 3-6, 256x256, 16 spp, depth 8; Adam lr 1e-2 on "
-                       "base rgb, roughness, metallic of bark, leaf, ground (targets: seeded +-20% perturbation)",
-           "scenes": len(specs), "params": int(prob.mask.sum()) * len(specs),
+                       "16-bin base-colour spectra, roughness, metallic of bark, leaf, ground (targets: seeded "
+                       "+-20% perturbation)",
+           "scenes": len(specs), "params": prob.parameter_count(),
            "triangles": int(sum(s.triangles.shape[0] for s in specs)),
            "scene_generation_s": round(gen_s, 2), "setup_s": round(setup_s, 2), **res}
     e0, e1 = res["param_mae_initial"], res["param_mae_final"]
     print(f"[bench] C4 {steps} steps in {res['wall_s']:.1f} s ({res['grad_samples_per_s'] / 1e6:.1f} grad-Msamples/s), "
-          f"parameter error {e0['all']:.4f} -> {e1['all']:.4f} (base colour {e0['base_color']:.4f} -> "
+          f"parameter error {e0['all']:.4f} -> {e1['all']:.4f} (projected base colour {e0['base_color']:.4f} -> "
           f"{e1['base_color']:.4f}), step loss {res['loss_first_steps']:.4g} -> {res['loss_last_steps']:.4g}",
           file=sys.stderr, flush=True)
     del prob

@@ -2,17 +2,24 @@
 target images with Adam through the adjoint kernel.
 
 Each scene is a stochastic bush of the C1 generator (seed s, iterations drawn from 3..6, 256x256,
-16 spp, max depth 8) whose bark, leaf and ground groups carry a seeded target material set (every
-optimised parameter scaled by an independent factor in [0.8, 1.2]). The target image is the
-fixed-spp render at the target parameters; the optimisation starts from the unperturbed
-parameters and minimises the summed L2 image loss over all scenes with Adam (lr 1e-2) on base
-colour, roughness and metallic. The target is rendered once (seed base 0); every step renders
-the loss image with a fresh sample set and replays another fresh, independent set in the
-adjoint, so each step's gradient is an unbiased estimate of the gradient of the expected-image
-loss |E[image](theta) - target|^2. With the same samples for the residual and the derivatives
-the gradient would carry the covariance of image noise and its derivative, and with fixed
-sample sets the sampling error would be a fixed function of the parameters: at 16 spp either
-drives poorly determined parameters away from the target.
+16 spp, max depth 8) with the C2 materials. The optimised parameters of its bark, leaf and ground
+groups are, by default, a 16-bin base-colour spectrum, roughness and metallic (binned.py: the
+bins enter the renderer through their linear-sRGB projection); with binned_colour=False, the RGB
+base colour, roughness and metallic. Specular and the lights stay fixed. The target parameters are
+a seeded perturbation (every optimised value scaled by an independent factor in [0.8, 1.2]); the
+target image is the fixed-spp render at them. The optimisation starts from the unperturbed
+parameters and minimises the summed L2 image loss over all scenes with Adam (lr 1e-2).
+
+The target is rendered once (seed base 0); every step renders the loss image with a fresh sample
+set and replays another fresh, independent set in the adjoint, so each step's gradient is an
+unbiased estimate of the gradient of the expected-image loss |E[image](theta) - target|^2. With
+the same samples for the residual and the derivatives the gradient would carry the covariance of
+image noise and its derivative, and with fixed sample sets the sampling error would be a fixed
+function of the parameters: at 16 spp either drives poorly determined parameters away from the
+target. All scenes render in one batch wavefront and differentiate in one batch adjoint.
+
+RGB light transport sees a binned spectrum only through its projection (3 of its 16 degrees of
+freedom), so the bins' error is reported beside the error of their projected colour.
 """
 from __future__ import annotations
 
@@ -21,10 +28,11 @@ import time
 import numpy as np
 import torch
 
-from . import abi, diff, scenes
+from . import abi, binned, diff, scenes
 
 OPT_PARAMS = (0, 1, 2, 3, 4)          # base r, g, b, roughness, metallic (diff.PARAM_NAMES order)
-OPT_GROUPS = (0, 1, 2)                # bark, leaf, ground; the two lights stay fixed
+OPT_GROUPS = (0, 1, 2)                # bark, leaf, ground
+LIGHT_GROUPS = (3, 4)                 # the two lights stay fixed
 
 
 def c4_specs(count: int = 32, res: int = 256, spp: int = 16, depth: int = 8, iterations=(3, 6),
@@ -46,7 +54,7 @@ def c4_specs(count: int = 32, res: int = 256, spp: int = 16, depth: int = 8, ite
 
 
 def target_theta(theta0: np.ndarray, seed: int, spread: float = 0.2) -> np.ndarray:
-    """The unperturbed parameters with the optimised ones replaced by their seeded perturbation."""
+    """The unperturbed parameters with the optimised RGB ones replaced by their seeded perturbation."""
     pert = scenes.perturbed_target(theta0, seed=seed, spread=spread)
     out = theta0.copy()
     for gidx in OPT_GROUPS:
@@ -54,68 +62,153 @@ def target_theta(theta0: np.ndarray, seed: int, spread: float = 0.2) -> np.ndarr
     return out
 
 
-class InverseProblem:
-    """Scenes, renderers, target images and the parameters being optimised."""
+class _RgbParams:
+    """One scene's (materials, 7) parameter table, optimised on OPT_GROUPS x OPT_PARAMS."""
 
-    def __init__(self, specs: list[scenes.SceneSpec], bvh: str = "sah"):
-        self.specs = specs
-        self.renderers = []
-        self.targets = []
-        self.theta_true = []
-        self.theta = []
-        mask = torch.zeros((specs[0].materials.shape[0], len(diff.PARAM_NAMES)))
+    def __init__(self, theta0: np.ndarray, seed: int):
+        self.leaf = torch.from_numpy(theta0.copy()).requires_grad_(True)
+        self.true = torch.from_numpy(target_theta(theta0, seed))
+        mask = torch.zeros(theta0.shape)
         for gidx in OPT_GROUPS:
             mask[gidx, list(OPT_PARAMS)] = 1.0
         self.mask = mask
+
+    def leaves(self) -> list[torch.Tensor]:
+        return [self.leaf]
+
+    def theta(self) -> torch.Tensor:
+        return self.leaf
+
+    def theta_true(self) -> torch.Tensor:
+        return self.true
+
+    def mask_grads_(self) -> None:
+        self.leaf.grad.mul_(self.mask)
+
+    def clamp_(self) -> None:
+        with torch.no_grad():
+            self.leaf[:, 0:3].clamp_(0.0, 1.0)
+            self.leaf[:, 3:5].clamp_(0.01, 0.99)
+
+    def errors(self) -> dict[str, torch.Tensor]:
+        d = (self.leaf.detach() - self.true).abs()[list(OPT_GROUPS)]
+        return {"base_color": d[:, 0:3].reshape(-1), "roughness": d[:, 3], "metallic": d[:, 4]}
+
+
+class _BinnedParams:
+    """One scene's 16-bin base-colour spectra, roughness and metallic (binned.BinnedParams; specular
+    and the lights' emission bins stay fixed)."""
+
+    def __init__(self, theta0: np.ndarray, seed: int):
+        self.bp = binned.BinnedParams(torch.from_numpy(theta0), lights=list(LIGHT_GROUPS))
+        assert self.bp.surfaces == list(OPT_GROUPS)
+        q = self.bp.perturbed(seed=0xC4B00000 + seed)
+        q[1][:, 2] = self.bp.scal.detach()[:, 2]    # specular fixed
+        q[2] = self.bp.emis.detach().clone()        # emission fixed
+        self.true = q
+        self.m = torch.from_numpy(binned.projection()).float()
+
+    def leaves(self) -> list[torch.Tensor]:
+        return [self.bp.refl, self.bp.scal]
+
+    def theta(self) -> torch.Tensor:
+        return self.bp.theta()
+
+    def theta_true(self) -> torch.Tensor:
+        return self.bp.theta(*self.true).detach()
+
+    def mask_grads_(self) -> None:
+        self.bp.scal.grad[:, 2] = 0.0
+
+    def clamp_(self) -> None:
+        with torch.no_grad():
+            self.bp.refl.clamp_(0.0, 1.0)
+            self.bp.scal[:, 0:2].clamp_(0.01, 0.99)
+
+    def errors(self) -> dict[str, torch.Tensor]:
+        dr = self.bp.refl.detach() - self.true[0]
+        ds = (self.bp.scal.detach() - self.true[1]).abs()
+        return {"base_color": (dr @ self.m.T).abs().reshape(-1), "bins": dr.abs().reshape(-1),
+                "roughness": ds[:, 0], "metallic": ds[:, 1]}
+
+
+class InverseProblem:
+    """Scenes, renderers, target images and the parameters being optimised."""
+
+    def __init__(self, specs: list[scenes.SceneSpec], bvh: str = "sah", binned_colour: bool = True):
+        self.specs = specs
+        self.binned = binned_colour
+        self.renderers = []
+        self.params = []
+        self.targets = []
         for i, spec in enumerate(specs):
             sc = abi.Scene(spec.triangles, spec.materials, bvh=bvh)
             r = diff.DiffRenderer(sc, spec.materials, spec.camera, spec.params, spec.width, spec.height, spec.spp)
             th0 = r.theta0().numpy()
-            tt = torch.from_numpy(target_theta(th0, seed=int(spec.info.get("seed", i))))
-            self.targets.append(r.forward(tt, seed_base=0).detach().clone())
-            self.theta_true.append(tt)
-            self.theta.append(torch.from_numpy(th0.copy()).requires_grad_(True))
+            seed = int(spec.info.get("seed", i))
+            p = _BinnedParams(th0, seed) if binned_colour else _RgbParams(th0, seed)
+            self.targets.append(r.forward(p.theta_true(), seed_base=0).detach().clone())
+            self.params.append(p)
             self.renderers.append(r)
         self.samples_per_render = [s.width * s.height * s.spp for s in specs]
         # every scene's loss render and adjoint in one batch wavefront
         self.batch = diff.BatchDiffRenderer(self.renderers)
         self.target = torch.cat(self.targets)
 
+    def leaves(self) -> list[torch.Tensor]:
+        return [t for p in self.params for t in p.leaves()]
+
+    def parameter_count(self) -> int:
+        """Optimised values over all scenes."""
+        per = self.params[0].errors()
+        return len(self.params) * sum(per[k].numel() for k in per if k != ("base_color" if self.binned else ""))
+
     def loss(self, step: int = 0) -> torch.Tensor:
         """Summed L2 image loss of step `step`: the loss render uses seed base (2 step + 1) spp,
-        the adjoint replays seed base (2 step + 2) spp (the target is seed base 0); all scenes in
-        one batch render and one batch adjoint."""
+        the adjoint replays seed base (2 step + 2) spp (the target is seed base 0)."""
         spp = self.batch.spp
-        img = diff.render_batch_decorrelated(self.theta, self.batch, (2 * step + 1) * spp, (2 * step + 2) * spp)
+        thetas = [p.theta() for p in self.params]
+        img = diff.render_batch_decorrelated(thetas, self.batch, (2 * step + 1) * spp, (2 * step + 2) * spp)
         return ((img - self.target) ** 2).sum().double()
 
-    def param_error(self, params=OPT_PARAMS) -> float:
-        """Mean absolute error against the targets over the optimised groups and the given
-        parameter columns (default: every optimised parameter)."""
-        errs = []
-        cols = list(params)
-        for th, tt in zip(self.theta, self.theta_true):
-            d = (th.detach() - tt).abs()[list(OPT_GROUPS)][:, cols]
-            errs.append(d.reshape(-1))
-        return float(torch.cat(errs).mean())
-
-    def param_errors(self) -> dict:
-        return {"all": self.param_error(), "base_color": self.param_error((0, 1, 2)),
-                "roughness": self.param_error((3,)), "metallic": self.param_error((4,))}
+    def mask_grads_(self) -> None:
+        for p in self.params:
+            p.mask_grads_()
 
     def clamp_(self) -> None:
-        with torch.no_grad():
-            for th in self.theta:
-                th[:, 0:3].clamp_(0.0, 1.0)
-                th[:, 3:5].clamp_(0.01, 0.99)
+        for p in self.params:
+            p.clamp_()
+
+    def param_errors(self) -> dict:
+        """Mean absolute error against the targets: every optimised value ("all"), and per kind
+        (base_color: RGB, or for binned spectra their projected colour; bins; roughness; metallic)."""
+        per = [p.errors() for p in self.params]
+        out = {k: float(torch.cat([e[k] for e in per]).mean()) for k in per[0]}
+        raw = [torch.cat([e["bins" if self.binned else "base_color"], e["roughness"], e["metallic"]]) for e in per]
+        out["all"] = float(torch.cat(raw).mean())
+        return out
+
+    def param_error(self) -> float:
+        return self.param_errors()["all"]
+
+    def fd_check(self, fd_step: float) -> dict:
+        """Central-difference spot check of scene 0's gradient (detached estimator)."""
+        r0, p0 = self.renderers[0], self.params[0]
+        if self.binned:
+            res = binned.gradient_check(r0, p0.bp, self.targets[0], h=fd_step)
+            rel = res["rel_err"]
+        else:
+            res = diff.gradient_check(r0, p0.theta().detach().clone(), self.targets[0], h=fd_step)
+            live = res["live"] & p0.mask.numpy().astype(bool)
+            rel = res["rel_err"][live]
+        return {"params": int(rel.size), "max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel))}
 
 
 def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_every: int = 50,
              fd_step: float = 1e-2, log=None) -> dict:
     """Adam over every scene's parameters; central-difference spot checks of scene 0's gradient
     (detached estimator, step fd_step) every check_every steps."""
-    opt = torch.optim.Adam(problem.theta, lr=lr)
-    mask = problem.mask
+    opt = torch.optim.Adam(problem.leaves(), lr=lr)
     checks = []
     err0 = problem.param_errors()
     losses = []
@@ -130,17 +223,11 @@ def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_
         torch.cuda.synchronize()
         render_s += time.perf_counter() - tr
         losses.append(float(loss.detach()))
-        for th in problem.theta:
-            th.grad.mul_(mask)
+        problem.mask_grads_()
         opt.step()
         problem.clamp_()
         if check_every and (step + 1) % check_every == 0:
-            r0 = problem.renderers[0]
-            res = diff.gradient_check(r0, problem.theta[0].detach().clone(), problem.targets[0], h=fd_step)
-            live = res["live"] & mask.numpy().astype(bool)
-            rel = res["rel_err"][live]
-            checks.append({"step": step + 1, "params": int(live.sum()), "max_rel_err": float(rel.max()),
-                           "median_rel_err": float(np.median(rel))})
+            checks.append(dict(step=step + 1, **problem.fd_check(fd_step)))
             if log:
                 log(f"step {step + 1}: loss {losses[-1]:.6g}, parameter error {problem.param_error():.4g}, "
                     f"FD check max rel err {checks[-1]['max_rel_err']:.3g}")
@@ -149,6 +236,8 @@ def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_
     samples = steps * sum(problem.samples_per_render)
     window = max(1, min(10, steps // 4))
     return {"steps": steps, "lr": lr, "wall_s": wall, "render_and_adjoint_s": render_s,
+            "parameters": "16-bin base-colour spectra (linear-sRGB projection), roughness, metallic"
+                          if problem.binned else "RGB base colour, roughness, metallic",
             "renders_per_step": "loss render of a fresh sample set + the adjoint's render and replay of another",
             "grad_samples_per_s": samples / render_s,
             "loss_first_steps": float(np.mean(losses[:window])), "loss_last_steps": float(np.mean(losses[-window:])),
