@@ -193,6 +193,47 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
             "binned_spectral": binned_out}
 
 
+def facade_bench(spec) -> dict:
+    """The C++ renderer facade (class PathTracer) rendering the C1 frame to completion on its
+    worker thread, SAH build with its 4-wide tree: the reference's cadence (one pass per worker
+    iteration with a frame-ready callback each) and 256 passes per launch (one wavefront call)."""
+    from lsystemeditor_amd.tracer import Tracer
+    p = spec.params
+    pi = p.view(np.int32)
+    out = {"workload": "C1 through class PathTracer (configure, camera and settings, start, wait until idle)"}
+    for per_launch in (1, 256):
+        t = Tracer()
+        t.set_bvh_build_mode(1)
+        t.set_passes_per_launch(per_launch)
+        t0 = time.perf_counter()
+        if not t.configure(spec.width, spec.height, spec.triangles, spec.materials, None):
+            raise RuntimeError("facade configure failed")
+        setup_s = time.perf_counter() - t0
+        t.set_camera(spec.camera)
+        t.set_clear_color(float(p[0]), float(p[1]), float(p[2]))
+        t.set_environment_intensity(float(p[3]))
+        t.set_max_path_depth(int(pi[4]))
+        t.set_russian_roulette_min_depth(int(pi[5]))
+        t.set_min_samples(int(pi[6]))
+        t.set_max_samples_per_pixel(int(pi[7]))
+        t.set_relative_error_threshold(float(p[10]))
+        t1 = time.perf_counter()
+        t.start()
+        if not t.wait_until_idle(600000):
+            raise RuntimeError("facade render did not finish")
+        dt = time.perf_counter() - t1
+        t.stop()
+        samples = int(np.asarray(t.read_accumulator()["counts"], np.int64).sum())
+        t.close()
+        key = "reference_cadence" if per_launch == 1 else f"passes_per_launch_{per_launch}"
+        out[key] = {"msamples_s": samples / dt / 1e6, "samples": samples, "seconds": round(dt, 3),
+                    "configure_s": round(setup_s, 2)}
+    print(f"[bench] facade (class PathTracer): reference cadence {out['reference_cadence']['msamples_s']:.1f}, "
+          f"256 passes per launch {out['passes_per_launch_256']['msamples_s']:.1f} Msamples/s", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def batch_bench(bvh: str) -> dict:
     """C3: 64 bushes (seeds 0..63, iterations 3..7, 128^2..1024^2, 64 spp, depth 8) rendered by one
     batch wavefront replayed from a graph, against the per-scene graph (every scene's own render
@@ -427,6 +468,8 @@ def main():
         print(f"[bench] persistent megakernel {extra['persistent_megakernel_msamples_s']:.1f}, reference cadence "
               f"(one launch per pass) {extra['reference_cadence_msamples_s']:.1f} Msamples/s", file=sys.stderr,
               flush=True)
+        if args.config == "c1":
+            extra["facade"] = facade_bench(spec)
 
     if not args.no_spectral and args.config == "c1":
         extra["spectral"] = spectral_bench(scene, spec, inputs, acc, passes, args.bvh)

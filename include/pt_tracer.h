@@ -105,11 +105,27 @@ float ptTracerExposureMultiplier(const PtTracer* t);
  * 0 means 256 and negative values 32, as the integrator resolves them. Resets accumulation. */
 void ptTracerSetMaxPathDepth(PtTracer* t, int depth);
 int ptTracerMaxPathDepth(const PtTracer* t);
+/* Full-resolution passes per worker iteration (1..256, default 1 = the reference's cadence: one
+ * pass, then a frame-ready callback). Above 1 the worker renders that many passes of the active
+ * pixels in one wavefront call (pathTracerAdaptiveRenderWavefront: the same per-pixel results as
+ * the passes one by one), compacts the active list, adds them to the sample count and calls the
+ * frame-ready callback once. For offline rendering; no reference counterpart. */
+void ptTracerSetPassesPerLaunch(PtTracer* t, int passes);
+int ptTracerPassesPerLaunch(const PtTracer* t);
+/* BVH build of the next configure / rebuildMeshScene (ptSceneSetBuildMode: 0 = the reference's
+ * median split, the default; 1 = binned SAH with its 4-wide tree; 2 = binned SAH alone). No
+ * reference counterpart. */
+void ptTracerSetBvhBuildMode(PtTracer* t, int mode);
+int ptTracerBvhBuildMode(const PtTracer* t);
+/* The error text of the worker thread's last failed launch (pathTracerLastError is per thread);
+ * false when none failed. No reference counterpart. */
+bool ptTracerLastWorkerError(const PtTracer* t, char* buffer, int capacity);
 
 #ifdef __cplusplus
 } /* extern "C" */
 
 #include <functional>
+#include <string>
 #include <memory>
 
 namespace ptdetail {
@@ -196,6 +212,11 @@ public:
     float exposureMultiplier() const;
     void setMaxPathDepth(int depth);
     int maxPathDepth() const;
+    void setPassesPerLaunch(int passes);
+    int passesPerLaunch() const;
+    std::string lastWorkerError() const;
+    void setBvhBuildMode(int mode);
+    int bvhBuildMode() const;
 
 private:
     void renderLoop();

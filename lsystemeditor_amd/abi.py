@@ -128,6 +128,11 @@ SIGNATURES = {
     "ptTracerDebugOverlayMode": (_i, [_p]),
     "ptTracerSetBrdfDebugFlags": (None, [_p, _i]),
     "ptTracerBrdfDebugFlags": (_i, [_p]),
+    "ptTracerSetPassesPerLaunch": (None, [_p, _i]),
+    "ptTracerPassesPerLaunch": (_i, [_p]),
+    "ptTracerLastWorkerError": (_b, [_p, _p, _i]),
+    "ptTracerSetBvhBuildMode": (None, [_p, _i]),
+    "ptTracerBvhBuildMode": (_i, [_p]),
     "ptTracerSetPreviewStepsPerLevel": (None, [_p, _i]),
     "ptTracerPreviewStepsPerLevel": (_i, [_p]),
     "ptTracerCurrentSampleCount": (_i, [_p]),

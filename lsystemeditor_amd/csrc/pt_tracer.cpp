@@ -12,8 +12,10 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -188,6 +190,8 @@ struct TracerImpl {
     mutable std::mutex lastSampleCameraMutex;
     mutable std::mutex sceneMutex;
     std::mutex streamMutex;
+    std::mutex errorMutex;
+    std::string workerError;  // the last failed launch of the worker thread (its own error text)
     std::mutex callbackMutex;
     PathTracer::FrameReadyCallback callback;
 
@@ -210,6 +214,8 @@ struct TracerImpl {
     std::atomic<int> brdfDebugFlags{0};
     std::atomic<int> activePixelCount{0};
     std::atomic<int> previewStepsPerLevel{0};
+    std::atomic<int> passesPerLaunch{1};
+    std::atomic<int> bvhBuildMode{0};
     std::atomic<int> sampleCount{0};
     std::atomic<int> finestCompletedPreview{-1};
     std::atomic<bool> displayBuffersValid{false};
@@ -576,6 +582,9 @@ bool buildScene(TracerImpl* impl, const PtHostMesh& mesh)
         if (impl->scene == nullptr) {
             return false;
         }
+    }
+    if (!ptSceneSetBuildMode(impl->scene, impl->bvhBuildMode.load())) {
+        return false;
     }
     return ptSceneBuild(impl->scene, mesh.triangles, mesh.triangleCount, mesh.materials, mesh.materialCount,
                         mesh.textures, mesh.textureCount, impl->sampleStream);
@@ -1138,6 +1147,23 @@ void PathTracer::setMaxPathDepth(int depth)
 }
 int PathTracer::maxPathDepth() const { return m_impl->hostParams.maxPathDepth; }
 
+std::string PathTracer::lastWorkerError() const
+{
+    std::lock_guard<std::mutex> lock(m_impl->errorMutex);
+    return m_impl->workerError;
+}
+
+void PathTracer::setPassesPerLaunch(int passes)
+{
+    m_impl->passesPerLaunch.store(passes < 1 ? 1 : (passes > 256 ? 256 : passes));
+}
+
+int PathTracer::passesPerLaunch() const { return m_impl->passesPerLaunch.load(); }
+
+void PathTracer::setBvhBuildMode(int mode) { m_impl->bvhBuildMode.store(mode < 0 ? 0 : (mode > 2 ? 2 : mode)); }
+
+int PathTracer::bvhBuildMode() const { return m_impl->bvhBuildMode.load(); }
+
 void PathTracer::notifyWorker()
 {
     {
@@ -1223,6 +1249,7 @@ void PathTracer::renderLoop()
                                  iter < static_cast<int>(impl->previewLevels.size());
         bool launched = false;
         bool pendingUpdate = false;
+        int passesTaken = 1;
         static thread_local int pendingCount = 0;
         {
             std::lock_guard<std::mutex> streamLock(impl->streamMutex);
@@ -1249,24 +1276,36 @@ void PathTracer::renderLoop()
             } else {
                 const int active = impl->activePixelCount.load();
                 const int fullResWave = iter - previewCount + 1;
-                const bool runCompact = active > 0 && (fullResWave % kCompactActiveListInterval) == 0;
+                const int perLaunch = impl->passesPerLaunch.load();
+                bool runCompact = active > 0 && (fullResWave % kCompactActiveListInterval) == 0;
                 auto& a = impl->acc;
-                if (active > 0) {
+                if (active > 0 && perLaunch > 1) {
+                    // several passes in one wavefront render (same per-pixel results), then a
+                    // compaction of the caller-side active list
+                    const int depth = impl->hostParams.maxPathDepth;
+                    const int resolved = depth > 0 ? depth : (depth == 0 ? 256 : 32);
+                    passesTaken = perLaunch;
+                    launched = pathTracerAdaptiveRenderWavefront(
+                        a.buffer, a.samples, a.lumMean, a.m2, a.converged, a.activeIndices, active, a.width, a.height,
+                        impl->dCamera, d_scene, d_env, impl->dParams, globalSeed, perLaunch, resolved, 0,
+                        impl->sampleStream);
+                    runCompact = true;
+                } else if (active > 0) {
                     launched = pathTracerAdaptiveSample(a.buffer, a.samples, a.lumMean, a.m2, a.converged,
                                                         a.activeIndices, active, a.width, a.height, impl->dCamera,
                                                         d_scene, d_env, impl->dParams, globalSeed, impl->sampleStream);
-                    if (launched && runCompact) {
-                        launched = pathTracerCompactActiveList(
-                            a.activeIndices, a.activeScratch, a.activeCount, a.converged, a.samples,
-                            impl->maxSamplesPerPixel.load(), active, a.width, a.height, impl->regionEnabled.load(),
-                            impl->regionMinX.load(), impl->regionMinY.load(), impl->regionMaxX.load(),
-                            impl->regionMaxY.load(), impl->sampleStream);
-                        if (launched) {
-                            pendingCount = 0;
-                            launched = hipMemcpyAsync(&pendingCount, a.activeCount, sizeof(int), hipMemcpyDeviceToHost,
-                                                      impl->sampleStream) == hipSuccess;
-                            pendingUpdate = launched;
-                        }
+                }
+                if (launched && runCompact) {
+                    launched = pathTracerCompactActiveList(
+                        a.activeIndices, a.activeScratch, a.activeCount, a.converged, a.samples,
+                        impl->maxSamplesPerPixel.load(), active, a.width, a.height, impl->regionEnabled.load(),
+                        impl->regionMinX.load(), impl->regionMinY.load(), impl->regionMaxX.load(),
+                        impl->regionMaxY.load(), impl->sampleStream);
+                    if (launched) {
+                        pendingCount = 0;
+                        launched = hipMemcpyAsync(&pendingCount, a.activeCount, sizeof(int), hipMemcpyDeviceToHost,
+                                                  impl->sampleStream) == hipSuccess;
+                        pendingUpdate = launched;
                     }
                 }
             }
@@ -1276,6 +1315,10 @@ void PathTracer::renderLoop()
             }
         }
         if (!launched) {
+            {
+                std::lock_guard<std::mutex> errorLock(impl->errorMutex);
+                impl->workerError = pathTracerLastError();
+            }
             if (!previewPass && impl->activePixelCount.load() > 0) {
                 std::this_thread::sleep_for(std::chrono::milliseconds(1));
             }
@@ -1298,7 +1341,7 @@ void PathTracer::renderLoop()
         if (previewCount > 0 && iter < previewCount) {
             impl->finestCompletedPreview.store(iter);
         }
-        impl->sampleCount.fetch_add(1);
+        impl->sampleCount.fetch_add(passesTaken);
         impl->displayBuffersValid.store(true);
         invokeFrameReadyCallback();
     }
@@ -1421,5 +1464,18 @@ bool ptTracerReadPreviewLevel(PtTracer* t, int level, float* acc4, int* w, int* 
 float ptTracerExposureMultiplier(const PtTracer* t) { return t ? t->tracer.exposureMultiplier() : 0.0f; }
 void ptTracerSetMaxPathDepth(PtTracer* t, int depth) { if (t) t->tracer.setMaxPathDepth(depth); }
 int ptTracerMaxPathDepth(const PtTracer* t) { return t ? t->tracer.maxPathDepth() : 0; }
+void ptTracerSetPassesPerLaunch(PtTracer* t, int passes) { if (t) t->tracer.setPassesPerLaunch(passes); }
+bool ptTracerLastWorkerError(const PtTracer* t, char* buffer, int capacity)
+{
+    if (t == nullptr || buffer == nullptr || capacity <= 0) {
+        return false;
+    }
+    const std::string e = t->tracer.lastWorkerError();
+    std::snprintf(buffer, static_cast<size_t>(capacity), "%s", e.c_str());
+    return !e.empty();
+}
+int ptTracerPassesPerLaunch(const PtTracer* t) { return t ? t->tracer.passesPerLaunch() : 0; }
+void ptTracerSetBvhBuildMode(PtTracer* t, int mode) { if (t) t->tracer.setBvhBuildMode(mode); }
+int ptTracerBvhBuildMode(const PtTracer* t) { return t ? t->tracer.bvhBuildMode() : 0; }
 
 } // extern "C"

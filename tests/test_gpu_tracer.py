@@ -23,10 +23,11 @@ class _Acc:
         self.acc, self.counts, self.mean, self.m2, self.conv = d["acc"], d["counts"], d["mean"], d["m2"], d["conv"]
 
 
-def setup_tracer(w, h, spp, min_spp, thr, depth=8, preview=0):
+def setup_tracer(w, h, spp, min_spp, thr, depth=8, preview=0, bvh_mode=0):
     tris, mats, tex, cam, params = dense_plant(w, h, spp=spp, min_spp=min_spp, threshold=thr, max_depth=depth)
     t = Tracer()
     t.set_preview_steps_per_level(preview)
+    t.set_bvh_build_mode(bvh_mode)
     assert t.configure(w, h, tris, mats, tex), abi.last_error()
     t.set_camera(cam)
     t.set_max_samples_per_pixel(spp)
@@ -60,6 +61,31 @@ def test_facade_render_matches_oracle(cuda):
     assert t.current_active_pixel_count() == 0
     assert t.is_sample_budget_exhausted()
     assert t.sample_budget_total_iterations() == 16
+    t.close()
+
+
+@pytest.mark.parametrize("per_launch,bvh_mode", [(8, 0), (64, 1)])
+def test_facade_passes_per_launch_matches_oracle(cuda, per_launch, bvh_mode):
+    """Several full-resolution passes per worker iteration (one wavefront call each, then a
+    compaction), on the median tree and on the SAH build with its 4-wide tree: the accumulator
+    equals the oracle's per-pass schedule bit for bit."""
+    w = h = 40
+    t, (tris, mats, tex, cam, params) = setup_tracer(w, h, spp=24, min_spp=4, thr=0.08, bvh_mode=bvh_mode)
+    t.set_passes_per_launch(per_launch)
+    frames = []
+    t.set_frame_ready_callback(lambda: frames.append(1))
+    t.start()
+    assert t.wait_until_idle(60000)
+    t.stop()
+    got = t.read_accumulator()
+    ref = oracle_scene(tris, mats, tex)
+    acc_o, passes = O.render_adaptive(ref, cam, oracle_params(params, 0.08), w, h, threads=8)
+    compare_accumulators(got, acc_o, f"facade/{per_launch} per launch")
+    assert t.current_active_pixel_count() == 0
+    # the worker launches until no pixel is active: as often as the longest-running pixel needs
+    most = int(np.asarray(acc_o.counts).max())
+    assert len(frames) == (most + per_launch - 1) // per_launch
+    assert t.current_sample_count() == len(frames) * per_launch
     t.close()
 
 

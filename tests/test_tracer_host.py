@@ -45,6 +45,12 @@ def test_defaults_match_reference(t):
     ("set_russian_roulette_min_depth", "russian_roulette_min_depth", -3, 0),
     ("set_russian_roulette_min_depth", "russian_roulette_min_depth", 999, 256),
     ("set_brdf_debug_flags", "brdf_debug_flags", 16, 16),
+    ("set_passes_per_launch", "passes_per_launch", 0, 1),
+    ("set_passes_per_launch", "passes_per_launch", 1000, 256),
+    ("set_passes_per_launch", "passes_per_launch", 8, 8),
+    ("set_bvh_build_mode", "bvh_build_mode", -1, 0),
+    ("set_bvh_build_mode", "bvh_build_mode", 7, 2),
+    ("set_bvh_build_mode", "bvh_build_mode", 1, 1),
 ])
 def test_int_setter_clamps(t, setter, getter, value, expected):
     getattr(t, setter)(value)
