@@ -355,15 +355,17 @@ def inverse_bench(bvh: str, steps: int) -> dict:
                            log=lambda m: print(f"[bench] C4 {m}", file=sys.stderr, flush=True))
     out = {"workload": "C4: 32 bushes, seeds 0-31, iterations 3-6, 256x256, 16 spp, depth 8; Adam lr 1e-2 on "
                        "16-bin base-colour spectra, roughness, metallic of bark, leaf, ground (targets: seeded "
-                       "+-20% perturbation)",
+                       "+-20% perturbation, rendered at 256 spp)",
            "scenes": len(specs), "params": prob.parameter_count(),
            "triangles": int(sum(s.triangles.shape[0] for s in specs)),
            "scene_generation_s": round(gen_s, 2), "setup_s": round(setup_s, 2), **res}
     e0, e1 = res["param_mae_initial"], res["param_mae_final"]
     print(f"[bench] C4 {steps} steps in {res['wall_s']:.1f} s ({res['grad_samples_per_s'] / 1e6:.1f} grad-Msamples/s), "
           f"parameter error {e0['all']:.4f} -> {e1['all']:.4f} (projected base colour {e0['base_color']:.4f} -> "
-          f"{e1['base_color']:.4f}), step loss {res['loss_first_steps']:.4g} -> {res['loss_last_steps']:.4g}",
-          file=sys.stderr, flush=True)
+          f"{e1['base_color']:.4f}), step loss {res['loss_first_steps']:.4g} -> {res['loss_last_steps']:.4g}; "
+          f"256-spp image loss {res['image_loss']['initial']:.5g} -> {res['image_loss']['final']:.5g} "
+          f"(at the target parameters {res['image_loss']['target_params']:.5g}: excess "
+          f"{100 * res['image_loss']['final_excess_fraction']:.1f} % of the initial)", file=sys.stderr, flush=True)
     del prob
     return out
 

@@ -7,19 +7,27 @@ groups are, by default, a 16-bin base-colour spectrum, roughness and metallic (b
 bins enter the renderer through their linear-sRGB projection); with binned_colour=False, the RGB
 base colour, roughness and metallic. Specular and the lights stay fixed. The target parameters are
 a seeded perturbation (every optimised value scaled by an independent factor in [0.8, 1.2]); the
-target image is the fixed-spp render at them. The optimisation starts from the unperturbed
-parameters and minimises the summed L2 image loss over all scenes with Adam (lr 1e-2).
+target image is their render converged over 16 fixed-spp renders (256 spp, seeds disjoint from
+every step's). The optimisation starts from the unperturbed parameters and minimises the summed
+L2 image loss over all scenes with Adam (lr 1e-2).
 
-The target is rendered once (seed base 0); every step renders the loss image with a fresh sample
-set and replays another fresh, independent set in the adjoint, so each step's gradient is an
-unbiased estimate of the gradient of the expected-image loss |E[image](theta) - target|^2. With
-the same samples for the residual and the derivatives the gradient would carry the covariance of
-image noise and its derivative, and with fixed sample sets the sampling error would be a fixed
-function of the parameters: at 16 spp either drives poorly determined parameters away from the
-target. All scenes render in one batch wavefront and differentiate in one batch adjoint.
+Every step renders the loss image with a fresh sample set and replays another fresh, independent
+set in the adjoint, so each step's gradient is an unbiased estimate of the gradient of the
+expected-image loss |E[image](theta) - target|^2. With the same samples for the residual and the
+derivatives the gradient would carry the covariance of image noise and its derivative, and with
+fixed sample sets the sampling error would be a fixed function of the parameters: at 16 spp
+either drives poorly determined parameters away from the target. A 16-spp target would be such a
+fixed error too (its noise moves the loss minimum), hence the converged target. All scenes render
+in one batch wavefront and differentiate in one batch adjoint.
 
-RGB light transport sees a binned spectrum only through its projection (3 of its 16 degrees of
-freedom), so the bins' error is reported beside the error of their projected colour.
+Convergence is measured in image space as well as in parameter space: the loss of 256-spp renders
+(one seed set, disjoint from the target's and the steps') at the start, at the end and at the
+target parameters, whose loss is the noise floor of two 256-spp images. Roughness and metallic
+are nearly degenerate with the base colours in these images (the recovered colours with the true
+roughness and metallic render further from the target than the recovered parameters do; the
+loss of that combination is reported too), so their parameter errors stay well above the image
+loss's. RGB light transport sees a binned spectrum only through its projection (3 of its 16
+degrees of freedom), so the bins' error is reported beside the error of their projected colour.
 """
 from __future__ import annotations
 
@@ -33,6 +41,9 @@ from . import abi, binned, diff, scenes
 OPT_PARAMS = (0, 1, 2, 3, 4)          # base r, g, b, roughness, metallic (diff.PARAM_NAMES order)
 OPT_GROUPS = (0, 1, 2)                # bark, leaf, ground
 LIGHT_GROUPS = (3, 4)                 # the two lights stay fixed
+TARGET_SEED_BASE = 1 << 24            # target renders; steps use seed bases below (2 steps + 2) spp
+EVAL_SEED_BASE = 1 << 25              # image-loss evaluation renders
+CONVERGED_PASSES = 16                 # fixed-spp renders averaged into a target or evaluation image
 
 
 def c4_specs(count: int = 32, res: int = 256, spp: int = 16, depth: int = 8, iterations=(3, 6),
@@ -82,6 +93,12 @@ class _RgbParams:
     def theta_true(self) -> torch.Tensor:
         return self.true
 
+    def theta_true_scalars(self) -> torch.Tensor:
+        """The current base colours with the target roughness and metallic."""
+        out = self.leaf.detach().clone()
+        out[:, 3:5] = self.true[:, 3:5]
+        return out
+
     def mask_grads_(self) -> None:
         self.leaf.grad.mul_(self.mask)
 
@@ -117,6 +134,12 @@ class _BinnedParams:
     def theta_true(self) -> torch.Tensor:
         return self.bp.theta(*self.true).detach()
 
+    def theta_true_scalars(self) -> torch.Tensor:
+        """The current base-colour spectra with the target roughness and metallic."""
+        scal = self.bp.scal.detach().clone()
+        scal[:, 0:2] = self.true[1][:, 0:2]
+        return self.bp.theta(self.bp.refl.detach(), scal, self.bp.emis.detach()).detach()
+
     def mask_grads_(self) -> None:
         self.bp.scal.grad[:, 2] = 0.0
 
@@ -140,20 +163,40 @@ class InverseProblem:
         self.binned = binned_colour
         self.renderers = []
         self.params = []
-        self.targets = []
         for i, spec in enumerate(specs):
             sc = abi.Scene(spec.triangles, spec.materials, bvh=bvh)
             r = diff.DiffRenderer(sc, spec.materials, spec.camera, spec.params, spec.width, spec.height, spec.spp)
             th0 = r.theta0().numpy()
             seed = int(spec.info.get("seed", i))
-            p = _BinnedParams(th0, seed) if binned_colour else _RgbParams(th0, seed)
-            self.targets.append(r.forward(p.theta_true(), seed_base=0).detach().clone())
-            self.params.append(p)
+            self.params.append(_BinnedParams(th0, seed) if binned_colour else _RgbParams(th0, seed))
             self.renderers.append(r)
         self.samples_per_render = [s.width * s.height * s.spp for s in specs]
         # every scene's loss render and adjoint in one batch wavefront
         self.batch = diff.BatchDiffRenderer(self.renderers)
-        self.target = torch.cat(self.targets)
+        self.target_spp = CONVERGED_PASSES * self.batch.spp
+        self.target = self.converged_image([p.theta_true() for p in self.params], TARGET_SEED_BASE)
+        self.targets = [self.target[self.batch.pixels(i)] for i in range(len(specs))]
+
+    def converged_image(self, thetas, seed_base: int, passes: int = CONVERGED_PASSES) -> torch.Tensor:
+        """(total pixels, 3) mean of `passes` batch renders at thetas, render j with seed base
+        seed_base + j spp (passes x spp samples per pixel)."""
+        acc = None
+        for j in range(passes):
+            im = self.batch.forward([t.detach() for t in thetas], seed_base=seed_base + j * self.batch.spp)
+            acc = im if acc is None else acc.add_(im)
+        return acc / passes
+
+    def image_loss(self, thetas) -> float:
+        """Summed L2 loss of the thetas' converged image against the target (one evaluation seed set
+        for every call, so differences between calls carry little sampling noise)."""
+        return float(((self.converged_image(thetas, EVAL_SEED_BASE) - self.target) ** 2).sum())
+
+    def image_losses(self) -> dict:
+        """image_loss at the current parameters, at the target parameters (the noise floor), and
+        at the current base colours with the target roughness and metallic."""
+        return {"current": self.image_loss([p.theta() for p in self.params]),
+                "target_params": self.image_loss([p.theta_true() for p in self.params]),
+                "target_roughness_metallic": self.image_loss([p.theta_true_scalars() for p in self.params])}
 
     def leaves(self) -> list[torch.Tensor]:
         return [t for p in self.params for t in p.leaves()]
@@ -211,6 +254,7 @@ def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_
     opt = torch.optim.Adam(problem.leaves(), lr=lr)
     checks = []
     err0 = problem.param_errors()
+    img0 = problem.image_losses()
     losses = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -233,6 +277,8 @@ def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_
                     f"FD check max rel err {checks[-1]['max_rel_err']:.3g}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    img1 = problem.image_losses()
+    floor = img1["target_params"]
     samples = steps * sum(problem.samples_per_render)
     window = max(1, min(10, steps // 4))
     return {"steps": steps, "lr": lr, "wall_s": wall, "render_and_adjoint_s": render_s,
@@ -242,4 +288,9 @@ def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_
             "grad_samples_per_s": samples / render_s,
             "loss_first_steps": float(np.mean(losses[:window])), "loss_last_steps": float(np.mean(losses[-window:])),
             "loss_window": window, "param_mae_initial": err0, "param_mae_final": problem.param_errors(),
+            "target_spp": problem.target_spp,
+            "image_loss": {"evaluation_spp": problem.target_spp, "initial": img0["current"],
+                           "final": img1["current"], "target_params": floor,
+                           "final_colours_target_roughness_metallic": img1["target_roughness_metallic"],
+                           "final_excess_fraction": (img1["current"] - floor) / max(img0["current"] - floor, 1e-30)},
             "fd_checks": checks}
