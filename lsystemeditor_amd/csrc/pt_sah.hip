@@ -4,7 +4,7 @@
 // child boxes, one triangle per leaf slot), so every traversal kernel runs on it unchanged.
 //   1. Triangles are put in Morton order of their centroids (30-bit codes, radix sort).
 //   2. The tree is built top down, one level of segments at a time. Per level: each segment's
-//      triangle and centroid bounds (block-level LDS aggregation, then global atomics); 32 bins
+//      triangle and centroid bounds (segmented wave scans, then LDS or global atomics); 32 bins
 //      along the segment's longest centroid axis, staged in LDS per block; one thread per
 //      segment sweeps the bins for the lowest surface-area cost; a stable partition (scan)
 //      moves each segment's left side to its front. The partition is stable, so the Morton
@@ -34,6 +34,9 @@ constexpr int kB = 256;
 constexpr int kBins = 32;
 constexpr int kBinWords = 7;   // count, min xyz, max xyz (sortable integers)
 constexpr int kSegWords = 12;  // triangle bounds (6) and centroid bounds (6), sortable integers
+constexpr int kWave = 64;
+constexpr unsigned kPrepBlocks = 1024;
+constexpr int kChunk = 8;       // positions per block of seg_bounds_kernel: kChunk x kB
 static_assert(kMaxSahDepth <= 56, "traversal stacks hold 64 entries");
 
 __device__ __forceinline__ uint32_t sortable(float f)
@@ -65,12 +68,21 @@ struct SplitRec {
     int internal;  // 1 when the segment is an internal node
 };
 
-__global__ void sah_prep_kernel(const float* __restrict__ tris, int n, float4* __restrict__ bounds,
-                                float4* __restrict__ cent, float4* __restrict__ shade, uint32_t* __restrict__ gcb)
+// Grid-stride over the triangles (at most kPrepBlocks blocks): per-triangle bounds, centroid and
+// shading record; the centroid bounds reduce per lane, per wave (shuffles) and per block (LDS),
+// so the scene bounds take one set of global atomics per block.
+__global__ __launch_bounds__(kB) void sah_prep_kernel(const float* __restrict__ tris, int n,
+                                                      float4* __restrict__ bounds, float4* __restrict__ cent,
+                                                      float4* __restrict__ shade, uint32_t* __restrict__ gcb)
 {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    __shared__ uint32_t lds[6];
+    if (threadIdx.x < 6) {
+        lds[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+    }
+    __syncthreads();
     uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    if (i < n) {
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
         const float* t = tris + 25 * static_cast<size_t>(i);
         const float third = 1.0f / 3.0f;
         const float cx = ((t[0] + t[3]) + t[6]) * third;
@@ -85,21 +97,29 @@ __global__ void sah_prep_kernel(const float* __restrict__ tris, int n, float4* _
         shade[4 * i + 1] = make_float4(t[13], t[14], t[15], t[16]);
         shade[4 * i + 2] = make_float4(t[17], t[18], t[19], t[20]);
         shade[4 * i + 3] = make_float4(t[21], t[22], t[23], t[24]);
-        v[0] = v[3] = sortable(cx);
-        v[1] = v[4] = sortable(cy);
-        v[2] = v[5] = sortable(cz);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t c[3] = {sortable(cx), sortable(cy), sortable(cz)};
         for (int k = 0; k < 3; ++k) {
-            v[k] = min(v[k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[k]), off, 64)));
-            v[3 + k] = max(v[3 + k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + k]), off, 64)));
+            v[k] = min(v[k], c[k]);
+            v[3 + k] = max(v[3 + k], c[k]);
         }
     }
-    if ((threadIdx.x & 63u) == 0u) {
+    for (int off = kWave / 2; off > 0; off >>= 1) {
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&gcb[k], v[k]);
-            atomicMax(&gcb[3 + k], v[3 + k]);
+            v[k] = min(v[k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[k]), off, kWave)));
+            v[3 + k] = max(v[3 + k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + k]), off, kWave)));
         }
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0u) {
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&lds[k], v[k]);
+            atomicMax(&lds[3 + k], v[3 + k]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        atomicMin(&gcb[threadIdx.x], lds[threadIdx.x]);
+    } else if (threadIdx.x < 6) {
+        atomicMax(&gcb[threadIdx.x], lds[threadIdx.x]);
     }
 }
 
@@ -144,37 +164,71 @@ __global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, int segs)
     segStats[i] = (w % 6) < 3 ? 0xFFFFFFFFu : 0u;
 }
 
-// Block-uniform blocks (one segment) reduce in LDS first; others use global atomics per lane.
+// A segment occupies one contiguous range of positions, so its lanes within a wave form one run.
+// A block covers kChunk x kB consecutive positions. A segmented inclusive scan over each wave
+// (shuffles) leaves each run's bounds in the run's last lane, and only those lanes update the
+// block's LDS copy (blocks inside one segment, flushed to global once per block) or the segment's
+// global record.
 __global__ __launch_bounds__(kB) void seg_bounds_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
                                                         const float4* __restrict__ bounds,
                                                         const float4* __restrict__ cent, int n,
                                                         uint32_t* __restrict__ segStats)
 {
     __shared__ uint32_t lds[kSegWords];
-    const int base = static_cast<int>(blockIdx.x * blockDim.x);
-    const int i = base + static_cast<int>(threadIdx.x);
-    const int last = min(base + kB, n) - 1;
+    const int base = static_cast<int>(blockIdx.x) * kB * kChunk;
+    const int last = min(base + kB * kChunk, n) - 1;
     const int s0 = posSeg[base];
     const bool uniform = s0 >= 0 && posSeg[last] == s0;
     if (uniform && threadIdx.x < kSegWords) {
         lds[threadIdx.x] = (threadIdx.x % 6) < 3 ? 0xFFFFFFFFu : 0u;
     }
     __syncthreads();
-    const int s = i < n ? posSeg[i] : -1;
-    if (s >= 0) {
-        const int tri = perm[i];
-        const float4 lo = bounds[2 * tri];
-        const float4 hi = bounds[2 * tri + 1];
-        const float4 c = cent[tri];
-        const uint32_t v[kSegWords] = {sortable(lo.x), sortable(lo.y), sortable(lo.z), sortable(hi.x),
-                                       sortable(hi.y), sortable(hi.z), sortable(c.x),  sortable(c.y),
-                                       sortable(c.z),  sortable(c.x),  sortable(c.y),  sortable(c.z)};
-        uint32_t* dst = uniform ? lds : segStats + static_cast<size_t>(kSegWords) * s;
-        for (int k = 0; k < kSegWords; ++k) {
-            if ((k % 6) < 3) {
-                atomicMin(&dst[k], v[k]);
-            } else {
-                atomicMax(&dst[k], v[k]);
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    for (int it = 0; it < kChunk; ++it) {
+        const int i = base + it * kB + static_cast<int>(threadIdx.x);
+        const int s = i < n ? posSeg[i] : -1;
+        uint32_t v[kSegWords];
+        if (s >= 0) {
+            const int tri = perm[i];
+            const float4 lo = bounds[2 * tri];
+            const float4 hi = bounds[2 * tri + 1];
+            const float4 c = cent[tri];
+            v[0] = sortable(lo.x);
+            v[1] = sortable(lo.y);
+            v[2] = sortable(lo.z);
+            v[3] = sortable(hi.x);
+            v[4] = sortable(hi.y);
+            v[5] = sortable(hi.z);
+            v[6] = sortable(c.x);
+            v[7] = sortable(c.y);
+            v[8] = sortable(c.z);
+            v[9] = v[6];
+            v[10] = v[7];
+            v[11] = v[8];
+        } else {
+            for (int k = 0; k < kSegWords; ++k) {
+                v[k] = (k % 6) < 3 ? 0xFFFFFFFFu : 0u;
+            }
+        }
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int so = __shfl_up(s, d, kWave);
+            const bool take = lane >= d && so == s;
+            for (int k = 0; k < kSegWords; ++k) {
+                const uint32_t o = __shfl_up(v[k], d, kWave);
+                if (take) {
+                    v[k] = (k % 6) < 3 ? min(v[k], o) : max(v[k], o);
+                }
+            }
+        }
+        const int sn = __shfl_down(s, 1, kWave);
+        if (s >= 0 && (lane == kWave - 1 || sn != s)) {
+            uint32_t* dst = uniform ? lds : segStats + static_cast<size_t>(kSegWords) * s;
+            for (int k = 0; k < kSegWords; ++k) {
+                if ((k % 6) < 3) {
+                    atomicMin(&dst[k], v[k]);
+                } else {
+                    atomicMax(&dst[k], v[k]);
+                }
             }
         }
     }
@@ -270,6 +324,11 @@ __device__ __forceinline__ int bin_of(const SplitRec& r, const float4& c)
     return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
 }
 
+// Blocks inside one segment histogram in LDS and flush once; others update the segment's global
+// bins. Consecutive lanes often share a (segment, bin) key (positions are in Morton order), so each
+// wave first reduces its runs of equal keys: a max-scan of run heads gives every lane its run's
+// first lane, a segmented scan over those run ids leaves each run's count and bounds in its last
+// lane, and only those lanes update the histogram.
 __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
                                                  const float4* __restrict__ bounds, const float4* __restrict__ cent,
                                                  const SplitRec* __restrict__ split, int n, uint32_t* __restrict__ bins)
@@ -287,24 +346,62 @@ __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, c
         }
     }
     __syncthreads();
-    const int s = i < n ? posSeg[i] : -1;
+    int s = i < n ? posSeg[i] : -1;
+    int b = -1;  // (s, b): the lane's histogram cell; b < 0 for lanes with nothing to add
+    uint32_t v[kBinWords] = {0u, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
     if (s >= 0) {
         const SplitRec r = split[s];
         if (r.axis >= 0) {
             const int tri = perm[i];
-            const int b = bin_of(r, cent[tri]);
+            b = bin_of(r, cent[tri]);
             const float4 lo = bounds[2 * tri];
             const float4 hi = bounds[2 * tri + 1];
-            uint32_t* dst = uniform ? lds + b * kBinWords
-                                    : bins + static_cast<size_t>(kBins * kBinWords) * s + b * kBinWords;
-            atomicAdd(&dst[0], 1u);
-            atomicMin(&dst[1], sortable(lo.x));
-            atomicMin(&dst[2], sortable(lo.y));
-            atomicMin(&dst[3], sortable(lo.z));
-            atomicMax(&dst[4], sortable(hi.x));
-            atomicMax(&dst[5], sortable(hi.y));
-            atomicMax(&dst[6], sortable(hi.z));
+            v[0] = 1u;
+            v[1] = sortable(lo.x);
+            v[2] = sortable(lo.y);
+            v[3] = sortable(lo.z);
+            v[4] = sortable(hi.x);
+            v[5] = sortable(hi.y);
+            v[6] = sortable(hi.z);
         }
+    }
+    if (b < 0) {
+        s = -1;
+    }
+    // every shuffle runs with the whole wave active (a shuffle under a lane-dependent condition
+    // would read lanes that did not take part)
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int sPrev = __shfl_up(s, 1, kWave);
+    const int bPrev = __shfl_up(b, 1, kWave);
+    int run = (lane == 0 || sPrev != s || bPrev != b) ? lane : 0;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(run, d, kWave);
+        if (lane >= d) {
+            run = max(run, o);
+        }
+    }
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int runPrev = __shfl_up(run, d, kWave);
+        const bool take = lane >= d && runPrev == run;
+        for (int k = 0; k < kBinWords; ++k) {
+            const uint32_t o = __shfl_up(v[k], d, kWave);
+            if (take) {
+                v[k] = k == 0 ? v[k] + o : (k < 4 ? min(v[k], o) : max(v[k], o));
+            }
+        }
+    }
+    const int sNext = __shfl_down(s, 1, kWave);
+    const int bNext = __shfl_down(b, 1, kWave);
+    if (b >= 0 && (lane == kWave - 1 || sNext != s || bNext != b)) {
+        uint32_t* dst = uniform ? lds + b * kBinWords
+                                : bins + static_cast<size_t>(kBins * kBinWords) * s + b * kBinWords;
+        atomicAdd(&dst[0], v[0]);
+        atomicMin(&dst[1], v[1]);
+        atomicMin(&dst[2], v[2]);
+        atomicMin(&dst[3], v[3]);
+        atomicMax(&dst[4], v[4]);
+        atomicMax(&dst[5], v[5]);
+        atomicMax(&dst[6], v[6]);
     }
     __syncthreads();
     if (uniform) {
@@ -490,6 +587,7 @@ __global__ void fill_u32_kernel(uint32_t* p, int n, uint32_t v0, uint32_t v1, in
 }
 
 unsigned grid(long long n) { return static_cast<unsigned>((n + kB - 1) / kB); }
+unsigned chunks(long long n, int chunk) { return static_cast<unsigned>((n + kB * chunk - 1) / (kB * chunk)); }
 
 template <typename T>
 bool dalloc(T** p, size_t count)
@@ -560,7 +658,8 @@ bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, 
             break;
         }
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kB), 0, stream, gcb, 6, 0xFFFFFFFFu, 0u, 3);
-        hipLaunchKernelGGL(sah_prep_kernel, dim3(grid(n)), dim3(kB), 0, stream, dTris, n, bounds, cent, triShade, gcb);
+        hipLaunchKernelGGL(sah_prep_kernel, dim3(std::min(grid(n), kPrepBlocks)), dim3(kB), 0, stream, dTris, n,
+                           bounds, cent, triShade, gcb);
         hipLaunchKernelGGL(morton_kernel, dim3(grid(n)), dim3(kB), 0, stream, cent, n, gcb, keys, vals);
         size_t bytes = tmpBytes;
         if (hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keysOut, vals, perm, n, 0, 30, stream) != hipSuccess) {
@@ -580,8 +679,8 @@ bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, 
         while (segCount > 0 && levelOk) {
             hipLaunchKernelGGL(seg_reset_kernel, dim3(grid(static_cast<long long>(segCount) * kSegWords)), dim3(kB), 0,
                                stream, segStats, segCount);
-            hipLaunchKernelGGL(seg_bounds_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, bounds, cent, n,
-                               segStats);
+            hipLaunchKernelGGL(seg_bounds_kernel, dim3(chunks(n, kChunk)), dim3(kB), 0, stream, perm, posSeg, bounds,
+                               cent, n, segStats);
             hipLaunchKernelGGL(seg_setup_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, segStats,
                                perm, dTris, level, reinterpret_cast<float*>(nodes), triPos, rootBox, split, bins);
             hipLaunchKernelGGL(bin_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, bounds, cent, split, n,

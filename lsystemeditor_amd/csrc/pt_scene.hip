@@ -33,6 +33,8 @@
 namespace {
 
 constexpr int kB = 256;
+constexpr int kWave = 64;
+constexpr int kBoundsChunk = 8;  // positions per block of bounds_kernel: kBoundsChunk x kB
 
 __device__ __forceinline__ uint32_t to_sortable(float f)
 {
@@ -157,46 +159,69 @@ __global__ void fill_bounds_kernel(uint32_t* segBounds, int segs)
 // Segment bounds at depth d. Min/max are taken in the sortable-integer order, in which -0 < +0:
 // the union is then independent of the visiting order (the reference's vecMin2/vecMax2 loop
 // keeps whichever signed zero it met first; both compare equal in every traversal test).
+// Segments at depth d are contiguous position ranges, so one segment's lanes within a wave form
+// one run: a segmented inclusive scan (shuffles) leaves each run's union in the run's last lane,
+// which alone updates the block's LDS copy (blocks inside one segment, flushed to global once per
+// block) or the segment's record. A block covers kBoundsChunk x kB consecutive positions.
 __global__ __launch_bounds__(kB) void bounds_kernel(const int* __restrict__ list0, const float4* __restrict__ bounds,
                                                     int n, int d, uint32_t* __restrict__ segBounds)
 {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    int j = -1;
-    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    if (i < n) {
-        const Seg s = walk_pos(i, d, n);
-        if (s.exists) {
-            j = s.j;
-            const int tri = list0[i];
-            const float4 lo = bounds[2 * tri];
-            const float4 hi = bounds[2 * tri + 1];
-            v[0] = to_sortable(lo.x);
-            v[1] = to_sortable(lo.y);
-            v[2] = to_sortable(lo.z);
-            v[3] = to_sortable(hi.x);
-            v[4] = to_sortable(hi.y);
-            v[5] = to_sortable(hi.z);
+    __shared__ uint32_t lds[6];
+    const int base = static_cast<int>(blockIdx.x) * kB * kBoundsChunk;
+    const int last = min(base + kB * kBoundsChunk, n) - 1;
+    const Seg first = walk_pos(base, d, n);
+    const int j0 = first.exists && last < first.begin + first.count ? first.j : -1;
+    const bool uniform = j0 >= 0;
+    if (uniform && threadIdx.x < 6) {
+        lds[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+    }
+    __syncthreads();
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    for (int it = 0; it < kBoundsChunk; ++it) {
+        const int i = base + it * kB + static_cast<int>(threadIdx.x);
+        int j = -1;
+        uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+        if (i < n) {
+            const Seg s = uniform ? first : walk_pos(i, d, n);
+            if (s.exists) {
+                j = s.j;
+                const int tri = list0[i];
+                const float4 lo = bounds[2 * tri];
+                const float4 hi = bounds[2 * tri + 1];
+                v[0] = to_sortable(lo.x);
+                v[1] = to_sortable(lo.y);
+                v[2] = to_sortable(lo.z);
+                v[3] = to_sortable(hi.x);
+                v[4] = to_sortable(hi.y);
+                v[5] = to_sortable(hi.z);
+            }
+        }
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+            const int jo = __shfl_up(j, dd, kWave);
+            const bool take = lane >= dd && jo == j;
+            for (int k = 0; k < 6; ++k) {
+                const uint32_t o = __shfl_up(v[k], dd, kWave);
+                if (take) {
+                    v[k] = k < 3 ? min(v[k], o) : max(v[k], o);
+                }
+            }
+        }
+        const int jn = __shfl_down(j, 1, kWave);
+        if (j >= 0 && (lane == kWave - 1 || jn != j)) {
+            uint32_t* dst = uniform ? lds : segBounds + 6 * j;
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&dst[k], v[k]);
+                atomicMax(&dst[3 + k], v[3 + k]);
+            }
         }
     }
-    const int j0 = __shfl(j, 0, 64);
-    const bool uniform = __all(j == j0) && j0 >= 0;
-    if (uniform) {
-        for (int off = 32; off > 0; off >>= 1) {
-            for (int k = 0; k < 3; ++k) {
-                v[k] = min(v[k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[k]), off, 64)));
-                v[3 + k] = max(v[3 + k], static_cast<uint32_t>(__shfl_xor(static_cast<int>(v[3 + k]), off, 64)));
-            }
-        }
-        if ((threadIdx.x & 63) == 0) {
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(&segBounds[6 * j0 + k], v[k]);
-                atomicMax(&segBounds[6 * j0 + 3 + k], v[3 + k]);
-            }
-        }
-    } else if (j >= 0) {
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&segBounds[6 * j + k], v[k]);
-            atomicMax(&segBounds[6 * j + 3 + k], v[3 + k]);
+    __syncthreads();
+    if (uniform && threadIdx.x < 6) {
+        const int k = static_cast<int>(threadIdx.x);
+        if (k < 3) {
+            atomicMin(&segBounds[6 * j0 + k], lds[k]);
+        } else {
+            atomicMax(&segBounds[6 * j0 + k], lds[k]);
         }
     }
 }
@@ -544,7 +569,8 @@ bool build_bvh(PtScene* sc, const float* dTris, int n, hipStream_t stream)
         for (int d = 0; d <= depth && levelOk; ++d) {
             const int segs = 1 << d;
             hipLaunchKernelGGL(fill_bounds_kernel, dim3(g1(segs)), dim3(kB), 0, stream, segBounds, segs);
-            hipLaunchKernelGGL(bounds_kernel, dim3(g1(n)), dim3(kB), 0, stream, lists, bounds, n, d, segBounds);
+            hipLaunchKernelGGL(bounds_kernel, dim3(g1((n + kBoundsChunk - 1) / kBoundsChunk)), dim3(kB), 0, stream, lists,
+                               bounds, n, d, segBounds);
             hipLaunchKernelGGL(seg_kernel, dim3(g1(segs)), dim3(kB), 0, stream, segBounds, d, n, segs, lists, dTris,
                                sc->refNodes, reinterpret_cast<float*>(sc->nodes), sc->triPos, segAxis, rootBox);
             if (d == depth) {
