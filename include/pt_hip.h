@@ -441,6 +441,22 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
 /* Milliseconds of the last GPU BVH build (device time between events). */
 float ptSceneLastBuildMs(const PtScene* scene);
 
+/* Tree rebuilds that a caller can capture into a graph (no reference counterpart: the reference
+ * builds its tree once on the CPU). A scene built in a binned-SAH mode (1 or 2) keeps its
+ * triangles on the device; ptSceneRebuild enqueues the whole build again on them, with the
+ * 4-wide collapse in mode 1, into the scene's own buffers: a fixed sequence of launches on
+ * `stream` with no allocation and no host synchronisation. The tree is the one ptSceneBuild made
+ * (the build is deterministic). The workspace holds the build's temporaries for scenes of up to
+ * maxTriangles triangles (about 1.2 KB per triangle); rebuilds that share one workspace must be
+ * ordered on one stream. ptSceneRebuildCheck waits for `stream` and returns false, with
+ * pathTracerLastError set, if the last rebuild enqueued with the workspace failed or produced a
+ * tree of another shape than the scene's. */
+typedef struct PtBuildWorkspace PtBuildWorkspace;
+PtBuildWorkspace* ptBuildWorkspaceCreate(int maxTriangles);
+void ptBuildWorkspaceDestroy(PtBuildWorkspace* workspace);
+bool ptSceneRebuild(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream);
+bool ptSceneRebuildCheck(const PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream);
+
 /* Replaces the material table of a built scene (no reference counterpart; for differentiable
  * rendering and optimisation loops): same count; a material may not start or stop emitting.
  * The emissive list keeps its triangles; their radiance and the selection CDF are recomputed

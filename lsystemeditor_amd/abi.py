@@ -96,6 +96,10 @@ SIGNATURES = {
     "pathTracerDetachedRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "ptSceneReadLayout": (_b, [_p, _p, _p, _p]),
     "ptSceneReadWide": (_b, [_p, _p, _p, _p]),
+    "ptBuildWorkspaceCreate": (_p, [_i]),
+    "ptBuildWorkspaceDestroy": (None, [_p]),
+    "ptSceneRebuild": (_b, [_p, _p, _p]),
+    "ptSceneRebuildCheck": (_b, [_p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
     "ptEnvironmentDevice": (_p, [_p]),
     "ptEnvironmentReadTables": (_b, [_p, _p, _p]),
@@ -329,6 +333,14 @@ class Scene:
     def build_ms(self) -> float:
         return lib().ptSceneLastBuildMs(self.handle)
 
+    def rebuild(self, workspace: "BuildWorkspace", stream: int = 0) -> None:
+        """Enqueues the scene's binned-SAH build again on its device triangles (capturable)."""
+        check(lib().ptSceneRebuild(self.handle, workspace.handle, stream), "ptSceneRebuild")
+
+    def rebuild_check(self, workspace: "BuildWorkspace", stream: int = 0) -> None:
+        """Waits for the stream; raises if the last rebuild with `workspace` did not reproduce the tree."""
+        check(lib().ptSceneRebuildCheck(self.handle, workspace.handle, stream), "ptSceneRebuildCheck")
+
     def read_layout(self) -> tuple[np.ndarray, np.ndarray, int]:
         """Device layout: internal nodes (n-1, 16) float32, leaf slots (n, 12) float32, SAH levels."""
         n = self.triangle_count
@@ -369,6 +381,28 @@ class Scene:
     def close(self):
         if self.handle:
             lib().ptSceneDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BuildWorkspace:
+    """Temporaries of the binned-SAH build for scenes of up to max_triangles triangles
+    (ptBuildWorkspaceCreate); rebuilds that share one must be ordered on one stream."""
+
+    def __init__(self, max_triangles: int):
+        self.max_triangles = int(max_triangles)
+        self.handle = lib().ptBuildWorkspaceCreate(self.max_triangles)
+        if not self.handle:
+            raise RuntimeError(f"ptBuildWorkspaceCreate failed: {last_error()}")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ptBuildWorkspaceDestroy(self.handle)
             self.handle = None
 
     def __del__(self):

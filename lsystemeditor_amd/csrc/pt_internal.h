@@ -18,6 +18,22 @@ constexpr int kMaxSahDepth = 48;
 // levelStarts (optional) receives the first internal node of every level.
 bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
                       float rootOut[7], int* levelsOut, std::vector<int>* levelStarts);
+// Device workspace of the binned-SAH build for scenes of up to `capacity` triangles.
+struct PtSahWork;
+PtSahWork* pt_sah_work_create(int capacity);
+void pt_sah_work_destroy(PtSahWork* w);
+// Enqueues a whole binned-SAH build of n <= capacity triangles into nodes/triPos/triShade (as
+// pt_build_bvh_sah), and when wide is not null the 4-wide collapse into wide (room for
+// wideCapacity nodes): a fixed sequence of launches with no allocation and no host
+// synchronisation, so a caller can capture it into a graph. The root box and reference, the level
+// starts and any error stay in the workspace until pt_sah_result reads them.
+bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade,
+                    float4* wide, int wideCapacity, hipStream_t stream);
+// Waits for the stream and checks the last build enqueued with w (n triangles): false with the
+// error set if it did not produce a complete tree; otherwise the outputs of pt_build_bvh_sah and
+// the 4-wide node count (each optional).
+bool pt_sah_result(PtSahWork* w, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
+                   std::vector<int>* levelStarts, int* wideCount);
 // 4-wide tree collapsed from a level-numbered binary tree of internalCount internal nodes
 // (pt_sah.hip): wideOut (device, 8 float4 per node) and its node count; the wide root is node 0
 // where the binary root is internal.

@@ -25,6 +25,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -67,6 +68,21 @@ struct SplitRec {
     int leftCount;
     int internal;  // 1 when the segment is an internal node
 };
+
+// Device-resident state of a build. The host enqueues a fixed number of levels (enough for the
+// depth cap); every per-level kernel reads the live segment count here, and a level with no
+// segments does nothing, so a build needs no host readback.
+struct SahCtl {
+    int segCount;   // segments of the current level
+    int levelBase;  // internal nodes numbered before the current level
+    int level;      // levels that produced internal nodes so far
+    int internals;  // internal segments of the current level
+    int error;      // 1: segment table overflow, 2: the 4-wide tree does not fit its buffer
+    int wideCount;  // 4-wide nodes of the collapse
+    int levelStart[kMaxSahDepth + 2];
+    int wideStart[kMaxSahDepth + 2];
+};
+constexpr int kSahLevels = kMaxSahDepth + 1;  // leaves sit at depth <= kMaxSahDepth
 
 // Grid-stride over the triangles (at most kPrepBlocks blocks): per-triangle bounds, centroid and
 // shading record; the centroid bounds reduce per lane, per wave (shuffles) and per block (LDS),
@@ -154,10 +170,10 @@ __global__ void morton_kernel(const float4* __restrict__ cent, int n, const uint
     vals[i] = i;
 }
 
-__global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, int segs)
+__global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, const SahCtl* __restrict__ ctl)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= segs * kSegWords) {
+    if (i >= ctl->segCount * kSegWords) {
         return;
     }
     const int w = i % kSegWords;
@@ -172,9 +188,12 @@ __global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, int segs)
 __global__ __launch_bounds__(kB) void seg_bounds_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
                                                         const float4* __restrict__ bounds,
                                                         const float4* __restrict__ cent, int n,
-                                                        uint32_t* __restrict__ segStats)
+                                                        uint32_t* __restrict__ segStats, const SahCtl* __restrict__ ctl)
 {
     __shared__ uint32_t lds[kSegWords];
+    if (ctl->segCount == 0) {
+        return;
+    }
     const int base = static_cast<int>(blockIdx.x) * kB * kChunk;
     const int last = min(base + kB * kChunk, n) - 1;
     const int s0 = posSeg[base];
@@ -254,13 +273,14 @@ __device__ __forceinline__ int ceil_log2(int c)
 }
 
 // One thread per segment: box into the parent (or the root), leaves, bin setup.
-__global__ void seg_setup_kernel(const SegRec* __restrict__ segs, int segCount, const uint32_t* __restrict__ segStats,
+__global__ void seg_setup_kernel(const SegRec* __restrict__ segs, const SahCtl* __restrict__ ctl,
+                                 const uint32_t* __restrict__ segStats,
                                  const int* __restrict__ perm, const float* __restrict__ tris, int level,
                                  float* __restrict__ nodes, float4* __restrict__ triPos, float* __restrict__ rootBox,
                                  SplitRec* __restrict__ split, uint32_t* __restrict__ bins)
 {
     const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= segCount) {
+    if (j >= ctl->segCount) {
         return;
     }
     const SegRec sg = segs[j];
@@ -331,9 +351,13 @@ __device__ __forceinline__ int bin_of(const SplitRec& r, const float4& c)
 // lane, and only those lanes update the histogram.
 __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
                                                  const float4* __restrict__ bounds, const float4* __restrict__ cent,
-                                                 const SplitRec* __restrict__ split, int n, uint32_t* __restrict__ bins)
+                                                 const SplitRec* __restrict__ split, int n, uint32_t* __restrict__ bins,
+                                                 const SahCtl* __restrict__ ctl)
 {
     __shared__ uint32_t lds[kBins * kBinWords];
+    if (ctl->segCount == 0) {
+        return;
+    }
     const int base = static_cast<int>(blockIdx.x * blockDim.x);
     const int i = base + static_cast<int>(threadIdx.x);
     const int last = min(base + kB, n) - 1;
@@ -429,11 +453,17 @@ __device__ __forceinline__ float half_area(float x0, float y0, float z0, float x
 
 // One thread per internal segment: lowest-cost bin boundary (left-to-right sweep of prefix areas,
 // right-to-left of suffix areas); the middle position when no boundary separates the segment.
-__global__ void split_kernel(const SegRec* __restrict__ segs, int segCount, const uint32_t* __restrict__ bins,
-                             SplitRec* __restrict__ split, int* __restrict__ internalFlag)
+// Also clears internalFlag past the level's segments, so the scan over all maxSegs entries counts
+// this level's internal segments only.
+__global__ void split_kernel(const SegRec* __restrict__ segs, const SahCtl* __restrict__ ctl, int maxSegs,
+                             const uint32_t* __restrict__ bins, SplitRec* __restrict__ split,
+                             int* __restrict__ internalFlag)
 {
     const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= segCount) {
+    if (j >= ctl->segCount) {
+        if (j < maxSegs) {
+            internalFlag[j] = 0;
+        }
         return;
     }
     SplitRec r = split[j];
@@ -505,16 +535,16 @@ __global__ void split_kernel(const SegRec* __restrict__ segs, int segCount, cons
 }
 
 // Internal nodes of this level get consecutive indices; each emits its two child segments.
-__global__ void emit_kernel(const SegRec* __restrict__ segs, int segCount, const SplitRec* __restrict__ split,
-                            const int* __restrict__ rank, int levelBase, float* __restrict__ nodes,
+__global__ void emit_kernel(const SegRec* __restrict__ segs, const SahCtl* __restrict__ ctl,
+                            const SplitRec* __restrict__ split, const int* __restrict__ rank, float* __restrict__ nodes,
                             float* __restrict__ rootBox, SegRec* __restrict__ next)
 {
     const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= segCount || split[j].internal == 0) {
+    if (ctl->internals == 0 || j >= ctl->segCount || split[j].internal == 0) {
         return;
     }
     const SegRec sg = segs[j];
-    const int idx = levelBase + rank[j];
+    const int idx = ctl->levelBase + rank[j];
     if (sg.parent < 0) {
         rootBox[6] = __int_as_float(idx);
     } else {
@@ -527,10 +557,10 @@ __global__ void emit_kernel(const SegRec* __restrict__ segs, int segCount, const
 
 __global__ void flag_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg, const SegRec* __restrict__ segs,
                             const SplitRec* __restrict__ split, const float4* __restrict__ cent, int n,
-                            int* __restrict__ flags)
+                            int* __restrict__ flags, const SahCtl* __restrict__ ctl)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) {
+    if (i >= n || ctl->segCount == 0) {
         return;
     }
     const int s = posSeg[i];
@@ -549,14 +579,14 @@ __global__ void sah_scatter_kernel(const int* __restrict__ perm, const int* __re
                                    const SegRec* __restrict__ segs, const SplitRec* __restrict__ split,
                                    const int* __restrict__ rank, const int* __restrict__ flags,
                                    const int* __restrict__ scan, int n, int* __restrict__ permOut,
-                                   int* __restrict__ posSegOut)
+                                   int* __restrict__ posSegOut, const SahCtl* __restrict__ ctl)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) {
+    if (i >= n || ctl->segCount == 0) {
         return;
     }
     const int s = posSeg[i];
-    if (s < 0 || split[s].internal == 0) {
+    if (s < 0 || ctl->internals == 0 || split[s].internal == 0) {
         permOut[i] = perm[i];
         posSegOut[i] = -1;
         return;
@@ -568,6 +598,71 @@ __global__ void sah_scatter_kernel(const int* __restrict__ perm, const int* __re
     const int pos = left ? begin + before : begin + lc + (i - begin - before);
     permOut[pos] = perm[i];
     posSegOut[pos] = 2 * rank[s] + (left ? 0 : 1);
+}
+
+// Start of a build: one root segment holding every triangle, empty scene bounds.
+__global__ void sah_init_kernel(SahCtl* __restrict__ ctl, SegRec* __restrict__ segs, uint32_t* __restrict__ gcb, int n)
+{
+    if (threadIdx.x == 0) {
+        ctl->segCount = 1;
+        ctl->levelBase = 0;
+        ctl->level = 0;
+        ctl->internals = 0;
+        ctl->error = 0;
+        ctl->wideCount = 0;
+        segs[0] = SegRec{0, n, -1, 0};
+    }
+    if (threadIdx.x < 6) {
+        gcb[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+    }
+}
+
+// After the scan of internalFlag: the level's internal segments, and the first node number of a
+// level that has any.
+__global__ void level_tail_kernel(SahCtl* __restrict__ ctl, const int* __restrict__ rank,
+                                  const int* __restrict__ internalFlag, int maxSegs)
+{
+    const int sc = ctl->segCount;
+    int internals = sc > 0 ? rank[sc - 1] + internalFlag[sc - 1] : 0;
+    if (2 * static_cast<long long>(internals) > maxSegs) {
+        ctl->error = 1;
+        internals = 0;
+    }
+    ctl->internals = internals;
+    if (internals > 0) {
+        ctl->levelStart[ctl->level] = ctl->levelBase;
+    }
+}
+
+__global__ void level_advance_kernel(SahCtl* __restrict__ ctl)
+{
+    const int internals = ctl->internals;
+    if (internals > 0) {
+        ctl->levelBase += internals;
+        ctl->level += 1;
+    }
+    ctl->segCount = 2 * internals;
+    ctl->internals = 0;
+}
+
+// First 4-wide node of every level (the even levels hold the wide nodes) from the binary tree's
+// level starts, with internalCount as the end sentinel.
+__global__ void wide_starts_kernel(SahCtl* __restrict__ ctl, int internalCount, int wideCapacity)
+{
+    const int levels = ctl->level;
+    ctl->levelStart[levels] = internalCount;
+    int count = 0;
+    for (int L = 0; L < levels; ++L) {
+        ctl->wideStart[L] = count;
+        if ((L & 1) == 0) {
+            count += ctl->levelStart[L + 1] - ctl->levelStart[L];
+        }
+    }
+    ctl->wideStart[levels] = count;
+    ctl->wideCount = count;
+    if (count > wideCapacity) {
+        ctl->error = 2;
+    }
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v)
@@ -607,168 +702,6 @@ void dfree(T*& p)
 
 } // namespace
 
-bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
-                      float rootOut[7], int* levelsOut, std::vector<int>* levelStarts)
-{
-    if (levelStarts != nullptr) {
-        levelStarts->clear();
-    }
-    float4* bounds = nullptr;
-    float4* cent = nullptr;
-    uint32_t* gcb = nullptr;
-    uint32_t* keys = nullptr;
-    uint32_t* keysOut = nullptr;
-    int* vals = nullptr;
-    int* perm = nullptr;
-    int* permAlt = nullptr;
-    int* posSeg = nullptr;
-    int* posSegAlt = nullptr;
-    SegRec* segs = nullptr;
-    SegRec* segsNext = nullptr;
-    SplitRec* split = nullptr;
-    uint32_t* segStats = nullptr;
-    uint32_t* bins = nullptr;
-    int* internalFlag = nullptr;
-    int* rank = nullptr;
-    int* flags = nullptr;
-    int* scan = nullptr;
-    float* rootBox = nullptr;
-    void* tmp = nullptr;
-    bool ok = false;
-    const size_t maxSegs = static_cast<size_t>(std::max(1, n)); // segments of a level <= triangles
-    do {
-        if (!dalloc(&bounds, 2 * static_cast<size_t>(n)) || !dalloc(&cent, n) || !dalloc(&gcb, 6) ||
-            !dalloc(&keys, n) || !dalloc(&keysOut, n) || !dalloc(&vals, n) || !dalloc(&perm, n) ||
-            !dalloc(&permAlt, n) || !dalloc(&posSeg, n) || !dalloc(&posSegAlt, n) || !dalloc(&segs, maxSegs) ||
-            !dalloc(&segsNext, maxSegs) || !dalloc(&split, maxSegs) ||
-            !dalloc(&segStats, static_cast<size_t>(kSegWords) * maxSegs) ||
-            !dalloc(&bins, static_cast<size_t>(kBins * kBinWords) * maxSegs) || !dalloc(&internalFlag, maxSegs) ||
-            !dalloc(&rank, maxSegs) || !dalloc(&flags, n) || !dalloc(&scan, n) || !dalloc(&rootBox, 8)) {
-            ptSetError("SAH build: device allocation failed");
-            break;
-        }
-        size_t sortBytes = 0, scanBytes = 0, segScanBytes = 0;
-        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keysOut, vals, perm, n, 0, 30, stream);
-        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, flags, scan, n, stream);
-        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, segScanBytes, internalFlag, rank, static_cast<int>(maxSegs),
-                                               stream);
-        const size_t tmpBytes = std::max(sortBytes, std::max(scanBytes, segScanBytes));
-        if (hipMalloc(&tmp, tmpBytes) != hipSuccess) {
-            ptSetError("SAH build: temporary allocation failed");
-            break;
-        }
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kB), 0, stream, gcb, 6, 0xFFFFFFFFu, 0u, 3);
-        hipLaunchKernelGGL(sah_prep_kernel, dim3(std::min(grid(n), kPrepBlocks)), dim3(kB), 0, stream, dTris, n,
-                           bounds, cent, triShade, gcb);
-        hipLaunchKernelGGL(morton_kernel, dim3(grid(n)), dim3(kB), 0, stream, cent, n, gcb, keys, vals);
-        size_t bytes = tmpBytes;
-        if (hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keysOut, vals, perm, n, 0, 30, stream) != hipSuccess) {
-            ptSetError("SAH build: radix sort failed");
-            break;
-        }
-        hipLaunchKernelGGL(fill_int_kernel, dim3(grid(n)), dim3(kB), 0, stream, posSeg, n, 0);
-        const SegRec root{0, n, -1, 0};
-        if (hipMemcpyAsync(segs, &root, sizeof(root), hipMemcpyHostToDevice, stream) != hipSuccess) {
-            ptSetError("SAH build: upload failed");
-            break;
-        }
-        int segCount = 1;
-        int levelBase = 0;
-        int level = 0;
-        bool levelOk = true;
-        while (segCount > 0 && levelOk) {
-            hipLaunchKernelGGL(seg_reset_kernel, dim3(grid(static_cast<long long>(segCount) * kSegWords)), dim3(kB), 0,
-                               stream, segStats, segCount);
-            hipLaunchKernelGGL(seg_bounds_kernel, dim3(chunks(n, kChunk)), dim3(kB), 0, stream, perm, posSeg, bounds,
-                               cent, n, segStats);
-            hipLaunchKernelGGL(seg_setup_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, segStats,
-                               perm, dTris, level, reinterpret_cast<float*>(nodes), triPos, rootBox, split, bins);
-            hipLaunchKernelGGL(bin_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, bounds, cent, split, n,
-                               bins);
-            hipLaunchKernelGGL(split_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, bins, split,
-                               internalFlag);
-            bytes = tmpBytes;
-            levelOk = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, internalFlag, rank, segCount, stream) == hipSuccess;
-            int tail[2] = {0, 0};
-            levelOk = levelOk &&
-                      hipMemcpyAsync(&tail[0], rank + segCount - 1, sizeof(int), hipMemcpyDeviceToHost, stream) ==
-                          hipSuccess &&
-                      hipMemcpyAsync(&tail[1], internalFlag + segCount - 1, sizeof(int), hipMemcpyDeviceToHost,
-                                     stream) == hipSuccess &&
-                      hipStreamSynchronize(stream) == hipSuccess;
-            if (!levelOk) {
-                break;
-            }
-            const int internals = tail[0] + tail[1];
-            if (internals == 0) {
-                break;
-            }
-            if (2 * static_cast<size_t>(internals) > maxSegs) {
-                ptSetError("SAH build: segment table overflow");
-                levelOk = false;
-                break;
-            }
-            if (levelStarts != nullptr) {
-                levelStarts->push_back(levelBase);
-            }
-            hipLaunchKernelGGL(emit_kernel, dim3(grid(segCount)), dim3(kB), 0, stream, segs, segCount, split, rank,
-                               levelBase, reinterpret_cast<float*>(nodes), rootBox, segsNext);
-            hipLaunchKernelGGL(flag_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, split, cent, n,
-                               flags);
-            bytes = tmpBytes;
-            levelOk = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flags, scan, n, stream) == hipSuccess;
-            hipLaunchKernelGGL(sah_scatter_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, split, rank,
-                               flags, scan, n, permAlt, posSegAlt);
-            std::swap(perm, permAlt);
-            std::swap(posSeg, posSegAlt);
-            std::swap(segs, segsNext);
-            levelBase += internals;
-            segCount = 2 * internals;
-            ++level;
-        }
-        if (!levelOk || hipGetLastError() != hipSuccess) {
-            if (levelOk) {
-                ptSetError("SAH build: kernel launch failed");
-            }
-            break;
-        }
-        if (levelBase != n - 1) {
-            ptSetError("SAH build: %d internal nodes for %d triangles", levelBase, n);
-            break;
-        }
-        if (hipMemcpyAsync(rootOut, rootBox, 7 * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess) {
-            ptSetError("SAH build: synchronisation failed");
-            break;
-        }
-        *levelsOut = level + 1;
-        ok = true;
-    } while (false);
-    dfree(bounds);
-    dfree(cent);
-    dfree(gcb);
-    dfree(keys);
-    dfree(keysOut);
-    dfree(vals);
-    dfree(perm);
-    dfree(permAlt);
-    dfree(posSeg);
-    dfree(posSegAlt);
-    dfree(segs);
-    dfree(segsNext);
-    dfree(split);
-    dfree(segStats);
-    dfree(bins);
-    dfree(internalFlag);
-    dfree(rank);
-    dfree(flags);
-    dfree(scan);
-    dfree(rootBox);
-    if (tmp != nullptr) {
-        (void)hipFree(tmp);
-    }
-    return ok;
-}
 
 // ---- 4-wide tree -------------------------------------------------------------------------
 //
@@ -801,13 +734,15 @@ __device__ void binary_children(const float4* __restrict__ nodes, int node, Chil
 // levelStart[L]: first internal node of level L (L < levels; levelStart[levels] = internal count);
 // wideStart[L]: first wide node made from level L (even L).
 __global__ void wide_collapse_kernel(const float4* __restrict__ nodes, int internalCount,
-                                     const int* __restrict__ levelStart, const int* __restrict__ wideStart, int levels,
+                                     const int* __restrict__ levelStart, const int* __restrict__ wideStart,
+                                     const int* __restrict__ levelsPtr, const int* __restrict__ errorPtr,
                                      float4* __restrict__ wide)
 {
     const int node = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (node >= internalCount) {
+    if (node >= internalCount || (errorPtr != nullptr && *errorPtr != 0)) {
         return;
     }
+    const int levels = *levelsPtr;
     int level = 0;
     while (level + 1 < levels && levelStart[level + 1] <= node) {
         ++level;
@@ -851,6 +786,229 @@ __global__ void wide_collapse_kernel(const float4* __restrict__ nodes, int inter
 
 } // namespace
 
+struct PtSahWork {
+    int capacity = 0;  // triangles
+    float4* bounds = nullptr;
+    float4* cent = nullptr;
+    uint32_t* gcb = nullptr;
+    uint32_t* keys = nullptr;
+    uint32_t* keysOut = nullptr;
+    int* vals = nullptr;
+    int* perm = nullptr;
+    int* permAlt = nullptr;
+    int* posSeg = nullptr;
+    int* posSegAlt = nullptr;
+    SegRec* segs = nullptr;
+    SegRec* segsNext = nullptr;
+    SplitRec* split = nullptr;
+    uint32_t* segStats = nullptr;
+    uint32_t* bins = nullptr;
+    int* internalFlag = nullptr;
+    int* rank = nullptr;
+    int* flags = nullptr;
+    int* scan = nullptr;
+    float* rootBox = nullptr;
+    SahCtl* ctl = nullptr;
+    void* tmp = nullptr;
+    size_t tmpBytes = 0;
+};
+
+void pt_sah_work_destroy(PtSahWork* w)
+{
+    if (w == nullptr) {
+        return;
+    }
+    dfree(w->bounds);
+    dfree(w->cent);
+    dfree(w->gcb);
+    dfree(w->keys);
+    dfree(w->keysOut);
+    dfree(w->vals);
+    dfree(w->perm);
+    dfree(w->permAlt);
+    dfree(w->posSeg);
+    dfree(w->posSegAlt);
+    dfree(w->segs);
+    dfree(w->segsNext);
+    dfree(w->split);
+    dfree(w->segStats);
+    dfree(w->bins);
+    dfree(w->internalFlag);
+    dfree(w->rank);
+    dfree(w->flags);
+    dfree(w->scan);
+    dfree(w->rootBox);
+    dfree(w->ctl);
+    if (w->tmp != nullptr) {
+        (void)hipFree(w->tmp);
+    }
+    delete w;
+}
+
+PtSahWork* pt_sah_work_create(int capacity)
+{
+    if (capacity < 1) {
+        ptSetError("SAH build workspace: capacity %d < 1", capacity);
+        return nullptr;
+    }
+    PtSahWork* w = new PtSahWork;
+    w->capacity = capacity;
+    const size_t n = static_cast<size_t>(capacity);
+    const size_t maxSegs = n;  // segments of a level <= triangles
+    if (!dalloc(&w->bounds, 2 * n) || !dalloc(&w->cent, n) || !dalloc(&w->gcb, 6) || !dalloc(&w->keys, n) ||
+        !dalloc(&w->keysOut, n) || !dalloc(&w->vals, n) || !dalloc(&w->perm, n) || !dalloc(&w->permAlt, n) ||
+        !dalloc(&w->posSeg, n) || !dalloc(&w->posSegAlt, n) || !dalloc(&w->segs, maxSegs) ||
+        !dalloc(&w->segsNext, maxSegs) || !dalloc(&w->split, maxSegs) ||
+        !dalloc(&w->segStats, static_cast<size_t>(kSegWords) * maxSegs) ||
+        !dalloc(&w->bins, static_cast<size_t>(kBins * kBinWords) * maxSegs) || !dalloc(&w->internalFlag, maxSegs) ||
+        !dalloc(&w->rank, maxSegs) || !dalloc(&w->flags, n) || !dalloc(&w->scan, n) || !dalloc(&w->rootBox, 8) ||
+        !dalloc(&w->ctl, 1)) {
+        ptSetError("SAH build: device allocation failed");
+        pt_sah_work_destroy(w);
+        return nullptr;
+    }
+    size_t sortBytes = 0, scanBytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, w->keys, w->keysOut, w->vals, w->perm, capacity, 0,
+                                             30);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, w->flags, w->scan, capacity);
+    w->tmpBytes = std::max(sortBytes, scanBytes);
+    if (hipMalloc(&w->tmp, std::max<size_t>(w->tmpBytes, 1)) != hipSuccess) {
+        ptSetError("SAH build: temporary allocation failed");
+        pt_sah_work_destroy(w);
+        return nullptr;
+    }
+    return w;
+}
+
+bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade,
+                    float4* wide, int wideCapacity, hipStream_t stream)
+{
+    if (w == nullptr || n < 1 || n > w->capacity) {
+        ptSetError("SAH build: %d triangles for a workspace of %d", n, w != nullptr ? w->capacity : 0);
+        return false;
+    }
+    size_t sortBytes = 0, scanBytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, w->keys, w->keysOut, w->vals, w->perm, n, 0, 30,
+                                             stream);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, w->flags, w->scan, n, stream);
+    if (std::max(sortBytes, scanBytes) > w->tmpBytes) {
+        ptSetError("SAH build: temporary storage too small");
+        return false;
+    }
+    const int maxSegs = n;
+    int* perm = w->perm;
+    int* permAlt = w->permAlt;
+    int* posSeg = w->posSeg;
+    int* posSegAlt = w->posSegAlt;
+    SegRec* segs = w->segs;
+    SegRec* segsNext = w->segsNext;
+    SahCtl* ctl = w->ctl;
+    hipLaunchKernelGGL(sah_init_kernel, dim3(1), dim3(64), 0, stream, ctl, segs, w->gcb, n);
+    hipLaunchKernelGGL(sah_prep_kernel, dim3(std::min(grid(n), kPrepBlocks)), dim3(kB), 0, stream, dTris, n,
+                       w->bounds, w->cent, triShade, w->gcb);
+    hipLaunchKernelGGL(morton_kernel, dim3(grid(n)), dim3(kB), 0, stream, w->cent, n, w->gcb, w->keys, w->vals);
+    size_t bytes = w->tmpBytes;
+    if (hipcub::DeviceRadixSort::SortPairs(w->tmp, bytes, w->keys, w->keysOut, w->vals, perm, n, 0, 30, stream) !=
+        hipSuccess) {
+        ptSetError("SAH build: radix sort failed");
+        return false;
+    }
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid(n)), dim3(kB), 0, stream, posSeg, n, 0);
+    bool ok = true;
+    // every level the depth cap allows; levels past the last one find no segments and do nothing
+    for (int level = 0; level < kSahLevels && ok; ++level) {
+        hipLaunchKernelGGL(seg_reset_kernel, dim3(grid(static_cast<long long>(maxSegs) * kSegWords)), dim3(kB), 0,
+                           stream, w->segStats, ctl);
+        hipLaunchKernelGGL(seg_bounds_kernel, dim3(chunks(n, kChunk)), dim3(kB), 0, stream, perm, posSeg, w->bounds,
+                           w->cent, n, w->segStats, ctl);
+        hipLaunchKernelGGL(seg_setup_kernel, dim3(grid(maxSegs)), dim3(kB), 0, stream, segs, ctl, w->segStats, perm,
+                           dTris, level, reinterpret_cast<float*>(nodes), triPos, w->rootBox, w->split, w->bins);
+        hipLaunchKernelGGL(bin_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, w->bounds, w->cent, w->split,
+                           n, w->bins, ctl);
+        hipLaunchKernelGGL(split_kernel, dim3(grid(maxSegs)), dim3(kB), 0, stream, segs, ctl, maxSegs, w->bins,
+                           w->split, w->internalFlag);
+        bytes = w->tmpBytes;
+        ok = hipcub::DeviceScan::ExclusiveSum(w->tmp, bytes, w->internalFlag, w->rank, maxSegs, stream) == hipSuccess;
+        hipLaunchKernelGGL(level_tail_kernel, dim3(1), dim3(1), 0, stream, ctl, w->rank, w->internalFlag, maxSegs);
+        hipLaunchKernelGGL(emit_kernel, dim3(grid(maxSegs)), dim3(kB), 0, stream, segs, ctl, w->split, w->rank,
+                           reinterpret_cast<float*>(nodes), w->rootBox, segsNext);
+        hipLaunchKernelGGL(flag_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, w->split, w->cent, n,
+                           w->flags, ctl);
+        bytes = w->tmpBytes;
+        ok = ok && hipcub::DeviceScan::ExclusiveSum(w->tmp, bytes, w->flags, w->scan, n, stream) == hipSuccess;
+        hipLaunchKernelGGL(sah_scatter_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, segs, w->split,
+                           w->rank, w->flags, w->scan, n, permAlt, posSegAlt, ctl);
+        hipLaunchKernelGGL(level_advance_kernel, dim3(1), dim3(1), 0, stream, ctl);
+        std::swap(perm, permAlt);
+        std::swap(posSeg, posSegAlt);
+        std::swap(segs, segsNext);
+    }
+    if (ok && wide != nullptr && n > 1) {
+        hipLaunchKernelGGL(wide_starts_kernel, dim3(1), dim3(1), 0, stream, ctl, n - 1, wideCapacity);
+        // device addresses of the control block's fields (host pointer arithmetic only)
+        char* c = reinterpret_cast<char*>(ctl);
+        hipLaunchKernelGGL(wide_collapse_kernel, dim3(grid(n - 1)), dim3(kB), 0, stream,
+                           reinterpret_cast<const float4*>(nodes), n - 1,
+                           reinterpret_cast<const int*>(c + offsetof(SahCtl, levelStart)),
+                           reinterpret_cast<const int*>(c + offsetof(SahCtl, wideStart)),
+                           reinterpret_cast<const int*>(c + offsetof(SahCtl, level)),
+                           reinterpret_cast<const int*>(c + offsetof(SahCtl, error)), wide);
+    }
+    if (!ok || hipGetLastError() != hipSuccess) {
+        ptSetError("SAH build: kernel launch failed");
+        return false;
+    }
+    return true;
+}
+
+bool pt_sah_result(PtSahWork* w, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
+                   std::vector<int>* levelStarts, int* wideCount)
+{
+    SahCtl ctl{};
+    if (hipMemcpyAsync(&ctl, w->ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        (rootOut != nullptr &&
+         hipMemcpyAsync(rootOut, w->rootBox, 7 * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess) ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        ptSetError("SAH build: synchronisation failed");
+        return false;
+    }
+    if (ctl.error == 1) {
+        ptSetError("SAH build: segment table overflow");
+        return false;
+    }
+    if (ctl.error == 2) {
+        ptSetError("SAH build: the 4-wide tree (%d nodes) does not fit its buffer", ctl.wideCount);
+        return false;
+    }
+    if (ctl.levelBase != n - 1 || ctl.segCount != 0) {
+        ptSetError("SAH build: %d internal nodes for %d triangles", ctl.levelBase, n);
+        return false;
+    }
+    if (levelsOut != nullptr) {
+        *levelsOut = ctl.level + 1;
+    }
+    if (levelStarts != nullptr) {
+        levelStarts->assign(ctl.levelStart, ctl.levelStart + ctl.level);
+    }
+    if (wideCount != nullptr) {
+        *wideCount = ctl.wideCount;
+    }
+    return true;
+}
+
+bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade, hipStream_t stream,
+                      float rootOut[7], int* levelsOut, std::vector<int>* levelStarts)
+{
+    PtSahWork* w = pt_sah_work_create(n);
+    if (w == nullptr) {
+        return false;
+    }
+    const bool ok = pt_sah_enqueue(w, dTris, n, nodes, triPos, triShade, nullptr, 0, stream) &&
+                    pt_sah_result(w, n, stream, rootOut, levelsOut, levelStarts, nullptr);
+    pt_sah_work_destroy(w);
+    return ok;
+}
+
 bool pt_build_wide(const float4* nodes, int internalCount, const std::vector<int>& levelStarts, float4** wideOut,
                    int* wideCount, hipStream_t stream)
 {
@@ -871,6 +1029,7 @@ bool pt_build_wide(const float4* nodes, int internalCount, const std::vector<int
         }
     }
     wideStart[static_cast<size_t>(levels)] = count;
+    wideStart.push_back(levels);  // read by the kernel through a device pointer
     int* dStarts = nullptr;
     int* dWide = nullptr;
     float4* wide = nullptr;
@@ -883,7 +1042,7 @@ bool pt_build_wide(const float4* nodes, int internalCount, const std::vector<int
                              stream) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(wide_collapse_kernel, dim3(static_cast<unsigned>((internalCount + 255) / 256)), dim3(256), 0,
-                           stream, nodes, internalCount, dStarts, dWide, levels, wide);
+                           stream, nodes, internalCount, dStarts, dWide, dWide + levels + 1, nullptr, wide);
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(stream) == hipSuccess;
     }
     if (dStarts != nullptr) {

@@ -465,6 +465,8 @@ struct PtScene {
     float4* wide = nullptr; // 4-wide tree (mode 1)
     int wideCount = 0;
     int levels = 0;
+    float* srcTris = nullptr; // the triangles (25-word records) the SAH modes keep for ptSceneRebuild
+    int triangleCount = 0;
     // host copies for ptSceneUpdateMaterials: materials, and per emissive entry its area and
     // material index
     std::vector<MaterialGpu> matsHost;
@@ -488,6 +490,9 @@ struct PtScene {
         release(emTris);
         release(textures);
         release(refNodes);
+        release(srcTris);
+        srcTris = nullptr;
+        triangleCount = 0;
         d_desc = nullptr;
         nodes = nullptr;
         triPos = nullptr;
@@ -748,7 +753,12 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
             sc->refCount = ok ? static_cast<uint32_t>(2 * n - 1) : 0u;
         }
     }
-    release(dTris);
+    if (ok && n > 0 && (sc->buildMode == 1 || sc->buildMode == 2)) {
+        sc->srcTris = dTris;  // kept for ptSceneRebuild
+        sc->triangleCount = static_cast<int>(n);
+    } else {
+        release(dTris);
+    }
     ok = ok && upload(&sc->materials, mats.data(), mats.size(), stream) &&
          upload(&sc->textures, textures, textureCount, stream) && upload(&sc->emIdx, emIdx.data(), emIdx.size(), stream) &&
          upload(&sc->emCdf, emCdf.data(), emCdf.size(), stream) && upload(&sc->emTris, emTris.data(), emTris.size(), stream);
@@ -938,6 +948,57 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
 }
 
 float ptSceneLastBuildMs(const PtScene* scene) { return scene != nullptr ? scene->buildMs : 0.0f; }
+
+PtBuildWorkspace* ptBuildWorkspaceCreate(int maxTriangles)
+{
+    return reinterpret_cast<PtBuildWorkspace*>(pt_sah_work_create(maxTriangles));
+}
+
+void ptBuildWorkspaceDestroy(PtBuildWorkspace* workspace)
+{
+    pt_sah_work_destroy(reinterpret_cast<PtSahWork*>(workspace));
+}
+
+bool ptSceneRebuild(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
+{
+    if (scene == nullptr || workspace == nullptr) {
+        ptSetError("ptSceneRebuild: null scene or workspace");
+        return false;
+    }
+    if (scene->srcTris == nullptr || scene->triangleCount < 1) {
+        ptSetError("ptSceneRebuild: the scene was not built with a binned-SAH build mode");
+        return false;
+    }
+    return pt_sah_enqueue(reinterpret_cast<PtSahWork*>(workspace), scene->srcTris, scene->triangleCount,
+                          scene->nodes, scene->triPos, scene->triShade, scene->buildMode == 1 ? scene->wide : nullptr,
+                          scene->wideCount, stream);
+}
+
+bool ptSceneRebuildCheck(const PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
+{
+    if (scene == nullptr || workspace == nullptr || scene->srcTris == nullptr) {
+        ptSetError("ptSceneRebuildCheck: null scene or workspace, or a scene without a binned-SAH build");
+        return false;
+    }
+    const int n = scene->triangleCount;
+    float root[7] = {0};
+    int levels = 0, wideCount = 0;
+    if (!pt_sah_result(reinterpret_cast<PtSahWork*>(workspace), n, stream, root, &levels, nullptr, &wideCount)) {
+        return false;
+    }
+    int rootRef = 0;
+    std::memcpy(&rootRef, &root[6], sizeof(int));
+    bool same = levels == scene->levels && rootRef == scene->desc.rootRef &&
+                (scene->buildMode != 1 || n < 2 || wideCount == scene->wideCount);
+    for (int k = 0; k < 3; ++k) {
+        same = same && root[k] == scene->desc.rootMin[k] && root[3 + k] == scene->desc.rootMax[k];
+    }
+    if (!same) {
+        ptSetError("ptSceneRebuildCheck: the rebuilt tree's root or shape differs from the scene's");
+        return false;
+    }
+    return true;
+}
 
 } // extern "C"
 
