@@ -237,8 +237,10 @@ def facade_bench(spec) -> dict:
 def batch_bench(bvh: str) -> dict:
     """C3: 64 bushes (seeds 0..63, iterations 3..7, 128^2..1024^2, 64 spp, depth 8) rendered by one
     batch wavefront replayed from a graph, against the per-scene graph (every scene's own render
-    back to back) and the same 64 renders issued one by one through the adaptive entry point.
-    Paths-in-flight bounds are autotuned (per-scene: on the largest scene; batch: on the batch)."""
+    back to back) and the same 64 renders issued one by one through the adaptive entry point;
+    then every scene's tree rebuild and the batch wavefront in one graph, against the sequential
+    renders plus the eager builds. Paths-in-flight bounds are autotuned (per-scene: on the
+    largest scene; batch: on the batch)."""
     import torch
     from lsystemeditor_amd import batch
 
@@ -272,11 +274,33 @@ def batch_bench(bvh: str) -> dict:
     br.replay_batched()
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    # every scene's tree rebuild and the batch wavefront in one graph
+    full = None
+    if bvh in ("sah", "sah-binary"):
+        br.capture_build_and_render()
+        br.replay_build_and_render()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        br.replay_build_and_render()
+        torch.cuda.synchronize()
+        t_full = time.perf_counter() - t0
+        br.check_rebuild()
+        t_seq_build = t_seq + br.build_ms / 1e3
+        full = {"build_and_render_graph_msamples_s": samples / t_full / 1e6,
+                "build_and_render_graph_ms": round(t_full * 1e3, 2),
+                "builds_in_graph_ms": round((t_full - t_batch) * 1e3, 2),
+                "sequential_with_builds_msamples_s": samples / t_seq_build / 1e6,
+                "build_and_render_speedup_vs_sequential_with_builds": t_seq_build / t_full}
     tri = sum(s.triangles.shape[0] for s in specs)
     print(f"[bench] C3 {len(specs)} scenes, {tri} triangles, {samples / 1e6:.0f} Msamples: batch wavefront "
           f"{samples / t_batch / 1e6:.1f} Msamples/s (paths in flight {bbest}), per-scene graph "
           f"{samples / t_graph / 1e6:.1f}, sequential {samples / t_seq / 1e6:.1f} Msamples/s",
           file=sys.stderr, flush=True)
+    if full is not None:
+        print(f"[bench] C3 with the 64 tree builds: one graph (builds + batch wavefront) "
+              f"{full['build_and_render_graph_msamples_s']:.1f} Msamples/s ({full['builds_in_graph_ms']:.1f} ms of "
+              f"builds), sequential renders after eager builds {full['sequential_with_builds_msamples_s']:.1f}",
+              file=sys.stderr, flush=True)
     return {"workload": "C3: 64 bushes, seeds 0-63, iterations 3-7, resolutions 128^2-1024^2, 64 spp, depth 8",
             "scenes": len(specs), "triangles": int(tri), "samples": samples,
             "batch_wavefront_msamples_s": samples / t_batch / 1e6,
@@ -285,7 +309,8 @@ def batch_bench(bvh: str) -> dict:
             "autotuned_paths_in_flight": best, "autotune_ms": {str(k): round(v, 2) for k, v in tune.items()},
             "batch_autotuned_paths_in_flight": bbest,
             "batch_autotune_ms": {str(k): round(v, 2) for k, v in btune.items()},
-            "bvh_build_ms_total": round(br.build_ms, 1), "scene_generation_s": round(gen_s, 1)}
+            "bvh_build_ms_total": round(br.build_ms, 1), "scene_generation_s": round(gen_s, 1),
+            **(full or {})}
 
 
 def spectral_bench(scene, spec, inputs, acc, passes: int, bvh: str) -> dict:

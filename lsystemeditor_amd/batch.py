@@ -7,6 +7,9 @@
   wavefront render back to back.
 
 Both need no host synchronisation; the paths-in-flight bound is picked by a timing autotuner.
+With a binned-SAH tree, capture_build_and_render records every scene's tree rebuild
+(ptSceneRebuild, one shared workspace) and then the batch wavefront in one graph, so one replay
+builds all 64 trees from the scenes' device triangles and renders them.
 
 Workload C3: 64 stochastic bushes, seeds 0..63, iterations uniform in 3..7 (3 k .. 2 M triangles),
 resolutions drawn from 128^2 .. 1024^2, 64 spp, max depth 8.
@@ -57,6 +60,8 @@ class BatchRenderer:
         self.graph = None
         self.batch = None
         self.batch_graph = None
+        self.build_graph = None
+        self.workspace = None
         torch.cuda.synchronize()
 
     @property
@@ -117,6 +122,31 @@ class BatchRenderer:
 
     def replay_batched(self):
         self.batch_graph.replay()
+
+    def capture_build_and_render(self):
+        """Records every scene's tree rebuild and then render_batched into one graph (binned-SAH
+        build modes only)."""
+        if any(sc.bvh not in ("sah", "sah-binary") for sc in self.scenes):
+            raise ValueError("tree rebuilds need a binned-SAH build mode")
+        b = self._batch()
+        b.reserve(self.max_paths)
+        if self.workspace is None:
+            self.workspace = abi.BuildWorkspace(max(sc.triangle_count for sc in self.scenes))
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            s = torch.cuda.current_stream()
+            for sc in self.scenes:
+                sc.rebuild(self.workspace, s.cuda_stream)
+            b.render_fixed(self.specs[0].spp, max_paths=self.max_paths, stream=s)
+        self.build_graph = g
+
+    def replay_build_and_render(self):
+        self.build_graph.replay()
+
+    def check_rebuild(self):
+        """Waits for the batch stream; raises unless the last scene's rebuild reproduced its tree."""
+        self.scenes[-1].rebuild_check(self.workspace, int(self.stream.cuda_stream))
 
     def batched_image(self, i: int) -> torch.Tensor:
         """Scene i's (pixels, 4) accumulator rows of the last batch render."""

@@ -49,3 +49,28 @@ def test_batch_graph_replay_equals_eager(cuda):
             for k in ("acc", "counts", "mean", "m2"):
                 assert np.array_equal(np.asarray(got[k]).view(np.uint8), np.asarray(eager[i][k]).view(np.uint8)), \
                     (rep, i, k)
+
+
+def test_build_and_render_graph_equals_eager(cuda):
+    """Every scene's tree rebuild and the batch wavefront in one graph: each replay gives the eager
+    batch render, and the last rebuild reproduces its scene's tree."""
+    import torch
+    from lsystemeditor_amd import batch
+    specs = batch.batch_specs(count=4, spp=4, iterations=(3, 4))
+    for s in specs:   # keep the test small: at most 256^2
+        if s.width > 256:
+            s.width = s.height = 256
+            s.camera[9] = np.float32(1.0)
+    br = batch.BatchRenderer(specs, bvh="sah", max_paths=1 << 20)
+    br.render_batched()
+    torch.cuda.synchronize()
+    eager = br._batch().acc.host()
+    br.capture_build_and_render()
+    for rep in range(3):
+        br._batch().acc.acc.zero_()   # other GPU work between replays
+        br.replay_build_and_render()
+        torch.cuda.synchronize()
+        br.check_rebuild()
+        got = br._batch().acc.host()
+        for k in ("acc", "counts", "mean", "m2", "conv"):
+            assert np.array_equal(np.asarray(got[k]).view(np.uint8), np.asarray(eager[k]).view(np.uint8)), (rep, k)
