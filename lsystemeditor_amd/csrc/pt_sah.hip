@@ -170,14 +170,16 @@ __global__ void morton_kernel(const float4* __restrict__ cent, int n, const uint
     vals[i] = i;
 }
 
+// Grid-stride over the level's segments (the grid is sized for the largest level, at most
+// kPrepBlocks blocks).
 __global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, const SahCtl* __restrict__ ctl)
 {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= ctl->segCount * kSegWords) {
-        return;
+    const long long words = static_cast<long long>(ctl->segCount) * kSegWords;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) {
+        const int w = static_cast<int>(i % kSegWords);
+        segStats[i] = (w % 6) < 3 ? 0xFFFFFFFFu : 0u;
     }
-    const int w = i % kSegWords;
-    segStats[i] = (w % 6) < 3 ? 0xFFFFFFFFu : 0u;
 }
 
 // A segment occupies one contiguous range of positions, so its lanes within a wave form one run.
@@ -917,8 +919,8 @@ bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, floa
     bool ok = true;
     // every level the depth cap allows; levels past the last one find no segments and do nothing
     for (int level = 0; level < kSahLevels && ok; ++level) {
-        hipLaunchKernelGGL(seg_reset_kernel, dim3(grid(static_cast<long long>(maxSegs) * kSegWords)), dim3(kB), 0,
-                           stream, w->segStats, ctl);
+        hipLaunchKernelGGL(seg_reset_kernel, dim3(std::min(grid(static_cast<long long>(maxSegs) * kSegWords), kPrepBlocks)),
+                           dim3(kB), 0, stream, w->segStats, ctl);
         hipLaunchKernelGGL(seg_bounds_kernel, dim3(chunks(n, kChunk)), dim3(kB), 0, stream, perm, posSeg, w->bounds,
                            w->cent, n, w->segStats, ctl);
         hipLaunchKernelGGL(seg_setup_kernel, dim3(grid(maxSegs)), dim3(kB), 0, stream, segs, ctl, w->segStats, perm,
