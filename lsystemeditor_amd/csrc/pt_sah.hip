@@ -431,13 +431,15 @@ __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, c
     }
     __syncthreads();
     if (uniform) {
+        // flush the bins this block filled (an empty bin's words would not change the record)
         uint32_t* g = bins + static_cast<size_t>(kBins * kBinWords) * s0;
         for (int k = static_cast<int>(threadIdx.x); k < kBins * kBinWords; k += kB) {
             const int w = k % kBinWords;
+            if (lds[k - w] == 0u) {
+                continue;
+            }
             if (w == 0) {
-                if (lds[k] != 0u) {
-                    atomicAdd(&g[k], lds[k]);
-                }
+                atomicAdd(&g[k], lds[k]);
             } else if (w < 4) {
                 atomicMin(&g[k], lds[k]);
             } else {
