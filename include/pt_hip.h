@@ -446,7 +446,7 @@ float ptSceneLastBuildMs(const PtScene* scene);
  * triangles on the device; ptSceneRebuild enqueues the whole build again on them, with the
  * 4-wide collapse in mode 1, into the scene's own buffers: a fixed sequence of launches on
  * `stream` with no allocation and no host synchronisation. The tree is the one ptSceneBuild made
- * (the build is deterministic). The workspace holds the build's temporaries for scenes of up to
+ * (the build is deterministic), so the rebuild enqueues just the levels of that build. The workspace holds the build's temporaries for scenes of up to
  * maxTriangles triangles (about 1.1 KB per triangle); rebuilds that share one workspace must be
  * ordered on one stream. ptSceneRebuildCheck waits for `stream` and returns false, with
  * pathTracerLastError set, if the last rebuild enqueued with the workspace failed or produced a

@@ -25,10 +25,12 @@ void pt_sah_work_destroy(PtSahWork* w);
 // Enqueues a whole binned-SAH build of n <= capacity triangles into nodes/triPos/triShade (as
 // pt_build_bvh_sah), and when wide is not null the 4-wide collapse into wide (room for
 // wideCapacity nodes): a fixed sequence of launches with no allocation and no host
-// synchronisation, so a caller can capture it into a graph. The root box and reference, the level
-// starts and any error stay in the workspace until pt_sah_result reads them.
+// synchronisation, so a caller can capture it into a graph. It enqueues `levels` levels (0: as
+// many as the depth cap allows; a rebuild of unchanged triangles needs the levels of their first
+// build, and pt_sah_result reports a build left unfinished). The root box and reference, the
+// level starts and any error stay in the workspace until pt_sah_result reads them.
 bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade,
-                    float4* wide, int wideCapacity, hipStream_t stream);
+                    float4* wide, int wideCapacity, int levels, hipStream_t stream);
 // Waits for the stream and checks the last build enqueued with w (n triangles): false with the
 // error set if it did not produce a complete tree; otherwise the outputs of pt_build_bvh_sah and
 // the 4-wide node count (each optional).

@@ -885,8 +885,9 @@ PtSahWork* pt_sah_work_create(int capacity)
 }
 
 bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, float4* triPos, float4* triShade,
-                    float4* wide, int wideCapacity, hipStream_t stream)
+                    float4* wide, int wideCapacity, int levels, hipStream_t stream)
 {
+    const int levelCount = levels > 0 ? std::min(levels, kSahLevels) : kSahLevels;
     if (w == nullptr || n < 1 || n > w->capacity) {
         ptSetError("SAH build: %d triangles for a workspace of %d", n, w != nullptr ? w->capacity : 0);
         return false;
@@ -919,8 +920,9 @@ bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, floa
     }
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid(n)), dim3(kB), 0, stream, posSeg, n, 0);
     bool ok = true;
-    // every level the depth cap allows; levels past the last one find no segments and do nothing
-    for (int level = 0; level < kSahLevels && ok; ++level) {
+    // every level the depth cap allows (or the given count); levels past the last one find no
+    // segments and do nothing
+    for (int level = 0; level < levelCount && ok; ++level) {
         hipLaunchKernelGGL(seg_reset_kernel, dim3(std::min(grid(static_cast<long long>(maxSegs) * kSegWords), kPrepBlocks)),
                            dim3(kB), 0, stream, w->segStats, ctl);
         hipLaunchKernelGGL(seg_bounds_kernel, dim3(chunks(n, kChunk)), dim3(kB), 0, stream, perm, posSeg, w->bounds,
@@ -1007,7 +1009,7 @@ bool pt_build_bvh_sah(const float* dTris, int n, float4* nodes, float4* triPos, 
     if (w == nullptr) {
         return false;
     }
-    const bool ok = pt_sah_enqueue(w, dTris, n, nodes, triPos, triShade, nullptr, 0, stream) &&
+    const bool ok = pt_sah_enqueue(w, dTris, n, nodes, triPos, triShade, nullptr, 0, 0, stream) &&
                     pt_sah_result(w, n, stream, rootOut, levelsOut, levelStarts, nullptr);
     pt_sah_work_destroy(w);
     return ok;

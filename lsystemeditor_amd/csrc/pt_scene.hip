@@ -971,7 +971,7 @@ bool ptSceneRebuild(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t str
     }
     return pt_sah_enqueue(reinterpret_cast<PtSahWork*>(workspace), scene->srcTris, scene->triangleCount,
                           scene->nodes, scene->triPos, scene->triShade, scene->buildMode == 1 ? scene->wide : nullptr,
-                          scene->wideCount, stream);
+                          scene->wideCount, scene->levels, stream);
 }
 
 bool ptSceneRebuildCheck(const PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
