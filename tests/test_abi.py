@@ -73,3 +73,19 @@ def test_record_layouts():
     d = abi.pack_params()
     assert np.allclose(d[:4], [10 / 255, 10 / 255, 10 / 255, 1.0])
     assert list(d[4:10].view(np.int32)) == [32, 3, 16, 1024, 0, 0] and np.isclose(d[10], 0.02)
+
+
+def test_rebuild_entry_points_validate_without_gpu(L):
+    """The rebuild entry points reject what they cannot rebuild before any device call: a workspace
+    of no triangles, null handles, and a scene that has no binned-SAH build (an empty scene)."""
+    assert not L.ptBuildWorkspaceCreate(0)
+    assert "capacity 0" in abi.last_error()
+    assert L.ptSceneRebuild(None, None, None) is False
+    assert "null scene or workspace" in abi.last_error()
+    assert L.ptSceneRebuildCheck(None, None, None) is False
+    h = L.ptSceneCreate()
+    assert L.ptSceneRebuild(h, FAKE, None) is False
+    assert "binned-SAH" in abi.last_error()
+    assert L.ptSceneRebuildCheck(h, FAKE, None) is False
+    L.ptSceneDestroy(h)
+    L.ptBuildWorkspaceDestroy(None)   # no-op
