@@ -346,32 +346,45 @@ __device__ __forceinline__ int bin_of(const SplitRec& r, const float4& c)
     return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
 }
 
-// Blocks inside one segment histogram in LDS and flush once; others update the segment's global
-// bins. Consecutive lanes often share a (segment, bin) key (positions are in Morton order), so each
-// wave first reduces its runs of equal keys: a max-scan of run heads gives every lane its run's
-// first lane, a segmented scan over those run ids leaves each run's count and bounds in its last
-// lane, and only those lanes update the histogram.
+// Blocks inside one segment histogram in LDS and flush once. Blocks that span at most
+// kLocalSegs segments histogram per segment in LDS and flush the cells they filled; others
+// update the segments' global bins directly. Consecutive lanes often share a (segment, bin) key
+// (positions are in Morton order), so each wave first reduces its runs of equal keys: a max-scan
+// of run heads gives every lane its run's first lane, a segmented scan over those run ids leaves
+// each run's count and bounds in its last lane, and only those lanes update a histogram.
+constexpr int kLocalSegs = 32;
+
+__device__ __forceinline__ void flush_cell_word(uint32_t* g, int w, uint32_t v)
+{
+    if (w == 0) {
+        atomicAdd(g, v);
+    } else if (w < 4) {
+        atomicMin(g, v);
+    } else {
+        atomicMax(g, v);
+    }
+}
+
 __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
                                                  const float4* __restrict__ bounds, const float4* __restrict__ cent,
                                                  const SplitRec* __restrict__ split, int n, uint32_t* __restrict__ bins,
                                                  const SahCtl* __restrict__ ctl)
 {
-    __shared__ uint32_t lds[kBins * kBinWords];
+    __shared__ uint32_t lds[kLocalSegs * kBins * kBinWords];
+    __shared__ int range[2];
     if (ctl->segCount == 0) {
         return;
     }
+    constexpr int kCell = kBins * kBinWords;
     const int base = static_cast<int>(blockIdx.x * blockDim.x);
     const int i = base + static_cast<int>(threadIdx.x);
     const int last = min(base + kB, n) - 1;
     const int s0 = posSeg[base];
     const bool uniform = s0 >= 0 && posSeg[last] == s0 && split[s0].axis >= 0;
-    if (uniform) {
-        for (int k = static_cast<int>(threadIdx.x); k < kBins * kBinWords; k += kB) {
-            const int w = k % kBinWords;
-            lds[k] = w == 0 ? 0u : (w < 4 ? 0xFFFFFFFFu : 0u);
-        }
+    if (threadIdx.x == 0) {
+        range[0] = 0x7FFFFFFF;
+        range[1] = -1;
     }
-    __syncthreads();
     int s = i < n ? posSeg[i] : -1;
     int b = -1;  // (s, b): the lane's histogram cell; b < 0 for lanes with nothing to add
     uint32_t v[kBinWords] = {0u, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
@@ -394,6 +407,29 @@ __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, c
     if (b < 0) {
         s = -1;
     }
+    // the block's segment range (wave minimum and maximum, then LDS)
+    int sMin = s >= 0 ? s : 0x7FFFFFFF;
+    int sMax = s;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        sMin = min(sMin, __shfl_xor(sMin, off, kWave));
+        sMax = max(sMax, __shfl_xor(sMax, off, kWave));
+    }
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0 && sMax >= 0) {
+        atomicMin(&range[0], sMin);
+        atomicMax(&range[1], sMax);
+    }
+    __syncthreads();
+    const int sLo = uniform ? s0 : range[0];
+    const int span = uniform ? 1 : range[1] - range[0] + 1;
+    const bool local = span >= 1 && span <= kLocalSegs;
+    if (local) {
+        for (int k = static_cast<int>(threadIdx.x); k < span * kCell; k += kB) {
+            const int w = k % kBinWords;
+            lds[k] = w == 0 ? 0u : (w < 4 ? 0xFFFFFFFFu : 0u);
+        }
+    }
+    __syncthreads();
     // every shuffle runs with the whole wave active (a shuffle under a lane-dependent condition
     // would read lanes that did not take part)
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
@@ -419,8 +455,8 @@ __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, c
     const int sNext = __shfl_down(s, 1, kWave);
     const int bNext = __shfl_down(b, 1, kWave);
     if (b >= 0 && (lane == kWave - 1 || sNext != s || bNext != b)) {
-        uint32_t* dst = uniform ? lds + b * kBinWords
-                                : bins + static_cast<size_t>(kBins * kBinWords) * s + b * kBinWords;
+        uint32_t* dst = local ? lds + (s - sLo) * kCell + b * kBinWords
+                              : bins + static_cast<size_t>(kCell) * s + b * kBinWords;
         atomicAdd(&dst[0], v[0]);
         atomicMin(&dst[1], v[1]);
         atomicMin(&dst[2], v[2]);
@@ -430,20 +466,13 @@ __global__ __launch_bounds__(kB) void bin_kernel(const int* __restrict__ perm, c
         atomicMax(&dst[6], v[6]);
     }
     __syncthreads();
-    if (uniform) {
-        // flush the bins this block filled (an empty bin's words would not change the record)
-        uint32_t* g = bins + static_cast<size_t>(kBins * kBinWords) * s0;
-        for (int k = static_cast<int>(threadIdx.x); k < kBins * kBinWords; k += kB) {
+    if (local) {
+        // flush the cells this block filled (an empty cell's words would not change the record)
+        uint32_t* g = bins + static_cast<size_t>(kCell) * sLo;
+        for (int k = static_cast<int>(threadIdx.x); k < span * kCell; k += kB) {
             const int w = k % kBinWords;
-            if (lds[k - w] == 0u) {
-                continue;
-            }
-            if (w == 0) {
-                atomicAdd(&g[k], lds[k]);
-            } else if (w < 4) {
-                atomicMin(&g[k], lds[k]);
-            } else {
-                atomicMax(&g[k], lds[k]);
+            if (lds[k - w] != 0u) {
+                flush_cell_word(&g[k], w, lds[k]);
             }
         }
     }
