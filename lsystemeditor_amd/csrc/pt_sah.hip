@@ -190,7 +190,8 @@ __global__ void seg_reset_kernel(uint32_t* __restrict__ segStats, const SahCtl* 
 __global__ __launch_bounds__(kB) void seg_bounds_kernel(const int* __restrict__ perm, const int* __restrict__ posSeg,
                                                         const float4* __restrict__ bounds,
                                                         const float4* __restrict__ cent, int n,
-                                                        uint32_t* __restrict__ segStats, const SahCtl* __restrict__ ctl)
+                                                        uint32_t* __restrict__ segStats, const SahCtl* __restrict__ ctl,
+                                                        const SegRec* __restrict__ segs)
 {
     __shared__ uint32_t lds[kSegWords];
     if (ctl->segCount == 0) {
@@ -244,11 +245,21 @@ __global__ __launch_bounds__(kB) void seg_bounds_kernel(const int* __restrict__ 
         const int sn = __shfl_down(s, 1, kWave);
         if (s >= 0 && (lane == kWave - 1 || sn != s)) {
             uint32_t* dst = uniform ? lds : segStats + static_cast<size_t>(kSegWords) * s;
-            for (int k = 0; k < kSegWords; ++k) {
-                if ((k % 6) < 3) {
-                    atomicMin(&dst[k], v[k]);
-                } else {
-                    atomicMax(&dst[k], v[k]);
+            // a segment that lies inside this wave is this run alone: no other lane or block
+            // touches its record, so plain stores replace the atomics
+            const SegRec sg = uniform ? SegRec{0, 0, 0, 0} : segs[s];
+            if (!uniform && sg.begin >= i - lane && sg.begin + sg.count - 1 == i) {
+                uint4* q = reinterpret_cast<uint4*>(dst);
+                q[0] = make_uint4(v[0], v[1], v[2], v[3]);
+                q[1] = make_uint4(v[4], v[5], v[6], v[7]);
+                q[2] = make_uint4(v[8], v[9], v[10], v[11]);
+            } else {
+                for (int k = 0; k < kSegWords; ++k) {
+                    if ((k % 6) < 3) {
+                        atomicMin(&dst[k], v[k]);
+                    } else {
+                        atomicMax(&dst[k], v[k]);
+                    }
                 }
             }
         }
@@ -955,7 +966,7 @@ bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, floa
         hipLaunchKernelGGL(seg_reset_kernel, dim3(std::min(grid(static_cast<long long>(maxSegs) * kSegWords), kPrepBlocks)),
                            dim3(kB), 0, stream, w->segStats, ctl);
         hipLaunchKernelGGL(seg_bounds_kernel, dim3(chunks(n, kChunk)), dim3(kB), 0, stream, perm, posSeg, w->bounds,
-                           w->cent, n, w->segStats, ctl);
+                           w->cent, n, w->segStats, ctl, segs);
         hipLaunchKernelGGL(seg_setup_kernel, dim3(grid(maxSegs)), dim3(kB), 0, stream, segs, ctl, w->segStats, perm,
                            dTris, level, reinterpret_cast<float*>(nodes), triPos, w->rootBox, w->split, w->bins);
         hipLaunchKernelGGL(bin_kernel, dim3(grid(n)), dim3(kB), 0, stream, perm, posSeg, w->bounds, w->cent, w->split,
