@@ -173,6 +173,18 @@ bool pathTracerCompactActiveList(const int* d_activeIndicesIn, int* d_activeIndi
                                  int inCount, int width, int height, bool useRegionFilter, int regionMinX,
                                  int regionMinY, int regionMaxX, int regionMaxY, hipStream_t stream);
 
+/* No counterpart in the reference: pathTracerCompactActiveList for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and result; its tile counts live in a fixed 256 KiB device
+ * array per GPU instead of the library's grow-only allocation, so inCount is at most 64M
+ * (67,108,864) entries and larger counts return false with an error. Calls that overlap on one
+ * GPU (other streams or threads) share that array and race. */
+bool pathTracerCompactActiveListNoAlloc(const int* d_activeIndicesIn, int* d_activeIndicesOut, int* d_activeCount,
+                                        const uint8_t* d_converged, const uint32_t* d_counts,
+                                        int maxSamplesPerPixel, int inCount, int width, int height,
+                                        bool useRegionFilter, int regionMinX, int regionMinY, int regionMaxX,
+                                        int regionMaxY, hipStream_t stream);
+
 /* PathTracerCuda.h:85-94 (PathTracer.cu:824-846): no jitter, sampleIndex 0, overwrite. */
 bool pathTracerPreviewSample(PtFloat4* d_buffer, int width, int height, const CameraGpu* d_camera,
                              const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,

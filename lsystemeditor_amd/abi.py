@@ -51,6 +51,7 @@ SIGNATURES = {
     "pathTracerRebuildActiveList": (_b, [_p, _p, _p, _p, _i, _i, _i, _p]),
     "pathTracerAdaptiveSample": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _p]),
     "pathTracerCompactActiveList": (_b, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _b, _i, _i, _i, _i, _p]),
+    "pathTracerCompactActiveListNoAlloc": (_b, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _b, _i, _i, _i, _i, _p]),
     "pathTracerPreviewSample": (_b, [_p, _i, _i, _p, _p, _p, _p, _u, _p]),
     "pathTracerCopyToPbo": (_b, [_p, _p, _p, _p, _i, _i, _p, _f, _p]),
     "pathTracerUpsamplePreviewToPbo": (_b, [_p, _i, _i, _i, _p, _i, _i, _p, _f, _p]),

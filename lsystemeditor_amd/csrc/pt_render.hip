@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 using namespace pt;
 
@@ -526,6 +527,21 @@ int* scratch_ints(size_t n)
     return gScratch.ptr;
 }
 
+// The three compaction launches over a caller-chosen tile-count buffer of at least
+// ceil(count / kTile) ints; every entry the scan reads is written by the count kernel first.
+bool compact_launch(const int* in, int count, int* out, int* activeCount, const uint8_t* conv,
+                    const uint32_t* counts, int maxSpp, int width, bool region, int rx0, int ry0, int rx1, int ry1,
+                    int* tileCounts, hipStream_t stream, const char* what)
+{
+    const int tiles = (count + kTile - 1) / kTile;
+    hipLaunchKernelGGL(compact_count_kernel, dim3(tiles), dim3(kBlock), 0, stream, in, count, conv, counts, maxSpp,
+                       width, region ? 1 : 0, rx0, ry0, rx1, ry1, tileCounts);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(kBlock), 0, stream, tileCounts, tiles, activeCount);
+    hipLaunchKernelGGL(compact_scatter_kernel, dim3(tiles), dim3(kBlock), 0, stream, in, count, conv, counts, maxSpp,
+                       width, region ? 1 : 0, rx0, ry0, rx1, ry1, tileCounts, out);
+    return launched(what);
+}
+
 bool compact_impl(const int* in, int count, int* out, int* activeCount, const uint8_t* conv, const uint32_t* counts,
                   int maxSpp, int width, bool region, int rx0, int ry0, int rx1, int ry1, hipStream_t stream,
                   const char* what)
@@ -535,12 +551,42 @@ bool compact_impl(const int* in, int count, int* out, int* activeCount, const ui
     if (tileCounts == nullptr) {
         return false;
     }
-    hipLaunchKernelGGL(compact_count_kernel, dim3(tiles), dim3(kBlock), 0, stream, in, count, conv, counts, maxSpp,
-                       width, region ? 1 : 0, rx0, ry0, rx1, ry1, tileCounts);
-    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(kBlock), 0, stream, tileCounts, tiles, activeCount);
-    hipLaunchKernelGGL(compact_scatter_kernel, dim3(tiles), dim3(kBlock), 0, stream, in, count, conv, counts, maxSpp,
-                       width, region ? 1 : 0, rx0, ry0, rx1, ry1, tileCounts, out);
-    return launched(what);
+    return compact_launch(in, count, out, activeCount, conv, counts, maxSpp, width, region, rx0, ry0, rx1, ry1,
+                          tileCounts, stream, what);
+}
+
+// Tile counts of the allocation-free compaction: one int per kTile list entries, so one call
+// compacts at most kNoAllocMaxTiles * kTile = 64M entries (an 8192 x 8192 frame). 256 KiB per
+// GPU, held from the library's load until the process ends. Every call on a GPU shares its
+// copy, so calls that overlap on one GPU (two streams, two threads) race on it.
+constexpr int kNoAllocMaxTiles = 65536;
+static __device__ __align__(256) int scratch[kNoAllocMaxTiles];
+
+// Device address of `scratch` per device, looked up once per device with that device current.
+// In host code the array's name is only a stand-in for the symbol, never a device address.
+constexpr int kMaxDevices = 64;
+std::atomic<int*> gNoAllocScratch[kMaxDevices];
+
+int* no_alloc_scratch(const char* what)
+{
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) {
+        ptSetError("%s: no current device", what);
+        return nullptr;
+    }
+    int* p = gNoAllocScratch[device].load(std::memory_order_acquire);
+    if (p != nullptr) {
+        return p;
+    }
+    void* addr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || addr == nullptr) {
+        ptSetError("%s: scratch symbol lookup failed: %s", what, hipGetErrorString(e));
+        return nullptr;
+    }
+    p = static_cast<int*>(addr);
+    gNoAllocScratch[device].store(p, std::memory_order_release);
+    return p;
 }
 
 } // namespace
@@ -705,6 +751,31 @@ bool pathTracerCompactActiveList(const int* d_activeIndicesIn, int* d_activeIndi
     return compact_impl(d_activeIndicesIn, inCount, d_activeIndicesOut, d_activeCount, d_converged, d_counts,
                         maxSamplesPerPixel, width, useRegionFilter, regionMinX, regionMinY, regionMaxX, regionMaxY,
                         stream, "pathTracerCompactActiveList");
+}
+
+bool pathTracerCompactActiveListNoAlloc(const int* d_activeIndicesIn, int* d_activeIndicesOut, int* d_activeCount,
+                                        const uint8_t* d_converged, const uint32_t* d_counts,
+                                        int maxSamplesPerPixel, int inCount, int width, int height,
+                                        bool useRegionFilter, int regionMinX, int regionMinY, int regionMaxX,
+                                        int regionMaxY, hipStream_t stream)
+{
+    if (d_activeIndicesIn == nullptr || d_activeIndicesOut == nullptr || d_activeCount == nullptr ||
+        d_converged == nullptr || d_counts == nullptr || inCount <= 0 || width <= 0 || height <= 0) {
+        ptSetError("pathTracerCompactActiveListNoAlloc: invalid arguments");
+        return false;
+    }
+    if (static_cast<long long>(inCount) > static_cast<long long>(kNoAllocMaxTiles) * kTile) {
+        ptSetError("pathTracerCompactActiveListNoAlloc: %d entries exceed the maximum of %lld", inCount,
+                   static_cast<long long>(kNoAllocMaxTiles) * kTile);
+        return false;
+    }
+    int* tileCounts = no_alloc_scratch("pathTracerCompactActiveListNoAlloc");
+    if (tileCounts == nullptr) {
+        return false;
+    }
+    return compact_launch(d_activeIndicesIn, inCount, d_activeIndicesOut, d_activeCount, d_converged, d_counts,
+                          maxSamplesPerPixel, width, useRegionFilter, regionMinX, regionMinY, regionMaxX, regionMaxY,
+                          tileCounts, stream, "pathTracerCompactActiveListNoAlloc");
 }
 
 bool pathTracerPreviewSample(PtFloat4* d_buffer, int width, int height, const CameraGpu* d_camera,

@@ -38,6 +38,9 @@ FAKE = 0x1000  # never dereferenced: every call below fails validation first
     ("pathTracerAdaptiveSample", [FAKE] * 6 + [16, 4, 4, FAKE, None, None, FAKE, 1, None], "no scene"),
     ("pathTracerAdaptiveSample", [FAKE] * 6 + [16, 4, 4, FAKE, FAKE, None, None, 1, None], "invalid"),
     ("pathTracerCompactActiveList", [FAKE] * 5 + [16, 0, 4, 4, False, 0, 0, 0, 0, None], "invalid"),
+    ("pathTracerCompactActiveListNoAlloc", [FAKE] * 5 + [16, 0, 4, 4, False, 0, 0, 0, 0, None], "invalid"),
+    ("pathTracerCompactActiveListNoAlloc", [FAKE] * 5 + [16, 65536 * 1024 + 1, 8192, 8193, False, 0, 0, 0, 0, None],
+     "exceed the maximum"),
     ("pathTracerPreviewSample", [FAKE, 4, 4, None, FAKE, None, FAKE, 1, None], "invalid"),
     ("pathTracerCopyToPbo", [FAKE, FAKE, FAKE, FAKE, 4, 4, None, 1.0, None], "invalid"),
     ("pathTracerUpsamplePreviewToPbo", [FAKE, 0, 4, 2, FAKE, 8, 8, FAKE, 1.0, None], "invalid"),
