@@ -539,6 +539,18 @@ PT_DEV bool tri_test(const Ray& r, const float4& a, const float4& b, const float
     return true;
 }
 
+// Closest-hit order: the smaller distance, and among triangles at the same distance the one
+// with the smaller original index. tri_test accepts t <= the current best, so this decides
+// exact ties. The reference keeps whichever tied triangle its traversal reaches last
+// (MeshAccelIntersectCore.h:160-166 with the `t > tMax` rejection of :87), which depends on the
+// tree's leaf order; its median split places equal centroids by std::nth_element
+// (MeshBvhBuilder.cpp:91-105), so that order is not fixed by its source. With this rule the
+// closest hit is a function of the scene alone, the same for every tree and traversal here.
+PT_DEV bool closer(float t, uint32_t tri, float bestT, uint32_t bestTri)
+{
+    return t < bestT || tri < bestTri;
+}
+
 struct Hit {
     bool hit;
     float t;
@@ -615,11 +627,12 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
                 stats->tris += 1;
             }
             float t, u, v;
-            if (tri_test(r, a, b, c, tMin, bestT, t, u, v)) {
+            const uint32_t id = __float_as_uint(a.w);
+            if (tri_test(r, a, b, c, tMin, bestT, t, u, v) && closer(t, id, bestT, bestTri)) {
                 bestT = t;
                 bestU = u;
                 bestV = v;
-                bestTri = __float_as_uint(a.w);
+                bestTri = id;
                 if (ANY_HIT && t < tMax) {
                     return true;
                 }

@@ -429,11 +429,12 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     ts.tris += 1u;
                 }
                 float t, u, v;
-                if (tri_test(r, a, b, c, tMin, bestT, t, u, v)) {
+                const uint32_t id = __float_as_uint(a.w);
+                if (tri_test(r, a, b, c, tMin, bestT, t, u, v) && closer(t, id, bestT, bestTri)) {
                     bestT = t;
                     bestU = u;
                     bestV = v;
-                    bestTri = __float_as_uint(a.w);
+                    bestTri = id;
                     bestMat = __float_as_int(b.w);
                     if (ANY_HIT && t < kRayTMax) {
                         finished = true;

@@ -269,7 +269,7 @@ OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, 
     float closestT = tMax;
     OVec3 closestNormal = v3(0.0f, 0.0f, 0.0f);
     OVec2 closestUv = {0.0f, 0.0f};
-    uint32_t closestTri = 0;
+    uint32_t closestTri = 0xFFFFFFFFu;
     uint64_t nodes = 0, tests = 0;
 
     while (stackSize > 0) {
@@ -293,7 +293,11 @@ OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, 
                 OVec3 n;
                 OVec2 uv;
                 ++tests;
-                if (orc_ray_triangle(ro, rd, &scene->triangles[ti], tMin, closestT, &tHit, &n, &uv)) {
+                /* Exact ties of t go to the smaller original index (the reference keeps the last
+                 * tied triangle it visits, an order its nth_element tie placement leaves open,
+                 * MeshBvhBuilder.cpp:91-105); every tree then gives the same closest hit. */
+                if (orc_ray_triangle(ro, rd, &scene->triangles[ti], tMin, closestT, &tHit, &n, &uv) &&
+                    (tHit < closestT || ti < closestTri)) {
                     closestT = tHit;
                     closestNormal = n;
                     closestUv = uv;

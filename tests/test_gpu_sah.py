@@ -219,3 +219,52 @@ def test_wide_batch_render_equals_binary(cuda):
         b.close()
     for k in ("acc", "counts", "mean", "m2", "conv"):
         assert np.array_equal(np.asarray(res["sah"][k]).view(np.uint8), np.asarray(res["sah-binary"][k]).view(np.uint8)), k
+
+
+def tie_scene(n=600, seed=21):
+    """Random triangles of which a quarter are exact copies (same vertices, later index, another
+    material) of others: rays hitting a copied triangle meet two triangles at the same t."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-4, 4, (n, 1, 3))
+    v = (c + rng.normal(0, 0.8, (n, 3, 3))).astype(np.float32)
+    k = n // 4
+    v[n - k:] = v[:k]
+    mat = np.zeros(n, np.uint32)
+    mat[n - k:] = 1
+    tris = abi.pack_host_triangles(v[:, 0], v[:, 1], v[:, 2], material=mat)
+    mats = np.stack([abi.pack_material(0.8, 0.2, 0.1, roughness=0.5), abi.pack_material(0.1, 0.3, 0.9, roughness=0.2)])
+    return tris, mats, v, k
+
+
+def test_exact_ties_take_the_smaller_index_on_every_tree(cuda):
+    """Closest hits at equal t report the smaller original index on the median, binned-SAH and
+    4-wide trees alike (so renders do not depend on the tree): rays at copied triangles hit the
+    original; renders of the scene on every tree and schedule equal the oracle's bit for bit."""
+    from lsystemeditor_amd import render
+    tris, mats, v, k = tie_scene()
+    ref = oracle_scene(tris, mats)
+    rng = np.random.default_rng(4)
+    m = 2000
+    tgt = v[rng.integers(0, k, m)].mean(1)
+    o = (tgt + rng.normal(0, 6.0, (m, 3))).astype(np.float32)
+    d = (tgt - o).astype(np.float32)
+    want_tri = np.array([ref.trace(o[i], d[i]).triangleIndex if ref.trace(o[i], d[i]).hit else -1 for i in range(m)])
+    hit_copies = np.isin(want_tri, np.arange(k))
+    assert hit_copies.sum() > m // 4
+    for bvh in ("median", "sah-binary"):
+        sc = abi.Scene(tris, mats, bvh=bvh)
+        got = render.trace_rays(sc, o, d)
+        g = np.where(got[:, 0].view(np.int32) != 0, got[:, 7].view(np.uint32).astype(np.int64), -1)
+        assert np.array_equal(g, want_tri), f"{bvh}: {int((g != want_tri).sum())} rays differ"
+    w = h = 48
+    cam = abi.pack_camera((0.0, 0.0, 14.0), abi.quat_look(np.array([0.0, 0.0, -1.0])), fov_y=np.radians(50.0),
+                          aspect=1.0, near=0.05)
+    params = abi.pack_params(background=(0.6, 0.7, 0.9), max_path_depth=6, min_samples=8, max_samples_per_pixel=8)
+    acc_o, _ = O.render_adaptive(ref, cam, params, w, h, threads=8)
+    inputs = render.DeviceInputs(cam, params)
+    for bvh in ("median", "sah-binary", "sah"):
+        sc = abi.Scene(tris, mats, bvh=bvh)
+        for sched in ("loop", "wavefront"):
+            dev = render.Accumulator.allocate(w, h)
+            (render.render_loop if sched == "loop" else render.render_wavefront)(sc, dev, inputs)
+            compare_accumulators(dev.host(), acc_o, f"ties/{bvh}/{sched}")

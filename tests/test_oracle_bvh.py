@@ -100,7 +100,9 @@ def brute(tris, ro, rd, tmin=0.0, tmax=1e6):
         t = C.c_float()
         n = O.Vec3()
         uv = O.Vec2()
-        if L.orc_ray_triangle(O.v3(ro), rdn, tris[i].ctypes.data, tmin, best[0], C.byref(t), C.byref(n), C.byref(uv)):
+        # the first (smallest) index wins an exact tie of t
+        if L.orc_ray_triangle(O.v3(ro), rdn, tris[i].ctypes.data, tmin, best[0], C.byref(t), C.byref(n), C.byref(uv)) \
+                and (best[1] is None or t.value < best[0]):
             best = (t.value, i)
     return best
 
@@ -182,3 +184,42 @@ def test_emissive_cdf_closed_form():
     w0 = np.float32(2.0) * np.float32(3.0 * np.float32(np.float32(0.2126) + np.float32(0.7152) + np.float32(0.0722)))
     w1 = np.float32(0.5) * np.float32(2.0 * 0.2126)
     assert sc.em_cdf[1] == w0 and abs(sc.em_cdf[2] - (w0 + w1)) <= 1e-6 * w0
+
+
+def test_exact_ties_go_to_the_smaller_index():
+    """Two copies of a triangle (indices 0 and 5, five others in between): every ray through it
+    reports index 0, whichever order the tree visits the copies in; the brute force agrees."""
+    rng = np.random.default_rng(2)
+    v = rng.uniform(-3, 3, (6, 3, 3)).astype(np.float32)
+    v[0] = [[-1, -1, 0], [1, -1, 0], [0, 1, 0]]
+    v[5] = v[0]
+    v[1:5, :, 2] += 6.0   # the others lie behind the viewer
+    tris = abi.host_to_gpu_triangles(abi.pack_host_triangles(v[:, 0], v[:, 1], v[:, 2]))
+    sc = O.OracleScene(tris, abi.pack_material()[None])
+    for x, y in ((0.0, 0.0), (0.3, -0.5), (-0.2, 0.1)):
+        ro = np.array([x, y, 2.0], np.float32)
+        rd = np.array([0.01, 0.02, -1.0], np.float32)   # (axis-parallel rays miss: invRd = 0 quirk)
+        h = sc.trace(ro, rd)
+        assert h.hit and h.triangleIndex == 0
+        assert brute(tris, ro, rd) == (h.t, 0)
+
+
+def test_traversal_equals_brute_force_with_copies():
+    """A soup in which an eighth of the triangles are exact copies: ties everywhere."""
+    tris = soup(400, 13, dup=True)
+    sc = O.OracleScene(tris, abi.pack_material()[None])
+    rng = np.random.default_rng(6)
+    v = tris[:, 0:9].reshape(-1, 3, 3)
+    hits = 0
+    for j in range(300):
+        tgt = v[j % 50].mean(0)   # the copied triangles
+        ro = (tgt + rng.normal(0, 5.0, 3)).astype(np.float32)
+        rd = (tgt - ro).astype(np.float32)
+        h = sc.trace(ro, rd)
+        t, i = brute(tris, ro, rd)
+        if i is None:
+            assert not h.hit
+        else:
+            hits += 1
+            assert h.hit and h.t == np.float32(t) and h.triangleIndex == i
+    assert hits > 100
