@@ -185,6 +185,11 @@ float orc_dielectric_fresnel(float cosThetaI, float etaI, float etaT);
 OVec3 orc_schlick_f(float cosTheta, OVec3 f0);
 float orc_energy_compensation(float roughness, float cosTheta);
 float orc_oren_nayar_factor(float sigma, float cosI, float cosO, float dotWoWi);
+/* Burley diffusion cores (SubsurfaceCore.h:20-64). */
+float orc_burley_sample_radius(float u, float d);
+float orc_burley_profile(float r, float d);
+float orc_burley_pdf_radius(float r, float d);
+float orc_subsurface_effective_radius(const OMaterial* m, float uChannel);
 OVec3 orc_sample_cosine_local(float u1, float u2);
 OVec3 orc_sample_ggx_half_local(float alpha, float u1, float u2, float* pdf);
 

@@ -180,6 +180,10 @@ def lib():
                                                C.POINTER(SpecModel), vp, vp]),
             "orc_set_spectral": (None, [C.POINTER(SpecModel)]),
             "orc_rng_init": (None, [C.POINTER(Rng), C.c_int, C.c_int, C.c_uint32]),
+            "orc_burley_sample_radius": (f, [f, f]), "orc_burley_profile": (f, [f, f]),
+            "orc_burley_pdf_radius": (f, [f, f]),
+            "orc_subsurface_effective_radius": (f, [C.POINTER(Material), f]),
+            "orc_resolve_material": (Material, [C.POINTER(Material), Vec2, C.POINTER(Scene)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -215,6 +215,11 @@ static float burley_pdf_radius(float r, float d)
     return fmax2(0.0f, (t0 + t1) / fmax2(2.0f * sd * norm, 1.0e-8f));
 }
 
+/* Test hooks for the Burley cores (SubsurfaceCore.h:20-64). */
+float orc_burley_sample_radius(float u, float d) { return burley_sample_radius(u, d); }
+float orc_burley_profile(float r, float d) { return burley_profile(r, d); }
+float orc_burley_pdf_radius(float r, float d) { return burley_pdf_radius(r, d); }
+
 /* SubsurfaceCore.h:43-64 */
 static OVec3 channel_weights(const OMaterial* m)
 {
@@ -237,6 +242,8 @@ static float effective_radius(const OMaterial* m, float uc)
     }
     return fmax2(r.z, K_MIN_RADIUS);
 }
+
+float orc_subsurface_effective_radius(const OMaterial* m, float uChannel) { return effective_radius(m, uChannel); }
 
 /* SubsurfaceCore.h:66-82 */
 static OVec3 eval_burley(const OMaterial* m, float dist, float cosI, float cosO)
