@@ -209,6 +209,18 @@ bool pathTracerWriteUvDebugToPbo(const CameraGpu* d_camera, const MeshAccelScene
  * allocated while it is stored. */
 hipError_t spectralUploadStateToSymbol(const SpectralStateGpu* hostState);
 
+/* ---- light sampling (no reference counterpart) ------------------------------------------------
+ * 0 (default): the reference's light sampling. Its emissive next-event shadow ray runs to 1e6 from
+ * p + wi * 1e-4 and meets the sampled light itself (LightSamplingCore.h:360-369), so area lights are
+ * lit by BSDF sampling alone, with unweighted emission (PathIntegratorRandCore.h:110-167, 248).
+ * 1: multiple importance sampling on area lights. The shadow ray stops at 0.9999 of the distance
+ * to the light point, and emission found by BSDF sampling takes the balance weight against the
+ * next-event estimate of the vertex before. The expected image is the same as in mode 0, with less
+ * noise where area lights dominate. Applies to every forward launcher and wavefront render. The
+ * adjoint and detached renders accept mode 0 only. Returns false for other modes. */
+bool pathTracerSetLightSampling(int mode);
+int pathTracerLightSampling(void);
+
 /* ---- spectral state (role of SpectralStateHost, Spectral/SpectralState.cpp:15-141) --------- */
 
 typedef struct PtSpectral PtSpectral;

@@ -62,6 +62,8 @@ SIGNATURES = {
     "ptSpectralLoadCoeffFile": (_p, [C.c_char_p]),
     "ptSpectralGetState": (_b, [_p, _p]),
     "ptSpectralDestroy": (None, [_p]),
+    "pathTracerSetLightSampling": (_b, [_i]),
+    "pathTracerLightSampling": (_i, []),
     "pathTracerSetSpectralSampling": (_b, [_i, _f, _f]),
     "pathTracerSpectralSampling": (_i, []),
     "pathTracerAdaptiveRenderPasses": (_b, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _u, _i, _p]),
@@ -193,6 +195,23 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+class LightSampling:
+    """Context manager for pathTracerSetLightSampling (0: the reference's light sampling, 1: MIS on
+    area lights); restores the previous mode on exit."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = lib().pathTracerLightSampling()
+        check(lib().pathTracerSetLightSampling(self.mode), "pathTracerSetLightSampling")
+        return self
+
+    def __exit__(self, *exc):
+        lib().pathTracerSetLightSampling(self.prev)
 
 
 def last_error() -> str:

@@ -198,6 +198,10 @@ extern "C" bool pathTracerDetachedRender(float* d_image, int width, int height, 
         ptSetError("pathTracerDetachedRender: RGB rendering only (hero-wavelength sampling is on)");
         return false;
     }
+    if (pt_light_mode() != 0) {
+        ptSetError("pathTracerDetachedRender: the reference light sampling only (pathTracerSetLightSampling(0))");
+        return false;
+    }
     const long long n = static_cast<long long>(width) * height;
     hipLaunchKernelGGL(detached_kernel, dim3(static_cast<unsigned>((n + kAdBlock - 1) / kAdBlock)), dim3(kAdBlock), 0,
                        stream, d_image, width, height, samplesPerPixel, d_camera, d_scene, d_evalMaterials, d_env,
@@ -223,6 +227,10 @@ extern "C" bool pathTracerAdjointRenderSeeded(const float* d_gradImage, int widt
     }
     if (pt_spectral_enabled()) {
         ptSetError("pathTracerAdjointRender: RGB rendering only (hero-wavelength sampling is on)");
+        return false;
+    }
+    if (pt_light_mode() != 0) {
+        ptSetError("pathTracerAdjointRender: the reference light sampling only (pathTracerSetLightSampling(0))");
         return false;
     }
     return pt_adjoint_wavefront(d_gradImage, width, height, samplesPerPixel, d_camera, d_scene, d_env, d_params,

@@ -419,8 +419,28 @@ PT_DEV float rand01(Rng& r)
 // Scene access
 // ----------------------------------------------------------------------------------------
 constexpr float kRayTMax = 1.0e6f;
+// Boxes are culled against the closest hit widened by 2^-12 of its distance: the ray-triangle t
+// can fall short of the entry distance into the triangle's own box by more than rounding (thin
+// triangles seen from afar), so a box entered just past the current hit can hold a closer one.
+// The margin keeps such triangles in reach, so the closest hit does not depend on the tree or the
+// visit order (the reference culls at the hit itself).
+constexpr float kCullSlack = 1.000244140625f;
 constexpr float kShadowBias = 1.0e-4f;
 constexpr float kRelShadowBias = 1.0e-3f;
+
+// Light sampling of the forward renderers. kLightReference is the reference's: the emissive
+// next-event shadow ray runs to kRayTMax from p + wi * bias (LightSamplingCore.h:360-369), so it
+// meets the sampled light itself and reports it occluded, and emission found by BSDF sampling is
+// added unweighted (PathIntegratorRandCore.h:248): area lights are lit by BSDF sampling alone.
+// kLightMis is multiple importance sampling on area lights: the shadow ray runs from p + wi * bias
+// towards the light point and stops at kBoundedShadowScale of that distance, and emission that a
+// BSDF-sampled ray finds takes the balance weight pdf_bsdf / (pdf_bsdf + pdf_nee), with pdf_nee
+// the solid-angle density with which the previous vertex's next-event estimate picks that point
+// (zero on the light's back face and for directions below the previous vertex's normal, which
+// next-event estimation never samples). Both modes estimate the same image.
+constexpr int kLightReference = 0;
+constexpr int kLightMis = 1;
+constexpr float kBoundedShadowScale = 0.9999f;
 constexpr float kMinPdf = 1.0e-8f;
 constexpr float kPi = 3.14159265f;
 constexpr float kInvPi = 1.0f / kPi;
@@ -607,8 +627,9 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
     if (s.triangleCount == 0) {
         return false;
     }
+    float cullT = tMax * kCullSlack;
     float te;
-    if (!slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], tMin, tMax, te)) {
+    if (!slab(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], tMin, cullT, te)) {
         return false;
     }
     // stack entries below kRingEntries live in LDS, deeper ones in a private array
@@ -630,6 +651,7 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
             const uint32_t id = __float_as_uint(a.w);
             if (tri_test(r, a, b, c, tMin, bestT, t, u, v) && closer(t, id, bestT, bestTri)) {
                 bestT = t;
+                cullT = t * kCullSlack;
                 bestU = u;
                 bestV = v;
                 bestTri = id;
@@ -646,8 +668,8 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
                 stats->nodes += 1;
             }
             float le, re;
-            const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, bestT, le);
-            const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, bestT, re);
+            const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, cullT, le);
+            const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, cullT, re);
             const int left = __float_as_int(n3.x);
             const int right = __float_as_int(n3.y);
             if (lh && rh) {
@@ -691,7 +713,7 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
                 ref = stackRef[sp - kRingEntries];
                 t = stackT[sp - kRingEntries];
             }
-            if (!(t > bestT)) {
+            if (!(t > cullT)) {
                 cur = ref;
                 found = true;
                 break;
@@ -1701,6 +1723,75 @@ PT_DEV bool sample_emissive(const MeshAccelSceneGpu& s, float uTri, float u1, fl
     const float sel = (s.emissiveTriangleCdf[lo + 1] - s.emissiveTriangleCdf[lo]) / total;
     pdf = sel / area;
     return pdf > kMinPdf;
+}
+
+// Position of original triangle `tri` in the emissive list (ascending triangle indices), or -1.
+PT_DEV int emissive_slot(const MeshAccelSceneGpu& s, uint32_t tri)
+{
+    int lo = 0, hi = static_cast<int>(s.emissiveTriangleCount) - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        const uint32_t v = s.emissiveTriangleIndices[mid];
+        if (v == tri) {
+            return mid;
+        }
+        if (v < tri) {
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return -1;
+}
+
+// Solid-angle density with which emissive next-event estimation (sample_emissive, then
+// areaPdf * d2 / cosL as the integrator forms it) picks the point at distance t along dir on
+// triangle tri; 0 where it never does (not an emissive entry, zero weight, back face).
+PT_DEV float emissive_pdf_hit(const MeshAccelSceneGpu& s, uint32_t tri, V3 dir, float t)
+{
+    if (s.emissiveTriangleCount == 0 || s.emissiveTriangleCdf == nullptr || s.emissiveTris == nullptr) {
+        return 0.0f;
+    }
+    const float total = s.emissiveTriangleCdf[s.emissiveTriangleCount];
+    if (total <= kMinPdf) {
+        return 0.0f;
+    }
+    const int li = emissive_slot(s, tri);
+    if (li < 0) {
+        return 0.0f;
+    }
+    const float4 a = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(li) + 0u);
+    const float4 b = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(li) + 1u);
+    const float4 c = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(li) + 2u);
+    const V3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
+    const V3 cr = cross(sub(v1, v0), sub(v2, v0));
+    const V3 ln = normalize(cr);
+    const float area = 0.5f * length(cr);
+    if (area <= kMinPdf) {
+        return 0.0f;
+    }
+    const float cosL = max2(0.0f, dot(ln, scale(dir, -1.0f)));
+    if (cosL <= 0.0f) {
+        return 0.0f;
+    }
+    const float sel = (s.emissiveTriangleCdf[li + 1] - s.emissiveTriangleCdf[li]) / total;
+    const float areaPdf = sel / area;
+    return areaPdf * (t * t) / cosL;
+}
+
+// Balance weight of emission found by a BSDF-sampled ray (kLightMis): prevPdf is the pdf of the
+// sampled direction, reach whether the previous vertex's next-event estimate covers it.
+PT_DEV float emission_mis_weight(const MeshAccelSceneGpu& s, uint32_t tri, V3 dir, float t, float prevPdf,
+                                 bool reach)
+{
+    if (!reach) {
+        return 1.0f;
+    }
+    const float lightPdf = emissive_pdf_hit(s, tri, dir, t);
+    if (lightPdf <= 0.0f) {
+        return 1.0f;
+    }
+    return mis_balance(prevPdf, lightPdf);
 }
 
 // ----------------------------------------------------------------------------------------

@@ -43,11 +43,14 @@ PT_DEV typename Pol::Col env_nee_t(const MeshAccelSceneGpu& s, const RenderParam
     return mul(f, scale(L, sc));
 }
 
-// PathIntegratorRandCore.h:110-167
+// PathIntegratorRandCore.h:110-167; lightMode kLightMis bounds the shadow ray short of the light
+// point (pt_device.h). Under kLightMis the last vertex of a path (last) adds nothing: the light its
+// estimate finds lies one segment past the path's length, where the reference never gathers
+// emission, so both modes estimate the same image. Its draws are taken all the same.
 template <class Pol>
 PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, const Ctx& c,
                                         const typename Pol::Col& base, const typename Pol::Col& rad,
-                                        const Pol& pol, Rng& rng, TraceStats* st)
+                                        const Pol& pol, Rng& rng, TraceStats* st, int lightMode, bool last)
 {
     using C = typename Pol::Col;
     const C zero = ColN<C>::splat(0.0f);
@@ -57,6 +60,9 @@ PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, cons
     const float uTri = rand01(rng);
     const float u1 = rand01(rng);
     const float u2 = rand01(rng);
+    if (lightMode == kLightMis && last) {
+        return zero;
+    }
     V3 lp, ln, lr;
     float areaPdf = 0.0f;
     if (!sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
@@ -80,7 +86,12 @@ PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, cons
         return zero;
     }
     const V3 o = add(pos, scale(wi, kShadowBias));
-    if (trace_any(s, o, wi, kShadowBias, kRayTMax, st)) {
+    if (lightMode == kLightMis) {
+        const V3 toP = sub(lp, o);
+        if (trace_any(s, o, toP, kShadowBias, length(toP) * kBoundedShadowScale, st)) {
+            return zero;
+        }
+    } else if (trace_any(s, o, wi, kShadowBias, kRayTMax, st)) {
         return zero;
     }
     const C f = brdf_eval_t(c, wi, base, rad);
@@ -117,11 +128,12 @@ PT_DEV C subsurface_tier_b_t(C T, const Mat& m, const C& albedo, const Sample& s
     return T;
 }
 
-// PathIntegratorRandCore.h:206-363
+// PathIntegratorRandCore.h:206-363. lightMode kLightMis weights emission found by BSDF sampling
+// against the emissive next-event estimate of the vertex before (pt_device.h).
 template <class Pol>
 PT_DEV typename Pol::Col trace_path_t(const MeshAccelSceneGpu& s, const RenderParamsGpu& p,
                                       const EnvironmentMapGpu* env, V3 ro, V3 rd, const Pol& pol, Rng& rng,
-                                      TraceStats* st)
+                                      TraceStats* st, int lightMode = kLightReference)
 {
     using C = typename Pol::Col;
     Hit hit = trace_closest(s, ro, rd, 0.0f, kRayTMax, st);
@@ -142,6 +154,8 @@ PT_DEV typename Pol::Col trace_path_t(const MeshAccelSceneGpu& s, const RenderPa
     V3 origin = ro;
     V3 dir = rd;
     float eta = 1.0f;
+    float prevPdf = 0.0f;   // pdf of the BSDF sample that found the current hit
+    bool reach = false;     // whether that direction lies where the emissive NEE before samples
     for (int depth = 0; depth < maxDepth; ++depth) {
         const V3 pos = add(origin, scale(dir, hit.t));
         Ctx c;
@@ -155,9 +169,14 @@ PT_DEV typename Pol::Col trace_path_t(const MeshAccelSceneGpu& s, const RenderPa
         const C base = pol.base(c.m);
         const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
 
-        L = add(L, mul(T, pol.emission(c.m)));
+        if (lightMode == kLightMis && depth > 0 && c.m.emission > 0.0f) {
+            const float w = emission_mis_weight(s, hit.tri, dir, hit.t, prevPdf, reach);
+            L = add(L, mul(T, scale(pol.emission(c.m), w)));
+        } else {
+            L = add(L, mul(T, pol.emission(c.m)));
+        }
         L = add(L, mul(T, env_nee_t(s, p, env, pos, c, base, rad, pol, rng, st)));
-        L = add(L, mul(T, emissive_nee_t(s, pos, c, base, rad, pol, rng, st)));
+        L = add(L, mul(T, emissive_nee_t(s, pos, c, base, rad, pol, rng, st, lightMode, depth + 1 == maxDepth)));
 
         c.flags = flags;
         const float u1 = rand01(rng);
@@ -185,6 +204,8 @@ PT_DEV typename Pol::Col trace_path_t(const MeshAccelSceneGpu& s, const RenderPa
             T = scale(T, 1.0f / surv);
         }
         eta = smp.nextEta;
+        prevPdf = smp.pdf;
+        reach = dot(c.n, smp.dir) > 0.0f;
         const float bias = max2(kShadowBias, kRelShadowBias * hit.t);
         const float sgn = dot(c.n, smp.dir) >= 0.0f ? 1.0f : -1.0f;
         origin = add(pos, scale(c.n, sgn * bias));
@@ -209,7 +230,7 @@ PT_DEV V3 env_nee(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const En
 
 PT_DEV V3 emissive_nee(const MeshAccelSceneGpu& s, V3 pos, const Ctx& c, Rng& rng, TraceStats* st)
 {
-    return emissive_nee_t(s, pos, c, mat_base(c.m), mat_radii(c.m), RgbColors{}, rng, st);
+    return emissive_nee_t(s, pos, c, mat_base(c.m), mat_radii(c.m), RgbColors{}, rng, st, kLightReference, false);
 }
 
 PT_DEV V3 subsurface_tier_b(V3 T, const Mat& m, const Sample& smp, Rng& rng)
@@ -218,9 +239,9 @@ PT_DEV V3 subsurface_tier_b(V3 T, const Mat& m, const Sample& smp, Rng& rng)
 }
 
 PT_DEV V3 trace_path(const MeshAccelSceneGpu& s, const RenderParamsGpu& p, const EnvironmentMapGpu* env, V3 ro,
-                     V3 rd, Rng& rng, TraceStats* st)
+                     V3 rd, Rng& rng, TraceStats* st, int lightMode = kLightReference)
 {
-    return trace_path_t(s, p, env, ro, rd, RgbColors{}, rng, st);
+    return trace_path_t(s, p, env, ro, rd, RgbColors{}, rng, st, lightMode);
 }
 
 } // namespace pt

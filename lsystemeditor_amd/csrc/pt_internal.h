@@ -49,6 +49,9 @@ bool pt_scene_is_wide(const MeshAccelSceneGpu* d);
 // render hero wavelengths, and the model (coefficient table, white normalisation, wavelength
 // range) the kernels then read.
 bool pt_spectral_enabled();
+// Light sampling mode set through pathTracerSetLightSampling (pt_render.hip): 0 the reference's,
+// 1 multiple importance sampling on area lights.
+int pt_light_mode();
 pt::SpecModel pt_spectral_model();
 
 // Adjoint of the fixed-spp render (pathTracerAdjointRender after argument checks), scheduled on

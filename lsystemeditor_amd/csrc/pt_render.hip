@@ -21,7 +21,7 @@ constexpr int kBlock = 256;
 template <bool SPEC>
 PT_DEV V3 pixel_radiance(int idx, int x, int y, int width, int height, bool jitter, int sampleIndex,
                          const CameraGpu& cam, const MeshAccelSceneGpu& s, const EnvironmentMapGpu* env,
-                         const RenderParamsGpu& p, uint32_t seed, TraceStats* st, const SpecModel& sm)
+                         const RenderParamsGpu& p, uint32_t seed, TraceStats* st, const SpecModel& sm, int lm)
 {
     Rng rng;
     rng_init(rng, idx, sampleIndex, seed);
@@ -35,10 +35,10 @@ PT_DEV V3 pixel_radiance(int idx, int x, int y, int width, int height, bool jitt
     camera_ray(cam, u, v, l1, l2, ro, rd);
     if constexpr (SPEC) {
         const HeroColors pol = hero_colors(sm, rand01(rng));
-        return clamp_firefly_spectral(pol.to_rgb(trace_path_t(s, p, env, ro, rd, pol, rng, st)));
+        return clamp_firefly_spectral(pol.to_rgb(trace_path_t(s, p, env, ro, rd, pol, rng, st, lm)));
     } else {
         (void)sm;
-        return clamp_firefly(trace_path(s, p, env, ro, rd, rng, st));
+        return clamp_firefly(trace_path(s, p, env, ro, rd, rng, st, lm));
     }
 }
 
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void sample_adaptive_kernel(
     float4* __restrict__ acc, uint32_t* __restrict__ counts, float* __restrict__ lumMean, float* __restrict__ m2,
     uint8_t* __restrict__ converged, const int* __restrict__ active, int activeCount, const CameraGpu* camera,
     const MeshAccelSceneGpu* scene, const EnvironmentMapGpu* env, const RenderParamsGpu* params, int width,
-    int height, uint32_t seed, SpecModel sm)
+    int height, uint32_t seed, SpecModel sm, int lm)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= activeCount) {
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void sample_adaptive_kernel(
     const CameraGpu cam = *camera;
     const uint32_t n = counts[idx];
     const V3 r = pixel_radiance<SPEC>(idx, idx % width, idx / width, width, height, true, static_cast<int>(n), cam, s,
-                                      env, p, seed, nullptr, sm);
+                                      env, p, seed, nullptr, sm, lm);
     float4 a = acc[idx];
     float mean = lumMean[idx];
     float mm = m2[idx];
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void render_passes_kernel(
     float4* __restrict__ acc, uint32_t* __restrict__ counts, float* __restrict__ lumMean, float* __restrict__ m2,
     uint8_t* __restrict__ converged, const int* __restrict__ active, int activeCount, const CameraGpu* camera,
     const MeshAccelSceneGpu* scene, const EnvironmentMapGpu* env, const RenderParamsGpu* params, int width,
-    int height, uint32_t firstSeed, int passCount, int* workCounter, unsigned long long* totals, SpecModel sm)
+    int height, uint32_t firstSeed, int passCount, int* workCounter, unsigned long long* totals, SpecModel sm, int lm)
 {
     const MeshAccelSceneGpu s = *scene;
     const RenderParamsGpu p = *params;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void render_passes_kernel(
         for (int pass = 0; pass < passCount && !conv; ++pass) {
             TraceStats st{0u, 0u, 0u, 0u};
             const V3 r = pixel_radiance<SPEC>(idx, x, y, width, height, true, static_cast<int>(n), cam, s, env, p,
-                                              firstSeed + static_cast<uint32_t>(pass), STATS ? &st : nullptr, sm);
+                                              firstSeed + static_cast<uint32_t>(pass), STATS ? &st : nullptr, sm, lm);
             if (STATS) {
                 tot[0] += st.nodes;
                 tot[1] += st.tris;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void count_traversal_kernel(const uint32_t*
                                                                  const EnvironmentMapGpu* env,
                                                                  const RenderParamsGpu* params, int width,
                                                                  int height, uint32_t seed, uint32_t* out,
-                                                                 SpecModel sm)
+                                                                 SpecModel sm, int lm)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= activeCount) {
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void count_traversal_kernel(const uint32_t*
         TraceStats st{0u, 0u, 0u, 0u};
         const MeshAccelSceneGpu s = *scene;
         (void)pixel_radiance<SPEC>(idx, idx % width, idx / width, width, height, true,
-                                   static_cast<int>(counts[idx]), *camera, s, env, *params, seed, &st, sm);
+                                   static_cast<int>(counts[idx]), *camera, s, env, *params, seed, &st, sm, lm);
         o = make_uint4(st.nodes, st.tris, st.rays, st.hits);
     }
     reinterpret_cast<uint4*>(out)[t] = o;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void preview_kernel(float4* __restrict__ bu
                                                          const MeshAccelSceneGpu* scene,
                                                          const EnvironmentMapGpu* env,
                                                          const RenderParamsGpu* params, int width, int height,
-                                                         uint32_t seed, SpecModel sm)
+                                                         uint32_t seed, SpecModel sm, int lm)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= width * height) {
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void preview_kernel(float4* __restrict__ bu
                                                                             1024, 0, 0, 0.02f};
     const MeshAccelSceneGpu s = *scene;
     const V3 r =
-        pixel_radiance<SPEC>(i, i % width, i / width, width, height, false, 0, *camera, s, env, p, seed, nullptr, sm);
+        pixel_radiance<SPEC>(i, i % width, i / width, width, height, false, 0, *camera, s, env, p, seed, nullptr, sm, lm);
     buf[i] = make_float4(r.x, r.y, r.z, 1.0f);
 }
 
@@ -591,7 +591,24 @@ int* no_alloc_scratch(const char* what)
 
 } // namespace
 
+// Light sampling of the forward renderers (pt_device.h kLightReference / kLightMis).
+static std::atomic<int> gLightMode{kLightReference};
+
+int pt_light_mode() { return gLightMode.load(); }
+
 extern "C" {
+
+bool pathTracerSetLightSampling(int mode)
+{
+    if (mode != kLightReference && mode != kLightMis) {
+        ptSetError("pathTracerSetLightSampling: mode must be 0 (reference) or 1 (MIS on area lights), got %d", mode);
+        return false;
+    }
+    gLightMode.store(mode);
+    return true;
+}
+
+int pathTracerLightSampling(void) { return gLightMode.load(); }
 
 bool pathTracerClearAccumulator(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean, float* d_m2,
                                 uint8_t* d_converged, int* d_activeIndices, int* d_activeScratch, int* d_activeCount,
@@ -668,7 +685,7 @@ bool pathTracerAdaptiveSample(PtFloat4* d_buffer, uint32_t* d_samples, float* d_
     hipLaunchKernelGGL(spec ? sample_adaptive_kernel<true> : sample_adaptive_kernel<false>,
                        dim3(grid1(activeCount, kBlock)), dim3(kBlock), 0, stream, reinterpret_cast<float4*>(d_buffer),
                        d_samples, d_lumMean, d_m2, d_converged, d_activeIndices, activeCount, d_camera, d_scene, d_env,
-                       d_params, width, height, globalSeed, sm);
+                       d_params, width, height, globalSeed, sm, pt_light_mode());
     return launched("pathTracerAdaptiveSample");
 }
 
@@ -707,7 +724,7 @@ static bool render_passes_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d
                                       : (spec ? render_passes_kernel<false, true> : render_passes_kernel<false, false>);
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, stream, reinterpret_cast<float4*>(d_buffer), d_samples,
                        d_lumMean, d_m2, d_converged, d_activeIndices, activeCount, d_camera, d_scene, d_env, d_params,
-                       width, height, firstGlobalSeed, passCount, counter, d_totals, sm);
+                       width, height, firstGlobalSeed, passCount, counter, d_totals, sm, pt_light_mode());
     return launched("pathTracerAdaptiveRenderPasses");
 }
 
@@ -794,7 +811,7 @@ bool pathTracerPreviewSample(PtFloat4* d_buffer, int width, int height, const Ca
     hipLaunchKernelGGL(spec ? preview_kernel<true> : preview_kernel<false>,
                        dim3(grid1(static_cast<long long>(width) * height, kBlock)), dim3(kBlock), 0, stream,
                        reinterpret_cast<float4*>(d_buffer), d_camera, d_scene, d_env, d_params, width, height,
-                       globalSeed, pt_spectral_model());
+                       globalSeed, pt_spectral_model(), pt_light_mode());
     return launched("pathTracerPreviewSample");
 }
 
@@ -870,7 +887,7 @@ bool pathTracerCountTraversal(const uint32_t* d_samples, const int* d_activeIndi
     hipLaunchKernelGGL(spec ? count_traversal_kernel<true> : count_traversal_kernel<false>,
                        dim3(grid1(activeCount, kBlock)), dim3(kBlock), 0, stream, d_samples, d_activeIndices,
                        activeCount, d_camera, d_scene, d_env, d_params, width, height, globalSeed, d_stats,
-                       pt_spectral_model());
+                       pt_spectral_model(), pt_light_mode());
     return launched("pathTracerCountTraversal");
 }
 

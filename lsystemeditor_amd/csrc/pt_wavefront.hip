@@ -56,6 +56,7 @@ enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
     kFlagEnv = 2u,   // environment NEE shadow ray pending
     kFlagEmis = 4u,  // emissive NEE shadow ray pending
+    kFlagReach = 8u, // the extension direction lies where the vertex's emissive NEE samples (kLightMis)
 };
 
 // Path state, structure of arrays over the slots of one bounce's shade queue. Two buffers
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                                                             const int* __restrict__ countPtr, int* fetchCounters,
                                                             unsigned long long* stats, int4* __restrict__ hitA,
                                                             float2* __restrict__ hitB,
-                                                            const BatchScene* __restrict__ tab)
+                                                            const BatchScene* __restrict__ tab, int lightMode)
 {
     __shared__ int2 ring[kLdsEntries * kWfBlock];
     TraceScene s = BATCH ? TraceScene{} : trace_scene(*scene);
@@ -327,6 +328,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
     r.d = r.o;
     r.inv = r.o;
     float tMin = 0.0f, bestT = kRayTMax, bestU = 0.0f, bestV = 0.0f;
+    float rayMax = kRayTMax;  // the ray's tMax: kRayTMax, or short of the light point (kLightMis)
+    float cullT = kRayTMax * kCullSlack;  // box culling distance: the closest hit widened (kCullSlack)
     uint32_t bestTri = 0xFFFFFFFFu;
     int cur = kDone;
     while (true) {
@@ -369,7 +372,16 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                         const float4 a = P.shadowQ[2 * idx];
                         const float4 b = P.shadowQ[2 * idx + 1];
                         tag = __float_as_uint(a.w);
-                        r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+                        if (lightMode == kLightMis && (tag & 3u) == 2u) {
+                            // emissive NEE under kLightMis: (origin, light point), bounded
+                            const V3 o = mk(a.x, a.y, a.z);
+                            const V3 toP = sub(mk(b.x, b.y, b.z), o);
+                            r = make_ray(o, toP);
+                            rayMax = length(toP) * kBoundedShadowScale;
+                        } else {
+                            r = make_ray(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+                            rayMax = kRayTMax;
+                        }
                         tMin = kShadowBias;
                         if constexpr (BATCH) {
                             s = trace_scene(tab[__float_as_uint(b.w)].scene);
@@ -383,7 +395,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                             s = trace_scene(tab[PS.scene[q]].scene);
                         }
                     }
-                    bestT = kRayTMax;
+                    bestT = rayMax;
+                    cullT = rayMax * kCullSlack;
                     bestTri = 0xFFFFFFFFu;
                     bestMat = -1;
                     bestU = 0.0f;
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     }
                     float te;
                     cur = (s.triCount != 0u && slab(r, s.rmin[0], s.rmin[1], s.rmin[2], s.rmax[0], s.rmax[1],
-                                                    s.rmax[2], tMin, kRayTMax, te))
+                                                    s.rmax[2], tMin, cullT, te))
                         ? (WIDE ? s.wideRoot : s.rootRef)
                         : kDone;
                 } else {
@@ -432,11 +445,12 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 const uint32_t id = __float_as_uint(a.w);
                 if (tri_test(r, a, b, c, tMin, bestT, t, u, v) && closer(t, id, bestT, bestTri)) {
                     bestT = t;
+                    cullT = t * kCullSlack;
                     bestU = u;
                     bestV = v;
                     bestTri = id;
                     bestMat = __float_as_int(b.w);
-                    if (ANY_HIT && t < kRayTMax) {
+                    if (ANY_HIT && t < rayMax) {
                         finished = true;
                     }
                 }
@@ -456,13 +470,13 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 int r0 = __float_as_int(refs.x), r1 = __float_as_int(refs.y), r2 = __float_as_int(refs.z),
                     r3 = __float_as_int(refs.w);
                 float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
-                float d0 = (r0 != kDone && slab(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, bestT, e0))
+                float d0 = (r0 != kDone && slab(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, cullT, e0))
                     ? e0 : kInfDist;
-                float d1 = (r1 != kDone && slab(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, bestT, e1))
+                float d1 = (r1 != kDone && slab(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, cullT, e1))
                     ? e1 : kInfDist;
-                float d2 = (r2 != kDone && slab(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, bestT, e2))
+                float d2 = (r2 != kDone && slab(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, cullT, e2))
                     ? e2 : kInfDist;
-                float d3 = (r3 != kDone && slab(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, bestT, e3))
+                float d3 = (r3 != kDone && slab(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, cullT, e3))
                     ? e3 : kInfDist;
                 // sorting network for four
                 order2(d0, r0, d1, r1);
@@ -493,8 +507,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                     ts.nodes += 1u;
                 }
                 float le, re;
-                const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, bestT, le);
-                const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, bestT, re);
+                const bool lh = slab(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tMin, cullT, le);
+                const bool rh = slab(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tMin, cullT, re);
                 const int left = __float_as_int(n3.x);
                 const int right = __float_as_int(n3.y);
                 if (lh && rh) {
@@ -516,7 +530,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
             finished = true;
             while (st.sp > 0) {
                 const int2 e = st.pop();
-                if (!(__int_as_float(e.y) > bestT)) {
+                if (!(__int_as_float(e.y) > cullT)) {
                     cur = e.x;
                     finished = false;
                     break;
@@ -524,7 +538,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
             }
         }
         if (finished) {
-            const bool hit = bestT < kRayTMax;
+            const bool hit = bestT < rayMax;
             if (ANY_HIT) {
                 PS.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
             } else {
@@ -718,7 +732,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
     const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
     PathState O, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA, const float2* __restrict__ inB,
     const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA, int* counters, float* capture,
-    int captureSpp, SpecModel sm, const BatchScene* __restrict__ tab)
+    int captureSpp, SpecModel sm, const BatchScene* __restrict__ tab, int lightMode)
 {
     using C = typename Pol::Col;
     constexpr bool kSpectral = ColN<C>::n == 4;
@@ -739,7 +753,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
         const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
         const bool live = i < n;
         // what goes to the item's slot of the next shade queue
-        bool wantExt = false, wantEnv = false, wantEmis = false;
+        bool wantExt = false, wantEnv = false, wantEmis = false, reach = false;
         V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO, nextO = envO, nextD = envO;
         C L = ColN<C>::splat(0.0f), T = L, ce = L, cm = L;
         float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f, hero = 0.0f;
@@ -802,7 +816,12 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                     c.flags = 0;
                     const C base = pol.base(c.m);
                     const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
-                    L = add(L, mul(T, pol.emission(c.m)));
+                    if (lightMode == kLightMis && bounce > 0 && c.m.emission > 0.0f) {
+                        const float w = emission_mis_weight(s, tri, dir, t, I.pdf[i], (flags & kFlagReach) != 0u);
+                        L = add(L, mul(T, scale(pol.emission(c.m), w)));
+                    } else {
+                        L = add(L, mul(T, pol.emission(c.m)));
+                    }
                     // environment NEE (:65-108): value computed now, added if the shadow ray is clear
                     if (!skips_env_nee(c.m)) {
                         const float u1 = rand01(rng);
@@ -824,14 +843,16 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                             }
                         }
                     }
-                    // emissive-triangle NEE (:110-167)
+                    // emissive-triangle NEE (:110-167); under kLightMis the path's last vertex
+                    // takes the draws and adds nothing (pt_integrator.h emissive_nee_t)
                     if (s.emissiveTriangleCount != 0u) {
                         const float uTri = rand01(rng);
                         const float u1 = rand01(rng);
                         const float u2 = rand01(rng);
                         V3 lp, ln, lr;
                         float areaPdf = 0.0f;
-                        if (sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
+                        if (!(lightMode == kLightMis && bounce + 1 == depthLimit) &&
+                            sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
                             const V3 toL = sub(lp, pos);
                             const float d2 = dot(toL, toL);
                             if (d2 > kMinPdf) {
@@ -848,7 +869,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                                         cm = mul(T, mul(f, scale(pol.radiance(lr), sc)));
                                         wantEmis = true;
                                         emO = add(pos, scale(wi, kShadowBias));
-                                        emD = wi;
+                                        // kLightMis: the shadow ray runs to the light point (the
+                                        // trace kernel bounds it); otherwise along wi to kRayTMax
+                                        emD = lightMode == kLightMis ? lp : wi;
                                     }
                                 }
                             }
@@ -889,11 +912,13 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                         nextEta = smp.nextEta;
                         nextPdf = smp.pdf;
                         wantExt = true;
+                        reach = lightMode == kLightMis && dot(c.n, smp.dir) > 0.0f;
                     }
                     draws = rng_draws(rng);
                 }
             }
-            newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
+            newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u) |
+                       (reach ? kFlagReach : 0u);
             if (newFlags == 0u) {
                 if constexpr (!kSpectral) {
                     if (capture != nullptr && sample < static_cast<uint32_t>(captureSpp)) {
@@ -1705,7 +1730,7 @@ auto trace_kernel_for(bool counted, bool batch, bool wide)
 template <class ShadeLaunch>
 void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_scene, int maxPathDepth,
                          long long maxPaths, int persistentBlocks, ShadeLaunch shade, const BatchScene* tab,
-                         bool wide)
+                         bool wide, int lightMode)
 {
     const unsigned traceBlocks = static_cast<unsigned>(std::min<long long>(persistentBlocks, blocks_for(maxPaths)));
     const unsigned shadowBlocks =
@@ -1723,12 +1748,12 @@ void launch_bounces_with(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* 
         const int* extCount = b == 0 ? &P.counters[3] : &cIn[0];
         prof_mark(stream, 0, false, b);
         hipLaunchKernelGGL(extKernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, P, P.S[b & 1], extQueue,
-                           extCount, extFetch, stats, P.hitA[b & 1], P.hitB[b & 1], tab);
+                           extCount, extFetch, stats, P.hitA[b & 1], P.hitB[b & 1], tab, lightMode);
         prof_mark(stream, 0, true, b);
         if (b > 0) {
             prof_mark(stream, 1, false, b);
             hipLaunchKernelGGL(shadowKernel, dim3(shadowBlocks), dim3(kWfBlock), 0, stream, d_scene, P, P.S[b & 1],
-                               nullptr, &cIn[2], shadowFetch, stats, nullptr, nullptr, tab);
+                               nullptr, &cIn[2], shadowFetch, stats, nullptr, nullptr, tab, lightMode);
             prof_mark(stream, 1, true, b);
         }
         const int* shadeCount = b == 0 ? &P.counters[3] : &cIn[1];
@@ -1742,6 +1767,7 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
                     const EnvironmentMapGpu* d_env, int maxPathDepth, long long maxPaths, int persistentBlocks,
                     bool spec, const SpecModel& sm, const BatchScene* tab, bool wide)
 {
+    const int lightMode = pt_light_mode();
     const auto kernel = tab != nullptr ? (spec ? wf_shade_kernel<HeroColors, true> : wf_shade_kernel<RgbColors, true>)
                                        : (spec ? wf_shade_kernel<HeroColors, false> : wf_shade_kernel<RgbColors, false>);
     launch_bounces_with(
@@ -1750,9 +1776,9 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
             hipLaunchKernelGGL(kernel, dim3(traceBlocks), dim3(kWfBlock), 0, stream, d_scene, d_params, d_env,
                                P.S[b & 1], P.S[(b + 1) & 1], P, b, maxPathDepth, P.hitA[b & 1], P.hitB[b & 1],
                                shadeCount, P.extQ[(b + 1) & 1], P.hitA[(b + 1) & 1], cOut, gCapture, gCaptureSpp, sm,
-                               tab);
+                               tab, lightMode);
         },
-        tab, wide);
+        tab, wide, lightMode);
 }
 
 extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean,
@@ -2258,7 +2284,7 @@ bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels,
                                    maxDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
                                    P.hitA[(b + 1) & 1], cOut, perCopy / kGradParams, copies, gAd.partial, tab);
             },
-            tab, wide);
+            tab, wide, kLightReference);
     }
     hipLaunchKernelGGL(wf_adjoint_reduce_kernel, dim3(static_cast<unsigned>(perCopy)), dim3(kWfBlock), 0, stream,
                        gAd.partial, persistentBlocks, perCopy, d_grad);
@@ -2503,6 +2529,10 @@ extern "C" bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtB
     }
     if (pt_spectral_enabled()) {
         ptSetError("pathTracerAdjointRenderBatch: RGB rendering only (hero-wavelength sampling is on)");
+        return false;
+    }
+    if (pt_light_mode() != 0) {
+        ptSetError("pathTracerAdjointRenderBatch: the reference light sampling only (pathTracerSetLightSampling(0))");
         return false;
     }
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;

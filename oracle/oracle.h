@@ -288,6 +288,12 @@ void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* scene, const OPar
                              ORng* rng, const OSpecModel* model, const float* lambda4, float* L4);
 /* Hero-wavelength sampling for orc_pixel_radiance and everything built on it (NULL: RGB). The
  * hero draw follows the lens draws. Process-wide; not for concurrent use with other settings. */
+/* Light sampling of orc_trace_path / orc_trace_path_spectral: 0 the reference's, 1 MIS on area
+ * lights (bounded emissive shadow rays, balance-weighted emission at BSDF hits). */
+void orc_set_light_sampling(int mode);
+/* Test hook: the solid-angle pdf with which emissive NEE picks the point at distance t along dir on
+ * triangle tri (0 where it never does). */
+float orc_emissive_pdf_hit(const OScene* scene, uint32_t tri, OVec3 dir, float t);
 void orc_set_spectral(const OSpecModel* model);
 
 /* UV debug view: pinhole primary ray, fractional hit UV as red/green. */

@@ -179,6 +179,8 @@ def lib():
             "orc_trace_path_spectral": (None, [Vec3, Vec3, C.POINTER(Scene), vp, C.POINTER(Env), vp,
                                                C.POINTER(SpecModel), vp, vp]),
             "orc_set_spectral": (None, [C.POINTER(SpecModel)]),
+            "orc_set_light_sampling": (None, [C.c_int]),
+            "orc_emissive_pdf_hit": (f, [C.POINTER(Scene), C.c_uint32, Vec3, f]),
             "orc_rng_init": (None, [C.POINTER(Rng), C.c_int, C.c_int, C.c_uint32]),
             "orc_burley_sample_radius": (f, [f, f]), "orc_burley_profile": (f, [f, f]),
             "orc_burley_pdf_radius": (f, [f, f]),

@@ -247,6 +247,8 @@ int orc_ray_triangle(OVec3 ro, OVec3 rd, const OTriangle* tri, float tMin, float
     return 1;
 }
 
+#define K_CULL_SLACK 1.000244140625f /* 1 + 2^-12 */
+
 /* MeshAccelIntersectCore.h:110-221 */
 OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, float tMax,
                            uint64_t* nodeFetches, uint64_t* triTests)
@@ -267,6 +269,13 @@ OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, 
     stack[stackSize++] = (int)scene->rootIndex;
 
     float closestT = tMax;
+    /* Boxes are culled against the closest hit widened by 2^-12 of its distance. The reference
+     * culls against the closest hit itself, but the ray-triangle t can fall short of the entry
+     * distance into the triangle's own box by more than rounding (thin triangles seen from afar),
+     * so a box whose entry lies just past the current hit can hold a closer one; which of them the
+     * reference reports then depends on its visit order. The margin keeps every such triangle in
+     * reach, so the closest hit does not depend on the tree. */
+    float cullT = tMax * K_CULL_SLACK;
     OVec3 closestNormal = v3(0.0f, 0.0f, 0.0f);
     OVec2 closestUv = {0.0f, 0.0f};
     uint32_t closestTri = 0xFFFFFFFFu;
@@ -280,7 +289,7 @@ OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, 
         const ONode* node = &scene->nodes[nodeIndex];
         ++nodes;
         float tEnter = 0.0f, tExit = 0.0f;
-        if (!orc_ray_aabb(ro, rd, invRd, node->boundsMin, node->boundsMax, tMin, closestT, &tEnter, &tExit)) {
+        if (!orc_ray_aabb(ro, rd, invRd, node->boundsMin, node->boundsMax, tMin, cullT, &tEnter, &tExit)) {
             continue;
         }
         if (node->triCount > 0) {
@@ -299,6 +308,7 @@ OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, 
                 if (orc_ray_triangle(ro, rd, &scene->triangles[ti], tMin, closestT, &tHit, &n, &uv) &&
                     (tHit < closestT || ti < closestTri)) {
                     closestT = tHit;
+                    cullT = tHit * K_CULL_SLACK;
                     closestNormal = n;
                     closestUv = uv;
                     closestTri = ti;
@@ -316,12 +326,12 @@ OHit orc_trace_ray_counted(OVec3 ro, OVec3 rd, const OScene* scene, float tMin, 
         if (left >= 0 && left < (int)scene->nodeCount) {
             const ONode* ln = &scene->nodes[left];
             ++nodes;
-            leftHit = orc_ray_aabb(ro, rd, invRd, ln->boundsMin, ln->boundsMax, tMin, closestT, &le, &lx);
+            leftHit = orc_ray_aabb(ro, rd, invRd, ln->boundsMin, ln->boundsMax, tMin, cullT, &le, &lx);
         }
         if (right >= 0 && right < (int)scene->nodeCount) {
             const ONode* rn = &scene->nodes[right];
             ++nodes;
-            rightHit = orc_ray_aabb(ro, rd, invRd, rn->boundsMin, rn->boundsMax, tMin, closestT, &re, &rx);
+            rightHit = orc_ray_aabb(ro, rd, invRd, rn->boundsMin, rn->boundsMax, tMin, cullT, &re, &rx);
         }
         if (leftHit && rightHit) {
             if (le < re) {

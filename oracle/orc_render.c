@@ -61,6 +61,82 @@ static int occluded_from(OVec3 p, OVec3 wi, const OScene* s)
     return trace(o, wi, s, K_SHADOW_BIAS, K_RAY_TMAX, 0).hit;
 }
 
+/* Light sampling (no reference counterpart; mode 0 is the reference's). Mode 1, multiple
+ * importance sampling on area lights: the emissive shadow ray runs from p + wi * bias towards the
+ * light point and stops at 0.9999 of that distance, and emission found by a BSDF-sampled ray takes
+ * the balance weight pdf_bsdf / (pdf_bsdf + pdf_nee). */
+static int gLightMode = 0;
+
+void orc_set_light_sampling(int mode) { gLightMode = mode; }
+
+static int occluded_toward(OVec3 p, OVec3 wi, OVec3 lp, const OScene* s)
+{
+    const OVec3 o = vadd(p, vscale(wi, K_SHADOW_BIAS));
+    const OVec3 toP = vsub(lp, o);
+    return trace(o, toP, s, K_SHADOW_BIAS, vlen(toP) * 0.9999f, 0).hit;
+}
+
+/* Solid-angle density with which emissive NEE picks the point at distance t along dir on original
+ * triangle tri (sample_emissive's areaPdf * d2 / cosL); 0 where it never does. */
+static float emissive_pdf_hit(const OScene* s, uint32_t tri, OVec3 dir, float t)
+{
+    if (s == NULL || s->emissiveCount == 0 || s->emissiveCdf == NULL || s->emissiveIndices == NULL) {
+        return 0.0f;
+    }
+    const float total = s->emissiveCdf[s->emissiveCount];
+    if (total <= K_MIN_PDF) {
+        return 0.0f;
+    }
+    int lo = 0, hi = (int)s->emissiveCount - 1, li = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        const uint32_t v = s->emissiveIndices[mid];
+        if (v == tri) {
+            li = mid;
+            break;
+        }
+        if (v < tri) {
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    if (li < 0) {
+        return 0.0f;
+    }
+    const OTriangle* T = &s->triangles[s->emissiveIndices[li]];
+    const OVec3 cr = vcross(vsub(T->v1, T->v0), vsub(T->v2, T->v0));
+    const OVec3 ln = vnorm(cr);
+    const float area = 0.5f * vlen(cr);
+    if (area <= K_MIN_PDF) {
+        return 0.0f;
+    }
+    const float cosL = fmax2(0.0f, vdot(ln, vscale(dir, -1.0f)));
+    if (cosL <= 0.0f) {
+        return 0.0f;
+    }
+    const float sel = (s->emissiveCdf[li + 1] - s->emissiveCdf[li]) / total;
+    const float areaPdf = sel / area;
+    return areaPdf * (t * t) / cosL;
+}
+
+float orc_emissive_pdf_hit(const OScene* s, uint32_t tri, OVec3 dir, float t) { return emissive_pdf_hit(s, tri, dir, t); }
+
+/* Balance weight of emission found by a BSDF-sampled ray in mode 1: prevPdf is the pdf of the
+ * sampled direction, reach whether it lies above the previous vertex's normal (the directions its
+ * emissive NEE samples). */
+static float emission_weight(const OScene* s, uint32_t tri, OVec3 dir, float t, float prevPdf, int reach)
+{
+    if (!reach) {
+        return 1.0f;
+    }
+    const float lightPdf = emissive_pdf_hit(s, tri, dir, t);
+    if (lightPdf <= 0.0f) {
+        return 1.0f;
+    }
+    return orc_mis_balance(prevPdf, lightPdf);
+}
+
 static OBrdfCtx make_ctx(OVec3 n, OVec3 wo, const OMaterial* m, float eta, int flags)
 {
     OBrdfCtx c;
@@ -103,8 +179,11 @@ static OVec3 env_nee(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta, 
     return vmul(f, vscale(L, scale));
 }
 
-/* PathIntegratorRandCore.h:110-167 */
-static OVec3 emissive_nee(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta, const OScene* s, ORng* rng)
+/* PathIntegratorRandCore.h:110-167. In light mode 1 the path's last vertex (last) takes the draws
+ * and adds nothing: its light lies one segment past the path length at which the reference
+ * gathers emission, so both modes estimate the same image. */
+static OVec3 emissive_nee(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta, const OScene* s, ORng* rng,
+                          int last)
 {
     if (s == NULL || s->emissiveCount == 0) {
         return v3(0.0f, 0.0f, 0.0f);
@@ -112,6 +191,9 @@ static OVec3 emissive_nee(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float 
     const float uTri = orc_rand01(rng);
     const float u1 = orc_rand01(rng);
     const float u2 = orc_rand01(rng);
+    if (gLightMode == 1 && last) {
+        return v3(0.0f, 0.0f, 0.0f);
+    }
     OVec3 lp, ln, lr;
     float areaPdf = 0.0f;
     if (!orc_light_sample_emissive(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf)) {
@@ -128,7 +210,7 @@ static OVec3 emissive_nee(OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float 
     if (cosT <= 0.0f || cosL <= 0.0f) {
         return v3(0.0f, 0.0f, 0.0f);
     }
-    if (occluded_from(p, wi, s)) {
+    if (gLightMode == 1 ? occluded_toward(p, wi, lp, s) : occluded_from(p, wi, s)) {
         return v3(0.0f, 0.0f, 0.0f);
     }
     const float lightPdf = areaPdf * d2 / cosL;
@@ -200,6 +282,8 @@ OVec3 orc_trace_path(OVec3 ro, OVec3 rd, const OScene* s, const OParams* params,
     OVec3 dir = rd;
     OHit hit = first;
     float eta = 1.0f;
+    float prevPdf = 0.0f;
+    int reach = 0;
 
     for (int depth = 0; depth < maxDepth; ++depth) {
         const OVec3 p = vray(origin, dir, hit.t);
@@ -208,9 +292,14 @@ OVec3 orc_trace_path(OVec3 ro, OVec3 rd, const OScene* s, const OParams* params,
         const OMaterial src = orc_fetch_material(s, hit.triangleIndex);
         const OMaterial m = orc_resolve_material(&src, hit.uv, s);
 
-        L = vadd(L, vmul(T, v3(m.r * m.emission, m.g * m.emission, m.b * m.emission)));
+        if (gLightMode == 1 && depth > 0 && m.emission > 0.0f) {
+            const float w = emission_weight(s, hit.triangleIndex, dir, hit.t, prevPdf, reach);
+            L = vadd(L, vmul(T, vscale(v3(m.r * m.emission, m.g * m.emission, m.b * m.emission), w)));
+        } else {
+            L = vadd(L, vmul(T, v3(m.r * m.emission, m.g * m.emission, m.b * m.emission)));
+        }
         L = vadd(L, vmul(T, env_nee(p, n, wo, &m, eta, s, env, params, rng)));
-        L = vadd(L, vmul(T, emissive_nee(p, n, wo, &m, eta, s, rng)));
+        L = vadd(L, vmul(T, emissive_nee(p, n, wo, &m, eta, s, rng, depth + 1 == maxDepth)));
 
         const OBrdfCtx ctx = make_ctx(n, wo, &m, eta, flags);
         const float u1 = orc_rand01(rng);
@@ -237,6 +326,8 @@ OVec3 orc_trace_path(OVec3 ro, OVec3 rd, const OScene* s, const OParams* params,
             T = vscale(T, 1.0f / surv);
         }
         eta = smp.nextMediumEta;
+        prevPdf = smp.pdf;
+        reach = vdot(n, smp.direction) > 0.0f;
         const float bias = fmax2(K_SHADOW_BIAS, K_REL_SHADOW_BIAS * hit.t);
         const float sign = vdot(n, smp.direction) >= 0.0f ? 1.0f : -1.0f;
         origin = vadd(p, vscale(n, sign * bias));
@@ -324,7 +415,7 @@ static int spec_env_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, const O
 }
 
 static int spec_emissive_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta,
-                             const OScene* s, ORng* rng, float* out4)
+                             const OScene* s, ORng* rng, float* out4, int last)
 {
     out4[0] = out4[1] = out4[2] = out4[3] = 0.0f;
     if (s == NULL || s->emissiveCount == 0) {
@@ -333,6 +424,9 @@ static int spec_emissive_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, co
     const float uTri = orc_rand01(rng);
     const float u1 = orc_rand01(rng);
     const float u2 = orc_rand01(rng);
+    if (gLightMode == 1 && last) {
+        return 0;
+    }
     OVec3 lp, ln, lr;
     float areaPdf = 0.0f;
     if (!orc_light_sample_emissive(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf)) {
@@ -346,7 +440,8 @@ static int spec_emissive_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, co
     const OVec3 wi = vnorm(toL);
     const float cosT = fmax2(0.0f, vdot(n, wi));
     const float cosL = fmax2(0.0f, vdot(ln, vscale(wi, -1.0f)));
-    if (cosT <= 0.0f || cosL <= 0.0f || occluded_from(p, wi, s)) {
+    if (cosT <= 0.0f || cosL <= 0.0f ||
+        (gLightMode == 1 ? occluded_toward(p, wi, lp, s) : occluded_from(p, wi, s))) {
         return 0;
     }
     const float lightPdf = areaPdf * d2 / cosL;
@@ -392,6 +487,8 @@ void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* s, const OParams*
     OVec3 dir = rd;
     OHit hit = first;
     float eta = 1.0f;
+    float prevPdf = 0.0f;
+    int reach = 0;
     SpecVertex v;
     v.model = model;
     for (int k = 0; k < 4; ++k) {
@@ -406,17 +503,19 @@ void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* s, const OParams*
         const OMaterial m = orc_resolve_material(&src, hit.uv, s);
         spec_vertex(&v, &m, n, wo, eta, 0);
 
+        const int weighted = gLightMode == 1 && depth > 0 && m.emission > 0.0f;
+        const float w = weighted ? emission_weight(s, hit.triangleIndex, dir, hit.t, prevPdf, reach) : 1.0f;
         for (int k = 0; k < 4; ++k) {
             const float Le = orc_rgb2spec_radiance(model, m.r * m.emission, m.g * m.emission, m.b * m.emission,
                                                    v.lambda[k]);
-            L4[k] = L4[k] + T[k] * Le;
+            L4[k] = L4[k] + T[k] * (weighted ? Le * w : Le);
         }
         float c[4];
         spec_env_nee(&v, p, n, wo, &m, eta, s, env, params, rng, c);
         for (int k = 0; k < 4; ++k) {
             L4[k] = L4[k] + T[k] * c[k];
         }
-        spec_emissive_nee(&v, p, n, wo, &m, eta, s, rng, c);
+        spec_emissive_nee(&v, p, n, wo, &m, eta, s, rng, c, depth + 1 == maxDepth);
         for (int k = 0; k < 4; ++k) {
             L4[k] = L4[k] + T[k] * c[k];
         }
@@ -477,6 +576,8 @@ void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* s, const OParams*
             }
         }
         eta = smp.nextMediumEta;
+        prevPdf = smp.pdf;
+        reach = vdot(n, smp.direction) > 0.0f;
         const float bias = fmax2(K_SHADOW_BIAS, K_REL_SHADOW_BIAS * hit.t);
         const float sign = vdot(n, smp.direction) >= 0.0f ? 1.0f : -1.0f;
         origin = vadd(p, vscale(n, sign * bias));

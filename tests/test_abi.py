@@ -92,3 +92,17 @@ def test_rebuild_entry_points_validate_without_gpu(L):
     assert L.ptSceneRebuildCheck(h, FAKE, None) is False
     L.ptSceneDestroy(h)
     L.ptBuildWorkspaceDestroy(None)   # no-op
+
+
+def test_light_sampling_mode_validation_and_adjoint_refusal(L):
+    """pathTracerSetLightSampling takes 0 or 1; the adjoint and detached renders refuse mode 1 before
+    any device call."""
+    assert L.pathTracerLightSampling() == 0
+    assert L.pathTracerSetLightSampling(2) is False and "mode must be 0" in abi.last_error()
+    with abi.LightSampling(1):
+        assert L.pathTracerLightSampling() == 1
+        assert L.pathTracerAdjointRender(FAKE, 4, 4, 2, FAKE, FAKE, None, FAKE, 2, None, FAKE, None) is False
+        assert "reference light sampling only" in abi.last_error()
+        assert L.pathTracerDetachedRender(FAKE, 4, 4, 2, FAKE, FAKE, FAKE, None, FAKE, None) is False
+        assert "reference light sampling only" in abi.last_error()
+    assert L.pathTracerLightSampling() == 0
