@@ -462,6 +462,14 @@ bool ptSceneDescription(const PtScene* scene, MeshAccelSceneGpu* out);
 bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capacity, uint32_t* outCount,
                     uint32_t* outRoot);
 
+/* Test and inspection hook: copies the device emissive list built by ptSceneBuild
+ * (MeshAccelScene.cpp:62-95): the emissive triangles' original indices (ascending), the
+ * unnormalised prefix CDF of area x emission x luminance (count + 1 floats) and the packed
+ * records the sampler reads (3 x float4 per entry: v0, v1, v2 with r, g, b x emission in .w).
+ * outCount receives the entry count; NULL outputs are skipped; capacity is in entries. */
+bool ptSceneReadEmissive(const PtScene* scene, uint32_t* indices, float* cdf, float* records, uint32_t capacity,
+                         uint32_t* outCount);
+
 /* Milliseconds of the last GPU BVH build (device time between events). */
 float ptSceneLastBuildMs(const PtScene* scene);
 

@@ -84,6 +84,7 @@ SIGNATURES = {
     "ptSceneDescription": (_b, [_p, _p]),
     "ptSceneReadBvh": (_b, [_p, _p, _u, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ptSceneLastBuildMs": (_f, [_p]),
+    "ptSceneReadEmissive": (_b, [_p, _p, _p, _p, _u, C.POINTER(C.c_uint32)]),
     "ptSceneSetBuildMode": (_b, [_p, _i]),
     "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
     "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
@@ -385,6 +386,23 @@ class Scene:
         buf = (C.c_uint8 * SCENE_DESC_BYTES)()
         check(lib().ptSceneDescription(self.handle, C.addressof(buf)), "ptSceneDescription")
         return bytes(buf)
+
+    def read_emissive(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Device emissive list: original triangle indices, prefix CDF (count + 1) and the packed
+        (count, 12) float32 records the sampler reads."""
+        cnt = C.c_uint32(0)
+        check(lib().ptSceneReadEmissive(self.handle, None, None, None, 0, C.byref(cnt)), "ptSceneReadEmissive")
+        n = cnt.value
+        idx = np.zeros(n, np.uint32)
+        cdf = np.zeros(n + 1, np.float32)
+        rec = np.zeros((n, 12), np.float32)
+        if n:
+            check(lib().ptSceneReadEmissive(self.handle, idx.ctypes.data, cdf.ctypes.data, rec.ctypes.data, n,
+                                            C.byref(cnt)), "ptSceneReadEmissive")
+        else:
+            check(lib().ptSceneReadEmissive(self.handle, None, cdf.ctypes.data, None, 1, C.byref(cnt)),
+                  "ptSceneReadEmissive")
+        return idx, cdf, rec
 
     def read_bvh(self) -> tuple[np.ndarray, int]:
         """Nodes in the reference's 48-byte pre-order layout, as (count, 12) float32 words."""

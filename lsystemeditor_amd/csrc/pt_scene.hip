@@ -947,6 +947,43 @@ bool ptSceneReadBvh(const PtScene* scene, MeshBvhNode* hostNodes, uint32_t capac
     return true;
 }
 
+bool ptSceneReadEmissive(const PtScene* scene, uint32_t* indices, float* cdf, float* records, uint32_t capacity,
+                         uint32_t* outCount)
+{
+    if (scene == nullptr || outCount == nullptr) {
+        ptSetError("ptSceneReadEmissive: invalid arguments");
+        return false;
+    }
+    const uint32_t n = scene->desc.emissiveTriangleCount;
+    *outCount = n;
+    if (n == 0 || (indices == nullptr && cdf == nullptr && records == nullptr)) {
+        if (cdf != nullptr && capacity >= 1 && scene->emCdf != nullptr &&
+            hipMemcpy(cdf, scene->emCdf, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+            ptSetError("ptSceneReadEmissive: copy failed");
+            return false;
+        }
+        return true;
+    }
+    if (capacity < n) {
+        ptSetError("ptSceneReadEmissive: capacity %u < %u entries", capacity, n);
+        return false;
+    }
+    bool ok = true;
+    if (indices != nullptr) {
+        ok = ok && hipMemcpy(indices, scene->emIdx, n * sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (cdf != nullptr) {
+        ok = ok && hipMemcpy(cdf, scene->emCdf, (n + 1) * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (records != nullptr) {
+        ok = ok && hipMemcpy(records, scene->emTris, 3 * n * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!ok) {
+        ptSetError("ptSceneReadEmissive: copy failed");
+    }
+    return ok;
+}
+
 float ptSceneLastBuildMs(const PtScene* scene) { return scene != nullptr ? scene->buildMs : 0.0f; }
 
 PtBuildWorkspace* ptBuildWorkspaceCreate(int maxTriangles)

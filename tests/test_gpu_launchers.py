@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from lsystemeditor_amd import abi
+from lsystemeditor_amd import abi, scenes
 from oracle import oracle as O
 from tests.helpers import dense_plant, mismatch_report
 from tests.test_gpu_parity import compare_accumulators, oracle_scene
@@ -197,3 +197,30 @@ def test_region_workflow_matches_oracle(R):
     for seed in range(17, 20):
         R.adaptive_pass(sc, dev, inputs, g_rebuilt, seed)
     compare_accumulators(dev.host(), a, "region")
+
+
+@pytest.mark.parametrize("which", ["c0", "zoo", "c1small"])
+def test_emissive_list_matches_oracle(cuda, which):
+    """The device emissive list and CDF (MeshAccelScene.cpp:62-95) equal the oracle's bit for bit,
+    and the packed records hold the emitters' vertices and radiance."""
+    if which == "c0":
+        s = scenes.config0()
+        tris, mats, tex = s.triangles, s.materials, None
+    elif which == "zoo":
+        tris, mats, tex, _, _ = dense_plant()
+    else:
+        s = scenes.config1(width=64, height=64, iterations=4)
+        tris, mats, tex = s.triangles, s.materials, None
+    sc = abi.Scene(tris, mats, tex, bvh="sah" if which == "c1small" else "median")
+    idx, cdf, rec = sc.read_emissive()
+    ref = O.OracleScene(abi.host_to_gpu_triangles(tris), mats, tex)
+    n = ref.em_count
+    assert idx.size == n and n >= 2
+    assert np.array_equal(idx, ref.em_idx[:n])
+    assert np.array_equal(cdf.view(np.uint32), ref.em_cdf[: n + 1].view(np.uint32))
+    assert np.all(np.diff(cdf) > 0) and cdf[0] == 0.0
+    v = tris[idx, 0:9].reshape(n, 3, 3)
+    m = mats[tris[idx, 24].view(np.uint32)]
+    assert np.array_equal(rec[:, [0, 1, 2]], v[:, 0]) and np.array_equal(rec[:, [4, 5, 6]], v[:, 1])
+    assert np.array_equal(rec[:, [8, 9, 10]], v[:, 2])
+    assert np.array_equal(rec[:, [3, 7, 11]], (m[:, 0:3] * m[:, 9:10]).astype(np.float32))
