@@ -14,6 +14,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libpt_hip.so"
 HEADER = PKG.parent / "include" / "pt_hip.h"
+PROCEDURAL_HEADER = PKG.parent / "include" / "pt_procedural.h"
 TRACER_HEADER = PKG.parent / "include" / "pt_tracer.h"
 
 # Record sizes in 32-bit words (reference layouts, include/pt_hip.h).
@@ -173,10 +174,12 @@ SIGNATURES = {
 
 
 def header_symbols() -> list[str]:
-    """Function names declared with C linkage in include/pt_hip.h and include/pt_tracer.h."""
+    """Function names declared with C linkage in include/pt_hip.h, include/pt_tracer.h and
+    include/pt_procedural.h."""
     text = HEADER.read_text()
     tracer = TRACER_HEADER.read_text()
     text += tracer[: tracer.index("} /* extern \"C\" */")]
+    text += PROCEDURAL_HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)) - {
         "if", "defined", "alignas", "_Alignas", "sizeof", "PT_ALIGN16", "void"})

@@ -305,6 +305,11 @@ int orc_rebuild_active(const uint8_t* converged, const uint32_t* counts, int* ou
 void orc_uv_debug(const OCamera* cam, const OScene* scene, uint8_t* rgba, int width, int height,
                   const OParams* params);
 
+/* Procedural mesh (orc_mesh.c): the reference's lofted tube of a frame list (13 floats per frame:
+ * position, tangent, normal, binormal, radius) as 25-word HostTriangles, and its root transform. */
+int orc_loft_triangles(const float* frames, int frameCount, int circularSegments, float* out);
+void orc_root_transform(float* tris, int count, const float translation[3], const float rotationDeg[3]);
+
 #ifdef __cplusplus
 }
 #endif

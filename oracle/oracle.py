@@ -15,6 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 REF_PATH = HERE / "_ref" / "libref_policy.so"
+REF_PROC_PATH = HERE / "_ref" / "libref_procedural.so"
 
 
 def build(force: bool = False) -> Path:
@@ -181,6 +182,8 @@ def lib():
             "orc_set_spectral": (None, [C.POINTER(SpecModel)]),
             "orc_set_light_sampling": (None, [C.c_int]),
             "orc_emissive_pdf_hit": (f, [C.POINTER(Scene), C.c_uint32, Vec3, f]),
+            "orc_loft_triangles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+            "orc_root_transform": (None, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
             "orc_rng_init": (None, [C.POINTER(Rng), C.c_int, C.c_int, C.c_uint32]),
             "orc_burley_sample_radius": (f, [f, f]), "orc_burley_profile": (f, [f, f]),
             "orc_burley_pdf_radius": (f, [f, f]),
@@ -223,6 +226,47 @@ def ref_policy():
     for n in ("ref_rgb2spec_reflectance", "ref_rgb2spec_radiance"):
         getattr(L, n).restype = C.c_float
         getattr(L, n).argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_float] * 4
+    return L
+
+
+def loft_triangles(frames: np.ndarray, circular_segments: int) -> np.ndarray:
+    """orc_loft_triangles: (K, 13) frames -> (T, 25) HostTriangle words (material word 0)."""
+    f = np.ascontiguousarray(frames, np.float32).reshape(-1, 13)
+    L = lib()
+    n = L.orc_loft_triangles(f.ctypes.data, f.shape[0], circular_segments, None)
+    out = np.zeros((n, 25), np.float32)
+    if n:
+        L.orc_loft_triangles(f.ctypes.data, f.shape[0], circular_segments, out.ctypes.data)
+    return out
+
+
+def root_transform(tris: np.ndarray, translation, rotation_deg) -> np.ndarray:
+    """orc_root_transform on a copy of (T, 25) HostTriangle words."""
+    out = np.ascontiguousarray(tris, np.float32).copy()
+    t = np.asarray(translation, np.float32)
+    r = np.asarray(rotation_deg, np.float32)
+    lib().orc_root_transform(out.ctypes.data, out.shape[0], t.ctypes.data, r.ctypes.data)
+    return out
+
+
+def ref_procedural():
+    """The reference's L-system library, turtle and spline compiled from its sources, or None."""
+    if not REF_PROC_PATH.exists():
+        return None
+    L = C.CDLL(str(REF_PROC_PATH))
+    p, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+    L.ref_last_error.restype = C.c_char_p
+    L.ref_last_error.argtypes = []
+    L.ref_lsystem_parse_dump.restype = C.c_int64
+    L.ref_lsystem_parse_dump.argtypes = [C.c_int, C.c_char_p, p, u64]
+    L.ref_lsystem_evaluate_text.restype = C.c_int64
+    L.ref_lsystem_evaluate_text.argtypes = [C.c_char_p, u64, u64, C.c_int, p, u64]
+    L.ref_lsystem_materials_text.restype = C.c_int64
+    L.ref_lsystem_materials_text.argtypes = [C.c_char_p, p, u64]
+    L.ref_turtle_execute.restype = C.c_int
+    L.ref_turtle_execute.argtypes = [C.c_char_p, u64, C.c_float, C.c_float, p, p, u32, p, u32, p, u32, p, u32]
+    L.ref_spline_frames.restype = C.c_int
+    L.ref_spline_frames.argtypes = [p, u32, C.c_float, C.c_int, C.c_int, p, u32, p, p]
     return L
 
 
