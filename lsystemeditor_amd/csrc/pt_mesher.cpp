@@ -775,7 +775,10 @@ void loft_triangles(const std::vector<Frame>& frames, int segments, std::vector<
     const Ring ring(segments);
     const int rings = static_cast<int>(frames.size());
     const float vStep = 1.0f / static_cast<float>(rings - 1);
-    out.reserve(out.size() + static_cast<size_t>((rings - 1) * segments * 2 + segments * 2));
+    const size_t need = out.size() + static_cast<size_t>((rings - 1) * segments * 2 + segments * 2);
+    if (need > out.capacity()) {
+        out.reserve(std::max(need, 2 * out.capacity()));   // geometric growth: appends stay linear
+    }
     // side wall: column k in 0..segments (column `segments` repeats column 0's position with u = 1)
     auto wall = [&](const Frame& f, int k, float v) {
         const Vec3 p = ring.at(f, k < segments ? k : 0);

@@ -15,8 +15,9 @@ def dense_plant(width=64, height=64, spp=8, iterations=3, radius=0.35, max_depth
     materials="zoo" cycles the branch material over diffuse, rough metal, glass, thin glass,
     subsurface, textured (grid albedo, stripe roughness) and a diffuse-roughness override.
     """
-    ls = scenes.LSystem("F", {"F": [(1.0, "F[+F]F[-F]F")]}, iterations)
-    t = scenes.run_turtle(ls.expand(), scenes.TurtleParams(radius=radius, radius_decay=0.9))
+    body, _, _, lo, hi = scenes.plant(scenes.PLANAR_PLANT, iterations, scenes.planar_params(8, radius))
+    per = 8 * 4     # one capped 8-sided tube per segment: 16 wall + 16 cap triangles
+    seg = np.arange(body.shape[0]) // per
     parts = []
     nmat = 0
     mats = []
@@ -34,9 +35,6 @@ def dense_plant(width=64, height=64, spp=8, iterations=3, radius=0.35, max_depth
             abi.pack_material(1.0, 1.0, 1.0, roughness=0.35, transmission=1.0, ior=1.33),
         ]
         nmat = len(mats)
-        per = 8 * 2
-        body = scenes.cylinders(t, 8, 0)
-        seg = np.arange(body.shape[0]) // per
         body[:, 24] = (seg % nmat).astype(np.uint32).view(np.float32)
         parts.append(body)
     elif materials == "diff":
@@ -44,16 +42,13 @@ def dense_plant(width=64, height=64, spp=8, iterations=3, radius=0.35, max_depth
         mats = [abi.pack_material(0.4, 0.3, 0.2, roughness=0.6, metallic=0.2, specular=0.5),
                 abi.pack_material(0.2, 0.6, 0.15, roughness=0.35, metallic=0.1, specular=0.7)]
         nmat = 2
-        body = scenes.cylinders(t, 8, 0)
-        seg = np.arange(body.shape[0]) // 16
         body[:, 24] = (seg % nmat).astype(np.uint32).view(np.float32)
         parts.append(body)
     else:
         mats = [abi.pack_material(0.4, 0.3, 0.2, roughness=0.6)]
         nmat = 1
-        parts.append(scenes.cylinders(t, 8, 0))
-    lo = np.minimum(t.p0.min(0), t.p1.min(0))
-    hi = np.maximum(t.p0.max(0), t.p1.max(0))
+        body[:, 24] = np.zeros(body.shape[0], np.uint32).view(np.float32)
+        parts.append(body)
     c = 0.5 * (lo + hi)
     ext = 0.5 * (hi - lo) + 2.0
     box_m = nmat
