@@ -530,6 +530,13 @@ bool ptEnvironmentReadTables(const PtEnvironment* env, float* marginal, float* r
 
 void ptEnvironmentDestroy(PtEnvironment* env);
 
+/* Reads a Radiance RGBE picture (.hdr; the reference's stbi_loadf path, EnvironmentMap.cpp:191-210):
+ * flat or run-length scanlines, orientation -Y H +X W, texel = (r, g, b) * 2^(e - 136) (0 when
+ * e = 0). Returns width * height * 3 floats (rows top to bottom, as ptEnvironmentCreate takes
+ * them) to be released with ptEnvironmentFreeImage, or NULL with the error set. */
+float* ptEnvironmentLoadHdr(const char* path, int* width, int* height);
+void ptEnvironmentFreeImage(float* rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@
 #define PT_TRACER_H
 
 #include "pt_hip.h"
+#include "pt_procedural.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -39,6 +40,15 @@ bool ptTracerConfigure(PtTracer* t, int width, int height, const HostTriangle* t
 bool ptTracerRebuildMeshScene(PtTracer* t, const HostTriangle* triangles, uint32_t triangleCount,
                               const MaterialGpu* materials, uint32_t materialCount, const TextureDescGpu* textures,
                               uint32_t textureCount);
+/* PathTracer::configure / rebuildMeshScene with the reference's scene description
+ * (PathTracer.h:44-50, 137-139): the instances go through meshSceneBuild
+ * (MeshSceneContent.cpp:5-52; ptProceduralBuildScene with circularSegments and creaseAngleDeg
+ * from params, NULL = {64, 50}). An empty list builds an empty scene. false, with the mesher's
+ * message ("Procedural instance <i>: ..."), when an instance fails. */
+bool ptTracerConfigureProcedural(PtTracer* t, int width, int height, const PtProceduralInstance* instances,
+                                 uint32_t count, const PtMeshSceneBuildParams* params);
+bool ptTracerRebuildMeshSceneProcedural(PtTracer* t, const PtProceduralInstance* instances, uint32_t count,
+                                        const PtMeshSceneBuildParams* params);
 
 void ptTracerStart(PtTracer* t);
 void ptTracerStop(PtTracer* t);
@@ -125,8 +135,9 @@ bool ptTracerLastWorkerError(const PtTracer* t, char* buffer, int capacity);
 } /* extern "C" */
 
 #include <functional>
-#include <string>
 #include <memory>
+#include <string>
+#include <vector>
 
 namespace ptdetail {
 struct TracerImpl;
@@ -158,6 +169,8 @@ public:
 
     void setFrameReadyCallback(FrameReadyCallback callback);
     bool configure(int width, int height, const PtHostMesh& mesh);
+    bool configure(int width, int height, const std::vector<PtProceduralInstance>& proceduralInstances,
+                   const PtMeshSceneBuildParams& meshParams = PtMeshSceneBuildParams{64, 50.0f});
     void start();
     void stop();
     void resetAccumulation();
@@ -200,6 +213,8 @@ public:
     void setRussianRouletteMinDepth(int depth);
     int russianRouletteMinDepth() const;
     bool rebuildMeshScene(const PtHostMesh& mesh);
+    bool rebuildMeshScene(const std::vector<PtProceduralInstance>& proceduralInstances,
+                          const PtMeshSceneBuildParams& meshParams = PtMeshSceneBuildParams{64, 50.0f});
     void setRegionRenderEnabled(bool enabled);
     bool regionRenderEnabled() const;
     void setRenderRegion(int minX, int minY, int maxX, int maxY);

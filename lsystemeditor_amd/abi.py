@@ -115,6 +115,10 @@ SIGNATURES = {
     "ptTracerSetFrameReadyCallback": (None, [_p, _p, _p]),
     "ptTracerConfigure": (_b, [_p, _i, _i, _p, _u, _p, _u, _p, _u]),
     "ptTracerRebuildMeshScene": (_b, [_p, _p, _u, _p, _u, _p, _u]),
+    "ptTracerConfigureProcedural": (_b, [_p, _i, _i, _p, _u, _p]),
+    "ptEnvironmentLoadHdr": (_p, [C.c_char_p, _p, _p]),
+    "ptEnvironmentFreeImage": (None, [_p]),
+    "ptTracerRebuildMeshSceneProcedural": (_b, [_p, _p, _u, _p]),
     "ptTracerStart": (None, [_p]),
     "ptTracerStop": (None, [_p]),
     "ptTracerResetAccumulation": (None, [_p]),
@@ -216,6 +220,21 @@ class LightSampling:
 
     def __exit__(self, *exc):
         lib().pathTracerSetLightSampling(self.prev)
+
+
+def load_hdr(path) -> np.ndarray:
+    """ptEnvironmentLoadHdr: a Radiance .hdr picture as a (height, width, 3) float32 array."""
+    L = lib()
+    w, h = C.c_int(0), C.c_int(0)
+    ptr = L.ptEnvironmentLoadHdr(str(path).encode(), C.byref(w), C.byref(h))
+    if not ptr:
+        raise RuntimeError(f"ptEnvironmentLoadHdr failed: {last_error()}")
+    try:
+        out = np.zeros((h.value, w.value, 3), np.float32)
+        C.memmove(out.ctypes.data, ptr, out.nbytes)
+        return out
+    finally:
+        L.ptEnvironmentFreeImage(ptr)
 
 
 def last_error() -> str:

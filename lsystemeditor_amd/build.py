@@ -22,7 +22,7 @@ INCLUDE = PKG.parent / "include"
 OUT = PKG / "libpt_hip.so"
 BUILD_DIR = PKG / "_build"
 
-SOURCES = ["pt_render.hip", "pt_wavefront.hip", "pt_scene.hip", "pt_sah.hip", "pt_adjoint.hip", "pt_spectral.hip", "pt_tracer.cpp", "pt_errors.cpp", "pt_lsystem.cpp", "pt_mesher.cpp"]
+SOURCES = ["pt_render.hip", "pt_wavefront.hip", "pt_scene.hip", "pt_sah.hip", "pt_adjoint.hip", "pt_spectral.hip", "pt_tracer.cpp", "pt_errors.cpp", "pt_lsystem.cpp", "pt_mesher.cpp", "pt_hdr.cpp"]
 
 COMMON = [
     "-O3",

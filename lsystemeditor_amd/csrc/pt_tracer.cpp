@@ -1,10 +1,20 @@
-// Headless renderer facade: class PathTracer over the launcher ABI (pt_hip.h).
+// pt_tracer.cpp -- the headless renderer facade (class PathTracer of include/pt_tracer.h).
 //
-// Follows PathTracerLib/src/PathTracer.cpp: configure (1196-1395), start/stop/reset (1397-1434),
-// region handling (1436-1470), display publish (1477-1708), setters with their clamps (1710-1990),
-// pickSurface (1991-2013) and the worker's renderLoop (2220-2543). Policies restated from
-// PathTracerSampleBudget.h, PathTracerPreviewLevels.h and PhysicalCamera.cpp.
-
+// Public behaviour follows the reference's PathTracer (PathTracerLib/src/PathTracer.h:30-152):
+// the same methods, clamps, reset rules, sample-budget policy (PathTracerSampleBudget.h), preview
+// pyramid (PathTracerPreviewLevels.h), exposure (PhysicalCamera.cpp:397-402) and per-pass
+// schedule of its render loop (PathTracer.cpp:2220-2543: preview levels first, then adaptive
+// passes with globalSeed = iteration + 1 and an active-list compaction every fourth
+// full-resolution pass). The internals are this library's own:
+//   - Controls: every user setting in one plain struct under one lock; the worker copies it once
+//     per step, and device copies of the camera and render parameters are refreshed by comparing
+//     them with what was last uploaded (no dirty flags);
+//   - Surfaces: the device state of one resolution (accumulator, two active lists, preview
+//     pyramid, two display buffers), owned through RAII arrays;
+//   - two device parameter blocks, one per stream (sampling and display), so a display publish
+//     never rewrites what a pass in flight reads;
+//   - the worker is a loop of plan -> launch -> await -> commit over an explicit Step record,
+//     with resets delivered as request bits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,227 +29,781 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/pt_procedural.h"
 #include "../../include/pt_tracer.h"
 #include "pt_internal.h"
 
 namespace {
 
-// PathTracer.cpp:126-131
-constexpr int kMaxSamplesUpperBound = 1000000;
-constexpr int kMaxPreviewStepsPerLevel = 128;
-constexpr int kMinMinSamples = 1;
-constexpr int kMaxMinSamples = 10000;
-constexpr float kMinRelativeErrorThreshold = 0.001f;
-constexpr float kMaxRelativeErrorThreshold = 1.0f;
-constexpr int kCompactActiveListInterval = 4; // PathTracerAdaptiveSampling.h:7
-constexpr int kPreviewMaxPathDepth = 4;       // PathTracerPreviewLevels.h:13
+// ---- reference policies ---------------------------------------------------------------------
 
-// PhysicalCamera.h:36-51
-constexpr float kDefaultFStop = 2.8f;
-constexpr float kDefaultShutterSpeedSeconds = 1.0f / 125.0f;
-constexpr float kDefaultIso = 100.0f;
-constexpr float kMinIso = 100.0f;
-constexpr float kMaxIso = 25600.0f;
-constexpr float kMinFStop = 1.2f;
-constexpr float kMaxFStop = 22.0f;
-constexpr float kMinShutterSpeedSeconds = 1.0f / 8000.0f;
-constexpr float kMaxShutterSpeedSeconds = 120.0f;
-constexpr float kDefaultFarPlane = 1000000.0f;
-constexpr float kDefaultFocusDistance = 1000.0f;
-constexpr float kSensorHeightMm = 23.9f;
-constexpr float kMinFocalLengthMm = 14.0f;
-constexpr float kMaxFocalLengthMm = 1000.0f;
-constexpr float kDefaultFocalLengthMm = 24.0f;
-
-int clampMaxSamples(int v) { return v < 0 ? 0 : (v > kMaxSamplesUpperBound ? kMaxSamplesUpperBound : v); }
-int clampMinSamples(int v) { return v < kMinMinSamples ? kMinMinSamples : (v > kMaxMinSamples ? kMaxMinSamples : v); }
-float clampRelativeErrorThreshold(float v)
+template <typename T>
+T clampTo(T v, T lo, T hi)
 {
-    return v < kMinRelativeErrorThreshold ? kMinRelativeErrorThreshold
-                                          : (v > kMaxRelativeErrorThreshold ? kMaxRelativeErrorThreshold : v);
-}
-int clampPreviewStepsPerLevel(int v) { return v < 0 ? 0 : (v > kMaxPreviewStepsPerLevel ? kMaxPreviewStepsPerLevel : v); }
-
-// PathTracerSampleBudget.h
-int budgetTotalIterations(int previewSteps, int maxSpp) { return maxSpp <= 0 ? -1 : previewSteps + maxSpp; }
-bool canTakeSampleAtIteration(int sampleCount, int previewSteps, int maxSpp)
-{
-    return maxSpp <= 0 || sampleCount < previewSteps + maxSpp;
-}
-bool budgetExhaustedAtIteration(int sampleCount, int previewSteps, int maxSpp)
-{
-    return maxSpp > 0 && sampleCount >= previewSteps + maxSpp;
+    return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// PathTracerPreviewLevels.h: level 0 is 1/2^N of full resolution, level N-1 is 1/2.
-struct LevelDims {
+// setter ranges (PathTracer.cpp:126-131, 1710-1990) and physical-camera ranges (PhysicalCamera.h:36-51)
+struct Range {
+    int lo, hi;
+};
+constexpr Range kMaxSppRange{0, 1000000};
+constexpr Range kMinSppRange{1, 10000};
+constexpr Range kOverlayRange{0, 3};
+constexpr Range kPreviewStepRange{0, 128};
+constexpr Range kRouletteRange{0, 256};
+constexpr Range kPassesRange{1, 256};
+constexpr Range kBuildModeRange{0, 2};
+constexpr float kThresholdLo = 0.001f, kThresholdHi = 1.0f;
+constexpr float kIntensityLo = 0.0f, kIntensityHi = 100.0f;
+constexpr int kDepthCap = 256;
+constexpr float kFStopLo = 1.2f, kFStopHi = 22.0f;
+constexpr float kShutterLo = 1.0f / 8000.0f, kShutterHi = 120.0f;
+constexpr float kIsoLo = 100.0f, kIsoHi = 25600.0f;
+
+constexpr int kCompactEvery = 4;      // PathTracerAdaptiveSampling.h:7
+constexpr int kPreviewDepth = 4;      // PathTracerPreviewLevels.h:13
+constexpr int kUvOverlay = 3;         // RenderViewOverlayMode::Uv
+
+// PathTracerSampleBudget.h: a positive sample limit bounds preview steps + full passes.
+int budgetTotal(int previewSteps, int maxSpp) { return maxSpp <= 0 ? -1 : previewSteps + maxSpp; }
+bool budgetAllows(int count, int previewSteps, int maxSpp) { return maxSpp <= 0 || count < previewSteps + maxSpp; }
+
+// PathTracerPreviewLevels.h: level i of n is 1/2^(n-i) of full resolution (at least 1 pixel).
+struct LevelSize {
     int width, height, downscale;
 };
-LevelDims previewLevelDims(int fullW, int fullH, int levels, int index)
+LevelSize levelSize(int w, int h, int levels, int index)
 {
-    int ds = 1;
-    if (levels > 0 && index >= 0 && index < levels) {
-        ds = 1 << (levels - index);
+    const int ds = levels > 0 && index >= 0 && index < levels ? 1 << (levels - index) : 1;
+    auto axis = [ds](int full) { return full <= 0 ? 0 : std::max(1, full / ds); };
+    return {axis(w), axis(h), ds};
+}
+
+// Camera model of PhysicalCamera.cpp: exposure (397-402), aperture from focal length and f-stop
+// (291-298), focal length from the vertical field of view on a 23.9 mm sensor (348-363).
+struct Optics {
+    static constexpr float kSensorMm = 23.9f;
+    static constexpr float kFocalLo = 14.0f, kFocalHi = 1000.0f;
+
+    CameraGpu geometry{};
+    float fStop = 2.8f;
+    float shutter = 1.0f / 125.0f;
+    float iso = 100.0f;
+    float focalMm = 24.0f;
+    float focusDistance = 1000.0f;
+    bool focusOn = true;
+
+    Optics()
+    {
+        geometry.orientation = PtFloat4{1.0f, 0.0f, 0.0f, 0.0f};
+        geometry.fovY = 2.0f * std::atan(kSensorMm / (2.0f * focalMm));
+        geometry.aspect = 1.0f;
+        geometry.nearPlane = focalMm;
+        geometry.farPlane = 1000000.0f;
     }
-    auto axis = [ds](int full) { return full <= 0 || ds <= 0 ? 0 : std::max(1, full / ds); };
-    return {axis(fullW), axis(fullH), ds};
-}
 
-// PhysicalCamera.cpp:397-402
-float exposureMultiplierOf(float fStop, float shutter, float iso)
-{
-    const float f = fStop < 0.1f ? 0.1f : fStop;
-    const float i = iso < 1.0f ? 1.0f : iso;
-    return shutter * (i / 100.0f) / (f * f);
-}
+    float exposure() const
+    {
+        const float f = std::max(fStop, 0.1f);
+        return shutter * (std::max(iso, 1.0f) / 100.0f) / (f * f);
+    }
 
-float clampFocal(float mm) { return std::max(kMinFocalLengthMm, std::min(mm, kMaxFocalLengthMm)); }
-float fovYToFocalLengthMm(float fovY)
-{
-    const float tanHalf = std::tan(fovY * 0.5f);
-    return tanHalf <= 0.0f ? kMaxFocalLengthMm : clampFocal(kSensorHeightMm / (2.0f * tanHalf));
-}
+    // -Z of camera space in world space: (0, 0, -1) turned by the orientation quaternion
+    // (w, x, y, z), v + w t + u x t with t = 2 u x v (CameraDevice.cuh:8-25 uses the same rotation).
+    Vec3 viewAxis() const
+    {
+        const PtFloat4& q = geometry.orientation;
+        const float w = q.x, x = q.y, y = q.z, z = q.w;
+        const Vec3 v = {0.0f, 0.0f, -1.0f};
+        const Vec3 t = {2.0f * (y * v.z - z * v.y), 2.0f * (z * v.x - x * v.z), 2.0f * (x * v.y - y * v.x)};
+        return Vec3{v.x + w * t.x + (y * t.z - z * t.y), v.y + w * t.y + (z * t.x - x * t.z),
+                    v.z + w * t.z + (x * t.y - y * t.x)};
+    }
 
-// Rotates v by the unit quaternion stored as (w, x, y, z) in CameraGpu::orientation.
-Vec3 rotateByQuat(const PtFloat4& q, Vec3 v)
-{
-    const float w = q.x, qx = q.y, qy = q.z, qz = q.w;
-    const Vec3 t = {2.0f * (qy * v.z - qz * v.y), 2.0f * (qz * v.x - qx * v.z), 2.0f * (qx * v.y - qy * v.x)};
-    return {v.x + w * t.x + (qy * t.z - qz * t.y), v.y + w * t.y + (qz * t.x - qx * t.z),
-            v.z + w * t.z + (qx * t.y - qy * t.x)};
-}
+    void takeGeometry(const CameraGpu& c)
+    {
+        geometry = c;
+        const float t = std::tan(c.fovY * 0.5f);
+        focalMm = t <= 0.0f ? kFocalHi : clampTo(kSensorMm / (2.0f * t), kFocalLo, kFocalHi);
+        focusDistance = c.focusDistance;
+        focusOn = c.focusValid != 0;
+    }
 
-enum class WaitResult { Completed, ResetDuringWait, Stopped, Error };
+    CameraGpu forDevice(int width, int height) const
+    {
+        CameraGpu g = geometry;
+        g.apertureRadius = focusOn && fStop > 0.0f ? focalMm / (2.0f * fStop) : 0.0f;
+        g.focusDistance = focusOn ? focusDistance : 0.0f;
+        const Vec3 a = viewAxis();
+        g.focusPoint = Vec3{geometry.position.x + a.x * focusDistance, geometry.position.y + a.y * focusDistance,
+                            geometry.position.z + a.z * focusDistance};
+        g.focusValid = focusOn ? 1 : 0;
+        if (width > 0 && height > 0) {
+            g.aspect = static_cast<float>(width) / static_cast<float>(height);
+        }
+        return g;
+    }
+};
+
+// Everything a caller can set (defaults: PathTracer.cpp:60-84, RenderTypes.h:15-29).
+struct Controls {
+    int maxSpp = 8;
+    int minSpp = 16;
+    float threshold = 0.02f;
+    int overlay = 0;
+    int brdfFlags = 0;
+    int previewSteps = 0;
+    int passesPerLaunch = 1;
+    int buildMode = 0;
+    int maxDepth = 32;
+    int rouletteDepth = 3;
+    float envIntensity = 1.0f;
+    float clear[3] = {10.0f / 255.0f, 10.0f / 255.0f, 10.0f / 255.0f};
+    bool regionOn = false;
+    PtRegion region{0, 0, 0, 0};
+    Optics optics;
+
+    RenderParamsGpu params(bool preview) const
+    {
+        RenderParamsGpu p{};
+        p.backgroundR = clear[0];
+        p.backgroundG = clear[1];
+        p.backgroundB = clear[2];
+        p.environmentIntensity = envIntensity;
+        p.maxPathDepth = preview ? kPreviewDepth : maxDepth;
+        p.russianRouletteMinDepth = rouletteDepth;
+        p.minSamples = minSpp;
+        p.maxSamplesPerPixel = maxSpp;
+        p.debugOverlayMode = overlay;
+        p.brdfDebugFlags = brdfFlags;
+        p.relativeErrorThreshold = threshold;
+        return p;
+    }
+};
+
+// ---- device memory --------------------------------------------------------------------------
+
+template <typename T>
+class DeviceArray {
+public:
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept { swap(o); }
+    DeviceArray& operator=(DeviceArray&& o) noexcept
+    {
+        reset();
+        swap(o);
+        return *this;
+    }
+    ~DeviceArray() { reset(); }
+
+    bool allocate(size_t n)
+    {
+        reset();
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+            return false;
+        }
+        ptr_ = static_cast<T*>(p);
+        size_ = n;
+        return true;
+    }
+    void reset()
+    {
+        if (ptr_ != nullptr) {
+            (void)hipFree(ptr_);
+        }
+        ptr_ = nullptr;
+        size_ = 0;
+    }
+    T* get() const { return ptr_; }
+    size_t size() const { return size_; }
+    explicit operator bool() const { return ptr_ != nullptr; }
+    void swap(DeviceArray& o)
+    {
+        std::swap(ptr_, o.ptr_);
+        std::swap(size_, o.size_);
+    }
+
+private:
+    T* ptr_ = nullptr;
+    size_t size_ = 0;
+};
+
+// A device copy of a small POD with the host value it was last given: upload() copies only when
+// the value changed (or was never uploaded).
+template <typename T>
+class DeviceValue {
+public:
+    bool allocate()
+    {
+        known_ = false;
+        return mem_.allocate(1);
+    }
+    void release()
+    {
+        mem_.reset();
+        known_ = false;
+    }
+    void forget() { known_ = false; }
+    const T* get() const { return mem_.get(); }
+    bool upload(const T& v, hipStream_t s)
+    {
+        if (!mem_) {
+            return false;
+        }
+        if (known_ && std::memcmp(&v, &last_, sizeof(T)) == 0) {
+            return true;
+        }
+        last_ = v;
+        if (hipMemcpyAsync(mem_.get(), &last_, sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            known_ = false;
+            return false;
+        }
+        known_ = true;
+        return true;
+    }
+
+private:
+    DeviceArray<T> mem_;
+    T last_{};
+    bool known_ = false;
+};
+
+struct PreviewBuffer {
+    DeviceArray<PtFloat4> texels;
+    LevelSize size{0, 0, 1};
+};
+
+// Device state of one configured resolution.
+struct Surfaces {
+    int width = 0, height = 0;
+    DeviceArray<PtFloat4> radiance;
+    DeviceArray<uint32_t> counts;
+    DeviceArray<float> mean, m2;
+    DeviceArray<uint8_t> converged;
+    DeviceArray<int> active, activeNext, activeCount;
+    std::vector<PreviewBuffer> pyramid;
+    DeviceArray<PtUchar4> display[2];
+    DeviceValue<CameraGpu> sampleCamera, displayCamera;
+    DeviceValue<RenderParamsGpu> sampleParams, displayParams;
+
+    size_t pixels() const { return static_cast<size_t>(width) * static_cast<size_t>(height); }
+    bool ready() const { return static_cast<bool>(radiance); }
+
+    bool allocate(int w, int h, int previewLevels)
+    {
+        release();
+        width = w;
+        height = h;
+        const size_t n = pixels();
+        const bool ok = radiance.allocate(n) && counts.allocate(n) && mean.allocate(n) && m2.allocate(n) &&
+                        converged.allocate(n) && active.allocate(n) && activeNext.allocate(n) &&
+                        activeCount.allocate(1) && display[0].allocate(n) && display[1].allocate(n) &&
+                        sampleCamera.allocate() && displayCamera.allocate() && sampleParams.allocate() &&
+                        displayParams.allocate() && buildPyramid(previewLevels);
+        if (!ok) {
+            release();
+        }
+        return ok;
+    }
+    bool buildPyramid(int levels)
+    {
+        pyramid.clear();
+        pyramid.resize(static_cast<size_t>(levels));
+        for (int i = 0; i < levels; ++i) {
+            PreviewBuffer& b = pyramid[static_cast<size_t>(i)];
+            b.size = levelSize(width, height, levels, i);
+            if (!b.texels.allocate(static_cast<size_t>(b.size.width) * static_cast<size_t>(b.size.height))) {
+                pyramid.clear();
+                return false;
+            }
+        }
+        return true;
+    }
+    bool zeroPyramid(hipStream_t s) const
+    {
+        for (const PreviewBuffer& b : pyramid) {
+            if (hipMemsetAsync(b.texels.get(), 0, b.texels.size() * sizeof(PtFloat4), s) != hipSuccess) {
+                return false;
+            }
+        }
+        return true;
+    }
+    void release()
+    {
+        radiance.reset();
+        counts.reset();
+        mean.reset();
+        m2.reset();
+        converged.reset();
+        active.reset();
+        activeNext.reset();
+        activeCount.reset();
+        pyramid.clear();
+        display[0].reset();
+        display[1].reset();
+        sampleCamera.release();
+        displayCamera.release();
+        sampleParams.release();
+        displayParams.release();
+        width = height = 0;
+    }
+};
+
+// One unit of worker work.
+struct Step {
+    bool preview = false;
+    int iteration = 0;     // sample count when planned (preview level index for previews)
+    int passes = 1;
+    int active = 0;
+    bool compact = false;
+};
+
+enum : unsigned { kFullReset = 1u, kRegionReset = 2u };
+enum class Wait { Done, Interrupted, Stopped, Failed };
 
 } // namespace
 
 namespace ptdetail {
 
-struct Accumulator {
-    PtFloat4* buffer = nullptr;
-    uint32_t* samples = nullptr;
-    float* lumMean = nullptr;
-    float* m2 = nullptr;
-    uint8_t* converged = nullptr;
-    int* activeIndices = nullptr;
-    int* activeScratch = nullptr;
-    int* activeCount = nullptr;
-    int width = 0;
-    int height = 0;
-};
-
-struct PreviewLevel {
-    PtFloat4* buffer = nullptr;
-    int width = 0, height = 0, downscale = 1;
-};
-
-// PhysicalCamera state the renderer uses (geometry, exposure, focus).
-struct HostCamera {
-    CameraGpu geom{};
-    float fStop = kDefaultFStop;
-    float shutter = kDefaultShutterSpeedSeconds;
-    float iso = kDefaultIso;
-    float focalLengthMm = kDefaultFocalLengthMm;
-    float focusDistance = kDefaultFocusDistance;
-    bool focusValid = true;
-
-    Vec3 forward() const { return rotateByQuat(geom.orientation, Vec3{0.0f, 0.0f, -1.0f}); }
-
-    // PhysicalCamera::toGpu (PhysicalCamera.cpp:314-329) with the aperture of 291-298.
-    CameraGpu toGpu() const
-    {
-        CameraGpu g = geom;
-        g.apertureRadius = focusValid && fStop > 0.0f ? focalLengthMm / (2.0f * fStop) : 0.0f;
-        g.focusDistance = focusValid ? focusDistance : 0.0f;
-        const Vec3 f = forward();
-        g.focusPoint = Vec3{geom.position.x + f.x * focusDistance, geom.position.y + f.y * focusDistance,
-                            geom.position.z + f.z * focusDistance};
-        g.focusValid = focusValid ? 1 : 0;
-        return g;
-    }
-
-    // PhysicalCamera::applyGpuGeometry (PhysicalCamera.cpp:348-363)
-    void applyGpuGeometry(const CameraGpu& c)
-    {
-        geom = c;
-        focalLengthMm = fovYToFocalLengthMm(c.fovY);
-        focusDistance = c.focusDistance;
-        focusValid = c.focusValid != 0;
-    }
-};
-
 struct TracerImpl {
-    Accumulator acc;
-    std::vector<PreviewLevel> previewLevels;
-    PtUchar4* display[2] = {nullptr, nullptr};
-    CameraGpu* dCamera = nullptr;
-    RenderParamsGpu* dParams = nullptr;
+    // settings
+    mutable std::mutex controlsLock;
+    Controls controls;
+    CameraGpu lastCamera{};
+
+    // device state; streamLock orders everything enqueued on the two streams
+    std::mutex streamLock;
+    Surfaces surf;
+    mutable std::mutex sceneLock;
     PtScene* scene = nullptr;
     PtEnvironment* env = nullptr;
-    std::vector<float> envImage;
-    int envWidth = 0, envHeight = 0;
-    hipStream_t sampleStream = nullptr;
-    hipStream_t displayStream = nullptr;
-    hipEvent_t sampleCompleteEvent = nullptr;
-    hipEvent_t displayReadyEvent = nullptr;
+    std::vector<float> envPixels;
+    int envSize[2] = {0, 0};
+    unsigned envRevision = 1, envUploaded = 0;   // guarded by streamLock
+    hipStream_t sampling = nullptr, displaying = nullptr;
+    hipEvent_t stepDone = nullptr, publishDone = nullptr;
 
+    // worker
     std::thread worker;
-    std::mutex workerMutex;
-    std::condition_variable workerCv;
-    mutable std::mutex cameraMutex;
-    mutable std::mutex lastSampleCameraMutex;
-    mutable std::mutex sceneMutex;
-    std::mutex streamMutex;
-    std::mutex errorMutex;
-    std::string workerError;  // the last failed launch of the worker thread (its own error text)
-    std::mutex callbackMutex;
-    PathTracer::FrameReadyCallback callback;
+    std::mutex wakeLock;
+    std::condition_variable wake;
+    std::atomic<bool> running{false}, configured{false}, idle{false};
+    std::atomic<unsigned> requests{0};
+    std::atomic<int> sampleCount{0}, activePixels{0}, finestPreview{-1};
+    std::atomic<bool> displayable{false};
+    int countReadback = 0;                 // written by the compaction's device-to-host copy
+    mutable std::mutex errorLock;
+    std::string workerError;
+    std::mutex callbackLock;
+    PathTracer::FrameReadyCallback frameReady;
 
-    HostCamera hostCamera;
-    CameraGpu lastSampleCamera{};
-    RenderParamsGpu hostParams{};
-    bool deviceParamsArePreview = false;
+    // display publish
+    std::atomic<bool> publishing{false};
+    std::atomic<int> publishSlot{-1};
 
-    std::atomic<bool> cameraDirty{true};
-    std::atomic<bool> displayPublishInFlight{false};
-    std::atomic<int> displayPublishSlot{-1};
-    std::atomic<bool> running{false};
-    std::atomic<bool> resetRequested{false};
-    std::atomic<bool> regionResetRequested{false};
-    std::atomic<bool> configured{false};
-    std::atomic<int> maxSamplesPerPixel{8};
-    std::atomic<int> minSamples{16};
-    std::atomic<float> relativeErrorThreshold{0.02f};
-    std::atomic<int> debugOverlayMode{0};
-    std::atomic<int> brdfDebugFlags{0};
-    std::atomic<int> activePixelCount{0};
-    std::atomic<int> previewStepsPerLevel{0};
-    std::atomic<int> passesPerLaunch{1};
-    std::atomic<int> bvhBuildMode{0};
-    std::atomic<int> sampleCount{0};
-    std::atomic<int> finestCompletedPreview{-1};
-    std::atomic<bool> displayBuffersValid{false};
-    std::atomic<bool> environmentDirty{true};
-    std::atomic<bool> regionEnabled{false};
-    std::atomic<int> regionMinX{0}, regionMinY{0}, regionMaxX{0}, regionMaxY{0};
-    std::atomic<bool> renderParamsDirty{true};
-    std::atomic<bool> idle{false};
-
-    TracerImpl()
+    ~TracerImpl()
     {
-        hostParams.backgroundR = hostParams.backgroundG = hostParams.backgroundB = 10.0f / 255.0f;
-        hostParams.environmentIntensity = 1.0f;
-        hostParams.maxPathDepth = 32;
-        hostParams.russianRouletteMinDepth = 3;
-        hostParams.minSamples = 16;
-        hostParams.maxSamplesPerPixel = 1024;
-        hostParams.relativeErrorThreshold = 0.02f;
-        CameraGpu& g = hostCamera.geom;
-        g.orientation = PtFloat4{1.0f, 0.0f, 0.0f, 0.0f};
-        g.fovY = 2.0f * std::atan(kSensorHeightMm / (2.0f * kDefaultFocalLengthMm));
-        g.aspect = 1.0f;
-        g.nearPlane = kDefaultFocalLengthMm;
-        g.farPlane = kDefaultFarPlane;
+        releaseDevice();
+        if (stepDone != nullptr) {
+            (void)hipEventDestroy(stepDone);
+        }
+        if (publishDone != nullptr) {
+            (void)hipEventDestroy(publishDone);
+        }
+        if (sampling != nullptr) {
+            (void)hipStreamDestroy(sampling);
+        }
+        if (displaying != nullptr) {
+            (void)hipStreamDestroy(displaying);
+        }
+    }
+
+    Controls snapshot() const
+    {
+        std::lock_guard<std::mutex> g(controlsLock);
+        return controls;
+    }
+
+    template <typename F>
+    void edit(F&& change)
+    {
+        {
+            std::lock_guard<std::mutex> g(controlsLock);
+            change(controls);
+        }
+        poke();
+    }
+
+    void poke()
+    {
+        { std::lock_guard<std::mutex> g(wakeLock); }
+        wake.notify_all();
+    }
+
+    void request(unsigned bits)
+    {
+        requests.fetch_or(bits);
+        poke();
+    }
+
+    // resets that setters ask for apply only once a scene is configured
+    void resetIfConfigured()
+    {
+        if (configured.load()) {
+            request(kFullReset);
+        }
+    }
+
+    bool ensureStreams()
+    {
+        return (sampling != nullptr || hipStreamCreate(&sampling) == hipSuccess) &&
+               (displaying != nullptr || hipStreamCreate(&displaying) == hipSuccess) &&
+               (stepDone != nullptr || hipEventCreateWithFlags(&stepDone, hipEventDisableTiming) == hipSuccess) &&
+               (publishDone != nullptr || hipEventCreateWithFlags(&publishDone, hipEventDisableTiming) == hipSuccess);
+    }
+
+    void releaseDevice()
+    {
+        surf.release();
+        if (env != nullptr) {
+            ptEnvironmentDestroy(env);
+            env = nullptr;
+        }
+        envUploaded = 0;
+        std::lock_guard<std::mutex> g(sceneLock);
+        if (scene != nullptr) {
+            ptSceneDestroy(scene);
+            scene = nullptr;
+        }
+    }
+
+    bool buildScene(const PtHostMesh& mesh, int buildMode)
+    {
+        std::lock_guard<std::mutex> g(sceneLock);
+        if (scene == nullptr && (scene = ptSceneCreate()) == nullptr) {
+            return false;
+        }
+        return ptSceneSetBuildMode(scene, buildMode) &&
+               ptSceneBuild(scene, mesh.triangles, mesh.triangleCount, mesh.materials, mesh.materialCount,
+                            mesh.textures, mesh.textureCount, sampling);
+    }
+
+    const MeshAccelSceneGpu* deviceScene() const
+    {
+        std::lock_guard<std::mutex> g(sceneLock);
+        return ptSceneDevice(scene);
+    }
+
+    // ---- accumulator resets (PathTracer.cpp:511-632 behaviour), streamLock held -------------
+
+    bool clearAll()
+    {
+        Surfaces& s = surf;
+        if (!pathTracerClearAccumulator(s.radiance.get(), s.counts.get(), s.mean.get(), s.m2.get(), s.converged.get(),
+                                        s.active.get(), s.activeNext.get(), s.activeCount.get(), s.width, s.height,
+                                        sampling) ||
+            !s.zeroPyramid(sampling)) {
+            return false;
+        }
+        activePixels.store(s.width * s.height);
+        return true;
+    }
+
+    bool clearRegion(const PtRegion& r)
+    {
+        Surfaces& s = surf;
+        if (!pathTracerClearRegionAccumulator(s.radiance.get(), s.counts.get(), s.mean.get(), s.m2.get(),
+                                              s.converged.get(), s.active.get(), s.activeCount.get(), s.width,
+                                              s.height, r.minX, r.minY, r.maxX, r.maxY, sampling)) {
+            return false;
+        }
+        activePixels.store((r.maxX - r.minX + 1) * (r.maxY - r.minY + 1));
+        return true;
+    }
+
+    bool relistActive(int maxSpp)
+    {
+        Surfaces& s = surf;
+        int n = 0;
+        if (!pathTracerRebuildActiveList(s.converged.get(), s.counts.get(), s.active.get(), s.activeCount.get(),
+                                         s.width, s.height, maxSpp, sampling) ||
+            hipMemcpyAsync(&n, s.activeCount.get(), sizeof(int), hipMemcpyDeviceToHost, sampling) != hipSuccess ||
+            hipStreamSynchronize(sampling) != hipSuccess) {
+            return false;
+        }
+        activePixels.store(n);
+        return true;
+    }
+
+    bool refreshEnvironment()
+    {
+        if (envUploaded == envRevision) {
+            return true;
+        }
+        if (env != nullptr) {
+            ptEnvironmentDestroy(env);
+            env = nullptr;
+        }
+        if (!envPixels.empty() && (env = ptEnvironmentCreate(envPixels.data(), envSize[0], envSize[1], sampling)) == nullptr) {
+            return false;
+        }
+        envUploaded = envRevision;
+        return true;
+    }
+
+    // ---- worker ---------------------------------------------------------------------------
+
+    bool wantsSample(const Controls& c) const
+    {
+        const int n = sampleCount.load();
+        return n < c.previewSteps ? budgetAllows(n, c.previewSteps, c.maxSpp) : activePixels.load() > 0;
+    }
+
+    bool hasPictureToShow(const Controls& c) const
+    {
+        const int n = sampleCount.load();
+        if (n <= 0) {
+            return false;
+        }
+        return c.previewSteps == 0 || n > c.previewSteps || finestPreview.load() >= 0;
+    }
+
+    // Applies pending reset requests; returns whether the caller should tell the frame callback.
+    void serveRequests()
+    {
+        const unsigned r = requests.exchange(0u);
+        if (r == 0u) {
+            return;
+        }
+        idle.store(false);
+        const Controls c = snapshot();
+        const bool full = (r & kFullReset) != 0;
+        bool done = false, wiped = false;
+        {
+            std::lock_guard<std::mutex> g(streamLock);
+            if (surf.ready()) {
+                if (c.regionOn) {
+                    done = clearRegion(c.region);
+                } else if (full) {
+                    done = wiped = clearAll();
+                } else {
+                    done = relistActive(c.maxSpp);
+                }
+            }
+        }
+        const bool synced = done && hipStreamSynchronize(sampling) == hipSuccess;
+        if (wiped) {
+            displayable.store(false);
+        } else if (synced && sampleCount.load() > 0) {
+            notifyFrame();
+        }
+        if (full) {
+            sampleCount.store(0);
+            finestPreview.store(-1);
+        }
+    }
+
+    // Blocks until there is a sample to take, a request, or a stop; false on stop.
+    bool awaitWork()
+    {
+        std::unique_lock<std::mutex> g(wakeLock);
+        auto ready = [this] { return !running.load() || requests.load() != 0u || wantsSample(snapshot()); };
+        if (!ready()) {
+            idle.store(true);
+            wake.wait(g, ready);
+        }
+        return running.load();
+    }
+
+    Step plan(const Controls& c) const
+    {
+        Step s;
+        s.iteration = sampleCount.load();
+        s.preview = !c.regionOn && c.previewSteps > 0 && s.iteration < c.previewSteps &&
+                    s.iteration < static_cast<int>(surf.pyramid.size());
+        if (!s.preview) {
+            s.active = activePixels.load();
+            const int wave = s.iteration - c.previewSteps + 1;   // full-resolution passes so far, this one included
+            s.passes = s.active > 0 ? c.passesPerLaunch : 1;
+            s.compact = s.active > 0 && (s.passes > 1 || wave % kCompactEvery == 0);
+        }
+        return s;
+    }
+
+    // Enqueues the step and its completion event; streamLock held.
+    bool launch(const Step& st, const Controls& c)
+    {
+        Surfaces& s = surf;
+        if (publishing.load() && hipStreamWaitEvent(sampling, publishDone, 0) != hipSuccess) {
+            return false;
+        }
+        if (!s.sampleCamera.upload(c.optics.forDevice(s.width, s.height), sampling) ||
+            !s.sampleParams.upload(c.params(st.preview), sampling) || !refreshEnvironment()) {
+            return false;
+        }
+        const unsigned seed = static_cast<unsigned>(st.iteration) + 1u;
+        const MeshAccelSceneGpu* scn = deviceScene();
+        const EnvironmentMapGpu* e = env != nullptr ? ptEnvironmentDevice(env) : nullptr;
+        bool ok = true;
+        if (st.preview) {
+            const PreviewBuffer& b = s.pyramid[static_cast<size_t>(st.iteration)];
+            ok = pathTracerPreviewSample(b.texels.get(), b.size.width, b.size.height, s.sampleCamera.get(), scn, e,
+                                         s.sampleParams.get(), seed, sampling);
+        } else if (st.active > 0 && st.passes > 1) {
+            const int depth = c.maxDepth > 0 ? c.maxDepth : (c.maxDepth == 0 ? 256 : 32);
+            ok = pathTracerAdaptiveRenderWavefront(s.radiance.get(), s.counts.get(), s.mean.get(), s.m2.get(),
+                                                   s.converged.get(), s.active.get(), st.active, s.width, s.height,
+                                                   s.sampleCamera.get(), scn, e, s.sampleParams.get(), seed,
+                                                   st.passes, depth, 0, sampling);
+        } else if (st.active > 0) {
+            ok = pathTracerAdaptiveSample(s.radiance.get(), s.counts.get(), s.mean.get(), s.m2.get(),
+                                          s.converged.get(), s.active.get(), st.active, s.width, s.height,
+                                          s.sampleCamera.get(), scn, e, s.sampleParams.get(), seed, sampling);
+        }
+        if (ok && st.compact) {
+            countReadback = 0;
+            ok = pathTracerCompactActiveList(s.active.get(), s.activeNext.get(), s.activeCount.get(),
+                                             s.converged.get(), s.counts.get(), c.maxSpp, st.active, s.width,
+                                             s.height, c.regionOn, c.region.minX, c.region.minY, c.region.maxX,
+                                             c.region.maxY, sampling) &&
+                 hipMemcpyAsync(&countReadback, s.activeCount.get(), sizeof(int), hipMemcpyDeviceToHost,
+                                sampling) == hipSuccess;
+        }
+        return ok && hipEventRecord(stepDone, sampling) == hipSuccess;
+    }
+
+    // Polls the step's event; a reset that arrives meanwhile voids the step.
+    Wait await()
+    {
+        bool interrupted = false;
+        for (;;) {
+            if (!running.load()) {
+                return Wait::Stopped;
+            }
+            interrupted = interrupted || requests.load() != 0u;
+            const hipError_t q = hipEventQuery(stepDone);
+            if (q == hipSuccess) {
+                return interrupted ? Wait::Interrupted : Wait::Done;
+            }
+            if (q != hipErrorNotReady) {
+                return Wait::Failed;
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+    }
+
+    void commit(const Step& st)
+    {
+        if (st.compact) {
+            activePixels.store(countReadback);
+            surf.active.swap(surf.activeNext);
+        }
+        if (st.preview) {
+            finestPreview.store(st.iteration);
+        }
+        sampleCount.fetch_add(st.passes);
+        displayable.store(true);
+        notifyFrame();
+    }
+
+    void notifyFrame()
+    {
+        PathTracer::FrameReadyCallback cb;
+        {
+            std::lock_guard<std::mutex> g(callbackLock);
+            cb = frameReady;
+        }
+        if (cb) {
+            cb();
+        }
+    }
+
+    void workerMain()
+    {
+        while (running.load()) {
+            serveRequests();
+            if (!surf.ready()) {
+                break;
+            }
+            if (!awaitWork()) {
+                break;
+            }
+            const Controls c = snapshot();
+            if (!wantsSample(c)) {
+                continue;
+            }
+            idle.store(false);
+            const Step st = plan(c);
+            bool queued;
+            {
+                std::lock_guard<std::mutex> g(streamLock);
+                queued = launch(st, c);
+            }
+            if (!queued) {
+                {
+                    std::lock_guard<std::mutex> g(errorLock);
+                    workerError = pathTracerLastError();
+                }
+                if (!st.preview && st.active > 0) {
+                    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                }
+                continue;
+            }
+            const Wait w = await();
+            if (w == Wait::Stopped) {
+                break;
+            }
+            if (w == Wait::Done) {
+                commit(st);
+            }
+        }
+        idle.store(false);
+    }
+
+    // ---- display (PathTracer.cpp:1477-1708 behaviour) ----------------------------------------
+
+    bool enqueuePublish(int slot)
+    {
+        Surfaces& s = surf;
+        if (slot < 0 || slot > 1 || !s.display[slot]) {
+            return false;
+        }
+        const Controls c = snapshot();
+        const bool uv = c.overlay == kUvOverlay;
+        if (!uv && (!displayable.load() || !hasPictureToShow(c))) {
+            return false;
+        }
+        if (!uv && hipStreamWaitEvent(displaying, stepDone, 0) != hipSuccess) {
+            return false;
+        }
+        if (!s.displayParams.upload(c.params(false), displaying) ||
+            !s.displayCamera.upload(c.optics.forDevice(s.width, s.height), displaying)) {
+            return false;
+        }
+        const int n = sampleCount.load();
+        const int finest = finestPreview.load();
+        const bool fromPyramid = !uv && !c.regionOn && c.previewSteps > 0 && finest >= 0 &&
+                                 finest < static_cast<int>(s.pyramid.size()) && n <= c.previewSteps;
+        const float exposure = c.optics.exposure();
+        bool ok;
+        if (uv) {
+            ok = pathTracerWriteUvDebugToPbo(s.displayCamera.get(), deviceScene(), s.display[slot].get(), s.width,
+                                             s.height, s.displayParams.get(), displaying);
+        } else if (fromPyramid) {
+            const PreviewBuffer& b = s.pyramid[static_cast<size_t>(finest)];
+            ok = pathTracerUpsamplePreviewToPbo(b.texels.get(), b.size.width, b.size.height, b.size.downscale,
+                                                s.display[slot].get(), s.width, s.height, s.displayParams.get(),
+                                                exposure, displaying);
+        } else {
+            ok = pathTracerCopyToPbo(s.radiance.get(), s.counts.get(), s.converged.get(), s.display[slot].get(),
+                                     s.width, s.height, s.displayParams.get(), exposure, displaying);
+        }
+        return ok && hipEventRecord(publishDone, displaying) == hipSuccess;
     }
 };
 
@@ -247,445 +811,110 @@ struct TracerImpl {
 
 using ptdetail::TracerImpl;
 
-namespace {
-
-template <typename T>
-bool devAlloc(T** p, size_t count)
-{
-    *p = nullptr;
-    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(1, count) * sizeof(T)) == hipSuccess;
-}
-
-template <typename T>
-void devFree(T*& p)
-{
-    if (p != nullptr) {
-        (void)hipFree(p);
-        p = nullptr;
-    }
-}
-
-void freeAccumulator(ptdetail::Accumulator& a)
-{
-    devFree(a.buffer);
-    devFree(a.samples);
-    devFree(a.lumMean);
-    devFree(a.m2);
-    devFree(a.converged);
-    devFree(a.activeIndices);
-    devFree(a.activeScratch);
-    devFree(a.activeCount);
-    a.width = a.height = 0;
-}
-
-bool initAccumulator(ptdetail::Accumulator& a, int w, int h)
-{
-    freeAccumulator(a);
-    const size_t n = static_cast<size_t>(w) * static_cast<size_t>(h);
-    if (!devAlloc(&a.buffer, n) || !devAlloc(&a.samples, n) || !devAlloc(&a.lumMean, n) || !devAlloc(&a.m2, n) ||
-        !devAlloc(&a.converged, n) || !devAlloc(&a.activeIndices, n) || !devAlloc(&a.activeScratch, n) ||
-        !devAlloc(&a.activeCount, 1)) {
-        freeAccumulator(a);
-        return false;
-    }
-    a.width = w;
-    a.height = h;
-    return true;
-}
-
-void freePreviewLevels(std::vector<ptdetail::PreviewLevel>& levels)
-{
-    for (auto& l : levels) {
-        devFree(l.buffer);
-    }
-    levels.clear();
-}
-
-bool initPreviewLevels(std::vector<ptdetail::PreviewLevel>& levels, int w, int h, int count)
-{
-    freePreviewLevels(levels);
-    for (int i = 0; i < count; ++i) {
-        const LevelDims d = previewLevelDims(w, h, count, i);
-        ptdetail::PreviewLevel l;
-        l.width = d.width;
-        l.height = d.height;
-        l.downscale = d.downscale;
-        if (!devAlloc(&l.buffer, static_cast<size_t>(d.width) * static_cast<size_t>(d.height))) {
-            freePreviewLevels(levels);
-            return false;
-        }
-        levels.push_back(l);
-    }
-    return true;
-}
-
-bool clearPreviewLevels(TracerImpl* impl)
-{
-    for (const auto& l : impl->previewLevels) {
-        if (hipMemsetAsync(l.buffer, 0, sizeof(PtFloat4) * static_cast<size_t>(l.width) * l.height,
-                           impl->sampleStream) != hipSuccess) {
-            return false;
-        }
-    }
-    return true;
-}
-
-// PathTracer.cpp:511-542
-bool clearAccumulator(TracerImpl* impl)
-{
-    auto& a = impl->acc;
-    if (a.buffer == nullptr || a.width <= 0) {
-        return true;
-    }
-    if (!pathTracerClearAccumulator(a.buffer, a.samples, a.lumMean, a.m2, a.converged, a.activeIndices,
-                                    a.activeScratch, a.activeCount, a.width, a.height, impl->sampleStream)) {
-        return false;
-    }
-    impl->activePixelCount.store(a.width * a.height);
-    return true;
-}
-
-// PathTracer.cpp:544-583
-bool clearRegionAccumulator(TracerImpl* impl)
-{
-    auto& a = impl->acc;
-    if (a.buffer == nullptr || a.width <= 0) {
-        return true;
-    }
-    const int minX = impl->regionMinX.load(), minY = impl->regionMinY.load();
-    const int maxX = impl->regionMaxX.load(), maxY = impl->regionMaxY.load();
-    if (!pathTracerClearRegionAccumulator(a.buffer, a.samples, a.lumMean, a.m2, a.converged, a.activeIndices,
-                                          a.activeCount, a.width, a.height, minX, minY, maxX, maxY,
-                                          impl->sampleStream)) {
-        return false;
-    }
-    impl->activePixelCount.store((maxX - minX + 1) * (maxY - minY + 1));
-    return true;
-}
-
-// PathTracer.cpp:585-632
-bool rebuildFullActiveList(TracerImpl* impl)
-{
-    auto& a = impl->acc;
-    if (a.buffer == nullptr || a.width <= 0) {
-        return true;
-    }
-    if (!pathTracerRebuildActiveList(a.converged, a.samples, a.activeIndices, a.activeCount, a.width, a.height,
-                                     impl->maxSamplesPerPixel.load(), impl->sampleStream)) {
-        return false;
-    }
-    int count = 0;
-    if (hipMemcpyAsync(&count, a.activeCount, sizeof(int), hipMemcpyDeviceToHost, impl->sampleStream) != hipSuccess ||
-        hipStreamSynchronize(impl->sampleStream) != hipSuccess) {
-        return false;
-    }
-    impl->activePixelCount.store(count);
-    return true;
-}
-
-CameraGpu buildCameraGpu(const TracerImpl* impl)
-{
-    CameraGpu c;
-    {
-        std::lock_guard<std::mutex> lock(impl->cameraMutex);
-        c = impl->hostCamera.toGpu();
-    }
-    if (impl->acc.width > 0 && impl->acc.height > 0) {
-        c.aspect = static_cast<float>(impl->acc.width) / static_cast<float>(impl->acc.height);
-    }
-    return c;
-}
-
-// PathTracer.cpp:873-898
-bool uploadCameraIfDirty(TracerImpl* impl, hipStream_t stream)
-{
-    if (impl->dCamera == nullptr) {
-        return false;
-    }
-    if (!impl->cameraDirty.load()) {
-        return true;
-    }
-    static thread_local CameraGpu staged;
-    staged = buildCameraGpu(impl);
-    if (hipMemcpyAsync(impl->dCamera, &staged, sizeof(CameraGpu), hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess) {
-        return false;
-    }
-    impl->cameraDirty.store(false);
-    return true;
-}
-
-// PathTracer.cpp:727-775 (previewPass: depth kPreviewMaxPathDepth)
-bool syncRenderParamsToDevice(TracerImpl* impl, hipStream_t stream, bool previewPass, bool force = false)
-{
-    if (impl->dParams == nullptr) {
-        return false;
-    }
-    if (previewPass) {
-        if (impl->deviceParamsArePreview) {
-            return true;
-        }
-    } else if (!force && !impl->renderParamsDirty.load() && !impl->deviceParamsArePreview) {
-        return true;
-    }
-    RenderParamsGpu p = impl->hostParams;
-    p.minSamples = clampMinSamples(impl->minSamples.load());
-    p.maxSamplesPerPixel = clampMaxSamples(impl->maxSamplesPerPixel.load());
-    p.relativeErrorThreshold = clampRelativeErrorThreshold(impl->relativeErrorThreshold.load());
-    p.debugOverlayMode = impl->debugOverlayMode.load();
-    p.brdfDebugFlags = impl->brdfDebugFlags.load();
-    if (previewPass) {
-        p.maxPathDepth = kPreviewMaxPathDepth;
-    }
-    if (hipMemcpyAsync(impl->dParams, &p, sizeof(p), hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess) {
-        return false;
-    }
-    impl->deviceParamsArePreview = previewPass;
-    if (!previewPass) {
-        impl->renderParamsDirty.store(false);
-    }
-    return true;
-}
-
-bool uploadEnvironmentIfDirty(TracerImpl* impl, hipStream_t stream)
-{
-    if (!impl->environmentDirty.load()) {
-        return true;
-    }
-    if (impl->env != nullptr) {
-        ptEnvironmentDestroy(impl->env);
-        impl->env = nullptr;
-    }
-    if (!impl->envImage.empty()) {
-        impl->env = ptEnvironmentCreate(impl->envImage.data(), impl->envWidth, impl->envHeight, stream);
-        if (impl->env == nullptr) {
-            return false;
-        }
-    }
-    impl->environmentDirty.store(false);
-    return true;
-}
-
-bool canTakeSample(const TracerImpl* impl)
-{
-    const int previewSteps = impl->previewStepsPerLevel.load();
-    const int count = impl->sampleCount.load();
-    if (count < previewSteps) {
-        return canTakeSampleAtIteration(count, previewSteps, impl->maxSamplesPerPixel.load());
-    }
-    return impl->activePixelCount.load() > 0;
-}
-
-bool hasDisplayableContent(const TracerImpl& impl)
-{
-    const int count = impl.sampleCount.load();
-    const int previewCount = impl.previewStepsPerLevel.load();
-    if (count <= 0) {
-        return false;
-    }
-    if (previewCount > 0 && count <= previewCount) {
-        return impl.finestCompletedPreview.load() >= 0;
-    }
-    return true;
-}
-
-WaitResult waitForSampleEvent(TracerImpl* impl)
-{
-    bool resetSeen = false;
-    for (;;) {
-        if (!impl->running.load()) {
-            return WaitResult::Stopped;
-        }
-        if (impl->resetRequested.load() || impl->regionResetRequested.load()) {
-            resetSeen = true;
-        }
-        const hipError_t q = hipEventQuery(impl->sampleCompleteEvent);
-        if (q == hipSuccess) {
-            return resetSeen ? WaitResult::ResetDuringWait : WaitResult::Completed;
-        }
-        if (q != hipErrorNotReady) {
-            return WaitResult::Error;
-        }
-        std::this_thread::sleep_for(std::chrono::microseconds(100));
-    }
-}
-
-// PathTracer.cpp:1477-1617 with the two GL PBOs replaced by device buffers.
-bool launchDisplayPublishLocked(TracerImpl* impl, int slot)
-{
-    if (slot < 0 || slot > 1 || impl->display[slot] == nullptr) {
-        return false;
-    }
-    const int overlay = impl->debugOverlayMode.load();
-    const bool uvMode = overlay == 3; // RenderViewOverlayMode::Uv
-    if (!uvMode && (!impl->displayBuffersValid.load() || !hasDisplayableContent(*impl))) {
-        return false;
-    }
-    if (!uvMode && hipStreamWaitEvent(impl->displayStream, impl->sampleCompleteEvent, 0) != hipSuccess) {
-        return false;
-    }
-    if (!syncRenderParamsToDevice(impl, impl->displayStream, false, impl->renderParamsDirty.load()) ||
-        !uploadCameraIfDirty(impl, impl->displayStream)) {
-        return false;
-    }
-    const int previewCount = impl->previewStepsPerLevel.load();
-    const int iter = impl->sampleCount.load();
-    const int finest = impl->finestCompletedPreview.load();
-    const bool showPreview = !uvMode && !impl->regionEnabled.load() && previewCount > 0 && finest >= 0 &&
-                             finest < static_cast<int>(impl->previewLevels.size()) && iter <= previewCount;
-    float exposure = 1.0f;
-    {
-        std::lock_guard<std::mutex> lock(impl->cameraMutex);
-        exposure = exposureMultiplierOf(impl->hostCamera.fStop, impl->hostCamera.shutter, impl->hostCamera.iso);
-    }
-    bool ok = false;
-    if (uvMode) {
-        ok = pathTracerWriteUvDebugToPbo(impl->dCamera, ptSceneDevice(impl->scene), impl->display[slot],
-                                         impl->acc.width, impl->acc.height, impl->dParams, impl->displayStream);
-    } else if (showPreview) {
-        const auto& l = impl->previewLevels[static_cast<size_t>(finest)];
-        ok = pathTracerUpsamplePreviewToPbo(l.buffer, l.width, l.height, l.downscale, impl->display[slot],
-                                            impl->acc.width, impl->acc.height, impl->dParams, exposure,
-                                            impl->displayStream);
-    } else {
-        ok = pathTracerCopyToPbo(impl->acc.buffer, impl->acc.samples, impl->acc.converged, impl->display[slot],
-                                 impl->acc.width, impl->acc.height, impl->dParams, exposure, impl->displayStream);
-    }
-    return ok && hipEventRecord(impl->displayReadyEvent, impl->displayStream) == hipSuccess;
-}
-
-void releaseAll(TracerImpl* impl)
-{
-    freeAccumulator(impl->acc);
-    freePreviewLevels(impl->previewLevels);
-    devFree(impl->display[0]);
-    devFree(impl->display[1]);
-    devFree(impl->dCamera);
-    devFree(impl->dParams);
-    if (impl->env != nullptr) {
-        ptEnvironmentDestroy(impl->env);
-        impl->env = nullptr;
-    }
-    impl->environmentDirty.store(true);
-    if (impl->scene != nullptr) {
-        ptSceneDestroy(impl->scene);
-        impl->scene = nullptr;
-    }
-}
-
-bool buildScene(TracerImpl* impl, const PtHostMesh& mesh)
-{
-    std::lock_guard<std::mutex> lock(impl->sceneMutex);
-    if (impl->scene == nullptr) {
-        impl->scene = ptSceneCreate();
-        if (impl->scene == nullptr) {
-            return false;
-        }
-    }
-    if (!ptSceneSetBuildMode(impl->scene, impl->bvhBuildMode.load())) {
-        return false;
-    }
-    return ptSceneBuild(impl->scene, mesh.triangles, mesh.triangleCount, mesh.materials, mesh.materialCount,
-                        mesh.textures, mesh.textureCount, impl->sampleStream);
-}
-
-} // namespace
-
 // ---- PathTracer -----------------------------------------------------------------------------
 
 PathTracer::PathTracer() : m_impl(new TracerImpl()) {}
 
-PathTracer::~PathTracer()
-{
-    stop();
-    releaseAll(m_impl.get());
-    if (m_impl->sampleCompleteEvent != nullptr) {
-        (void)hipEventDestroy(m_impl->sampleCompleteEvent);
-    }
-    if (m_impl->displayReadyEvent != nullptr) {
-        (void)hipEventDestroy(m_impl->displayReadyEvent);
-    }
-    if (m_impl->sampleStream != nullptr) {
-        (void)hipStreamDestroy(m_impl->sampleStream);
-    }
-    if (m_impl->displayStream != nullptr) {
-        (void)hipStreamDestroy(m_impl->displayStream);
-    }
-}
+PathTracer::~PathTracer() { stop(); }
 
 void PathTracer::setFrameReadyCallback(FrameReadyCallback callback)
 {
-    std::lock_guard<std::mutex> lock(m_impl->callbackMutex);
-    m_impl->callback = std::move(callback);
+    std::lock_guard<std::mutex> g(m_impl->callbackLock);
+    m_impl->frameReady = std::move(callback);
 }
 
 bool PathTracer::configure(int width, int height, const PtHostMesh& mesh)
 {
-    TracerImpl* impl = m_impl.get();
+    TracerImpl& t = *m_impl;
     if (width <= 0 || height <= 0) {
         ptSetError("PathTracer::configure: invalid dimensions");
         return false;
     }
-    if (impl->running.load()) {
-        stop();
-    }
-    impl->configured.store(false);
-    releaseAll(impl);
-    if ((impl->sampleStream == nullptr && hipStreamCreate(&impl->sampleStream) != hipSuccess) ||
-        (impl->displayStream == nullptr && hipStreamCreate(&impl->displayStream) != hipSuccess) ||
-        (impl->sampleCompleteEvent == nullptr &&
-         hipEventCreateWithFlags(&impl->sampleCompleteEvent, hipEventDisableTiming) != hipSuccess) ||
-        (impl->displayReadyEvent == nullptr &&
-         hipEventCreateWithFlags(&impl->displayReadyEvent, hipEventDisableTiming) != hipSuccess)) {
+    stop();
+    t.configured.store(false);
+    t.releaseDevice();
+    if (!t.ensureStreams()) {
         ptSetError("PathTracer::configure: stream or event creation failed");
         return false;
     }
-    impl->displayPublishInFlight.store(false);
-    impl->displayPublishSlot.store(-1);
-    const size_t px = static_cast<size_t>(width) * static_cast<size_t>(height);
-    bool ok = initAccumulator(impl->acc, width, height) &&
-              initPreviewLevels(impl->previewLevels, width, height, impl->previewStepsPerLevel.load()) &&
-              devAlloc(&impl->display[0], px) && devAlloc(&impl->display[1], px) && devAlloc(&impl->dCamera, 1) &&
-              devAlloc(&impl->dParams, 1);
-    if (!ok) {
+    t.publishing.store(false);
+    t.publishSlot.store(-1);
+    const Controls c = t.snapshot();
+    if (!t.surf.allocate(width, height, c.previewSteps)) {
         ptSetError("PathTracer::configure: device allocation failed");
-        releaseAll(impl);
         return false;
     }
-    impl->cameraDirty.store(true);
-    impl->renderParamsDirty.store(true);
-    impl->deviceParamsArePreview = false;
-    ok = buildScene(impl, mesh) && uploadEnvironmentIfDirty(impl, impl->sampleStream) &&
-         uploadCameraIfDirty(impl, impl->sampleStream) && syncRenderParamsToDevice(impl, impl->sampleStream, false, true) &&
-         clearAccumulator(impl) && clearPreviewLevels(impl) && hipStreamSynchronize(impl->sampleStream) == hipSuccess &&
-         hipStreamSynchronize(impl->displayStream) == hipSuccess;
+    bool ok;
+    {
+        std::lock_guard<std::mutex> g(t.streamLock);
+        ok = t.buildScene(mesh, c.buildMode) && t.refreshEnvironment() &&
+             t.surf.sampleCamera.upload(c.optics.forDevice(width, height), t.sampling) &&
+             t.surf.sampleParams.upload(c.params(false), t.sampling) && t.clearAll() &&
+             hipStreamSynchronize(t.sampling) == hipSuccess && hipStreamSynchronize(t.displaying) == hipSuccess;
+    }
     if (!ok) {
         if (pathTracerLastError()[0] == '\0') {
             ptSetError("PathTracer::configure: scene build or upload failed");
         }
-        releaseAll(impl);
+        t.releaseDevice();
         return false;
     }
-    impl->resetRequested.store(false);
-    impl->regionResetRequested.store(false);
-    impl->sampleCount.store(0);
-    impl->finestCompletedPreview.store(-1);
-    impl->displayBuffersValid.store(false);
-    impl->configured.store(true);
+    t.requests.store(0u);
+    t.sampleCount.store(0);
+    t.finestPreview.store(-1);
+    t.displayable.store(false);
+    t.configured.store(true);
     return true;
+}
+
+namespace {
+
+// meshSceneBuild (MeshSceneContent.cpp:5-52): the instances' HostMesh, or NULL with the error set.
+PtHostMeshBuf* proceduralMesh(const std::vector<PtProceduralInstance>& instances, const PtMeshSceneBuildParams& params)
+{
+    PtProceduralBuildParams p;
+    ptProceduralDefaultParams(&p);
+    p.circularSegments = params.circularSegments;
+    p.creaseAngleDeg = params.creaseAngleDeg;
+    return ptProceduralBuildScene(instances.empty() ? nullptr : instances.data(),
+                                  static_cast<uint32_t>(instances.size()), &p);
+}
+
+PtHostMesh viewOf(const PtHostMeshBuf* m)
+{
+    PtHostMesh v;
+    ptHostMeshCounts(m, &v.triangleCount, &v.materialCount, &v.textureCount);
+    v.triangles = ptHostMeshTriangles(m);
+    v.materials = ptHostMeshMaterials(m);
+    v.textures = ptHostMeshTextures(m);
+    return v;
+}
+
+} // namespace
+
+bool PathTracer::configure(int width, int height, const std::vector<PtProceduralInstance>& instances,
+                           const PtMeshSceneBuildParams& params)
+{
+    PtHostMeshBuf* m = proceduralMesh(instances, params);
+    if (m == nullptr) {
+        return false;
+    }
+    const bool ok = configure(width, height, viewOf(m));
+    ptHostMeshDestroy(m);
+    return ok;
 }
 
 bool PathTracer::rebuildMeshScene(const PtHostMesh& mesh)
 {
-    TracerImpl* impl = m_impl.get();
-    if (!impl->configured.load()) {
+    TracerImpl& t = *m_impl;
+    if (!t.configured.load()) {
         return false;
     }
-    bool ok = false;
+    const int mode = t.snapshot().buildMode;
+    bool ok;
     {
-        std::lock_guard<std::mutex> streamLock(impl->streamMutex);
-        ok = hipStreamSynchronize(impl->sampleStream) == hipSuccess && buildScene(impl, mesh);
+        std::lock_guard<std::mutex> g(t.streamLock);
+        ok = hipStreamSynchronize(t.sampling) == hipSuccess && t.buildScene(mesh, mode);
     }
     if (ok) {
         resetAccumulation();
@@ -693,84 +922,82 @@ bool PathTracer::rebuildMeshScene(const PtHostMesh& mesh)
     return ok;
 }
 
+bool PathTracer::rebuildMeshScene(const std::vector<PtProceduralInstance>& instances,
+                                  const PtMeshSceneBuildParams& params)
+{
+    PtHostMeshBuf* m = proceduralMesh(instances, params);
+    if (m == nullptr) {
+        return false;
+    }
+    const bool ok = rebuildMeshScene(viewOf(m));
+    ptHostMeshDestroy(m);
+    return ok;
+}
+
 void PathTracer::start()
 {
-    if (m_impl->running.load() || !m_impl->configured.load()) {
+    TracerImpl& t = *m_impl;
+    if (t.running.load() || !t.configured.load()) {
         return;
     }
-    m_impl->running.store(true);
-    m_impl->worker = std::thread([this]() { renderLoop(); });
+    if (t.worker.joinable()) {
+        t.worker.join();
+    }
+    t.running.store(true);
+    t.worker = std::thread([this] { renderLoop(); });
 }
 
 void PathTracer::stop()
 {
-    if (!m_impl->running.load()) {
-        if (m_impl->worker.joinable()) {
-            m_impl->worker.join();
-        }
-        return;
-    }
-    m_impl->running.store(false);
-    notifyWorker();
-    if (m_impl->worker.joinable()) {
-        m_impl->worker.join();
+    TracerImpl& t = *m_impl;
+    t.running.store(false);
+    t.poke();
+    if (t.worker.joinable()) {
+        t.worker.join();
     }
 }
 
-void PathTracer::resetAccumulation()
-{
-    m_impl->resetRequested.store(true);
-    notifyWorker();
-}
-
-void PathTracer::resetRegionAccumulation()
-{
-    m_impl->regionResetRequested.store(true);
-    notifyWorker();
-}
-
+void PathTracer::resetAccumulation() { m_impl->request(kFullReset); }
+void PathTracer::resetRegionAccumulation() { m_impl->request(kRegionReset); }
 bool PathTracer::isRunning() const { return m_impl->running.load(); }
 
 void PathTracer::releaseOutputSurfaces()
 {
     stop();
-    devFree(m_impl->display[0]);
-    devFree(m_impl->display[1]);
+    m_impl->surf.display[0].reset();
+    m_impl->surf.display[1].reset();
     m_impl->configured.store(false);
 }
 
 bool PathTracer::beginDisplayPublish(int slot)
 {
-    TracerImpl* impl = m_impl.get();
-    if (slot < 0 || slot > 1 || !impl->configured.load() || impl->displayPublishInFlight.load()) {
+    TracerImpl& t = *m_impl;
+    if (slot < 0 || slot > 1 || !t.configured.load() || t.publishing.load()) {
         return false;
     }
-    std::lock_guard<std::mutex> streamLock(impl->streamMutex);
-    if (!launchDisplayPublishLocked(impl, slot)) {
+    std::lock_guard<std::mutex> g(t.streamLock);
+    if (!t.enqueuePublish(slot)) {
         return false;
     }
-    impl->displayPublishSlot.store(slot);
-    impl->displayPublishInFlight.store(true);
+    t.publishSlot.store(slot);
+    t.publishing.store(true);
     return true;
 }
 
 bool PathTracer::isDisplayPublishReady() const
 {
-    if (!m_impl->displayPublishInFlight.load() || m_impl->displayReadyEvent == nullptr) {
-        return false;
-    }
-    return hipEventQuery(m_impl->displayReadyEvent) == hipSuccess;
+    return m_impl->publishing.load() && m_impl->publishDone != nullptr && hipEventQuery(m_impl->publishDone) == hipSuccess;
 }
 
-bool PathTracer::hasDisplayPublishInFlight() const { return m_impl->displayPublishInFlight.load(); }
+bool PathTracer::hasDisplayPublishInFlight() const { return m_impl->publishing.load(); }
 
 int PathTracer::finishDisplayPublish()
 {
     if (!isDisplayPublishReady()) {
         return -1;
     }
-    const int slot = m_impl->displayPublishSlot.exchange(-1);
-    m_impl->displayPublishInFlight.store(false);
+    const int slot = m_impl->publishSlot.exchange(-1);
+    m_impl->publishing.store(false);
     return slot;
 }
 
@@ -787,301 +1014,237 @@ bool PathTracer::publishDisplayFrame(int slot)
 
 bool PathTracer::readDisplay(int slot, PtUchar4* hostRgba) const
 {
-    const TracerImpl* impl = m_impl.get();
-    if (slot < 0 || slot > 1 || hostRgba == nullptr || impl->display[slot] == nullptr) {
+    const TracerImpl& t = *m_impl;
+    if (slot < 0 || slot > 1 || hostRgba == nullptr || !t.surf.display[slot]) {
         return false;
     }
-    const size_t n = static_cast<size_t>(impl->acc.width) * static_cast<size_t>(impl->acc.height);
-    return hipStreamSynchronize(impl->displayStream) == hipSuccess &&
-           hipMemcpy(hostRgba, impl->display[slot], n * sizeof(PtUchar4), hipMemcpyDeviceToHost) == hipSuccess;
+    return hipStreamSynchronize(t.displaying) == hipSuccess &&
+           hipMemcpy(hostRgba, t.surf.display[slot].get(), t.surf.pixels() * sizeof(PtUchar4),
+                     hipMemcpyDeviceToHost) == hipSuccess;
 }
 
 void PathTracer::setMaxSamplesPerPixel(int max)
 {
-    const int c = clampMaxSamples(max);
-    m_impl->maxSamplesPerPixel.store(c);
-    m_impl->hostParams.maxSamplesPerPixel = c;
-    m_impl->renderParamsDirty.store(true);
-    notifyWorker();
+    m_impl->edit([max](Controls& c) { c.maxSpp = clampTo(max, kMaxSppRange.lo, kMaxSppRange.hi); });
 }
-int PathTracer::maxSamplesPerPixel() const { return m_impl->maxSamplesPerPixel.load(); }
+int PathTracer::maxSamplesPerPixel() const { return m_impl->snapshot().maxSpp; }
 
 void PathTracer::setMinSamples(int min)
 {
-    const int c = clampMinSamples(min);
-    m_impl->minSamples.store(c);
-    m_impl->hostParams.minSamples = c;
-    m_impl->renderParamsDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
-    }
-    notifyWorker();
+    m_impl->edit([min](Controls& c) { c.minSpp = clampTo(min, kMinSppRange.lo, kMinSppRange.hi); });
+    m_impl->resetIfConfigured();
 }
-int PathTracer::minSamples() const { return m_impl->minSamples.load(); }
+int PathTracer::minSamples() const { return m_impl->snapshot().minSpp; }
 
 void PathTracer::setRelativeErrorThreshold(float threshold)
 {
-    const float c = clampRelativeErrorThreshold(threshold);
-    m_impl->relativeErrorThreshold.store(c);
-    m_impl->hostParams.relativeErrorThreshold = c;
-    m_impl->renderParamsDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
-    }
-    notifyWorker();
+    m_impl->edit([threshold](Controls& c) { c.threshold = clampTo(threshold, kThresholdLo, kThresholdHi); });
+    m_impl->resetIfConfigured();
 }
-float PathTracer::relativeErrorThreshold() const { return m_impl->relativeErrorThreshold.load(); }
+float PathTracer::relativeErrorThreshold() const { return m_impl->snapshot().threshold; }
 
 void PathTracer::setDebugOverlayMode(int mode)
 {
-    mode = mode < 0 ? 0 : (mode > 3 ? 3 : mode);
-    m_impl->debugOverlayMode.store(mode);
-    m_impl->hostParams.debugOverlayMode = mode;
-    m_impl->renderParamsDirty.store(true);
-    notifyWorker();
+    m_impl->edit([mode](Controls& c) { c.overlay = clampTo(mode, kOverlayRange.lo, kOverlayRange.hi); });
 }
-int PathTracer::debugOverlayMode() const { return m_impl->debugOverlayMode.load(); }
+int PathTracer::debugOverlayMode() const { return m_impl->snapshot().overlay; }
 
 void PathTracer::setBrdfDebugFlags(int flags)
 {
-    m_impl->brdfDebugFlags.store(flags);
-    m_impl->hostParams.brdfDebugFlags = flags;
-    m_impl->renderParamsDirty.store(true);
-    notifyWorker();
+    m_impl->edit([flags](Controls& c) { c.brdfFlags = flags; });
 }
-int PathTracer::brdfDebugFlags() const { return m_impl->brdfDebugFlags.load(); }
+int PathTracer::brdfDebugFlags() const { return m_impl->snapshot().brdfFlags; }
 
 void PathTracer::setPreviewStepsPerLevel(int steps)
 {
-    TracerImpl* impl = m_impl.get();
-    const int c = clampPreviewStepsPerLevel(steps);
-    impl->previewStepsPerLevel.store(c);
-    if (impl->configured.load() && impl->acc.width > 0) {
-        std::lock_guard<std::mutex> streamLock(impl->streamMutex);
-        if (hipStreamSynchronize(impl->sampleStream) == hipSuccess &&
-            initPreviewLevels(impl->previewLevels, impl->acc.width, impl->acc.height, c) && clearPreviewLevels(impl)) {
-            (void)hipStreamSynchronize(impl->sampleStream);
+    TracerImpl& t = *m_impl;
+    const int n = clampTo(steps, kPreviewStepRange.lo, kPreviewStepRange.hi);
+    t.edit([n](Controls& c) { c.previewSteps = n; });
+    if (t.configured.load() && t.surf.ready()) {
+        std::lock_guard<std::mutex> g(t.streamLock);
+        if (hipStreamSynchronize(t.sampling) == hipSuccess && t.surf.buildPyramid(n) && t.surf.zeroPyramid(t.sampling)) {
+            (void)hipStreamSynchronize(t.sampling);
         }
     }
-    notifyWorker();
+    t.poke();
 }
-int PathTracer::previewStepsPerLevel() const { return m_impl->previewStepsPerLevel.load(); }
+int PathTracer::previewStepsPerLevel() const { return m_impl->snapshot().previewSteps; }
 
 int PathTracer::currentSampleCount() const { return m_impl->sampleCount.load(); }
-int PathTracer::currentActivePixelCount() const { return m_impl->activePixelCount.load(); }
+int PathTracer::currentActivePixelCount() const { return m_impl->activePixels.load(); }
 
 int PathTracer::sampleBudgetTotalIterations() const
 {
-    return budgetTotalIterations(m_impl->previewStepsPerLevel.load(), m_impl->maxSamplesPerPixel.load());
+    const Controls c = m_impl->snapshot();
+    return budgetTotal(c.previewSteps, c.maxSpp);
 }
 
 bool PathTracer::isSampleBudgetExhausted() const
 {
-    const int previewSteps = m_impl->previewStepsPerLevel.load();
-    const int count = m_impl->sampleCount.load();
-    if (count < previewSteps) {
-        return budgetExhaustedAtIteration(count, previewSteps, m_impl->maxSamplesPerPixel.load());
-    }
-    return m_impl->activePixelCount.load() <= 0; // isAdaptiveSampleBudgetExhausted
+    const Controls c = m_impl->snapshot();
+    const int n = m_impl->sampleCount.load();
+    return n < c.previewSteps ? !budgetAllows(n, c.previewSteps, c.maxSpp) : m_impl->activePixels.load() <= 0;
 }
 
 void PathTracer::setCamera(const CameraGpu& camera)
 {
-    CameraGpu stored = camera;
-    if (m_impl->acc.width > 0 && m_impl->acc.height > 0) {
-        stored.aspect = static_cast<float>(m_impl->acc.width) / static_cast<float>(m_impl->acc.height);
+    TracerImpl& t = *m_impl;
+    CameraGpu cam = camera;
+    if (t.surf.width > 0 && t.surf.height > 0) {
+        cam.aspect = static_cast<float>(t.surf.width) / static_cast<float>(t.surf.height);
     }
-    {
-        std::lock_guard<std::mutex> lock(m_impl->cameraMutex);
-        m_impl->hostCamera.applyGpuGeometry(stored);
-    }
-    {
-        std::lock_guard<std::mutex> lock(m_impl->lastSampleCameraMutex);
-        m_impl->lastSampleCamera = stored;
-    }
-    m_impl->cameraDirty.store(true);
-    notifyWorker();
+    t.edit([&cam](Controls& c) { c.optics.takeGeometry(cam); });
+    std::lock_guard<std::mutex> g(t.controlsLock);
+    t.lastCamera = cam;
 }
 
 CameraGpu PathTracer::lastSampleCamera() const
 {
-    std::lock_guard<std::mutex> lock(m_impl->lastSampleCameraMutex);
-    return m_impl->lastSampleCamera;
+    std::lock_guard<std::mutex> g(m_impl->controlsLock);
+    return m_impl->lastCamera;
 }
 
 void PathTracer::setClearColor(float r, float g, float b)
 {
-    m_impl->hostParams.backgroundR = r;
-    m_impl->hostParams.backgroundG = g;
-    m_impl->hostParams.backgroundB = b;
-    m_impl->renderParamsDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
-    }
+    m_impl->edit([r, g, b](Controls& c) {
+        c.clear[0] = r;
+        c.clear[1] = g;
+        c.clear[2] = b;
+    });
+    m_impl->resetIfConfigured();
 }
 
 void PathTracer::setPhysicalCamera(float fStop, float shutterSpeedSeconds, float iso)
 {
-    std::lock_guard<std::mutex> lock(m_impl->cameraMutex);
-    m_impl->hostCamera.fStop = std::max(kMinFStop, std::min(fStop, kMaxFStop));
-    m_impl->hostCamera.shutter = std::max(kMinShutterSpeedSeconds, std::min(shutterSpeedSeconds, kMaxShutterSpeedSeconds));
-    m_impl->hostCamera.iso = std::max(kMinIso, std::min(iso, kMaxIso));
-    m_impl->cameraDirty.store(true);
+    m_impl->edit([=](Controls& c) {
+        c.optics.fStop = clampTo(fStop, kFStopLo, kFStopHi);
+        c.optics.shutter = clampTo(shutterSpeedSeconds, kShutterLo, kShutterHi);
+        c.optics.iso = clampTo(iso, kIsoLo, kIsoHi);
+    });
 }
 
 void PathTracer::setFocusPoint(const Vec3& p)
 {
-    {
-        std::lock_guard<std::mutex> lock(m_impl->cameraMutex);
-        auto& c = m_impl->hostCamera;
-        const Vec3 f = c.forward();
-        const float d = (p.x - c.geom.position.x) * f.x + (p.y - c.geom.position.y) * f.y +
-                        (p.z - c.geom.position.z) * f.z;
-        c.focusDistance = d > 0.0f ? d : 0.0f;
-        c.focusValid = c.focusDistance > 0.0f;
-    }
-    m_impl->cameraDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
-    }
-    notifyWorker();
+    m_impl->edit([&p](Controls& c) {
+        Optics& o = c.optics;
+        const Vec3 a = o.viewAxis();
+        const Vec3& e = o.geometry.position;
+        const float d = (p.x - e.x) * a.x + (p.y - e.y) * a.y + (p.z - e.z) * a.z;
+        o.focusDistance = d > 0.0f ? d : 0.0f;
+        o.focusOn = o.focusDistance > 0.0f;
+    });
+    m_impl->resetIfConfigured();
 }
 
 void PathTracer::clearFocusPoint()
 {
-    {
-        std::lock_guard<std::mutex> lock(m_impl->cameraMutex);
-        m_impl->hostCamera.focusValid = false;
-    }
-    m_impl->cameraDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
-    }
-    notifyWorker();
+    m_impl->edit([](Controls& c) { c.optics.focusOn = false; });
+    m_impl->resetIfConfigured();
 }
 
-// PathTracer.cpp:1991-2013: closest hit in [0.001, 1e30) along normalize(rd), through the GPU
-// traversal (the reference walks its host copy of the BVH).
+// Closest hit in [0.001, 1e30) along normalize(rd) (PathTracer.cpp:1991-2013), traced on the GPU.
 bool PathTracer::pickSurface(const Vec3& ro, const Vec3& rd, Vec3* hitPoint) const
 {
-    if (hitPoint == nullptr) {
-        return false;
-    }
-    std::lock_guard<std::mutex> lock(m_impl->sceneMutex);
-    const MeshAccelSceneGpu* d_scene = ptSceneDevice(m_impl->scene);
-    if (d_scene == nullptr) {
-        return false;
-    }
+    const MeshAccelSceneGpu* scn = m_impl->deviceScene();
     const float len = std::sqrt(rd.x * rd.x + rd.y * rd.y + rd.z * rd.z);
-    if (!(len > 0.0f)) {
+    if (hitPoint == nullptr || scn == nullptr || !(len > 0.0f)) {
         return false;
     }
-    const float o[3] = {ro.x, ro.y, ro.z};
-    const float d[3] = {rd.x / len, rd.y / len, rd.z / len};
-    float* dev = nullptr;
-    if (!devAlloc(&dev, 14)) {
-        return false;
-    }
+    const float ray[6] = {ro.x, ro.y, ro.z, rd.x / len, rd.y / len, rd.z / len};
+    DeviceArray<float> dev;
     float out[8] = {};
-    bool ok = hipMemcpy(dev, o, sizeof(o), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(dev + 3, d, sizeof(d), hipMemcpyHostToDevice) == hipSuccess &&
-              pathTracerTraceRays(d_scene, dev, dev + 3, 1, 0.001f, 1.0e30f, dev + 6, nullptr) &&
-              hipMemcpy(out, dev + 6, sizeof(out), hipMemcpyDeviceToHost) == hipSuccess;
-    devFree(dev);
+    const bool ok = dev.allocate(14) && hipMemcpy(dev.get(), ray, sizeof(ray), hipMemcpyHostToDevice) == hipSuccess &&
+                    pathTracerTraceRays(scn, dev.get(), dev.get() + 3, 1, 0.001f, 1.0e30f, dev.get() + 6, nullptr) &&
+                    hipMemcpy(out, dev.get() + 6, sizeof(out), hipMemcpyDeviceToHost) == hipSuccess;
     int hit = 0;
     std::memcpy(&hit, &out[0], sizeof(int));
     if (!ok || hit == 0) {
         return false;
     }
-    *hitPoint = Vec3{ro.x + d[0] * out[1], ro.y + d[1] * out[1], ro.z + d[2] * out[1]};
+    *hitPoint = Vec3{ro.x + ray[3] * out[1], ro.y + ray[4] * out[1], ro.z + ray[5] * out[1]};
     return true;
 }
 
 bool PathTracer::setEnvironmentImage(const float* rgb, int width, int height)
 {
+    TracerImpl& t = *m_impl;
     if (rgb != nullptr && (width <= 0 || height <= 0)) {
         return false;
     }
     {
-        std::lock_guard<std::mutex> streamLock(m_impl->streamMutex);
+        std::lock_guard<std::mutex> g(t.streamLock);
         if (rgb == nullptr) {
-            m_impl->envImage.clear();
-            m_impl->envWidth = m_impl->envHeight = 0;
+            t.envPixels.clear();
+            t.envSize[0] = t.envSize[1] = 0;
         } else {
-            m_impl->envImage.assign(rgb, rgb + 3 * static_cast<size_t>(width) * static_cast<size_t>(height));
-            m_impl->envWidth = width;
-            m_impl->envHeight = height;
+            t.envPixels.assign(rgb, rgb + 3 * static_cast<size_t>(width) * static_cast<size_t>(height));
+            t.envSize[0] = width;
+            t.envSize[1] = height;
         }
-        m_impl->environmentDirty.store(true);
+        ++t.envRevision;
     }
-    if (m_impl->configured.load()) {
-        resetAccumulation();
-    }
+    t.resetIfConfigured();
     return true;
 }
 
 void PathTracer::setEnvironmentIntensity(float intensity)
 {
-    intensity = intensity < 0.0f ? 0.0f : (intensity > 100.0f ? 100.0f : intensity);
-    if (m_impl->hostParams.environmentIntensity == intensity) {
-        return;
-    }
-    m_impl->hostParams.environmentIntensity = intensity;
-    m_impl->renderParamsDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
+    const float v = clampTo(intensity, kIntensityLo, kIntensityHi);
+    bool changed = false;
+    m_impl->edit([&](Controls& c) {
+        changed = c.envIntensity != v;
+        c.envIntensity = v;
+    });
+    if (changed) {
+        m_impl->resetIfConfigured();
     }
 }
-float PathTracer::environmentIntensity() const { return m_impl->hostParams.environmentIntensity; }
+float PathTracer::environmentIntensity() const { return m_impl->snapshot().envIntensity; }
 
 void PathTracer::setRussianRouletteMinDepth(int depth)
 {
-    depth = depth < 0 ? 0 : (depth > 256 ? 256 : depth);
-    if (m_impl->hostParams.russianRouletteMinDepth == depth) {
-        return;
-    }
-    m_impl->hostParams.russianRouletteMinDepth = depth;
-    m_impl->renderParamsDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
+    const int v = clampTo(depth, kRouletteRange.lo, kRouletteRange.hi);
+    bool changed = false;
+    m_impl->edit([&](Controls& c) {
+        changed = c.rouletteDepth != v;
+        c.rouletteDepth = v;
+    });
+    if (changed) {
+        m_impl->resetIfConfigured();
     }
 }
-int PathTracer::russianRouletteMinDepth() const { return m_impl->hostParams.russianRouletteMinDepth; }
+int PathTracer::russianRouletteMinDepth() const { return m_impl->snapshot().rouletteDepth; }
 
 void PathTracer::setRegionRenderEnabled(bool enabled)
 {
-    m_impl->regionEnabled.store(enabled);
-    notifyWorker();
+    m_impl->edit([enabled](Controls& c) { c.regionOn = enabled; });
 }
-bool PathTracer::regionRenderEnabled() const { return m_impl->regionEnabled.load(); }
+bool PathTracer::regionRenderEnabled() const { return m_impl->snapshot().regionOn; }
 
 void PathTracer::setRenderRegion(int minX, int minY, int maxX, int maxY)
 {
-    const int mx = std::max(0, m_impl->acc.width - 1);
-    const int my = std::max(0, m_impl->acc.height - 1);
-    m_impl->regionMinX.store(std::max(0, std::min(minX, maxX)));
-    m_impl->regionMaxX.store(std::min(mx, std::max(minX, maxX)));
-    m_impl->regionMinY.store(std::max(0, std::min(minY, maxY)));
-    m_impl->regionMaxY.store(std::min(my, std::max(minY, maxY)));
+    const int xMax = std::max(0, m_impl->surf.width - 1);
+    const int yMax = std::max(0, m_impl->surf.height - 1);
+    const PtRegion r{std::max(0, std::min(minX, maxX)), std::max(0, std::min(minY, maxY)),
+                     std::min(xMax, std::max(minX, maxX)), std::min(yMax, std::max(minY, maxY))};
+    std::lock_guard<std::mutex> g(m_impl->controlsLock);
+    m_impl->controls.region = r;
 }
 
-PtRegion PathTracer::renderRegion() const
-{
-    return PtRegion{m_impl->regionMinX.load(), m_impl->regionMinY.load(), m_impl->regionMaxX.load(),
-                    m_impl->regionMaxY.load()};
-}
+PtRegion PathTracer::renderRegion() const { return m_impl->snapshot().region; }
 
 bool PathTracer::waitUntilIdle(int timeoutMs)
 {
-    const auto end = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeoutMs);
+    TracerImpl& t = *m_impl;
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeoutMs);
     for (;;) {
-        if (m_impl->idle.load() && !m_impl->resetRequested.load() && !m_impl->regionResetRequested.load()) {
+        if (t.idle.load() && t.requests.load() == 0u) {
             return true;
         }
-        if (!m_impl->running.load()) {
-            return !canTakeSample(m_impl.get());
+        if (!t.running.load()) {
+            return !t.wantsSample(t.snapshot());
         }
-        if (std::chrono::steady_clock::now() >= end) {
+        if (std::chrono::steady_clock::now() >= deadline) {
             return false;
         }
         std::this_thread::sleep_for(std::chrono::milliseconds(1));
@@ -1090,263 +1253,79 @@ bool PathTracer::waitUntilIdle(int timeoutMs)
 
 bool PathTracer::readAccumulator(float* acc4, uint32_t* counts, float* lumMean, float* m2, uint8_t* converged) const
 {
-    const auto& a = m_impl->acc;
-    if (a.buffer == nullptr) {
+    TracerImpl& t = *m_impl;
+    if (!t.surf.ready()) {
         return false;
     }
-    const size_t n = static_cast<size_t>(a.width) * static_cast<size_t>(a.height);
-    std::lock_guard<std::mutex> streamLock(m_impl->streamMutex);
-    if (hipStreamSynchronize(m_impl->sampleStream) != hipSuccess) {
-        return false;
-    }
-    return (acc4 == nullptr || hipMemcpy(acc4, a.buffer, n * sizeof(PtFloat4), hipMemcpyDeviceToHost) == hipSuccess) &&
-           (counts == nullptr || hipMemcpy(counts, a.samples, n * 4, hipMemcpyDeviceToHost) == hipSuccess) &&
-           (lumMean == nullptr || hipMemcpy(lumMean, a.lumMean, n * 4, hipMemcpyDeviceToHost) == hipSuccess) &&
-           (m2 == nullptr || hipMemcpy(m2, a.m2, n * 4, hipMemcpyDeviceToHost) == hipSuccess) &&
-           (converged == nullptr || hipMemcpy(converged, a.converged, n, hipMemcpyDeviceToHost) == hipSuccess);
+    const size_t n = t.surf.pixels();
+    std::lock_guard<std::mutex> g(t.streamLock);
+    auto fetch = [](void* dst, const void* src, size_t bytes) {
+        return dst == nullptr || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    return hipStreamSynchronize(t.sampling) == hipSuccess && fetch(acc4, t.surf.radiance.get(), n * sizeof(PtFloat4)) &&
+           fetch(counts, t.surf.counts.get(), n * sizeof(uint32_t)) && fetch(lumMean, t.surf.mean.get(), n * sizeof(float)) &&
+           fetch(m2, t.surf.m2.get(), n * sizeof(float)) && fetch(converged, t.surf.converged.get(), n);
 }
 
 bool PathTracer::readPreviewLevel(int level, float* acc4, int* width, int* height, int* downscale) const
 {
-    std::lock_guard<std::mutex> streamLock(m_impl->streamMutex);
-    if (level < 0 || level >= static_cast<int>(m_impl->previewLevels.size())) {
+    TracerImpl& t = *m_impl;
+    std::lock_guard<std::mutex> g(t.streamLock);
+    if (level < 0 || level >= static_cast<int>(t.surf.pyramid.size())) {
         return false;
     }
-    const auto& l = m_impl->previewLevels[static_cast<size_t>(level)];
+    const PreviewBuffer& b = t.surf.pyramid[static_cast<size_t>(level)];
     if (width != nullptr) {
-        *width = l.width;
+        *width = b.size.width;
     }
     if (height != nullptr) {
-        *height = l.height;
+        *height = b.size.height;
     }
     if (downscale != nullptr) {
-        *downscale = l.downscale;
+        *downscale = b.size.downscale;
     }
-    return hipStreamSynchronize(m_impl->sampleStream) == hipSuccess &&
-           (acc4 == nullptr || hipMemcpy(acc4, l.buffer, sizeof(PtFloat4) * static_cast<size_t>(l.width) * l.height,
-                                         hipMemcpyDeviceToHost) == hipSuccess);
+    return hipStreamSynchronize(t.sampling) == hipSuccess &&
+           (acc4 == nullptr ||
+            hipMemcpy(acc4, b.texels.get(), b.texels.size() * sizeof(PtFloat4), hipMemcpyDeviceToHost) == hipSuccess);
 }
 
-float PathTracer::exposureMultiplier() const
-{
-    std::lock_guard<std::mutex> lock(m_impl->cameraMutex);
-    return exposureMultiplierOf(m_impl->hostCamera.fStop, m_impl->hostCamera.shutter, m_impl->hostCamera.iso);
-}
+float PathTracer::exposureMultiplier() const { return m_impl->snapshot().optics.exposure(); }
 
 void PathTracer::setMaxPathDepth(int depth)
 {
-    depth = depth > 256 ? 256 : depth;
-    if (m_impl->hostParams.maxPathDepth == depth) {
-        return;
-    }
-    m_impl->hostParams.maxPathDepth = depth;
-    m_impl->renderParamsDirty.store(true);
-    if (m_impl->configured.load()) {
-        resetAccumulation();
+    const int v = std::min(depth, kDepthCap);
+    bool changed = false;
+    m_impl->edit([&](Controls& c) {
+        changed = c.maxDepth != v;
+        c.maxDepth = v;
+    });
+    if (changed) {
+        m_impl->resetIfConfigured();
     }
 }
-int PathTracer::maxPathDepth() const { return m_impl->hostParams.maxPathDepth; }
+int PathTracer::maxPathDepth() const { return m_impl->snapshot().maxDepth; }
 
 std::string PathTracer::lastWorkerError() const
 {
-    std::lock_guard<std::mutex> lock(m_impl->errorMutex);
+    std::lock_guard<std::mutex> g(m_impl->errorLock);
     return m_impl->workerError;
 }
 
 void PathTracer::setPassesPerLaunch(int passes)
 {
-    m_impl->passesPerLaunch.store(passes < 1 ? 1 : (passes > 256 ? 256 : passes));
+    m_impl->edit([passes](Controls& c) { c.passesPerLaunch = clampTo(passes, kPassesRange.lo, kPassesRange.hi); });
 }
+int PathTracer::passesPerLaunch() const { return m_impl->snapshot().passesPerLaunch; }
 
-int PathTracer::passesPerLaunch() const { return m_impl->passesPerLaunch.load(); }
-
-void PathTracer::setBvhBuildMode(int mode) { m_impl->bvhBuildMode.store(mode < 0 ? 0 : (mode > 2 ? 2 : mode)); }
-
-int PathTracer::bvhBuildMode() const { return m_impl->bvhBuildMode.load(); }
-
-void PathTracer::notifyWorker()
+void PathTracer::setBvhBuildMode(int mode)
 {
-    {
-        std::lock_guard<std::mutex> lock(m_impl->workerMutex);
-    }
-    m_impl->workerCv.notify_all();
+    m_impl->edit([mode](Controls& c) { c.buildMode = clampTo(mode, kBuildModeRange.lo, kBuildModeRange.hi); });
 }
+int PathTracer::bvhBuildMode() const { return m_impl->snapshot().buildMode; }
 
-void PathTracer::invokeFrameReadyCallback()
-{
-    FrameReadyCallback cb;
-    {
-        std::lock_guard<std::mutex> lock(m_impl->callbackMutex);
-        cb = m_impl->callback;
-    }
-    if (cb) {
-        cb();
-    }
-}
-
-// PathTracer.cpp:2220-2543
-void PathTracer::renderLoop()
-{
-    TracerImpl* impl = m_impl.get();
-    while (impl->running.load()) {
-        const bool fullReset = impl->resetRequested.exchange(false);
-        const bool regionReset = impl->regionResetRequested.exchange(false);
-        if (fullReset || regionReset) {
-            impl->idle.store(false);
-            bool invalidateDisplay = false;
-            bool cleared = false;
-            {
-                std::lock_guard<std::mutex> streamLock(impl->streamMutex);
-                if (impl->acc.buffer != nullptr) {
-                    if (fullReset) {
-                        if (impl->regionEnabled.load()) {
-                            cleared = clearRegionAccumulator(impl);
-                        } else if (clearAccumulator(impl) && clearPreviewLevels(impl)) {
-                            cleared = true;
-                            invalidateDisplay = true;
-                        }
-                    } else {
-                        cleared = impl->regionEnabled.load() ? clearRegionAccumulator(impl) : rebuildFullActiveList(impl);
-                    }
-                }
-            }
-            if (cleared && hipStreamSynchronize(impl->sampleStream) == hipSuccess && !invalidateDisplay &&
-                impl->sampleCount.load() > 0) {
-                invokeFrameReadyCallback();
-            }
-            if (invalidateDisplay) {
-                impl->displayBuffersValid.store(false);
-            }
-            if (fullReset) {
-                impl->sampleCount.store(0);
-                impl->finestCompletedPreview.store(-1);
-            }
-        }
-        if (impl->acc.width <= 0 || impl->acc.height <= 0) {
-            continue;
-        }
-        {
-            std::unique_lock<std::mutex> lock(impl->workerMutex);
-            if (!canTakeSample(impl) && !impl->resetRequested.load() && !impl->regionResetRequested.load()) {
-                impl->idle.store(true);
-            }
-            impl->workerCv.wait(lock, [impl]() {
-                return !impl->running.load() || impl->resetRequested.load() || impl->regionResetRequested.load() ||
-                       canTakeSample(impl);
-            });
-            if (!impl->running.load()) {
-                break;
-            }
-        }
-        if (!canTakeSample(impl)) {
-            continue;
-        }
-        impl->idle.store(false);
-
-        const int previewCount = impl->previewStepsPerLevel.load();
-        const int iter = impl->sampleCount.load();
-        const bool previewPass = !impl->regionEnabled.load() && previewCount > 0 && iter < previewCount &&
-                                 iter < static_cast<int>(impl->previewLevels.size());
-        bool launched = false;
-        bool pendingUpdate = false;
-        int passesTaken = 1;
-        static thread_local int pendingCount = 0;
-        {
-            std::lock_guard<std::mutex> streamLock(impl->streamMutex);
-            if (impl->displayPublishInFlight.load() &&
-                hipStreamWaitEvent(impl->sampleStream, impl->displayReadyEvent, 0) != hipSuccess) {
-                continue;
-            }
-            if (!uploadCameraIfDirty(impl, impl->sampleStream) ||
-                !syncRenderParamsToDevice(impl, impl->sampleStream, previewPass) ||
-                !uploadEnvironmentIfDirty(impl, impl->sampleStream)) {
-                continue;
-            }
-            const unsigned int globalSeed = static_cast<unsigned int>(iter) + 1u;
-            const MeshAccelSceneGpu* d_scene = nullptr;
-            {
-                std::lock_guard<std::mutex> sceneLock(impl->sceneMutex);
-                d_scene = ptSceneDevice(impl->scene);
-            }
-            const EnvironmentMapGpu* d_env = impl->env != nullptr ? ptEnvironmentDevice(impl->env) : nullptr;
-            if (previewPass) {
-                const auto& l = impl->previewLevels[static_cast<size_t>(iter)];
-                launched = pathTracerPreviewSample(l.buffer, l.width, l.height, impl->dCamera, d_scene, d_env,
-                                                   impl->dParams, globalSeed, impl->sampleStream);
-            } else {
-                const int active = impl->activePixelCount.load();
-                const int fullResWave = iter - previewCount + 1;
-                const int perLaunch = impl->passesPerLaunch.load();
-                bool runCompact = active > 0 && (fullResWave % kCompactActiveListInterval) == 0;
-                auto& a = impl->acc;
-                if (active > 0 && perLaunch > 1) {
-                    // several passes in one wavefront render (same per-pixel results), then a
-                    // compaction of the caller-side active list
-                    const int depth = impl->hostParams.maxPathDepth;
-                    const int resolved = depth > 0 ? depth : (depth == 0 ? 256 : 32);
-                    passesTaken = perLaunch;
-                    launched = pathTracerAdaptiveRenderWavefront(
-                        a.buffer, a.samples, a.lumMean, a.m2, a.converged, a.activeIndices, active, a.width, a.height,
-                        impl->dCamera, d_scene, d_env, impl->dParams, globalSeed, perLaunch, resolved, 0,
-                        impl->sampleStream);
-                    runCompact = true;
-                } else if (active > 0) {
-                    launched = pathTracerAdaptiveSample(a.buffer, a.samples, a.lumMean, a.m2, a.converged,
-                                                        a.activeIndices, active, a.width, a.height, impl->dCamera,
-                                                        d_scene, d_env, impl->dParams, globalSeed, impl->sampleStream);
-                }
-                if (launched && runCompact) {
-                    launched = pathTracerCompactActiveList(
-                        a.activeIndices, a.activeScratch, a.activeCount, a.converged, a.samples,
-                        impl->maxSamplesPerPixel.load(), active, a.width, a.height, impl->regionEnabled.load(),
-                        impl->regionMinX.load(), impl->regionMinY.load(), impl->regionMaxX.load(),
-                        impl->regionMaxY.load(), impl->sampleStream);
-                    if (launched) {
-                        pendingCount = 0;
-                        launched = hipMemcpyAsync(&pendingCount, a.activeCount, sizeof(int), hipMemcpyDeviceToHost,
-                                                  impl->sampleStream) == hipSuccess;
-                        pendingUpdate = launched;
-                    }
-                }
-            }
-            if (launched && hipEventRecord(impl->sampleCompleteEvent, impl->sampleStream) != hipSuccess) {
-                launched = false;
-                pendingUpdate = false;
-            }
-        }
-        if (!launched) {
-            {
-                std::lock_guard<std::mutex> errorLock(impl->errorMutex);
-                impl->workerError = pathTracerLastError();
-            }
-            if (!previewPass && impl->activePixelCount.load() > 0) {
-                std::this_thread::sleep_for(std::chrono::milliseconds(1));
-            }
-            continue;
-        }
-        const WaitResult wr = waitForSampleEvent(impl);
-        if (wr == WaitResult::Error) {
-            continue;
-        }
-        if (wr == WaitResult::Stopped) {
-            break;
-        }
-        if (wr == WaitResult::ResetDuringWait) {
-            continue;
-        }
-        if (pendingUpdate) {
-            impl->activePixelCount.store(pendingCount);
-            std::swap(impl->acc.activeIndices, impl->acc.activeScratch);
-        }
-        if (previewCount > 0 && iter < previewCount) {
-            impl->finestCompletedPreview.store(iter);
-        }
-        impl->sampleCount.fetch_add(passesTaken);
-        impl->displayBuffersValid.store(true);
-        invokeFrameReadyCallback();
-    }
-    impl->idle.store(false);
-}
+void PathTracer::notifyWorker() { m_impl->poke(); }
+void PathTracer::invokeFrameReadyCallback() { m_impl->notifyFrame(); }
+void PathTracer::renderLoop() { m_impl->workerMain(); }
 
 // ---- C wrappers -----------------------------------------------------------------------------
 
@@ -1366,6 +1345,16 @@ PtHostMesh meshOf(const HostTriangle* tris, uint32_t n, const MaterialGpu* mats,
     mesh.textures = tex;
     mesh.textureCount = t;
     return mesh;
+}
+
+std::vector<PtProceduralInstance> instancesOf(const PtProceduralInstance* p, uint32_t n)
+{
+    return p == nullptr ? std::vector<PtProceduralInstance>{} : std::vector<PtProceduralInstance>(p, p + n);
+}
+
+PtMeshSceneBuildParams meshParamsOf(const PtMeshSceneBuildParams* p)
+{
+    return p != nullptr ? *p : PtMeshSceneBuildParams{64, 50.0f};
 }
 } // namespace
 
@@ -1391,12 +1380,30 @@ bool ptTracerConfigure(PtTracer* t, int width, int height, const HostTriangle* t
     return t != nullptr && t->tracer.configure(width, height, meshOf(triangles, triangleCount, materials,
                                                                       materialCount, textures, textureCount));
 }
+bool ptTracerConfigureProcedural(PtTracer* t, int width, int height, const PtProceduralInstance* instances,
+                                 uint32_t count, const PtMeshSceneBuildParams* params)
+{
+    if (t == nullptr || (instances == nullptr && count != 0)) {
+        ptSetError("ptTracerConfigureProcedural: invalid arguments");
+        return false;
+    }
+    return t->tracer.configure(width, height, instancesOf(instances, count), meshParamsOf(params));
+}
 bool ptTracerRebuildMeshScene(PtTracer* t, const HostTriangle* triangles, uint32_t triangleCount,
                               const MaterialGpu* materials, uint32_t materialCount, const TextureDescGpu* textures,
                               uint32_t textureCount)
 {
     return t != nullptr &&
            t->tracer.rebuildMeshScene(meshOf(triangles, triangleCount, materials, materialCount, textures, textureCount));
+}
+bool ptTracerRebuildMeshSceneProcedural(PtTracer* t, const PtProceduralInstance* instances, uint32_t count,
+                                        const PtMeshSceneBuildParams* params)
+{
+    if (t == nullptr || (instances == nullptr && count != 0)) {
+        ptSetError("ptTracerRebuildMeshSceneProcedural: invalid arguments");
+        return false;
+    }
+    return t->tracer.rebuildMeshScene(instancesOf(instances, count), meshParamsOf(params));
 }
 void ptTracerStart(PtTracer* t) { if (t) t->tracer.start(); }
 void ptTracerStop(PtTracer* t) { if (t) t->tracer.stop(); }

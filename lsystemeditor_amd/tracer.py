@@ -52,6 +52,25 @@ class Tracer:
             self.width, self.height = width, height
         return bool(ok)
 
+    def configure_procedural(self, width: int, height: int, instances, circular_segments: int = 64,
+                             crease_angle_deg: float = 50.0) -> bool:
+        """configure(w, h, proceduralInstances, meshParams) (PathTracer.h:44-50): the scene is built
+        from procedural.ProceduralInstance records by the library's meshSceneBuild."""
+        from . import procedural
+        arr, _keep = procedural._instances(instances)
+        mp = procedural.MeshSceneParams(circular_segments, crease_angle_deg)
+        ok = self._lib.ptTracerConfigureProcedural(self.handle, width, height, arr, len(instances), C.byref(mp))
+        if ok:
+            self.width, self.height = width, height
+        return bool(ok)
+
+    def rebuild_mesh_scene_procedural(self, instances, circular_segments: int = 64,
+                                      crease_angle_deg: float = 50.0) -> bool:
+        from . import procedural
+        arr, _keep = procedural._instances(instances)
+        mp = procedural.MeshSceneParams(circular_segments, crease_angle_deg)
+        return bool(self._lib.ptTracerRebuildMeshSceneProcedural(self.handle, arr, len(instances), C.byref(mp)))
+
     def set_frame_ready_callback(self, fn):
         if fn is None:
             self._callback = None
