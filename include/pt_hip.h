@@ -475,19 +475,27 @@ float ptSceneLastBuildMs(const PtScene* scene);
 
 /* Tree rebuilds that a caller can capture into a graph (no reference counterpart: the reference
  * builds its tree once on the CPU). A scene built in a binned-SAH mode (1 or 2) keeps its
- * triangles on the device; ptSceneRebuild enqueues the whole build again on them, with the
- * 4-wide collapse in mode 1, into the scene's own buffers: a fixed sequence of launches on
- * `stream` with no allocation and no host synchronisation. The tree is the one ptSceneBuild made
- * (the build is deterministic), so the rebuild enqueues just the levels of that build. The workspace holds the build's temporaries for scenes of up to
- * maxTriangles triangles (about 1.1 KB per triangle); rebuilds that share one workspace must be
- * ordered on one stream. ptSceneRebuildCheck waits for `stream` and returns false, with
- * pathTracerLastError set, if the last rebuild enqueued with the workspace failed or produced a
- * tree of another shape than the scene's. */
+ * triangles on the device; ptSceneRebuild enqueues the whole build again on them, in the mode the
+ * scene was built with (with the 4-wide collapse in mode 1; a later ptSceneSetBuildMode applies
+ * to the next ptSceneBuild only), into the scene's own buffers: a fixed sequence of launches on
+ * `stream` with no allocation and no host synchronisation. The rebuilt tree is the one ptSceneBuild
+ * made (the build is deterministic and the triangles are unchanged), so a rebuild enqueues just
+ * the levels that first build needed. The workspace holds the build's temporaries for scenes of
+ * up to maxTriangles triangles (about 1.1 KB per triangle); rebuilds that share one workspace
+ * must be ordered on one stream. Every rebuild ends by copying its status into the scene's own
+ * status record, so ptSceneRebuildCheck (which waits for `stream`) checks that scene's last
+ * rebuild whatever was rebuilt after it, and returns false, with pathTracerLastError set, if that
+ * rebuild failed, produced a tree of another shape than the scene's, or none has run since the
+ * scene's build or its last check (each replay of a captured rebuild rewrites the record). The
+ * workspace argument of the check is not used. */
 typedef struct PtBuildWorkspace PtBuildWorkspace;
 PtBuildWorkspace* ptBuildWorkspaceCreate(int maxTriangles);
 void ptBuildWorkspaceDestroy(PtBuildWorkspace* workspace);
 bool ptSceneRebuild(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream);
-bool ptSceneRebuildCheck(const PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream);
+/* As ptSceneRebuild, but enqueues every level the depth cap allows (49), as a build of triangles
+ * whose tree depth is not known yet must: the cost of rebuilding changed geometry in a graph. */
+bool ptSceneRebuildCold(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream);
+bool ptSceneRebuildCheck(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream);
 
 /* Replaces the material table of a built scene (no reference counterpart; for differentiable
  * rendering and optimisation loops): same count; a material may not start or stop emitting.

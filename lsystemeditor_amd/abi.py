@@ -105,6 +105,7 @@ SIGNATURES = {
     "ptBuildWorkspaceDestroy": (None, [_p]),
     "ptSceneRebuild": (_b, [_p, _p, _p]),
     "ptSceneRebuildCheck": (_b, [_p, _p, _p]),
+    "ptSceneRebuildCold": (_b, [_p, _p, _p]),
     "ptEnvironmentCreate": (_p, [_p, _i, _i, _p]),
     "ptEnvironmentDevice": (_p, [_p]),
     "ptEnvironmentReadTables": (_b, [_p, _p, _p]),
@@ -117,6 +118,9 @@ SIGNATURES = {
     "ptTracerRebuildMeshScene": (_b, [_p, _p, _u, _p, _u, _p, _u]),
     "ptTracerConfigureProcedural": (_b, [_p, _i, _i, _p, _u, _p]),
     "ptEnvironmentLoadHdr": (_p, [C.c_char_p, _p, _p]),
+    "pathTracerWavefrontPins": (_i, []),
+    "pathTracerWavefrontPin": (_b, []),
+    "pathTracerWavefrontUnpin": (_b, [_i]),
     "ptEnvironmentFreeImage": (None, [_p]),
     "ptTracerRebuildMeshSceneProcedural": (_b, [_p, _p, _u, _p]),
     "ptTracerStart": (None, [_p]),
@@ -376,9 +380,11 @@ class Scene:
     def build_ms(self) -> float:
         return lib().ptSceneLastBuildMs(self.handle)
 
-    def rebuild(self, workspace: "BuildWorkspace", stream: int = 0) -> None:
-        """Enqueues the scene's binned-SAH build again on its device triangles (capturable)."""
-        check(lib().ptSceneRebuild(self.handle, workspace.handle, stream), "ptSceneRebuild")
+    def rebuild(self, workspace: "BuildWorkspace", stream: int = 0, cold: bool = False) -> None:
+        """Enqueues the scene's binned-SAH build again on its device triangles (capturable): the
+        levels its first build needed, or with cold every level the depth cap allows."""
+        fn = lib().ptSceneRebuildCold if cold else lib().ptSceneRebuild
+        check(fn(self.handle, workspace.handle, stream), "ptSceneRebuildCold" if cold else "ptSceneRebuild")
 
     def rebuild_check(self, workspace: "BuildWorkspace", stream: int = 0) -> None:
         """Waits for the stream; raises if the last rebuild with `workspace` did not reproduce the tree."""

@@ -62,6 +62,7 @@ class BatchRenderer:
         self.batch_graph = None
         self.build_graph = None
         self.workspace = None
+        self._pins = 0          # workspace pins taken by this renderer's captures
         torch.cuda.synchronize()
 
     @property
@@ -87,10 +88,12 @@ class BatchRenderer:
         self._reserve()
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
+        p0 = render.workspace_pins()
         with torch.cuda.graph(g, stream=self.stream):
             for sc, acc, inp, spec in zip(self.scenes, self.accs, self.inputs, self.specs):
                 render.render_fixed(sc, acc, inp, spec.spp, max_paths=self.max_paths,
                                     stream=torch.cuda.current_stream())
+        self._pins += render.workspace_pins() - p0
         self.graph = g
 
     def replay(self):
@@ -116,16 +119,20 @@ class BatchRenderer:
         b.reserve(self.max_paths)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
+        p0 = render.workspace_pins()
         with torch.cuda.graph(g, stream=self.stream):
             b.render_fixed(self.specs[0].spp, max_paths=self.max_paths, stream=torch.cuda.current_stream())
+        self._pins += render.workspace_pins() - p0
         self.batch_graph = g
 
     def replay_batched(self):
         self.batch_graph.replay()
 
-    def capture_build_and_render(self):
+    def capture_build_and_render(self, cold: bool = False):
         """Records every scene's tree rebuild and then render_batched into one graph (binned-SAH
-        build modes only)."""
+        build modes only). The rebuilds enqueue the levels each scene's first build needed (a
+        rebuild of unchanged geometry), or with cold every level the depth cap allows (what a build
+        of new geometry enqueues)."""
         if any(sc.bvh not in ("sah", "sah-binary") for sc in self.scenes):
             raise ValueError("tree rebuilds need a binned-SAH build mode")
         b = self._batch()
@@ -134,19 +141,31 @@ class BatchRenderer:
             self.workspace = abi.BuildWorkspace(max(sc.triangle_count for sc in self.scenes))
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
+        p0 = render.workspace_pins()
         with torch.cuda.graph(g, stream=self.stream):
             s = torch.cuda.current_stream()
             for sc in self.scenes:
-                sc.rebuild(self.workspace, s.cuda_stream)
+                sc.rebuild(self.workspace, s.cuda_stream, cold=cold)
             b.render_fixed(self.specs[0].spp, max_paths=self.max_paths, stream=s)
+        self._pins += render.workspace_pins() - p0
         self.build_graph = g
+
+    def release_graphs(self):
+        """Drops this renderer's captured graphs and the workspace pins they took, so later calls
+        may grow the shared workspace again."""
+        torch.cuda.synchronize()
+        self.graph = self.batch_graph = self.build_graph = None
+        if self._pins:
+            render.release_workspace_pins(self._pins)
+            self._pins = 0
 
     def replay_build_and_render(self):
         self.build_graph.replay()
 
     def check_rebuild(self):
-        """Waits for the batch stream; raises unless the last scene's rebuild reproduced its tree."""
-        self.scenes[-1].rebuild_check(self.workspace, int(self.stream.cuda_stream))
+        """Waits for the batch stream; raises unless every scene's last rebuild reproduced its tree."""
+        for sc in self.scenes:
+            sc.rebuild_check(self.workspace, int(self.stream.cuda_stream))
 
     def batched_image(self, i: int) -> torch.Tensor:
         """Scene i's (pixels, 4) accumulator rows of the last batch render."""

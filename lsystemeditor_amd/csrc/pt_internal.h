@@ -36,6 +36,16 @@ bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, floa
 // the 4-wide node count (each optional).
 bool pt_sah_result(PtSahWork* w, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
                    std::vector<int>* levelStarts, int* wideCount);
+// Status of a build as a flat record (the control block, then the root box and reference as 8
+// words): pt_sah_snapshot enqueues a copy of the last build's status into dStatus (a kernel, so
+// it can be captured), pt_sah_status_result checks such a copy as pt_sah_result does.
+size_t pt_sah_status_words();
+bool pt_sah_snapshot(PtSahWork* w, int* dStatus, hipStream_t stream);
+// Marks a status copy as read (synchronous): until a rebuild's snapshot overwrites it,
+// pt_sah_status_result reports that no rebuild ran.
+bool pt_sah_status_consume(int* dStatus);
+bool pt_sah_status_result(const int* dStatus, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
+                          int* wideCount);
 // 4-wide tree collapsed from a level-numbered binary tree of internalCount internal nodes
 // (pt_sah.hip): wideOut (device, 8 float4 per node) and its node count; the wide root is node 0
 // where the binary root is internal.

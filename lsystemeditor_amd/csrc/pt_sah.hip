@@ -1007,15 +1007,14 @@ bool pt_sah_enqueue(PtSahWork* w, const float* dTris, int n, float4* nodes, floa
     return true;
 }
 
-bool pt_sah_result(PtSahWork* w, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
-                   std::vector<int>* levelStarts, int* wideCount)
+namespace {
+
+// Checks a host copy of a build's control block: false with the error set unless it describes
+// a finished tree of n - 1 internal nodes.
+bool sah_check(const SahCtl& ctl, int n)
 {
-    SahCtl ctl{};
-    if (hipMemcpyAsync(&ctl, w->ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        (rootOut != nullptr &&
-         hipMemcpyAsync(rootOut, w->rootBox, 7 * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess) ||
-        hipStreamSynchronize(stream) != hipSuccess) {
-        ptSetError("SAH build: synchronisation failed");
+    if (ctl.error == 3) {
+        ptSetError("no rebuild of this scene has run since its build or its last check");
         return false;
     }
     if (ctl.error == 1) {
@@ -1028,6 +1027,82 @@ bool pt_sah_result(PtSahWork* w, int n, hipStream_t stream, float rootOut[7], in
     }
     if (ctl.levelBase != n - 1 || ctl.segCount != 0) {
         ptSetError("SAH build: %d internal nodes for %d triangles", ctl.levelBase, n);
+        return false;
+    }
+    return true;
+}
+
+constexpr int kCtlWords = static_cast<int>(sizeof(SahCtl) / sizeof(int));
+
+__global__ void sah_snapshot_kernel(const int* __restrict__ ctl, const int* __restrict__ box, int* __restrict__ out)
+{
+    for (int i = threadIdx.x; i < kCtlWords + 8; i += blockDim.x) {
+        out[i] = i < kCtlWords ? ctl[i] : (i - kCtlWords < 7 ? box[i - kCtlWords] : 0);
+    }
+}
+
+} // namespace
+
+size_t pt_sah_status_words() { return static_cast<size_t>(kCtlWords) + 8; }
+
+bool pt_sah_status_consume(int* dStatus)
+{
+    // the control block's error word 3: "no rebuild since" (a rebuild's snapshot overwrites it)
+    const int none = 3;
+    return hipMemcpy(dStatus + offsetof(SahCtl, error) / sizeof(int), &none, sizeof(int), hipMemcpyHostToDevice) ==
+           hipSuccess;
+}
+
+bool pt_sah_snapshot(PtSahWork* w, int* dStatus, hipStream_t stream)
+{
+    if (w == nullptr || dStatus == nullptr) {
+        ptSetError("SAH build: no status buffer");
+        return false;
+    }
+    hipLaunchKernelGGL(sah_snapshot_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<const int*>(w->ctl),
+                       reinterpret_cast<const int*>(w->rootBox), dStatus);
+    return hipGetLastError() == hipSuccess;
+}
+
+bool pt_sah_status_result(const int* dStatus, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
+                          int* wideCount)
+{
+    std::vector<int> words(pt_sah_status_words());
+    if (hipMemcpyAsync(words.data(), dStatus, words.size() * sizeof(int), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        ptSetError("SAH build: synchronisation failed");
+        return false;
+    }
+    SahCtl ctl{};
+    std::memcpy(&ctl, words.data(), sizeof(ctl));
+    if (!sah_check(ctl, n)) {
+        return false;
+    }
+    if (rootOut != nullptr) {
+        std::memcpy(rootOut, words.data() + kCtlWords, 7 * sizeof(float));
+    }
+    if (levelsOut != nullptr) {
+        *levelsOut = ctl.level + 1;
+    }
+    if (wideCount != nullptr) {
+        *wideCount = ctl.wideCount;
+    }
+    return true;
+}
+
+bool pt_sah_result(PtSahWork* w, int n, hipStream_t stream, float rootOut[7], int* levelsOut,
+                   std::vector<int>* levelStarts, int* wideCount)
+{
+    SahCtl ctl{};
+    if (hipMemcpyAsync(&ctl, w->ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        (rootOut != nullptr &&
+         hipMemcpyAsync(rootOut, w->rootBox, 7 * sizeof(float), hipMemcpyDeviceToHost, stream) != hipSuccess) ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        ptSetError("SAH build: synchronisation failed");
+        return false;
+    }
+    if (!sah_check(ctl, n)) {
         return false;
     }
     if (levelsOut != nullptr) {

@@ -467,6 +467,8 @@ struct PtScene {
     int levels = 0;
     float* srcTris = nullptr; // the triangles (25-word records) the SAH modes keep for ptSceneRebuild
     int triangleCount = 0;
+    int builtMode = -1;        // the build mode of the last ptSceneBuild (what the device layout holds)
+    int* rebuildStatus = nullptr; // this scene's copy of its last rebuild's status (pt_sah_snapshot)
     // host copies for ptSceneUpdateMaterials: materials, and per emissive entry its area and
     // material index
     std::vector<MaterialGpu> matsHost;
@@ -492,6 +494,9 @@ struct PtScene {
         release(refNodes);
         release(srcTris);
         srcTris = nullptr;
+        release(rebuildStatus);
+        rebuildStatus = nullptr;
+        builtMode = -1;
         triangleCount = 0;
         d_desc = nullptr;
         nodes = nullptr;
@@ -756,6 +761,7 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
     if (ok && n > 0 && (sc->buildMode == 1 || sc->buildMode == 2)) {
         sc->srcTris = dTris;  // kept for ptSceneRebuild
         sc->triangleCount = static_cast<int>(n);
+        ok = alloc(&sc->rebuildStatus, pt_sah_status_words()) && pt_sah_status_consume(sc->rebuildStatus);
     } else {
         release(dTris);
     }
@@ -796,6 +802,7 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
     if (wide) {
         wide_registry(sc->d_desc, true);
     }
+    sc->builtMode = sc->buildMode;
     return true;
 }
 
@@ -805,7 +812,7 @@ bool ptSceneReadWide(const PtScene* scene, float* nodes, int* count, int* rootRe
         ptSetError("ptSceneReadWide: invalid arguments");
         return false;
     }
-    if (scene->buildMode != 1) {
+    if (scene->builtMode != 1) {
         ptSetError("ptSceneReadWide: the scene has no 4-wide tree (build mode 1)");
         return false;
     }
@@ -996,37 +1003,59 @@ void ptBuildWorkspaceDestroy(PtBuildWorkspace* workspace)
     pt_sah_work_destroy(reinterpret_cast<PtSahWork*>(workspace));
 }
 
-bool ptSceneRebuild(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
+namespace {
+
+bool rebuild_levels(PtScene* scene, PtBuildWorkspace* workspace, int levels, hipStream_t stream, const char* who)
 {
     if (scene == nullptr || workspace == nullptr) {
-        ptSetError("ptSceneRebuild: null scene or workspace");
+        ptSetError("%s: null scene or workspace", who);
         return false;
     }
-    if (scene->srcTris == nullptr || scene->triangleCount < 1) {
-        ptSetError("ptSceneRebuild: the scene was not built with a binned-SAH build mode");
+    if (scene->srcTris == nullptr || scene->triangleCount < 1 || (scene->builtMode != 1 && scene->builtMode != 2)) {
+        ptSetError("%s: the scene was not built with a binned-SAH build mode", who);
         return false;
     }
-    return pt_sah_enqueue(reinterpret_cast<PtSahWork*>(workspace), scene->srcTris, scene->triangleCount,
-                          scene->nodes, scene->triPos, scene->triShade, scene->buildMode == 1 ? scene->wide : nullptr,
-                          scene->wideCount, scene->levels, stream);
+    // the layout the scene was built with (a later ptSceneSetBuildMode applies to the next build)
+    PtSahWork* w = reinterpret_cast<PtSahWork*>(workspace);
+    return pt_sah_enqueue(w, scene->srcTris, scene->triangleCount, scene->nodes, scene->triPos, scene->triShade,
+                          scene->builtMode == 1 ? scene->wide : nullptr, scene->wideCount, levels, stream) &&
+           pt_sah_snapshot(w, scene->rebuildStatus, stream);
 }
 
-bool ptSceneRebuildCheck(const PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
+} // namespace
+
+bool ptSceneRebuildCold(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
 {
-    if (scene == nullptr || workspace == nullptr || scene->srcTris == nullptr) {
-        ptSetError("ptSceneRebuildCheck: null scene or workspace, or a scene without a binned-SAH build");
+    return rebuild_levels(scene, workspace, 0, stream, "ptSceneRebuildCold");
+}
+
+bool ptSceneRebuild(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
+{
+    return rebuild_levels(scene, workspace, scene != nullptr ? scene->levels : 0, stream, "ptSceneRebuild");
+}
+
+bool ptSceneRebuildCheck(PtScene* scene, PtBuildWorkspace* workspace, hipStream_t stream)
+{
+    (void)workspace;   // each scene keeps its own copy of its last rebuild's status
+    if (scene == nullptr || scene->srcTris == nullptr || scene->rebuildStatus == nullptr) {
+        ptSetError("ptSceneRebuildCheck: null scene, or a scene without a binned-SAH build");
         return false;
     }
     const int n = scene->triangleCount;
     float root[7] = {0};
     int levels = 0, wideCount = 0;
-    if (!pt_sah_result(reinterpret_cast<PtSahWork*>(workspace), n, stream, root, &levels, nullptr, &wideCount)) {
+    const bool read = pt_sah_status_result(scene->rebuildStatus, n, stream, root, &levels, &wideCount);
+    if (!pt_sah_status_consume(scene->rebuildStatus) && read) {
+        ptSetError("ptSceneRebuildCheck: resetting the scene's rebuild status failed");
+        return false;
+    }
+    if (!read) {
         return false;
     }
     int rootRef = 0;
     std::memcpy(&rootRef, &root[6], sizeof(int));
     bool same = levels == scene->levels && rootRef == scene->desc.rootRef &&
-                (scene->buildMode != 1 || n < 2 || wideCount == scene->wideCount);
+                (scene->builtMode != 1 || n < 2 || wideCount == scene->wideCount);
     for (int k = 0; k < 3; ++k) {
         same = same && root[k] == scene->desc.rootMin[k] && root[3 + k] == scene->desc.rootMax[k];
     }

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -1562,6 +1563,14 @@ struct Workspace {
 };
 Workspace gWs;
 
+// The workspace's host-side record is shared by every entry point below: one recursive lock
+// serialises them while they size the workspace and enqueue (kernels of calls on different
+// streams still must not overlap on the device). A graph captured over a fixed-spp render keeps
+// the workspace's addresses, so each captured call adds a pin, and while pins are held the
+// workspace does not move: a call that would need a larger one fails instead.
+std::recursive_mutex gWsLock;
+int gWsPins = 0;
+
 // Opt-in measurement: hipEvents around every launch of the render stream, accumulated per
 // kernel class, and (counted mode) traversal counters of the trace kernels.
 struct Profile {
@@ -1678,6 +1687,14 @@ bool ensure_workspace(int cap, int pixels, int depth, int spillThreads)
         gWs.spillThreads >= spillThreads) {
         return true;
     }
+    if (gWsPins > 0) {
+        ptSetError("wavefront workspace: %d graph capture(s) hold it at %d paths, %d pixels, depth %d; it cannot grow "
+                   "to %d paths, %d pixels, depth %d until they are released (pathTracerWavefrontUnpin). Reserve "
+                   "enough with pathTracerWavefrontReserve before capturing",
+                   gWsPins, gWs.capacity, gWs.pixels, gWs.depth, std::max(cap, gWs.capacity),
+                   std::max(pixels, gWs.pixels), std::max(depth, gWs.depth));
+        return false;
+    }
     cap = std::max(cap, gWs.capacity);
     pixels = std::max(pixels, gWs.pixels);
     depth = std::max(depth, gWs.depth);
@@ -1789,6 +1806,7 @@ extern "C" bool pathTracerAdaptiveRenderWavefront(PtFloat4* d_buffer, uint32_t* 
                                                   int passCount, int maxPathDepth, int maxPathsInFlight,
                                                   hipStream_t stream)
 {
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
     if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
         d_converged == nullptr || d_activeIndices == nullptr || d_camera == nullptr || d_params == nullptr ||
         d_scene == nullptr || width <= 0 || height <= 0 || activeCount <= 0 || passCount <= 0 ||
@@ -1991,8 +2009,33 @@ extern "C" bool pathTracerWavefrontCaptureRadiance(float* d_out, int samplesPerP
     return true;
 }
 
+extern "C" int pathTracerWavefrontPins(void)
+{
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
+    return gWsPins;
+}
+
+extern "C" bool pathTracerWavefrontPin(void)
+{
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
+    ++gWsPins;
+    return true;
+}
+
+extern "C" bool pathTracerWavefrontUnpin(int count)
+{
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
+    if (count < 0 || count > gWsPins) {
+        ptSetError("pathTracerWavefrontUnpin: %d pins requested, %d held", count, gWsPins);
+        return false;
+    }
+    gWsPins -= count;
+    return true;
+}
+
 extern "C" bool pathTracerWavefrontReserve(int maxPathsInFlight, int pixels, int maxPathDepth)
 {
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
     if (maxPathsInFlight <= 0 || pixels <= 0 || maxPathDepth < 0 || maxPathDepth > 256) {
         ptSetError("pathTracerWavefrontReserve: invalid arguments");
         return false;
@@ -2014,6 +2057,7 @@ bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean
                        int maxPathDepth, int maxPathsInFlight, unsigned int firstSeed, hipStream_t stream,
                        const BatchScene* tab = nullptr, int tabCount = 0, int batchPixels = 0, bool batchWide = false)
 {
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
     const bool single = tab == nullptr;
     const bool wide = single ? pt_scene_is_wide(d_scene) : batchWide;
     if (d_buffer == nullptr || d_samples == nullptr || d_lumMean == nullptr || d_m2 == nullptr ||
@@ -2051,6 +2095,9 @@ bool render_fixed_impl(PtFloat4* d_buffer, uint32_t* d_samples, float* d_lumMean
         }
     }
     gCapturing = capturing;
+    if (capturing) {
+        ++gWsPins;   // the graph being captured keeps the workspace's addresses
+    }
     Paths P = gWs.P;
     int* list = gWs.lists[0];
     int* count = gWs.listCount;
@@ -2302,6 +2349,7 @@ bool pt_adjoint_wavefront(const float* d_gradImage, int width, int height, int s
                           const RenderParamsGpu* d_params, int materialCount, const float* d_sampleRadiance,
                           double* d_grad, unsigned int firstSeed, hipStream_t stream)
 {
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
     RenderParamsGpu hp;
     if (hipMemcpyAsync(&hp, d_params, sizeof(hp), hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess) {
@@ -2519,6 +2567,7 @@ extern "C" bool pathTracerRenderFixedBatch(PtFloat4* d_buffer, uint32_t* d_sampl
 extern "C" bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtBatch* batch, int samplesPerPixel,
                                              unsigned int firstSeed, double* d_grad, hipStream_t stream)
 {
+    std::lock_guard<std::recursive_mutex> wsLock(gWsLock);
     if (d_gradImage == nullptr || batch == nullptr || d_grad == nullptr || samplesPerPixel <= 0) {
         ptSetError("pathTracerAdjointRenderBatch: invalid arguments");
         return false;

@@ -236,6 +236,18 @@ def count_traversal(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, ac
     return out.cpu().numpy().view(np.uint32)
 
 
+def workspace_pins() -> int:
+    """Pins on the shared wavefront workspace: one per graph-captured fixed-spp render; while any
+    is held the workspace does not move and calls that need a larger one fail."""
+    return int(abi.lib().pathTracerWavefrontPins())
+
+
+def release_workspace_pins(count: int | None = None) -> None:
+    """Releases `count` pins (all when None); call it once the graphs that took them are gone."""
+    n = workspace_pins() if count is None else count
+    abi.check(abi.lib().pathTracerWavefrontUnpin(n), "pathTracerWavefrontUnpin")
+
+
 def render_fixed(scene: abi.Scene, acc: Accumulator, inputs: DeviceInputs, spp: int, max_paths: int = 0,
                  clear: bool = True, stream=None) -> None:
     """Fixed-spp wavefront render without host synchronisation (pathTracerRenderFixedWavefront);

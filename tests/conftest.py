@@ -20,3 +20,15 @@ def cuda():
     if not torch.cuda.is_available():
         pytest.skip("no GPU visible to torch")
     return torch
+
+
+@pytest.fixture(autouse=True)
+def _release_workspace_pins():
+    """Graphs a test captures die with the test; the wavefront workspace pins they took are
+    released after it, so a later test may grow the workspace."""
+    yield
+    from lsystemeditor_amd import abi
+    if abi._lib is not None:
+        n = abi._lib.pathTracerWavefrontPins()
+        if n:
+            abi._lib.pathTracerWavefrontUnpin(n)

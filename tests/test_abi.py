@@ -106,3 +106,18 @@ def test_light_sampling_mode_validation_and_adjoint_refusal(L):
         assert L.pathTracerDetachedRender(FAKE, 4, 4, 2, FAKE, FAKE, FAKE, None, FAKE, None) is False
         assert "reference light sampling only" in abi.last_error()
     assert L.pathTracerLightSampling() == 0
+
+
+def test_workspace_pins_refuse_growth_without_gpu(L):
+    """While graph captures pin the shared wavefront workspace, a call that would need a larger one
+    fails with a clear error before any device allocation; pins are released explicitly."""
+    assert L.pathTracerWavefrontPins() == 0
+    assert L.pathTracerWavefrontPin()
+    assert L.pathTracerWavefrontPins() == 1
+    assert L.pathTracerWavefrontReserve(1 << 20, 4096, 8) is False
+    msg = abi.last_error()
+    assert "graph capture(s) hold it" in msg and "pathTracerWavefrontUnpin" in msg
+    assert L.pathTracerWavefrontUnpin(2) is False and "2 pins requested, 1 held" in abi.last_error()
+    assert L.pathTracerWavefrontUnpin(1)
+    assert L.pathTracerWavefrontPins() == 0
+    assert L.pathTracerWavefrontUnpin(-1) is False

@@ -115,7 +115,8 @@ def test_rebuilds_and_batch_render_in_one_graph(cuda):
         b.acc.counts.zero_()
         g.replay()
         torch.cuda.synchronize()
-        items[-1][0].rebuild_check(ws)
+        for sc, _, _ in items:          # every scene's own status record, not only the last rebuild's
+            sc.rebuild_check(ws)
         got = b.acc.host()
         for k in ("acc", "counts", "mean", "m2", "conv"):
             assert np.array_equal(np.asarray(got[k]).view(np.uint8), np.asarray(eager[k]).view(np.uint8)), (rep, k)
@@ -123,4 +124,29 @@ def test_rebuilds_and_batch_render_in_one_graph(cuda):
             for k, (x, y) in enumerate(zip(_layout(sc, sc.bvh == "sah"), want)):
                 assert np.array_equal(np.asarray(x), np.asarray(y)), (rep, k)
     b.close()
+    ws.close()
+
+
+def test_each_scene_checks_its_own_last_rebuild(cuda):
+    """Several scenes rebuilt in a row on one workspace: every scene's check reads its own status
+    record (a failure of an earlier rebuild is not hidden by a later one), a check with no rebuild
+    since the last one is refused, and a rebuild uses the mode the scene was built with."""
+    scs = [abi.Scene(random_soup(n, 40 + n), abi.pack_material()[None], bvh="sah") for n in (5, 300, 2049)]
+    wants = [_layout(sc, True) for sc in scs]
+    ws = abi.BuildWorkspace(2049)
+    for sc in scs:
+        _clobber(sc)
+        sc.rebuild(ws)
+    for sc, want in zip(scs, wants):
+        sc.rebuild_check(ws)
+        assert all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(_layout(sc, True), want))
+    with pytest.raises(RuntimeError, match="no rebuild of this scene has run"):
+        scs[0].rebuild_check(ws)
+    # switching the scene's build mode after its build does not change what a rebuild makes
+    L = abi.lib()
+    abi.check(L.ptSceneSetBuildMode(scs[1].handle, 2), "ptSceneSetBuildMode")
+    _clobber(scs[1])
+    scs[1].rebuild(ws)
+    scs[1].rebuild_check(ws)
+    assert all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(_layout(scs[1], True), wants[1]))
     ws.close()
