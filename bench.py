@@ -45,7 +45,7 @@ def parse():
                     help="paths in flight per wavefront group of the C1 render (0 = the library default)")
     ap.add_argument("--bvh", choices=["median", "sah", "sah-binary"], default="sah",
                     help="BVH build: the reference's median split, or binned SAH (same results, faster traversal)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every host core this process may use")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-pass (reference cadence) timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-render (C2) measurement")
@@ -74,10 +74,24 @@ def read_traffic(name: str):
         return None
 
 
-def cpu_baseline(spec, threads: int):
-    """The oracle (CPU restatement, C with pthreads) on a bounded sample of the same workload:
-    a centred 512x512 window of the full-resolution frame for its first 64 passes (about ten
-    seconds of CPU work on 16 host threads)."""
+def host_cores() -> int:
+    """Cores this process may run on: its CPU affinity, capped by a cgroup CPU quota if one is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if quota != "max":
+            n = min(n, max(1, int(quota) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_baseline(spec, threads: int, runs: int = 5):
+    """The oracle (CPU restatement, C with pthreads) on a bounded sample of the same workload: a
+    centred 256x256 window of the full-resolution frame for its first 32 passes, on every host core
+    this process may use. Two builds of the same sources: the parity build (FP contraction off, the
+    order of operations the GPU matches bit for bit) and a contraction build (FMA, AVX2). Each is
+    timed as the median of `runs` runs after one warm-up run; the BVH build is timed separately."""
     import ctypes as C
     from lsystemeditor_amd import abi
     from oracle import oracle as O
@@ -86,24 +100,37 @@ def cpu_baseline(spec, threads: int):
     sc = O.OracleScene(abi.host_to_gpu_triangles(spec.triangles), spec.materials)
     build_s = time.perf_counter() - t0
     w, h = spec.width, spec.height
-    win = 512
+    win, passes = 256, 32
     x0, y0 = (w - win) // 2, (h - win) // 2
     idx = ((np.arange(y0, y0 + win)[:, None] * w) + np.arange(x0, x0 + win)[None, :]).astype(np.int32).ravel()
-    acc = O.Accumulator(w, h)
-    acc.clear()
     env = O.Env()
-    L = O.lib()
-    passes = 64
-    t0 = time.perf_counter()
-    for p in range(passes):
-        L.orc_adaptive_sample(O.ptr(acc.acc), O.ptr(acc.counts), O.ptr(acc.mean), O.ptr(acc.m2), O.ptr(acc.conv),
-                              O.ptr(idx), idx.size, w, h, O.ptr(spec.camera), C.byref(sc.scene), C.byref(env),
-                              O.ptr(spec.params), p + 1, threads)
-    dt = time.perf_counter() - t0
-    samples = int(acc.counts[idx].sum())
-    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C, pthreads) on a centred {win}x{win} window of the {w}x{h} frame, passes 1-{passes} "
-                      f"({samples} samples, {dt:.1f} s; BVH build {build_s:.1f} s excluded)"}
+    builds = {}
+    for variant in ("parity", "contract"):
+        L = O.lib(variant)
+        times, samples = [], 0
+        for run in range(runs + 1):
+            acc = O.Accumulator(w, h)
+            acc.clear()
+            t0 = time.perf_counter()
+            for p in range(passes):
+                L.orc_adaptive_sample(O.ptr(acc.acc), O.ptr(acc.counts), O.ptr(acc.mean), O.ptr(acc.m2),
+                                      O.ptr(acc.conv), O.ptr(idx), idx.size, w, h, O.ptr(spec.camera),
+                                      C.byref(sc.scene), C.byref(env), O.ptr(spec.params), p + 1, threads)
+            dt = time.perf_counter() - t0
+            samples = int(acc.counts[idx].sum())
+            if run > 0:
+                times.append(dt)
+        med = float(np.median(times))
+        builds[variant] = {"msamples_s": samples / med / 1e6, "median_s": round(med, 3),
+                           "runs_s": [round(t, 3) for t in times], "samples": samples}
+    best = max(builds, key=lambda k: builds[k]["msamples_s"])
+    return {"value": builds[best]["msamples_s"], "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "build": best, "builds": builds,
+            "sample": f"oracle (C, pthreads, {threads} threads) on a centred {win}x{win} window of the {w}x{h} frame, "
+                      f"passes 1-{passes} ({builds[best]['samples']} samples); median of {runs} runs after a warm-up; "
+                      f"parity build (-O3 -ffp-contract=off) {builds['parity']['msamples_s']:.3f}, contraction build "
+                      f"(-O3 -march=x86-64-v3 -ffp-contract=fast) {builds['contract']['msamples_s']:.3f} Msamples/s; "
+                      f"BVH build {build_s:.1f} s excluded"}
 
 
 def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
@@ -522,17 +549,17 @@ def main():
         extra["batch"] = batch_bench(args.bvh)
 
     if not args.no_adjoint and args.config == "c1":
-        extra["adjoint"] = adjoint_bench(spec, args.bvh, args.cpu_threads or min(16, os.cpu_count() or 1))
+        extra["adjoint"] = adjoint_bench(spec, args.bvh, args.cpu_threads or host_cores())
 
     if not args.no_inverse and args.config == "c1":
         extra["inverse"] = inverse_bench(args.bvh, args.inverse_steps)
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cores()
         cpu = cpu_baseline(spec, threads)
-        print(f"[bench] cpu baseline {cpu['value']:.3f} Msamples/s on {threads} threads", file=sys.stderr,
-              flush=True)
+        print(f"[bench] cpu baseline {cpu['value']:.3f} Msamples/s on {threads} threads ({cpu['sample']})",
+              file=sys.stderr, flush=True)
 
     if args.copy_calibration:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -566,6 +593,9 @@ def main():
                    "samples_started_per_step": started // args.steps, **extra},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"wf_trace_{spec.name.lower()}_traffic.txt"),
+                     "traffic_unit": "HBM bytes per trace launch (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a "
+                                     "512 MiB copy; profiles/wf_trace_c1_traffic.txt)",
+                     "algorithmic_bytes_per_launch": trace_bytes / max(1, (n_ext + n_sh) // args.steps),
                      "kernel": ("wf_trace_kernel<false,false,false,W> (extension rays) + "
                                 "wf_trace_kernel<true,false,false,W> (shadow rays), W = 4-wide tree"
                                 if args.bvh == "sah" else
@@ -573,6 +603,7 @@ def main():
                      "kernel_ms_per_step": trace_ms_per_step,
                      "launches_per_step": (n_ext + n_sh) / args.steps,
                      "bytes_per_step": trace_bytes,
+                     "bytes_per_step_unit": "algorithmic bytes of all trace launches of one step (model below)",
                      "model": {"ext_nodes": ext_nodes, "ext_triangles": ext_tris, "ext_rays": ext_rays,
                                "shadow_nodes": sh_nodes, "shadow_triangles": sh_tris, "shadow_rays": sh_rays,
                                "node_bytes": node_bytes, "tri_bytes": TRI_BYTES, "ext_ray_bytes": EXT_RAY_BYTES,

@@ -14,14 +14,17 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
+# The same sources with FP contraction allowed (FMA, AVX2 target): a CPU-baseline timing
+# variant only; its results are not bit-equal to the reference order of operations.
+LIB_CONTRACT_PATH = HERE / "liboracle_contract.so"
 REF_PATH = HERE / "_ref" / "libref_policy.so"
 REF_PROC_PATH = HERE / "_ref" / "libref_procedural.so"
 
 
-def build(force: bool = False) -> Path:
-    if force or not LIB_PATH.exists():
-        subprocess.run(["make", "-C", str(HERE), "liboracle.so"], check=True, capture_output=True)
-    return LIB_PATH
+def build(force: bool = False, path: Path = LIB_PATH) -> Path:
+    if force or not path.exists():
+        subprocess.run(["make", "-C", str(HERE), path.name], check=True, capture_output=True)
+    return path
 
 
 class Vec3(C.Structure):
@@ -89,14 +92,16 @@ class TraceStats(C.Structure):
 assert C.sizeof(Material) == 96
 assert C.sizeof(Scene) == 80
 
-_lib = None
+_libs: dict = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        build()
-        L = C.CDLL(str(LIB_PATH))
+def lib(variant: str = "parity"):
+    """The oracle library: "parity" (FP contraction off, the checker) or "contract" (contraction
+    allowed; used only to time the CPU baseline)."""
+    if variant not in _libs:
+        path = LIB_PATH if variant == "parity" else LIB_CONTRACT_PATH
+        build(path=path)
+        L = C.CDLL(str(path))
         f = C.c_float
         fp = C.POINTER(C.c_float)
         vp = C.c_void_p
@@ -194,8 +199,8 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def ref_policy():
