@@ -1684,7 +1684,7 @@ PT_DEV float env_pdf(const EnvironmentMapGpu* env, V3 wi)
 
 // LightSamplingCore.h:257-313 on the emissive array (v0, v1, v2 with r*e, g*e, b*e in .w).
 PT_DEV bool sample_emissive(const MeshAccelSceneGpu& s, float uTri, float u1, float u2, V3& pos, V3& ln, V3& L,
-                            float& pdf)
+                            float& pdf, int* entry = nullptr)
 {
     pdf = 0.0f;
     if (s.emissiveTriangleCount == 0 || s.emissiveTriangleCdf == nullptr || s.emissiveTris == nullptr) {
@@ -1708,6 +1708,9 @@ PT_DEV bool sample_emissive(const MeshAccelSceneGpu& s, float uTri, float u1, fl
     const float4 b = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(lo) + 1u);
     const float4 c = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(lo) + 2u);
     const V3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
+    if (entry != nullptr) {
+        *entry = lo;
+    }
     const float su = sqrtf(max2(0.0f, u1));
     const float b0 = 1.0f - su;
     const float b1 = u2 * su;
@@ -1723,6 +1726,32 @@ PT_DEV bool sample_emissive(const MeshAccelSceneGpu& s, float uTri, float u1, fl
     const float sel = (s.emissiveTriangleCdf[lo + 1] - s.emissiveTriangleCdf[lo]) / total;
     pdf = sel / area;
     return pdf > kMinPdf;
+}
+
+// Whether an any-hit trace of r over [tMin, tMax) is certain to report a hit because of emissive
+// list entry `entry` (its vertex floats are the scene triangle's). Every traversal here tests a
+// triangle exactly when the slab test of its own box (its vertex floats as planes) admits the ray
+// within [tMin, tMax * kCullSlack]: the boxes above it contain that box, and slab distances move
+// monotonically with the planes, so no ancestor rejects what the triangle's box admits. When that
+// box admits the ray and tri_test finds the triangle at t < tMax, every traversal reports the ray
+// occluded. The reference's emissive next-event shadow ray runs to kRayTMax along the direction of
+// a point sampled on the triangle, so it nearly always meets the triangle itself: one triangle test
+// settles it, and only the rays this cannot settle are traced.
+PT_DEV bool emissive_blocks(const MeshAccelSceneGpu& s, int entry, const Ray& r, float tMin, float tMax)
+{
+    const float4 a = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(entry) + 0u);
+    const float4 b = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(entry) + 1u);
+    const float4 c = ld4(s.emissiveTris, 3u * static_cast<uint32_t>(entry) + 2u);
+    float te;
+    if (!slab(r, min2(min2(a.x, b.x), c.x), min2(min2(a.y, b.y), c.y), min2(min2(a.z, b.z), c.z),
+              max2(max2(a.x, b.x), c.x), max2(max2(a.y, b.y), c.y), max2(max2(a.z, b.z), c.z), tMin,
+              tMax * kCullSlack, te)) {
+        return false;
+    }
+    const float4 e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+    const float4 e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+    float t, u, v;
+    return tri_test(r, a, e1, e2, tMin, tMax, t, u, v) && t < tMax;
 }
 
 // Position of original triangle `tri` in the emissive list (ascending triangle indices), or -1.

@@ -65,7 +65,8 @@ PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, cons
     }
     V3 lp, ln, lr;
     float areaPdf = 0.0f;
-    if (!sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
+    int entry = 0;
+    if (!sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf, &entry)) {
         return zero;
     }
     const V3 toL = sub(lp, pos);
@@ -91,7 +92,8 @@ PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, cons
         if (trace_any(s, o, toP, kShadowBias, length(toP) * kBoundedShadowScale, st)) {
             return zero;
         }
-    } else if (trace_any(s, o, wi, kShadowBias, kRayTMax, st)) {
+    } else if (emissive_blocks(s, entry, make_ray(o, wi), kShadowBias, kRayTMax) ||
+               trace_any(s, o, wi, kShadowBias, kRayTMax, st)) {
         return zero;
     }
     const C f = brdf_eval_t(c, wi, base, rad);

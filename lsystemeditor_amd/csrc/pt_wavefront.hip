@@ -223,6 +223,24 @@ struct LdsStack {
         ++sp;
     }
 
+    // Moves the oldest ring entries to the spill area until n more entries fit in the ring.
+    PT_DEV void make_room(int n)
+    {
+        while (sp + n - lo > kLdsEntries) {
+            spill[static_cast<size_t>(lo) * spillStride] = ring[(lo & (kLdsEntries - 1)) * kWfBlock];
+            ++lo;
+        }
+    }
+
+    // Writes stack position k (lo <= k < lo + kLdsEntries) without moving sp.
+    PT_DEV void put(int k, int ref, float t)
+    {
+        NativeInt2 e;
+        e.x = ref;
+        e.y = __float_as_int(t);
+        ring[(k & (kLdsEntries - 1)) * kWfBlock] = e;
+    }
+
     // Requires sp > 0. When the ring is empty the entries below it are reloaded from the spill
     // area first, so the pop itself is always an LDS read.
     PT_DEV int2 pop()
@@ -266,17 +284,37 @@ PT_DEV TraceScene trace_scene(const MeshAccelSceneGpu& m)
                       {m.rootMax[0], m.rootMax[1], m.rootMax[2]}};
 }
 
-// Orders two (entry distance, reference) pairs by distance.
+// Orders two (entry distance, reference) pairs by distance (selects, no branch).
 PT_DEV void order2(float& da, int& ra, float& db, int& rb)
 {
-    if (db < da) {
-        const float d = da;
-        const int r = ra;
-        da = db;
-        ra = rb;
-        db = d;
-        rb = r;
-    }
+    const bool swap = db < da;
+    const float dLo = swap ? db : da;
+    const float dHi = swap ? da : db;
+    const int rLo = swap ? rb : ra;
+    const int rHi = swap ? ra : rb;
+    da = dLo;
+    db = dHi;
+    ra = rLo;
+    rb = rHi;
+}
+
+// slab() for one child box of a 4-wide node: the entry distance, or kInfDist where the ray misses
+// the box within [tMin, tMax]. The slab distances are the same floats as slab()'s: where slab()
+// swaps the two distances of an axis for a negative reciprocal, the smaller of them is the near
+// one (rounding is monotonic, so (max - o) * inv <= (min - o) * inv then), and min/max select
+// the same values without a branch or a per-ray plane order. The running max and min are taken in
+// another order, which gives the same values (no distance is NaN: the planes and the ray are
+// finite, and a zero reciprocal gives zero distances; the sign of a zero changes no comparison),
+// and te <= tx alone decides the hit, since te >= tMin.
+PT_DEV float wide_child(const Ray& r, float x0, float y0, float z0, float x1, float y1, float z1, float tMin,
+                        float tMax)
+{
+    const float ax = (x0 - r.o.x) * r.inv.x, bx = (x1 - r.o.x) * r.inv.x;
+    const float ay = (y0 - r.o.y) * r.inv.y, by = (y1 - r.o.y) * r.inv.y;
+    const float az = (z0 - r.o.z) * r.inv.z, bz = (z1 - r.o.z) * r.inv.z;
+    const float te = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tMin));
+    const float tx = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tMax));
+    return te <= tx ? te : kInfDist;
 }
 
 // Persistent traversal: every lane takes the next ray from its wave's claimed range of the queue as
@@ -293,8 +331,11 @@ PT_DEV void order2(float& da, int& ra, float& db, int& rb)
 //   WIDE:            traverse the scene's 4-wide tree: the four child boxes of a node are tested
 //                    at once (the binary tree's box floats, so the same entry distances), the
 //                    nearest hit child is visited next and the others are pushed farthest first.
+// The kernels are bound by the latency of their dependent node fetches, so they are held to 72
+// VGPRs: 7 waves per SIMD instead of 6 (C1: extension trace 43.8 -> 42.2 ms per frame); at 64
+// VGPRs (8 waves) the compiler spills and the extension trace takes 57 ms.
 template <bool ANY_HIT, bool STATS, bool BATCH, bool WIDE>
-__global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
+__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
                                                             PathState PS, const int* __restrict__ queue,
                                                             const int* __restrict__ countPtr, int* fetchCounters,
                                                             unsigned long long* stats, int4* __restrict__ hitA,
@@ -436,8 +477,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 const float4 a = ld4(s.tris, 3u * slot + 0u);
                 const float4 b = ld4(s.tris, 3u * slot + 1u);
                 const float4 c = ld4(s.tris, 3u * slot + 2u);
-                // all 48 bytes in one round trip: keeps the compiler from deferring v0's load
-                // past the determinant test
+                // all 48 bytes in one round trip: keeps the compiler from deferring v0's
+                // load past the determinant test
                 asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w));
                 if (STATS) {
                     ts.tris += 1u;
@@ -470,16 +511,15 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 }
                 int r0 = __float_as_int(refs.x), r1 = __float_as_int(refs.y), r2 = __float_as_int(refs.z),
                     r3 = __float_as_int(refs.w);
-                float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
-                float d0 = (r0 != kDone && slab(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, cullT, e0))
-                    ? e0 : kInfDist;
-                float d1 = (r1 != kDone && slab(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, cullT, e1))
-                    ? e1 : kInfDist;
-                float d2 = (r2 != kDone && slab(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, cullT, e2))
-                    ? e2 : kInfDist;
-                float d3 = (r3 != kDone && slab(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, cullT, e3))
-                    ? e3 : kInfDist;
-                // sorting network for four
+                const float c0 = wide_child(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, cullT);
+                const float c1 = wide_child(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, cullT);
+                const float c2 = wide_child(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, cullT);
+                const float c3 = wide_child(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, cullT);
+                float d0 = r0 != kDone ? c0 : kInfDist;
+                float d1 = r1 != kDone ? c1 : kInfDist;
+                float d2 = r2 != kDone ? c2 : kInfDist;
+                float d3 = r3 != kDone ? c3 : kInfDist;
+                // sorting network for four; the children hit come first
                 order2(d0, r0, d1, r1);
                 order2(d2, r2, d3, r3);
                 order2(d0, r0, d2, r2);
@@ -488,15 +528,22 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
                 if (d0 == kInfDist) {
                     pop = true;
                 } else {
-                    if (d3 != kInfDist && st.sp < kSpillEntries) {
-                        st.push(r3, d3);
+                    // the others farthest first, so the nearest of them is popped first
+                    // (the stack never holds more than 3 entries per level, 72 in all)
+                    const int np = min(static_cast<int>(d1 != kInfDist) + static_cast<int>(d2 != kInfDist) +
+                                           static_cast<int>(d3 != kInfDist),
+                                       kSpillEntries - st.sp);
+                    st.make_room(np);
+                    if (np == 3) {
+                        st.put(st.sp, r3, d3);
                     }
-                    if (d2 != kInfDist && st.sp < kSpillEntries) {
-                        st.push(r2, d2);
+                    if (np >= 2) {
+                        st.put(st.sp + np - 2, r2, d2);
                     }
-                    if (d1 != kInfDist && st.sp < kSpillEntries) {
-                        st.push(r1, d1);
+                    if (np >= 1) {
+                        st.put(st.sp + np - 1, r1, d1);
                     }
+                    st.sp += np;
                     cur = r0;
                 }
             } else {
@@ -543,8 +590,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_trace_kernel(const MeshAccelScene
             if (ANY_HIT) {
                 PS.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
             } else {
-                hitA[tag] = make_int4(static_cast<int>(tag), hit ? static_cast<int>(bestTri) : -1, hit ? bestMat : -1,
-                                      __float_as_int(hit ? bestT : 0.0f));
+                hitA[tag] = make_int4(static_cast<int>(tag), hit ? static_cast<int>(bestTri) : -1,
+                                      hit ? bestMat : -1, __float_as_int(hit ? bestT : 0.0f));
                 hitB[tag] = make_float2(bestU, bestV);
             }
             item = -1;
@@ -852,8 +899,9 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                         const float u2 = rand01(rng);
                         V3 lp, ln, lr;
                         float areaPdf = 0.0f;
+                        int entry = 0;
                         if (!(lightMode == kLightMis && bounce + 1 == depthLimit) &&
-                            sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
+                            sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf, &entry)) {
                             const V3 toL = sub(lp, pos);
                             const float d2 = dot(toL, toL);
                             if (d2 > kMinPdf) {
@@ -868,11 +916,15 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                                         const float mis = mis_balance(lightPdf, bpdf);
                                         const float sc = mis * cosT / lightPdf;
                                         cm = mul(T, mul(f, scale(pol.radiance(lr), sc)));
-                                        wantEmis = true;
                                         emO = add(pos, scale(wi, kShadowBias));
                                         // kLightMis: the shadow ray runs to the light point (the
-                                        // trace kernel bounds it); otherwise along wi to kRayTMax
+                                        // trace kernel bounds it); otherwise along wi to kRayTMax,
+                                        // where the sampled triangle itself nearly always stops
+                                        // it: then no ray is queued, which the next bounce reads
+                                        // exactly as a traced ray found occluded
                                         emD = lightMode == kLightMis ? lp : wi;
+                                        wantEmis = lightMode == kLightMis ||
+                                                   !emissive_blocks(s, entry, make_ray(emO, wi), kShadowBias, kRayTMax);
                                     }
                                 }
                             }
