@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-render (C2) measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-scene (C3) measurement")
     ap.add_argument("--no-spectral", action="store_true", help="skip the hero-wavelength (spectral) measurement")
+    ap.add_argument("--no-mis", action="store_true", help="skip the light sampling (MIS vs reference) measurement")
     ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-rendering (C4) measurement")
     ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -357,6 +358,87 @@ def batch_bench(bvh: str) -> dict:
             **(full or {})}
 
 
+def light_sampling_bench(scene, spec) -> dict:
+    """The two light sampling modes on the C1 frame, at equal samples and at equal error: mode 0 is
+    the reference's (its emissive shadow ray is always occluded, so the area lights are reached by
+    BSDF sampling alone), mode 1 adds multiple importance sampling on the area lights. A reference
+    image takes 1024 spp per pixel in mode 1 (pass seeds 100001..); each mode then renders 64 spp
+    per pixel (seeds 1..64). Reported per mode: Msamples/s, the RMSE of luminance against the
+    reference (raw, and with the reference's own expected error removed), and the time the mode
+    needs to reach mode 1's RMSE (RMSE falls as 1/sqrt(spp)). Then the adaptive C1 render (relative
+    error 0.01, 16-256 spp) in each mode: samples, time and RMSE against the same reference."""
+    import math
+
+    import torch
+
+    from lsystemeditor_amd import abi, render
+    w, h = spec.width, spec.height
+    lum_w = torch.tensor([0.2126, 0.7152, 0.0722], device="cuda", dtype=torch.float64)
+
+    def fixed_inputs(spp: int):
+        p = spec.params.copy()
+        ints = p[4:10].view(np.int32).copy()
+        ints[2] = spp  # minSamples
+        ints[3] = spp  # maxSamplesPerPixel
+        p[4:10] = ints.view(np.float32)
+        return render.DeviceInputs(spec.camera, p)
+
+    def lum(a):
+        return a.acc[:, :3].double() @ lum_w
+
+    acc = render.Accumulator.allocate(w, h)
+    ref_spp, spp = 1024, 64
+    t0 = time.perf_counter()
+    with abi.LightSampling(1):
+        render.render_wavefront(scene, acc, fixed_inputs(ref_spp), first_iter=100000)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    ref = lum(acc).clone()
+    ref_noise = float((acc.m2.double() / (ref_spp - 1)).mean()) / ref_spp  # expected MSE the reference adds
+    out = {"workload": "C1 frame, light sampling mode 0 (reference) vs 1 (MIS on area lights)",
+           "reference": f"{ref_spp} spp per pixel in mode 1, pass seeds 100001-{100000 + ref_spp} "
+                        f"({ref_s:.1f} s)", "fixed_spp": spp}
+    modes = {}
+    for mode in (0, 1):
+        inp = fixed_inputs(spp)
+        with abi.LightSampling(mode):
+            render.render_wavefront(scene, acc, inp)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            render.render_wavefront(scene, acc, inp)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+        mse = float(((lum(acc) - ref) ** 2).mean())
+        modes[mode] = {"msamples_s": w * h * spp / dt / 1e6, "seconds": dt, "rmse": math.sqrt(mse),
+                       "rmse_without_reference_noise": math.sqrt(max(mse - ref_noise, 0.0))}
+    for mode in (0, 1):
+        m = modes[mode]
+        # seconds to reach mode 1's error at 64 spp: spp scales with (own RMSE / target RMSE)^2
+        m["seconds_to_mode1_rmse"] = m["seconds"] * (m["rmse_without_reference_noise"] /
+                                                     modes[1]["rmse_without_reference_noise"]) ** 2
+    out["mode0"], out["mode1"] = modes[0], modes[1]
+    out["equal_rmse_speedup_mode1"] = modes[0]["seconds_to_mode1_rmse"] / modes[1]["seconds_to_mode1_rmse"]
+    inputs = render.DeviceInputs(spec.camera, spec.params)
+    for mode in (0, 1):
+        with abi.LightSampling(mode):
+            render.render_wavefront(scene, acc, inputs)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            render.render_wavefront(scene, acc, inputs)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t2
+        n = int(acc.counts.to(torch.int64).sum().item())
+        out[f"adaptive_mode{mode}"] = {"samples": n, "seconds": dt, "msamples_s": n / dt / 1e6,
+                                       "rmse": math.sqrt(float(((lum(acc) - ref) ** 2).mean()))}
+    print(f"[bench] light sampling at {spp} spp: mode 0 {modes[0]['msamples_s']:.0f} Msamples/s RMSE "
+          f"{modes[0]['rmse_without_reference_noise']:.4g}, mode 1 {modes[1]['msamples_s']:.0f} Msamples/s RMSE "
+          f"{modes[1]['rmse_without_reference_noise']:.4g}; time to equal RMSE: mode 1 "
+          f"{out['equal_rmse_speedup_mode1']:.2f}x faster; adaptive C1: mode 0 {out['adaptive_mode0']['samples']} "
+          f"samples RMSE {out['adaptive_mode0']['rmse']:.4g}, mode 1 {out['adaptive_mode1']['samples']} samples "
+          f"RMSE {out['adaptive_mode1']['rmse']:.4g}", file=sys.stderr, flush=True)
+    return out
+
+
 def spectral_bench(scene, spec, inputs, acc, passes: int, bvh: str) -> dict:
     """Hero-wavelength rendering: the C1 frame and the C0 workload (four wavelengths in 380..720 nm)
     with the project's rgb2spec table; Msamples/s beside the RGB figures."""
@@ -541,6 +623,9 @@ def main():
               flush=True)
         if args.config == "c1":
             extra["facade"] = facade_bench(spec)
+
+    if not args.no_mis and args.config == "c1":
+        extra["light_sampling"] = light_sampling_bench(scene, spec)
 
     if not args.no_spectral and args.config == "c1":
         extra["spectral"] = spectral_bench(scene, spec, inputs, acc, passes, args.bvh)
