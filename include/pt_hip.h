@@ -309,6 +309,16 @@ bool pathTracerRenderFixedWavefront(PtFloat4* d_buffer, uint32_t* d_samples, flo
  * maxPathDepth (grow-only; a single allocation). */
 bool pathTracerWavefrontReserve(int maxPathsInFlight, int pixels, int maxPathDepth);
 
+/* Workspace pins (no reference counterpart). A fixed-spp or batch render enqueued while its stream
+ * is capturing a graph takes one pin: the graph keeps the workspace's addresses, so while any pin
+ * is held the workspace does not move, and a call that needs a larger one fails with an error
+ * (reserve enough first). pathTracerWavefrontPin takes a pin for a caller's own capture,
+ * pathTracerWavefrontPins reports the pins held, and pathTracerWavefrontUnpin releases `count` of
+ * them once the graphs that took them are destroyed (false if fewer are held). */
+bool pathTracerWavefrontPin(void);
+int pathTracerWavefrontPins(void);
+bool pathTracerWavefrontUnpin(int count);
+
 /* Measurement hooks of pathTracerAdaptiveRenderWavefront (no reference counterpart).
  * timing != 0: every launch of the render is bracketed by hipEvents on its stream and the device
  * time is accumulated per kernel class; counted != 0: the trace kernels also count internal

@@ -1,6 +1,6 @@
 """Parity at the configured workload sizes (the renderer's five workloads, at full size on one GPU).
 
-- C1 (1024^2, ~2M triangles, adaptive 16-256 spp at relative error 0.01, depth 12, two area lights
+- C1 (1024^2, 3.84M triangles, adaptive 16-256 spp at relative error 0.01, depth 12, two area lights
   and a uniform environment): the bench's render (binned-SAH tree with its 4-wide collapse,
   wavefront schedule) of the whole frame; 48x48 windows replayed by the oracle pass by pass
   (PathTracer.cpp:2220-2543 schedule: pass p uses globalSeed p + 1; converged pixels are skipped,
