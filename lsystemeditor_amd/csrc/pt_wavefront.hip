@@ -60,19 +60,64 @@ enum : uint32_t {
     kFlagReach = 8u, // the extension direction lies where the vertex's emissive NEE samples (kLightMis)
 };
 
-// Path state, structure of arrays over the slots of one bounce's shade queue. Two buffers
-// alternate by bounce parity: the shade kernel reads item i of its queue at slot i of one buffer
-// and writes the paths that go on at their slot of the next queue in the other, so every state
-// access is coalesced (the generation kernel fills buffer 0 in path order).
+// Path state over the slots of one bounce's shade queue, in blocks of 64 slots (one wave's items):
+// field k of slot i is the 4-byte word at base + (i / 64) * kPathBlockBytes + 256 k + 4 (i % 64).
+// A wave's access to one field is one contiguous 256-byte segment, as in a structure of arrays,
+// while a kernel needs one base pointer per state instead of one per field (the shade kernel reads
+// one state and writes another: with a pointer per field the pointers alone outgrew its scalar
+// registers). Two states alternate by bounce parity: the shade kernel reads item i of its queue at
+// slot i of one and writes the paths that go on at their slot of the next queue in the other (the
+// generation kernel fills state 0 in path order).
+enum PathField : uint32_t {
+    kPfTx, kPfTy, kPfTz, kPfTw, kPfLx, kPfLy, kPfLz, kPfLw, kPfEta, kPfPdf, kPfOx, kPfOy, kPfOz, kPfDx, kPfDy, kPfDz, kPfTmin, kPfCex, kPfCey, kPfCez, kPfCew, kPfCmx, kPfCmy, kPfCmz, kPfCmw, kPfHero, kPfPixel, kPfSample, kPfSeed, kPfDraws, kPfFlags, kPfSlot0, kPfScene,
+    kPfOcc,  // the two shadow results of the slot: byte 0 environment, byte 1 emissive
+    kPathFields
+};
+constexpr size_t kPathBlockBytes = 256u * kPathFields;
+
 struct PathState {
-    float *Tx, *Ty, *Tz, *Tw, *Lx, *Ly, *Lz, *Lw, *eta, *pdf;
-    float *ox, *oy, *oz, *dx, *dy, *dz, *tmin;
-    float *cex, *cey, *cez, *cew, *cmx, *cmy, *cmz, *cmw; // pending NEE contributions
-    float* hero;                                          // hero-wavelength draw (spectral mode)
-    uint32_t *pixel, *sample, *seed, *draws, *flags;
-    uint32_t* slot0; // the path's generation slot, where its final radiance goes
-    uint32_t* scene; // batch renders: the path's scene (pixel is then the scene-local pixel)
-    uint8_t* occ;    // 2 per slot: environment, emissive shadow results
+    char* base;
+
+    PT_DEV char* word(uint32_t k, uint32_t i) const
+    {
+        return base + static_cast<size_t>(i >> 6) * kPathBlockBytes + 256u * k + 4u * (i & 63u);
+    }
+    PT_DEV float& fl(uint32_t k, uint32_t i) const { return *reinterpret_cast<float*>(word(k, i)); }
+    PT_DEV uint32_t& u32(uint32_t k, uint32_t i) const { return *reinterpret_cast<uint32_t*>(word(k, i)); }
+    PT_DEV uint8_t& occ(uint32_t i, uint32_t type) const { return *reinterpret_cast<uint8_t*>(word(kPfOcc, i) + type); }
+    PT_DEV float& Tx(uint32_t i) const { return fl(kPfTx, i); }
+    PT_DEV float& Ty(uint32_t i) const { return fl(kPfTy, i); }
+    PT_DEV float& Tz(uint32_t i) const { return fl(kPfTz, i); }
+    PT_DEV float& Tw(uint32_t i) const { return fl(kPfTw, i); }
+    PT_DEV float& Lx(uint32_t i) const { return fl(kPfLx, i); }
+    PT_DEV float& Ly(uint32_t i) const { return fl(kPfLy, i); }
+    PT_DEV float& Lz(uint32_t i) const { return fl(kPfLz, i); }
+    PT_DEV float& Lw(uint32_t i) const { return fl(kPfLw, i); }
+    PT_DEV float& eta(uint32_t i) const { return fl(kPfEta, i); }
+    PT_DEV float& pdf(uint32_t i) const { return fl(kPfPdf, i); }
+    PT_DEV float& ox(uint32_t i) const { return fl(kPfOx, i); }
+    PT_DEV float& oy(uint32_t i) const { return fl(kPfOy, i); }
+    PT_DEV float& oz(uint32_t i) const { return fl(kPfOz, i); }
+    PT_DEV float& dx(uint32_t i) const { return fl(kPfDx, i); }
+    PT_DEV float& dy(uint32_t i) const { return fl(kPfDy, i); }
+    PT_DEV float& dz(uint32_t i) const { return fl(kPfDz, i); }
+    PT_DEV float& tmin(uint32_t i) const { return fl(kPfTmin, i); }
+    PT_DEV float& cex(uint32_t i) const { return fl(kPfCex, i); }
+    PT_DEV float& cey(uint32_t i) const { return fl(kPfCey, i); }
+    PT_DEV float& cez(uint32_t i) const { return fl(kPfCez, i); }
+    PT_DEV float& cew(uint32_t i) const { return fl(kPfCew, i); }
+    PT_DEV float& cmx(uint32_t i) const { return fl(kPfCmx, i); }
+    PT_DEV float& cmy(uint32_t i) const { return fl(kPfCmy, i); }
+    PT_DEV float& cmz(uint32_t i) const { return fl(kPfCmz, i); }
+    PT_DEV float& cmw(uint32_t i) const { return fl(kPfCmw, i); }
+    PT_DEV float& hero(uint32_t i) const { return fl(kPfHero, i); }
+    PT_DEV uint32_t& pixel(uint32_t i) const { return u32(kPfPixel, i); }
+    PT_DEV uint32_t& sample(uint32_t i) const { return u32(kPfSample, i); }
+    PT_DEV uint32_t& seed(uint32_t i) const { return u32(kPfSeed, i); }
+    PT_DEV uint32_t& draws(uint32_t i) const { return u32(kPfDraws, i); }
+    PT_DEV uint32_t& flags(uint32_t i) const { return u32(kPfFlags, i); }
+    PT_DEV uint32_t& slot0(uint32_t i) const { return u32(kPfSlot0, i); }
+    PT_DEV uint32_t& scene(uint32_t i) const { return u32(kPfScene, i); }
 };
 
 struct Paths {
@@ -431,10 +476,10 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                     } else {
                         const int q = queue != nullptr ? queue[idx] : idx;
                         tag = static_cast<uint32_t>(q);
-                        r = make_ray(mk(PS.ox[q], PS.oy[q], PS.oz[q]), mk(PS.dx[q], PS.dy[q], PS.dz[q]));
-                        tMin = PS.tmin[q];
+                        r = make_ray(mk(PS.ox(q), PS.oy(q), PS.oz(q)), mk(PS.dx(q), PS.dy(q), PS.dz(q)));
+                        tMin = PS.tmin(q);
                         if constexpr (BATCH) {
-                            s = trace_scene(tab[PS.scene[q]].scene);
+                            s = trace_scene(tab[PS.scene(q)].scene);
                         }
                     }
                     bestT = rayMax;
@@ -588,7 +633,7 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
         if (finished) {
             const bool hit = bestT < rayMax;
             if (ANY_HIT) {
-                PS.occ[2 * (tag >> 2) + ((tag & 3u) - 1u)] = hit ? 1u : 0u;
+                PS.occ(tag >> 2, (tag & 3u) - 1u) = hit ? 1u : 0u;
             } else {
                 hitA[tag] = make_int4(static_cast<int>(tag), hit ? static_cast<int>(bestTri) : -1,
                                       hit ? bestMat : -1, __float_as_int(hit ? bestT : 0.0f));
@@ -703,34 +748,34 @@ __global__ __launch_bounds__(kWfBlock) void wf_gen_kernel(const int* __restrict_
         camera_ray(cam, u, v, l1, l2, ro, rd);
         PathState& O = P.S[0];
         if (tab != nullptr) {
-            O.scene[p] = static_cast<uint32_t>(oSid);
+            O.scene(p) = static_cast<uint32_t>(oSid);
         }
         if (spectral) {
-            O.hero[p] = rand01(rng);  // the hero-wavelength draw follows the lens draws
-            O.Tw[p] = 1.0f;
-            O.Lw[p] = 0.0f;
+            O.hero(p) = rand01(rng);  // the hero-wavelength draw follows the lens draws
+            O.Tw(p) = 1.0f;
+            O.Lw(p) = 0.0f;
         }
-        O.pixel[p] = static_cast<uint32_t>(oIdx);
-        O.sample[p] = oN + static_cast<uint32_t>(j);
-        O.seed[p] = seed;
-        O.draws[p] = rng_draws(rng);
-        O.slot0[p] = static_cast<uint32_t>(p);
-        O.ox[p] = ro.x;
-        O.oy[p] = ro.y;
-        O.oz[p] = ro.z;
-        O.dx[p] = rd.x;
-        O.dy[p] = rd.y;
-        O.dz[p] = rd.z;
-        O.tmin[p] = 0.0f;
-        O.Tx[p] = 1.0f;
-        O.Ty[p] = 1.0f;
-        O.Tz[p] = 1.0f;
-        O.Lx[p] = 0.0f;
-        O.Ly[p] = 0.0f;
-        O.Lz[p] = 0.0f;
-        O.eta[p] = 1.0f;
-        O.pdf[p] = 0.0f;
-        O.flags[p] = kFlagExt;
+        O.pixel(p) = static_cast<uint32_t>(oIdx);
+        O.sample(p) = oN + static_cast<uint32_t>(j);
+        O.seed(p) = seed;
+        O.draws(p) = rng_draws(rng);
+        O.slot0(p) = static_cast<uint32_t>(p);
+        O.ox(p) = ro.x;
+        O.oy(p) = ro.y;
+        O.oz(p) = ro.z;
+        O.dx(p) = rd.x;
+        O.dy(p) = rd.y;
+        O.dz(p) = rd.z;
+        O.tmin(p) = 0.0f;
+        O.Tx(p) = 1.0f;
+        O.Ty(p) = 1.0f;
+        O.Tz(p) = 1.0f;
+        O.Lx(p) = 0.0f;
+        O.Ly(p) = 0.0f;
+        O.Lz(p) = 0.0f;
+        O.eta(p) = 1.0f;
+        O.pdf(p) = 0.0f;
+        O.flags(p) = kFlagExt;
     }
 }
 
@@ -743,29 +788,27 @@ PT_DEV void store_shadow(Paths& P, int slot, V3 o, V3 d, int item, uint32_t type
 }
 
 // One bounce of PathIntegratorRandCore.h:241-347 for every path in the shade queue.
-#ifndef PT_SHADE_ATTR
-#define PT_SHADE_ATTR
-#endif
-PT_DEV V3 ld_col(const float* x, const float* y, const float* z, const float*, int p, const V3*)
+// Colours stored as four consecutive fields from k (RGB uses the first three).
+PT_DEV V3 ld_col(const PathState& S, uint32_t k, uint32_t p, const V3*)
 {
-    return mk(x[p], y[p], z[p]);
+    return mk(S.fl(k, p), S.fl(k + 1u, p), S.fl(k + 2u, p));
 }
-PT_DEV S4 ld_col(const float* x, const float* y, const float* z, const float* w, int p, const S4*)
+PT_DEV S4 ld_col(const PathState& S, uint32_t k, uint32_t p, const S4*)
 {
-    return mk4(x[p], y[p], z[p], w[p]);
+    return mk4(S.fl(k, p), S.fl(k + 1u, p), S.fl(k + 2u, p), S.fl(k + 3u, p));
 }
-PT_DEV void st_col(float* x, float* y, float* z, float*, int p, V3 v)
+PT_DEV void st_col(const PathState& S, uint32_t k, uint32_t p, V3 v)
 {
-    x[p] = v.x;
-    y[p] = v.y;
-    z[p] = v.z;
+    S.fl(k, p) = v.x;
+    S.fl(k + 1u, p) = v.y;
+    S.fl(k + 2u, p) = v.z;
 }
-PT_DEV void st_col(float* x, float* y, float* z, float* w, int p, S4 v)
+PT_DEV void st_col(const PathState& S, uint32_t k, uint32_t p, S4 v)
 {
-    x[p] = v.x;
-    y[p] = v.y;
-    z[p] = v.z;
-    w[p] = v.w;
+    S.fl(k, p) = v.x;
+    S.fl(k + 1u, p) = v.y;
+    S.fl(k + 2u, p) = v.z;
+    S.fl(k + 3u, p) = v.w;
 }
 PT_DEV RgbColors make_colors(float, const SpecModel&, const RgbColors*) { return RgbColors{}; }
 PT_DEV HeroColors make_colors(float hero, const SpecModel& sm, const HeroColors*) { return hero_colors(sm, hero); }
@@ -776,11 +819,11 @@ PT_DEV V3 finish_sample(const HeroColors& pol, S4 L) { return clamp_firefly_spec
 // environment through its own pointer into the batch table (lanes of a wave nearly always share
 // a scene, so these reads share cache lines), and its depth limit is its scene's.
 template <class Pol, bool BATCH>
-__global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
-    const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
-    PathState O, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA, const float2* __restrict__ inB,
-    const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA, int* counters, float* capture,
-    int captureSpp, SpecModel sm, const BatchScene* __restrict__ tab, int lightMode)
+PT_DEV void shade_items(const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env,
+                        PathState I, PathState O, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA,
+                        const float2* __restrict__ inB, const int* __restrict__ inCountPtr, int* extOut,
+                        int4* __restrict__ outA, int* counters, float* capture, int captureSpp, SpecModel sm,
+                        const BatchScene* __restrict__ tab, int lightMode)
 {
     using C = typename Pol::Col;
     constexpr bool kSpectral = ColN<C>::n == 4;
@@ -811,26 +854,26 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
             const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
             const int4 ha = inA[i];
             const float2 hb = inB[i];
-            const uint32_t flags = I.flags[i];
-            pixel = I.pixel[i];
-            sample = I.sample[i];
-            seed = I.seed[i];
-            slot0 = I.slot0[i];
+            const uint32_t flags = I.flags(i);
+            pixel = I.pixel(i);
+            sample = I.sample(i);
+            seed = I.seed(i);
+            slot0 = I.slot0(i);
             if constexpr (kSpectral) {
-                hero = I.hero[i];
+                hero = I.hero(i);
             }
             const Pol pol = make_colors(hero, sm, static_cast<const Pol*>(nullptr));
-            L = ld_col(I.Lx, I.Ly, I.Lz, I.Lw, i, ct);
+            L = ld_col(I, kPfLx, i, ct);
             // shadow results of the previous bounce, in the reference's order (env, then emissive)
-            if ((flags & kFlagEnv) != 0u && I.occ[2 * i] == 0u) {
-                L = add(L, ld_col(I.cex, I.cey, I.cez, I.cew, i, ct));
+            if ((flags & kFlagEnv) != 0u && I.occ(i, 0) == 0u) {
+                L = add(L, ld_col(I, kPfCex, i, ct));
             }
-            if ((flags & kFlagEmis) != 0u && I.occ[2 * i + 1] == 0u) {
-                L = add(L, ld_col(I.cmx, I.cmy, I.cmz, I.cmw, i, ct));
+            if ((flags & kFlagEmis) != 0u && I.occ(i, 1) == 0u) {
+                L = add(L, ld_col(I, kPfCmx, i, ct));
             }
             if ((flags & kFlagExt) != 0u) {
-                const V3 origin = mk(I.ox[i], I.oy[i], I.oz[i]);
-                const V3 dir = mk(I.dx[i], I.dy[i], I.dz[i]);
+                const V3 origin = mk(I.ox(i), I.oy(i), I.oz(i));
+                const V3 dir = mk(I.dx(i), I.dy(i), I.dz(i));
                 const uint32_t tri = static_cast<uint32_t>(ha.y);
                 if (tri == 0xFFFFFFFFu) {
                     if (bounce == 0) {
@@ -838,8 +881,8 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                     } else {                                         // :338-345
                         const C envL = pol.radiance(env_eval(env, prm, dir));
                         const float lpdf = env_pdf(env, dir);
-                        const float mis = mis_balance(I.pdf[i], lpdf);
-                        T = ld_col(I.Tx, I.Ty, I.Tz, I.Tw, i, ct);
+                        const float mis = mis_balance(I.pdf(i), lpdf);
+                        T = ld_col(I, kPfTx, i, ct);
                         L = add(L, mul(T, scale(envL, mis)));
                     }
                 } else if (bounce < depthLimit) {
@@ -850,8 +893,8 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                     h.t = t;
                     finish_hit(s, ray, tri, hb.x, hb.y, h);
                     Rng rng;
-                    rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws[i]);
-                    T = ld_col(I.Tx, I.Ty, I.Tz, I.Tw, i, ct);
+                    rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws(i));
+                    T = ld_col(I, kPfTx, i, ct);
                     const V3 pos = add(origin, scale(dir, t));
                     Ctx c;
                     c.n = h.normal;
@@ -860,12 +903,12 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
                     c.m = resolve_material_index(s, mi, s.materials != nullptr && tri < s.triangleCount &&
                                                             mi < s.materialCount, h.uvx, h.uvy);
                     c.w = lobe_weights(c.m);
-                    c.eta = I.eta[i];
+                    c.eta = I.eta(i);
                     c.flags = 0;
                     const C base = pol.base(c.m);
                     const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
                     if (lightMode == kLightMis && bounce > 0 && c.m.emission > 0.0f) {
-                        const float w = emission_mis_weight(s, tri, dir, t, I.pdf[i], (flags & kFlagReach) != 0u);
+                        const float w = emission_mis_weight(s, tri, dir, t, I.pdf(i), (flags & kFlagReach) != 0u);
                         L = add(L, mul(T, scale(pol.emission(c.m), w)));
                     } else {
                         L = add(L, mul(T, pol.emission(c.m)));
@@ -989,7 +1032,7 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
             }
         };
         if constexpr (BATCH) {
-            sid = live ? I.scene[i] : 0u;
+            sid = live ? I.scene(i) : 0u;
             if (live) {
                 const BatchScene* B = tab + sid;
                 body(B->scene, B->params, &B->env, B->maxDepth, static_cast<uint32_t>(B->pixelOffset));
@@ -1006,36 +1049,36 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
         if (next) {
             // the next bounce's record (the extension trace fills in its hit) and state
             outA[sh] = make_int4(sh, -1, -1, 0);
-            O.flags[sh] = newFlags;
-            O.pixel[sh] = pixel;
-            O.sample[sh] = sample;
-            O.seed[sh] = seed;
-            O.slot0[sh] = slot0;
+            O.flags(sh) = newFlags;
+            O.pixel(sh) = pixel;
+            O.sample(sh) = sample;
+            O.seed(sh) = seed;
+            O.slot0(sh) = slot0;
             if constexpr (BATCH) {
-                O.scene[sh] = sid;
+                O.scene(sh) = sid;
             }
             if constexpr (kSpectral) {
-                O.hero[sh] = hero;
+                O.hero(sh) = hero;
             }
-            st_col(O.Lx, O.Ly, O.Lz, O.Lw, sh, L);
+            st_col(O, kPfLx, sh, L);
             if (wantEnv) {
-                st_col(O.cex, O.cey, O.cez, O.cew, sh, ce);
+                st_col(O, kPfCex, sh, ce);
             }
             if (wantEmis) {
-                st_col(O.cmx, O.cmy, O.cmz, O.cmw, sh, cm);
+                st_col(O, kPfCmx, sh, cm);
             }
             if (wantExt) {
-                O.ox[sh] = nextO.x;
-                O.oy[sh] = nextO.y;
-                O.oz[sh] = nextO.z;
-                O.dx[sh] = nextD.x;
-                O.dy[sh] = nextD.y;
-                O.dz[sh] = nextD.z;
-                O.tmin[sh] = nextTmin;
-                O.eta[sh] = nextEta;
-                O.pdf[sh] = nextPdf;
-                O.draws[sh] = draws;
-                st_col(O.Tx, O.Ty, O.Tz, O.Tw, sh, T);
+                O.ox(sh) = nextO.x;
+                O.oy(sh) = nextO.y;
+                O.oz(sh) = nextO.z;
+                O.dx(sh) = nextD.x;
+                O.dy(sh) = nextD.y;
+                O.dz(sh) = nextD.z;
+                O.tmin(sh) = nextTmin;
+                O.eta(sh) = nextEta;
+                O.pdf(sh) = nextPdf;
+                O.draws(sh) = draws;
+                st_col(O, kPfTx, sh, T);
             }
         }
         if (wantEnv) {
@@ -1047,9 +1090,37 @@ __global__ __launch_bounds__(kWfBlock) PT_SHADE_ATTR void wf_shade_kernel(
     }
 }
 
+// The shade kernels. The RGB instantiation fits 128 VGPRs with one register spilled, so it is held
+// to 4 waves per SIMD (C1: shade 26.2 -> 23.7 ms per frame); the hero-wavelength one would spill
+// about 40 registers there and runs unbounded at 3 waves.
+#define PT_SHADE_PARAMS                                                                                          \
+    const MeshAccelSceneGpu *scene, const RenderParamsGpu *params, const EnvironmentMapGpu *env, PathState I,      \
+        PathState O, Paths P, int bounce, int maxDepth, const int4 *__restrict__ inA,                            \
+        const float2 *__restrict__ inB, const int *__restrict__ inCountPtr, int *extOut,                         \
+        int4 *__restrict__ outA, int *counters, float *capture, int captureSpp, SpecModel sm,                    \
+        const BatchScene *__restrict__ tab, int lightMode
+#define PT_SHADE_ARGS                                                                                            \
+    scene, params, env, I, O, P, bounce, maxDepth, inA, inB, inCountPtr, extOut, outA, counters, capture,        \
+        captureSpp, sm, tab, lightMode
+
+template <bool BATCH>
+__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(4))) void wf_shade_rgb_kernel(
+    PT_SHADE_PARAMS)
+{
+    shade_items<RgbColors, BATCH>(PT_SHADE_ARGS);
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(kWfBlock) void wf_shade_hero_kernel(PT_SHADE_PARAMS)
+{
+    shade_items<HeroColors, BATCH>(PT_SHADE_ARGS);
+}
+#undef PT_SHADE_PARAMS
+#undef PT_SHADE_ARGS
+
 // ----------------------------------------------------------------------------------------
 // Adjoint (path replay) of the fixed-spp render on the same schedule. Every path is the primal
-// path (same RNG use and decisions as wf_shade_kernel); per vertex the adjoint accumulates the
+// path (same RNG use and decisions as shade_items); per vertex the adjoint accumulates the
 // derivatives of that vertex's contributions with respect to its material: emission, the
 // next-event BSDF values and the BSDF value of the sampled continuation, which scales everything
 // the path gathers afterwards, R = C - (what the path has gathered so far). A next-event
@@ -1078,8 +1149,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_init_kernel(PathState S, 
     if (p >= *countPtr) {
         return;
     }
-    const uint32_t px = S.pixel[p] + (tab != nullptr ? static_cast<uint32_t>(tab[S.scene[p]].pixelOffset) : 0u);
-    const size_t q = 3 * (static_cast<size_t>(px) * spp + S.sample[p]);
+    const uint32_t px = S.pixel(p) + (tab != nullptr ? static_cast<uint32_t>(tab[S.scene(p)].pixelOffset) : 0u);
+    const size_t q = 3 * (static_cast<size_t>(px) * spp + S.sample(p));
     const V3 C = mk(radiance[q], radiance[q + 1], radiance[q + 2]);
     const V3 gs = mk(gradImg[3 * px] * invSpp, gradImg[3 * px + 1] * invSpp, gradImg[3 * px + 2] * invSpp);
     const V3 gw = clamp_vjp(C, gs);
@@ -1087,11 +1158,11 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_init_kernel(PathState S, 
     A.gy[p] = gw.y;
     A.gz[p] = gw.z;
     A.pmi[p] = -1;
-    S.Lx[p] = C.x;
-    S.Ly[p] = C.y;
-    S.Lz[p] = C.z;
+    S.Lx(p) = C.x;
+    S.Ly(p) = C.y;
+    S.Lz(p) = C.z;
     if (gw.x == 0.0f && gw.y == 0.0f && gw.z == 0.0f) {
-        S.flags[p] = 0u;  // nothing to differentiate: the path stops at its first shade
+        S.flags(p) = 0u;  // nothing to differentiate: the path stops at its first shade
     }
 }
 
@@ -1101,7 +1172,7 @@ PT_DEV V3 safe_ratio(V3 g, V3 x, V3 f)
               f.z != 0.0f ? g.z * x.z / f.z : 0.0f);
 }
 
-// BATCH as in wf_shade_kernel; a scene's material m sums into gradient row gradRow + m, and
+// BATCH as in shade_items; a scene's material m sums into gradient row gradRow + m, and
 // materialCount is the batch's row count.
 template <bool BATCH>
 __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
@@ -1146,29 +1217,29 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             const int rrMin = prm.russianRouletteMinDepth >= 0 ? prm.russianRouletteMinDepth : 3;
             const int4 ha = inA[i];
             const float2 hb = inB[i];
-            const uint32_t flags = I.flags[i];
-            pixel = I.pixel[i];
-            sample = I.sample[i];
-            seed = I.seed[i];
-            slot0 = I.slot0[i];
+            const uint32_t flags = I.flags(i);
+            pixel = I.pixel(i);
+            sample = I.sample(i);
+            seed = I.seed(i);
+            slot0 = I.slot0(i);
             gw = mk(AI.gx[i], AI.gy[i], AI.gz[i]);
-            R = mk(I.Lx[i], I.Ly[i], I.Lz[i]);
+            R = mk(I.Lx(i), I.Ly(i), I.Lz(i));
             // the previous vertex's next-event terms, in the forward's order (env, then emissive)
             const int pmi = AI.pmi[i];
             float prev[kGradParams];
             for (int k = 0; k < kGradParams; ++k) {
                 prev[k] = 0.0f;
             }
-            if ((flags & kFlagEnv) != 0u && I.occ[2 * i] == 0u) {
-                R = sub(R, mk(I.cex[i], I.cey[i], I.cez[i]));
+            if ((flags & kFlagEnv) != 0u && I.occ(i, 0) == 0u) {
+                R = sub(R, mk(I.cex(i), I.cey(i), I.cez(i)));
                 if (pmi >= 0) {
                     for (int k = 0; k < kGradParams; ++k) {
                         prev[k] += AI.pe[k][i];
                     }
                 }
             }
-            if ((flags & kFlagEmis) != 0u && I.occ[2 * i + 1] == 0u) {
-                R = sub(R, mk(I.cmx[i], I.cmy[i], I.cmz[i]));
+            if ((flags & kFlagEmis) != 0u && I.occ(i, 1) == 0u) {
+                R = sub(R, mk(I.cmx(i), I.cmy(i), I.cmz(i)));
                 if (pmi >= 0) {
                     for (int k = 0; k < kGradParams; ++k) {
                         prev[k] += AI.pm[k][i];
@@ -1180,8 +1251,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             }
             const uint32_t tri = static_cast<uint32_t>(ha.y);
             if ((flags & kFlagExt) != 0u && tri != 0xFFFFFFFFu && bounce < depthLimit) {
-                const V3 origin = mk(I.ox[i], I.oy[i], I.oz[i]);
-                const V3 dir = mk(I.dx[i], I.dy[i], I.dz[i]);
+                const V3 origin = mk(I.ox(i), I.oy(i), I.oz(i));
+                const V3 dir = mk(I.dx(i), I.dy(i), I.dz(i));
                 const float t = __int_as_float(ha.w);
                 const Ray ray = make_ray(origin, dir);
                 Hit h;
@@ -1189,8 +1260,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 h.t = t;
                 finish_hit(s, ray, tri, hb.x, hb.y, h);
                 Rng rng;
-                rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws[i]);
-                T = mk(I.Tx[i], I.Ty[i], I.Tz[i]);
+                rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws(i));
+                T = mk(I.Tx(i), I.Ty(i), I.Tz(i));
                 const V3 pos = add(origin, scale(dir, t));
                 Ctx c;
                 c.n = h.normal;
@@ -1199,7 +1270,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 const bool valid = s.materials != nullptr && tri < s.triangleCount && mi < s.materialCount;
                 c.m = resolve_material_index(s, mi, valid, h.uvx, h.uvy);
                 c.w = lobe_weights(c.m);
-                c.eta = I.eta[i];
+                c.eta = I.eta(i);
                 c.flags = 0;
                 int gmi = -1;
                 Live lm{false, false, false, false, false, false, false};
@@ -1338,7 +1409,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u);
         };
         if constexpr (BATCH) {
-            sid = live ? I.scene[i] : 0u;
+            sid = live ? I.scene(i) : 0u;
             if (live) {
                 const BatchScene* B = tab + sid;
                 body(B->scene, B->params, &B->env, B->maxDepth, B->gradRow);
@@ -1354,51 +1425,51 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
         }
         if (next) {
             outA[sh] = make_int4(sh, -1, -1, 0);
-            O.flags[sh] = newFlags;
-            O.pixel[sh] = pixel;
-            O.sample[sh] = sample;
-            O.seed[sh] = seed;
-            O.slot0[sh] = slot0;
+            O.flags(sh) = newFlags;
+            O.pixel(sh) = pixel;
+            O.sample(sh) = sample;
+            O.seed(sh) = seed;
+            O.slot0(sh) = slot0;
             if constexpr (BATCH) {
-                O.scene[sh] = sid;
+                O.scene(sh) = sid;
             }
-            O.Lx[sh] = R.x;
-            O.Ly[sh] = R.y;
-            O.Lz[sh] = R.z;
+            O.Lx(sh) = R.x;
+            O.Ly(sh) = R.y;
+            O.Lz(sh) = R.z;
             AO.gx[sh] = gw.x;
             AO.gy[sh] = gw.y;
             AO.gz[sh] = gw.z;
             AO.pmi[sh] = pendMat;
             if (wantEnv) {
-                O.cex[sh] = ce.x;
-                O.cey[sh] = ce.y;
-                O.cez[sh] = ce.z;
+                O.cex(sh) = ce.x;
+                O.cey(sh) = ce.y;
+                O.cez(sh) = ce.z;
                 for (int k = 0; k < kGradParams; ++k) {
                     AO.pe[k][sh] = e1[k];
                 }
             }
             if (wantEmis) {
-                O.cmx[sh] = cm.x;
-                O.cmy[sh] = cm.y;
-                O.cmz[sh] = cm.z;
+                O.cmx(sh) = cm.x;
+                O.cmy(sh) = cm.y;
+                O.cmz(sh) = cm.z;
                 for (int k = 0; k < kGradParams; ++k) {
                     AO.pm[k][sh] = e2[k];
                 }
             }
             if (wantExt) {
-                O.ox[sh] = nextO.x;
-                O.oy[sh] = nextO.y;
-                O.oz[sh] = nextO.z;
-                O.dx[sh] = nextD.x;
-                O.dy[sh] = nextD.y;
-                O.dz[sh] = nextD.z;
-                O.tmin[sh] = nextTmin;
-                O.eta[sh] = nextEta;
-                O.pdf[sh] = nextPdf;
-                O.draws[sh] = draws;
-                O.Tx[sh] = T.x;
-                O.Ty[sh] = T.y;
-                O.Tz[sh] = T.z;
+                O.ox(sh) = nextO.x;
+                O.oy(sh) = nextO.y;
+                O.oz(sh) = nextO.z;
+                O.dx(sh) = nextD.x;
+                O.dy(sh) = nextD.y;
+                O.dz(sh) = nextD.z;
+                O.tmin(sh) = nextTmin;
+                O.eta(sh) = nextEta;
+                O.pdf(sh) = nextPdf;
+                O.draws(sh) = draws;
+                O.Tx(sh) = T.x;
+                O.Ty(sh) = T.y;
+                O.Tz(sh) = T.z;
             }
         }
         if (wantEnv) {
@@ -1700,17 +1771,7 @@ size_t layout(Workspace& w, char* base, int cap, int pixels, int depth, int spil
     Paths& P = w.P;
     const size_t c = static_cast<size_t>(cap);
     for (PathState& S : P.S) {
-        float** fs[] = {&S.Tx,  &S.Ty,  &S.Tz,  &S.Tw,  &S.Lx,  &S.Ly,  &S.Lz,  &S.Lw,  &S.eta,
-                        &S.pdf, &S.ox,  &S.oy,  &S.oz,  &S.dx,  &S.dy,  &S.dz,  &S.tmin, &S.cex,
-                        &S.cey, &S.cez, &S.cew, &S.cmx, &S.cmy, &S.cmz, &S.cmw, &S.hero};
-        for (float** f : fs) {
-            *f = carve<float>(cur, c);
-        }
-        uint32_t** us[] = {&S.pixel, &S.sample, &S.seed, &S.draws, &S.flags, &S.slot0, &S.scene};
-        for (uint32_t** u : us) {
-            *u = carve<uint32_t>(cur, c);
-        }
-        S.occ = carve<uint8_t>(cur, 2 * c);
+        S.base = carve<char>(cur, (c + 63u) / 64u * kPathBlockBytes);
     }
     P.rx = carve<float>(cur, c);
     P.ry = carve<float>(cur, c);
@@ -1837,8 +1898,8 @@ void launch_bounces(hipStream_t stream, Paths& P, const MeshAccelSceneGpu* d_sce
                     bool spec, const SpecModel& sm, const BatchScene* tab, bool wide)
 {
     const int lightMode = pt_light_mode();
-    const auto kernel = tab != nullptr ? (spec ? wf_shade_kernel<HeroColors, true> : wf_shade_kernel<RgbColors, true>)
-                                       : (spec ? wf_shade_kernel<HeroColors, false> : wf_shade_kernel<RgbColors, false>);
+    const auto kernel = tab != nullptr ? (spec ? wf_shade_hero_kernel<true> : wf_shade_rgb_kernel<true>)
+                                       : (spec ? wf_shade_hero_kernel<false> : wf_shade_rgb_kernel<false>);
     launch_bounces_with(
         stream, P, d_scene, maxPathDepth, maxPaths, persistentBlocks,
         [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
