@@ -954,20 +954,23 @@ PT_DEV void shade_items(const MeshAccelSceneGpu* scene, const RenderParamsGpu* p
                                 if (cosT > 0.0f && cosL > 0.0f) {
                                     const float lightPdf = areaPdf * d2 / cosL;
                                     if (lightPdf > kMinPdf) {
-                                        const C f = brdf_eval_t(c, wi, base, rad);
-                                        const float bpdf = brdf_pdf(c, wi);
-                                        const float mis = mis_balance(lightPdf, bpdf);
-                                        const float sc = mis * cosT / lightPdf;
-                                        cm = mul(T, mul(f, scale(pol.radiance(lr), sc)));
                                         emO = add(pos, scale(wi, kShadowBias));
                                         // kLightMis: the shadow ray runs to the light point (the
                                         // trace kernel bounds it); otherwise along wi to kRayTMax,
                                         // where the sampled triangle itself nearly always stops
                                         // it: then no ray is queued, which the next bounce reads
-                                        // exactly as a traced ray found occluded
+                                        // exactly as a traced ray found occluded, and the
+                                        // contribution it would gate is not evaluated
                                         emD = lightMode == kLightMis ? lp : wi;
                                         wantEmis = lightMode == kLightMis ||
                                                    !emissive_blocks(s, entry, make_ray(emO, wi), kShadowBias, kRayTMax);
+                                        if (wantEmis) {
+                                            const C f = brdf_eval_t(c, wi, base, rad);
+                                            const float bpdf = brdf_pdf(c, wi);
+                                            const float mis = mis_balance(lightPdf, bpdf);
+                                            const float sc = mis * cosT / lightPdf;
+                                            cm = mul(T, mul(f, scale(pol.radiance(lr), sc)));
+                                        }
                                     }
                                 }
                             }
