@@ -169,9 +169,10 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
     samples = w * h * spp
     print(f"[bench] C2 forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms ({samples / t_adj / 1e6:.1f} "
           f"grad-Msamples/s)", file=sys.stderr, flush=True)
+    # central differences of the FP32 detached estimator on the GPU: the float rounding of the two
+    # renders a difference divides by 2h dominates small gradients at h = 1e-3; h = 1e-2 trades
+    # it for truncation error. The FP64 differences on a window below carry neither.
     res = diff.gradient_check(r, theta0.cuda(), target, h=1e-3)
-    # the float renders' rounding noise dominates small-gradient differences at h = 1e-3; a wider
-    # step shows the agreement once that noise is below the truncation error
     res2 = diff.gradient_check(r, theta0.cuda(), target, h=1e-2)
     print(f"[bench] C2 gradient vs central differences: h=1e-3 max rel err {res['max_rel_err']:.2e} (median "
           f"{res['median_rel_err']:.2e}), h=1e-2 max {res2['max_rel_err']:.2e} (median {res2['median_rel_err']:.2e}) "
@@ -190,6 +191,48 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
     g_cpu = g_cpu.reshape(-1, 7)
     live = res["live"] & (g_cpu != 0.0)
     rel_cpu = np.abs(g_gpu - g_cpu)[live] / np.maximum(np.abs(g_cpu[live]), 1e-3 * np.abs(g_cpu).max())
+    # central differences (h = 1e-4) of the detached estimator evaluated in double on the same
+    # window, for the same linear loss: the derivative without float rounding in the difference
+    t0 = time.perf_counter()
+    g_fd = np.zeros_like(g_cpu)
+    base_mats = np.ascontiguousarray(r.base.copy())
+    words = list(diff.PARAM_WORDS)
+    win_np = np.ascontiguousarray(gi)
+
+    def window_loss(mats):
+        im = np.zeros((w * h, 3))
+        O.lib().orc_render_detached_f64_seeded(O.ptr(im), w, h, O.ptr(spec.camera), C.byref(ref.scene),
+                                               O.ptr(mats), C.byref(O.Env()), O.ptr(r.params_host), spp, 1,
+                                               O.ptr(win_np), cpu_threads)
+        return float((win_np * im).sum())
+
+    for i, k in zip(*np.nonzero(res["live"])):
+        up, dn = base_mats.copy(), base_mats.copy()
+        up[i, words[k]] += np.float32(1e-4)
+        dn[i, words[k]] -= np.float32(1e-4)
+        g_fd[i, k] = (window_loss(up) - window_loss(dn)) / (float(up[i, words[k]]) - float(dn[i, words[k]]))
+    t_fd64 = time.perf_counter() - t0
+    live64 = res["live"] & (g_fd != 0.0)
+    fd64_floor = 1e-3 * np.abs(g_fd).max()
+    rel_fd64 = np.zeros_like(g_fd)
+    rel_fd64[live64] = np.abs(g_gpu - g_fd)[live64] / np.maximum(np.abs(g_fd[live64]), fd64_floor)
+    print(f"[bench] C2 GPU adjoint vs FP64 central differences of the detached estimator (32x32 window, h=1e-4): "
+          f"max rel err {rel_fd64[live64].max():.2e}, median {np.median(rel_fd64[live64]):.2e} over "
+          f"{int(live64.sum())} parameters ({t_fd64:.1f} s)", file=sys.stderr, flush=True)
+    rel_cpu_all = np.zeros_like(g_cpu)
+    rel_cpu_all[live] = rel_cpu
+    groups = ["bark", "leaf", "ground", "light 1", "light 2"]
+    per_param = []
+    for i, k in zip(*np.nonzero(res["live"])):
+        per_param.append({"group": groups[i] if i < len(groups) else f"material {i}", "param": diff.PARAM_NAMES[k],
+                          "adjoint": float(res["adjoint"][i, k]), "fd_h1e-3": float(res["fd"][i, k]),
+                          "fd_rel_err": float(res["rel_err"][i, k]), "fd_h1e-2": float(res2["fd"][i, k]),
+                          "fd_h1e-2_rel_err": float(res2["rel_err"][i, k]),
+                          "fp64_window_rel_err": float(rel_cpu_all[i, k]) if live[i, k] else None,
+                          "fp64_fd_window": float(g_fd[i, k]), "fp64_window_adjoint": float(g_gpu[i, k]),
+                          "fp64_fd_window_rel_err": float(rel_fd64[i, k]) if live64[i, k] else None})
+    print(f"[bench] C2 GPU adjoint vs the FP64 oracle adjoint (same window): max rel err "
+          f"{float(rel_cpu.max()) if rel_cpu.size else float('nan'):.2e}", file=sys.stderr, flush=True)
     # 16-bin spectral parameters: base colour (surfaces) and emission (lights) as binned spectra
     # projected to linear sRGB; gradients through the projection against central differences
     from lsystemeditor_amd import binned
@@ -218,6 +261,14 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
             "fd_max_rel_err": res["max_rel_err"], "fd_median_rel_err": res["median_rel_err"],
             "fd_step_1e-2_max_rel_err": res2["max_rel_err"], "fd_step_1e-2_median_rel_err": res2["median_rel_err"],
             "fp64_cpu_window": "32x32", "fp64_cpu_max_rel_err": float(rel_cpu.max()) if rel_cpu.size else None,
+            "fp64_fd_window_max_rel_err": float(rel_fd64[live64].max()),
+            "fp64_fd_window_median_rel_err": float(np.median(rel_fd64[live64])),
+            "fp64_fd_window_note": "GPU adjoint of the 32x32-window loss against central differences (h = 1e-4) of "
+                                   "the detached estimator evaluated in double (orc_render_detached_f64_seeded)",
+            "per_parameter": per_param,
+            "per_parameter_note": "adjoint, fd_h1e-3, fd_h1e-2: full-frame L2 loss, FP32 detached estimator on the "
+                                  "GPU; fp64_*: the 32x32-window linear loss, GPU adjoint vs FP64 oracle adjoint and "
+                                  "vs FP64 central differences (h = 1e-4); rel errors floored at 1e-3 of the largest",
             "binned_spectral": binned_out}
 
 

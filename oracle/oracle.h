@@ -252,6 +252,8 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* scene, const OMaterial
 /* Contracted derivative of the opaque Principled BSDF value: out6[k] += sum_c a3[c] df_c/dtheta_k,
  * theta = (r, g, b, roughness, metallic, specular); raw = the unresolved material (live mask). */
 void orc_brdf_grad(const OBrdfCtx* ctx, OVec3 wi, const double* a3, const OMaterial* raw, double* out6);
+/* The value orc_brdf_grad differentiates, in double (f3 = 3 doubles). */
+void orc_brdf_eval_f64(const OBrdfCtx* ctx, OVec3 wi, const OMaterial* raw, double* f3);
 void orc_clamp_vjp(OVec3 C, const double* g3, double* out3);
 void orc_render_detached(float* img3, int width, int height, const OCamera* cam, const OScene* scene,
                          const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp, int threads);
@@ -264,6 +266,10 @@ void orc_render_detached_seeded(float* img3, int width, int height, const OCamer
 void orc_render_adjoint_seeded(const double* gradImg3, int width, int height, const OCamera* cam,
                                const OScene* scene, const OEnv* env, const OParams* params, int spp,
                                uint32_t firstSeed, double* grad, int threads);
+/* The detached estimator evaluated in double, added into img3; mask3 (or NULL) selects pixels. */
+void orc_render_detached_f64_seeded(double* img3, int width, int height, const OCamera* cam, const OScene* scene,
+                                    const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp,
+                                    uint32_t firstSeed, const double* mask3, int threads);
 
 /* ---- spectral (Spectral/Rgb2Spec.h, SpectralCore.h, SpectralState.cpp) ----------------------- */
 typedef struct {

@@ -161,6 +161,9 @@ def lib(variant: str = "parity"):
             "orc_trace_path": (Vec3, [Vec3, Vec3, C.POINTER(Scene), vp, C.POINTER(Env), vp]),
             "orc_brdf_grad": (None, [vp, Vec3, vp, vp, vp]),
             "orc_clamp_vjp": (None, [Vec3, vp, vp]),
+            "orc_brdf_eval_f64": (None, [vp, Vec3, vp, vp]),
+            "orc_render_detached_f64_seeded": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, C.POINTER(Env),
+                                                      vp, C.c_int, C.c_uint32, vp, C.c_int]),
             "orc_render_detached": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, C.POINTER(Env), vp, C.c_int,
                                            C.c_int]),
             "orc_render_adjoint": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), C.POINTER(Env), vp, C.c_int, vp,

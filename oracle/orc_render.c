@@ -1202,6 +1202,95 @@ static void brdf_grad(const OBrdfCtx* c, OVec3 wi, const double a[3], LiveMask l
     }
 }
 
+/* The value of the BSDF brdf_grad differentiates, in double: the same float geometry (cosines,
+ * half vector, Oren-Nayar angles) with the parameters' algebra in double, so central differences
+ * of it in a parameter carry no float rounding. Materials with transmission or subsurface weight
+ * (whose BSDF values brdf_grad leaves undifferentiated) use the float evaluation. */
+static void brdf_value_f64(const OBrdfCtx* c, OVec3 wi, LiveMask lm, double f[3])
+{
+    f[0] = f[1] = f[2] = 0.0;
+    if (!lm.opaque) {
+        const OVec3 v = orc_brdf_eval(c, wi);
+        f[0] = v.x;
+        f[1] = v.y;
+        f[2] = v.z;
+        return;
+    }
+    const OMaterial* m = &c->material;
+    const float cosWi = fmax2(0.0f, vdot(c->normal, wi));
+    const float cosWo = fmax2(0.0f, vdot(c->normal, c->wo));
+    if (cosWi <= 0.0f || cosWo <= 0.0f) {
+        return;
+    }
+    const double pi = (double)3.14159265f;
+    const double base[3] = {m->r, m->g, m->b};
+    const double mm = clamp01f(m->metallic);
+    const double wd = (1.0 - mm) / (2.0 - mm);
+    const double ws = 1.0 / (2.0 - mm);
+    /* diffuse lobe: Oren-Nayar A(s2) + B(s2) geo, sigma = diffuse roughness * pi / 2 */
+    const float dr = fmax2(0.0f, fmin2(1.0f, m->diffuseRoughness < 0.0f ? m->roughness : m->diffuseRoughness));
+    double on = 1.0;
+    if (dr * 0.5f * 3.14159265f > 1.0e-4f) {
+        const double sigma = (double)dr * 0.5 * pi;
+        const double s2 = sigma * sigma;
+        const float sinI = sqrtf(fmax2(0.0f, 1.0f - cosWi * cosWi));
+        const float sinO = sqrtf(fmax2(0.0f, 1.0f - cosWo * cosWo));
+        float sa = cosWi > cosWo ? sinI : sinO;
+        float sb = cosWi > cosWo ? sinO : sinI;
+        float cosPhi = 1.0f;
+        const float d = vdot(wi, c->wo) - cosWi * cosWo;
+        if (d <= 0.0f) {
+            cosPhi = fmax2(-1.0f, fmin2(1.0f, d / fmax2(sa * sb, 1.0e-8f)));
+        } else {
+            sa = 0.0f;
+            sb = 0.0f;
+        }
+        const double geo = (double)fmax2(0.0f, cosPhi) * sa * (sb / fmax2(cosWi, cosWo));
+        on = 1.0 - 0.5 * s2 / (s2 + 0.33) + 0.45 * s2 / (s2 + 0.09) * geo;
+    }
+    for (int k = 0; k < 3; ++k) {
+        f[k] = base[k] * wd * on / pi;
+    }
+    /* specular lobe: F ws ec D G / max(4 cosO cosI, 1e-8) */
+    const OVec3 h = vnorm(vadd(wi, c->wo));
+    const float cosH = fmax2(0.0f, vdot(c->normal, h));
+    const float cosHo = fmax2(0.0f, vdot(h, c->wo));
+    if (cosHo <= 0.0f) {
+        return;
+    }
+    const double rc = fmax((double)m->roughness, 0.04);
+    const double alpha = rc * rc;
+    const double a2 = alpha * alpha;
+    const double c2 = (double)cosH * cosH;
+    const double dd = c2 * (a2 - 1.0) + 1.0;
+    const double D = a2 / (pi * dd * dd);
+    double G = 1.0;
+    const float cs[2] = {cosWo, cosWi};
+    for (int q = 0; q < 2; ++q) {
+        const double cc = fmax((double)cs[q] * cs[q], 1.0e-8);
+        G *= 2.0 / (1.0 + sqrt(1.0 + a2 * (1.0 - cc) / cc));
+    }
+    const double ior = fmax2(m->ior, 1.0e-3f);
+    const double f0s = (ior - 1.0) / (ior + 1.0);
+    const double f0v = f0s * f0s * clamp01f(clamp01f(m->specular));
+    const double t = fmax(0.0, fmin(1.0, 1.0 - (double)cosHo));
+    const double t5 = t * t * t * t * t;
+    const double cw = fmax(cosWo, 1.0e-4);
+    const double ess = 1.0 - alpha * (0.3316 + 0.3237 * alpha);
+    const double ec = 1.0 + (1.0 - ess) * (1.0 - cw) / fmax(ess, 1.0e-4);
+    const double S = ws * ec * D * G / fmax(4.0 * cosWo * cosWi, 1.0e-8);
+    for (int k = 0; k < 3; ++k) {
+        f[k] += ((f0v + (base[k] - f0v) * mm) * (1.0 - t5) + t5) * S;
+    }
+}
+
+void orc_brdf_eval_f64(const OBrdfCtx* c, OVec3 wi, const OMaterial* raw, double* f3)
+{
+    OMaterial none = *raw;
+    none.albedoTex = none.roughnessTex = none.metallicTex = none.emissionTex = 0u;
+    brdf_value_f64(c, wi, live_mask(&none, &c->material, NULL), f3);
+}
+
 void orc_brdf_grad(const OBrdfCtx* c, OVec3 wi, const double* a3, const OMaterial* raw, double* out6)
 {
     OMaterial none = *raw;
@@ -1287,12 +1376,31 @@ static OMaterial fetch_from(const OScene* s, const OMaterial* table, uint32_t tr
     return orc_fetch_material(s, tri);
 }
 
-OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* evalMats, const OParams* params,
-                        const OEnv* env, ORng* rng, const double* gw, OVec3 C, double* grad)
+/* Ld += Td * f(wi) * w with the BSDF value in double. */
+static void add_term_f64(double* Ld, const double Td[3], const OBrdfCtx* ctxE, OVec3 wi, LiveMask lm, OVec3 w)
+{
+    double fd[3];
+    brdf_value_f64(ctxE, wi, lm, fd);
+    Ld[0] += Td[0] * fd[0] * w.x;
+    Ld[1] += Td[1] * fd[1] * w.y;
+    Ld[2] += Td[2] * fd[2] * w.z;
+}
+
+/* With Ld != NULL the radiance is also evaluated in double (Ld, 3 doubles): the same sampling,
+ * pdfs, MIS weights and roulette factors (float, parameter-free), with emission, BSDF values and
+ * the throughput they build in double. */
+static OVec3 trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* evalMats, const OParams* params,
+                           const OEnv* env, ORng* rng, const double* gw, OVec3 C, double* grad, double* Ld)
 {
     const OHit first = trace(ro, rd, s, 0.0f, K_RAY_TMAX, 1);
     if (!first.hit) {
-        return orc_env_eval(env, params, rd);
+        const OVec3 e = orc_env_eval(env, params, rd);
+        if (Ld != NULL) {
+            Ld[0] = e.x;
+            Ld[1] = e.y;
+            Ld[2] = e.z;
+        }
+        return e;
     }
     int maxDepth = 32, rrMin = 3, flags = 0;
     if (params != NULL) {
@@ -1307,6 +1415,10 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
     OVec3 origin = ro, dir = rd;
     OHit hit = first;
     float eta = 1.0f;
+    double Td[3] = {1.0, 1.0, 1.0};
+    if (Ld != NULL) {
+        Ld[0] = Ld[1] = Ld[2] = 0.0;
+    }
     for (int depth = 0; depth < maxDepth; ++depth) {
         const OVec3 p = vray(origin, dir, hit.t);
         const OVec3 n = hit.normal;
@@ -1321,6 +1433,12 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
         double* g = (grad != NULL && gw != NULL && mi >= 0) ? grad + 7 * mi : NULL;
 
         L = vadd(L, vmul(T, v3(mE.r * mE.emission, mE.g * mE.emission, mE.b * mE.emission)));
+        if (Ld != NULL) {
+            const double base[3] = {mE.r, mE.g, mE.b};
+            for (int k = 0; k < 3; ++k) {
+                Ld[k] += Td[k] * base[k] * (double)mE.emission;
+            }
+        }
         if (g != NULL) {
             const double tb[3] = {(double)T.x * mE.r, (double)T.y * mE.g, (double)T.z * mE.b};
             if (lm.emis) {
@@ -1335,6 +1453,9 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
         OVec3 wi, w;
         if (env_nee_terms(p, n, wo, &m, eta, s, env, params, rng, &wi, &w)) {
             L = vadd(L, vmul(T, vmul(orc_brdf_eval(&ctxE, wi), w)));
+            if (Ld != NULL) {
+                add_term_f64(Ld, Td, &ctxE, wi, lm, w);
+            }
             if (g != NULL) {
                 const double a[3] = {gw[0] * T.x * w.x, gw[1] * T.y * w.y, gw[2] * T.z * w.z};
                 brdf_grad(&ctxE, wi, a, lm, g);
@@ -1342,6 +1463,9 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
         }
         if (emissive_nee_terms(p, n, wo, &m, eta, s, rng, &wi, &w)) {
             L = vadd(L, vmul(T, vmul(orc_brdf_eval(&ctxE, wi), w)));
+            if (Ld != NULL) {
+                add_term_f64(Ld, Td, &ctxE, wi, lm, w);
+            }
             if (g != NULL) {
                 const double a[3] = {gw[0] * T.x * w.x, gw[1] * T.y * w.y, gw[2] * T.z * w.z};
                 brdf_grad(&ctxE, wi, a, lm, g);
@@ -1368,6 +1492,16 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
         }
         const OVec3 T0b = T0;
         T0 = orc_apply_throughput(T0, &ctx, &smp, f0);
+        const OVec3 T0mid = T0;
+        if (Ld != NULL) {
+            /* the factor the throughput update multiplies the BSDF value by (cosine / pdf) */
+            double fd[3];
+            brdf_value_f64(&ctxE, smp.direction, lm, fd);
+            const OVec3 sc = orc_apply_throughput(v3(1.0f, 1.0f, 1.0f), &ctx, &smp, v3(1.0f, 1.0f, 1.0f));
+            Td[0] *= fd[0] * sc.x;
+            Td[1] *= fd[1] * sc.y;
+            Td[2] *= fd[2] * sc.z;
+        }
         if (evalMats == NULL) {
             T = T0;
             T0 = subsurface_tier_b(T0, &m, n, &smp, rng);
@@ -1381,10 +1515,21 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
             T = vmul(T, k3);
         }
         (void)T0b;
+        if (Ld != NULL) {
+            /* the tier-B walk's factor on the primal throughput */
+            Td[0] *= T0mid.x != 0.0f ? (double)T0.x / T0mid.x : 0.0;
+            Td[1] *= T0mid.y != 0.0f ? (double)T0.y / T0mid.y : 0.0;
+            Td[2] *= T0mid.z != 0.0f ? (double)T0.z / T0mid.z : 0.0;
+        }
         if ((flags & (1 << 4)) != 0) {
             const OVec3 tint = smp.transmitted ? v3(0.2f, 1.0f, 0.2f) : v3(1.0f, 0.2f, 0.2f);
             T0 = vmul(T0, tint);
             T = vmul(T, tint);
+            if (Ld != NULL) {
+                Td[0] *= tint.x;
+                Td[1] *= tint.y;
+                Td[2] *= tint.z;
+            }
         }
         const float tl = lum3(T0);
         if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
@@ -1397,6 +1542,11 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
             }
             T0 = vscale(T0, 1.0f / surv);
             T = vscale(T, 1.0f / surv);
+            if (Ld != NULL) {
+                for (int k = 0; k < 3; ++k) {
+                    Td[k] *= (double)(1.0f / surv);
+                }
+            }
         }
         eta = smp.nextMediumEta;
         const float bias = fmax2(K_SHADOW_BIAS, K_REL_SHADOW_BIAS * hit.t);
@@ -1409,10 +1559,40 @@ OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* ev
             const float lpdf = orc_env_pdf(env, params, dir);
             const float mis = orc_mis_balance(smp.pdf, lpdf);
             L = vadd(L, vmul(T, vscale(envL, mis)));
+            if (Ld != NULL) {
+                const OVec3 e = vscale(envL, mis);
+                Ld[0] += Td[0] * e.x;
+                Ld[1] += Td[1] * e.y;
+                Ld[2] += Td[2] * e.z;
+            }
             break;
         }
     }
     return L;
+}
+
+OVec3 orc_trace_path_ad(OVec3 ro, OVec3 rd, const OScene* s, const OMaterial* evalMats, const OParams* params,
+                        const OEnv* env, ORng* rng, const double* gw, OVec3 C, double* grad)
+{
+    return trace_path_ad(ro, rd, s, evalMats, params, env, rng, gw, C, grad, NULL);
+}
+
+/* The firefly clamp (PathTracer.cu:49-67) in double. */
+static void clamp_firefly_f64(const double r[3], double out[3])
+{
+    if (!isfinite(r[0]) || !isfinite(r[1]) || !isfinite(r[2])) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    for (int k = 0; k < 3; ++k) {
+        out[k] = fmax(0.0, r[k]);
+    }
+    const double lum = out[0] * 0.2126 + out[1] * 0.7152 + out[2] * 0.0722;
+    if (lum > 100.0) {
+        for (int k = 0; k < 3; ++k) {
+            out[k] *= 100.0 / lum;
+        }
+    }
 }
 
 /* d(clamped radiance)/d(radiance)^T g for the firefly clamp (PathTracer.cu:49-67). */
@@ -1441,6 +1621,8 @@ void orc_clamp_vjp(OVec3 C, const double* g, double* out)
 
 typedef struct {
     float* img;
+    double* imgD;        /* FP64 detached image (orc_render_detached_f64_seeded) */
+    const double* mask;  /* with imgD: pixels whose three mask values are 0 are skipped */
     const double* gradImg;
     double* grad;
     int w, h, spp, y0, y1, gradCount;
@@ -1469,7 +1651,19 @@ static void ad_rows(const AdJob* j)
                 const float l2 = orc_rand01(&rng);
                 OVec3 ro, rd;
                 orc_camera_ray(j->cam, u, v, l1, l2, &ro, &rd);
-                if (j->gradImg == NULL) {
+                if (j->imgD != NULL) {
+                    if (j->mask != NULL && j->mask[3 * idx] == 0.0 && j->mask[3 * idx + 1] == 0.0 &&
+                        j->mask[3 * idx + 2] == 0.0) {
+                        break;
+                    }
+                    double Ld[3], c[3];
+                    (void)trace_path_ad(ro, rd, j->s, j->evalMats, j->params, j->env, &rng, NULL, v3(0, 0, 0), NULL,
+                                        Ld);
+                    clamp_firefly_f64(Ld, c);
+                    for (int q = 0; q < 3; ++q) {
+                        j->imgD[3 * idx + q] += c[q] / j->spp;
+                    }
+                } else if (j->gradImg == NULL) {
                     const OVec3 c = orc_clamp_firefly(
                         orc_trace_path_ad(ro, rd, j->s, j->evalMats, j->params, j->env, &rng, NULL, v3(0, 0, 0), NULL));
                     const float inv = 1.0f / (float)(k + 1);
@@ -1559,6 +1753,31 @@ void orc_render_detached_seeded(float* img3, int width, int height, const OCamer
     memset(&j, 0, sizeof(j));
     j.firstSeed = firstSeed;
     j.img = img3;
+    j.w = width;
+    j.h = height;
+    j.spp = spp;
+    j.cam = cam;
+    j.s = s;
+    j.evalMats = evalMats;
+    j.env = env;
+    j.params = params;
+    ad_run(j, threads, NULL);
+}
+
+/* The same estimator evaluated in double (trace_path_ad with Ld): the per-pixel mean of the
+ * clamped radiance over seeds firstSeed .. firstSeed + spp - 1, added into img3 (3 doubles per
+ * pixel, zero it first). mask3 != NULL restricts it to the pixels with a nonzero mask value. Its
+ * central differences in a parameter are those of the detached estimator without float rounding,
+ * so they pin orc_render_adjoint_seeded's derivative directly. */
+void orc_render_detached_f64_seeded(double* img3, int width, int height, const OCamera* cam, const OScene* s,
+                                    const OMaterial* evalMats, const OEnv* env, const OParams* params, int spp,
+                                    uint32_t firstSeed, const double* mask3, int threads)
+{
+    AdJob j;
+    memset(&j, 0, sizeof(j));
+    j.firstSeed = firstSeed;
+    j.imgD = img3;
+    j.mask = mask3;
     j.w = width;
     j.h = height;
     j.spp = spp;

@@ -9,8 +9,10 @@
   launcher (pathTracerAdaptiveSample, compaction every 4 passes); size-independent invariants of
   the adaptive state hold on every pixel.
 - C2 (C1 bush at 512^2, 32 spp, depth 8, five material groups): adjoint of an L2 loss restricted
-  to a 32x32 window against the oracle's FP64 adjoint (1e-4 relative), and of the full-image loss
-  against central differences of the detached estimator.
+  to a 32x32 window against the oracle's FP64 adjoint (1e-4 relative) and against central
+  differences (h = 1e-4) of the detached estimator evaluated in double (1e-5 relative, all 32
+  parameters), and of the full-image loss against central differences of the FP32 detached
+  estimator on the GPU.
 - C3 (64 bushes, seeds 0-63, iterations 3-7, 128^2-1024^2, 64 spp, depth 8): the whole batch in
   one graph-replayed batch wavefront; scenes of every resolution (the largest scene included)
   bit-equal to their own fixed-spp renders; a 32x32 window of a 1024^2 scene bit-equal to the
@@ -147,6 +149,29 @@ def test_c2_adjoint_matches_fp64_oracle_and_differences(cuda, c1):
     print(f"C2 window adjoint vs FP64 oracle: max rel err {rel[g_cpu != 0].max():.3g}", flush=True)
     assert np.all(err <= 1e-4 * np.abs(g_cpu) + 1e-6 * scale), f"max rel err {rel[g_cpu != 0].max():.3g}"
     assert (g_cpu[:, :3] != 0).sum() >= 9, "the window sees every surface's base colour"
+    # the window loss: the GPU adjoint against central differences (h = 1e-4) of the detached
+    # estimator evaluated in double, which carry no float rounding (all 32 live parameters)
+    base = np.ascontiguousarray(r.base.copy())
+    words = list(diff.PARAM_WORDS)
+
+    def window_loss(mats):
+        im = np.zeros((w * h, 3))
+        O.lib().orc_render_detached_f64_seeded(O.ptr(im), w, h, O.ptr(spec.camera), C.byref(ref.scene),
+                                               O.ptr(mats), C.byref(O.Env()), O.ptr(r.params_host), spp, 1,
+                                               O.ptr(gi), THREADS)
+        return float((gi * im).sum())
+
+    live = r.live_mask()
+    fd64 = np.zeros_like(g_cpu)
+    for i, k in zip(*np.nonzero(live)):
+        up, dn = base.copy(), base.copy()
+        up[i, words[k]] += np.float32(1e-4)
+        dn[i, words[k]] -= np.float32(1e-4)
+        fd64[i, k] = (window_loss(up) - window_loss(dn)) / (float(up[i, words[k]]) - float(dn[i, words[k]]))
+    rel64 = np.abs(g_gpu - fd64)[live] / np.maximum(np.abs(fd64[live]), 1e-3 * np.abs(fd64).max())
+    print(f"C2 window adjoint vs FP64 central differences: max rel err {rel64.max():.3g} over {int(live.sum())} "
+          f"parameters", flush=True)
+    assert int(live.sum()) == 32 and rel64.max() < 1e-5, rel64.max()
     # the full-image loss: adjoint against central differences of the detached estimator
     res = diff.gradient_check(r, theta0.cuda(), target, h=1e-2)
     print(f"C2 adjoint vs central differences (h=1e-2): median {res['median_rel_err']:.3g}, "

@@ -3,7 +3,9 @@ forward integrator at the primal parameters (bit for bit), the analytic BSDF der
 central differences of orc_brdf_eval, the firefly-clamp VJP matches differences of
 orc_clamp_firefly, and the FP64 adjoint gradient of an image loss matches central finite
 differences of the detached estimator (h = 1e-3) on a small scene, for the default sample set
-(seeds 1..spp) and another one."""
+(seeds 1..spp) and another one. The same checks without float rounding: the detached estimator
+and the BSDF value evaluated in double equal their float forms, and the adjoint and
+orc_brdf_grad match central differences of them to 1e-5."""
 import ctypes as C
 
 import numpy as np
@@ -157,5 +159,125 @@ def test_adjoint_matches_finite_differences_of_the_detached_estimator(first_seed
                 losses.append(float(((im.astype(np.float64) - target) ** 2).sum()))
             fd = (losses[0] - losses[1]) / (2 * hstep)
             assert abs(grad[mi, k] - fd) <= 2e-3 * abs(fd) + 2e-4 * scale, (mi, name, grad[mi, k], fd)
+            checked += 1
+    assert checked >= 20
+
+
+def _word_steps(words, name, hstep):
+    """Parameter tables at theta0 +- hstep (stored in float32) and the exact step between them."""
+    up, dn = words.copy(), words.copy()
+    up[..., WORD[name]] += np.float32(hstep)
+    dn[..., WORD[name]] -= np.float32(hstep)
+    return up, dn
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fp64_brdf_value_is_the_float_value_and_brdf_grad_its_derivative(seed):
+    """orc_brdf_eval_f64 equals the forward's float BSDF value, and orc_brdf_grad equals central
+    differences of it at h = 1e-5, where double rounding leaves the quotient exact to ~1e-10. The
+    bar is 1e-5 of the derivative plus 1e-7 of the value: orc_brdf_grad takes the Oren-Nayar factor
+    from the float evaluation, which carries 1e-7 of the value into derivatives (metallic's) that
+    are small differences of value-sized terms."""
+    rng = np.random.default_rng(100 + seed)
+    L = O.lib()
+    words = abi.pack_material(*rng.uniform(0.2, 0.8, 3), roughness=rng.uniform(0.1, 0.9),
+                              metallic=rng.uniform(0.05, 0.9), specular=rng.uniform(0.1, 0.9))
+
+    def value(ww, wo, wi):
+        f = np.zeros(3)
+        L.orc_brdf_eval_f64(C.byref(O.ctx(O.Material.from_buffer_copy(ww.tobytes()), wo=wo)), O.v3(wi), O.ptr(ww),
+                            O.ptr(f))
+        return f
+
+    worst_val = 0.0
+    for trial in range(40):
+        wo = rng.normal(size=3)
+        wo[2] = abs(wo[2]) + 0.05
+        wi = rng.normal(size=3)
+        wi[2] = abs(wi[2]) + 0.05
+        wo /= np.linalg.norm(wo)
+        wi /= np.linalg.norm(wi)
+        f64 = value(words, wo, wi)
+        f32 = L.orc_brdf_eval(C.byref(O.ctx(O.Material.from_buffer_copy(words.tobytes()), wo=wo)), O.v3(wi))
+        f32 = np.array([f32.x, f32.y, f32.z])
+        worst_val = max(worst_val, float(np.max(np.abs(f64 - f32) / np.maximum(np.abs(f64), 1e-12))))
+        a = rng.uniform(0.5, 1.5, 3)
+        grad = np.zeros(6)
+        L.orc_brdf_grad(C.byref(O.ctx(O.Material.from_buffer_copy(words.tobytes()), wo=wo)), O.v3(wi),
+                        O.ptr(np.ascontiguousarray(a)), O.ptr(words), O.ptr(grad))
+        for k, name in enumerate(PARAMS[:6]):
+            up, dn = _word_steps(words, name, 1e-5)
+            step = float(up[WORD[name]]) - float(dn[WORD[name]])
+            fd = float(a @ (value(up, wo, wi) - value(dn, wo, wi))) / step
+            assert abs(grad[k] - fd) <= 1e-5 * abs(fd) + 1e-7 * abs(float(a @ f64)), (trial, name, grad[k], fd)
+    assert worst_val <= 2e-5, worst_val
+
+
+def render_f64(sc, cam, params, w, h, spp, eval_mats=None, first_seed=1, mask=None):
+    img = np.zeros((w * h, 3))
+    O.lib().orc_render_detached_f64_seeded(O.ptr(img), w, h, O.ptr(cam), C.byref(sc.scene),
+                                           None if eval_mats is None else O.ptr(eval_mats), C.byref(O.Env()),
+                                           O.ptr(params), spp, first_seed,
+                                           None if mask is None else O.ptr(np.ascontiguousarray(mask)), 8)
+    return img
+
+
+def test_fp64_detached_estimator_is_the_float_estimator():
+    """The double evaluation is the same estimator: equal to the float image to float rounding,
+    at the primal parameters and at perturbed ones; a mask skips the other pixels."""
+    w = h = 12
+    spp = 4
+    tris, mats, cam, params = small_scene(w, h, spp)
+    sc = O.OracleScene(abi.host_to_gpu_triangles(tris), mats)
+    em = np.ascontiguousarray(sc.mats.copy())
+    em[:, WORD["roughness"]] = np.clip(em[:, WORD["roughness"]] * np.float32(1.1), 0.0, 1.0)
+    em[:, WORD["r"]] *= np.float32(0.9)
+    for table in (None, em):
+        a = render(sc, cam, params, w, h, spp, table)
+        b = render_f64(sc, cam, params, w, h, spp, table)
+        assert np.abs(a).max() > 0
+        assert np.all(np.abs(b - a) <= 1e-5 * np.maximum(np.abs(b), 1e-3 * np.abs(b).max())), \
+            float(np.max(np.abs(b - a)))
+    mask = np.zeros((w * h, 3))
+    mask[5 * w + 3: 5 * w + 9, 1] = 1.0
+    c = render_f64(sc, cam, params, w, h, spp, mask=mask)
+    full = render_f64(sc, cam, params, w, h, spp)
+    on = mask.any(axis=1)
+    assert np.array_equal(c[on], full[on]) and not c[~on].any()
+
+
+@pytest.mark.parametrize("first_seed", [1, 37])
+def test_adjoint_matches_fp64_differences_of_the_detached_estimator(first_seed):
+    """The FP64 adjoint against central differences (h = 1e-4) of the detached estimator evaluated
+    in double, for a random linear loss: no float rounding in the loss difference, so the bar is
+    1e-5 relative (200 times below the float-difference test above). What remains is the
+    adjoint's own float inputs (throughput, remaining radiance, Oren-Nayar factor): 4e-7 for 49
+    of the 50 parameters here, 2.3e-6 for one small roughness gradient."""
+    w = h = 12
+    spp = 4
+    tris, mats, cam, params = small_scene(w, h, spp)
+    sc = O.OracleScene(abi.host_to_gpu_triangles(tris), mats)
+    rng = np.random.default_rng(first_seed)
+    gi = np.ascontiguousarray(rng.normal(size=(w * h, 3)))
+    grad = np.zeros(7 * sc.mats.shape[0])
+    O.lib().orc_render_adjoint_seeded(O.ptr(gi), w, h, O.ptr(cam), C.byref(sc.scene), C.byref(O.Env()),
+                                      O.ptr(params), spp, first_seed, O.ptr(grad), 8)
+    grad = grad.reshape(-1, 7)
+    scale = np.abs(grad).max()
+    checked = 0
+    for mi in range(sc.mats.shape[0]):
+        for k, name in enumerate(PARAMS):
+            v = float(sc.mats[mi, WORD[name]])
+            if name == "emission" and v <= 0.0:
+                continue
+            up, dn = _word_steps(sc.mats, name, 1e-4)
+            up[np.arange(len(up)) != mi] = sc.mats[np.arange(len(up)) != mi]
+            dn[np.arange(len(dn)) != mi] = sc.mats[np.arange(len(dn)) != mi]
+            step = float(up[mi, WORD[name]]) - float(dn[mi, WORD[name]])
+            lu = float((gi * render_f64(sc, cam, params, w, h, spp, np.ascontiguousarray(up), first_seed)).sum())
+            ld = float((gi * render_f64(sc, cam, params, w, h, spp, np.ascontiguousarray(dn), first_seed)).sum())
+            fd = (lu - ld) / step
+            err = abs(grad[mi, k] - fd) / max(abs(fd), 1e-6 * scale)
+            assert err <= 1e-5, (mi, name, grad[mi, k], fd)
             checked += 1
     assert checked >= 20
