@@ -134,6 +134,8 @@ typedef struct {
                                            < 0 leaf slot ~ref, INT_MIN none), one unused */
     uint32_t wideNodeCount;
     int wideRootRef;                      /* as rootRef, in the wide tree */
+    const float* materialSpectra;        /* binned spectra for hero-wavelength rendering (NULL: none):
+                                           kPtSpectrumWords floats per material, see ptSceneSetSpectra */
 } MeshAccelSceneGpu;
 
 /* ---- error reporting ------------------------------------------------------------------ */
@@ -513,6 +515,22 @@ bool ptSceneRebuildCheck(PtScene* scene, PtBuildWorkspace* workspace, hipStream_
  * as the build does. Synchronous. */
 bool ptSceneUpdateMaterials(PtScene* scene, const MaterialGpu* materials, uint32_t materialCount,
                             hipStream_t stream);
+
+/* Binned spectra of the scene's materials for hero-wavelength rendering (no reference
+ * counterpart): kPtSpectrumWords floats per material, kPtSpectrumBins reflectance values, then
+ * kPtSpectrumBins emission values. Bin b covers [lambdaMin + b w, lambdaMin + (b + 1) w) of the
+ * hero-wavelength range, w = (lambdaMax - lambdaMin) / kPtSpectrumBins, so a table is read
+ * against the range pathTracerSetSpectralSampling sets. Where a material's first reflectance
+ * value is >= 0, its base colour at a hero wavelength is that wavelength's bin (instead of the
+ * rgb2spec upsampling of r, g, b); where its first emission value is >= 0, its emitted radiance
+ * is emission x that wavelength's bin (on surface hits and for next-event light samples). RGB
+ * rendering, light selection and the emissive list keep using r, g, b and emission. NULL (or
+ * materialCount 0) removes the table. The table survives ptSceneBuild calls that keep the
+ * material count and ptSceneUpdateMaterials; it is uploaded on `stream` and copied from the host
+ * before the call returns. */
+#define kPtSpectrumBins 16
+#define kPtSpectrumWords 32
+bool ptSceneSetSpectra(PtScene* scene, const float* spectra, uint32_t materialCount, hipStream_t stream);
 
 /* BVH build used by the next ptSceneBuild (no reference counterpart): 0 = the reference's
  * median split (default; node arrays equal the reference's), 1 = binned SAH over Morton-ordered

@@ -25,7 +25,9 @@ TEXTURE_WORDS = 16         # TextureDescGpu, 64 bytes
 CAMERA_WORDS = 20          # CameraGpu, 80 bytes
 PARAMS_WORDS = 11          # RenderParamsGpu, 44 bytes
 BVH_NODE_WORDS = 12        # MeshBvhNode, 48 bytes
-SCENE_DESC_BYTES = 128     # MeshAccelSceneGpu
+SCENE_DESC_BYTES = 144     # MeshAccelSceneGpu
+SPECTRUM_BINS = 16         # kPtSpectrumBins
+SPECTRUM_WORDS = 32        # kPtSpectrumWords: reflectance bins, then emission bins
 
 
 class LibraryMissing(ImportError):
@@ -88,6 +90,7 @@ SIGNATURES = {
     "ptSceneReadEmissive": (_b, [_p, _p, _p, _p, _u, C.POINTER(C.c_uint32)]),
     "ptSceneSetBuildMode": (_b, [_p, _i]),
     "ptSceneUpdateMaterials": (_b, [_p, _p, _u, _p]),
+    "ptSceneSetSpectra": (_b, [_p, _p, _u, _p]),
     "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
     "pathTracerAdjointRenderSeeded": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _u, _p]),
     "ptBatchCreate": (_p, [_p, _i]),
@@ -375,6 +378,18 @@ class Scene:
     @property
     def device(self) -> int:
         return lib().ptSceneDevice(self.handle)
+
+    def set_spectra(self, spectra: np.ndarray | None, stream: int = 0) -> None:
+        """Binned spectra for hero-wavelength rendering (ptSceneSetSpectra): (materials, 32)
+        float32, 16 reflectance then 16 emission values per material (a negative first value of
+        either half keeps the material's rgb2spec colour for it); None removes them."""
+        if spectra is None:
+            check(lib().ptSceneSetSpectra(self.handle, None, 0, stream), "ptSceneSetSpectra")
+            self._spectra = None
+            return
+        self._spectra = np.ascontiguousarray(spectra, dtype=np.float32).reshape(-1, SPECTRUM_WORDS)
+        check(lib().ptSceneSetSpectra(self.handle, self._spectra.ctypes.data, self._spectra.shape[0], stream),
+              "ptSceneSetSpectra")
 
     @property
     def build_ms(self) -> float:

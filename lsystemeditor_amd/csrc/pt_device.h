@@ -769,6 +769,7 @@ PT_DEV bool trace_any(const MeshAccelSceneGpu& s, V3 ro, V3 rd, float tMin, floa
 struct Mat {
     float r, g, b, roughness, metallic, transmission, thin, ior, subsurface, emission, diffuseRoughness, srR, srG,
         srB, specular;
+    const float* spec;  // the material's binned spectra record (MeshAccelSceneGpu::materialSpectra) or null
 };
 
 PT_DEV float tex_fract(float v) { return v - floorf(v); }
@@ -845,6 +846,8 @@ PT_DEV Mat resolve_material_index(const MeshAccelSceneGpu& s, uint32_t mi, bool 
     r.srG = fmaxf(0.0f, m.scatterRadiusG);
     r.srB = fmaxf(0.0f, m.scatterRadiusB);
     r.specular = tex_clamp01(m.specular);
+    r.spec = valid && s.materialSpectra != nullptr ? s.materialSpectra + static_cast<size_t>(mi) * kPtSpectrumWords
+                                                    : nullptr;
     return r;
 }
 

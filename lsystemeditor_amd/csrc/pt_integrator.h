@@ -100,7 +100,7 @@ PT_DEV typename Pol::Col emissive_nee_t(const MeshAccelSceneGpu& s, V3 pos, cons
     const float bpdf = brdf_pdf(c, wi);
     const float mis = mis_balance(lightPdf, bpdf);
     const float sc = mis * cosT / lightPdf;
-    return mul(f, scale(pol.radiance(lr), sc));
+    return mul(f, scale(pol.light(s, entry, lr), sc));
 }
 
 // PathIntegratorRandCore.h:169-204 (the step direction is computed and discarded there;

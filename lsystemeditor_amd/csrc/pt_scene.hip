@@ -475,6 +475,9 @@ struct PtScene {
     std::vector<float> emAreaHost;
     std::vector<uint32_t> emMatHost;
     std::vector<float4> emTrisHost;
+    // binned spectra (ptSceneSetSpectra); kept across builds that keep the material count
+    float* spectra = nullptr;
+    uint32_t spectraCount = 0;
 
     void clear()
     {
@@ -651,6 +654,7 @@ void ptSceneDestroy(PtScene* scene)
     if (scene != nullptr) {
         (void)hipDeviceSynchronize();
         scene->clear();
+        release(scene->spectra);
         delete scene;
     }
 }
@@ -660,6 +664,7 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
 {
     static_assert(sizeof(HostTriangle) == 100, "HostTriangle must be 100 bytes");
     static_assert(sizeof(MaterialGpu) == 96, "MaterialGpu must be 96 bytes");
+    static_assert(sizeof(MeshAccelSceneGpu) == 144, "MeshAccelSceneGpu must be 144 bytes");
     static_assert(sizeof(TextureDescGpu) == 64, "TextureDescGpu must be 64 bytes");
     static_assert(sizeof(MeshBvhNode) == 48, "MeshBvhNode must be 48 bytes");
     static_assert(sizeof(CameraGpu) == 80, "CameraGpu must be 80 bytes");
@@ -794,6 +799,7 @@ bool ptSceneBuild(PtScene* sc, const HostTriangle* triangles, uint32_t triangleC
     d.wideNodes = wide && sc->wideCount > 0 ? reinterpret_cast<const PtFloat4*>(sc->wide) : nullptr;
     d.wideNodeCount = wide ? static_cast<uint32_t>(sc->wideCount) : 0u;
     d.wideRootRef = wide ? (d.rootRef >= 0 ? 0 : d.rootRef) : 0;
+    d.materialSpectra = sc->spectra != nullptr && sc->spectraCount == d.materialCount && n > 0 ? sc->spectra : nullptr;
     if (!upload(&sc->d_desc, &d, 1, stream) || hipStreamSynchronize(stream) != hipSuccess) {
         sc->clear();
         ptSetError("ptSceneBuild: scene upload failed");
@@ -887,6 +893,49 @@ bool ptSceneUpdateMaterials(PtScene* sc, const MaterialGpu* materials, uint32_t 
         return false;
     }
     sc->matsHost.assign(materials, materials + materialCount);
+    return true;
+}
+
+bool ptSceneSetSpectra(PtScene* sc, const float* spectra, uint32_t materialCount, hipStream_t stream)
+{
+    if (sc == nullptr || (spectra != nullptr && materialCount > 0 && sc->d_desc != nullptr &&
+                          materialCount != sc->desc.materialCount)) {
+        ptSetError("ptSceneSetSpectra: invalid arguments (one spectrum record per material of the scene)");
+        return false;
+    }
+    (void)hipStreamSynchronize(stream);
+    if (spectra == nullptr || materialCount == 0) {
+        release(sc->spectra);
+        sc->spectra = nullptr;
+        sc->spectraCount = 0;
+    } else {
+        if (materialCount != sc->spectraCount) {
+            release(sc->spectra);
+            sc->spectra = nullptr;
+            sc->spectraCount = 0;
+            if (!alloc(&sc->spectra, static_cast<size_t>(materialCount) * kPtSpectrumWords)) {
+                return false;
+            }
+            sc->spectraCount = materialCount;
+        }
+        if (hipMemcpyAsync(sc->spectra, spectra, static_cast<size_t>(materialCount) * kPtSpectrumWords * sizeof(float),
+                           hipMemcpyHostToDevice, stream) != hipSuccess) {
+            ptSetError("ptSceneSetSpectra: upload failed");
+            return false;
+        }
+    }
+    if (sc->d_desc != nullptr) {
+        sc->desc.materialSpectra = sc->spectra;
+        if (hipMemcpyAsync(&sc->d_desc->materialSpectra, &sc->desc.materialSpectra, sizeof(const float*),
+                           hipMemcpyHostToDevice, stream) != hipSuccess) {
+            ptSetError("ptSceneSetSpectra: upload failed");
+            return false;
+        }
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) {
+        ptSetError("ptSceneSetSpectra: upload failed");
+        return false;
+    }
     return true;
 }
 

@@ -216,16 +216,28 @@ struct RgbColors {
     PT_DEV V3 base(const Mat& m) const { return mk(m.r, m.g, m.b); }
     PT_DEV V3 radii(const Mat& m) const { return mk(max2(0.0f, m.srR), max2(0.0f, m.srG), max2(0.0f, m.srB)); }
     PT_DEV V3 emission(const Mat& m) const { return mk(m.r * m.emission, m.g * m.emission, m.b * m.emission); }
+    // radiance of the emissive-list entry a next-event light sample chose (lr: its r, g, b x emission)
+    PT_DEV V3 light(const MeshAccelSceneGpu&, int, V3 lr) const { return lr; }
     PT_DEV V3 radiance(V3 rgb) const { return rgb; }
     PT_DEV V3 reflectance(V3 rgb) const { return rgb; }
     // BrdfDispatch.h:29-33
     PT_DEV float lum(V3 t) const { return 0.2126f * t.x + 0.7152f * t.y + 0.0722f * t.z; }
 };
 
+// Bin of a hero wavelength in a binned spectrum: kPtSpectrumBins equal bins over the hero range.
+PT_HD int spectrum_bin(const SpecModel& m, float lambdaNm)
+{
+    const int b = static_cast<int>((lambdaNm - m.lambdaMin) * (static_cast<float>(kPtSpectrumBins) / m.lambdaRange));
+    return b < 0 ? 0 : (b > kPtSpectrumBins - 1 ? kPtSpectrumBins - 1 : b);
+}
+
 struct HeroColors {
     using Col = S4;
     SpecModel m;
     S4 lambda;
+    int bin[4];  // spectrum_bin of each hero wavelength
+
+    PT_DEV S4 binned(const float* v) const { return mk4(v[bin[0]], v[bin[1]], v[bin[2]], v[bin[3]]); }
 
     PT_DEV S4 sig4(const float c[3]) const
     {
@@ -249,14 +261,40 @@ struct HeroColors {
         rgb2spec_coeffs(m, r, g, b, c);
         return scale(sig4(c), s);
     }
-    PT_DEV S4 base(const Mat& mt) const { return reflectance(mk(mt.r, mt.g, mt.b)); }
+    // binned reflectance where the material has one, else the rgb2spec upsampling of its colour
+    PT_DEV S4 base(const Mat& mt) const
+    {
+        if (mt.spec != nullptr && mt.spec[0] >= 0.0f) {
+            return binned(mt.spec);
+        }
+        return reflectance(mk(mt.r, mt.g, mt.b));
+    }
     PT_DEV S4 radii(const Mat& mt) const
     {
         return radiance(mk(max2(0.0f, mt.srR), max2(0.0f, mt.srG), max2(0.0f, mt.srB)));
     }
+    // emission x binned emission where the material has one, else the upsampled r, g, b x emission
     PT_DEV S4 emission(const Mat& mt) const
     {
+        if (mt.spec != nullptr && mt.spec[kPtSpectrumBins] >= 0.0f) {
+            return scale(binned(mt.spec + kPtSpectrumBins), mt.emission);
+        }
         return radiance(mk(mt.r * mt.emission, mt.g * mt.emission, mt.b * mt.emission));
+    }
+    // a next-event light sample's radiance: its material's binned emission (x the raw emission,
+    // as the emissive list's r, g, b are) where it has one, else lr upsampled
+    PT_DEV S4 light(const MeshAccelSceneGpu& s, int entry, V3 lr) const
+    {
+        if (s.materialSpectra != nullptr) {
+            const uint32_t mi = tri_material(s, s.emissiveTriangleIndices[entry]);
+            if (mi < s.materialCount) {
+                const float* sp = s.materialSpectra + static_cast<size_t>(mi) * kPtSpectrumWords;
+                if (sp[kPtSpectrumBins] >= 0.0f) {
+                    return scale(binned(sp + kPtSpectrumBins), s.materials[mi].emission);
+                }
+            }
+        }
+        return radiance(lr);
     }
     PT_DEV float lum(S4 t) const { return 0.25f * (((t.x + t.y) + t.z) + t.w); }
 
@@ -288,6 +326,9 @@ PT_DEV HeroColors hero_colors(const SpecModel& m, float u)
         v[k] = m.lambdaMin + f * m.lambdaRange;
     }
     h.lambda = mk4(v[0], v[1], v[2], v[3]);
+    for (int k = 0; k < 4; ++k) {
+        h.bin[k] = spectrum_bin(m, v[k]);
+    }
     return h;
 }
 

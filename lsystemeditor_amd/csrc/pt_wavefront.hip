@@ -969,7 +969,7 @@ PT_DEV void shade_items(const MeshAccelSceneGpu* scene, const RenderParamsGpu* p
                                             const float bpdf = brdf_pdf(c, wi);
                                             const float mis = mis_balance(lightPdf, bpdf);
                                             const float sc = mis * cosT / lightPdf;
-                                            cm = mul(T, mul(f, scale(pol.radiance(lr), sc)));
+                                            cm = mul(T, mul(f, scale(pol.light(s, entry, lr), sc)));
                                         }
                                     }
                                 }

@@ -74,7 +74,13 @@ typedef struct {
     const float* emissiveCdf;
     const OTexDesc* textures;
     uint32_t nodeCount, triangleCount, rootIndex, materialCount, emissiveCount, textureCount;
+    /* binned spectra for the hero-wavelength integrator (NULL: none): ORC_SPECTRUM_WORDS floats per
+     * material, ORC_SPECTRUM_BINS reflectance values then ORC_SPECTRUM_BINS emission values
+     * (MeshAccelSceneGpu::materialSpectra; no counterpart in the reference) */
+    const float* spectra;
 } OScene;
+#define ORC_SPECTRUM_BINS 16
+#define ORC_SPECTRUM_WORDS 32
 
 /* CameraGpu.h:5-16 (80 B; float4 member forces 16-byte alignment) */
 typedef struct {
@@ -202,6 +208,8 @@ float orc_env_pdf(const OEnv* env, const OParams* params, OVec3 dir);
 OVec3 orc_env_sample(const OEnv* env, const OParams* params, float u1, float u2, float* pdf);
 int orc_light_sample_emissive(const OScene* scene, float uTri, float u1, float u2, OVec3* pos,
                               OVec3* normal, OVec3* radiance, float* pdf);
+int orc_light_sample_emissive_tri(const OScene* scene, float uTri, float u1, float u2, OVec3* pos, OVec3* normal,
+                                  OVec3* radiance, float* pdf, uint32_t* triOut);
 OMaterial orc_fetch_material(const OScene* scene, uint32_t tri);
 OMaterial orc_resolve_material(const OMaterial* m, OVec2 uv, const OScene* scene);
 int orc_env_build_tables(const float* rgb, int width, int height, float* marginalCdf, float* rowCdf);

@@ -73,7 +73,7 @@ class Scene(C.Structure):
                 ("emissiveIndices", C.c_void_p), ("emissiveCdf", C.c_void_p), ("textures", C.c_void_p),
                 ("nodeCount", C.c_uint32), ("triangleCount", C.c_uint32), ("rootIndex", C.c_uint32),
                 ("materialCount", C.c_uint32), ("emissiveCount", C.c_uint32), ("textureCount", C.c_uint32),
-                ("_pad", C.c_uint32 * 2)]
+                ("spectra", C.c_void_p)]
 
 
 class Env(C.Structure):
@@ -345,6 +345,16 @@ class OracleScene:
         s.emissiveCount = self.em_count
         s.textureCount = 0 if self.textures is None else self.textures.shape[0]
         self.scene = s
+        self.spectra = None
+
+    def set_spectra(self, spectra: np.ndarray | None) -> None:
+        """(materials, 32) float32 binned spectra for the hero-wavelength integrator, or None."""
+        if spectra is None:
+            self.spectra = None
+            self.scene.spectra = None
+            return
+        self.spectra = np.ascontiguousarray(spectra, dtype=np.float32).reshape(self.mats.shape[0], 32)
+        self.scene.spectra = ptr(self.spectra)
 
     def trace(self, ro, rd, tmin=0.0, tmax=1.0e6) -> Hit:
         return lib().orc_trace_ray(v3(ro), v3(rd), C.byref(self.scene), tmin, tmax)

@@ -322,6 +322,13 @@ static float emissive_selection_pdf(const OScene* s, uint32_t li)
 int orc_light_sample_emissive(const OScene* s, float uTri, float u1, float u2, OVec3* pos, OVec3* normal,
                               OVec3* radiance, float* pdf)
 {
+    return orc_light_sample_emissive_tri(s, uTri, u1, u2, pos, normal, radiance, pdf, NULL);
+}
+
+/* As orc_light_sample_emissive; *triOut (if not NULL) receives the chosen triangle's index. */
+int orc_light_sample_emissive_tri(const OScene* s, float uTri, float u1, float u2, OVec3* pos, OVec3* normal,
+                                  OVec3* radiance, float* pdf, uint32_t* triOut)
+{
     *pdf = 0.0f;
     if (s == NULL || s->emissiveCount == 0 || s->emissiveIndices == NULL || s->emissiveCdf == NULL ||
         s->triangles == NULL) {
@@ -343,6 +350,9 @@ int orc_light_sample_emissive(const OScene* s, float uTri, float u1, float u2, O
     }
     const uint32_t li = (uint32_t)lo;
     const uint32_t ti = s->emissiveIndices[li];
+    if (triOut != NULL) {
+        *triOut = ti;
+    }
     const OTriangle* t = &s->triangles[ti];
     const OMaterial m = orc_fetch_material(s, ti);
     const float su = sqrtf(fmax2(0.0f, u1));

@@ -365,14 +365,33 @@ static OMaterial grey_material(const OMaterial* m, float base, float radius)
 typedef struct {
     const OSpecModel* model;
     float lambda[4];
+    int bin[4]; /* binned-spectrum bin of each wavelength */
     OMaterial grey[4];
     OBrdfCtx ctx[4];
 } SpecVertex;
 
-static void spec_vertex(SpecVertex* v, const OMaterial* m, OVec3 n, OVec3 wo, float eta, int flags)
+/* Bin of a hero wavelength: ORC_SPECTRUM_BINS equal bins over the hero range. */
+static int spectrum_bin(const OSpecModel* m, float lambdaNm)
+{
+    const int b = (int)((lambdaNm - m->lambdaMin) * ((float)ORC_SPECTRUM_BINS / m->lambdaRange));
+    return b < 0 ? 0 : (b > ORC_SPECTRUM_BINS - 1 ? ORC_SPECTRUM_BINS - 1 : b);
+}
+
+/* The binned spectra record of a triangle's material, or NULL. */
+static const float* material_spectrum(const OScene* s, uint32_t tri)
+{
+    if (s == NULL || s->spectra == NULL || s->materials == NULL || s->triangles == NULL || tri >= s->triangleCount) {
+        return NULL;
+    }
+    const uint32_t mi = s->triangles[tri].materialIndex;
+    return mi < s->materialCount ? s->spectra + (size_t)mi * ORC_SPECTRUM_WORDS : NULL;
+}
+
+static void spec_vertex(SpecVertex* v, const OMaterial* m, const float* sp, OVec3 n, OVec3 wo, float eta, int flags)
 {
     for (int k = 0; k < 4; ++k) {
-        const float b = orc_rgb2spec_reflectance(v->model, m->r, m->g, m->b, v->lambda[k]);
+        const float b = sp != NULL && sp[0] >= 0.0f ? sp[v->bin[k]]
+                                                    : orc_rgb2spec_reflectance(v->model, m->r, m->g, m->b, v->lambda[k]);
         const float r = orc_rgb2spec_radiance(v->model, fmax2(0.0f, m->scatterRadiusR), fmax2(0.0f, m->scatterRadiusG),
                                               fmax2(0.0f, m->scatterRadiusB), v->lambda[k]);
         v->grey[k] = grey_material(m, b, r);
@@ -380,13 +399,34 @@ static void spec_vertex(SpecVertex* v, const OMaterial* m, OVec3 n, OVec3 wo, fl
     }
 }
 
-/* f_k * (light_k * scale) for a light sample of RGB radiance rgb */
-static void spec_light_term(const SpecVertex* v, OVec3 wi, OVec3 rgb, float scale, float* out4)
+/* f_k * (light_k * scale) for a light sample of radiance Le4 at the four wavelengths */
+static void spec_light_term4(const SpecVertex* v, OVec3 wi, const float* Le4, float scale, float* out4)
 {
     for (int k = 0; k < 4; ++k) {
         const float f = orc_brdf_eval(&v->ctx[k], wi).x;
-        const float Le = orc_rgb2spec_radiance(v->model, rgb.x, rgb.y, rgb.z, v->lambda[k]);
-        out4[k] = f * (Le * scale);
+        out4[k] = f * (Le4[k] * scale);
+    }
+}
+
+/* the same for a light sample of RGB radiance rgb (upsampled as radiance) */
+static void spec_light_term(const SpecVertex* v, OVec3 wi, OVec3 rgb, float scale, float* out4)
+{
+    float Le[4];
+    for (int k = 0; k < 4; ++k) {
+        Le[k] = orc_rgb2spec_radiance(v->model, rgb.x, rgb.y, rgb.z, v->lambda[k]);
+    }
+    spec_light_term4(v, wi, Le, scale, out4);
+}
+
+/* Emitted radiance of a material at the four wavelengths: emission x its binned emission where
+ * it has one, else the upsampled r, g, b x emission. */
+static void spec_emission(const SpecVertex* v, const OMaterial* m, const float* sp, float* Le4)
+{
+    for (int k = 0; k < 4; ++k) {
+        Le4[k] = sp != NULL && sp[ORC_SPECTRUM_BINS] >= 0.0f
+                     ? sp[ORC_SPECTRUM_BINS + v->bin[k]] * m->emission
+                     : orc_rgb2spec_radiance(v->model, m->r * m->emission, m->g * m->emission, m->b * m->emission,
+                                             v->lambda[k]);
     }
 }
 
@@ -429,7 +469,8 @@ static int spec_emissive_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, co
     }
     OVec3 lp, ln, lr;
     float areaPdf = 0.0f;
-    if (!orc_light_sample_emissive(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf)) {
+    uint32_t lightTri = 0u;
+    if (!orc_light_sample_emissive_tri(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf, &lightTri)) {
         return 0;
     }
     const OVec3 toL = vsub(lp, p);
@@ -450,7 +491,18 @@ static int spec_emissive_nee(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, co
     }
     const OBrdfCtx rgb = make_ctx(n, wo, m, eta, 0);
     const float mis = orc_mis_balance(lightPdf, orc_brdf_pdf(&rgb, wi));
-    spec_light_term(v, wi, lr, mis * cosT / lightPdf, out4);
+    /* a light with binned emission: its raw emission x the bins (as the emissive list's r, g, b) */
+    const float* lsp = material_spectrum(s, lightTri);
+    if (lsp != NULL && lsp[ORC_SPECTRUM_BINS] >= 0.0f) {
+        const OMaterial lm = orc_fetch_material(s, lightTri);
+        float Le[4];
+        for (int k = 0; k < 4; ++k) {
+            Le[k] = lsp[ORC_SPECTRUM_BINS + v->bin[k]] * lm.emission;
+        }
+        spec_light_term4(v, wi, Le, mis * cosT / lightPdf, out4);
+    } else {
+        spec_light_term(v, wi, lr, mis * cosT / lightPdf, out4);
+    }
     return 1;
 }
 
@@ -493,6 +545,7 @@ void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* s, const OParams*
     v.model = model;
     for (int k = 0; k < 4; ++k) {
         v.lambda[k] = lambda4[k];
+        v.bin[k] = spectrum_bin(model, lambda4[k]);
     }
 
     for (int depth = 0; depth < maxDepth; ++depth) {
@@ -501,14 +554,15 @@ void orc_trace_path_spectral(OVec3 ro, OVec3 rd, const OScene* s, const OParams*
         const OVec3 wo = vscale(dir, -1.0f);
         const OMaterial src = orc_fetch_material(s, hit.triangleIndex);
         const OMaterial m = orc_resolve_material(&src, hit.uv, s);
-        spec_vertex(&v, &m, n, wo, eta, 0);
+        const float* sp = material_spectrum(s, hit.triangleIndex);
+        spec_vertex(&v, &m, sp, n, wo, eta, 0);
 
         const int weighted = gLightMode == 1 && depth > 0 && m.emission > 0.0f;
         const float w = weighted ? emission_weight(s, hit.triangleIndex, dir, hit.t, prevPdf, reach) : 1.0f;
+        float Le4[4];
+        spec_emission(&v, &m, sp, Le4);
         for (int k = 0; k < 4; ++k) {
-            const float Le = orc_rgb2spec_radiance(model, m.r * m.emission, m.g * m.emission, m.b * m.emission,
-                                                   v.lambda[k]);
-            L4[k] = L4[k] + T[k] * (weighted ? Le * w : Le);
+            L4[k] = L4[k] + T[k] * (weighted ? Le4[k] * w : Le4[k]);
         }
         float c[4];
         spec_env_nee(&v, p, n, wo, &m, eta, s, env, params, rng, c);

@@ -152,3 +152,71 @@ def test_adjoint_rejects_spectral(R, cuda, spectral):
     r = diff.DiffRenderer(sc, mats, cam, params, w, h, 2)
     with pytest.raises(RuntimeError, match="RGB rendering only"):
         r.adjoint(cuda.zeros((w * h, 3)))
+
+
+def binned_table(mats: np.ndarray, seed: int) -> np.ndarray:
+    """Random binned spectra: every other material keeps its rgb2spec reflectance, emissive
+    materials get a binned emission shape (the plain-colour ones keep theirs on odd seeds)."""
+    rng = np.random.default_rng(seed)
+    n = mats.shape[0]
+    sp = np.empty((n, abi.SPECTRUM_WORDS), np.float32)
+    sp[:, :16] = rng.uniform(0.05, 0.95, (n, 16))
+    sp[:, 16:] = rng.uniform(0.2, 1.5, (n, 16))
+    sp[1::2, 0] = -1.0
+    emissive = mats[:, 9] > 0.0
+    sp[~emissive, 16] = -1.0
+    if seed % 2:
+        sp[emissive, 16] = -1.0
+    return sp
+
+
+@pytest.mark.parametrize("kind,mode,seed", [("zoo", 0, 0), ("zoo", 1, 2), ("plain", 1, 1), ("diff", 0, 4)])
+def test_binned_spectra_match_oracle(R, cuda, table, kind, mode, seed):
+    """Materials with 16-bin reflectance and emission spectra (ptSceneSetSpectra): the per-pass
+    launcher and the wavefront scheduler against the oracle's spectral integrator, bit for bit,
+    in both light sampling modes (mode 1 evaluates next-event light samples' binned emission)."""
+    s = abi.Spectral(*table)
+    s.enable(380.0, 720.0)
+    O.lib().orc_set_light_sampling(mode)
+    try:
+        with abi.LightSampling(mode):
+            w = h = 32
+            tris, mats, tex, cam, params = dense_plant(w, h, spp=8, min_spp=4, threshold=0.05, materials=kind)
+            sp = binned_table(mats, seed)
+            sc = abi.Scene(tris, mats, tex)
+            sc.set_spectra(sp)
+            ref = oracle_scene(tris, mats, tex)
+            ref.set_spectra(sp)
+            with O.OracleSpectral(*table, lambda_min=380.0, lambda_max=720.0):
+                acc_o, passes_o = O.render_adaptive(ref, cam, params, w, h, threads=8)
+            dev = R.Accumulator.allocate(w, h)
+            passes = R.render_loop(sc, dev, R.DeviceInputs(cam, params))
+            assert passes == passes_o
+            compare_accumulators(dev.host(), acc_o, f"binned loop/{kind}/{mode}")
+            dev2 = R.Accumulator.allocate(w, h)
+            R.render_wavefront(sc, dev2, R.DeviceInputs(cam, params))
+            compare_accumulators(dev2.host(), acc_o, f"binned wavefront/{kind}/{mode}")
+            # the spectra change the image: without them it differs
+            sc.set_spectra(None)
+            dev3 = R.Accumulator.allocate(w, h)
+            R.render_wavefront(sc, dev3, R.DeviceInputs(cam, params))
+            assert not np.array_equal(dev3.host()["acc"], dev2.host()["acc"])
+    finally:
+        O.lib().orc_set_light_sampling(0)
+        abi.Spectral.disable()
+        s.close()
+
+
+def test_set_spectra_validation(cuda):
+    tris, mats, tex, cam, params = dense_plant(8, 8, spp=2)
+    sc = abi.Scene(tris, mats, tex)
+    lib = abi.lib()
+    bad = np.zeros((mats.shape[0] + 1, abi.SPECTRUM_WORDS), np.float32)
+    assert not lib.ptSceneSetSpectra(sc.handle, bad.ctypes.data, bad.shape[0], 0)
+    assert "one spectrum record per material" in abi.last_error()
+    assert not lib.ptSceneSetSpectra(None, bad.ctypes.data, 1, 0)
+    sc.set_spectra(np.zeros((mats.shape[0], abi.SPECTRUM_WORDS), np.float32))
+    desc = sc.description()
+    assert int.from_bytes(desc[128:136], "little") != 0
+    sc.set_spectra(None)
+    assert int.from_bytes(sc.description()[128:136], "little") == 0
