@@ -246,7 +246,8 @@ void ptSpectralDestroy(PtSpectral* spectral);
  * each sample is converted to linear sRGB (CIE 1931 x D65, divided by the white response and
  * the wavelength pdf) before the firefly clamp, which keeps negative channels in this mode.
  * 0 returns to RGB. Needs 360 <= lambdaMin < lambdaMax <= 830 and a stored table. Process-wide
- * setting; change it only between renders. The adjoint entry points reject spectral mode. */
+ * setting; change it only between renders. The RGB adjoint entry points reject spectral mode;
+ * pathTracerAdjointRenderSpectral differentiates it. */
 bool pathTracerSetSpectralSampling(int wavelengthsPerSample, float lambdaMinNm, float lambdaMaxNm);
 int pathTracerSpectralSampling(void);
 
@@ -389,9 +390,27 @@ bool pathTracerAdjointRenderSeeded(const float* d_gradImage, int width, int heig
                                    int materialCount, const float* d_sampleRadiance, double* d_grad,
                                    unsigned int firstSeed, hipStream_t stream);
 
+/* The adjoint of a hero-wavelength render (pathTracerSetSpectralSampling(4, ...) on, light
+ * sampling 0): as pathTracerAdjointRenderSeeded, with d_grad accumulating kPtSpectrumGradParams
+ * doubles per material: the kPtSpectrumBins reflectance bins, the kPtSpectrumBins emission bins,
+ * roughness, metallic, specular and the emission scale of the material's binned spectra
+ * (ptSceneSetSpectra). The bins of a material are parameters where it has binned spectra (its
+ * base colour / emitted radiance at a hero wavelength is that wavelength's bin, x emission for
+ * the latter); roughness, metallic and specular are parameters as in the RGB adjoint. Each
+ * sample's loss weights pass through the firefly clamp and the conversion of its four
+ * wavelengths to linear sRGB. d_sampleRadiance, if given, holds 4 floats per sample
+ * (pathTracerWavefrontCaptureRadiance in spectral mode). No reference counterpart. */
+#define kPtSpectrumGradParams 36
+bool pathTracerAdjointRenderSpectral(const float* d_gradImage, int width, int height, int samplesPerPixel,
+                                     const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                     const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                     int materialCount, const float* d_sampleRadiance, double* d_grad,
+                                     unsigned int firstSeed, hipStream_t stream);
+
 /* While d_out is set, pathTracerAdaptiveRenderWavefront writes the unclamped radiance of every
  * finished sample with index < samplesPerPixel to d_out[3 * (pixel * samplesPerPixel + sample)]
- * (width * height * samplesPerPixel * 3 floats). NULL turns it off. No reference counterpart. */
+ * (width * height * samplesPerPixel * 3 floats; in hero-wavelength mode the radiance at the four
+ * wavelengths, 4 floats per sample). NULL turns it off. No reference counterpart. */
 bool pathTracerWavefrontCaptureRadiance(float* d_out, int samplesPerPixel);
 
 /* ---- batches of scenes (no reference counterpart) --------------------------------------------

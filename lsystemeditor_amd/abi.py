@@ -93,6 +93,7 @@ SIGNATURES = {
     "ptSceneSetSpectra": (_b, [_p, _p, _u, _p]),
     "pathTracerAdjointRender": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p]),
     "pathTracerAdjointRenderSeeded": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _u, _p]),
+    "pathTracerAdjointRenderSpectral": (_b, [_p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _u, _p]),
     "ptBatchCreate": (_p, [_p, _i]),
     "ptBatchDestroy": (None, [_p]),
     "ptBatchInfo": (_b, [_p, _p, _p, _p, _p, _p]),

@@ -237,6 +237,31 @@ extern "C" bool pathTracerAdjointRenderSeeded(const float* d_gradImage, int widt
                                 materialCount, d_sampleRadiance, d_grad, firstSeed, stream);
 }
 
+extern "C" bool pathTracerAdjointRenderSpectral(const float* d_gradImage, int width, int height, int samplesPerPixel,
+                                                const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
+                                                const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,
+                                                int materialCount, const float* d_sampleRadiance, double* d_grad,
+                                                unsigned int firstSeed, hipStream_t stream)
+{
+    if (d_gradImage == nullptr || d_camera == nullptr || d_scene == nullptr || d_params == nullptr ||
+        d_grad == nullptr || width <= 0 || height <= 0 || samplesPerPixel <= 0 || materialCount <= 0 ||
+        materialCount > kMaxGradMaterials) {
+        ptSetError("pathTracerAdjointRenderSpectral: invalid arguments (materialCount must be 1..%d)",
+                   kMaxGradMaterials);
+        return false;
+    }
+    if (!pt_spectral_enabled()) {
+        ptSetError("pathTracerAdjointRenderSpectral: hero-wavelength rendering only (pathTracerSetSpectralSampling)");
+        return false;
+    }
+    if (pt_light_mode() != 0) {
+        ptSetError("pathTracerAdjointRenderSpectral: the reference light sampling only (pathTracerSetLightSampling(0))");
+        return false;
+    }
+    return pt_adjoint_wavefront(d_gradImage, width, height, samplesPerPixel, d_camera, d_scene, d_env, d_params,
+                                materialCount, d_sampleRadiance, d_grad, firstSeed, stream);
+}
+
 extern "C" bool pathTracerAdjointRender(const float* d_gradImage, int width, int height, int samplesPerPixel,
                                         const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene,
                                         const EnvironmentMapGpu* d_env, const RenderParamsGpu* d_params,

@@ -1,7 +1,7 @@
 // pt_adjoint_math.h -- derivative cores shared by the adjoint kernels: which material parameters a
-// vertex is differentiable in, the contracted derivative of the opaque Principled BSDF value, the
-// vector-Jacobian product of the firefly clamp, and the FP64 LDS accumulation of per-vertex
-// gradients.
+// vertex is differentiable in, the contracted derivative of the opaque Principled BSDF value (RGB
+// or per hero wavelength), the vector-Jacobian products of the firefly clamps, and the FP64 LDS
+// accumulation of per-vertex gradients.
 #pragma once
 
 #include "pt_device.h"
@@ -35,11 +35,17 @@ PT_DEV Live live_of(const MaterialGpu& raw, const Mat& m, const MeshAccelSceneGp
     return l;
 }
 
-// out[k] += sum_c a_c d f_c / d theta_k for theta = (r, g, b, roughness, metallic, specular) of
-// the opaque Principled BSDF f = base wd ON / pi + F ws ec D G / max(4 cosO cosI, 1e-8), with the
-// normalised lobe weights wd = (1 - m) / (2 - m), ws = 1 / (2 - m).
-PT_DEV void brdf_grad(const Ctx& c, V3 wi, V3 a, const Live& lm, float out[6])
+// Contracted derivative of the opaque Principled BSDF f_k = base_k wd ON / pi + F_k ws ec D G /
+// max(4 cosO cosI, 1e-8), F_k = (f0 + (base_k - f0) m)(1 - t^5) + t^5, with the normalised lobe
+// weights wd = (1 - m) / (2 - m), ws = 1 / (2 - m), for a colour of n components (RGB, or the
+// base colour at the four hero wavelengths): outBase[k] += a_k df_k/dbase_k and
+// outScalar[0..2] += sum_k a_k df_k/d(roughness, metallic, specular). lm.rgb says whether the
+// base colour is a parameter. The RGB instantiation's arithmetic is brdf_grad's.
+template <class C>
+PT_DEV void brdf_grad_t(const Ctx& c, V3 wi, const C& base, const C& a, const Live& lm, float* outBase,
+                        float* outScalar)
 {
+    constexpr int n = ColN<C>::n;
     if (!lm.opaque) {
         return;
     }
@@ -59,13 +65,18 @@ PT_DEV void brdf_grad(const Ctx& c, V3 wi, V3 a, const Live& lm, float out[6])
     const float on = oren_nayar(sigma, cosWi, cosWo, dot(wi, c.wo));
     if (lm.rgb) {
         const float k = wd * on * kInvPi;
-        out[0] += a.x * k;
-        out[1] += a.y * k;
-        out[2] += a.z * k;
+#pragma unroll
+        for (int q = 0; q < n; ++q) {
+            outBase[q] += cget(a, q) * k;
+        }
     }
-    const float aBase = a.x * m.r + a.y * m.g + a.z * m.b;
+    float aBase = cget(a, 0) * cget(base, 0);
+#pragma unroll
+    for (int q = 1; q < n; ++q) {
+        aBase = aBase + cget(a, q) * cget(base, q);
+    }
     if (lm.metal) {
-        out[4] += aBase * on * kInvPi * (-inv2m * inv2m);
+        outScalar[1] += aBase * on * kInvPi * (-inv2m * inv2m);
     }
     if (lm.rough && lm.onRough && sigma > 1.0e-4f) {
         const float s2 = sigma * sigma;
@@ -85,7 +96,7 @@ PT_DEV void brdf_grad(const Ctx& c, V3 wi, V3 a, const Live& lm, float out[6])
         const float dA = -0.5f * 0.33f / ((s2 + 0.33f) * (s2 + 0.33f));
         const float dB = 0.45f * 0.09f / ((s2 + 0.09f) * (s2 + 0.09f));
         const float dOn = (dA + dB * geo) * 2.0f * sigma * 0.5f * kPi;
-        out[3] += aBase * wd * dOn * kInvPi;
+        outScalar[0] += aBase * wd * dOn * kInvPi;
     }
     // specular lobe
     const V3 h = normalize(add(wi, c.wo));
@@ -117,27 +128,33 @@ PT_DEV void brdf_grad(const Ctx& c, V3 wi, V3 a, const Live& lm, float out[6])
     const float t = max2(0.0f, min2(1.0f, 1.0f - cosHo));
     const float t2 = t * t;
     const float t5 = t2 * t2 * t;
-    const V3 F = mk((f0v + (m.r - f0v) * mm) * (1.0f - t5) + t5, (f0v + (m.g - f0v) * mm) * (1.0f - t5) + t5,
-                    (f0v + (m.b - f0v) * mm) * (1.0f - t5) + t5);
     const float cw = max2(cosWo, 1.0e-4f);
     const float ess = 1.0f - alpha * (0.3316f + 0.3237f * alpha);
     const float ems = (1.0f - ess) * (1.0f - cw);
     const float ec = 1.0f + ems / max2(ess, 1.0e-4f);
     const float denom = max2(4.0f * cosWo * cosWi, 1.0e-8f);
     const float S = ws * ec * D * G / denom;
-    const float aF = a.x * F.x + a.y * F.y + a.z * F.z;
+    float aF = cget(a, 0) * ((f0v + (cget(base, 0) - f0v) * mm) * (1.0f - t5) + t5);
+    float aSum = cget(a, 0);
+    float dF = cget(a, 0) * (cget(base, 0) - f0v);
+#pragma unroll
+    for (int q = 1; q < n; ++q) {
+        aF = aF + cget(a, q) * ((f0v + (cget(base, q) - f0v) * mm) * (1.0f - t5) + t5);
+        aSum = aSum + cget(a, q);
+        dF = dF + cget(a, q) * (cget(base, q) - f0v);
+    }
     if (lm.rgb) {
         const float k = mm * (1.0f - t5) * S;
-        out[0] += a.x * k;
-        out[1] += a.y * k;
-        out[2] += a.z * k;
+#pragma unroll
+        for (int q = 0; q < n; ++q) {
+            outBase[q] += cget(a, q) * k;
+        }
     }
     if (lm.metal) {
-        const float dF = (a.x * (m.r - f0v) + a.y * (m.g - f0v) + a.z * (m.b - f0v)) * (1.0f - t5);
-        out[4] += dF * S + aF * S * inv2m;
+        outScalar[1] += dF * (1.0f - t5) * S + aF * S * inv2m;
     }
     if (lm.spec) {
-        out[5] += (a.x + a.y + a.z) * (1.0f - mm) * (1.0f - t5) * f0s * f0s * S;
+        outScalar[2] += aSum * (1.0f - mm) * (1.0f - t5) * f0s * f0s * S;
     }
     if (lm.rough && rough > 0.04f) {
         const float dD = (dd - 2.0f * a2 * c2) / (kPi * dd * dd * dd) * 2.0f * alpha;
@@ -146,8 +163,15 @@ PT_DEV void brdf_grad(const Ctx& c, V3 wi, V3 a, const Live& lm, float out[6])
         const float dEms = -dEss * (1.0f - cw);
         const float dEc = ess > 1.0e-4f ? (dEms * ess - ems * dEss) / (ess * ess) : dEms / 1.0e-4f;
         const float dS = ws / denom * (dEc * D * G + ec * dD * G + ec * D * dG);
-        out[3] += aF * dS * 2.0f * rough;
+        outScalar[0] += aF * dS * 2.0f * rough;
     }
+}
+
+// out[k] += sum_c a_c d f_c / d theta_k for theta = (r, g, b, roughness, metallic, specular) of
+// the opaque Principled BSDF (brdf_grad_t on the material's RGB base colour).
+PT_DEV void brdf_grad(const Ctx& c, V3 wi, V3 a, const Live& lm, float out[6])
+{
+    brdf_grad_t(c, wi, mk(c.m.r, c.m.g, c.m.b), a, lm, out, out + 3);
 }
 
 // d(clamped radiance)/d(radiance)^T g for clamp_firefly (PathTracer.cu:49-67).
@@ -167,6 +191,28 @@ PT_DEV V3 clamp_vjp(V3 C, V3 g)
     return mk(C.x > 0.0f ? gr.x : 0.0f, C.y > 0.0f ? gr.y : 0.0f, C.z > 0.0f ? gr.z : 0.0f);
 }
 
+// d(clamped radiance)/d(radiance)^T g for clamp_firefly_spectral: negative channels pass through.
+PT_DEV V3 clamp_vjp_spectral(V3 C, V3 g)
+{
+    if (!isfinite(C.x) || !isfinite(C.y) || !isfinite(C.z)) {
+        return mk(0.0f, 0.0f, 0.0f);
+    }
+    const float lum = C.x * 0.2126f + C.y * 0.7152f + C.z * 0.0722f;
+    if (lum <= 100.0f) {
+        return g;
+    }
+    const float gdr = g.x * C.x + g.y * C.y + g.z * C.z;
+    const float k = 100.0f / lum;
+    return mk(k * (g.x - 0.2126f * gdr / lum), k * (g.y - 0.7152f * gdr / lum), k * (g.z - 0.0722f * gdr / lum));
+}
+
+// Parameters of the spectral adjoint per material: the kPtSpectrumBins reflectance bins, the
+// kPtSpectrumBins emission bins, roughness, metallic, specular and the emission scale.
+constexpr int kSpecGradParams = 2 * kPtSpectrumBins + 4;
+static_assert(kSpecGradParams == kPtSpectrumGradParams, "spectral gradient layout");
+constexpr int kSpecRough = 2 * kPtSpectrumBins;
+constexpr int kSpecEmission = 2 * kPtSpectrumBins + 3;
+
 // lg points at this lane's copy of the block's per-material FP64 sums (lanes of a wave spread over
 // several copies, so one LDS atomic instruction rarely has many lanes on the same address).
 PT_DEV void flush(double* lg, int mi, const float acc[kGradParams])
@@ -174,6 +220,24 @@ PT_DEV void flush(double* lg, int mi, const float acc[kGradParams])
     for (int k = 0; k < kGradParams; ++k) {
         if (acc[k] != 0.0f) {
             atomicAdd(&lg[mi * kGradParams + k], static_cast<double>(acc[k]));
+        }
+    }
+}
+
+// A spectral vertex record (derivatives with respect to the base colour at the four hero
+// wavelengths, then roughness, metallic, specular) into row mi: wavelength k's term goes to its
+// reflectance bin.
+PT_DEV void flush_spectral(double* lg, int mi, const int bin[4], const float acc[kGradParams])
+{
+    double* row = lg + mi * kSpecGradParams;
+    for (int k = 0; k < 4; ++k) {
+        if (acc[k] != 0.0f) {
+            atomicAdd(&row[bin[k]], static_cast<double>(acc[k]));
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (acc[4 + k] != 0.0f) {
+            atomicAdd(&row[kSpecRough + k], static_cast<double>(acc[4 + k]));
         }
     }
 }

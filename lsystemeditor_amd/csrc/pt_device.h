@@ -52,6 +52,7 @@ PT_DEV S4 mk4(float x, float y, float z, float w) { return S4{x, y, z, w}; }
 PT_DEV S4 add(S4 a, S4 b) { return mk4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 PT_DEV S4 scale(S4 v, float s) { return mk4(v.x * s, v.y * s, v.z * s, v.w * s); }
 PT_DEV S4 mul(S4 a, S4 b) { return mk4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+PT_DEV S4 sub(S4 a, S4 b) { return mk4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
 
 template <class C> struct ColN;
 template <> struct ColN<V3> {

@@ -1019,13 +1019,14 @@ PT_DEV void shade_items(const MeshAccelSceneGpu* scene, const RenderParamsGpu* p
             newFlags = (wantExt ? kFlagExt : 0u) | (wantEnv ? kFlagEnv : 0u) | (wantEmis ? kFlagEmis : 0u) |
                        (reach ? kFlagReach : 0u);
             if (newFlags == 0u) {
-                if constexpr (!kSpectral) {
-                    if (capture != nullptr && sample < static_cast<uint32_t>(captureSpp)) {
-                        // unclamped radiance of (pixel, sample) for the adjoint
-                        const size_t slot = 3 * ((static_cast<size_t>(pixelOffset) + pixel) * captureSpp + sample);
-                        capture[slot + 0] = L.x;
-                        capture[slot + 1] = L.y;
-                        capture[slot + 2] = L.z;
+                if (capture != nullptr && sample < static_cast<uint32_t>(captureSpp)) {
+                    // unclamped radiance of (pixel, sample) for the adjoint (RGB, or the four
+                    // hero wavelengths)
+                    const size_t slot =
+                        ColN<C>::n * ((static_cast<size_t>(pixelOffset) + pixel) * captureSpp + sample);
+#pragma unroll
+                    for (int k = 0; k < ColN<C>::n; ++k) {
+                        capture[slot + k] = cget(L, k);
                     }
                 }
                 const V3 r = finish_sample(pol, L);
@@ -1135,58 +1136,116 @@ __global__ __launch_bounds__(kWfBlock) void wf_shade_hero_kernel(PT_SHADE_PARAMS
 // row of a partial-sum buffer, reduced once per adjoint).
 // ----------------------------------------------------------------------------------------
 struct AdjointState {
-    float *gx, *gy, *gz;             // dLoss/dRadiance of the path's sample (after the clamp VJP)
+    float *gx, *gy, *gz, *gw;        // per colour component k: dLoss/dRadiance_k of the path's sample
+                                     // (RGB: after the clamp VJP; hero wavelengths: through the
+                                     // conversion to linear sRGB as well; gw is the fourth wavelength's)
     float* pe[kGradParams];          // pending terms of the previous vertex's environment NEE
     float* pm[kGradParams];          // pending terms of its emissive NEE
     int* pmi;                        // that vertex's material (-1: nothing pending)
 };
 
-// gw and R = C for the paths of a group (slot p = generation order).
+// Loss weights and R = C for the paths of a group (slot p = generation order). RGB: C is the
+// sample's radiance and the weights the clamp VJP of dLoss/dPixel / spp. Hero wavelengths: C is
+// the radiance at the four wavelengths, and wavelength k's weight is the clamp VJP contracted with
+// its linear-sRGB response, 0.25 cie(lambda_k) / (white * pdf) (zero where the conversion zeroes
+// the wavelength: L_k <= 0 or not finite).
+template <class Pol>
 __global__ __launch_bounds__(kWfBlock) void wf_adjoint_init_kernel(PathState S, AdjointState A,
                                                                    const int* __restrict__ countPtr,
                                                                    const float* __restrict__ gradImg,
                                                                    const float* __restrict__ radiance, int spp,
-                                                                   float invSpp, const BatchScene* __restrict__ tab)
+                                                                   float invSpp, const BatchScene* __restrict__ tab,
+                                                                   SpecModel sm)
 {
+    using C = typename Pol::Col;
+    constexpr int n = ColN<C>::n;
     const int p = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (p >= *countPtr) {
         return;
     }
     const uint32_t px = S.pixel(p) + (tab != nullptr ? static_cast<uint32_t>(tab[S.scene(p)].pixelOffset) : 0u);
-    const size_t q = 3 * (static_cast<size_t>(px) * spp + S.sample(p));
-    const V3 C = mk(radiance[q], radiance[q + 1], radiance[q + 2]);
+    const size_t q = n * (static_cast<size_t>(px) * spp + S.sample(p));
     const V3 gs = mk(gradImg[3 * px] * invSpp, gradImg[3 * px + 1] * invSpp, gradImg[3 * px + 2] * invSpp);
-    const V3 gw = clamp_vjp(C, gs);
-    A.gx[p] = gw.x;
-    A.gy[p] = gw.y;
-    A.gz[p] = gw.z;
+    bool any = false;
+    if constexpr (n == 3) {
+        const V3 Cs = mk(radiance[q], radiance[q + 1], radiance[q + 2]);
+        const V3 gw = clamp_vjp(Cs, gs);
+        A.gx[p] = gw.x;
+        A.gy[p] = gw.y;
+        A.gz[p] = gw.z;
+        st_col(S, kPfLx, p, Cs);
+        any = gw.x != 0.0f || gw.y != 0.0f || gw.z != 0.0f;
+    } else {
+        const S4 L = mk4(radiance[q], radiance[q + 1], radiance[q + 2], radiance[q + 3]);
+        const HeroColors pol = hero_colors(sm, S.hero(p));
+        const V3 gw = clamp_vjp_spectral(pol.to_rgb(L), gs);
+        const float k0 = 0.25f / sp_max(1.0f / sm.lambdaRange, 1.0e-8f);
+        const float wr = gw.x / sp_max(sm.whiteR, 1.0e-8f), wg = gw.y / sp_max(sm.whiteG, 1.0e-8f),
+                    wb = gw.z / sp_max(sm.whiteB, 1.0e-8f);
+        float g[4];
+        for (int k = 0; k < 4; ++k) {
+            const float Lk = cget(L, k);
+            g[k] = 0.0f;
+            if (isfinite(Lk) && Lk > 0.0f) {
+                float rr, rg, rb;
+                cie_rgb_response(cget(pol.lambda, k), rr, rg, rb);
+                g[k] = k0 * (wr * rr + wg * rg + wb * rb);
+            }
+            any = any || g[k] != 0.0f;
+        }
+        A.gx[p] = g[0];
+        A.gy[p] = g[1];
+        A.gz[p] = g[2];
+        A.gw[p] = g[3];
+        st_col(S, kPfLx, p, L);
+    }
     A.pmi[p] = -1;
-    S.Lx(p) = C.x;
-    S.Ly(p) = C.y;
-    S.Lz(p) = C.z;
-    if (gw.x == 0.0f && gw.y == 0.0f && gw.z == 0.0f) {
+    if (!any) {
         S.flags(p) = 0u;  // nothing to differentiate: the path stops at its first shade
     }
 }
 
-PT_DEV V3 safe_ratio(V3 g, V3 x, V3 f)
+template <class C>
+PT_DEV C safe_ratio(const C& g, const C& x, const C& f)
 {
-    return mk(f.x != 0.0f ? g.x * x.x / f.x : 0.0f, f.y != 0.0f ? g.y * x.y / f.y : 0.0f,
-              f.z != 0.0f ? g.z * x.z / f.z : 0.0f);
+    C r;
+#pragma unroll
+    for (int k = 0; k < ColN<C>::n; ++k) {
+        cset(r, k, cget(f, k) != 0.0f ? cget(g, k) * cget(x, k) / cget(f, k) : 0.0f);
+    }
+    return r;
 }
 
-// BATCH as in shade_items; a scene's material m sums into gradient row gradRow + m, and
-// materialCount is the batch's row count.
-template <bool BATCH>
+// A vertex's BSDF record (RGB: r, g, b, roughness, metallic, specular, emission; hero
+// wavelengths: the base colour at the four wavelengths, roughness, metallic, specular) into
+// gradient row mi.
+PT_DEV void flush_record(double* lg, int mi, const RgbColors&, const float acc[kGradParams]) { flush(lg, mi, acc); }
+PT_DEV void flush_record(double* lg, int mi, const HeroColors& pol, const float acc[kGradParams])
+{
+    flush_spectral(lg, mi, pol.bin, acc);
+}
+
+// Pol: RgbColors (7 parameters per material) or HeroColors (kSpecGradParams: the binned
+// reflectance and emission, roughness, metallic, specular, emission scale; a material's
+// reflectance bins are parameters where it has binned reflectance, its emission bins and scale
+// where it has binned emission). BATCH as in shade_items; a scene's material m sums into gradient
+// row gradRow + m, and materialCount is the batch's row count.
+template <class Pol, bool BATCH>
 __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
     const MeshAccelSceneGpu* scene, const RenderParamsGpu* params, const EnvironmentMapGpu* env, PathState I,
     PathState O, AdjointState AI, AdjointState AO, Paths P, int bounce, int maxDepth, const int4* __restrict__ inA,
     const float2* __restrict__ inB, const int* __restrict__ inCountPtr, int* extOut, int4* __restrict__ outA,
-    int* counters, int materialCount, int copies, double* __restrict__ partial, const BatchScene* __restrict__ tab)
+    int* counters, int materialCount, int copies, double* __restrict__ partial, const BatchScene* __restrict__ tab,
+    SpecModel sm)
 {
+    using C = typename Pol::Col;
+    constexpr int n = ColN<C>::n;
+    constexpr bool kSpectral = n == 4;
+    constexpr int kParams = kSpectral ? kSpecGradParams : kGradParams;
+    const C* ct = nullptr;
     __shared__ BlockAppendLds appendLds;
     extern __shared__ double lgAll[];
-    const int perCopy = materialCount * kGradParams;
+    const int perCopy = materialCount * kParams;
     for (int k = static_cast<int>(threadIdx.x); k < copies * perCopy; k += kWfBlock) {
         lgAll[k] = 0.0;
     }
@@ -1198,21 +1257,21 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
         scene1 = *scene;
         params1 = *params;
     }
-    const int n = *inCountPtr;
+    const int nItems = *inCountPtr;
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    const int trips = (n + stride - 1) / stride;
+    const int trips = (nItems + stride - 1) / stride;
     for (int it = 0; it < trips; ++it) {
         const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + it * stride;
-        const bool live = i < n;
+        const bool live = i < nItems;
         bool wantExt = false, wantEnv = false, wantEmis = false;
         V3 envO = mk(0.0f, 0.0f, 0.0f), envD = envO, emO = envO, emD = envO, nextO = envO, nextD = envO;
-        V3 R = envO, T = envO, ce = envO, cm = envO, gw = envO;
+        C R = ColN<C>::splat(0.0f), T = R, ce = R, cm = R, g = R;
         float e1[kGradParams], e2[kGradParams];
         for (int k = 0; k < kGradParams; ++k) {
             e1[k] = 0.0f;
             e2[k] = 0.0f;
         }
-        float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f;
+        float nextTmin = 0.0f, nextEta = 1.0f, nextPdf = 0.0f, hero = 0.0f;
         uint32_t pixel = 0u, sample = 0u, seed = 0u, slot0 = 0u, draws = 0u, newFlags = 0u, sid = 0u;
         int pendMat = -1;
         auto body = [&](const MeshAccelSceneGpu& s, const RenderParamsGpu& prm, const EnvironmentMapGpu* env,
@@ -1225,8 +1284,14 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             sample = I.sample(i);
             seed = I.seed(i);
             slot0 = I.slot0(i);
-            gw = mk(AI.gx[i], AI.gy[i], AI.gz[i]);
-            R = mk(I.Lx(i), I.Ly(i), I.Lz(i));
+            if constexpr (kSpectral) {
+                hero = I.hero(i);
+                g = mk4(AI.gx[i], AI.gy[i], AI.gz[i], AI.gw[i]);
+            } else {
+                g = mk(AI.gx[i], AI.gy[i], AI.gz[i]);
+            }
+            const Pol pol = make_colors(hero, sm, static_cast<const Pol*>(nullptr));
+            R = ld_col(I, kPfLx, i, ct);
             // the previous vertex's next-event terms, in the forward's order (env, then emissive)
             const int pmi = AI.pmi[i];
             float prev[kGradParams];
@@ -1234,7 +1299,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 prev[k] = 0.0f;
             }
             if ((flags & kFlagEnv) != 0u && I.occ(i, 0) == 0u) {
-                R = sub(R, mk(I.cex(i), I.cey(i), I.cez(i)));
+                R = sub(R, ld_col(I, kPfCex, i, ct));
                 if (pmi >= 0) {
                     for (int k = 0; k < kGradParams; ++k) {
                         prev[k] += AI.pe[k][i];
@@ -1242,7 +1307,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 }
             }
             if ((flags & kFlagEmis) != 0u && I.occ(i, 1) == 0u) {
-                R = sub(R, mk(I.cmx(i), I.cmy(i), I.cmz(i)));
+                R = sub(R, ld_col(I, kPfCmx, i, ct));
                 if (pmi >= 0) {
                     for (int k = 0; k < kGradParams; ++k) {
                         prev[k] += AI.pm[k][i];
@@ -1250,7 +1315,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 }
             }
             if (pmi >= 0) {
-                flush(lg, pmi, prev);
+                flush_record(lg, pmi, pol, prev);
             }
             const uint32_t tri = static_cast<uint32_t>(ha.y);
             if ((flags & kFlagExt) != 0u && tri != 0xFFFFFFFFu && bounce < depthLimit) {
@@ -1264,7 +1329,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 finish_hit(s, ray, tri, hb.x, hb.y, h);
                 Rng rng;
                 rng_resume(rng, static_cast<int>(pixel), static_cast<int>(sample), seed, I.draws(i));
-                T = mk(I.Tx(i), I.Ty(i), I.Tz(i));
+                T = ld_col(I, kPfTx, i, ct);
                 const V3 pos = add(origin, scale(dir, t));
                 Ctx c;
                 c.n = h.normal;
@@ -1275,25 +1340,47 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 c.w = lobe_weights(c.m);
                 c.eta = I.eta(i);
                 c.flags = 0;
+                const C base = pol.base(c.m);
+                const C rad = c.w.subsurface != 0.0f ? pol.radii(c.m) : ColN<C>::splat(0.0f);
                 int gmi = -1;
                 Live lm{false, false, false, false, false, false, false};
                 if (valid && gradRow + static_cast<int>(mi) < materialCount) {
                     gmi = gradRow + static_cast<int>(mi);
                     lm = live_of(s.materials[mi], c.m, s);
+                    if constexpr (kSpectral) {
+                        lm.rgb = c.m.spec != nullptr && c.m.spec[0] >= 0.0f;  // the reflectance bins
+                    }
                 }
                 float acc[kGradParams];
                 for (int k = 0; k < kGradParams; ++k) {
                     acc[k] = 0.0f;
                 }
                 // emission
-                R = sub(R, mul(T, mk(c.m.r * c.m.emission, c.m.g * c.m.emission, c.m.b * c.m.emission)));
-                if (lm.emis) {
-                    acc[6] += gw.x * T.x * c.m.r + gw.y * T.y * c.m.g + gw.z * T.z * c.m.b;
-                }
-                if (lm.rgb) {
-                    acc[0] += gw.x * T.x * c.m.emission;
-                    acc[1] += gw.y * T.y * c.m.emission;
-                    acc[2] += gw.z * T.z * c.m.emission;
+                R = sub(R, mul(T, pol.emission(c.m)));
+                if constexpr (kSpectral) {
+                    if (gmi >= 0 && c.m.spec != nullptr && c.m.spec[kPtSpectrumBins] >= 0.0f) {
+                        double* row = lg + gmi * kSpecGradParams;
+                        float dScale = 0.0f;
+                        for (int k = 0; k < 4; ++k) {
+                            const float gt = cget(g, k) * cget(T, k);
+                            if (gt != 0.0f) {
+                                atomicAdd(&row[kPtSpectrumBins + pol.bin[k]], static_cast<double>(gt * c.m.emission));
+                                dScale += gt * c.m.spec[kPtSpectrumBins + pol.bin[k]];
+                            }
+                        }
+                        if (lm.emis && dScale != 0.0f) {
+                            atomicAdd(&row[kSpecEmission], static_cast<double>(dScale));
+                        }
+                    }
+                } else {
+                    if (lm.emis) {
+                        acc[6] += g.x * T.x * c.m.r + g.y * T.y * c.m.g + g.z * T.z * c.m.b;
+                    }
+                    if (lm.rgb) {
+                        acc[0] += g.x * T.x * c.m.emission;
+                        acc[1] += g.y * T.y * c.m.emission;
+                        acc[2] += g.z * T.z * c.m.emission;
+                    }
                 }
                 // environment NEE
                 if (!skips_env_nee(c.m)) {
@@ -1304,12 +1391,12 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                     if (lightPdf > kMinPdf) {
                         const float cosT = max2(0.0f, dot(c.n, wi));
                         if (cosT > 0.0f) {
-                            const V3 f = brdf_eval(c, wi);
+                            const C f = brdf_eval_t(c, wi, base, rad);
                             const float mis = mis_balance(lightPdf, brdf_pdf(c, wi));
-                            const V3 w = scale(env_eval(env, prm, wi), mis * cosT / lightPdf);
+                            const C w = scale(pol.radiance(env_eval(env, prm, wi)), mis * cosT / lightPdf);
                             ce = mul(T, mul(f, w));
                             if (gmi >= 0) {
-                                brdf_grad(c, wi, mul(gw, mul(T, w)), lm, e1);
+                                brdf_grad_t(c, wi, base, mul(g, mul(T, w)), lm, e1, e1 + n);
                             }
                             wantEnv = true;
                             envO = add(pos, scale(c.n, kShadowBias));
@@ -1317,14 +1404,16 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                         }
                     }
                 }
-                // emissive-triangle NEE
+                // emissive-triangle NEE (under the reference light sampling the light itself nearly
+                // always blocks it, so the light's own emission derivative is not taken here)
                 if (s.emissiveTriangleCount != 0u) {
                     const float uTri = rand01(rng);
                     const float u1 = rand01(rng);
                     const float u2 = rand01(rng);
                     V3 lp, ln, lr;
                     float areaPdf = 0.0f;
-                    if (sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf)) {
+                    int entry = 0;
+                    if (sample_emissive(s, uTri, u1, u2, lp, ln, lr, areaPdf, &entry)) {
                         const V3 toL = sub(lp, pos);
                         const float d2 = dot(toL, toL);
                         if (d2 > kMinPdf) {
@@ -1334,12 +1423,12 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                             if (cosT > 0.0f && cosL > 0.0f) {
                                 const float lightPdf = areaPdf * d2 / cosL;
                                 if (lightPdf > kMinPdf) {
-                                    const V3 f = brdf_eval(c, wi);
+                                    const C f = brdf_eval_t(c, wi, base, rad);
                                     const float mis = mis_balance(lightPdf, brdf_pdf(c, wi));
-                                    const V3 w = scale(lr, mis * cosT / lightPdf);
+                                    const C w = scale(pol.light(s, entry, lr), mis * cosT / lightPdf);
                                     cm = mul(T, mul(f, w));
                                     if (gmi >= 0) {
-                                        brdf_grad(c, wi, mul(gw, mul(T, w)), lm, e2);
+                                        brdf_grad_t(c, wi, base, mul(g, mul(T, w)), lm, e2, e2 + n);
                                     }
                                     wantEmis = true;
                                     emO = add(pos, scale(wi, kShadowBias));
@@ -1356,30 +1445,31 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 const Sample smp = brdf_sample(c, u1, u2);
                 bool cont = smp.valid && smp.pdf > kMinPdf;
                 if (cont) {
-                    const V3 f = brdf_eval(c, smp.dir);
+                    const C f = brdf_eval_t(c, smp.dir, base, rad);
                     if (gmi >= 0) {
-                        brdf_grad(c, smp.dir, safe_ratio(gw, R, f), lm, acc);
+                        brdf_grad_t(c, smp.dir, base, safe_ratio(g, R, f), lm, acc, acc + n);
                         if (wantEnv) {
-                            float g[kGradParams] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                            brdf_grad(c, smp.dir, safe_ratio(gw, ce, f), lm, g);
+                            float gg[kGradParams] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                            brdf_grad_t(c, smp.dir, base, safe_ratio(g, ce, f), lm, gg, gg + n);
                             for (int k = 0; k < kGradParams; ++k) {
-                                e1[k] -= g[k];
+                                e1[k] -= gg[k];
                             }
                         }
                         if (wantEmis) {
-                            float g[kGradParams] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                            brdf_grad(c, smp.dir, safe_ratio(gw, cm, f), lm, g);
+                            float gg[kGradParams] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                            brdf_grad_t(c, smp.dir, base, safe_ratio(g, cm, f), lm, gg, gg + n);
                             for (int k = 0; k < kGradParams; ++k) {
-                                e2[k] -= g[k];
+                                e2[k] -= gg[k];
                             }
                         }
                     }
-                    T = apply_throughput(T, c, smp, f);
-                    T = subsurface_tier_b(T, c.m, smp, rng);
+                    T = apply_throughput_t(T, c, smp, f, base);
+                    T = subsurface_tier_b_t(T, c.m, base, smp, rng);
                     if ((c.flags & 16) != 0) {
-                        T = smp.transmitted ? mul(T, mk(0.2f, 1.0f, 0.2f)) : mul(T, mk(1.0f, 0.2f, 0.2f));
+                        T = smp.transmitted ? mul(T, pol.reflectance(mk(0.2f, 1.0f, 0.2f)))
+                                            : mul(T, pol.reflectance(mk(1.0f, 0.2f, 0.2f)));
                     }
-                    const float tl = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+                    const float tl = pol.lum(T);
                     if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
                         cont = false;
                     }
@@ -1393,7 +1483,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                     }
                 }
                 if (gmi >= 0) {
-                    flush(lg, gmi, acc);
+                    flush_record(lg, gmi, pol, acc);
                     if (wantEnv || wantEmis) {
                         pendMat = gmi;
                     }
@@ -1436,25 +1526,23 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
             if constexpr (BATCH) {
                 O.scene(sh) = sid;
             }
-            O.Lx(sh) = R.x;
-            O.Ly(sh) = R.y;
-            O.Lz(sh) = R.z;
-            AO.gx[sh] = gw.x;
-            AO.gy[sh] = gw.y;
-            AO.gz[sh] = gw.z;
+            st_col(O, kPfLx, sh, R);
+            AO.gx[sh] = cget(g, 0);
+            AO.gy[sh] = cget(g, 1);
+            AO.gz[sh] = cget(g, 2);
+            if constexpr (kSpectral) {
+                O.hero(sh) = hero;
+                AO.gw[sh] = cget(g, 3);
+            }
             AO.pmi[sh] = pendMat;
             if (wantEnv) {
-                O.cex(sh) = ce.x;
-                O.cey(sh) = ce.y;
-                O.cez(sh) = ce.z;
+                st_col(O, kPfCex, sh, ce);
                 for (int k = 0; k < kGradParams; ++k) {
                     AO.pe[k][sh] = e1[k];
                 }
             }
             if (wantEmis) {
-                O.cmx(sh) = cm.x;
-                O.cmy(sh) = cm.y;
-                O.cmz(sh) = cm.z;
+                st_col(O, kPfCmx, sh, cm);
                 for (int k = 0; k < kGradParams; ++k) {
                     AO.pm[k][sh] = e2[k];
                 }
@@ -1470,9 +1558,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_adjoint_shade_kernel(
                 O.eta(sh) = nextEta;
                 O.pdf(sh) = nextPdf;
                 O.draws(sh) = draws;
-                O.Tx(sh) = T.x;
-                O.Ty(sh) = T.y;
-                O.Tz(sh) = T.z;
+                st_col(O, kPfTx, sh, T);
             }
         }
         if (wantEnv) {
@@ -2308,6 +2394,7 @@ bool ensure_adjoint(int cap, int pixels, size_t partialDoubles)
             A.gx = carve<float>(cur, cap);
             A.gy = carve<float>(cur, cap);
             A.gz = carve<float>(cur, cap);
+            A.gw = carve<float>(cur, cap);
             for (int k = 0; k < kGradParams; ++k) {
                 A.pe[k] = carve<float>(cur, cap);
                 A.pm[k] = carve<float>(cur, cap);
@@ -2351,14 +2438,15 @@ __host__ __device__ inline int resolve_depth(int maxPathDepth)
     return maxPathDepth > 0 ? maxPathDepth : (maxPathDepth == 0 ? 256 : 32);
 }
 
-// Renders the samples of seeds firstSeed.. and records their per-sample radiance into the
-// adjoint's grow-only buffer (one scene, or the batch of tab); returns the radiance or null.
+// Renders the samples of seeds firstSeed.. and records their per-sample radiance (RGB, or the
+// four hero wavelengths in spectral mode) into the adjoint's grow-only buffer (one scene, or the
+// batch of tab); returns the radiance or null.
 const float* adjoint_primal(int width, int height, int pixels, int spp, int maxDepth, const CameraGpu* d_camera,
                             const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
                             const RenderParamsGpu* d_params, unsigned int firstSeed, hipStream_t stream,
                             const BatchScene* tab, int tabCount, bool batchWide)
 {
-    const size_t nRad = static_cast<size_t>(pixels) * spp * 3;
+    const size_t nRad = static_cast<size_t>(pixels) * spp * (pt_spectral_enabled() ? 4 : 3);
     const size_t bytes = nRad * sizeof(float) + static_cast<size_t>(pixels) * (16 + 4 + 4 + 4 + 1) + 6 * 256;
     if (!ensure_adjoint_tmp(bytes)) {
         return nullptr;
@@ -2388,7 +2476,7 @@ const float* adjoint_primal(int width, int height, int pixels, int spp, int maxD
 }
 
 // The replay of every sample (seeds firstSeed..) against the primal radiance: gradient rows
-// (7 doubles each) summed into d_grad.
+// (7 doubles each; kSpecGradParams in spectral mode) summed into d_grad.
 bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels, int spp, int maxDepth,
                     const CameraGpu* d_camera, const MeshAccelSceneGpu* d_scene, const EnvironmentMapGpu* d_env,
                     const RenderParamsGpu* d_params, int gradRows, const float* radiance, double* d_grad,
@@ -2404,7 +2492,9 @@ bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels,
     // paths per group: 16M, or as many as a larger reserved workspace holds
     const int target = std::max(1 << 24, gWs.capacity);
     const int cap = std::max(pixels, std::min(target, spp * pixels));
-    const int perCopy = gradRows * kGradParams;
+    const bool spec = pt_spectral_enabled();
+    const SpecModel sm = pt_spectral_model();
+    const int perCopy = gradRows * (spec ? kSpecGradParams : kGradParams);
     if (!ensure_workspace(cap, pixels, maxDepth, spillThreads) ||
         !ensure_adjoint(cap, pixels, static_cast<size_t>(persistentBlocks) * perCopy)) {
         return false;
@@ -2426,7 +2516,10 @@ bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels,
                        reinterpret_cast<uint32_t*>(gAd.conv), (pixels + 3) / 4, 0u);
     hipLaunchKernelGGL(wf_zero_f64_kernel, dim3(blocks_for(static_cast<long long>(persistentBlocks) * perCopy)),
                        dim3(kWfBlock), 0, stream, gAd.partial, static_cast<long long>(persistentBlocks) * perCopy);
-    const auto shadeKernel = tab != nullptr ? wf_adjoint_shade_kernel<true> : wf_adjoint_shade_kernel<false>;
+    const auto initKernel = spec ? wf_adjoint_init_kernel<HeroColors> : wf_adjoint_init_kernel<RgbColors>;
+    const auto shadeKernel =
+        tab != nullptr ? (spec ? wf_adjoint_shade_kernel<HeroColors, true> : wf_adjoint_shade_kernel<RgbColors, true>)
+                       : (spec ? wf_adjoint_shade_kernel<HeroColors, false> : wf_adjoint_shade_kernel<RgbColors, false>);
     for (int passBase = 0; passBase < spp; passBase += group) {
         const int g = std::min(group, spp - passBase);
         hipLaunchKernelGGL(wf_fill_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, gAd.counts, pixels,
@@ -2435,17 +2528,16 @@ bool adjoint_replay(const float* d_gradImage, int width, int height, int pixels,
                            counterInts);
         hipLaunchKernelGGL(wf_gen_kernel, dim3(blocks_for(pixels)), dim3(kWfBlock), 0, stream, list, count,
                            gAd.counts, gAd.conv, d_camera, d_params, width, height, firstSeed, passBase, spp, g, P,
-                           false, 0, tab, tabCount);
-        hipLaunchKernelGGL(wf_adjoint_init_kernel, dim3(blocks_for(static_cast<long long>(g) * pixels)),
-                           dim3(kWfBlock), 0, stream, P.S[0], gAd.A[0], &P.counters[3], d_gradImage, radiance, spp,
-                           invSpp, tab);
+                           spec, 0, tab, tabCount);
+        hipLaunchKernelGGL(initKernel, dim3(blocks_for(static_cast<long long>(g) * pixels)), dim3(kWfBlock), 0,
+                           stream, P.S[0], gAd.A[0], &P.counters[3], d_gradImage, radiance, spp, invSpp, tab, sm);
         launch_bounces_with(
             stream, P, d_scene, maxDepth, static_cast<long long>(g) * pixels, persistentBlocks,
             [&](int b, const int* shadeCount, int* cOut, unsigned traceBlocks) {
                 hipLaunchKernelGGL(shadeKernel, dim3(traceBlocks), dim3(kWfBlock), lds, stream, d_scene, d_params,
                                    d_env, P.S[b & 1], P.S[(b + 1) & 1], gAd.A[b & 1], gAd.A[(b + 1) & 1], P, b,
                                    maxDepth, P.hitA[b & 1], P.hitB[b & 1], shadeCount, P.extQ[(b + 1) & 1],
-                                   P.hitA[(b + 1) & 1], cOut, perCopy / kGradParams, copies, gAd.partial, tab);
+                                   P.hitA[(b + 1) & 1], cOut, gradRows, copies, gAd.partial, tab, sm);
             },
             tab, wide, kLightReference);
     }

@@ -111,6 +111,83 @@ class DiffRenderer:
         return live
 
 
+SPECTRAL_PARAMS = abi.SPECTRUM_WORDS + 4  # kPtSpectrumGradParams
+SPECTRAL_SCALAR_WORDS = (3, 4, 14, 9)       # roughness, metallic, specular, emission
+
+
+class SpectralDiffRenderer(DiffRenderer):
+    """Fixed-spp hero-wavelength render of a scene with binned spectra: the parameters of a
+    material are a row of 36 -- its 16 reflectance bins, 16 emission bins (ptSceneSetSpectra),
+    roughness, metallic, specular and emission -- and the backward is
+    pathTracerAdjointRenderSpectral. Hero-wavelength sampling must be on
+    (abi.Spectral.enable) while it renders. Materials whose spectra row starts with a negative
+    value keep their rgb2spec colours; their bins are not parameters."""
+
+    def __init__(self, scene: abi.Scene, materials: np.ndarray, spectra: np.ndarray, camera: np.ndarray,
+                 params: np.ndarray, width: int, height: int, spp: int, env: abi.Environment | None = None,
+                 stream=None):
+        super().__init__(scene, materials, camera, params, width, height, spp, env=env, stream=stream)
+        self.spectra = np.ascontiguousarray(spectra, dtype=np.float32).reshape(self.material_count,
+                                                                                abi.SPECTRUM_WORDS).copy()
+        self.radiance = torch.empty((width * height * spp, 4), dtype=torch.float32, device="cuda")
+        scene.set_spectra(self.spectra)
+
+    def theta0(self) -> torch.Tensor:
+        """(materials, 36) float32: spectra, then roughness, metallic, specular, emission."""
+        t = np.concatenate([self.spectra, self.base[:, list(SPECTRAL_SCALAR_WORDS)]], axis=1)
+        return torch.from_numpy(np.ascontiguousarray(t))
+
+    def _tables(self, theta: torch.Tensor) -> tuple[np.ndarray, np.ndarray]:
+        t = theta.detach().to("cpu", torch.float32).numpy()
+        mats = self.base.copy()
+        mats[:, list(SPECTRAL_SCALAR_WORDS)] = t[:, abi.SPECTRUM_WORDS:]
+        sp = self.spectra.copy()
+        refl = sp[:, 0] >= 0.0
+        emis = sp[:, abi.SPECTRUM_BINS] >= 0.0
+        sp[refl, :abi.SPECTRUM_BINS] = t[refl, :abi.SPECTRUM_BINS]
+        sp[emis, abi.SPECTRUM_BINS:] = t[emis, abi.SPECTRUM_BINS:abi.SPECTRUM_WORDS]
+        return np.ascontiguousarray(mats), np.ascontiguousarray(sp)
+
+    def set_theta(self, theta: torch.Tensor) -> None:
+        self.radiance_valid = False
+        mats, sp = self._tables(theta)
+        abi.check(abi.lib().ptSceneUpdateMaterials(self.scene.handle, mats.ctypes.data, mats.shape[0],
+                                                   render._stream_handle(self.stream)), "ptSceneUpdateMaterials")
+        self.scene.set_spectra(sp, stream=render._stream_handle(self.stream))
+        self.uploaded = theta.detach().to("cpu", torch.float32).clone()
+
+    def adjoint(self, grad_image: torch.Tensor, reuse_forward: bool = True, seed_base: int = 0) -> torch.Tensor:
+        """dLoss/dtheta (materials, 36) float64 for dLoss/dImage = grad_image, as DiffRenderer.adjoint
+        (pathTracerAdjointRenderSpectral)."""
+        g = grad_image.detach().to("cuda", torch.float32).contiguous()
+        out = torch.zeros((self.material_count, SPECTRAL_PARAMS), dtype=torch.float64, device="cuda")
+        rad = self.radiance.data_ptr() if (reuse_forward and self.radiance_valid and seed_base == 0) else None
+        abi.check(abi.lib().pathTracerAdjointRenderSpectral(
+            g.data_ptr(), self.width, self.height, self.spp, self.inputs.camera.data_ptr(), self.scene.device,
+            self.inputs.env_ptr, self.inputs.params.data_ptr(), self.material_count, rad, out.data_ptr(),
+            seed_base + 1, render._stream_handle(self.stream)), "pathTracerAdjointRenderSpectral")
+        return out
+
+    def detached(self, eval_theta: torch.Tensor) -> torch.Tensor:
+        raise NotImplementedError("the spectral detached estimator is the oracle's (orc_render_detached_spectral)")
+
+    def live_mask(self) -> np.ndarray:
+        """(materials, 36) bool: the bins of binned spectra (reflectance of opaque materials,
+        emission of emitting ones), roughness / metallic / specular inside (0, 1), and the emission
+        scale of a material with binned emission."""
+        live = np.zeros((self.material_count, SPECTRAL_PARAMS), bool)
+        b = self.base
+        opaque = (np.clip(b[:, 5], 0, 1) == 0.0) & (np.clip(b[:, 8], 0, 1) == 0.0)
+        emits = b[:, 9] > 0.0
+        live[:, :abi.SPECTRUM_BINS] = ((self.spectra[:, 0] >= 0.0) & opaque)[:, None]
+        emis = (self.spectra[:, abi.SPECTRUM_BINS] >= 0.0) & emits
+        live[:, abi.SPECTRUM_BINS:abi.SPECTRUM_WORDS] = emis[:, None]
+        for q, wd in enumerate(SPECTRAL_SCALAR_WORDS[:3]):
+            live[:, abi.SPECTRUM_WORDS + q] = (b[:, wd] > 0.0) & (b[:, wd] < 1.0)
+        live[:, abi.SPECTRUM_WORDS + 3] = emis
+        return live
+
+
 def gradient_check(r: DiffRenderer, theta0: torch.Tensor, target: torch.Tensor, h: float = 1e-3) -> dict:
     """Adjoint gradient of L = sum (image - target)^2 at theta0 against central differences
     (step h) of the detached estimator, for every live parameter."""

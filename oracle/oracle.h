@@ -287,6 +287,18 @@ typedef struct {
     float white[3];      /* white normalisation */
     float lambdaMin, lambdaRange; /* hero-wavelength range */
 } OSpecModel;
+/* Differentiable hero-wavelength rendering (light mode 0): the spectral detached estimator's image
+ * (img3 float, or imgD double added into with mask3 selecting pixels) and its FP64 adjoint,
+ * ORC_SPECTRAL_GRAD_PARAMS per material: reflectance bins, emission bins, roughness, metallic,
+ * specular, emission scale. */
+#define ORC_SPECTRAL_GRAD_PARAMS 36
+void orc_render_detached_spectral(float* img3, double* imgD, const double* mask3, int width, int height,
+                                  const OCamera* cam, const OScene* scene, const OMaterial* evalMats,
+                                  const float* evalSpectra, const OEnv* env, const OParams* params,
+                                  const OSpecModel* model, int spp, uint32_t firstSeed, int threads);
+void orc_render_adjoint_spectral(const double* gradImg, int width, int height, const OCamera* cam,
+                                 const OScene* scene, const OEnv* env, const OParams* params,
+                                 const OSpecModel* model, int spp, uint32_t firstSeed, double* grad, int threads);
 float orc_cie_interp(int which, float lambdaNm); /* 0 x, 1 y, 2 z, 3 D65 */
 void orc_cie_rgb_response(float lambdaNm, float* out3);
 void orc_rgb2spec_coeffs(const OSpecModel* m, float r, float g, float b, float* out3);

@@ -162,6 +162,10 @@ def lib(variant: str = "parity"):
             "orc_brdf_grad": (None, [vp, Vec3, vp, vp, vp]),
             "orc_clamp_vjp": (None, [Vec3, vp, vp]),
             "orc_brdf_eval_f64": (None, [vp, Vec3, vp, vp]),
+            "orc_render_detached_spectral": (None, [vp, vp, vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, vp,
+                                                    C.POINTER(Env), vp, vp, C.c_int, C.c_uint32, C.c_int]),
+            "orc_render_adjoint_spectral": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), C.POINTER(Env), vp,
+                                                   vp, C.c_int, C.c_uint32, vp, C.c_int]),
             "orc_render_detached_f64_seeded": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, C.POINTER(Env),
                                                       vp, C.c_int, C.c_uint32, vp, C.c_int]),
             "orc_render_detached": (None, [vp, C.c_int, C.c_int, vp, C.POINTER(Scene), vp, C.POINTER(Env), vp, C.c_int,

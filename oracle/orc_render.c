@@ -1789,6 +1789,543 @@ static void ad_run(AdJob base, int threads, double* gradOut)
     }
 }
 
+/* ---- differentiable hero-wavelength rendering -----------------------------------------------
+ * orc_trace_path_spectral with detached sampling: directions, pdfs, MIS weights and Russian
+ * roulette from the primal parameters (the scene's materials and spectra), per-wavelength values
+ * (base colour, emission, BSDF values) from evalMats / evalSpectra. Parameters per material
+ * (ORC_SPECTRAL_GRAD_PARAMS): the reflectance bins, the emission bins, roughness, metallic,
+ * specular and the emission scale; the bins are parameters where the material has binned spectra
+ * (the base colour at lambda_k is bin(lambda_k), the emitted radiance emission x bin(lambda_k)).
+ * Wavelength k is the RGB model's first channel on a grey material (spec_vertex), so the RGB
+ * derivative cores apply with the weight vector (a_k, 0, 0): their r-derivative is the
+ * derivative with respect to the base colour at lambda_k. Light mode 0 only. */
+
+/* wi and the per-wavelength light weights Le_k * scale of the environment NEE (0: none) */
+static int spec_env_weights(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta,
+                            const OScene* s, const OEnv* env, const OParams* params, ORng* rng, OVec3* wiOut,
+                            float* w4)
+{
+    if (orc_skips_env_nee(m)) {
+        return 0;
+    }
+    const float u1 = orc_rand01(rng);
+    const float u2 = orc_rand01(rng);
+    float lightPdf = 0.0f;
+    const OVec3 wi = orc_env_sample(env, params, u1, u2, &lightPdf);
+    if (lightPdf <= K_MIN_PDF || occluded(p, n, wi, s)) {
+        return 0;
+    }
+    const float cosT = fmax2(0.0f, vdot(n, wi));
+    if (cosT <= 0.0f) {
+        return 0;
+    }
+    const OBrdfCtx rgb = make_ctx(n, wo, m, eta, 0);
+    const float mis = orc_mis_balance(lightPdf, orc_brdf_pdf(&rgb, wi));
+    const float sc = mis * cosT / lightPdf;
+    const OVec3 bg = orc_env_eval(env, params, wi);
+    for (int k = 0; k < 4; ++k) {
+        w4[k] = orc_rgb2spec_radiance(v->model, bg.x, bg.y, bg.z, v->lambda[k]) * sc;
+    }
+    *wiOut = wi;
+    return 1;
+}
+
+/* the same for the emissive NEE; the light's radiance comes from `values` (its materials and
+ * spectra), the light choice from s */
+static int spec_emissive_weights(const SpecVertex* v, OVec3 p, OVec3 n, OVec3 wo, const OMaterial* m, float eta,
+                                 const OScene* s, const OScene* values, ORng* rng, OVec3* wiOut, float* w4)
+{
+    if (s == NULL || s->emissiveCount == 0) {
+        return 0;
+    }
+    const float uTri = orc_rand01(rng);
+    const float u1 = orc_rand01(rng);
+    const float u2 = orc_rand01(rng);
+    OVec3 lp, ln, lr;
+    float areaPdf = 0.0f;
+    uint32_t lightTri = 0u;
+    if (!orc_light_sample_emissive_tri(s, uTri, u1, u2, &lp, &ln, &lr, &areaPdf, &lightTri)) {
+        return 0;
+    }
+    const OVec3 toL = vsub(lp, p);
+    const float d2 = vdot(toL, toL);
+    if (d2 <= K_MIN_PDF) {
+        return 0;
+    }
+    const OVec3 wi = vnorm(toL);
+    const float cosT = fmax2(0.0f, vdot(n, wi));
+    const float cosL = fmax2(0.0f, vdot(ln, vscale(wi, -1.0f)));
+    if (cosT <= 0.0f || cosL <= 0.0f || occluded_from(p, wi, s)) {
+        return 0;
+    }
+    const float lightPdf = areaPdf * d2 / cosL;
+    if (lightPdf <= K_MIN_PDF) {
+        return 0;
+    }
+    const OBrdfCtx rgb = make_ctx(n, wo, m, eta, 0);
+    const float mis = orc_mis_balance(lightPdf, orc_brdf_pdf(&rgb, wi));
+    const float sc = mis * cosT / lightPdf;
+    const OMaterial lm = orc_fetch_material(values, lightTri);
+    const float* lsp = material_spectrum(values, lightTri);
+    for (int k = 0; k < 4; ++k) {
+        const float Le = lsp != NULL && lsp[ORC_SPECTRUM_BINS] >= 0.0f
+                             ? lsp[ORC_SPECTRUM_BINS + v->bin[k]] * lm.emission
+                             : orc_rgb2spec_radiance(v->model, lm.r * lm.emission, lm.g * lm.emission,
+                                                     lm.b * lm.emission, v->lambda[k]);
+        w4[k] = Le * sc;
+    }
+    *wiOut = wi;
+    return 1;
+}
+
+/* g += the per-wavelength BSDF derivative with weights a4 at the grey contexts of ve */
+static void spec_brdf_grad(const SpecVertex* ve, OVec3 wi, const double* a4, LiveMask lm, double* g)
+{
+    for (int k = 0; k < 4; ++k) {
+        if (a4[k] == 0.0) {
+            continue;
+        }
+        const double a[3] = {a4[k], 0.0, 0.0};
+        double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        brdf_grad(&ve->ctx[k], wi, a, lm, out);
+        g[ve->bin[k]] += out[0];
+        g[2 * ORC_SPECTRUM_BINS + 0] += out[3];
+        g[2 * ORC_SPECTRUM_BINS + 1] += out[4];
+        g[2 * ORC_SPECTRUM_BINS + 2] += out[5];
+    }
+}
+
+/* L4 (float) and, with Ld4, the same radiance in double; with gw4 (dLoss/dL_k) and C4 (the primal
+ * radiance at the four wavelengths) the replay accumulates grad. */
+static void trace_path_ad_hero(OVec3 ro, OVec3 rd, const OScene* s, const OScene* values, const OParams* params,
+                               const OEnv* env, ORng* rng, const OSpecModel* model, const float* lambda4,
+                               const double* gw4, const float* C4, double* grad, float* L4, double* Ld4)
+{
+    for (int k = 0; k < 4; ++k) {
+        L4[k] = 0.0f;
+        if (Ld4 != NULL) {
+            Ld4[k] = 0.0;
+        }
+    }
+    const OHit first = trace(ro, rd, s, 0.0f, K_RAY_TMAX, 1);
+    if (!first.hit) {
+        const OVec3 bg = orc_env_eval(env, params, rd);
+        for (int k = 0; k < 4; ++k) {
+            L4[k] = orc_rgb2spec_radiance(model, bg.x, bg.y, bg.z, lambda4[k]);
+            if (Ld4 != NULL) {
+                Ld4[k] = L4[k];
+            }
+        }
+        return;
+    }
+    int maxDepth = 32, rrMin = 3, flags = 0;
+    if (params != NULL) {
+        maxDepth = params->maxPathDepth > 0 ? params->maxPathDepth : (params->maxPathDepth == 0 ? 256 : 32);
+        rrMin = params->russianRouletteMinDepth >= 0 ? params->russianRouletteMinDepth : 3;
+        flags = params->brdfDebugFlags;
+    }
+    float T[4] = {1.0f, 1.0f, 1.0f, 1.0f};   /* evaluated throughput */
+    float T0[4] = {1.0f, 1.0f, 1.0f, 1.0f};  /* primal throughput: Russian roulette and termination */
+    double Td[4] = {1.0, 1.0, 1.0, 1.0};
+    OVec3 origin = ro, dir = rd;
+    OHit hit = first;
+    float eta = 1.0f;
+    SpecVertex v, ve;
+    v.model = ve.model = model;
+    for (int k = 0; k < 4; ++k) {
+        v.lambda[k] = ve.lambda[k] = lambda4[k];
+        v.bin[k] = ve.bin[k] = spectrum_bin(model, lambda4[k]);
+    }
+    for (int depth = 0; depth < maxDepth; ++depth) {
+        const OVec3 p = vray(origin, dir, hit.t);
+        const OVec3 n = hit.normal;
+        const OVec3 wo = vscale(dir, -1.0f);
+        const OMaterial src = orc_fetch_material(s, hit.triangleIndex);
+        const OMaterial m = orc_resolve_material(&src, hit.uv, s);
+        int mi = -1;
+        const OMaterial srcE = fetch_from(s, values->materials, hit.triangleIndex, &mi);
+        const OMaterial mE = orc_resolve_material(&srcE, hit.uv, s);
+        const float* sp = material_spectrum(s, hit.triangleIndex);
+        const float* spE = material_spectrum(values, hit.triangleIndex);
+        spec_vertex(&v, &m, sp, n, wo, eta, 0);
+        spec_vertex(&ve, &mE, spE, n, wo, eta, 0);
+        LiveMask lm = live_mask(&srcE, &mE, s);
+        lm.rgb = spE != NULL && spE[0] >= 0.0f;
+        double* g = (grad != NULL && gw4 != NULL && mi >= 0) ? grad + ORC_SPECTRAL_GRAD_PARAMS * mi : NULL;
+
+        /* emission */
+        float Le[4];
+        spec_emission(&ve, &mE, spE, Le);
+        const int binnedE = spE != NULL && spE[ORC_SPECTRUM_BINS] >= 0.0f;
+        for (int k = 0; k < 4; ++k) {
+            L4[k] = L4[k] + T[k] * Le[k];
+            if (Ld4 != NULL) {
+                Ld4[k] += Td[k] * (binnedE ? (double)spE[ORC_SPECTRUM_BINS + ve.bin[k]] * mE.emission : (double)Le[k]);
+            }
+            if (g != NULL && binnedE) {
+                g[ORC_SPECTRUM_BINS + ve.bin[k]] += gw4[k] * T[k] * mE.emission;
+                if (lm.emis) {
+                    g[ORC_SPECTRAL_GRAD_PARAMS - 1] += gw4[k] * T[k] * spE[ORC_SPECTRUM_BINS + ve.bin[k]];
+                }
+            }
+        }
+        /* next-event terms: sampling with the primal vertex, values with the evaluated one */
+        OVec3 wi;
+        float w4[4];
+        for (int q = 0; q < 2; ++q) {
+            const int ok = q == 0 ? spec_env_weights(&v, p, n, wo, &m, eta, s, env, params, rng, &wi, w4)
+                                  : spec_emissive_weights(&v, p, n, wo, &m, eta, s, values, rng, &wi, w4);
+            if (!ok) {
+                continue;
+            }
+            double a4[4];
+            for (int k = 0; k < 4; ++k) {
+                const float f = orc_brdf_eval(&ve.ctx[k], wi).x;
+                L4[k] = L4[k] + T[k] * (f * w4[k]);
+                if (Ld4 != NULL) {
+                    double fd[3];
+                    brdf_value_f64(&ve.ctx[k], wi, lm, fd);
+                    Ld4[k] += Td[k] * fd[0] * w4[k];
+                }
+                a4[k] = g != NULL ? gw4[k] * T[k] * w4[k] : 0.0;
+            }
+            if (g != NULL) {
+                spec_brdf_grad(&ve, wi, a4, lm, g);
+            }
+        }
+        /* BSDF sampling with the primal RGB context */
+        const OBrdfCtx ctx = make_ctx(n, wo, &m, eta, flags);
+        const float u1 = orc_rand01(rng);
+        const float u2 = orc_rand01(rng);
+        const OSample smp = orc_brdf_sample(&ctx, u1, u2);
+        if (!smp.valid || smp.pdf <= K_MIN_PDF) {
+            break;
+        }
+        double a4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k) {
+            OBrdfCtx ck = v.ctx[k];
+            ck.debugFlags = flags;
+            OBrdfCtx cke = ve.ctx[k];
+            cke.debugFlags = flags;
+            const float f0 = orc_brdf_eval(&ck, smp.direction).x;
+            const float fE = orc_brdf_eval(&cke, smp.direction).x;
+            if (g != NULL && fE != 0.0f) {
+                /* radiance still to come after this vertex: linear in fE */
+                a4[k] = gw4[k] * ((double)C4[k] - L4[k]) / fE;
+            }
+            if (Ld4 != NULL) {
+                double fd[3];
+                brdf_value_f64(&ve.ctx[k], smp.direction, lm, fd);
+                Td[k] *= fd[0] * orc_apply_throughput(v3(1.0f, 1.0f, 1.0f), &cke, &smp, v3(1.0f, 1.0f, 1.0f)).x;
+            }
+            T0[k] = orc_apply_throughput(v3(T0[k], T0[k], T0[k]), &ck, &smp, v3(f0, f0, f0)).x;
+            T[k] = orc_apply_throughput(v3(T[k], T[k], T[k]), &cke, &smp, v3(fE, fE, fE)).x;
+        }
+        if (g != NULL) {
+            spec_brdf_grad(&ve, smp.direction, a4, lm, g);
+        }
+        /* tier-B walk (PathIntegratorRandCore.h:169-204) with the RGB radius */
+        if (smp.subsurfaceScatter && smp.subsurfaceInternalSteps > 0) {
+            const float maxR = orc_max_scatter_radius(&m);
+            if (maxR > 1.0e-5f) {
+                for (int step = 0; step < smp.subsurfaceInternalSteps; ++step) {
+                    (void)orc_rand01(rng);
+                    (void)orc_rand01(rng);
+                    const float u3 = orc_rand01(rng);
+                    const float mfp = maxR * (-orc_logf(fmax2(1.0e-8f, u3)));
+                    const float surv = orc_expf(-mfp / maxR);
+                    for (int k = 0; k < 4; ++k) {
+                        T0[k] = T0[k] * (v.grey[k].r * surv);
+                        T[k] = T[k] * (ve.grey[k].r * surv);
+                        Td[k] *= (double)(ve.grey[k].r * surv);
+                    }
+                }
+                const float ed = vlen(smp.subsurfaceExitOffset);
+                if (ed > 0.0f) {
+                    const float a = orc_expf(-ed / maxR);
+                    for (int k = 0; k < 4; ++k) {
+                        T0[k] = T0[k] * a;
+                        T[k] = T[k] * a;
+                        Td[k] *= a;
+                    }
+                }
+            }
+        }
+        if ((flags & (1 << 4)) != 0) {
+            const OVec3 tint = smp.transmitted ? v3(0.2f, 1.0f, 0.2f) : v3(1.0f, 0.2f, 0.2f);
+            for (int k = 0; k < 4; ++k) {
+                const float r = orc_rgb2spec_reflectance(model, tint.x, tint.y, tint.z, v.lambda[k]);
+                T0[k] = T0[k] * r;
+                T[k] = T[k] * r;
+                Td[k] *= r;
+            }
+        }
+        const float tl = 0.25f * (((T0[0] + T0[1]) + T0[2]) + T0[3]);
+        if (!isfinite(tl) || tl <= 0.0f || tl > 1.0e8f) {
+            break;
+        }
+        if (depth >= rrMin) {
+            const float surv = fmin2(0.95f, fmax2(tl, 0.05f));
+            if (orc_rand01(rng) > surv) {
+                break;
+            }
+            for (int k = 0; k < 4; ++k) {
+                T0[k] = T0[k] * (1.0f / surv);
+                T[k] = T[k] * (1.0f / surv);
+                Td[k] *= (double)(1.0f / surv);
+            }
+        }
+        eta = smp.nextMediumEta;
+        const float bias = fmax2(K_SHADOW_BIAS, K_REL_SHADOW_BIAS * hit.t);
+        const float sign = vdot(n, smp.direction) >= 0.0f ? 1.0f : -1.0f;
+        origin = vadd(p, vscale(n, sign * bias));
+        dir = smp.direction;
+        hit = trace(origin, dir, s, bias, K_RAY_TMAX, 1);
+        if (!hit.hit) {
+            const OVec3 bg = orc_env_eval(env, params, dir);
+            const float lpdf = orc_env_pdf(env, params, dir);
+            const float mis = orc_mis_balance(smp.pdf, lpdf);
+            for (int k = 0; k < 4; ++k) {
+                const float envL = orc_rgb2spec_radiance(model, bg.x, bg.y, bg.z, v.lambda[k]);
+                L4[k] = L4[k] + T[k] * (envL * mis);
+                if (Ld4 != NULL) {
+                    Ld4[k] += Td[k] * (double)(envL * mis);
+                }
+            }
+            break;
+        }
+    }
+}
+
+/* The spectral firefly clamp in double (non-finite -> 0, luminance above 100 scaled down,
+ * negative channels kept). */
+static void clamp_firefly_spectral_f64(const double r[3], double out[3])
+{
+    if (!isfinite(r[0]) || !isfinite(r[1]) || !isfinite(r[2])) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double lum = r[0] * 0.2126 + r[1] * 0.7152 + r[2] * 0.0722;
+    const double k = lum > 100.0 ? 100.0 / lum : 1.0;
+    for (int q = 0; q < 3; ++q) {
+        out[q] = r[q] * k;
+    }
+}
+
+typedef struct {
+    float* img;
+    double* imgD;
+    const double* mask;
+    const double* gradImg;
+    double* grad;
+    int w, h, spp, y0, y1, gradCount;
+    uint32_t firstSeed;
+    const OCamera* cam;
+    const OScene* s;
+    const OScene* values;
+    const OEnv* env;
+    const OParams* params;
+    const OSpecModel* model;
+} HeroJob;
+
+static void hero_rows(const HeroJob* j)
+{
+    const float pdf = 1.0f / j->model->lambdaRange;
+    for (int y = j->y0; y < j->y1; ++y) {
+        for (int x = 0; x < j->w; ++x) {
+            const int idx = y * j->w + x;
+            const double* gp = j->gradImg != NULL ? j->gradImg + 3 * idx : (j->mask != NULL ? j->mask + 3 * idx : NULL);
+            if (gp != NULL && gp[0] == 0.0 && gp[1] == 0.0 && gp[2] == 0.0) {
+                continue; /* pixel outside the loss */
+            }
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            for (int k = 0; k < j->spp; ++k) {
+                ORng rng;
+                orc_rng_init(&rng, idx, k, j->firstSeed + (uint32_t)k);
+                const float ju = orc_rand01(&rng);
+                const float jv = orc_rand01(&rng);
+                const float u = ((float)x + ju) / (float)j->w;
+                const float v = ((float)y + jv) / (float)j->h;
+                const float l1 = orc_rand01(&rng);
+                const float l2 = orc_rand01(&rng);
+                OVec3 ro, rd;
+                orc_camera_ray(j->cam, u, v, l1, l2, &ro, &rd);
+                float lambda[4];
+                orc_hero_lambdas(j->model, orc_rand01(&rng), lambda);
+                float L4[4];
+                if (j->gradImg == NULL) {
+                    double Ld[4];
+                    trace_path_ad_hero(ro, rd, j->s, j->values, j->params, j->env, &rng, j->model, lambda, NULL, NULL,
+                                       NULL, L4, j->imgD != NULL ? Ld : NULL);
+                    if (j->imgD != NULL) {
+                        double rgb[3] = {0.0, 0.0, 0.0}, c[3];
+                        for (int q = 0; q < 4; ++q) {
+                            if (!isfinite(Ld[q]) || Ld[q] <= 0.0) {
+                                continue;
+                            }
+                            float resp[3];
+                            orc_cie_rgb_response(lambda[q], resp);
+                            for (int ch = 0; ch < 3; ++ch) {
+                                rgb[ch] += 0.25 * (Ld[q] / fmax2(pdf, 1.0e-8f)) * resp[ch] /
+                                           fmax2(j->model->white[ch], 1.0e-8f);
+                            }
+                        }
+                        clamp_firefly_spectral_f64(rgb, c);
+                        for (int ch = 0; ch < 3; ++ch) {
+                            j->imgD[3 * idx + ch] += c[ch] / j->spp;
+                        }
+                    } else {
+                        const OVec3 c = orc_clamp_firefly_spectral(orc_hero_to_rgb(j->model, L4, lambda));
+                        const float inv = 1.0f / (float)(k + 1);
+                        acc[0] = (acc[0] * (float)k + c.x) * inv;
+                        acc[1] = (acc[1] * (float)k + c.y) * inv;
+                        acc[2] = (acc[2] * (float)k + c.z) * inv;
+                    }
+                } else {
+                    ORng replay = rng;
+                    float C4[4];
+                    trace_path_ad_hero(ro, rd, j->s, j->s, j->params, j->env, &rng, j->model, lambda, NULL, NULL, NULL,
+                                       C4, NULL);
+                    /* clamp VJP of the sample's linear sRGB, then each wavelength's response */
+                    const OVec3 Crgb = orc_hero_to_rgb(j->model, C4, lambda);
+                    const double gs[3] = {j->gradImg[3 * idx] / j->spp, j->gradImg[3 * idx + 1] / j->spp,
+                                          j->gradImg[3 * idx + 2] / j->spp};
+                    double gw[3] = {gs[0], gs[1], gs[2]};
+                    if (!isfinite(Crgb.x) || !isfinite(Crgb.y) || !isfinite(Crgb.z)) {
+                        gw[0] = gw[1] = gw[2] = 0.0;
+                    } else {
+                        const double cr[3] = {Crgb.x, Crgb.y, Crgb.z};
+                        const double lum = cr[0] * 0.2126 + cr[1] * 0.7152 + cr[2] * 0.0722;
+                        if (lum > 100.0) {
+                            const double wl[3] = {0.2126, 0.7152, 0.0722};
+                            const double gdr = gs[0] * cr[0] + gs[1] * cr[1] + gs[2] * cr[2];
+                            for (int ch = 0; ch < 3; ++ch) {
+                                gw[ch] = 100.0 / lum * (gs[ch] - wl[ch] * gdr / lum);
+                            }
+                        }
+                    }
+                    double gw4[4];
+                    int any = 0;
+                    for (int q = 0; q < 4; ++q) {
+                        gw4[q] = 0.0;
+                        if (isfinite(C4[q]) && C4[q] > 0.0f) {
+                            float resp[3];
+                            orc_cie_rgb_response(lambda[q], resp);
+                            for (int ch = 0; ch < 3; ++ch) {
+                                gw4[q] += 0.25 / fmax2(pdf, 1.0e-8f) * gw[ch] * resp[ch] /
+                                          fmax2(j->model->white[ch], 1.0e-8f);
+                            }
+                        }
+                        any = any || gw4[q] != 0.0;
+                    }
+                    if (any) {
+                        trace_path_ad_hero(ro, rd, j->s, j->s, j->params, j->env, &replay, j->model, lambda, gw4, C4,
+                                           j->grad, L4, NULL);
+                    }
+                }
+            }
+            if (j->img != NULL) {
+                j->img[3 * idx] = acc[0];
+                j->img[3 * idx + 1] = acc[1];
+                j->img[3 * idx + 2] = acc[2];
+            }
+        }
+    }
+}
+
+static void* hero_worker(void* arg)
+{
+    hero_rows((const HeroJob*)arg);
+    return NULL;
+}
+
+static void hero_run(HeroJob base, int threads, double* gradOut)
+{
+    threads = threads < 1 ? 1 : (threads > 64 ? 64 : threads);
+    pthread_t tid[64];
+    HeroJob jobs[64];
+    double* partial = NULL;
+    if (gradOut != NULL) {
+        partial = (double*)calloc((size_t)threads * (size_t)base.gradCount, sizeof(double));
+    }
+    for (int t = 0; t < threads; ++t) {
+        jobs[t] = base;
+        jobs[t].y0 = base.h * t / threads;
+        jobs[t].y1 = base.h * (t + 1) / threads;
+        jobs[t].grad = partial != NULL ? partial + (size_t)t * base.gradCount : NULL;
+        pthread_create(&tid[t], NULL, hero_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(tid[t], NULL);
+    }
+    if (partial != NULL) {
+        for (int t = 0; t < threads; ++t) {
+            for (int k = 0; k < base.gradCount; ++k) {
+                gradOut[k] += partial[(size_t)t * base.gradCount + k];
+            }
+        }
+        free(partial);
+    }
+}
+
+/* The spectral detached estimator's image: paths sampled with the scene (its materials and
+ * spectra), values from evalMats / evalSpectra (NULL: the scene's). img3 (float, running mean of
+ * the clamped linear sRGB, as the forward) or imgD (double, added into, mask3 selecting pixels as
+ * orc_render_detached_f64_seeded) -- one of them. */
+void orc_render_detached_spectral(float* img3, double* imgD, const double* mask3, int width, int height,
+                                  const OCamera* cam, const OScene* s, const OMaterial* evalMats,
+                                  const float* evalSpectra, const OEnv* env, const OParams* params,
+                                  const OSpecModel* model, int spp, uint32_t firstSeed, int threads)
+{
+    OScene values = *s;
+    if (evalMats != NULL) {
+        values.materials = evalMats;
+    }
+    if (evalSpectra != NULL) {
+        values.spectra = evalSpectra;
+    }
+    HeroJob j;
+    memset(&j, 0, sizeof(j));
+    j.img = imgD != NULL ? NULL : img3;
+    j.imgD = imgD;
+    j.mask = imgD != NULL ? mask3 : NULL;
+    j.w = width;
+    j.h = height;
+    j.spp = spp;
+    j.firstSeed = firstSeed;
+    j.cam = cam;
+    j.s = s;
+    j.values = &values;
+    j.env = env;
+    j.params = params;
+    j.model = model;
+    hero_run(j, threads, NULL);
+}
+
+/* The FP64 adjoint of the hero-wavelength render: grad[ORC_SPECTRAL_GRAD_PARAMS * materialCount]
+ * += sum over pixels and samples of gradImg . d(pixel)/d(theta) (gradImg = dLoss/dImage, 3
+ * doubles per pixel; sample k uses seed firstSeed + k). */
+void orc_render_adjoint_spectral(const double* gradImg, int width, int height, const OCamera* cam, const OScene* s,
+                                 const OEnv* env, const OParams* params, const OSpecModel* model, int spp,
+                                 uint32_t firstSeed, double* grad, int threads)
+{
+    HeroJob j;
+    memset(&j, 0, sizeof(j));
+    j.gradImg = gradImg;
+    j.w = width;
+    j.h = height;
+    j.spp = spp;
+    j.firstSeed = firstSeed;
+    j.cam = cam;
+    j.s = s;
+    j.values = s;
+    j.env = env;
+    j.params = params;
+    j.model = model;
+    j.gradCount = ORC_SPECTRAL_GRAD_PARAMS * (int)s->materialCount;
+    hero_run(j, threads, grad);
+}
+
 /* Fixed-spp image of the detached estimator: sample k of every pixel uses sample index k and
  * seed k + 1 (the adaptive schedule with minSamples = maxSamplesPerPixel = spp); img holds the
  * running mean of the clamped radiance (3 floats per pixel). */

@@ -62,7 +62,8 @@ def test_brdf_gradient_matches_central_differences(seed):
         grad = np.zeros(6)
         m = O.Material.from_buffer_copy(words.tobytes())
         c = O.ctx(m, wo=wo)
-        L.orc_brdf_grad(C.byref(c), O.v3(wi), O.ptr(np.ascontiguousarray(a)), O.ptr(words), O.ptr(grad))
+        a = np.ascontiguousarray(a)
+        L.orc_brdf_grad(C.byref(c), O.v3(wi), O.ptr(a), O.ptr(words), O.ptr(grad))
         for k, name in enumerate(PARAMS[:6]):
             def fd_at(hstep):
                 vals = []
@@ -203,8 +204,9 @@ def test_fp64_brdf_value_is_the_float_value_and_brdf_grad_its_derivative(seed):
         worst_val = max(worst_val, float(np.max(np.abs(f64 - f32) / np.maximum(np.abs(f64), 1e-12))))
         a = rng.uniform(0.5, 1.5, 3)
         grad = np.zeros(6)
+        a = np.ascontiguousarray(a)
         L.orc_brdf_grad(C.byref(O.ctx(O.Material.from_buffer_copy(words.tobytes()), wo=wo)), O.v3(wi),
-                        O.ptr(np.ascontiguousarray(a)), O.ptr(words), O.ptr(grad))
+                        O.ptr(a), O.ptr(words), O.ptr(grad))
         for k, name in enumerate(PARAMS[:6]):
             up, dn = _word_steps(words, name, 1e-5)
             step = float(up[WORD[name]]) - float(dn[WORD[name]])
@@ -215,10 +217,11 @@ def test_fp64_brdf_value_is_the_float_value_and_brdf_grad_its_derivative(seed):
 
 def render_f64(sc, cam, params, w, h, spp, eval_mats=None, first_seed=1, mask=None):
     img = np.zeros((w * h, 3))
+    mask = None if mask is None else np.ascontiguousarray(mask)  # held: ptr() is a bare address
     O.lib().orc_render_detached_f64_seeded(O.ptr(img), w, h, O.ptr(cam), C.byref(sc.scene),
                                            None if eval_mats is None else O.ptr(eval_mats), C.byref(O.Env()),
                                            O.ptr(params), spp, first_seed,
-                                           None if mask is None else O.ptr(np.ascontiguousarray(mask)), 8)
+                                           None if mask is None else O.ptr(mask), 8)
     return img
 
 
