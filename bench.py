@@ -269,7 +269,126 @@ def adjoint_bench(c1, bvh: str, cpu_threads: int) -> dict:
             "per_parameter_note": "adjoint, fd_h1e-3, fd_h1e-2: full-frame L2 loss, FP32 detached estimator on the "
                                   "GPU; fp64_*: the 32x32-window linear loss, GPU adjoint vs FP64 oracle adjoint and "
                                   "vs FP64 central differences (h = 1e-4); rel errors floored at 1e-3 of the largest",
-            "binned_spectral": binned_out}
+            "binned_spectral": binned_out,
+            "hero_wavelength": spectral_adjoint_bench(spec, bvh, cpu_threads)}
+
+
+def spectral_adjoint_bench(spec, bvh: str, cpu_threads: int) -> dict:
+    """C2 in hero-wavelength mode (four wavelengths in 380..720 nm): every material's base colour
+    and the lights' emission as 16-bin spectra over that range (the uplift of their colours),
+    rendered spectrally; pathTracerAdjointRenderSpectral gives the gradient of an L2 loss with
+    respect to the bins, roughness, metallic, specular and the emission scales. Times the forward
+    and the adjoint, and checks the gradient on windows against the oracle's FP64 spectral adjoint
+    and against FP64 central differences of the oracle's spectral detached estimator."""
+    import ctypes as C
+    import torch
+    from lsystemeditor_amd import abi, binned, diff, rgb2spec
+    from oracle import oracle as O
+
+    w, h, spp = spec.width, spec.height, spec.spp
+    lights = [i for i in range(spec.materials.shape[0]) if spec.materials[i, 9] > 0.0]
+    sp = binned.hero_spectra(spec.materials, lights, 380.0, 720.0)
+    table = rgb2spec.table(16)
+    hero = abi.Spectral(*table)
+    hero.enable(380.0, 720.0)
+    try:
+        scene = abi.Scene(spec.triangles, spec.materials, bvh=bvh)
+        r = diff.SpectralDiffRenderer(scene, spec.materials, sp, spec.camera, spec.params, w, h, spp)
+        th0 = r.theta0()
+        g = np.random.default_rng(5)
+        target = r.forward((th0 * torch.from_numpy(g.uniform(0.8, 1.2, tuple(th0.shape))).float()).cuda())
+        img = r.forward(th0.cuda())
+        grad_img = (2.0 * (img - target)).contiguous()
+        r.adjoint(grad_img)
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r.forward(th0.cuda())
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r.adjoint(grad_img)
+        torch.cuda.synchronize()
+        t_adj = (time.perf_counter() - t0) / reps
+        samples = w * h * spp
+        live = r.live_mask()
+        print(f"[bench] C2 hero-wavelength: forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms "
+              f"({samples / t_adj / 1e6:.1f} grad-Msamples/s), {int(live.sum())} live parameters", file=sys.stderr,
+              flush=True)
+        ref = O.OracleScene(abi.host_to_gpu_triangles(spec.triangles), spec.materials)
+        ref.set_spectra(sp)
+        ospec = O.OracleSpectral(*table, lambda_min=380.0, lambda_max=720.0)
+
+        def window(size):
+            win = torch.zeros_like(grad_img).view(h, w, 3)
+            y0, x0 = h // 2 - size // 2, w // 2 - size // 2
+            win[y0:y0 + size, x0:x0 + size] = grad_img.view(h, w, 3)[y0:y0 + size, x0:x0 + size]
+            win = win.reshape(-1, 3).contiguous()
+            return win, np.ascontiguousarray(win.cpu().numpy().astype(np.float64))
+
+        # the 32x32 window: the GPU gradient against the oracle's FP64 spectral adjoint
+        win32, gi32 = window(32)
+        g32 = r.adjoint(win32).cpu().numpy()
+        want = np.zeros(diff.SPECTRAL_PARAMS * spec.materials.shape[0])
+        O.lib().orc_render_adjoint_spectral(O.ptr(gi32), w, h, O.ptr(spec.camera), C.byref(ref.scene),
+                                            C.byref(O.Env()), O.ptr(r.params_host), C.byref(ospec.model), spp, 1,
+                                            O.ptr(want), cpu_threads)
+        want = want.reshape(-1, diff.SPECTRAL_PARAMS)
+        nz = want != 0.0
+        rel_o = np.abs(g32 - want)[nz] / np.maximum(np.abs(want[nz]), 1e-3 * np.abs(want).max())
+        # the 16x16 window: central differences (h = 1e-4) of the FP64 spectral detached estimator
+        win16, gi16 = window(16)
+        g16 = r.adjoint(win16).cpu().numpy()
+        t0 = time.perf_counter()
+        mats0 = np.ascontiguousarray(r.base.copy())
+        sp0 = np.ascontiguousarray(sp.copy())
+
+        def loss(mats, spectra):
+            im = np.zeros((w * h, 3))
+            O.lib().orc_render_detached_spectral(None, O.ptr(im), O.ptr(gi16), w, h, O.ptr(spec.camera),
+                                                 C.byref(ref.scene), O.ptr(mats), O.ptr(spectra), C.byref(O.Env()),
+                                                 O.ptr(r.params_host), C.byref(ospec.model), spp, 1, cpu_threads)
+            return float((gi16 * im).sum())
+
+        fd = np.zeros_like(g16)
+        for i, k in zip(*np.nonzero(live)):
+            if k < abi.SPECTRUM_WORDS:
+                up, dn = sp0.copy(), sp0.copy()
+                up[i, k] += np.float32(1e-4)
+                dn[i, k] -= np.float32(1e-4)
+                fd[i, k] = (loss(mats0, up) - loss(mats0, dn)) / (float(up[i, k]) - float(dn[i, k]))
+            else:
+                wd = diff.SPECTRAL_SCALAR_WORDS[k - abi.SPECTRUM_WORDS]
+                up, dn = mats0.copy(), mats0.copy()
+                up[i, wd] += np.float32(1e-4)
+                dn[i, wd] -= np.float32(1e-4)
+                fd[i, k] = (loss(up, sp0) - loss(dn, sp0)) / (float(up[i, wd]) - float(dn[i, wd]))
+        t_fd = time.perf_counter() - t0
+        nzf = live & (fd != 0.0)
+        rel_f = np.abs(g16 - fd)[nzf] / np.maximum(np.abs(fd[nzf]), 1e-3 * np.abs(fd).max())
+        print(f"[bench] C2 hero-wavelength gradient: vs the FP64 oracle adjoint (32x32 window) max rel err "
+              f"{rel_o.max():.2e} over {int(nz.sum())} parameters; vs FP64 central differences (16x16 window, "
+              f"h=1e-4) max {rel_f.max():.2e}, median {np.median(rel_f):.2e} over {int(nzf.sum())} ({t_fd:.1f} s)",
+              file=sys.stderr, flush=True)
+        return {"workload": f"C2 in hero-wavelength mode: four wavelengths in 380..720 nm, {w}x{h}, {spp} spp, "
+                            "16-bin reflectance spectra for every material and 16-bin emission for the two lights "
+                            "(uplift of their colours over 380..720 nm); L2 loss against the render at every "
+                            "parameter scaled by U(0.8, 1.2) (seeded)",
+                "parameters_per_material": "16 reflectance bins, 16 emission bins, roughness, metallic, specular, "
+                                           "emission scale",
+                "live_parameters": int(live.sum()),
+                "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "grad_samples_per_s": samples / t_adj,
+                "cost_ratio_adjoint_vs_forward": t_adj / t_fwd,
+                "cost_ratio_forward_plus_adjoint_vs_forward": (t_fwd + t_adj) / t_fwd,
+                "fp64_oracle_window": "32x32", "fp64_oracle_params": int(nz.sum()),
+                "fp64_oracle_max_rel_err": float(rel_o.max()),
+                "fp64_fd_window": "16x16", "fp64_fd_params": int(nzf.sum()), "fp64_fd_step": 1e-4,
+                "fp64_fd_max_rel_err": float(rel_f.max()), "fp64_fd_median_rel_err": float(np.median(rel_f))}
+    finally:
+        abi.Spectral.disable()
+        hero.close()
 
 
 def facade_bench(spec) -> dict:

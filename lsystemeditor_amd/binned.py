@@ -39,17 +39,33 @@ def projection(bins: int = BINS) -> np.ndarray:
     return m
 
 
-def uplift(rgb, bins: int = BINS) -> np.ndarray:
-    """(bins,) float64: the rgb2spec spectrum of a linear-sRGB colour in [0, 1]^3, averaged per bin."""
+def uplift(rgb, bins: int = BINS, lo: float = rgb2spec.LAMBDA_MIN, hi: float = rgb2spec.LAMBDA_MAX) -> np.ndarray:
+    """(bins,) float64: the rgb2spec spectrum of a linear-sRGB colour in [0, 1]^3, averaged over
+    each of `bins` equal intervals of [lo, hi] nm (1-nm samples)."""
     fit = rgb2spec._Fit()
     target = np.clip(np.asarray(rgb, np.float64), 0.0, 1.0)[None, :]
     c, _ = fit.solve(np.zeros((1, 3)), target, 200)
-    lam = np.arange(rgb2spec.LAMBDA_MIN, rgb2spec.LAMBDA_MAX + 1.0, 1.0)
+    lam = np.arange(lo, hi + 1.0, 1.0)
     t = (lam - rgb2spec.LAMBDA_MIN) / rgb2spec.LAMBDA_RANGE
     x = c[0, 0] * t * t + c[0, 1] * t + c[0, 2]
     s = 0.5 + 0.5 * x / np.sqrt(1.0 + x * x)
-    b = np.minimum((t * bins).astype(np.int64), bins - 1)
+    b = np.minimum(((lam - lo) / (hi - lo) * bins).astype(np.int64), bins - 1)
     return np.array([s[b == k].mean() for k in range(bins)])
+
+
+def hero_spectra(materials: np.ndarray, lights: list[int], lo: float = 380.0, hi: float = 720.0,
+                 bins: int = BINS) -> np.ndarray:
+    """(materials, 2 bins) float32 table for the hero-wavelength renderer (ptSceneSetSpectra,
+    bins over the hero range [lo, hi]): every material's reflectance is the uplift of its base
+    colour; the lights' emission shapes are the uplift of their colours (their emission scales
+    them, as it does r, g, b); the other materials keep the rgb2spec emission (-1)."""
+    m = np.asarray(materials, np.float32).reshape(-1, 24)
+    out = np.full((m.shape[0], 2 * bins), -1.0, np.float32)
+    for i in range(m.shape[0]):
+        out[i, :bins] = uplift(m[i, 0:3], bins, lo, hi)
+        if i in lights:
+            out[i, bins:] = out[i, :bins]
+    return out
 
 
 class BinnedParams:
