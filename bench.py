@@ -481,24 +481,28 @@ def batch_bench(bvh: str) -> dict:
     if bvh in ("sah", "sah-binary"):
         batch_only = br._batch().acc.acc.clone()
         timings = {}
-        for cold in (False, True):
-            br.capture_build_and_render(cold=cold)
+        # the rebuilds as 4 parallel branches of the graph (the default), and in one branch
+        for cold, branches in ((False, 4), (True, 4), (False, 1)):
+            br.capture_build_and_render(cold=cold, branches=branches)
             br.replay_build_and_render()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             br.replay_build_and_render()
             torch.cuda.synchronize()
-            timings[cold] = time.perf_counter() - t0
+            timings[cold, branches] = time.perf_counter() - t0
             br.check_rebuild()                # every scene's own rebuild status
             if not torch.equal(br._batch().acc.acc, batch_only):
                 raise RuntimeError("C3: the build-and-render graph's image differs from the batch-only render")
-        t_full, t_cold = timings[False], timings[True]
+        t_full, t_cold, t_one = timings[False, 4], timings[True, 4], timings[False, 1]
         t_seq_build = t_seq + br.build_ms / 1e3
         full = {"build_and_render_graph_msamples_s": samples / t_full / 1e6,
                 "build_and_render_graph_ms": round(t_full * 1e3, 2),
                 "builds_in_graph_ms": round((t_full - t_batch) * 1e3, 2),
                 "builds_in_graph_note": "rebuild of unchanged geometry: each rebuild enqueues the levels its "
-                                        "scene's first build needed",
+                                        "scene's first build needed; the 64 rebuilds run as 4 parallel branches "
+                                        "of the graph (own stream and build workspace each)",
+                "one_branch_build_and_render_graph_msamples_s": samples / t_one / 1e6,
+                "one_branch_builds_in_graph_ms": round((t_one - t_batch) * 1e3, 2),
                 "cold_build_and_render_graph_msamples_s": samples / t_cold / 1e6,
                 "cold_builds_in_graph_ms": round((t_cold - t_batch) * 1e3, 2),
                 "build_and_render_image_equals_batch_only": True,
@@ -514,7 +518,8 @@ def batch_bench(bvh: str) -> dict:
         print(f"[bench] C3 with the 64 tree rebuilds of unchanged geometry: one graph (rebuilds + batch wavefront) "
               f"{full['build_and_render_graph_msamples_s']:.1f} Msamples/s ({full['builds_in_graph_ms']:.1f} ms of "
               f"rebuilds; cold, all 49 levels: {full['cold_builds_in_graph_ms']:.1f} ms, "
-              f"{full['cold_build_and_render_graph_msamples_s']:.1f} Msamples/s), sequential renders after eager "
+              f"{full['cold_build_and_render_graph_msamples_s']:.1f} Msamples/s; rebuilds in one branch "
+              f"{full['one_branch_builds_in_graph_ms']:.1f} ms), sequential renders after eager "
               f"builds {full['sequential_with_builds_msamples_s']:.1f}", file=sys.stderr, flush=True)
     return {"workload": "C3: 64 bushes, seeds 0-63, iterations 3-7, resolutions 128^2-1024^2, 64 spp, depth 8",
             "scenes": len(specs), "triangles": int(tri), "samples": samples,

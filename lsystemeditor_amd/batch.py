@@ -61,7 +61,8 @@ class BatchRenderer:
         self.batch = None
         self.batch_graph = None
         self.build_graph = None
-        self.workspace = None
+        self.workspaces = None  # build workspaces of the rebuild branches
+        self.branch_streams = []
         self._pins = 0          # workspace pins taken by this renderer's captures
         torch.cuda.synchronize()
 
@@ -128,24 +129,45 @@ class BatchRenderer:
     def replay_batched(self):
         self.batch_graph.replay()
 
-    def capture_build_and_render(self, cold: bool = False):
+    def capture_build_and_render(self, cold: bool = False, branches: int = 4):
         """Records every scene's tree rebuild and then render_batched into one graph (binned-SAH
         build modes only). The rebuilds enqueue the levels each scene's first build needed (a
         rebuild of unchanged geometry), or with cold every level the depth cap allows (what a build
-        of new geometry enqueues)."""
+        of new geometry enqueues). They form `branches` parallel branches of the graph, each on a
+        stream of its own with a build workspace of its own (scenes go largest first to the branch
+        with the fewest triangles), so that one branch's small-scene level kernels overlap the
+        others'; the batch render waits for all of them."""
         if any(sc.bvh not in ("sah", "sah-binary") for sc in self.scenes):
             raise ValueError("tree rebuilds need a binned-SAH build mode")
         b = self._batch()
         b.reserve(self.max_paths)
-        if self.workspace is None:
-            self.workspace = abi.BuildWorkspace(max(sc.triangle_count for sc in self.scenes))
+        k = max(1, min(int(branches), len(self.scenes)))
+        groups = [[] for _ in range(k)]
+        loads = [0] * k
+        for i in sorted(range(len(self.scenes)), key=lambda i: -self.scenes[i].triangle_count):
+            j = loads.index(min(loads))
+            groups[j].append(i)
+            loads[j] += self.scenes[i].triangle_count
+        need = [max(self.scenes[i].triangle_count for i in g) for g in groups]
+        if self.workspaces is None or len(self.workspaces) < k or \
+                any(ws.max_triangles < n for ws, n in zip(self.workspaces, need)):
+            self.workspaces = None
+            torch.cuda.synchronize()
+            self.workspaces = [abi.BuildWorkspace(n) for n in need]
+        if len(self.branch_streams) < k:
+            self.branch_streams += [torch.cuda.Stream() for _ in range(k - len(self.branch_streams))]
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         p0 = render.workspace_pins()
         with torch.cuda.graph(g, stream=self.stream):
             s = torch.cuda.current_stream()
-            for sc in self.scenes:
-                sc.rebuild(self.workspace, s.cuda_stream, cold=cold)
+            for grp, ws, bs in zip(groups, self.workspaces, self.branch_streams):
+                bs.wait_stream(s)
+                with torch.cuda.stream(bs):
+                    for i in grp:
+                        self.scenes[i].rebuild(ws, bs.cuda_stream, cold=cold)
+            for bs in self.branch_streams[:k]:
+                s.wait_stream(bs)
             b.render_fixed(self.specs[0].spp, max_paths=self.max_paths, stream=s)
         self._pins += render.workspace_pins() - p0
         self.build_graph = g
@@ -165,7 +187,7 @@ class BatchRenderer:
     def check_rebuild(self):
         """Waits for the batch stream; raises unless every scene's last rebuild reproduced its tree."""
         for sc in self.scenes:
-            sc.rebuild_check(self.workspace, int(self.stream.cuda_stream))
+            sc.rebuild_check(self.workspaces[0], int(self.stream.cuda_stream))
 
     def batched_image(self, i: int) -> torch.Tensor:
         """Scene i's (pixels, 4) accumulator rows of the last batch render."""
