@@ -666,6 +666,67 @@ def spectral_bench(scene, spec, inputs, acc, passes: int, bvh: str) -> dict:
     return out
 
 
+def hero_fd_check(problem, h: float = 1e-4, window: int = 16) -> dict:
+    """C4's FD spot check at hero wavelengths: scene 0's GPU spectral adjoint of its L2 loss
+    restricted to a centred window against central differences (step h) of the oracle's spectral
+    detached estimator evaluated in double, for every optimised parameter."""
+    import ctypes as C
+    import torch
+    from lsystemeditor_amd import abi, diff, rgb2spec
+    from lsystemeditor_amd.inverse import HERO_RANGE
+    from oracle import oracle as O
+    r, p, spec = problem.renderers[0], problem.params[0], problem.specs[0]
+    w, hh, spp = r.width, r.height, r.spp
+    th = p.theta().detach()
+    img = r.forward(th.cuda())
+    win = torch.zeros_like(img).view(hh, w, 3)
+    y0, x0 = hh // 2 - window // 2, w // 2 - window // 2
+    win[y0:y0 + window, x0:x0 + window] = (2.0 * (img - problem.targets[0])).view(hh, w, 3)[y0:y0 + window,
+                                                                                               x0:x0 + window]
+    win = win.reshape(-1, 3).contiguous()
+    g = r.adjoint(win).cpu().numpy()
+    if getattr(problem, "_oracle0", None) is None:
+        ref = O.OracleScene(abi.host_to_gpu_triangles(spec.triangles), spec.materials)
+        problem._oracle0 = (ref, O.OracleSpectral(*rgb2spec.table(16), lambda_min=HERO_RANGE[0],
+                                                  lambda_max=HERO_RANGE[1]))
+    ref, ospec = problem._oracle0
+    mats, sp = r._tables(th)
+    # the oracle samples with the current parameters too (the adjoint's detached sampling)
+    np.copyto(ref.mats, mats.reshape(ref.mats.shape))
+    ref.set_spectra(sp)
+    gi = np.ascontiguousarray(win.cpu().numpy().astype(np.float64))
+
+    def loss(m, s_):
+        im = np.zeros((w * hh, 3))
+        O.lib().orc_render_detached_spectral(None, O.ptr(im), O.ptr(gi), w, hh, O.ptr(spec.camera),
+                                             C.byref(ref.scene), O.ptr(m), O.ptr(s_), C.byref(O.Env()),
+                                             O.ptr(r.params_host), C.byref(ospec.model), spp, 1, 16)
+        return float((gi * im).sum())
+
+    rel = []
+    fds = {}
+    for i, k in zip(*np.nonzero(p.mask.numpy())):
+        if k < abi.SPECTRUM_WORDS:
+            up, dn = sp.copy(), sp.copy()
+            up[i, k] += np.float32(h)
+            dn[i, k] -= np.float32(h)
+            fds[i, k] = (loss(mats, up) - loss(mats, dn)) / (float(up[i, k]) - float(dn[i, k]))
+        else:
+            wd = diff.SPECTRAL_SCALAR_WORDS[k - abi.SPECTRUM_WORDS]
+            up, dn = mats.copy(), mats.copy()
+            up[i, wd] += np.float32(h)
+            dn[i, wd] -= np.float32(h)
+            fds[i, k] = (loss(up, sp) - loss(dn, sp)) / (float(up[i, wd]) - float(dn[i, wd]))
+    top = max(abs(v) for v in fds.values()) or 1.0
+    for (i, k), v in fds.items():
+        rel.append(abs(g[i, k] - v) / max(abs(v), 1e-3 * top))
+    rel = np.array(rel)
+    return {"params": int(rel.size), "max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
+            "reference": f"FP64 central differences (h = {h:g}) of the oracle's spectral detached estimator, "
+                         f"{window}x{window} window of scene 0"}
+
+
+
 def inverse_bench(bvh: str, steps: int) -> dict:
     """C4: 32 bushes (iterations 3-6, 256x256, 16 spp, depth 8), 200 Adam steps (lr 1e-2) through
     the adjoint on 16-bin base-colour spectra, roughness and metallic of bark, leaf and ground; FD
@@ -693,7 +754,42 @@ def inverse_bench(bvh: str, steps: int) -> dict:
           f"(at the target parameters {res['image_loss']['target_params']:.5g}: excess "
           f"{100 * res['image_loss']['final_excess_fraction']:.1f} % of the initial)", file=sys.stderr, flush=True)
     del prob
+    out["hero_wavelength"] = inverse_hero_bench(specs, bvh, steps)
     return out
+
+
+def inverse_hero_bench(specs, bvh: str, steps: int) -> dict:
+    """C4 rendered and differentiated at four hero wavelengths in 380..720 nm: the base colours as
+    16-bin reflectance spectra over that range (uplift of the C2 colours), optimised with
+    roughness and metallic through the batched spectral adjoint; FD spot checks against the
+    oracle's FP64 spectral detached estimator."""
+    from lsystemeditor_amd import abi, inverse, rgb2spec
+    hero = abi.Spectral(*rgb2spec.table(16))
+    hero.enable(*inverse.HERO_RANGE)
+    try:
+        t1 = time.perf_counter()
+        prob = inverse.InverseProblem(specs, bvh=bvh, transport="hero", fd_checker=hero_fd_check)
+        setup_s = time.perf_counter() - t1
+        res = inverse.optimise(prob, steps=steps, lr=1e-2, check_every=50,
+                               log=lambda m: print(f"[bench] C4 hero-wavelength {m}", file=sys.stderr, flush=True))
+        e0, e1 = res["param_mae_initial"], res["param_mae_final"]
+        print(f"[bench] C4 hero-wavelength {steps} steps in {res['wall_s']:.1f} s "
+              f"({res['grad_samples_per_s'] / 1e6:.1f} grad-Msamples/s), parameter error {e0['all']:.4f} -> "
+              f"{e1['all']:.4f} (bins {e0['bins']:.4f} -> {e1['bins']:.4f}, projected colour {e0['base_color']:.4f} -> "
+              f"{e1['base_color']:.4f}, roughness {e0['roughness']:.4f} -> {e1['roughness']:.4f}, metallic "
+              f"{e0['metallic']:.4f} -> {e1['metallic']:.4f}); 256-spp image loss {res['image_loss']['initial']:.5g} -> "
+              f"{res['image_loss']['final']:.5g} (floor {res['image_loss']['target_params']:.5g})", file=sys.stderr,
+              flush=True)
+        out = {"workload": "C4 at four hero wavelengths in 380..720 nm: 16-bin reflectance spectra over that range "
+                           "(uplift of the colours), roughness, metallic of bark, leaf, ground through the batched "
+                           "spectral adjoint; lights' emission spectra fixed; targets: every optimised value scaled "
+                           "by a seeded factor in [0.8, 1.2], rendered at 256 spp",
+               "params": prob.parameter_count(), "setup_s": round(setup_s, 2), **res}
+        del prob
+        return out
+    finally:
+        abi.Spectral.disable()
+        hero.close()
 
 
 def profile_read():

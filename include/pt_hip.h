@@ -454,8 +454,10 @@ bool pathTracerRenderFixedBatch(PtFloat4* d_buffer, uint32_t* d_samples, float* 
                                 unsigned int firstSeed, hipStream_t stream);
 
 /* pathTracerAdjointRenderSeeded over a batch: d_gradImage holds dLoss/dPixel for the batch's
- * pixels (3 floats each), d_grad accumulates 7 doubles per gradient row (at most 512 rows). The
- * samples (seeds firstSeed..) are rendered first, then replayed. RGB mode only; not capturable. */
+ * pixels (3 floats each), d_grad accumulates 7 doubles per gradient row (at most 512 rows); in
+ * hero-wavelength mode, kPtSpectrumGradParams doubles per row as pathTracerAdjointRenderSpectral
+ * (at most 224 rows). The samples (seeds firstSeed..) are rendered first, then replayed. Not
+ * capturable. */
 bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtBatch* batch, int samplesPerPixel,
                                   unsigned int firstSeed, double* d_grad, hipStream_t stream);
 

@@ -39,6 +39,18 @@ def projection(bins: int = BINS) -> np.ndarray:
     return m
 
 
+def projection_over(lo: float, hi: float, bins: int = BINS) -> np.ndarray:
+    """(3, bins) float64: linear sRGB of a spectrum that is 1 on bin b of [lo, hi] nm and 0
+    elsewhere, normalised so that a constant 1 over [lo, hi] projects to (1, 1, 1)."""
+    lam = np.arange(lo, hi + 1.0, 1.0)
+    resp = rgb2spec.rgb_response(lam)
+    b = np.minimum(((lam - lo) / (hi - lo) * bins).astype(np.int64), bins - 1)
+    m = np.zeros((3, bins))
+    for k in range(bins):
+        m[:, k] = resp[b == k].sum(0) / resp.sum(0)
+    return m
+
+
 def uplift(rgb, bins: int = BINS, lo: float = rgb2spec.LAMBDA_MIN, hi: float = rgb2spec.LAMBDA_MAX) -> np.ndarray:
     """(bins,) float64: the rgb2spec spectrum of a linear-sRGB colour in [0, 1]^3, averaged over
     each of `bins` equal intervals of [lo, hi] nm (1-nm samples)."""

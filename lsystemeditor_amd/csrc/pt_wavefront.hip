@@ -52,6 +52,7 @@ constexpr int kMaxBatchScenes = 4096;   // scenes per batch
 // latency tail of its deep bounces) small.
 constexpr int kDefaultPathsInFlight = 1 << 26;
 constexpr int kMaxBatchGradRows = 512;  // material rows of a batch adjoint (LDS sums)
+constexpr int kMaxBatchSpectralGradRows = 224;  // the same in hero-wavelength mode (36 doubles a row)
 
 enum : uint32_t {
     kFlagExt = 1u,   // an extension ray was traced for this path at this bounce
@@ -2780,12 +2781,10 @@ extern "C" bool pathTracerAdjointRenderBatch(const float* d_gradImage, const PtB
         ptSetError("pathTracerAdjointRenderBatch: invalid arguments");
         return false;
     }
-    if (batch->gradRows <= 0 || batch->gradRows > kMaxBatchGradRows) {
-        ptSetError("pathTracerAdjointRenderBatch: %d material rows (1..%d)", batch->gradRows, kMaxBatchGradRows);
-        return false;
-    }
-    if (pt_spectral_enabled()) {
-        ptSetError("pathTracerAdjointRenderBatch: RGB rendering only (hero-wavelength sampling is on)");
+    // hero-wavelength mode: 36 parameters per row, so fewer rows fit the LDS sums (64 KB)
+    const int maxRows = pt_spectral_enabled() ? kMaxBatchSpectralGradRows : kMaxBatchGradRows;
+    if (batch->gradRows <= 0 || batch->gradRows > maxRows) {
+        ptSetError("pathTracerAdjointRenderBatch: %d material rows (1..%d)", batch->gradRows, maxRows);
         return false;
     }
     if (pt_light_mode() != 0) {

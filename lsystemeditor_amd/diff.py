@@ -276,6 +276,8 @@ class BatchDiffRenderer:
     def __init__(self, renderers: list[DiffRenderer]):
         if not renderers or len({r.spp for r in renderers}) != 1:
             raise ValueError("a batch needs at least one renderer, all with the same spp")
+        if len({isinstance(r, SpectralDiffRenderer) for r in renderers}) != 1:
+            raise ValueError("a batch holds RGB renderers or spectral renderers, not both")
         self.renderers = list(renderers)
         self.spp = renderers[0].spp
         self.batch = render.SceneBatch([r.scene for r in renderers], [r.inputs for r in renderers],
@@ -303,9 +305,11 @@ class BatchDiffRenderer:
 
     def adjoint(self, grad_image: torch.Tensor, seed_base: int) -> list[torch.Tensor]:
         """Per renderer (materials, 7) float64 dLoss/dtheta at the uploaded parameters for
-        dLoss/dImage = grad_image (total pixels, 3), replaying the samples of seed base seed_base."""
+        dLoss/dImage = grad_image (total pixels, 3), replaying the samples of seed base seed_base
+        ((materials, 36) for SpectralDiffRenderers, rendered at hero wavelengths)."""
         g = grad_image.detach().to("cuda", torch.float32).contiguous()
-        out = torch.zeros((self.batch.grad_rows, 7), dtype=torch.float64, device="cuda")
+        cols = SPECTRAL_PARAMS if isinstance(self.renderers[0], SpectralDiffRenderer) else 7
+        out = torch.zeros((self.batch.grad_rows, cols), dtype=torch.float64, device="cuda")
         abi.check(abi.lib().pathTracerAdjointRenderBatch(g.data_ptr(), self.batch.handle, self.spp, seed_base + 1,
                                                          out.data_ptr(), None), "pathTracerAdjointRenderBatch")
         return [out[o:o + r.material_count] for o, r in zip(self.batch.grad_row_offsets, self.renderers)]

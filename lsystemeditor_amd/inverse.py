@@ -155,20 +155,88 @@ class _BinnedParams:
                 "roughness": ds[:, 0], "metallic": ds[:, 1]}
 
 
-class InverseProblem:
-    """Scenes, renderers, target images and the parameters being optimised."""
+HERO_RANGE = (380.0, 720.0)           # hero-wavelength range of the spectral variant (bins over it)
 
-    def __init__(self, specs: list[scenes.SceneSpec], bvh: str = "sah", binned_colour: bool = True):
+
+class _HeroParams:
+    """One scene's (materials, 36) spectral parameter rows (diff.SpectralDiffRenderer: 16
+    reflectance bins, 16 emission bins, roughness, metallic, specular, emission), optimised on the
+    reflectance bins, roughness and metallic of OPT_GROUPS; the target scales each of those by an
+    independent seeded factor in [0.8, 1.2]."""
+
+    ROUGH, METAL = abi.SPECTRUM_WORDS, abi.SPECTRUM_WORDS + 1
+
+    def __init__(self, theta0: np.ndarray, seed: int):
+        self.leaf = torch.from_numpy(theta0.copy()).requires_grad_(True)
+        mask = torch.zeros(theta0.shape)
+        for gidx in OPT_GROUPS:
+            mask[gidx, :abi.SPECTRUM_BINS] = 1.0
+            mask[gidx, [self.ROUGH, self.METAL]] = 1.0
+        self.mask = mask
+        g = np.random.default_rng(0xC4B00000 + seed)
+        factor = torch.from_numpy(g.uniform(0.8, 1.2, theta0.shape)).float()
+        self.true = torch.where(mask > 0, self.leaf.detach() * factor, self.leaf.detach())
+        self.m = torch.from_numpy(binned.projection_over(*HERO_RANGE)).float()
+
+    def leaves(self) -> list[torch.Tensor]:
+        return [self.leaf]
+
+    def theta(self) -> torch.Tensor:
+        return self.leaf
+
+    def theta_true(self) -> torch.Tensor:
+        return self.true
+
+    def theta_true_scalars(self) -> torch.Tensor:
+        """The current spectra with the target roughness and metallic."""
+        out = self.leaf.detach().clone()
+        out[:, [self.ROUGH, self.METAL]] = self.true[:, [self.ROUGH, self.METAL]]
+        return out
+
+    def mask_grads_(self) -> None:
+        self.leaf.grad.mul_(self.mask)
+
+    def clamp_(self) -> None:
+        with torch.no_grad():
+            self.leaf[list(OPT_GROUPS), :abi.SPECTRUM_BINS] = self.leaf[list(OPT_GROUPS), :abi.SPECTRUM_BINS].clamp(0, 1)
+            self.leaf[:, [self.ROUGH, self.METAL]] = self.leaf[:, [self.ROUGH, self.METAL]].clamp(0.01, 0.99)
+
+    def errors(self) -> dict[str, torch.Tensor]:
+        d = (self.leaf.detach() - self.true)[list(OPT_GROUPS)]
+        dr = d[:, :abi.SPECTRUM_BINS]
+        return {"base_color": (dr @ self.m.T).abs().reshape(-1), "bins": dr.abs().reshape(-1),
+                "roughness": d[:, self.ROUGH].abs(), "metallic": d[:, self.METAL].abs()}
+
+
+class InverseProblem:
+    """Scenes, renderers, target images and the parameters being optimised. transport="hero"
+    renders and differentiates at four hero wavelengths (HERO_RANGE) with the base colours as
+    16-bin reflectance spectra over that range (the lights' emission as the uplift of their
+    colours, fixed); the hero-wavelength state (abi.Spectral) must be enabled with that range
+    while the problem is used."""
+
+    def __init__(self, specs: list[scenes.SceneSpec], bvh: str = "sah", binned_colour: bool = True,
+                 transport: str = "rgb", fd_checker=None):
+        if transport not in ("rgb", "hero"):
+            raise ValueError("transport is 'rgb' or 'hero'")
         self.specs = specs
-        self.binned = binned_colour
+        self.fd_checker = fd_checker
+        self.hero = transport == "hero"
+        self.binned = binned_colour or self.hero
         self.renderers = []
         self.params = []
         for i, spec in enumerate(specs):
             sc = abi.Scene(spec.triangles, spec.materials, bvh=bvh)
-            r = diff.DiffRenderer(sc, spec.materials, spec.camera, spec.params, spec.width, spec.height, spec.spp)
-            th0 = r.theta0().numpy()
             seed = int(spec.info.get("seed", i))
-            self.params.append(_BinnedParams(th0, seed) if binned_colour else _RgbParams(th0, seed))
+            if self.hero:
+                sp = binned.hero_spectra(spec.materials, list(LIGHT_GROUPS), *HERO_RANGE)
+                r = diff.SpectralDiffRenderer(sc, spec.materials, sp, spec.camera, spec.params, spec.width,
+                                              spec.height, spec.spp)
+                self.params.append(_HeroParams(r.theta0().numpy(), seed))
+            else:
+                r = diff.DiffRenderer(sc, spec.materials, spec.camera, spec.params, spec.width, spec.height, spec.spp)
+                th0 = r.theta0().numpy()
+                self.params.append(_BinnedParams(th0, seed) if binned_colour else _RgbParams(th0, seed))
             self.renderers.append(r)
         self.samples_per_render = [s.width * s.height * s.spp for s in specs]
         # every scene's loss render and adjoint in one batch wavefront
@@ -235,8 +303,14 @@ class InverseProblem:
         return self.param_errors()["all"]
 
     def fd_check(self, fd_step: float) -> dict:
-        """Central-difference spot check of scene 0's gradient (detached estimator)."""
+        """Central-difference spot check of scene 0's gradient (detached estimator). At hero
+        wavelengths the caller's fd_checker(problem) makes it (no GPU spectral detached estimator
+        exists; the bench passes one built on an FP64 reference)."""
         r0, p0 = self.renderers[0], self.params[0]
+        if self.hero:
+            if self.fd_checker is None:
+                return {"params": 0, "max_rel_err": 0.0, "median_rel_err": 0.0, "reference": "none given"}
+            return self.fd_checker(self)
         if self.binned:
             res = binned.gradient_check(r0, p0.bp, self.targets[0], h=fd_step)
             rel = res["rel_err"]
@@ -282,8 +356,10 @@ def optimise(problem: InverseProblem, steps: int = 200, lr: float = 1e-2, check_
     samples = steps * sum(problem.samples_per_render)
     window = max(1, min(10, steps // 4))
     return {"steps": steps, "lr": lr, "wall_s": wall, "render_and_adjoint_s": render_s,
-            "parameters": "16-bin base-colour spectra (linear-sRGB projection), roughness, metallic"
-                          if problem.binned else "RGB base colour, roughness, metallic",
+            "parameters": ("16-bin base-colour spectra rendered at four hero wavelengths in 380..720 nm, roughness, "
+                           "metallic") if problem.hero else
+                          ("16-bin base-colour spectra (linear-sRGB projection), roughness, metallic"
+                           if problem.binned else "RGB base colour, roughness, metallic"),
             "renders_per_step": "loss render of a fresh sample set + the adjoint's render and replay of another",
             "grad_samples_per_s": samples / render_s,
             "loss_first_steps": float(np.mean(losses[:window])), "loss_last_steps": float(np.mean(losses[-window:])),

@@ -104,3 +104,28 @@ def test_spectral_adjoint_argument_checks(cuda):
     bad[8] = 0
     assert not lib.pathTracerAdjointRenderSpectral(*bad)
     assert "invalid arguments" in abi.last_error()
+
+
+def test_batch_spectral_adjoint_equals_per_scene_adjoints(hero):
+    """pathTracerAdjointRenderBatch at hero wavelengths (36 values per row) equals every scene's
+    own pathTracerAdjointRenderSpectral over the same seeds, and the batch render equals the
+    per-scene renders."""
+    from lsystemeditor_amd import inverse
+    specs = inverse.c4_specs(count=3, res=24, spp=4, iterations=(3, 4))
+    rs = []
+    for k, s in enumerate(specs):
+        sc = abi.Scene(s.triangles, s.materials, bvh="sah")
+        rs.append(diff.SpectralDiffRenderer(sc, s.materials, spectra_for(s.materials, 10 + k), s.camera, s.params,
+                                            s.width, s.height, s.spp))
+    br = diff.BatchDiffRenderer(rs)
+    thetas = [r.theta0() for r in rs]
+    img = br.forward(thetas, seed_base=6)
+    g = torch.randn((img.shape[0], 3), generator=torch.Generator().manual_seed(2)).cuda()
+    got = br.adjoint(g, seed_base=6)
+    for i, r in enumerate(rs):
+        own = r.forward(thetas[i].cuda(), seed_base=6)
+        assert torch.equal(own, img[br.pixels(i)]), i
+        want = r.adjoint(g[br.pixels(i)].contiguous(), reuse_forward=False, seed_base=6)
+        scale = float(want.abs().max())
+        assert scale > 0, i
+        assert float((got[i] - want).abs().max()) <= 1e-9 * scale, (i, float((got[i] - want).abs().max()) / scale)
