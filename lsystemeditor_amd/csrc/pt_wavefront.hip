@@ -518,7 +518,96 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
         bool finished = cur == kDone;
         bool pop = false;
         if (!finished) {
-            if (cur < 0) {
+            if constexpr (WIDE) {
+                // One memory round trip per step for the whole wave: lanes at a leaf load their
+                // triangle (3 float4) and lanes at a node its 4-wide record (7 float4) through the
+                // same load instructions, all issued before either test, so a wave whose lanes are
+                // split between triangles and nodes waits for memory once instead of twice.
+                const bool leaf = cur < 0;
+                const PtFloat4* src = leaf ? s.tris + 3u * static_cast<uint32_t>(~cur)
+                                           : s.wide + 8u * static_cast<uint32_t>(cur);
+                const float4 q0 = ld4(src, 0u);
+                const float4 q1 = ld4(src, 1u);
+                const float4 q2 = ld4(src, 2u);
+                float4 q3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q4 = q3, q5 = q3, q6 = q3;
+                if (!leaf) {
+                    q3 = ld4(src, 3u);
+                    q4 = ld4(src, 4u);
+                    q5 = ld4(src, 5u);
+                    q6 = ld4(src, 6u);
+                }
+                if (leaf) {
+                    const float4 a = q0;
+                    const float4 b = q1;
+                    const float4 c = q2;
+                    asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w));
+                    if (STATS) {
+                        ts.tris += 1u;
+                    }
+                    float t, u, v;
+                    const uint32_t id = __float_as_uint(a.w);
+                    if (tri_test(r, a, b, c, tMin, bestT, t, u, v) && closer(t, id, bestT, bestTri)) {
+                        bestT = t;
+                        cullT = t * kCullSlack;
+                        bestU = u;
+                        bestV = v;
+                        bestTri = id;
+                        bestMat = __float_as_int(b.w);
+                        if (ANY_HIT && t < rayMax) {
+                            finished = true;
+                        }
+                    }
+                    pop = !finished;
+                } else {
+                    const float4 mnx = q0;
+                    const float4 mny = q1;
+                    const float4 mnz = q2;
+                    const float4 mxx = q3;
+                    const float4 mxy = q4;
+                    const float4 mxz = q5;
+                    const float4 refs = q6;
+                    if (STATS) {
+                        ts.nodes += 1u;
+                    }
+                    int r0 = __float_as_int(refs.x), r1 = __float_as_int(refs.y), r2 = __float_as_int(refs.z),
+                        r3 = __float_as_int(refs.w);
+                    const float c0 = wide_child(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, cullT);
+                    const float c1 = wide_child(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, cullT);
+                    const float c2 = wide_child(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, cullT);
+                    const float c3 = wide_child(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, cullT);
+                    float d0 = r0 != kDone ? c0 : kInfDist;
+                    float d1 = r1 != kDone ? c1 : kInfDist;
+                    float d2 = r2 != kDone ? c2 : kInfDist;
+                    float d3 = r3 != kDone ? c3 : kInfDist;
+                    // sorting network for four; the children hit come first
+                    order2(d0, r0, d1, r1);
+                    order2(d2, r2, d3, r3);
+                    order2(d0, r0, d2, r2);
+                    order2(d1, r1, d3, r3);
+                    order2(d1, r1, d2, r2);
+                    if (d0 == kInfDist) {
+                        pop = true;
+                    } else {
+                        // the others farthest first, so the nearest of them is popped first
+                        // (the stack never holds more than 3 entries per level, 72 in all)
+                        const int np = min(static_cast<int>(d1 != kInfDist) + static_cast<int>(d2 != kInfDist) +
+                                               static_cast<int>(d3 != kInfDist),
+                                           kSpillEntries - st.sp);
+                        st.make_room(np);
+                        if (np == 3) {
+                            st.put(st.sp, r3, d3);
+                        }
+                        if (np >= 2) {
+                            st.put(st.sp + np - 2, r2, d2);
+                        }
+                        if (np >= 1) {
+                            st.put(st.sp + np - 1, r1, d1);
+                        }
+                        st.sp += np;
+                        cur = r0;
+                    }
+                }
+            } else if (cur < 0) {
                 const uint32_t slot = static_cast<uint32_t>(~cur);
                 const float4 a = ld4(s.tris, 3u * slot + 0u);
                 const float4 b = ld4(s.tris, 3u * slot + 1u);
@@ -543,55 +632,6 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                     }
                 }
                 pop = !finished;
-            } else if constexpr (WIDE) {
-                const uint32_t base = 8u * static_cast<uint32_t>(cur);
-                const float4 mnx = ld4(s.wide, base + 0u);
-                const float4 mny = ld4(s.wide, base + 1u);
-                const float4 mnz = ld4(s.wide, base + 2u);
-                const float4 mxx = ld4(s.wide, base + 3u);
-                const float4 mxy = ld4(s.wide, base + 4u);
-                const float4 mxz = ld4(s.wide, base + 5u);
-                const float4 refs = ld4(s.wide, base + 6u);
-                if (STATS) {
-                    ts.nodes += 1u;
-                }
-                int r0 = __float_as_int(refs.x), r1 = __float_as_int(refs.y), r2 = __float_as_int(refs.z),
-                    r3 = __float_as_int(refs.w);
-                const float c0 = wide_child(r, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tMin, cullT);
-                const float c1 = wide_child(r, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tMin, cullT);
-                const float c2 = wide_child(r, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tMin, cullT);
-                const float c3 = wide_child(r, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tMin, cullT);
-                float d0 = r0 != kDone ? c0 : kInfDist;
-                float d1 = r1 != kDone ? c1 : kInfDist;
-                float d2 = r2 != kDone ? c2 : kInfDist;
-                float d3 = r3 != kDone ? c3 : kInfDist;
-                // sorting network for four; the children hit come first
-                order2(d0, r0, d1, r1);
-                order2(d2, r2, d3, r3);
-                order2(d0, r0, d2, r2);
-                order2(d1, r1, d3, r3);
-                order2(d1, r1, d2, r2);
-                if (d0 == kInfDist) {
-                    pop = true;
-                } else {
-                    // the others farthest first, so the nearest of them is popped first
-                    // (the stack never holds more than 3 entries per level, 72 in all)
-                    const int np = min(static_cast<int>(d1 != kInfDist) + static_cast<int>(d2 != kInfDist) +
-                                           static_cast<int>(d3 != kInfDist),
-                                       kSpillEntries - st.sp);
-                    st.make_room(np);
-                    if (np == 3) {
-                        st.put(st.sp, r3, d3);
-                    }
-                    if (np >= 2) {
-                        st.put(st.sp + np - 2, r2, d2);
-                    }
-                    if (np >= 1) {
-                        st.put(st.sp + np - 1, r1, d1);
-                    }
-                    st.sp += np;
-                    cur = r0;
-                }
             } else {
                 const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
                 const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
