@@ -640,11 +640,23 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
     int sp = 0;
     int cur = s.rootRef;
     while (true) {
-        if (cur < 0) {
-            const uint32_t slot = static_cast<uint32_t>(~cur);
-            const float4 a = ld4(s.triPos, 3u * slot + 0u);
-            const float4 b = ld4(s.triPos, 3u * slot + 1u);
-            const float4 c = ld4(s.triPos, 3u * slot + 2u);
+        // One memory round trip per step for the whole wave: lanes at a leaf load its triangle
+        // (3 float4) and lanes at a node the node (4 float4) through the same load instructions,
+        // before either test, so a wave split between the two waits once.
+        const bool leaf = cur < 0;
+        const PtFloat4* src = leaf ? s.triPos + 3u * static_cast<uint32_t>(~cur)
+                                   : s.nodes + 4u * static_cast<uint32_t>(cur);
+        const float4 q0 = ld4(src, 0u);
+        const float4 q1 = ld4(src, 1u);
+        const float4 q2 = ld4(src, 2u);
+        float4 q3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!leaf) {
+            q3 = ld4(src, 3u);
+        }
+        if (leaf) {
+            const float4 a = q0;
+            const float4 b = q1;
+            const float4 c = q2;
             if (stats) {
                 stats->tris += 1;
             }
@@ -661,10 +673,10 @@ PT_DEV bool traverse(const MeshAccelSceneGpu& s, const Ray& r, float tMin, float
                 }
             }
         } else {
-            const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
-            const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
-            const float4 n2 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 2u);
-            const float4 n3 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 3u);
+            const float4 n0 = q0;
+            const float4 n1 = q1;
+            const float4 n2 = q2;
+            const float4 n3 = q3;
             if (stats) {
                 stats->nodes += 1;
             }
