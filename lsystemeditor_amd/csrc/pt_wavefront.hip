@@ -607,13 +607,22 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                         cur = r0;
                     }
                 }
-            } else if (cur < 0) {
-                const uint32_t slot = static_cast<uint32_t>(~cur);
-                const float4 a = ld4(s.tris, 3u * slot + 0u);
-                const float4 b = ld4(s.tris, 3u * slot + 1u);
-                const float4 c = ld4(s.tris, 3u * slot + 2u);
-                // all 48 bytes in one round trip: keeps the compiler from deferring v0's
-                // load past the determinant test
+            } else {
+                // binary trees: the same single round trip (3 float4 per triangle, 4 per node)
+                const bool leaf = cur < 0;
+                const PtFloat4* src = leaf ? s.tris + 3u * static_cast<uint32_t>(~cur)
+                                           : s.nodes + 4u * static_cast<uint32_t>(cur);
+                const float4 q0 = ld4(src, 0u);
+                const float4 q1 = ld4(src, 1u);
+                const float4 q2 = ld4(src, 2u);
+                float4 q3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (!leaf) {
+                    q3 = ld4(src, 3u);
+                }
+                if (leaf) {
+                const float4 a = q0;
+                const float4 b = q1;
+                const float4 c = q2;
                 asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w));
                 if (STATS) {
                     ts.tris += 1u;
@@ -632,11 +641,11 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                     }
                 }
                 pop = !finished;
-            } else {
-                const float4 n0 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 0u);
-                const float4 n1 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 1u);
-                const float4 n2 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 2u);
-                const float4 n3 = ld4(s.nodes, 4u * static_cast<uint32_t>(cur) + 3u);
+                } else {
+                const float4 n0 = q0;
+                const float4 n1 = q1;
+                const float4 n2 = q2;
+                const float4 n3 = q3;
                 if (STATS) {
                     ts.nodes += 1u;
                 }
@@ -657,6 +666,7 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                     cur = right;
                 } else {
                     pop = true;
+                }
                 }
             }
         }
