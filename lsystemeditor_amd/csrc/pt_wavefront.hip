@@ -30,7 +30,13 @@ using namespace pt;
 namespace {
 
 constexpr int kWfBlock = 256;
-constexpr int kLdsEntries = 8;     // LDS stack entries per thread (power of two)
+// LDS stack entries per thread (powers of two). Measured on the C1 frame: the closest-hit trace
+// on the 4-wide tree is fastest with 16 (38.5 ms against 39.3 with 8, although the 32 KiB ring
+// per block then holds it to 5 waves per SIMD); the any-hit trace (15.6 ms against 15.9 with 16)
+// and the closest-hit trace on binary trees (median-split tree: 679 against 643 Msamples/s with
+// 16) are fastest with 8; 4 entries spill to the global area too often (46.1 ms).
+constexpr int kLdsEntriesWideClosest = 16;
+constexpr int kLdsEntriesDefault = 8;
 // stack entries per thread in LDS ring + global spill: binary trees push one entry per level
 // (depth <= 48), 4-wide trees up to three (depth <= 24, so at most 72)
 constexpr int kSpillEntries = 96;
@@ -250,6 +256,7 @@ typedef int NativeInt2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) NativeInt2 LdsInt2;
 typedef __attribute__((address_space(1))) NativeInt2 GlobalInt2;
 
+template <int kLdsEntries>
 struct LdsStack {
     LdsInt2* ring;     // this thread's column of the block's LDS ring
     GlobalInt2* spill; // this thread's column of the global spill area
@@ -388,6 +395,7 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                                                             float2* __restrict__ hitB,
                                                             const BatchScene* __restrict__ tab, int lightMode)
 {
+    constexpr int kLdsEntries = (WIDE && !ANY_HIT) ? kLdsEntriesWideClosest : kLdsEntriesDefault;
     __shared__ int2 ring[kLdsEntries * kWfBlock];
     TraceScene s = BATCH ? TraceScene{} : trace_scene(*scene);
     const int n = *countPtr;
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) v
     if (static_cast<long long>(gtid >> 6) * kFetchChunk >= n) {
         return;
     }
-    LdsStack st{(LdsInt2*)(ring + threadIdx.x), (GlobalInt2*)(P.spill + gtid), P.spillThreads, 0, 0};
+    LdsStack<kLdsEntries> st{(LdsInt2*)(ring + threadIdx.x), (GlobalInt2*)(P.spill + gtid), P.spillThreads, 0, 0};
     TraceStats ts{0u, 0u, 0u, 0u};
     const unsigned long long lowerLanes = (1ull << lane_id()) - 1ull;
     int item = -1;        // queue index of the lane's current ray
