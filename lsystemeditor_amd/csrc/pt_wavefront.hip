@@ -386,7 +386,9 @@ PT_DEV float wide_child(const Ray& r, float x0, float y0, float z0, float x1, fl
 //                    nearest hit child is visited next and the others are pushed farthest first.
 // The kernels are bound by the latency of their dependent node fetches, so they are held to 72
 // VGPRs: 7 waves per SIMD instead of 6 (C1: extension trace 43.8 -> 42.2 ms per frame); at 64
-// VGPRs (8 waves) the compiler spills and the extension trace takes 57 ms.
+// VGPRs (8 waves) the compiler spills and the extension trace takes 57 ms. The extension-ray
+// kernel on the 4-wide tree is held to 5 waves per SIMD by its 32 KiB LDS ring instead (16 stack
+// entries per lane), which leaves it 77 VGPRs and was faster still.
 template <bool ANY_HIT, bool STATS, bool BATCH, bool WIDE>
 __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(7))) void wf_trace_kernel(const MeshAccelSceneGpu* scene, Paths P,
                                                             PathState PS, const int* __restrict__ queue,
